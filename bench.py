@@ -1,0 +1,258 @@
+"""Benchmark of the MI355X xTGEVC3 path on its headline workload.
+
+Workload (one "step"): dtgevc3 side='R', howmny='B' (back-transformed right eigenvectors)
+at n = 8192 on a synthetic quasi-triangular pencil -- P upper triangular with diagonal
+|N(0,1)|+0.1 and off-diagonal N(0,1), S with off-diagonal N(0,1) and 40 % of its diagonal in
+2x2 complex-pair blocks (matching P block diagonal, positive), Z the Q factor of the QR of a
+Gaussian matrix.  Inputs are resident in HBM before the timed region; each step restores
+VR = Z (device copy, inside the timed region) and calls the device-pointer entry point.
+
+Prints ONE JSON line (see the field list in main()).  --gpus N > 1 exits: this path runs on
+one GPU.  The CPU baseline times the reference library compiled from its own sources
+(oracle/_ref, linked to the image's OpenBLAS) on the host cores, on the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_inputs(n, seed, device):
+    import torch
+
+    from xtgevc3_amd import pencils as G
+
+    t0 = time.time()
+    S, P, ar, ai, be = G.real_quasi_triangular(n, seed)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed + 1)
+    A = torch.randn(n, n, dtype=torch.float64, device=device, generator=g)
+    Q, R = torch.linalg.qr(A)
+    Q = Q * torch.sign(torch.diagonal(R))[None, :]
+    # column-major storage of Z == row-major storage of Z^T
+    dZ = Q.t().contiguous()
+    dS = torch.from_numpy(np.ascontiguousarray(S.T)).to(device)
+    dP = torch.from_numpy(np.ascontiguousarray(P.T)).to(device)
+    torch.cuda.synchronize()
+    log(f"[bench] inputs n={n} ready in {time.time() - t0:.1f}s")
+    return S, P, ar, ai, be, dS, dP, dZ
+
+
+def residual_sample(S, P, ar, ai, be, dZ, dVR, ncols=64, seed=0):
+    """Metric A residual of a sample of back-transformed columns, in units of n*eps, computed
+    on the device in FP64: x = Z^T v, r = (beta S - alpha P) x."""
+    import torch
+
+    from oracle import oracle as O
+
+    n = S.shape[0]
+    kind, idx = O.real_layout("B", None, ai)
+    rng = np.random.default_rng(seed)
+    cols = np.unique(np.concatenate([rng.choice(n, ncols, replace=False), [0, n - 1]]))
+    # include both columns of any sampled pair
+    full = set()
+    for c in cols:
+        if kind[c] == 1:
+            full.update([c, c + 1])
+        elif kind[c] == 2:
+            full.update([c - 1, c])
+        else:
+            full.add(c)
+    cols = np.array(sorted(full))
+    V = dVR[cols, :].t()  # n x len(cols), column c = VR[:, c]
+    Zm = dZ.t()           # Z (n x n)
+    X = Zm.t() @ V        # back to the Schur basis
+    Sd = torch.from_numpy(np.ascontiguousarray(S)).cuda()
+    Pd = torch.from_numpy(np.ascontiguousarray(P)).cuda()
+    SX = (Sd @ X).cpu().numpy()
+    PX = (Pd @ X).cpu().numpy()
+    Xh = X.cpu().numpy()
+    nS = float(torch.linalg.norm(Sd))
+    nP = float(torch.linalg.norm(Pd))
+    worst = 0.0
+    eps = np.finfo(np.float64).eps
+    for j, c in enumerate(cols):
+        k = idx[c]
+        if kind[c] == 2:
+            continue
+        if kind[c] == 1:
+            jr = j
+            ji = list(cols).index(c + 1)
+            x = Xh[:, jr] + 1j * Xh[:, ji]
+            sx = SX[:, jr] + 1j * SX[:, ji]
+            px = PX[:, jr] + 1j * PX[:, ji]
+            a = ar[k] + 1j * ai[k]
+        else:
+            x, sx, px, a = Xh[:, j], SX[:, j], PX[:, j], ar[k]
+        b = be[k]
+        r = np.linalg.norm(b * sx - a * px) / ((abs(b) * nS + abs(a) * nP) * np.linalg.norm(x))
+        worst = max(worst, r / (n * eps))
+    return worst, len(cols)
+
+
+def cpu_baseline(n, seed, budget_s=40.0):
+    """Reference library (oracle/_ref) on the host cores: the same workload when it fits the
+    time budget, else a smaller pencil of the same construction scaled by n^2."""
+    from oracle import oracle as O
+    from xtgevc3_amd import pencils as G
+
+    if not O.have_ref():
+        return None
+    cores = len(os.sched_getaffinity(0))
+    S, P, ar, ai, be = G.real_quasi_triangular(2048, seed)
+    Z = G.random_orthogonal(2048, seed + 1)
+    VR = Z.copy(order="F")
+    t = time.time()
+    O.tgevc3("d", "R", "B", None, S, P, ar, be, alphai=ai, VR=VR, impl="ref")
+    t2048 = time.time() - t
+    predicted = t2048 * (n / 2048.0) ** 3 * 0.8
+    ns = n if predicted <= budget_s else 4096
+    if ns != 2048:
+        S, P, ar, ai, be = G.real_quasi_triangular(ns, seed)
+        Z = G.random_orthogonal(ns, seed + 1)
+        VR = Z.copy(order="F")
+        t = time.time()
+        O.tgevc3("d", "R", "B", None, S, P, ar, be, alphai=ai, VR=VR, impl="ref")
+        ts = time.time() - t
+    else:
+        ts = t2048
+    rate = ns / ts
+    value = rate * (ns / n) ** 2 if ns != n else rate
+    sample = (f"reference dtgevc3 (oracle/_ref, OpenBLAS {cores} threads) side=R howmny=B on the same "
+              f"construction at n={ns}: {ts:.2f} s" + ("" if ns == n else f"; eigvecs/s scaled by (n_s/n)^2 to n={n}"))
+    return {"value": value, "unit": "eigvecs/s", "cores": cores, "kind": "reference", "sample": sample,
+            "seconds": ts, "n_sample": ns}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-residual", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this path runs on one GPU (the eigenvector columns shard across GPUs without "
+              f"exchange, but no multi-GPU run is part of this build); got --gpus {args.gpus}", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import xtgevc3_amd as X
+
+    n = args.n
+    dev = "cuda"
+    S, P, ar, ai, be, dS, dP, dZ = make_inputs(n, args.seed, dev)
+    dVR = torch.empty_like(dZ)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        dVR.copy_(dZ)
+        m, info = X.dtgevc3_device("R", "B", None, dS, dP, ar, ai, be, VR=dVR, stream=stream.cuda_stream)
+        assert info == 0 and m == n, (m, info)
+        return m
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log("[bench] warmup done")
+    X.profile_enable(True)
+    t0 = time.perf_counter()
+    m = 0
+    for _ in range(args.steps):
+        m += step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    X.profile_enable(False)
+    prof = X.profile_read()
+    total_s = t1 - t0
+    ms_per_step = 1000.0 * total_s / args.steps
+    value = m / total_s
+    log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, {value:.1f} eigvecs/s")
+    for k, v in prof.items():
+        if v["launches"]:
+            avg = v["ms"] / v["launches"]
+            log(f"[bench]   {k:7s} launches={v['launches'] // args.steps}/step  total={v['ms'] / args.steps:.3f} "
+                f"ms/step  avg={avg * 1000:.1f} us  {v['flops'] / (v['ms'] * 1e9):.1f} TFLOP/s  "
+                f"{v['bytes'] / (v['ms'] * 1e6):.1f} GB/s")
+
+    # algorithmic flops of the whole path: sum_k 2k^2 (solve) + 2n(k+1) (back-transform), pairs on
+    # both columns
+    kk = np.arange(n, dtype=np.float64)
+    flops = float(np.sum(2 * kk * kk) + np.sum(2 * n * (kk + 1)))
+    path_tflops = flops * args.steps / total_s / 1e12
+
+    # dominant kernel: the one with the largest share of the timed region
+    dom = max((k for k in prof if prof[k]["launches"]), key=lambda k: prof[k]["ms"])
+    d = prof[dom]
+    avg_ms = d["ms"] / d["launches"]
+    if dom in ("update", "back"):
+        ach = d["flops"] / d["launches"] / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": None, "kernel": f"k_gemm_f64<{0 if dom == 'update' else 1}>",
+                "avg_launch_ms": round(avg_ms, 4), "launches_per_step": d["launches"] // args.steps}
+    else:
+        ach = d["bytes"] / d["launches"] / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "kernel": "k_solve_real<65>",
+                "avg_launch_ms": round(avg_ms, 4), "launches_per_step": d["launches"] // args.steps}
+    kernels = {k: {"ms_per_step": round(v["ms"] / args.steps, 4), "launches_per_step": v["launches"] // args.steps,
+                   "tflops": round(v["flops"] / (v["ms"] * 1e9), 3) if v["ms"] else None}
+               for k, v in prof.items() if v["launches"]}
+
+    res = None
+    if not args.no_residual:
+        step()
+        torch.cuda.synchronize()
+        res, ncols = residual_sample(S, P, ar, ai, be, dZ, dVR)
+        log(f"[bench] residual (metric A) over {ncols} sampled columns: {res:.3f} n*eps")
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(n, args.seed)
+        if cpu:
+            log(f"[bench] cpu baseline {cpu['value']:.1f} eigvecs/s ({cpu['sample']})")
+
+    out = {
+        "metric": "eigvecs/s at n=8192: mixed vs plain FP64 GPU at fixed residual; bwd/fwd ratio",
+        "value": round(value, 2),
+        "unit": "eigvecs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"dtgevc3 side=R howmny=B n={n} plain FP64, quasi-triangular pencil "
+                               f"(40% 2x2 pairs), Z orthogonal", "n": n, "side": "R", "howmny": "B",
+                   "form": "plain-fp64", "path_tflops": round(path_tflops, 3),
+                   "max_residual_n_eps": None if res is None else round(res, 4)},
+        "roofline": roof,
+        "kernels": kernels,
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

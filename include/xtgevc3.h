@@ -1,0 +1,90 @@
+/*
+ * libxtgevc3 -- C ABI of the MI355X (gfx950) xTGEVC3 eigenvector path.
+ *
+ * Every entry point takes plain pointers and sizes.  Matrices are column-major.
+ * The host-pointer entry points are drop-ins for the reference's interface
+ * (vlovero/xTGEVC3 include/tgevc3.h:10-13) with C linkage; the C++-linkage symbols
+ * dtgevc3/stgevc3/ztgevc3/ctgevc3 with the reference's exact declarations are exported by
+ * the same library (see include/tgevc3.h).
+ *
+ * Output vectors are normalised as LAPACK xTGEVC documents: each vector is scaled by a
+ * positive factor so that its largest component has |Re| + |Im| = 1 (for a complex pair of
+ * the real interface, over both columns).  The reference leaves vectors unnormalised; they
+ * differ from its output by that positive factor only.
+ *
+ * info: 0, -2 (bad howmny), -16 (mm < m; m already written), -19 (lwork too small) with
+ * the reference's meaning (src/dtgevc3.cpp:1179-1242, src/ztgevc3.cpp:462-516);
+ * XTGEVC3_INFO_DEVICE_ERROR when the GPU path fails (message: xtgevc3_last_error()).
+ */
+#ifndef XTGEVC3_H
+#define XTGEVC3_H
+
+#ifdef __cplusplus
+#include <complex>
+typedef std::complex<double> xtgevc3_zcomplex;
+typedef std::complex<float> xtgevc3_ccomplex;
+extern "C" {
+#else
+#include <complex.h>
+typedef double _Complex xtgevc3_zcomplex;
+typedef float _Complex xtgevc3_ccomplex;
+#endif
+
+#define XTGEVC3_ABI_VERSION 1
+#define XTGEVC3_INFO_DEVICE_ERROR (-1000)
+
+/* precision selector of the *_ex entry points */
+#define XTGEVC3_PREC_FP64 0      /* all arithmetic in FP64 (complex128) */
+#define XTGEVC3_PREC_MIXED_F32 1 /* updates/back-transform on FP32 MFMA, FP64 solves */
+#define XTGEVC3_PREC_MIXED_BF16X3 2 /* updates/back-transform on split-BF16 MFMA */
+
+int xtgevc3_abi_version(void);
+const char *xtgevc3_last_error(void);
+
+/* Host-pointer drop-ins.  Replace dtgevc3/stgevc3/ztgevc3/ctgevc3 of the reference
+ * (include/tgevc3.h:10-13; src/dtgevc3.cpp:1150, src/stgevc3.cpp:1122,
+ * src/ztgevc3.cpp:435, src/ctgevc3.cpp:435).  Same arguments, same info/m/lwork rules. */
+void xtgevc3_dtgevc3(char side, char howmny, const int *select, int n, const double *S, int lds, const double *P,
+                     int ldp, const double *alphar, const double *alphai, const double *beta, double *VL, int ldvl,
+                     double *VR, int ldvr, int mm, int *m, double *work, int lwork, int *info);
+void xtgevc3_stgevc3(char side, char howmny, const int *select, int n, const float *S, int lds, const float *P,
+                     int ldp, const float *alphar, const float *alphai, const float *beta, float *VL, int ldvl,
+                     float *VR, int ldvr, int mm, int *m, float *work, int lwork, int *info);
+void xtgevc3_ztgevc3(char side, char howmny, const int *select, int n, const xtgevc3_zcomplex *S, int lds,
+                     const xtgevc3_zcomplex *P, int ldp, const xtgevc3_zcomplex *alpha, const xtgevc3_zcomplex *beta,
+                     xtgevc3_zcomplex *VL, int ldvl, xtgevc3_zcomplex *VR, int ldvr, int mm, int *m,
+                     xtgevc3_zcomplex *work, int lwork, int *info);
+void xtgevc3_ctgevc3(char side, char howmny, const int *select, int n, const xtgevc3_ccomplex *S, int lds,
+                     const xtgevc3_ccomplex *P, int ldp, const xtgevc3_ccomplex *alpha, const xtgevc3_ccomplex *beta,
+                     xtgevc3_ccomplex *VL, int ldvl, xtgevc3_ccomplex *VR, int ldvr, int mm, int *m,
+                     xtgevc3_ccomplex *work, int lwork, int *info);
+
+/* Device-pointer variants (no counterpart in the reference, which is host-only; they are
+ * the same call as the drop-ins above without the staging copies).  S, P, VL, VR are device
+ * arrays; select, alphar/alphai/beta (alpha/beta) are host arrays; stream is a hipStream_t
+ * (NULL: the default stream).  work/lwork are not taken.  Returns 0, or
+ * XTGEVC3_INFO_DEVICE_ERROR on a HIP failure. */
+int xtgevc3_dtgevc3_device(char side, char howmny, const int *select, int n, const double *S, int lds,
+                           const double *P, int ldp, const double *alphar, const double *alphai, const double *beta,
+                           double *VL, int ldvl, double *VR, int ldvr, int mm, int *m, int *info, void *stream);
+int xtgevc3_ztgevc3_device(char side, char howmny, const int *select, int n, const xtgevc3_zcomplex *S, int lds,
+                           const xtgevc3_zcomplex *P, int ldp, const xtgevc3_zcomplex *alpha,
+                           const xtgevc3_zcomplex *beta, xtgevc3_zcomplex *VL, int ldvl, xtgevc3_zcomplex *VR,
+                           int ldvr, int mm, int *m, int *info, void *stream);
+
+/* Live kernel timing (no counterpart in the reference).  When enabled, every launch of the
+ * hot kernels is bracketed by HIP events on its stream; xtgevc3_profile_read synchronises,
+ * returns per-kind totals (milliseconds, launches, algorithmic flops and bytes) and resets.
+ * Kinds: */
+#define XTGEVC3_PROF_SOLVE 0  /* per-column diagonal-block solves */
+#define XTGEVC3_PROF_UPDATE 1 /* off-diagonal RHS update GEMM */
+#define XTGEVC3_PROF_BACK 2   /* back-transform GEMM */
+#define XTGEVC3_PROF_OTHER 3  /* copies, scale fix-up, normalisation */
+#define XTGEVC3_PROF_KINDS 4
+void xtgevc3_profile_enable(int on);
+int xtgevc3_profile_read(double *ms, long *launches, double *flops, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

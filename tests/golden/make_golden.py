@@ -1,0 +1,137 @@
+"""Writes the golden fixtures of tests/golden/*.npz.
+
+Inputs are the reference tests' own known-answer pencils (vlovero/xTGEVC3
+tests/test_dtgevc3.cpp:318-434, tests/test_ztgevc3.cpp:146-201) and seeded random pencils in
+the reference tests' construction (tests/test_lapack_d.cpp:170-237, test_lapack_z.cpp).
+Expected outputs are the raw (unnormalised) VL/VR, m and info returned by the reference
+library itself, compiled from its sources by ``make -C oracle ref`` (oracle/_ref, never
+committed).  Run from the repository root:  python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as O  # noqa: E402
+from xtgevc3_amd import pencils as G  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+# 4x4 known-answer pencil of tests/test_dtgevc3.cpp:322-326 (column-major literals)
+KAT_S = np.array([1, 0, 0, 0, 2, 2, -5, 0, 3, 5, 2, 0, 4, -1, 2, 3], dtype=np.float64).reshape(4, 4).T
+KAT_P = np.array([2, 0, 0, 0, 1, 1, 0, 0, -1, 0, 1, 0, 3, 2, -1, 2], dtype=np.float64).reshape(4, 4).T
+KAT_AR = np.array([1.0, 2.0, 2.0, 3.0])
+KAT_AI = np.array([0.0, 5.0, -5.0, 0.0])
+KAT_BE = np.array([2.0, 1.0, 1.0, 2.0])
+
+
+def kat_complex():
+    # tests/test_ztgevc3.cpp:158-184
+    S = np.zeros((4, 4), dtype=np.complex128)
+    P = np.zeros((4, 4), dtype=np.complex128)
+    S[0, 0], P[0, 0] = 1 + 1j, 2
+    S[0, 1], P[0, 1] = 2, 1 + 1j
+    S[1, 1], P[1, 1] = 2 - 1j, 1 - 1j
+    S[0, 2], P[0, 2] = -5 + 1j, 0
+    S[1, 2], P[1, 2] = 2j, -1
+    S[2, 2], P[2, 2] = 3, 1 + 1j
+    S[0, 3], P[0, 3] = 5, 0
+    S[1, 3], P[1, 3] = 2 + 1j, 2j
+    S[2, 3], P[2, 3] = 4 - 1j, 3
+    S[3, 3], P[3, 3] = 3 + 2j, 2 - 1j
+    return np.asfortranarray(S), np.asfortranarray(P), np.diag(S).copy(), np.diag(P).copy()
+
+
+def run_ref_real(prefix, side, howmny, select, S, P, ar, ai, be, vinit):
+    dt = np.float64 if prefix == "d" else np.float32
+    n = S.shape[0]
+    VL = np.asfortranarray(vinit.astype(dt).copy())
+    VR = np.asfortranarray(vinit.astype(dt).copy())
+    m, info, _ = O.tgevc3(prefix, side, howmny, select, S.astype(dt), P.astype(dt), ar.astype(dt), be.astype(dt),
+                          alphai=ai.astype(dt), VL=VL, VR=VR, mm=n, impl="ref")
+    return VL, VR, m, info
+
+
+def run_ref_cplx(prefix, side, howmny, select, S, P, al, be, vinit):
+    dt = np.complex128 if prefix == "z" else np.complex64
+    n = S.shape[0]
+    VL = np.asfortranarray(vinit.astype(dt).copy())
+    VR = np.asfortranarray(vinit.astype(dt).copy())
+    m, info, _ = O.tgevc3(prefix, side, howmny, select, S.astype(dt), P.astype(dt), al.astype(dt), be.astype(dt),
+                          VL=VL, VR=VR, mm=n, impl="ref")
+    return VL, VR, m, info
+
+
+def save(name, **kw):
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **kw)
+    print("wrote", name)
+
+
+def main():
+    assert O.have_ref(), "build the reference first: make -C oracle ref"
+    eye4 = np.eye(4)
+    # real KATs: plain, infinite eigenvalue (P33 = beta3 = 0), scaled by 1e150 / 1e-150
+    cases = {"kat_d": (KAT_S, KAT_P, KAT_AR, KAT_AI, KAT_BE)}
+    Pinf = KAT_P.copy()
+    Pinf[3, 3] = 0.0
+    binf = KAT_BE.copy()
+    binf[3] = 0.0
+    cases["kat_d_inf"] = (KAT_S, Pinf, KAT_AR, KAT_AI, binf)
+    for tag, s in (("kat_d_big", 1e150), ("kat_d_small", 1e-150)):
+        cases[tag] = (KAT_S * s, KAT_P * s, KAT_AR * s, KAT_AI * s, KAT_BE * s)
+    for tag, (S, P, ar, ai, be) in cases.items():
+        VL, VR, m, info = run_ref_real("d", "B", "B", None, S, P, ar, ai, be, eye4)
+        save(tag, S=S, P=P, ar=ar, ai=ai, be=be, side="B", howmny="B", VL=VL, VR=VR, m=m, info=info)
+    # float KAT scaled by 1e30 / 1e-30 (tests/test_stgevc3.cpp:241-278)
+    for tag, s in (("kat_s_big", 1e30), ("kat_s_small", 1e-30)):
+        S, P = (KAT_S * s).astype(np.float32), (KAT_P * s).astype(np.float32)
+        ar, ai, be = (KAT_AR * s).astype(np.float32), (KAT_AI * s).astype(np.float32), (KAT_BE * s).astype(np.float32)
+        VL, VR, m, info = run_ref_real("s", "B", "B", None, S, P, ar, ai, be, eye4)
+        save(tag, S=S, P=P, ar=ar, ai=ai, be=be, side="B", howmny="B", VL=VL, VR=VR, m=m, info=info)
+    # complex KAT
+    S, P, al, be = kat_complex()
+    VL, VR, m, info = run_ref_cplx("z", "B", "B", None, S, P, al, be, eye4)
+    save("kat_z", S=S, P=P, al=al, be=be, side="B", howmny="B", VL=VL, VR=VR, m=m, info=info)
+
+    # seeded random pencils in the reference tests' construction
+    for n, seed in ((10, 100), (40, 101), (75, 102)):
+        S, P, ar, ai, be = G.reference_style_real(n, seed)
+        rng = np.random.default_rng(seed)
+        Q = G.random_orthogonal(n, seed + 1)
+        for howmny in ("A", "B", "S"):
+            sel = (rng.random(n) < 0.4).astype(np.int32) if howmny == "S" else None
+            vin = Q if howmny == "B" else np.zeros((n, n))
+            VL, VR, m, info = run_ref_real("d", "B", howmny, sel, S, P, ar, ai, be, vin)
+            save(f"rand_d_n{n}_{howmny}", S=S, P=P, ar=ar, ai=ai, be=be, side="B", howmny=howmny,
+                 sel=np.array([]) if sel is None else sel, Vin=vin, VL=VL, VR=VR, m=m, info=info)
+        Sz, Pz, al, bz = G.reference_style_complex(n, seed)
+        Qz = G.random_unitary(n, seed + 2)
+        for howmny in ("A", "B", "S"):
+            sel = (rng.random(n) < 0.4).astype(np.int32) if howmny == "S" else None
+            vin = Qz if howmny == "B" else np.zeros((n, n), dtype=np.complex128)
+            VL, VR, m, info = run_ref_cplx("z", "B", howmny, sel, Sz, Pz, al, bz, vin)
+            save(f"rand_z_n{n}_{howmny}", S=Sz, P=Pz, al=al, be=bz, side="B", howmny=howmny,
+                 sel=np.array([]) if sel is None else sel, Vin=vin, VL=VL, VR=VR, m=m, info=info)
+
+    # near-overflow / underflow scaling cases (tests/test_lapack_d.cpp:328-344), n = 40
+    safemin = np.finfo(np.float64).tiny
+    eps = np.finfo(np.float64).eps
+    smlnum = safemin / eps
+    bignum = 1.0 / smlnum
+    n = 40
+    S0, P0, ar0, ai0, be0 = G.reference_style_real(n, 200)
+    mask = np.triu(np.ones((n, n)), -1)
+    for tag, sS, sP in (("ovS", bignum, 1.0), ("unS", smlnum, 1.0), ("ovP", 1.0, bignum), ("unP", 1.0, smlnum),
+                        ("ovSP", bignum, bignum), ("unSP", smlnum, smlnum)):
+        S = np.asfortranarray(S0 * mask * sS)
+        P = np.asfortranarray(np.triu(P0) * sP)
+        ar, ai, be = ar0 * sS, ai0 * sS, be0 * sP
+        VL, VR, m, info = run_ref_real("d", "B", "B", None, S, P, ar, ai, be, np.eye(n))
+        save(f"scale_d_{tag}", S=S, P=P, ar=ar, ai=ai, be=be, side="B", howmny="B", VL=VL, VR=VR, m=m, info=info)
+
+
+if __name__ == "__main__":
+    main()

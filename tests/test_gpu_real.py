@@ -1,0 +1,82 @@
+"""GPU parity of the real FP64 path (dtgevc3) against the oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+from tests.helpers import check_real_parity, run_real
+from xtgevc3_amd import pencils as G
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 33, 64, 65, 66, 130, 257])
+@pytest.mark.parametrize("gen", ["ref", "quasi", "qz"])
+def test_real_both_back(gen, n):
+    if gen == "ref":
+        S, P, ar, ai, be = G.reference_style_real(n, 11 + n)
+        Q = G.random_orthogonal(n, 1)
+        Z = G.random_orthogonal(n, 2)
+    elif gen == "quasi":
+        S, P, ar, ai, be = G.real_quasi_triangular(n, 21 + n)
+        Q = G.random_orthogonal(n, 3)
+        Z = G.random_orthogonal(n, 4)
+    else:
+        if n < 2:
+            pytest.skip("qz needs n >= 2")
+        S, P, Q, Z, ar, ai, be = G.qz_pencil(n, 31 + n)
+    check_real_parity(S, P, ar, ai, be, "B", "B", None, Q, Z, cond=n <= 130)
+
+
+@pytest.mark.parametrize("howmny", ["A", "S"])
+@pytest.mark.parametrize("side", ["R", "L", "B"])
+def test_real_all_selected(side, howmny):
+    n = 150
+    S, P, ar, ai, be = G.real_quasi_triangular(n, 5)
+    rng = np.random.default_rng(7)
+    sel = (rng.random(n) < 0.3).astype(np.int32) if howmny == "S" else None
+    check_real_parity(S, P, ar, ai, be, side, howmny, sel, cond=True)
+
+
+def test_real_select_edge():
+    n = 70
+    S, P, ar, ai, be = G.real_quasi_triangular(n, 9)
+    sel = np.zeros(n, dtype=np.int32)
+    sel[n - 1] = 1
+    sel[0] = 1
+    # pair members selected through their second column
+    pk = [k for k in range(n - 1) if ai[k] > 0]
+    sel[pk[0] + 1] = 1
+    check_real_parity(S, P, ar, ai, be, "B", "S", sel, cond=True)
+
+
+def test_real_info_codes_gpu():
+    n = 20
+    S, P, ar, ai, be = G.real_quasi_triangular(n, 3)
+    gl = run_real("gpu", "R", "X", None, S, P, ar, ai, be)
+    ol = run_real("oracle", "R", "X", None, S, P, ar, ai, be)
+    assert gl[3] == ol[3] == -2
+    gm = run_real("gpu", "R", "A", None, S, P, ar, ai, be, mm=5)
+    om = run_real("oracle", "R", "A", None, S, P, ar, ai, be, mm=5)
+    assert gm[2:] == om[2:] == (n, -16)
+
+
+def test_real_device_api():
+    import torch
+
+    import xtgevc3_amd as X
+    from oracle import oracle as O
+
+    n = 200
+    S, P, ar, ai, be = G.real_quasi_triangular(n, 13)
+    Z = G.random_orthogonal(n, 14)
+    dS = X.fortran_device(S)
+    dP = X.fortran_device(P)
+    dVR = X.fortran_device(Z)
+    m, info = X.dtgevc3_device("R", "B", None, dS, dP, ar, ai, be, VR=dVR)
+    torch.cuda.synchronize()
+    assert (m, info) == (n, 0)
+    g = X.from_fortran_device(dVR)
+    VR = Z.copy(order="F")
+    O.tgevc3("d", "R", "B", None, S, P, ar, be, alphai=ai, VR=VR)
+    kind, idx = O.real_layout("B", None, ai)
+    O.normalize(VR, kind)
+    assert np.max(np.abs(g - VR)) < 1e-8

@@ -1,0 +1,194 @@
+"""Pins the oracle (CPU restatement, oracle/tgevc3_oracle.cpp) before it is trusted:
+  * against the golden fixtures written by the reference library itself (tests/golden),
+  * against the reference library compiled in this container (oracle/_ref), when built,
+  * against LAPACK xTGEVC (independent algorithm, same eigenvectors up to scale / phase),
+  * against residual bounds (the pencil equation itself).
+A planted mistake in the oracle (dropped term, wrong sign, transposed operand) fails at
+least the golden and LAPACK comparisons."""
+import ctypes
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from xtgevc3_amd import pencils as G
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+EPS = np.finfo(np.float64).eps
+
+
+def _golden(name):
+    return dict(np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False))
+
+
+def _golden_names(prefix):
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLD, prefix + "*.npz")))
+
+
+def _rel(a, b):
+    scale = max(np.max(np.abs(b)), np.finfo(np.float64).tiny)
+    return np.max(np.abs(a - b)) / scale
+
+
+@pytest.mark.parametrize("name", _golden_names("kat_d") + _golden_names("rand_d") + _golden_names("scale_d"))
+def test_oracle_real_matches_golden_raw(name):
+    g = _golden(name)
+    n = g["S"].shape[0]
+    howmny = str(g["howmny"])
+    sel = g["sel"].astype(np.int32) if "sel" in g and g["sel"].size else None
+    vin = g.get("Vin", np.eye(n))
+    VL = np.asfortranarray(vin.copy())
+    VR = np.asfortranarray(vin.copy())
+    m, info, _ = O.tgevc3("d", str(g["side"]), howmny, sel, g["S"], g["P"], g["ar"], g["be"], alphai=g["ai"],
+                          VL=VL, VR=VR)
+    assert (m, info) == (int(g["m"]), int(g["info"]))
+    # raw vectors, scale included: same algorithm, BLAS rounding aside
+    for mine, ref in ((VL, g["VL"]), (VR, g["VR"])):
+        for c in range(m):
+            assert _rel(mine[:, c], ref[:, c]) < 1e-9, (name, c)
+
+
+@pytest.mark.parametrize("name", _golden_names("kat_z") + _golden_names("rand_z"))
+def test_oracle_complex_matches_golden_raw(name):
+    g = _golden(name)
+    n = g["S"].shape[0]
+    sel = g["sel"].astype(np.int32) if "sel" in g and g["sel"].size else None
+    vin = g.get("Vin", np.eye(n, dtype=np.complex128))
+    VL = np.asfortranarray(vin.astype(np.complex128).copy())
+    VR = np.asfortranarray(vin.astype(np.complex128).copy())
+    m, info, _ = O.tgevc3("z", str(g["side"]), str(g["howmny"]), sel, g["S"], g["P"], g["al"], g["be"], VL=VL, VR=VR)
+    assert (m, info) == (int(g["m"]), int(g["info"]))
+    for mine, ref in ((VL, g["VL"]), (VR, g["VR"])):
+        for c in range(m):
+            assert _rel(mine[:, c], ref[:, c]) < 1e-9, (name, c)
+
+
+@pytest.mark.parametrize("name", ["kat_s_big", "kat_s_small"])
+def test_oracle_float_matches_golden(name):
+    g = _golden(name)
+    VL = np.asfortranarray(np.eye(4, dtype=np.float32))
+    VR = np.asfortranarray(np.eye(4, dtype=np.float32))
+    m, info, _ = O.tgevc3("s", "B", "B", None, g["S"], g["P"], g["ar"], g["be"], alphai=g["ai"], VL=VL, VR=VR)
+    assert (m, info) == (int(g["m"]), int(g["info"]))
+    assert _rel(VL, g["VL"]) < 1e-5 and _rel(VR, g["VR"]) < 1e-5
+
+
+def _ref_or_skip():
+    if not O.have_ref():
+        pytest.skip("reference library not built here (needs /root/reference)")
+
+
+@pytest.mark.parametrize("prefix", ["d", "s", "z", "c"])
+@pytest.mark.parametrize("n", [1, 3, 31, 32, 33, 34, 65, 100])
+def test_oracle_matches_reference_library(prefix, n):
+    _ref_or_skip()
+    rng = np.random.default_rng(1000 + n)
+    for side in ("R", "L", "B"):
+        for howmny in ("A", "B", "S"):
+            sel = (rng.random(n) < 0.5).astype(np.int32) if howmny == "S" else None
+            if prefix in "ds":
+                S, P, ar, ai, be = G.reference_style_real(n, 7 * n + 1)
+                dt = np.float64 if prefix == "d" else np.float32
+                vin = G.random_orthogonal(n, 5) if howmny == "B" else np.zeros((n, n))
+                outs = []
+                for impl in ("oracle", "ref"):
+                    VL = np.asfortranarray(vin.astype(dt))
+                    VR = np.asfortranarray(vin.astype(dt))
+                    r = O.tgevc3(prefix, side, howmny, sel, S.astype(dt), P.astype(dt), ar.astype(dt), be.astype(dt),
+                                 alphai=ai.astype(dt), VL=VL, VR=VR, impl=impl)
+                    outs.append((VL, VR, r[0], r[1]))
+            else:
+                S, P, al, be = G.reference_style_complex(n, 7 * n + 2)
+                dt = np.complex128 if prefix == "z" else np.complex64
+                vin = G.random_unitary(n, 6) if howmny == "B" else np.zeros((n, n))
+                outs = []
+                for impl in ("oracle", "ref"):
+                    VL = np.asfortranarray(vin.astype(dt))
+                    VR = np.asfortranarray(vin.astype(dt))
+                    r = O.tgevc3(prefix, side, howmny, sel, S.astype(dt), P.astype(dt), al.astype(dt), be.astype(dt),
+                                 VL=VL, VR=VR, impl=impl)
+                    outs.append((VL, VR, r[0], r[1]))
+            (aL, aR, am, ai_), (bL, bR, bm, bi) = outs
+            assert (am, ai_) == (bm, bi)
+            tol = 1e-9 if prefix in "dz" else 2e-3
+            for x, y in ((aL, bL), (aR, bR)):
+                for c in range(am):
+                    assert _rel(x[:, c], y[:, c]) < tol, (prefix, n, side, howmny, c)
+
+
+def _lapack_dtgevc(S, P):
+    lib = O._openblas()
+    n = S.shape[0]
+    VL = np.zeros((n, n), order="F")
+    VR = np.zeros((n, n), order="F")
+    work = np.zeros(6 * n + 8)
+    m = ctypes.c_int(0)
+    info = ctypes.c_int(0)
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    Sf = np.asfortranarray(S)
+    Pf = np.asfortranarray(P)
+    lib.scipy_dtgevc_(ctypes.c_char_p(b"B"), ctypes.c_char_p(b"A"), None, ci(n), O._ptr(Sf), ci(n), O._ptr(Pf),
+                      ci(n), O._ptr(VL), ci(n), O._ptr(VR), ci(n), ci(n), ctypes.byref(m), O._ptr(work),
+                      ctypes.byref(info), ctypes.c_size_t(1), ctypes.c_size_t(1))
+    assert info.value == 0
+    return VL, VR
+
+
+def _phase_invariant_err(x, y):
+    """max|c x - y| / max|y| with c the least-squares complex factor (x^H y)/(x^H x): the
+    two vectors span the same line whatever phase convention each one follows."""
+    c = np.vdot(x, y) / np.vdot(x, x)
+    return np.max(np.abs(c * x - y)) / np.max(np.abs(y))
+
+
+@pytest.mark.parametrize("n", [5, 40, 90])
+def test_oracle_matches_lapack_directions(n):
+    S, P, Q, Z, ar, ai, be = G.qz_pencil(n, 300 + n)
+    kind, idx = O.real_layout("A", None, ai)
+    al, bb = O.eig_alpha_beta(ar, ai, be, idx)
+    VL = np.zeros((n, n), order="F")
+    VR = np.zeros((n, n), order="F")
+    O.tgevc3("d", "B", "A", None, S, P, ar, be, alphai=ai, VL=VL, VR=VR)
+    O.normalize(VL, kind)
+    O.normalize(VR, kind)
+    LL, LR = _lapack_dtgevc(S, P)
+    for left, mine, lap in ((True, VL, LL), (False, VR, LR)):
+        ck = O.eigvec_cond(S, P, al, bb, kind, idx, left=left)
+        a = O.complex_columns(mine, kind)
+        b = O.complex_columns(lap, kind)
+        for c in range(n):
+            err = _phase_invariant_err(a[:, c], b[:, c])
+            assert err < 1000 * EPS * max(ck[c], 1.0), (left, c, err, ck[c])
+            if kind[c] == 0:
+                # real vectors: same sign convention up to a positive factor
+                assert np.max(np.abs(mine[:, c] - lap[:, c])) < 1000 * EPS * max(ck[c], 1.0)
+
+
+@pytest.mark.parametrize("n", [4, 50, 120])
+def test_oracle_residuals(n):
+    S, P, ar, ai, be = G.real_quasi_triangular(n, 400 + n)
+    kind, idx = O.real_layout("A", None, ai)
+    al, bb = O.eig_alpha_beta(ar, ai, be, idx)
+    VL = np.zeros((n, n), order="F")
+    VR = np.zeros((n, n), order="F")
+    O.tgevc3("d", "B", "A", None, S, P, ar, be, alphai=ai, VL=VL, VR=VR)
+    for left, V in ((False, VR), (True, VL)):
+        r = O.residual_B(S, P, O.complex_columns(V, kind), al, bb, left=left)
+        assert r.max() < 10 * n, r.max()
+    Sz, Pz, alz, bz = G.complex_triangular(n, 500 + n)
+    VL = np.zeros((n, n), dtype=np.complex128, order="F")
+    VR = np.zeros((n, n), dtype=np.complex128, order="F")
+    O.tgevc3("z", "B", "A", None, Sz, Pz, alz, bz, VL=VL, VR=VR)
+    for left, V in ((False, VR), (True, VL)):
+        r = O.residual_B(Sz, Pz, V, alz, bz, left=left)
+        assert r.max() < 10 * n, r.max()
+
+
+def test_oracle_normalisation_convention():
+    V = np.asfortranarray(np.array([[2.0, 1.0, -3.0], [-4.0, 1.0, 0.5], [1.0, -2.0, 0.0]]))
+    kind = np.array([0, 1, 2], dtype=np.int32)
+    O.normalize(V, kind)
+    assert np.max(np.abs(V[:, 0])) == 1.0 and V[1, 0] == -1.0
+    assert np.isclose(np.max(np.abs(V[:, 1]) + np.abs(V[:, 2])), 1.0)

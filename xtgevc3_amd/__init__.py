@@ -1,0 +1,209 @@
+"""xtgevc3_amd -- MI355X (gfx950) implementation of the blocked xTGEVC3 eigenvector path.
+
+Python mirror of the reference's operator interface (vlovero/xTGEVC3 include/tgevc3.h:10-13):
+``dtgevc3``/``stgevc3``/``ztgevc3``/``ctgevc3`` take the same arguments with the same
+meaning (side, howmny, select, S, P, eigenvalues, VL, VR, mm, lwork) and return
+``(m, info, work0)`` where ``work0`` is ``work[0]`` after the call (the workspace size on a
+``lwork=-1`` query).  VL/VR are Fortran-ordered numpy arrays updated in place, as the
+reference updates its VL/VR arrays.
+
+``*_device`` variants take torch tensors already resident on the GPU (column-major
+matrices passed as their transposed row-major storage, see ``_fortran``).
+
+Everything calls into ``libxtgevc3.so`` through its C ABI (include/xtgevc3.h).  There is no
+CPU fallback: if the library is missing, import fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libxtgevc3.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+                      "or `make -C xtgevc3_amd/csrc`")
+
+_lib = ctypes.CDLL(LIB_PATH)
+_lib.xtgevc3_last_error.restype = ctypes.c_char_p
+_lib.xtgevc3_abi_version.restype = ctypes.c_int
+
+INFO_DEVICE_ERROR = -1000
+PREC_FP64, PREC_MIXED_F32, PREC_MIXED_BF16X3 = 0, 1, 2
+
+EXPORTED_SYMBOLS = [
+    "xtgevc3_abi_version", "xtgevc3_last_error",
+    "xtgevc3_dtgevc3", "xtgevc3_stgevc3", "xtgevc3_ztgevc3", "xtgevc3_ctgevc3",
+    "xtgevc3_dtgevc3_device", "xtgevc3_ztgevc3_device",
+    "xtgevc3_profile_enable", "xtgevc3_profile_read",
+]
+
+PROF_KINDS = ("solve", "update", "back", "other")
+
+
+def profile_enable(on=True):
+    _lib.xtgevc3_profile_enable(ctypes.c_int(1 if on else 0))
+
+
+def profile_read():
+    """-> {kind: dict(ms=total, launches=, flops=, bytes=)}; synchronises and resets."""
+    k = len(PROF_KINDS)
+    ms = (ctypes.c_double * k)()
+    la = (ctypes.c_long * k)()
+    fl = (ctypes.c_double * k)()
+    by = (ctypes.c_double * k)()
+    _lib.xtgevc3_profile_read(ms, la, fl, by)
+    return {PROF_KINDS[i]: dict(ms=ms[i], launches=la[i], flops=fl[i], bytes=by[i]) for i in range(k)}
+
+
+def lib():
+    return _lib
+
+
+def last_error():
+    return _lib.xtgevc3_last_error().decode()
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+_REAL = {"d": np.float64, "s": np.float32}
+_CPLX = {"z": np.complex128, "c": np.complex64}
+
+
+def _ld(a):
+    return max(1, a.shape[0]) if a is not None else 1
+
+
+def tgevc3(prefix, side, howmny, select, S, P, alpha, beta, alphai=None, VL=None, VR=None, mm=None, lwork=None,
+           work=None):
+    """Call ``xtgevc3_{prefix}tgevc3`` (host pointers) with the reference's argument meaning.
+
+    For the real prefixes ('d', 's') ``alpha`` is alphar and ``alphai`` must be given.
+    ``lwork=None`` performs the reference's two-call idiom (query, then call).  Returns
+    (m, info, work0)."""
+    real = prefix in _REAL
+    dt = _REAL[prefix] if real else _CPLX[prefix]
+    fn = getattr(_lib, f"xtgevc3_{prefix}tgevc3")
+    n = S.shape[0]
+    assert S.dtype == dt and P.dtype == dt and S.flags.f_contiguous and P.flags.f_contiguous
+    for V in (VL, VR):
+        if V is not None:
+            assert V.dtype == dt and V.flags.f_contiguous
+    if mm is None:
+        mm = n
+    sel = None if select is None else np.ascontiguousarray(select, dtype=np.int32)
+    m = ctypes.c_int(-1)
+    info = ctypes.c_int(-99)
+    if lwork is None:
+        q = np.zeros(1, dtype=dt)
+        _invoke(fn, real, side, howmny, sel, n, S, P, alpha, alphai, beta, dt, VL, VR, mm, m, q, -1, info)
+        if info.value != 0 or n == 0:
+            return m.value, info.value, q[0]
+        lwork = int(np.real(q[0]))
+    if work is None:
+        work = np.zeros(max(lwork, 1), dtype=dt)
+    _invoke(fn, real, side, howmny, sel, n, S, P, alpha, alphai, beta, dt, VL, VR, mm, m, work, lwork, info)
+    if info.value == INFO_DEVICE_ERROR:
+        raise RuntimeError("xtgevc3 device path failed: " + last_error())
+    return m.value, info.value, work[0]
+
+
+def _invoke(fn, real, side, howmny, sel, n, S, P, alpha, alphai, beta, dt, VL, VR, mm, m, work, lwork, info):
+    c = ctypes
+    if real:
+        ar = np.ascontiguousarray(alpha, dtype=dt)
+        ai = np.ascontiguousarray(alphai, dtype=dt)
+        be = np.ascontiguousarray(beta, dtype=dt)
+        ev = (_ptr(ar), _ptr(ai), _ptr(be))
+    else:
+        al = np.ascontiguousarray(alpha, dtype=dt)
+        be = np.ascontiguousarray(beta, dtype=dt)
+        ev = (_ptr(al), _ptr(be))
+    fn(c.c_char(side.encode()), c.c_char(howmny.encode()), _ptr(sel), c.c_int(n), _ptr(S), c.c_int(_ld(S)),
+       _ptr(P), c.c_int(_ld(P)), *ev, _ptr(VL), c.c_int(_ld(VL)), _ptr(VR), c.c_int(_ld(VR)), c.c_int(mm),
+       c.byref(m), _ptr(work), c.c_int(lwork), c.byref(info))
+
+
+def dtgevc3(side, howmny, select, S, P, alphar, alphai, beta, VL=None, VR=None, mm=None, lwork=None, work=None):
+    return tgevc3("d", side, howmny, select, S, P, alphar, beta, alphai=alphai, VL=VL, VR=VR, mm=mm, lwork=lwork,
+                  work=work)
+
+
+def stgevc3(side, howmny, select, S, P, alphar, alphai, beta, VL=None, VR=None, mm=None, lwork=None, work=None):
+    return tgevc3("s", side, howmny, select, S, P, alphar, beta, alphai=alphai, VL=VL, VR=VR, mm=mm, lwork=lwork,
+                  work=work)
+
+
+def ztgevc3(side, howmny, select, S, P, alpha, beta, VL=None, VR=None, mm=None, lwork=None, work=None):
+    return tgevc3("z", side, howmny, select, S, P, alpha, beta, VL=VL, VR=VR, mm=mm, lwork=lwork, work=work)
+
+
+def ctgevc3(side, howmny, select, S, P, alpha, beta, VL=None, VR=None, mm=None, lwork=None, work=None):
+    return tgevc3("c", side, howmny, select, S, P, alpha, beta, VL=VL, VR=VR, mm=mm, lwork=lwork, work=work)
+
+
+# ---------------------------------------------------------------------------------------
+# device-pointer variants (torch tensors on the GPU)
+# ---------------------------------------------------------------------------------------
+def _tptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def fortran_device(a_np, device="cuda"):
+    """Column-major numpy matrix -> torch tensor on the device holding the same column-major
+    storage (returned as the (cols, rows) row-major tensor == its transpose)."""
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a_np).T)).to(device)
+
+
+def from_fortran_device(t):
+    return np.asfortranarray(t.cpu().numpy().T)
+
+
+def tgevc3_device(prefix, side, howmny, select, S, P, alpha, beta, alphai=None, VL=None, VR=None, mm=None,
+                  stream=None):
+    """Device-pointer call.  S, P, VL, VR are torch tensors holding column-major storage
+    (row-major tensors of shape (cols, ld)); select and eigenvalues are host numpy arrays.
+    Returns (m, info)."""
+    import torch
+
+    c = ctypes
+    real = prefix in _REAL
+    dt = _REAL[prefix] if real else _CPLX[prefix]
+    fn = getattr(_lib, f"xtgevc3_{prefix}tgevc3_device")
+    n = S.shape[0]
+    lds = S.shape[1]
+    ldp = P.shape[1]
+    ldvl = VL.shape[1] if VL is not None else 1
+    ldvr = VR.shape[1] if VR is not None else 1
+    if mm is None:
+        mm = n
+    sel = None if select is None else np.ascontiguousarray(select, dtype=np.int32)
+    m = c.c_int(-1)
+    info = c.c_int(-99)
+    if real:
+        ev = [np.ascontiguousarray(alpha, dtype=dt), np.ascontiguousarray(alphai, dtype=dt),
+              np.ascontiguousarray(beta, dtype=dt)]
+    else:
+        ev = [np.ascontiguousarray(alpha, dtype=dt), np.ascontiguousarray(beta, dtype=dt)]
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    rc = fn(c.c_char(side.encode()), c.c_char(howmny.encode()), _ptr(sel), c.c_int(n), _tptr(S), c.c_int(lds),
+            _tptr(P), c.c_int(ldp), *[_ptr(e) for e in ev], _tptr(VL), c.c_int(ldvl), _tptr(VR), c.c_int(ldvr),
+            c.c_int(mm), c.byref(m), c.byref(info), c.c_void_p(stream))
+    if rc != 0:
+        raise RuntimeError("xtgevc3 device path failed: " + last_error())
+    return m.value, info.value
+
+
+def dtgevc3_device(side, howmny, select, S, P, alphar, alphai, beta, VL=None, VR=None, mm=None, stream=None):
+    return tgevc3_device("d", side, howmny, select, S, P, alphar, beta, alphai=alphai, VL=VL, VR=VR, mm=mm,
+                         stream=stream)
+
+
+def ztgevc3_device(side, howmny, select, S, P, alpha, beta, VL=None, VR=None, mm=None, stream=None):
+    return tgevc3_device("z", side, howmny, select, S, P, alpha, beta, VL=VL, VR=VR, mm=mm, stream=stream)

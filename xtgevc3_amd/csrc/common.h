@@ -1,0 +1,52 @@
+// Shared definitions of the gfx950 xTGEVC3 path: column descriptors, launch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace xtg {
+
+// One eigenvector column of the working problem (acoeff*S_w - bcoeff*P_w) x = 0, solved
+// bottom-up on the upper quasi-triangular working pencil.  Right eigenvectors use the
+// pencil itself; left eigenvectors and the adjoint solve use the reversed transpose
+// J S^T J (J = row/column reversal), so one solver serves every side.
+//
+// A complex-conjugate pair occupies two lanes: one holds the real part of the complex
+// vector, the other the imaginary part; each lane sees its partner through a lane shuffle.
+struct ColDesc {
+    double a;    // acoeff (real, > 0 unless the eigenvalue is infinite)
+    double bR;   // Re(bcoeff) of the working shift
+    double bI;   // Im(bcoeff) of the working shift (0 for real eigenvalues)
+    double pad;
+    int d;       // first row of the column's own diagonal block (init rows); >= n: none
+    int lo;      // rows < lo lie outside the column's system and stay zero
+    int partner; // packed index of the partner lane column (== own index when real)
+    int flags;   // bit0: imaginary-part lane, bit1: member of a pair, bits2-3: init mode
+};
+
+enum : int {
+    INIT_NONE = 0,     // all rows come from the right-hand side held in X
+    INIT_UNIT = 1,     // x_d = 1 (real eigenvalue, or complex interface)
+    INIT_PAIR_ROW1 = 2, // x_{d..d+1} = null vector of the first row of the 2x2 block
+    INIT_PAIR_ROW2 = 3, // x_{d..d+1} = null vector of the second row of the 2x2 block
+};
+
+// Same layout for the complex interface: the working vector entries are complex.
+struct ZColDesc {
+    double aR, aI; // acoeff (complex for the complex interface)
+    double bR, bI; // bcoeff
+    int d, lo, flags, pad;
+};
+
+constexpr int WAVE = 64;
+
+} // namespace xtg
+
+#define XTG_CHECK(x)                                                                               \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) {                                                                    \
+            xtg::set_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + __FILE__ + \
+                           ":" + std::to_string(__LINE__));                                        \
+            return -1000;                                                                          \
+        }                                                                                          \
+    } while (0)

@@ -1,0 +1,41 @@
+// Internal interfaces between the C-ABI layer and the per-type drivers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <complex>
+#include <string>
+#include <vector>
+
+namespace xtg {
+
+// One selected eigenvalue "unit": a real eigenvalue (w = 1) or a complex-conjugate pair
+// (w = 2, columns k and k+1), with its first output column.
+struct Unit {
+    int k;
+    int w;
+    int outc;
+};
+
+void set_error(const std::string &s);
+const char *last_error();
+
+int reference_lwork(int n, int b);
+template <class T>
+bool check_args(char howmny, int n, int num_sel, int mm, int *m, T *work, int lwork, int *info, bool have_work);
+
+int real_units(char howmny, const int *select, int n, const double *ai, std::vector<Unit> &units);
+int complex_units(char howmny, const int *select, int n, std::vector<Unit> &units);
+
+int real_device(char side, char howmny, const int *select, int n, const double *S, long lds, const double *P,
+                long ldp, const double *ar, const double *ai, const double *be, double *VL, long ldvl, double *VR,
+                long ldvr, hipStream_t st);
+
+int complex_device(char side, char howmny, const int *select, int n, const std::complex<double> *S, long lds,
+                   const std::complex<double> *P, long ldp, const std::complex<double> *al,
+                   const std::complex<double> *be, std::complex<double> *VL, long ldvl, std::complex<double> *VR,
+                   long ldvr, hipStream_t st);
+
+// live timing of kernel launches (profile.hip); begin returns -1 when disabled
+int prof_begin(hipStream_t st);
+void prof_end(int idx, hipStream_t st, int kind, double flops, double bytes);
+
+} // namespace xtg

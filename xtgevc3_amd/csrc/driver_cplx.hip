@@ -1,0 +1,296 @@
+// Host driver of the complex128 xTGEVC3 path (ztgevc3; ctgevc3 through it).
+//
+// Reference behaviour restated (src/ztgevc3.cpp):
+//   selection: one column per selected eigenvalue            :476-482
+//   global scales from |z| column sums over rows <= col      :528-545
+//   complex shift coefficients acoeff, bcoeff                :145-153, :619-627
+//   right: (a S - b P) x = 0, x_k = 1                       :134-210
+//   left : y^H (a S - b P) = 0, y_k = 1, via conj(a), conj(b) :281-357, :775-780
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <limits>
+#include <mutex>
+#include <vector>
+
+#include "common.h"
+#include "driver.h"
+
+namespace xtg {
+
+template <int MAXB>
+__global__ void kz_solve(const double2 *, long, const double2 *, long, int, int, double2 *, long, double2 *, long,
+                         const ZColDesc *, int, int, double *, int *, int *, int *);
+template <int TAG>
+__global__ void kz_gemm(int, int, int, int, const double2 *, long, const double2 *, long, const double2 *, long,
+                        double2 *, long, const int *, int, const double *, const int *);
+__global__ void kz_fixup_scale(double2 *, long, int, int, const int *, const int *, long, const int *, const int *);
+__global__ void kz_copy_out(const double2 *, long, int, int, double2 *, long, const int *, int);
+__global__ void kz_normalize(double2 *, long, int);
+__global__ void kz_reverse_conjtranspose(const double2 *, long, int, double2 *, long);
+__global__ void kz_colnorms(const double2 *, long, const double2 *, long, int, double *);
+__global__ void k_zero(double *, long);
+__global__ void k_zero_int(int *, long);
+
+constexpr int NB_Z = 32;
+constexpr int ZTA_H = 128, ZTB_H = 64;
+
+int complex_units(char howmny, const int *select, int n, std::vector<Unit> &units)
+{
+    bool do_all = howmny == 'A' || howmny == 'B';
+    bool do_sel = howmny == 'S';
+    units.clear();
+    int m = 0;
+    for (int k = 0; k < n; ++k) {
+        if (do_all || (do_sel && select != nullptr && select[k])) {
+            units.push_back(Unit{k, 1, m});
+            ++m;
+        }
+    }
+    return m;
+}
+
+namespace {
+
+struct DBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap) {
+            return hipSuccess;
+        }
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) {
+                return e;
+            }
+            p = nullptr;
+            cap = 0;
+        }
+        size_t b = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) {
+            cap = b;
+        }
+        return e;
+    }
+    template <class T> T *as() { return static_cast<T *>(p); }
+};
+
+struct ZWork {
+    DBuf X, W, sig, colexp, blkexp, flag, desc, outcol, kmax, rowstep, Sw, Pw, Zt, norms;
+};
+
+std::mutex g_zmutex;
+ZWork g_zws[64];
+
+typedef std::complex<double> zt;
+
+// complex shift coefficients, in the reference's operation order (ztgevc3.cpp:145-153)
+void shift_cplx(zt al, zt bv, double ascale, double bscale, zt &a, zt &b)
+{
+    const double safemin = std::numeric_limits<double>::min();
+    double t = 1.0 / std::max(std::max(std::abs(al) * ascale, std::abs(bv) * bscale), safemin);
+    a = zt(t * ascale, 0.0) * bv * zt(bscale, 0.0);
+    b = zt(t * bscale, 0.0) * al * zt(ascale, 0.0);
+    if (std::abs(bv) <= safemin && std::abs(al) > safemin) {
+        a = zt(0.0, 0.0);
+        b = zt(1.0, 0.0);
+    }
+}
+
+#define ZENSURE(buf, bytes) XTG_CHECK((buf).ensure(bytes))
+
+int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 *Pw, long ldpw, const zt *al,
+                  const zt *be, double ascale, double bscale, bool left, bool back, const double2 *Z, long ldz,
+                  double2 *V, long ldv, const std::vector<Unit> &units, int m, hipStream_t st)
+{
+    std::vector<ZColDesc> desc(m);
+    std::vector<int> outcol(m), kmaxv(m);
+    for (int p = 0; p < m; ++p) {
+        const Unit &un = left ? units[m - 1 - p] : units[p];
+        zt a, b;
+        shift_cplx(al[un.k], be[un.k], ascale, bscale, a, b);
+        if (left) {
+            a = std::conj(a);
+            b = std::conj(b);
+        }
+        ZColDesc c{};
+        c.aR = a.real();
+        c.aI = a.imag();
+        c.bR = b.real();
+        c.bI = b.imag();
+        c.d = left ? n - 1 - un.k : un.k;
+        c.lo = 0;
+        c.flags = INIT_UNIT << 2;
+        desc[p] = c;
+        outcol[p] = un.outc;
+        kmaxv[p] = c.d + 1;
+    }
+    std::vector<int> i0s, nbs;
+    for (int cur = n; cur > 0;) {
+        int i = std::max(0, cur - NB_Z);
+        i0s.push_back(i);
+        nbs.push_back(cur - i);
+        cur = i;
+    }
+    const int nsteps = (int)i0s.size();
+    std::vector<int> rowstep(n);
+    for (int s = 0; s < nsteps; ++s) {
+        for (int r = i0s[s]; r < i0s[s] + nbs[s]; ++r) {
+            rowstep[r] = s;
+        }
+    }
+    const long ldx = ((long)m + 7) / 8 * 8;
+    ZENSURE(ws.X, sizeof(double2) * (size_t)n * ldx);
+    ZENSURE(ws.W, sizeof(double2) * (size_t)(2 * NB_Z) * ldx);
+    ZENSURE(ws.sig, sizeof(double) * (size_t)m);
+    ZENSURE(ws.colexp, sizeof(int) * (size_t)m);
+    ZENSURE(ws.blkexp, sizeof(int) * (size_t)nsteps * ldx);
+    ZENSURE(ws.flag, sizeof(int) * 4);
+    ZENSURE(ws.desc, sizeof(ZColDesc) * (size_t)m);
+    ZENSURE(ws.outcol, sizeof(int) * (size_t)m);
+    ZENSURE(ws.kmax, sizeof(int) * (size_t)m);
+    ZENSURE(ws.rowstep, sizeof(int) * (size_t)n);
+    XTG_CHECK(hipMemcpyAsync(ws.desc.p, desc.data(), sizeof(ZColDesc) * m, hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.outcol.p, outcol.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.kmax.p, kmaxv.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.rowstep.p, rowstep.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    double2 *X = ws.X.as<double2>();
+    double2 *W = ws.W.as<double2>();
+    hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, (double *)X, 2L * n * ldx);
+    hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, ws.blkexp.as<int>(), (long)nsteps * ldx);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(256), 0, st, ws.colexp.as<int>(), (long)m);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, ws.flag.as<int>(), 4L);
+
+    int c_begin = m;
+    for (int s = 0; s < nsteps; ++s) {
+        const int i0 = i0s[s], nb = nbs[s];
+        while (c_begin > 0 && desc[c_begin - 1].d >= i0) {
+            --c_begin;
+        }
+        if (c_begin >= m) {
+            continue;
+        }
+        const int N = m - c_begin;
+        int pr = prof_begin(st);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kz_solve<NB_Z>), dim3((N + 255) / 256), dim3(256), 0, st, Sw, ldsw, Pw,
+                           ldpw, i0, nb, X, ldx, W, ldx, ws.desc.as<ZColDesc>(), c_begin, m, ws.sig.as<double>(),
+                           ws.colexp.as<int>(), ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>());
+        prof_end(pr, st, 0, 8.0 * nb * nb * (double)N, 16.0 * (4.0 * nb * N + 2.0 * nb * nb));
+        if (i0 > 0) {
+            dim3 grid((N + ZTB_H - 1) / ZTB_H, (i0 + ZTA_H - 1) / ZTA_H);
+            pr = prof_begin(st);
+            hipLaunchKernelGGL(kz_gemm<0>, grid, dim3(256), 0, st, i0, N, nb, nb, Sw + (long)i0 * ldsw, ldsw,
+                               Pw + (long)i0 * ldpw, ldpw, (const double2 *)(W + c_begin), ldx, X + c_begin, ldx,
+                               (const int *)nullptr, 1, (const double *)(ws.sig.as<double>() + c_begin),
+                               (const int *)nullptr);
+            prof_end(pr, st, 1, 8.0 * i0 * N * (2.0 * nb), 16.0 * (2.0 * i0 * N + 2.0 * nb * i0 + 2.0 * nb * N));
+        }
+    }
+    XTG_CHECK(hipGetLastError());
+    {
+        dim3 grid((m + 255) / 256, n);
+        hipLaunchKernelGGL(kz_fixup_scale, grid, dim3(256), 0, st, X, ldx, n, m, ws.rowstep.as<int>(),
+                           ws.blkexp.as<int>(), ldx, ws.colexp.as<int>(), ws.flag.as<int>());
+    }
+    if (back) {
+        const double2 *Rp = left ? Z + (long)(n - 1) * ldz : Z;
+        long ldr = left ? -ldz : ldz;
+        dim3 grid((n + ZTB_H - 1) / ZTB_H, (m + ZTA_H - 1) / ZTA_H);
+        double ksum = 0.0;
+        int kmx = 0;
+        for (int q = 0; q < m; ++q) {
+            ksum += kmaxv[q];
+            kmx = std::max(kmx, kmaxv[q]);
+        }
+        int pr = prof_begin(st);
+        hipLaunchKernelGGL(kz_gemm<1>, grid, dim3(256), 0, st, m, n, n, 0, (const double2 *)X, ldx,
+                           (const double2 *)nullptr, 0L, Rp, ldr, V, ldv, ws.outcol.as<int>(), 0,
+                           (const double *)nullptr, ws.kmax.as<int>());
+        prof_end(pr, st, 2, 8.0 * n * ksum, 16.0 * ((double)n * kmx + ksum + (double)n * m));
+    }
+    else {
+        dim3 grid((m + 31) / 32, (n + 31) / 32);
+        hipLaunchKernelGGL(kz_copy_out, grid, dim3(32, 8), 0, st, (const double2 *)X, ldx, n, m, V, ldv,
+                           ws.outcol.as<int>(), left ? 1 : 0);
+    }
+    hipLaunchKernelGGL(kz_normalize, dim3(m), dim3(256), 0, st, V, ldv, n);
+    XTG_CHECK(hipGetLastError());
+    return 0;
+}
+
+} // namespace
+
+int complex_device(char side, char howmny, const int *select, int n, const std::complex<double> *S, long lds,
+                   const std::complex<double> *P, long ldp, const std::complex<double> *al,
+                   const std::complex<double> *be, std::complex<double> *VL, long ldvl, std::complex<double> *VR,
+                   long ldvr, hipStream_t st)
+{
+    int dev = 0;
+    XTG_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_zmutex);
+    ZWork &ws = g_zws[dev & 63];
+    bool right = side == 'R' || side == 'B';
+    bool left = side == 'L' || side == 'B';
+    bool back = howmny == 'B';
+    std::vector<Unit> units;
+    int m = complex_units(howmny, select, n, units);
+    if (m == 0 || (!(right && VR) && !(left && VL))) {
+        return 0;
+    }
+    const double2 *dS = reinterpret_cast<const double2 *>(S);
+    const double2 *dP = reinterpret_cast<const double2 *>(P);
+    ZENSURE(ws.norms, sizeof(double) * 2 * (size_t)n);
+    hipLaunchKernelGGL(kz_colnorms, dim3(n), dim3(256), 0, st, dS, lds, dP, ldp, n, ws.norms.as<double>());
+    std::vector<double> nrm(2 * (size_t)n);
+    XTG_CHECK(hipMemcpyAsync(nrm.data(), ws.norms.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
+    XTG_CHECK(hipStreamSynchronize(st));
+    double an = 0.0, bn = 0.0;
+    for (int c = 0; c < n; ++c) {
+        an = std::max(an, nrm[c]);
+        bn = std::max(bn, nrm[n + c]);
+    }
+    const double safemin = std::numeric_limits<double>::min();
+    double ascale = 1.0 / std::max(an, safemin);
+    double bscale = 1.0 / std::max(bn, safemin);
+    if (right && VR) {
+        const double2 *Zp = nullptr;
+        if (back) {
+            ZENSURE(ws.Zt, sizeof(double2) * (size_t)n * n);
+            XTG_CHECK(hipMemcpy2DAsync(ws.Zt.p, sizeof(double2) * n, VR, sizeof(double2) * ldvr, sizeof(double2) * n,
+                                       n, hipMemcpyDeviceToDevice, st));
+            Zp = ws.Zt.as<double2>();
+        }
+        int rc = run_cplx_side(ws, n, dS, lds, dP, ldp, al, be, ascale, bscale, false, back, Zp, n,
+                               reinterpret_cast<double2 *>(VR), ldvr, units, m, st);
+        if (rc) {
+            return rc;
+        }
+    }
+    if (left && VL) {
+        ZENSURE(ws.Sw, sizeof(double2) * (size_t)n * n);
+        ZENSURE(ws.Pw, sizeof(double2) * (size_t)n * n);
+        dim3 grid((n + 31) / 32, (n + 31) / 32);
+        hipLaunchKernelGGL(kz_reverse_conjtranspose, grid, dim3(32, 8), 0, st, dS, lds, n, ws.Sw.as<double2>(), (long)n);
+        hipLaunchKernelGGL(kz_reverse_conjtranspose, grid, dim3(32, 8), 0, st, dP, ldp, n, ws.Pw.as<double2>(), (long)n);
+        const double2 *Zp = nullptr;
+        if (back) {
+            ZENSURE(ws.Zt, sizeof(double2) * (size_t)n * n);
+            XTG_CHECK(hipMemcpy2DAsync(ws.Zt.p, sizeof(double2) * n, VL, sizeof(double2) * ldvl, sizeof(double2) * n,
+                                       n, hipMemcpyDeviceToDevice, st));
+            Zp = ws.Zt.as<double2>();
+        }
+        int rc = run_cplx_side(ws, n, ws.Sw.as<double2>(), n, ws.Pw.as<double2>(), n, al, be, ascale, bscale, true,
+                               back, Zp, n, reinterpret_cast<double2 *>(VL), ldvl, units, m, st);
+        if (rc) {
+            return rc;
+        }
+    }
+    return 0;
+}
+
+} // namespace xtg

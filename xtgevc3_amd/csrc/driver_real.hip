@@ -1,0 +1,531 @@
+// Host driver of the real xTGEVC3 path: argument semantics of the reference entry point,
+// the column scheduler, and the launch sequence of the gfx950 kernels.
+//
+// Reference behaviour restated here (src/dtgevc3.cpp):
+//   side/howmny decoding and info = -2            :1173-1182
+//   n == 0 early exit                              :1184-1190
+//   selection count (pairs count twice), *m        :1192-1211
+//   mm < m -> info = -16                           :1213-1216
+//   lwork query / shrink / info = -19              :1219-1242
+//   global scales ascale, bscale                   :1252-1269
+//   shift coefficients acoeff, bcoeff              :647-655, :716-725, :1372-1381
+//   pair detection alphai[k] != 0 && k+1 < n       :1198, :1204-1206
+//   output columns packed in eigenvalue order      :1282, :1326, :1477, :1658
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "driver.h"
+
+namespace xtg {
+
+template <int MAXB>
+__global__ void k_solve_real(const double *, long, const double *, long, int, int, double *, long, double *, long,
+                             const ColDesc *, const int2 *, int, int, int, double *, int *, int *, int *);
+template <int TAG>
+__global__ void k_gemm_f64(int, int, int, int, const double *, long, const double *, long, const double *, long,
+                           double *, long, const int *, int, const double *, const int *);
+__global__ void k_fixup_scale(double *, long, int, int, const int *, const int *, long, const int *, const int *);
+__global__ void k_copy_out(const double *, long, int, int, double *, long, const int *, int);
+__global__ void k_normalize(double *, long, int, const int *, const int *);
+__global__ void k_reverse_transpose(const double *, long, int, double *, long);
+__global__ void k_colnorms(const double *, long, const double *, long, int, int, double *);
+__global__ void k_zero(double *, long);
+__global__ void k_zero_int(int *, long);
+
+constexpr int MAXB_REAL = 65;  // nominal block 64, +1 when a boundary would split a 2x2 block
+constexpr int NB_REAL = 64;
+constexpr int GEMM_TILE = 128;
+
+// ---------------------------------------------------------------------------------------
+// argument semantics shared by every real entry point
+// ---------------------------------------------------------------------------------------
+static bool pair_start(const double *ai, int k, int n) { return ai[k] != 0.0 && k + 1 < n; }
+
+int real_units(char howmny, const int *select, int n, const double *ai, std::vector<Unit> &units)
+{
+    bool do_all = howmny == 'A' || howmny == 'B';
+    bool do_sel = howmny == 'S';
+    units.clear();
+    int m = 0;
+    for (int k = 0; k < n;) {
+        bool pr = pair_start(ai, k, n);
+        bool sel = do_all;
+        if (do_sel && select != nullptr) {
+            sel = select[k] != 0 || (pr && select[k + 1] != 0);
+        }
+        int w = pr ? 2 : 1;
+        if (sel) {
+            units.push_back(Unit{k, w, m});
+            m += w;
+        }
+        k += w;
+    }
+    return m;
+}
+
+int reference_lwork(int n, int b) { return 2 * n * (b + 1) + 4 * (b + 1) * (b + 1) + 2 * (b + 1); }
+
+// Returns true when the caller should go on computing; sets *info / *m / work[0] exactly as
+// the reference does before its first floating-point operation.
+template <class T>
+bool check_args(char howmny, int n, int num_sel, int mm, int *m, T *work, int lwork, int *info, bool have_work)
+{
+    bool do_all = howmny == 'A' || howmny == 'B';
+    bool do_sel = howmny == 'S';
+    if (!do_all && !do_sel) {
+        *info = -2;
+        return false;
+    }
+    if (n == 0) {
+        *info = 0;
+        if (m != nullptr) {
+            *m = 0;
+        }
+        return false;
+    }
+    if (m != nullptr) {
+        *m = num_sel;
+    }
+    if (mm < num_sel) {
+        *info = -16;
+        return false;
+    }
+    if (have_work) {
+        int req = reference_lwork(n, 32);
+        if (lwork != -1 && lwork < req) {
+            for (int b = 31; b >= 1; --b) {
+                req = reference_lwork(n, b);
+                if (lwork >= req) {
+                    break;
+                }
+            }
+        }
+        if (lwork == -1) {
+            work[0] = T(reference_lwork(n, 32));
+            *info = 0;
+            return false;
+        }
+        if (lwork < req) {
+            *info = -19;
+            return false;
+        }
+    }
+    *info = 0;
+    return true;
+}
+template bool check_args<double>(char, int, int, int, int *, double *, int, int *, bool);
+template bool check_args<float>(char, int, int, int, int *, float *, int, int *, bool);
+template bool check_args<std::complex<double>>(char, int, int, int, int *, std::complex<double> *, int, int *, bool);
+template bool check_args<std::complex<float>>(char, int, int, int, int *, std::complex<float> *, int, int *, bool);
+
+// Shift coefficients for one eigenvalue (dtgevc3.cpp:716-725; with aI = 0 this is :647-655).
+static void shift_real(double aR, double aI, double bv, double ascale, double bscale, double &a, double &bR,
+                       double &bI, bool pairform)
+{
+    const double safemin = std::numeric_limits<double>::min();
+    double num = pairform ? std::max(std::max(std::fabs(aR) * ascale + std::fabs(aI) * ascale, std::fabs(bv) * bscale), safemin)
+                          : std::max(std::max(std::fabs(aR) * ascale, std::fabs(bv) * bscale), safemin);
+    double t = 1.0 / num;
+    a = (t * bv * bscale) * ascale;
+    bR = (t * aR * ascale) * bscale;
+    bI = pairform ? (t * aI * ascale) * bscale : 0.0;
+    if (std::fabs(bv) <= safemin && (std::fabs(aR) + std::fabs(aI)) > safemin) {
+        a = 0.0;
+        bR = 1.0;
+        bI = 0.0;
+    }
+}
+
+// Block partition of the working pencil from the bottom, never splitting a 2x2 block
+// (the same snapping rule as dtgevc3.cpp:32-47, with a nominal height of 64 rows).
+static void partition(int n, const std::vector<double> &subw, std::vector<int> &i0s, std::vector<int> &nbs)
+{
+    i0s.clear();
+    nbs.clear();
+    int cur = n;
+    while (cur > 0) {
+        int i = std::max(0, cur - NB_REAL);
+        if (i > 0 && subw[i] != 0.0) {
+            --i;
+        }
+        i0s.push_back(i);
+        nbs.push_back(cur - i);
+        cur = i;
+    }
+}
+
+static void make_tiles(const std::vector<ColDesc> &desc, std::vector<int2> &tiles)
+{
+    tiles.clear();
+    int m = (int)desc.size();
+    int s = 0;
+    while (s < m) {
+        int cnt = std::min(WAVE, m - s);
+        if (cnt == WAVE && (desc[s + WAVE - 1].flags & 2) && desc[s + WAVE - 1].partner == s + WAVE) {
+            cnt = WAVE - 1; // never split a pair across waves
+        }
+        tiles.push_back(make_int2(s, cnt));
+        s += cnt;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// workspace: grow-only device buffers kept per device for the life of the process
+// ---------------------------------------------------------------------------------------
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap) {
+            return hipSuccess;
+        }
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) {
+                return e;
+            }
+            p = nullptr;
+            cap = 0;
+        }
+        size_t b = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) {
+            cap = b;
+        }
+        return e;
+    }
+    template <class T> T *as() { return static_cast<T *>(p); }
+};
+
+struct RealWork {
+    DevBuf X, W, sig, colexp, blkexp, flag, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
+};
+
+static std::mutex g_ws_mutex;
+static RealWork g_ws[64];
+
+static thread_local std::string g_err;
+void set_error(const std::string &s) { g_err = s; }
+const char *last_error() { return g_err.c_str(); }
+
+#define ENSURE(buf, bytes) XTG_CHECK((buf).ensure(bytes))
+
+// ---------------------------------------------------------------------------------------
+// One side of the real problem on the device.
+//   Sw/Pw: working pencil (right: S, P; left: J S^T J, J P^T J), leading dims ldsw/ldpw
+//   units: selected eigenvalue units (ascending eigenvalue index), m columns in total
+//   left: working coordinates are reversed (row r <-> eigen row n-1-r)
+//   Vout: output (device, column-major, ldv); for back-transform Z (device, ldz) is read
+// ---------------------------------------------------------------------------------------
+struct RealSideArgs {
+    int n;
+    const double *Sw;
+    long ldsw;
+    const double *Pw;
+    long ldpw;
+    const double *ar, *ai, *be; // host
+    double ascale, bscale;
+    bool left;
+    bool back;
+    const double *Z; // back-transform matrix (Z or Q), device
+    long ldz;
+    double *V; // output, device
+    long ldv;
+    const std::vector<double> *subw; // working subdiagonal, host
+};
+
+static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<Unit> &units, int m, hipStream_t st)
+{
+    const int n = A.n;
+    // --- column descriptors in working order -------------------------------------------
+    std::vector<ColDesc> desc(m);
+    std::vector<int> outcol(m), kmaxv(m);
+    std::vector<int> vcol, vw;
+    int p = 0;
+    for (size_t u = 0; u < units.size(); ++u) {
+        const Unit &un = A.left ? units[units.size() - 1 - u] : units[u];
+        const int k = un.k;
+        double a, bR, bI;
+        if (un.w == 1) {
+            shift_real(A.ar[k], 0.0, A.be[k], A.ascale, A.bscale, a, bR, bI, false);
+            ColDesc c{};
+            c.a = a;
+            c.bR = bR;
+            c.bI = 0.0;
+            c.d = A.left ? n - 1 - k : k;
+            c.lo = 0;
+            c.partner = p;
+            c.flags = INIT_UNIT << 2;
+            desc[p] = c;
+            outcol[p] = un.outc;
+            kmaxv[p] = c.d + 1;
+            ++p;
+        }
+        else {
+            shift_real(A.ar[k], A.ai[k], A.be[k], A.ascale, A.bscale, a, bR, bI, true);
+            ColDesc c{};
+            c.a = a;
+            c.bR = bR;
+            c.bI = A.left ? -bI : bI; // left eigenvectors use conj(bcoeff) (dtgevc3.cpp:1584-1591)
+            c.d = A.left ? n - 2 - k : k;
+            c.lo = 0;
+            c.flags = 2 | ((A.left ? INIT_PAIR_ROW2 : INIT_PAIR_ROW1) << 2);
+            ColDesc ci = c;
+            c.partner = p + 1;
+            ci.partner = p;
+            ci.flags |= 1;
+            desc[p] = c;
+            desc[p + 1] = ci;
+            outcol[p] = un.outc;     // real part
+            outcol[p + 1] = un.outc + 1; // imaginary part
+            kmaxv[p] = kmaxv[p + 1] = c.d + 2;
+            p += 2;
+        }
+    }
+    for (const Unit &un : units) {
+        vcol.push_back(un.outc);
+        vw.push_back(un.w);
+    }
+    std::vector<int2> tiles;
+    make_tiles(desc, tiles);
+    std::vector<int> i0s, nbs;
+    partition(n, *A.subw, i0s, nbs);
+    const int nsteps = (int)i0s.size();
+    std::vector<int> rowstep(n);
+    for (int s = 0; s < nsteps; ++s) {
+        for (int r = i0s[s]; r < i0s[s] + nbs[s]; ++r) {
+            rowstep[r] = s;
+        }
+    }
+
+    const long ldx = ((long)m + 7) / 8 * 8;
+    const long ldw = ldx;
+    ENSURE(ws.X, sizeof(double) * (size_t)n * ldx);
+    ENSURE(ws.W, sizeof(double) * (size_t)(2 * MAXB_REAL) * ldw);
+    ENSURE(ws.sig, sizeof(double) * (size_t)m);
+    ENSURE(ws.colexp, sizeof(int) * (size_t)m);
+    ENSURE(ws.blkexp, sizeof(int) * (size_t)nsteps * ldx);
+    ENSURE(ws.flag, sizeof(int) * 4);
+    ENSURE(ws.desc, sizeof(ColDesc) * (size_t)m);
+    ENSURE(ws.tiles, sizeof(int2) * tiles.size());
+    ENSURE(ws.outcol, sizeof(int) * (size_t)m);
+    ENSURE(ws.kmax, sizeof(int) * (size_t)m);
+    ENSURE(ws.rowstep, sizeof(int) * (size_t)n);
+    ENSURE(ws.vcol, sizeof(int) * vcol.size());
+    ENSURE(ws.vw, sizeof(int) * vw.size());
+
+    XTG_CHECK(hipMemcpyAsync(ws.desc.p, desc.data(), sizeof(ColDesc) * m, hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.tiles.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.outcol.p, outcol.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.kmax.p, kmaxv.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.rowstep.p, rowstep.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.vcol.p, vcol.data(), sizeof(int) * vcol.size(), hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.vw.p, vw.data(), sizeof(int) * vw.size(), hipMemcpyHostToDevice, st));
+
+    double *X = ws.X.as<double>();
+    double *W = ws.W.as<double>();
+    hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, X, (long)n * ldx);
+    hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, ws.blkexp.as<int>(), (long)nsteps * ldx);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(256), 0, st, ws.colexp.as<int>(), (long)m);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, ws.flag.as<int>(), 4L);
+
+    // --- right-looking sweep over block rows, bottom to top -----------------------------
+    int c_begin = m;
+    for (int s = 0; s < nsteps; ++s) {
+        const int i0 = i0s[s], nb = nbs[s];
+        // active columns: own diagonal block at or below this block (d ascending)
+        while (c_begin > 0 && desc[c_begin - 1].d >= i0) {
+            --c_begin;
+        }
+        if (c_begin >= m) {
+            continue;
+        }
+        int t_begin = 0;
+        while (tiles[t_begin].x + tiles[t_begin].y <= c_begin) {
+            ++t_begin;
+        }
+        const int ntl = (int)tiles.size() - t_begin;
+        const double Nact = (double)(m - c_begin);
+        int pr = prof_begin(st);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_real<MAXB_REAL>), dim3((ntl + 3) / 4), dim3(256), 0, st, A.Sw,
+                           A.ldsw, A.Pw, A.ldpw, i0, nb, X, ldx, W, ldw, ws.desc.as<ColDesc>(), ws.tiles.as<int2>(),
+                           t_begin, (int)tiles.size(), c_begin, ws.sig.as<double>(), ws.colexp.as<int>(),
+                           ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>());
+        prof_end(pr, st, 0, 2.0 * nb * nb * Nact, 8.0 * (4.0 * nb * Nact + 2.0 * nb * nb));
+        if (i0 > 0) {
+            const int N = m - c_begin;
+            dim3 grid((N + GEMM_TILE - 1) / GEMM_TILE, (i0 + GEMM_TILE - 1) / GEMM_TILE);
+            pr = prof_begin(st);
+            hipLaunchKernelGGL(k_gemm_f64<0>, grid, dim3(256), 0, st, i0, N, nb, nb, A.Sw + (long)i0 * A.ldsw, A.ldsw,
+                               A.Pw + (long)i0 * A.ldpw, A.ldpw, (const double *)(W + c_begin), ldw, X + c_begin, ldx,
+                               (const int *)nullptr, 1, (const double *)(ws.sig.as<double>() + c_begin),
+                               (const int *)nullptr);
+            prof_end(pr, st, 1, 2.0 * i0 * N * (2.0 * nb), 8.0 * (2.0 * i0 * N + 2.0 * nb * i0 + 2.0 * nb * N));
+        }
+    }
+    XTG_CHECK(hipGetLastError());
+
+    // --- apply the per-column scale history to the solved blocks -------------------------
+    {
+        dim3 grid((m + 255) / 256, n);
+        hipLaunchKernelGGL(k_fixup_scale, grid, dim3(256), 0, st, X, ldx, n, m, ws.rowstep.as<int>(),
+                           ws.blkexp.as<int>(), ldx, ws.colexp.as<int>(), ws.flag.as<int>());
+    }
+
+    // --- output: back-transform or copy, then normalise ---------------------------------
+    if (A.back) {
+        // V[:, outcol(p)] = Zw * X[:, p]; for left eigenvectors Zw = Q J (reversed columns)
+        const double *Rp = A.left ? A.Z + (long)(n - 1) * A.ldz : A.Z;
+        long ldr = A.left ? -A.ldz : A.ldz;
+        dim3 grid((n + GEMM_TILE - 1) / GEMM_TILE, (m + GEMM_TILE - 1) / GEMM_TILE);
+        double ksum = 0.0;
+        int kmx = 0;
+        for (int q = 0; q < m; ++q) {
+            ksum += kmaxv[q];
+            kmx = std::max(kmx, kmaxv[q]);
+        }
+        int pr = prof_begin(st);
+        hipLaunchKernelGGL(k_gemm_f64<1>, grid, dim3(256), 0, st, m, n, n, 0, (const double *)X, ldx,
+                           (const double *)nullptr, 0L, Rp, ldr, A.V, A.ldv, ws.outcol.as<int>(), 0,
+                           (const double *)nullptr, ws.kmax.as<int>());
+        prof_end(pr, st, 2, 2.0 * n * ksum, 8.0 * ((double)n * kmx + ksum + (double)n * m));
+    }
+    else {
+        dim3 grid((m + 31) / 32, (n + 31) / 32);
+        hipLaunchKernelGGL(k_copy_out, grid, dim3(32, 8), 0, st, (const double *)X, ldx, n, m, A.V, A.ldv,
+                           ws.outcol.as<int>(), A.left ? 1 : 0);
+    }
+    hipLaunchKernelGGL(k_normalize, dim3((unsigned)vcol.size()), dim3(256), 0, st, A.V, A.ldv, n, ws.vcol.as<int>(),
+                       ws.vw.as<int>());
+    XTG_CHECK(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// Device-pointer entry for the real interface.  S, P, VL, VR are device arrays; select,
+// alphar, alphai, beta are host arrays.  No argument checks beyond the reference's: call
+// check_args first.
+// ---------------------------------------------------------------------------------------
+int real_device(char side, char howmny, const int *select, int n, const double *S, long lds, const double *P,
+                long ldp, const double *ar, const double *ai, const double *be, double *VL, long ldvl, double *VR,
+                long ldvr, hipStream_t st)
+{
+    int dev = 0;
+    XTG_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    RealWork &ws = g_ws[dev & 63];
+
+    bool right = side == 'R' || side == 'B';
+    bool left = side == 'L' || side == 'B';
+    bool back = howmny == 'B';
+    std::vector<Unit> units;
+    int m = real_units(howmny, select, n, ai, units);
+    if (m == 0 || (!(right && VR) && !(left && VL))) {
+        return 0;
+    }
+
+    // global scales from the 1-norms of S and P over the upper Hessenberg part
+    ENSURE(ws.norms, sizeof(double) * 2 * (size_t)n);
+    hipLaunchKernelGGL(k_colnorms, dim3(n), dim3(256), 0, st, S, lds, P, ldp, n, 1, ws.norms.as<double>());
+    std::vector<double> nrm(2 * (size_t)n);
+    XTG_CHECK(hipMemcpyAsync(nrm.data(), ws.norms.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
+    // subdiagonal of S (host copy) for the block snapping
+    std::vector<double> sub(n, 0.0);
+    if (n > 1) {
+        XTG_CHECK(hipMemcpy2DAsync(sub.data() + 1, sizeof(double), S + 1, sizeof(double) * (lds + 1),
+                                   sizeof(double), n - 1, hipMemcpyDeviceToHost, st));
+    }
+    XTG_CHECK(hipStreamSynchronize(st));
+    // sub[i] = S[i, i-1]
+    double an = 0.0, bn = 0.0;
+    for (int c = 0; c < n; ++c) {
+        an = std::max(an, nrm[c]);
+        bn = std::max(bn, nrm[n + c]);
+    }
+    const double safemin = std::numeric_limits<double>::min();
+    double ascale = 1.0 / std::max(an, safemin);
+    double bscale = 1.0 / std::max(bn, safemin);
+
+    if (right && VR != nullptr) {
+        RealSideArgs A{};
+        A.n = n;
+        A.Sw = S;
+        A.ldsw = lds;
+        A.Pw = P;
+        A.ldpw = ldp;
+        A.ar = ar;
+        A.ai = ai;
+        A.be = be;
+        A.ascale = ascale;
+        A.bscale = bscale;
+        A.left = false;
+        A.back = back;
+        A.subw = &sub;
+        A.V = VR;
+        A.ldv = ldvr;
+        if (back) {
+            // the input VR holds Z; keep a copy since the output overwrites it
+            ENSURE(ws.Zt, sizeof(double) * (size_t)n * n);
+            XTG_CHECK(hipMemcpy2DAsync(ws.Zt.p, sizeof(double) * n, VR, sizeof(double) * ldvr, sizeof(double) * n, n,
+                                       hipMemcpyDeviceToDevice, st));
+            A.Z = ws.Zt.as<double>();
+            A.ldz = n;
+        }
+        int rc = run_real_side(ws, A, units, m, st);
+        if (rc) {
+            return rc;
+        }
+    }
+    if (left && VL != nullptr) {
+        ENSURE(ws.Sw, sizeof(double) * (size_t)n * n);
+        ENSURE(ws.Pw, sizeof(double) * (size_t)n * n);
+        dim3 grid((n + 31) / 32, (n + 31) / 32);
+        hipLaunchKernelGGL(k_reverse_transpose, grid, dim3(32, 8), 0, st, S, lds, n, ws.Sw.as<double>(), (long)n);
+        hipLaunchKernelGGL(k_reverse_transpose, grid, dim3(32, 8), 0, st, P, ldp, n, ws.Pw.as<double>(), (long)n);
+        // working subdiagonal: Sw[i, i-1] = S[n-i, n-1-i]
+        std::vector<double> subw(n, 0.0);
+        for (int i = 1; i < n; ++i) {
+            subw[i] = sub[n - i];
+        }
+        RealSideArgs A{};
+        A.n = n;
+        A.Sw = ws.Sw.as<double>();
+        A.ldsw = n;
+        A.Pw = ws.Pw.as<double>();
+        A.ldpw = n;
+        A.ar = ar;
+        A.ai = ai;
+        A.be = be;
+        A.ascale = ascale;
+        A.bscale = bscale;
+        A.left = true;
+        A.back = back;
+        A.subw = &subw;
+        A.V = VL;
+        A.ldv = ldvl;
+        if (back) {
+            ENSURE(ws.Zt, sizeof(double) * (size_t)n * n);
+            XTG_CHECK(hipMemcpy2DAsync(ws.Zt.p, sizeof(double) * n, VL, sizeof(double) * ldvl, sizeof(double) * n, n,
+                                       hipMemcpyDeviceToDevice, st));
+            A.Z = ws.Zt.as<double>();
+            A.ldz = n;
+        }
+        int rc = run_real_side(ws, A, units, m, st);
+        if (rc) {
+            return rc;
+        }
+    }
+    return 0;
+}
+
+} // namespace xtg

@@ -1,0 +1,428 @@
+// gfx950 kernels of the complex128 xTGEVC3 path (ztgevc3 / ctgevc3 interface).
+//
+//   kz_solve     -- one block-row step: per-column shifted triangular solve of a diagonal
+//                   block (complex, no 2x2 bumps), one lane per eigenvector column, S/P
+//                   block in LDS, per-column power-of-two overflow scaling
+//                   (replaces zlalsr/zlalsl + ztrsm_, reference src/ztgevc3.cpp:116-358)
+//   kz_gemm      -- complex FP64 MFMA GEMM (four v_mfma_f64_16x16x4 per complex product) for
+//                   the RHS update (ztgevc3.cpp:664-665, :808-809) and the back-transform
+//                   (ztgevc3.cpp:676, :820)
+//   kz_*         -- fix-up of the scale history, copy-out, normalisation, conjugate
+//                   reversed transpose, column norms.
+#include <hip/hip_runtime.h>
+#include <cmath>
+
+#include "common.h"
+
+namespace xtg {
+
+struct zc {
+    double r, i;
+};
+
+__device__ __forceinline__ zc zmul(zc a, zc b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ __forceinline__ double zabs1(zc a) { return fabs(a.r) + fabs(a.i); }
+__device__ __forceinline__ zc zdiv(zc x, zc c)
+{
+    if (fabs(c.r) >= fabs(c.i)) {
+        double t = c.i / c.r;
+        double den = c.r + c.i * t;
+        return {(x.r + x.i * t) / den, (x.i - x.r * t) / den};
+    }
+    double t = c.r / c.i;
+    double den = c.i + c.r * t;
+    return {(x.r * t + x.i) / den, (x.i * t - x.r) / den};
+}
+
+constexpr double kzEps = 2.220446049250313e-16;
+constexpr double kzSafeMin = 2.2250738585072014e-308;
+constexpr int kzLog2Big = 960;
+
+// X, W: interleaved complex (re, im) doubles, row-major: X[(row * ldx + col) * 2 + {0,1}].
+template <int MAXB>
+__global__ __launch_bounds__(256) void kz_solve(const double2 *__restrict__ S, long lds, const double2 *__restrict__ P,
+                                                long ldp, int i0, int nb, double2 *__restrict__ X, long ldx,
+                                                double2 *__restrict__ W, long ldw, const ZColDesc *__restrict__ desc,
+                                                int c_begin, int m, double *__restrict__ sig, int *__restrict__ colexp,
+                                                int *__restrict__ blkexp, int *__restrict__ flag)
+{
+    __shared__ double2 ss[MAXB * MAXB];
+    __shared__ double2 pp[MAXB * MAXB];
+    for (int q = threadIdx.x; q < nb * nb; q += blockDim.x) {
+        int l = q % nb, r = q / nb;
+        ss[l + r * MAXB] = S[(i0 + l) + (long)(i0 + r) * lds];
+        pp[l + r * MAXB] = P[(i0 + l) + (long)(i0 + r) * ldp];
+    }
+    __syncthreads();
+    const int p = c_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = p < m;
+    const ZColDesc cd = desc[act ? p : c_begin];
+    const zc a = {cd.aR, cd.aI}, b = {cd.bR, cd.bI};
+    const double an1 = zabs1(a), bn1 = zabs1(b);
+    const int dl = act ? cd.d - i0 : -1 - MAXB;
+    const int lol = cd.lo - i0;
+    const int init = (cd.flags >> 2) & 3;
+
+    double vr[MAXB], vi[MAXB];
+#pragma clang loop unroll(full)
+    for (int r = 0; r < MAXB; ++r) {
+        vr[r] = 0.0;
+        vi[r] = 0.0;
+        if (r < nb && r < dl && r >= lol) {
+            double2 t = X[(long)(i0 + r) * ldx + p];
+            vr[r] = t.x;
+            vi[r] = t.y;
+        }
+    }
+    int my_exp = 0;
+#pragma clang loop unroll(full)
+    for (int r = MAXB - 1; r >= 0; --r) {
+        if (r < nb) {
+            double2 es = ss[r + r * MAXB], ep = pp[r + r * MAXB];
+            zc c = {a.r * es.x - a.i * es.y - (b.r * ep.x - b.i * ep.y), a.r * es.y + a.i * es.x - (b.r * ep.y + b.i * ep.x)};
+            double dmin = fmax(kzEps * (an1 * (fabs(es.x) + fabs(es.y)) + bn1 * (fabs(ep.x) + fabs(ep.y))), kzSafeMin);
+            if (zabs1(c) < dmin) {
+                c = {dmin, 0.0};
+            }
+            zc rhs = {vr[r], vi[r]};
+            const bool solve = r < dl && r >= lol;
+            double rn = zabs1(rhs);
+            int ex = 0;
+            if (solve && rn > 0.0 && ilogb(rn) - ilogb(zabs1(c)) > kzLog2Big - 2) {
+                ex = ilogb(rn) - ilogb(zabs1(c)) - kzLog2Big + 2;
+            }
+            if (__any(ex != 0)) {
+                double f = ldexp(1.0, -ex);
+#pragma clang loop unroll(full)
+                for (int j = 0; j < MAXB; ++j) {
+                    vr[j] *= f;
+                    vi[j] *= f;
+                }
+                rhs = {rhs.r * f, rhs.i * f};
+                my_exp -= ex;
+            }
+            zc x = solve ? zdiv(rhs, c) : ((r == dl && init == INIT_UNIT) ? zc{1.0, 0.0} : zc{0.0, 0.0});
+            vr[r] = x.r;
+            vi[r] = x.i;
+            const zc ax = zmul(a, x), bx = zmul(b, x);
+#pragma clang loop unroll(full)
+            for (int l = 0; l < r; ++l) {
+                double2 s2 = ss[l + r * MAXB], p2 = pp[l + r * MAXB];
+                // v_l -= S_lr * ax - P_lr * bx
+                vr[l] = fma(-s2.x, ax.r, vr[l]);
+                vr[l] = fma(s2.y, ax.i, vr[l]);
+                vr[l] = fma(p2.x, bx.r, vr[l]);
+                vr[l] = fma(-p2.y, bx.i, vr[l]);
+                vi[l] = fma(-s2.x, ax.i, vi[l]);
+                vi[l] = fma(-s2.y, ax.r, vi[l]);
+                vi[l] = fma(p2.x, bx.i, vi[l]);
+                vi[l] = fma(p2.y, bx.r, vi[l]);
+            }
+        }
+    }
+    if (!act) {
+        return;
+    }
+#pragma clang loop unroll(full)
+    for (int r = 0; r < MAXB; ++r) {
+        if (r < nb) {
+            zc x = {vr[r], vi[r]};
+            X[(long)(i0 + r) * ldx + p] = make_double2(x.r, x.i);
+            zc ax = zmul(a, x), bx = zmul(b, x);
+            W[(long)r * ldw + p] = make_double2(-ax.r, -ax.i);
+            W[(long)(nb + r) * ldw + p] = make_double2(bx.r, bx.i);
+        }
+    }
+    sig[p] = ldexp(1.0, my_exp);
+    int ce = colexp[p] + my_exp;
+    colexp[p] = ce;
+    blkexp[p] = ce;
+    if (my_exp != 0) {
+        *flag = 1;
+    }
+}
+
+template __global__ void kz_solve<32>(const double2 *, long, const double2 *, long, int, int, double2 *, long,
+                                      double2 *, long, const ZColDesc *, int, int, double *, int *, int *, int *);
+
+// ---------------------------------------------------------------------------------------
+// kz_gemm: D[a][b] = sum_k L[k][a] * R[k][b], complex, FP64 MFMA.
+// Tile 128 (a) x 64 (b) per 256-thread workgroup (waves 2x2, each 64 x 32), K chunks of 8.
+// Same operand / output conventions as k_gemm_f64 (element = complex double2).
+// ---------------------------------------------------------------------------------------
+typedef double zdouble4_t __attribute__((ext_vector_type(4)));
+constexpr int ZTA = 128, ZTB = 64, ZGK = 8;
+
+template <int TAG>
+__global__ __launch_bounds__(256) void kz_gemm(int M, int N, int K1, int K2, const double2 *__restrict__ L, long ldl,
+                                               const double2 *__restrict__ L2, long ldl2, const double2 *__restrict__ R,
+                                               long ldr, double2 *__restrict__ C, long ldc,
+                                               const int *__restrict__ outcol, int mode, const double *__restrict__ sig,
+                                               const int *__restrict__ kmax)
+{
+    __shared__ double Lr[2][ZGK][ZTA], Li[2][ZGK][ZTA];
+    __shared__ double Rr[2][ZGK][ZTB], Ri[2][ZGK][ZTB];
+    const int a0 = blockIdx.y * ZTA, b0 = blockIdx.x * ZTB;
+    const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+    const int wa = (wave >> 1) * 64, wb = (wave & 1) * 32;
+    int K = K1 + K2;
+    if (kmax != nullptr) {
+        int kk = 0;
+        int amax = min(a0 + ZTA, M) - 1;
+        for (int q = a0; q <= amax; ++q) {
+            kk = max(kk, kmax[q]);
+        }
+        K = min(K, kk);
+    }
+    zdouble4_t accr[4][2], acci[4][2];
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 4; ++i) {
+#pragma clang loop unroll(full)
+        for (int j = 0; j < 2; ++j) {
+            accr[i][j] = (zdouble4_t){0, 0, 0, 0};
+            acci[i][j] = (zdouble4_t){0, 0, 0, 0};
+        }
+    }
+    // staging: L chunk 8 x 128 complex (4 per thread), R chunk 8 x 64 complex (2 per thread)
+    const int lc = tid % ZTA, lk = tid / ZTA;  // lk in {0,1}
+    const int rc = tid % ZTB, rk = tid / ZTB;  // rk in {0..3}
+    double2 lreg[4], rreg[2];
+    auto load_chunk = [&](int k0) {
+#pragma clang loop unroll(full)
+        for (int s = 0; s < 4; ++s) {
+            int k = k0 + lk + 2 * s;
+            int aa = a0 + lc;
+            double2 v = make_double2(0.0, 0.0);
+            if (k < K && aa < M) {
+                v = (k < K1) ? L[aa + (long)k * ldl] : L2[aa + (long)(k - K1) * ldl2];
+            }
+            lreg[s] = v;
+        }
+#pragma clang loop unroll(full)
+        for (int s = 0; s < 2; ++s) {
+            int k = k0 + rk + 4 * s;
+            int bb = b0 + rc;
+            double2 v = make_double2(0.0, 0.0);
+            if (k < K && bb < N) {
+                v = R[(long)k * ldr + bb];
+            }
+            rreg[s] = v;
+        }
+    };
+    auto store_chunk = [&](int buf) {
+#pragma clang loop unroll(full)
+        for (int s = 0; s < 4; ++s) {
+            Lr[buf][lk + 2 * s][lc] = lreg[s].x;
+            Li[buf][lk + 2 * s][lc] = lreg[s].y;
+        }
+#pragma clang loop unroll(full)
+        for (int s = 0; s < 2; ++s) {
+            Rr[buf][rk + 4 * s][rc] = rreg[s].x;
+            Ri[buf][rk + 4 * s][rc] = rreg[s].y;
+        }
+    };
+    const int nch = (K + ZGK - 1) / ZGK;
+    if (nch > 0) {
+        load_chunk(0);
+        store_chunk(0);
+    }
+    __syncthreads();
+    const int fr = lane % 16, fk = lane / 16;
+    for (int ch = 0; ch < nch; ++ch) {
+        const int buf = ch & 1;
+        if (ch + 1 < nch) {
+            load_chunk((ch + 1) * ZGK);
+        }
+#pragma clang loop unroll(full)
+        for (int ks = 0; ks < ZGK; ks += 4) {
+            double ar_[4], ai_[4], br_[2], bi_[2];
+#pragma clang loop unroll(full)
+            for (int i = 0; i < 4; ++i) {
+                ar_[i] = Lr[buf][ks + fk][wa + i * 16 + fr];
+                ai_[i] = Li[buf][ks + fk][wa + i * 16 + fr];
+            }
+#pragma clang loop unroll(full)
+            for (int j = 0; j < 2; ++j) {
+                br_[j] = Rr[buf][ks + fk][wb + j * 16 + fr];
+                bi_[j] = Ri[buf][ks + fk][wb + j * 16 + fr];
+            }
+#pragma clang loop unroll(full)
+            for (int i = 0; i < 4; ++i) {
+                double nai = -ai_[i];
+#pragma clang loop unroll(full)
+                for (int j = 0; j < 2; ++j) {
+                    accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar_[i], br_[j], accr[i][j], 0, 0, 0);
+                    accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, bi_[j], accr[i][j], 0, 0, 0);
+                    acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar_[i], bi_[j], acci[i][j], 0, 0, 0);
+                    acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai_[i], br_[j], acci[i][j], 0, 0, 0);
+                }
+            }
+        }
+        if (ch + 1 < nch) {
+            store_chunk(buf ^ 1);
+        }
+        __syncthreads();
+    }
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 4; ++i) {
+#pragma clang loop unroll(full)
+        for (int vv = 0; vv < 4; ++vv) {
+            int aa = a0 + wa + i * 16 + 4 * fk + vv;
+            if (aa >= M) {
+                continue;
+            }
+            long rowoff = (long)(outcol ? outcol[aa] : aa) * ldc;
+#pragma clang loop unroll(full)
+            for (int j = 0; j < 2; ++j) {
+                int bb = b0 + wb + j * 16 + fr;
+                if (bb < N) {
+                    double dr = accr[i][j][vv], di = acci[i][j][vv];
+                    double2 *cp = C + rowoff + bb;
+                    if (mode == 1) {
+                        double2 o = *cp;
+                        double s = sig[bb];
+                        dr = fma(s, o.x, dr);
+                        di = fma(s, o.y, di);
+                    }
+                    *cp = make_double2(dr, di);
+                }
+            }
+        }
+    }
+}
+
+template __global__ void kz_gemm<0>(int, int, int, int, const double2 *, long, const double2 *, long, const double2 *,
+                                    long, double2 *, long, const int *, int, const double *, const int *);
+template __global__ void kz_gemm<1>(int, int, int, int, const double2 *, long, const double2 *, long, const double2 *,
+                                    long, double2 *, long, const int *, int, const double *, const int *);
+
+__global__ void kz_fixup_scale(double2 *__restrict__ X, long ldx, int n, int m, const int *__restrict__ rowstep,
+                               const int *__restrict__ blkexp, long ldb, const int *__restrict__ colexp,
+                               const int *__restrict__ flag)
+{
+    if (*flag == 0) {
+        return;
+    }
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = blockIdx.y;
+    if (p >= m || r >= n) {
+        return;
+    }
+    int e = colexp[p] - blkexp[(long)rowstep[r] * ldb + p];
+    if (e != 0) {
+        double2 *x = X + (long)r * ldx + p;
+        double2 v = *x;
+        *x = make_double2(ldexp(v.x, e), ldexp(v.y, e));
+    }
+}
+
+__global__ void kz_copy_out(const double2 *__restrict__ X, long ldx, int n, int m, double2 *__restrict__ V, long ldv,
+                            const int *__restrict__ outcol, int rev)
+{
+    __shared__ double2 tile[32][33];
+    int p0 = blockIdx.x * 32, i0 = blockIdx.y * 32;
+    int tx = threadIdx.x, ty = threadIdx.y;
+    for (int k = ty; k < 32; k += 8) {
+        int i = i0 + k, p = p0 + tx;
+        double2 v = make_double2(0.0, 0.0);
+        if (i < n && p < m) {
+            v = X[(long)(rev ? n - 1 - i : i) * ldx + p];
+        }
+        tile[k][tx] = v;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        int p = p0 + k, i = i0 + tx;
+        if (i < n && p < m) {
+            V[i + (long)outcol[p] * ldv] = tile[tx][k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void kz_normalize(double2 *__restrict__ V, long ldv, int n)
+{
+    __shared__ double red[256 / WAVE];
+    double2 *c0 = V + (long)blockIdx.x * ldv;
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double2 t = c0[i];
+        mx = fmax(mx, fabs(t.x) + fabs(t.y));
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, off));
+    }
+    if (threadIdx.x % WAVE == 0) {
+        red[threadIdx.x / WAVE] = mx;
+    }
+    __syncthreads();
+    mx = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / WAVE); ++w) {
+        mx = fmax(mx, red[w]);
+    }
+    if (!(mx > kzSafeMin)) {
+        return;
+    }
+    double sc = 1.0 / mx;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double2 t = c0[i];
+        c0[i] = make_double2(t.x * sc, t.y * sc);
+    }
+}
+
+// D = J A^H J: D[i + j*ldd] = conj(A[(n-1-j) + (n-1-i)*lda])
+__global__ void kz_reverse_conjtranspose(const double2 *__restrict__ A, long lda, int n, double2 *__restrict__ D,
+                                         long ldd)
+{
+    __shared__ double2 tile[32][33];
+    int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
+    int tx = threadIdx.x, ty = threadIdx.y;
+    for (int k = ty; k < 32; k += 8) {
+        int j = j0 + tx, i = i0 + k;
+        double2 v = make_double2(0.0, 0.0);
+        if (i < n && j < n) {
+            v = A[(long)(n - 1 - j) + (long)(n - 1 - i) * lda];
+        }
+        tile[k][tx] = make_double2(v.x, -v.y);
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        int i = i0 + tx, j = j0 + k;
+        if (i < n && j < n) {
+            D[i + (long)j * ldd] = tile[tx][k];
+        }
+    }
+}
+
+// column sums of |S| and |P| (modulus) over rows 0..c (ztgevc3.cpp:528-542)
+__global__ void kz_colnorms(const double2 *__restrict__ S, long lds, const double2 *__restrict__ P, long ldp, int n,
+                            double *__restrict__ out)
+{
+    int c = blockIdx.x;
+    double ss = 0.0, sp = 0.0;
+    for (int r = threadIdx.x; r <= c; r += blockDim.x) {
+        double2 a = S[r + (long)c * lds], b = P[r + (long)c * ldp];
+        ss += hypot(a.x, a.y);
+        sp += hypot(b.x, b.y);
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        ss += __shfl_xor(ss, off);
+        sp += __shfl_xor(sp, off);
+    }
+    __shared__ double red[2][4];
+    if (threadIdx.x % WAVE == 0) {
+        red[0][threadIdx.x / WAVE] = ss;
+        red[1][threadIdx.x / WAVE] = sp;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / WAVE); ++w) {
+            a += red[0][w];
+            b += red[1][w];
+        }
+        out[c] = a;
+        out[n + c] = b;
+    }
+}
+
+} // namespace xtg

@@ -1,0 +1,619 @@
+// gfx950 kernels of the real (FP64) xTGEVC3 path.
+//
+//   k_solve_real    -- one block-row step of the right-looking blocked back substitution:
+//                      per-column shifted quasi-triangular solve of a diagonal block,
+//                      one lane per eigenvector column, S/P block staged in LDS,
+//                      complex pairs on two lanes, per-column power-of-two overflow scaling.
+//                      (replaces dlalsr/dlalsl + dlauhs/dlau2s/dlalhs/dlal2s,
+//                       reference src/dtgevc3.cpp:132-1073)
+//   k_gemm_f64      -- FP64 MFMA (v_mfma_f64_16x16x4) GEMM used for the off-diagonal RHS
+//                      update (dtgevc3.cpp:1432-1433, :1624-1625) and for the back-transform
+//                      (dtgevc3.cpp:1446, :1638).
+//   k_fixup_scale   -- applies the per-column scale history to the solved blocks.
+//   k_copy_out      -- X (row-major working layout) -> V (column-major), zero fill.
+//   k_normalize     -- LAPACK-style normalisation of output vectors.
+//   k_reverse_transpose -- builds the working pencil J S^T J for left eigenvectors.
+//   k_colnorms      -- 1-norm column sums of S and P over the upper Hessenberg part.
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cmath>
+
+#include "common.h"
+
+namespace xtg {
+
+struct cpx {
+    double r, i;
+};
+
+__device__ __forceinline__ cpx cmul(cpx a, cpx b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ __forceinline__ cpx csub(cpx a, cpx b) { return {a.r - b.r, a.i - b.i}; }
+__device__ __forceinline__ double abs1(cpx a) { return fabs(a.r) + fabs(a.i); }
+
+// Smith's complex division (no overflow of |c|^2).  Real lanes have c.i == 0 and get an
+// exact real division.
+__device__ __forceinline__ cpx cdiv(cpx x, cpx c)
+{
+    if (fabs(c.r) >= fabs(c.i)) {
+        double t = c.i / c.r;
+        double den = c.r + c.i * t;
+        return {(x.r + x.i * t) / den, (x.i - x.r * t) / den};
+    }
+    double t = c.r / c.i;
+    double den = c.i + c.r * t;
+    return {(x.r * t + x.i) / den, (x.i * t - x.r) / den};
+}
+
+// Exponent e >= 0 such that 2^-e * num <= lim * den (lim = 2^k), computed without
+// forming num / den (which may overflow).
+__device__ __forceinline__ int scale_exponent(double num, double den, int log2lim)
+{
+    int e = ilogb(num) - ilogb(den) - log2lim + 2;
+    return e > 0 ? e : 0;
+}
+
+constexpr double kEps = 2.220446049250313e-16;     // DBL_EPSILON
+constexpr double kSafeMin = 2.2250738585072014e-308; // DBL_MIN
+constexpr int kLog2Big = 960;                        // |x| entries are kept <= 2^960
+
+struct Bump2 {
+    cpx x1, x2;
+    int e;
+};
+
+// 2x2 (complex) block solve with partial pivoting on the first column and overflow guard.
+// Restates the effect of the Hessenberg GEPP of dlauhs/dlau2s (dtgevc3.cpp:132-344) on a
+// 2x2 diagonal bump of S: [c11 c12; c21 c22] [x1; x2] = [r1; r2].
+__device__ __noinline__ Bump2 bump_solve(cpx r1, cpx r2, cpx c11, cpx c12, cpx c21, cpx c22, double dmin1,
+                                         double dmin2)
+{
+    bool sw = abs1(c21) > abs1(c11);
+    cpx p = sw ? c21 : c11, q = sw ? c22 : c12;
+    cpx s = sw ? c11 : c21, t = sw ? c12 : c22;
+    cpx rp = sw ? r2 : r1, rs = sw ? r1 : r2;
+    if (abs1(p) < dmin1) {
+        p = {dmin1, 0.0};
+    }
+    cpx mlt = cdiv(s, p);
+    cpx u = csub(t, cmul(mlt, q));
+    cpx y = csub(rs, cmul(mlt, rp));
+    if (abs1(u) < dmin2) {
+        u = {dmin2, 0.0};
+    }
+    Bump2 out;
+    out.e = 0;
+    double yn = abs1(y);
+    if (yn > 0.0 && ilogb(yn) - ilogb(abs1(u)) > kLog2Big - 2) {
+        int e = scale_exponent(yn, abs1(u), kLog2Big);
+        double f = ldexp(1.0, -e);
+        y = {y.r * f, y.i * f};
+        rp = {rp.r * f, rp.i * f};
+        out.e += e;
+    }
+    cpx x2 = cdiv(y, u);
+    cpx w = csub(rp, cmul(q, x2));
+    double wn = abs1(w);
+    if (wn > 0.0 && ilogb(wn) - ilogb(abs1(p)) > kLog2Big - 2) {
+        int e = scale_exponent(wn, abs1(p), kLog2Big);
+        double f = ldexp(1.0, -e);
+        w = {w.r * f, w.i * f};
+        x2 = {x2.r * f, x2.i * f};
+        out.e += e;
+    }
+    out.x1 = cdiv(w, p);
+    out.x2 = x2;
+    return out;
+}
+
+// ---------------------------------------------------------------------------------------
+// k_solve_real: one step (row block [i0, i0+nb)) for all active columns.
+//
+// X is the working matrix in row-major layout X[row * ldx + col] (rows = vector components,
+// columns = packed eigenvector columns).  Rows above the current block hold the running
+// right-hand sides, rows of the block hold RHS on entry and the solution on exit.
+// W receives the update operands, row-major: W[k * ldw + col], k < nb: -a * x,
+// k >= nb: (b x) own part, so that X_above := sig * X_above + [S_w | P_w] * W.
+// ---------------------------------------------------------------------------------------
+template <int MAXB>
+__global__ __launch_bounds__(256) void k_solve_real(const double *__restrict__ S, long lds, const double *__restrict__ P,
+                                                    long ldp, int i0, int nb, double *__restrict__ X, long ldx,
+                                                    double *__restrict__ W, long ldw, const ColDesc *__restrict__ desc,
+                                                    const int2 *__restrict__ tiles, int t_begin, int t_end,
+                                                    int c_begin, double *__restrict__ sig, int *__restrict__ colexp,
+                                                    int *__restrict__ blkexp, int *__restrict__ flag)
+{
+    __shared__ double2 sp[MAXB * MAXB];
+    for (int q = threadIdx.x; q < nb * nb; q += blockDim.x) {
+        int l = q % nb, r = q / nb;
+        sp[l + r * MAXB] = make_double2(S[(i0 + l) + (long)(i0 + r) * lds], P[(i0 + l) + (long)(i0 + r) * ldp]);
+    }
+    __syncthreads();
+
+    const int wave = threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    const int t = t_begin + blockIdx.x * (blockDim.x / WAVE) + wave;
+    if (t >= t_end) {
+        return;
+    }
+    const int2 tl = tiles[t];
+    const int p = tl.x + lane;
+    const bool act = lane < tl.y && p >= c_begin;
+    const ColDesc cd = desc[act ? p : tl.x];
+    const bool is_im = cd.flags & 1;
+    const bool is_pair = (cd.flags & 2) != 0;
+    const int init = (cd.flags >> 2) & 3;
+    const int plane = act ? (cd.partner - tl.x) : lane;
+    const double a = cd.a, bR = cd.bR, bI = cd.bI;
+    const double bIs = is_im ? bI : -bI;
+    const double bn1 = fabs(bR) + fabs(bI);
+    const int dl = act ? cd.d - i0 : -1 - MAXB; // inactive lanes: every row is "fixed zero"
+    const int lol = cd.lo - i0;
+
+    // initial values of the column's own diagonal block rows (complex)
+    cpx x0 = {0.0, 0.0}, x1v = {0.0, 0.0};
+    if (act && dl >= 0 && dl < nb) {
+        if (init == INIT_UNIT) {
+            x0 = {1.0, 0.0};
+        }
+        else if (init == INIT_PAIR_ROW1 && dl + 1 < nb) {
+            double2 e11 = sp[dl + dl * MAXB], e12 = sp[dl + (dl + 1) * MAXB];
+            x0 = {-(a * e12.x - bR * e12.y), bI * e12.y};
+            x1v = {a * e11.x - bR * e11.y, -bI * e11.y};
+        }
+        else if (init == INIT_PAIR_ROW2 && dl + 1 < nb) {
+            double2 e22 = sp[(dl + 1) + (dl + 1) * MAXB], e21 = sp[(dl + 1) + dl * MAXB];
+            x0 = {a * e22.x - bR * e22.y, -bI * e22.y};
+            x1v = {-a * e21.x, 0.0};
+        }
+    }
+
+    double v[MAXB];
+#pragma clang loop unroll(full)
+    for (int r = 0; r < MAXB; ++r) {
+        v[r] = 0.0;
+        if (r < nb) {
+            if (r < dl && r >= lol) {
+                v[r] = X[(long)(i0 + r) * ldx + p];
+            }
+        }
+    }
+
+    int my_exp = 0;
+    bool pre = false;      // row r was solved together with row r+1 (2x2 bump)
+    double pre_xp = 0.0;   // partner part of that row's solution
+#pragma clang loop unroll(full)
+    for (int r = MAXB - 1; r >= 0; --r) {
+        if (r < nb) {
+            double xs, xp;
+            bool bump_now = false;
+            if (pre) {
+                xs = v[r];
+                xp = pre_xp;
+                pre = false;
+            }
+            else if (r >= 1 && sp[r + (r - 1) * MAXB].x != 0.0) {
+                // 2x2 bump at rows (r-1, r)
+                bump_now = true;
+                double o1 = v[r - 1], o2 = v[r];
+                double q1 = __shfl(o1, plane), q2 = __shfl(o2, plane);
+                cpx r1 = is_im ? cpx{q1, o1} : cpx{o1, is_pair ? q1 : 0.0};
+                cpx r2 = is_im ? cpx{q2, o2} : cpx{o2, is_pair ? q2 : 0.0};
+                double2 e11 = sp[(r - 1) + (r - 1) * MAXB], e12 = sp[(r - 1) + r * MAXB];
+                double2 e21 = sp[r + (r - 1) * MAXB], e22 = sp[r + r * MAXB];
+                cpx c11 = {a * e11.x - bR * e11.y, -bI * e11.y};
+                cpx c12 = {a * e12.x - bR * e12.y, -bI * e12.y};
+                cpx c21 = {a * e21.x - bR * e21.y, -bI * e21.y};
+                cpx c22 = {a * e22.x - bR * e22.y, -bI * e22.y};
+                double dm1 = fmax(kEps * (fabs(a) * (fabs(e11.x) + fabs(e21.x)) + bn1 * fabs(e11.y)), kSafeMin);
+                double dm2 = fmax(kEps * (fabs(a) * (fabs(e12.x) + fabs(e22.x)) + bn1 * fabs(e22.y)), kSafeMin);
+                const bool solve1 = (r - 1) < dl && (r - 1) >= lol;
+                const bool solve2 = r < dl && r >= lol;
+                Bump2 bs = bump_solve(r1, r2, c11, c12, c21, c22, dm1, dm2);
+                int e = (solve1 || solve2) ? bs.e : 0;
+                if (__any(e != 0)) {
+                    double f = ldexp(1.0, -e);
+#pragma clang loop unroll(full)
+                    for (int j = 0; j < MAXB; ++j) {
+                        v[j] *= f;
+                    }
+                    my_exp -= e;
+                }
+                cpx s1 = solve1 ? bs.x1 : ((r - 1) == dl ? x0 : ((r - 1) == dl + 1 ? x1v : cpx{0.0, 0.0}));
+                cpx s2 = solve2 ? bs.x2 : (r == dl ? x0 : (r == dl + 1 ? x1v : cpx{0.0, 0.0}));
+                v[r - 1] = is_im ? s1.i : s1.r;
+                pre_xp = is_im ? s1.r : s1.i;
+                xs = is_im ? s2.i : s2.r;
+                xp = is_im ? s2.r : s2.i;
+                pre = true;
+            }
+            else {
+                double o = v[r];
+                double q = __shfl(o, plane);
+                cpx rhs = is_im ? cpx{q, o} : cpx{o, is_pair ? q : 0.0};
+                double2 e = sp[r + r * MAXB];
+                cpx c = {a * e.x - bR * e.y, -bI * e.y};
+                double dmin = fmax(kEps * (fabs(a) * fabs(e.x) + bn1 * fabs(e.y)), kSafeMin);
+                if (abs1(c) < dmin) {
+                    c = {dmin, 0.0};
+                }
+                const bool solve = r < dl && r >= lol;
+                double rn = abs1(rhs);
+                int ex = 0;
+                if (solve && rn > 0.0 && ilogb(rn) - ilogb(abs1(c)) > kLog2Big - 2) {
+                    ex = scale_exponent(rn, abs1(c), kLog2Big);
+                }
+                if (__any(ex != 0)) {
+                    double f = ldexp(1.0, -ex);
+#pragma clang loop unroll(full)
+                    for (int j = 0; j < MAXB; ++j) {
+                        v[j] *= f;
+                    }
+                    rhs = {rhs.r * f, rhs.i * f};
+                    my_exp -= ex;
+                }
+                cpx x = solve ? cdiv(rhs, c) : (r == dl ? x0 : (r == dl + 1 ? x1v : cpx{0.0, 0.0}));
+                xs = is_im ? x.i : x.r;
+                xp = is_im ? x.r : x.i;
+            }
+            v[r] = xs;
+            const double ax = a * xs;
+            const double bx = bR * xs + bIs * xp;
+#pragma clang loop unroll(full)
+            for (int l = 0; l < r; ++l) {
+                if (l == r - 1 && bump_now) {
+                    continue;
+                }
+                double2 e = sp[l + r * MAXB];
+                v[l] = fma(-e.x, ax, v[l]);
+                v[l] = fma(e.y, bx, v[l]);
+            }
+        }
+    }
+
+    if (!act) {
+        // keep every lane in the shuffles below
+    }
+#pragma clang loop unroll(full)
+    for (int r = 0; r < MAXB; ++r) {
+        if (r < nb) {
+            double o = v[r];
+            double q = __shfl(o, plane);
+            if (act) {
+                X[(long)(i0 + r) * ldx + p] = o;
+                W[(long)r * ldw + p] = -a * o;
+                W[(long)(nb + r) * ldw + p] = bR * o + bIs * (is_pair ? q : 0.0);
+            }
+        }
+    }
+    if (act) {
+        sig[p] = ldexp(1.0, my_exp);
+        int ce = colexp[p] + my_exp;
+        colexp[p] = ce;
+        blkexp[p] = ce;
+        if (my_exp != 0) {
+            *flag = 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// k_gemm_f64: D[a][b] = sum_k L[k][a] * R[k][b] on FP64 MFMA, 16x16x4 tiles.
+//   L: two K segments, L[a + k*ldl] (segment 1, K1 rows) then L2[a + (k-K1)*ldl2]
+//   R: R[k*ldr + b]
+//   C: row a starts at crow(a) = C + (outcol ? outcol[a] : a) * ldc; element b contiguous.
+//   mode 0: C = D;  mode 1: C = sig[b] * C + D.
+//   kmax (optional): per-a upper bound of k (exclusive); a tile uses the max over its rows,
+//   so triangular operands skip their zero part.
+// Workgroup: 256 threads (2x2 waves), tile 128 (a) x 128 (b), K chunks of 16 through LDS.
+// ---------------------------------------------------------------------------------------
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+constexpr int GT = 128; // tile edge
+constexpr int GK = 16;  // K chunk
+
+template <int TAG>
+__global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, const double *__restrict__ L, long ldl,
+                                                  const double *__restrict__ L2, long ldl2,
+                                                  const double *__restrict__ R, long ldr, double *__restrict__ C,
+                                                  long ldc, const int *__restrict__ outcol, int mode,
+                                                  const double *__restrict__ sig, const int *__restrict__ kmax)
+{
+    __shared__ double Ls[2][GK][GT];
+    __shared__ double Rs[2][GK][GT];
+
+    const int a0 = blockIdx.y * GT;
+    const int b0 = blockIdx.x * GT;
+    const int tid = threadIdx.x;
+    const int wave = tid / WAVE;
+    const int lane = tid % WAVE;
+    const int wa = (wave >> 1) * 64; // wave offset in a
+    const int wb = (wave & 1) * 64;  // wave offset in b
+
+    int K = K1 + K2;
+    if (kmax != nullptr) {
+        int amax = min(a0 + GT, M) - 1;
+        int kk = 0;
+        for (int q = a0; q <= amax; ++q) {
+            kk = max(kk, kmax[q]);
+        }
+        K = min(K, kk);
+    }
+
+    double4_t acc[4][4];
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 4; ++i) {
+#pragma clang loop unroll(full)
+        for (int j = 0; j < 4; ++j) {
+            acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+        }
+    }
+
+    // global -> register staging: each thread loads 8 doubles of L and 8 of R per chunk
+    // thread layout: column index c = tid % 128, k rows kr = tid / 128 + 2*s (s = 0..7)
+    const int lc = tid % GT;
+    const int lk = tid / GT;
+    double lreg[8], rreg[8];
+    auto load_chunk = [&](int k0) {
+#pragma clang loop unroll(full)
+        for (int s = 0; s < 8; ++s) {
+            int k = k0 + lk + 2 * s;
+            int aa = a0 + lc;
+            double lv = 0.0, rv = 0.0;
+            if (k < K) {
+                if (aa < M) {
+                    lv = (k < K1) ? L[aa + (long)k * ldl] : L2[aa + (long)(k - K1) * ldl2];
+                }
+                int bb = b0 + lc;
+                if (bb < N) {
+                    rv = R[(long)k * ldr + bb];
+                }
+            }
+            lreg[s] = lv;
+            rreg[s] = rv;
+        }
+    };
+    auto store_chunk = [&](int buf) {
+#pragma clang loop unroll(full)
+        for (int s = 0; s < 8; ++s) {
+            Ls[buf][lk + 2 * s][lc] = lreg[s];
+            Rs[buf][lk + 2 * s][lc] = rreg[s];
+        }
+    };
+
+    const int nchunks = (K + GK - 1) / GK;
+    if (nchunks > 0) {
+        load_chunk(0);
+        store_chunk(0);
+    }
+    __syncthreads();
+    const int fr = lane % 16; // row/col within a 16x16 tile for fragment loads
+    const int fk = lane / 16; // k within a 4-step
+    for (int ch = 0; ch < nchunks; ++ch) {
+        const int buf = ch & 1;
+        if (ch + 1 < nchunks) {
+            load_chunk((ch + 1) * GK);
+        }
+#pragma clang loop unroll(full)
+        for (int ks = 0; ks < GK; ks += 4) {
+            double af[4], bf[4];
+#pragma clang loop unroll(full)
+            for (int i = 0; i < 4; ++i) {
+                af[i] = Ls[buf][ks + fk][wa + i * 16 + fr];
+                bf[i] = Rs[buf][ks + fk][wb + i * 16 + fr];
+            }
+#pragma clang loop unroll(full)
+            for (int i = 0; i < 4; ++i) {
+#pragma clang loop unroll(full)
+                for (int j = 0; j < 4; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+                }
+            }
+        }
+        if (ch + 1 < nchunks) {
+            store_chunk(buf ^ 1);
+        }
+        __syncthreads();
+    }
+
+    // epilogue: lane holds D[4*(lane/16) + v][lane%16] of each 16x16 tile
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 4; ++i) {
+#pragma clang loop unroll(full)
+        for (int vv = 0; vv < 4; ++vv) {
+            int aa = a0 + wa + i * 16 + 4 * fk + vv;
+            if (aa >= M) {
+                continue;
+            }
+            long rowoff = (long)(outcol ? outcol[aa] : aa) * ldc;
+#pragma clang loop unroll(full)
+            for (int j = 0; j < 4; ++j) {
+                int bb = b0 + wb + j * 16 + fr;
+                if (bb < N) {
+                    double d = acc[i][j][vv];
+                    double *cp = C + rowoff + bb;
+                    if (mode == 1) {
+                        d = fma(sig[bb], *cp, d);
+                    }
+                    *cp = d;
+                }
+            }
+        }
+    }
+}
+
+// X[r][p] *= 2^(colexp[p] - blkexp[step(r)][p]) for the solved rows r <= top(p).
+__global__ void k_fixup_scale(double *__restrict__ X, long ldx, int n, int m, const int *__restrict__ rowstep,
+                              const int *__restrict__ blkexp, long ldb, const int *__restrict__ colexp,
+                              const int *__restrict__ flag)
+{
+    if (*flag == 0) {
+        return;
+    }
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = blockIdx.y;
+    if (p >= m || r >= n) {
+        return;
+    }
+    int s = rowstep[r];
+    int e = colexp[p] - blkexp[(long)s * ldb + p];
+    if (e != 0) {
+        double *x = X + (long)r * ldx + p;
+        *x = ldexp(*x, e);
+    }
+}
+
+// V[i + outcol(p)*ldv] = X[row(i)][p] with row(i) = rev ? n-1-i : i, through a 32x32 LDS tile.
+__global__ void k_copy_out(const double *__restrict__ X, long ldx, int n, int m, double *__restrict__ V, long ldv,
+                           const int *__restrict__ outcol, int rev)
+{
+    __shared__ double tile[32][33];
+    int p0 = blockIdx.x * 32, i0 = blockIdx.y * 32;
+    int tx = threadIdx.x, ty = threadIdx.y; // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        int i = i0 + k, p = p0 + tx;
+        double v = 0.0;
+        if (i < n && p < m) {
+            int row = rev ? n - 1 - i : i;
+            v = X[(long)row * ldx + p];
+        }
+        tile[k][tx] = v;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        int p = p0 + k, i = i0 + tx;
+        if (i < n && p < m) {
+            V[i + (long)outcol[p] * ldv] = tile[tx][k];
+        }
+    }
+}
+
+// One workgroup per vector: largest |v| (pairs: |re| + |im|) over n rows, then scale by its
+// reciprocal (LAPACK xTGEVC convention).  vcol[q] = first output column, vw[q] = 1 or 2.
+__global__ __launch_bounds__(256) void k_normalize(double *__restrict__ V, long ldv, int n, const int *__restrict__ vcol,
+                                                   const int *__restrict__ vw)
+{
+    __shared__ double red[256 / WAVE];
+    const int q = blockIdx.x;
+    double *c0 = V + (long)vcol[q] * ldv;
+    double *c1 = c0 + ldv;
+    const bool two = vw[q] == 2;
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double t = fabs(c0[i]);
+        if (two) {
+            t += fabs(c1[i]);
+        }
+        mx = fmax(mx, t);
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, off));
+    }
+    if (threadIdx.x % WAVE == 0) {
+        red[threadIdx.x / WAVE] = mx;
+    }
+    __syncthreads();
+    mx = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / WAVE); ++w) {
+        mx = fmax(mx, red[w]);
+    }
+    if (!(mx > kSafeMin)) {
+        return;
+    }
+    double sc = 1.0 / mx;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        c0[i] *= sc;
+        if (two) {
+            c1[i] *= sc;
+        }
+    }
+}
+
+// D[i + j*ldd] = A[(n-1-j) + (n-1-i)*lda]  (D = J A^T J), 32x32 LDS tiles.
+__global__ void k_reverse_transpose(const double *__restrict__ A, long lda, int n, double *__restrict__ D, long ldd)
+{
+    __shared__ double tile[32][33];
+    int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
+    int tx = threadIdx.x, ty = threadIdx.y;
+    // read A rows r = n-1-(j0..j0+31), cols c = n-1-(i0..i0+31); coalesce along r
+    for (int k = ty; k < 32; k += 8) {
+        int j = j0 + tx, i = i0 + k;
+        double v = 0.0;
+        if (i < n && j < n) {
+            v = A[(long)(n - 1 - j) + (long)(n - 1 - i) * lda];
+        }
+        tile[k][tx] = v; // tile[i-i0][j-j0]
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        int i = i0 + tx, j = j0 + k;
+        if (i < n && j < n) {
+            D[i + (long)j * ldd] = tile[tx][k];
+        }
+    }
+}
+
+// Column sums of |S| and |P| over rows 0..min(n-1, c+hess) (hess = 1: upper Hessenberg
+// part, dtgevc3.cpp:1252-1269; hess = 0: triangle, ztgevc3.cpp:528-542).
+__global__ void k_colnorms(const double *__restrict__ S, long lds, const double *__restrict__ P, long ldp, int n,
+                           int hess, double *__restrict__ out)
+{
+    int c = blockIdx.x;
+    int lim = min(n - 1, c + hess);
+    double ss = 0.0, sp = 0.0;
+    for (int r = threadIdx.x; r <= lim; r += blockDim.x) {
+        ss += fabs(S[r + (long)c * lds]);
+        sp += fabs(P[r + (long)c * ldp]);
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        ss += __shfl_xor(ss, off);
+        sp += __shfl_xor(sp, off);
+    }
+    __shared__ double red[2][4];
+    if (threadIdx.x % WAVE == 0) {
+        red[0][threadIdx.x / WAVE] = ss;
+        red[1][threadIdx.x / WAVE] = sp;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / WAVE); ++w) {
+            a += red[0][w];
+            b += red[1][w];
+        }
+        out[c] = a;
+        out[n + c] = b;
+    }
+}
+
+// Fill a row-major block with zeros (used instead of hipMemsetAsync so the sequence can be
+// captured in a HIP graph).
+__global__ void k_zero(double *__restrict__ X, long count)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) {
+        X[i] = 0.0;
+    }
+}
+
+__global__ void k_zero_int(int *__restrict__ X, long count)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) {
+        X[i] = 0;
+    }
+}
+
+} // namespace xtg
+
+namespace xtg {
+// TAG only separates the two uses in profiles: 0 = RHS update, 1 = back-transform.
+template __global__ void k_gemm_f64<0>(int, int, int, int, const double *, long, const double *, long, const double *,
+                                       long, double *, long, const int *, int, const double *, const int *);
+template __global__ void k_gemm_f64<1>(int, int, int, int, const double *, long, const double *, long, const double *,
+                                       long, double *, long, const int *, int, const double *, const int *);
+template __global__ void k_solve_real<65>(const double *, long, const double *, long, int, int, double *, long,
+                                          double *, long, const ColDesc *, const int2 *, int, int, int, double *,
+                                          int *, int *, int *);
+} // namespace xtg
