@@ -72,6 +72,20 @@ int xtgevc3_ztgevc3_device(char side, char howmny, const int *select, int n, con
                            const xtgevc3_zcomplex *beta, xtgevc3_zcomplex *VL, int ldvl, xtgevc3_zcomplex *VR,
                            int ldvr, int mm, int *m, int *info, void *stream);
 
+/* Backward (adjoint) of the selected right eigenvectors without back-transform (no
+ * counterpart in the reference): for the normalised output V (n x m) of
+ * xtgevc3_dtgevc3_device(side='R', howmny = select ? 'S' : 'A') and nbatch cotangent batches
+ * G (column b*m + c = dL/dV[:, c] of batch b), writes dL/dS and dL/dP of every batch
+ * (n x n column-major matrices at dS + b*n*ldds, dP + b*n*lddp).  alpha_k = S_kk and
+ * beta_k = P_kk are differentiated as functions of the diagonals.  Supports real, finite
+ * selected eigenvalues only: info = XTGEVC3_INFO_UNSUPPORTED otherwise.  Device arrays,
+ * host eigenvalues/select. */
+#define XTGEVC3_INFO_UNSUPPORTED (-1001)
+int xtgevc3_dtgevc3_backward_device(int n, const double *S, int lds, const double *P, int ldp, const double *alphar,
+                                    const double *alphai, const double *beta, const int *select, const double *V,
+                                    int ldv, const double *G, int ldg, int nbatch, double *dS, int ldds, double *dP,
+                                    int lddp, int *info, void *stream);
+
 /* Live kernel timing (no counterpart in the reference).  When enabled, every launch of the
  * hot kernels is bracketed by HIP events on its stream; xtgevc3_profile_read synchronises,
  * returns per-kind totals (milliseconds, launches, algorithmic flops and bytes) and resets.

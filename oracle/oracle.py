@@ -244,3 +244,37 @@ def _openblas():
 
     d = os.path.realpath(os.path.join(os.path.dirname(scipy.__file__), "..", "scipy.libs"))
     return _load(glob.glob(os.path.join(d, "libscipy_openblas*.so"))[0])
+
+
+# ---------------------------------------------------------------------------------------
+# FP64 implicit-differentiation reference gradient of selected right eigenvectors
+# ---------------------------------------------------------------------------------------
+def implicit_grad_right(S, P, idx, V, G):
+    """dL/dS, dL/dP for L = sum_c <G[:, c], V[:, c]>, V[:, c] the LAPACK-normalised right
+    eigenvector of the real eigenvalue alpha = S[k,k], beta = P[k,k] (k = idx[c]).
+
+    Implicit function theorem on (beta S - alpha P)[0:k, 0:k+1] [x; 1] = 0 with x the raw
+    vector (x_k = 1), v = x / max|x|: with g'' = g - (g.v) sign(v_p) e_p restricted to rows
+    < k and lambda = T^{-T} g'' (T = (beta S - alpha P)[0:k, 0:k], dense solve):
+        dL/dS[:k, :k+1] = -beta lambda v^T,  dL/dP[:k, :k+1] = alpha lambda v^T,
+        dL/dS[k,k] += lambda^T (P v)[:k],  dL/dP[k,k] += -lambda^T (S v)[:k].
+    Dense O(k^3) per column: small n only."""
+    n = S.shape[0]
+    dS = np.zeros((n, n))
+    dP = np.zeros((n, n))
+    for c, k in enumerate(idx):
+        v = V[:, c]
+        g = G[:, c]
+        al, be = S[k, k], P[k, k]
+        p = int(np.argmax(np.abs(v)))
+        gg = g.copy()
+        gg[p] -= np.dot(g, v) * np.sign(v[p])
+        if k == 0:
+            continue
+        T = (be * S - al * P)[:k, :k]
+        lam = np.linalg.solve(T.T, gg[:k])
+        dS[:k, :k + 1] += -be * np.outer(lam, v[:k + 1])
+        dP[:k, :k + 1] += al * np.outer(lam, v[:k + 1])
+        dS[k, k] += lam @ (P[:k, :] @ v)
+        dP[k, k] += -(lam @ (S[:k, :] @ v))
+    return dS, dP

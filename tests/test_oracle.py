@@ -192,3 +192,44 @@ def test_oracle_normalisation_convention():
     O.normalize(V, kind)
     assert np.max(np.abs(V[:, 0])) == 1.0 and V[1, 0] == -1.0
     assert np.isclose(np.max(np.abs(V[:, 1]) + np.abs(V[:, 2])), 1.0)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_implicit_gradient_reference_matches_finite_differences(seed):
+    """Pins oracle.implicit_grad_right (the CPU reference for the backward kernel) against
+    central differences (h = 1e-6) of the oracle forward."""
+    n = 20
+    S, P, ar, ai, be = G.real_quasi_triangular(n, seed)
+    real_k = [k for k in range(n) if ai[k] == 0 and (k == 0 or ai[k - 1] == 0)]
+    sel = np.zeros(n, dtype=np.int32)
+    sel[real_k[-4:]] = 1
+    sel[real_k[1]] = 1
+
+    def fwd(S_, P_):
+        a, b = ar.copy(), be.copy()
+        k = np.nonzero(sel)[0]
+        a[k], b[k] = S_[k, k], P_[k, k]
+        VR = np.zeros((n, n), order="F")
+        O.tgevc3("d", "R", "S", sel, S_, P_, a, b, alphai=ai, VR=VR)
+        kind, idx = O.real_layout("S", sel, ai)
+        return O.normalize(VR[:, :len(idx)].copy(order="F"), kind), idx
+
+    V, idx = fwd(S, P)
+    W = np.random.default_rng(seed).standard_normal(V.shape)
+    rS, rP = O.implicit_grad_right(S, P, idx, V, W)
+    scale = max(np.abs(rS).max(), np.abs(rP).max())
+    h = 1e-6
+    rng = np.random.default_rng(seed + 7)
+    entries = [(int(i), int(j)) for i, j in np.argwhere(np.triu(np.ones((n, n))) != 0)]
+    entries = [entries[q] for q in rng.choice(len(entries), 40, replace=False)] + [(k, k) for k in idx]
+    for i, j in entries:
+        for which in ("S", "P"):
+            Mp = (S if which == "S" else P).copy()
+            Mm = Mp.copy()
+            Mp[i, j] += h
+            Mm[i, j] -= h
+            Lp = np.sum(fwd(Mp, P)[0] * W) if which == "S" else np.sum(fwd(S, Mp)[0] * W)
+            Lm = np.sum(fwd(Mm, P)[0] * W) if which == "S" else np.sum(fwd(S, Mm)[0] * W)
+            fd = (Lp - Lm) / (2 * h)
+            ref = (rS if which == "S" else rP)[i, j]
+            assert abs(fd - ref) <= 1e-7 * scale, (which, i, j, fd, ref)

@@ -25,6 +25,14 @@ if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
                       "or `make -C xtgevc3_amd/csrc`")
 
+# torch ships its own copy of the HIP runtime (same soname, libamdhip64.so.7).  Loading torch
+# first lets this library bind to that copy, so a process that uses both talks to the GPU
+# through one runtime.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover - torch is only plumbing
+    torch = None
+
 _lib = ctypes.CDLL(LIB_PATH)
 _lib.xtgevc3_last_error.restype = ctypes.c_char_p
 _lib.xtgevc3_abi_version.restype = ctypes.c_int
@@ -36,7 +44,7 @@ EXPORTED_SYMBOLS = [
     "xtgevc3_abi_version", "xtgevc3_last_error",
     "xtgevc3_dtgevc3", "xtgevc3_stgevc3", "xtgevc3_ztgevc3", "xtgevc3_ctgevc3",
     "xtgevc3_dtgevc3_device", "xtgevc3_ztgevc3_device",
-    "xtgevc3_profile_enable", "xtgevc3_profile_read",
+    "xtgevc3_profile_enable", "xtgevc3_profile_read", "xtgevc3_dtgevc3_backward_device",
 ]
 
 PROF_KINDS = ("solve", "update", "back", "other")
@@ -207,3 +215,32 @@ def dtgevc3_device(side, howmny, select, S, P, alphar, alphai, beta, VL=None, VR
 
 def ztgevc3_device(side, howmny, select, S, P, alpha, beta, VL=None, VR=None, mm=None, stream=None):
     return tgevc3_device("z", side, howmny, select, S, P, alpha, beta, VL=VL, VR=VR, mm=mm, stream=stream)
+
+
+def dtgevc3_backward_device(S, P, alphar, alphai, beta, select, V, G, nbatch=1, dS=None, dP=None, stream=None):
+    """Backward of ``dtgevc3_device('R', 'S' or 'A', ...)`` for real selected eigenvalues.
+    S, P, V, G, dS, dP: torch tensors holding column-major storage (shape (cols, ld)).
+    G holds nbatch * m columns.  Returns (dS, dP) with nbatch stacked n x n matrices each
+    (tensors of shape (nbatch * n, n) in column-major storage)."""
+    import torch
+
+    c = ctypes
+    n = S.shape[0]
+    if dS is None:
+        dS = torch.zeros((nbatch * n, n), dtype=torch.float64, device=S.device)
+    if dP is None:
+        dP = torch.zeros((nbatch * n, n), dtype=torch.float64, device=S.device)
+    sel = None if select is None else np.ascontiguousarray(select, dtype=np.int32)
+    ev = [np.ascontiguousarray(x, dtype=np.float64) for x in (alphar, alphai, beta)]
+    info = c.c_int(-99)
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    rc = _lib.xtgevc3_dtgevc3_backward_device(
+        c.c_int(n), _tptr(S), c.c_int(S.shape[1]), _tptr(P), c.c_int(P.shape[1]), *[_ptr(e) for e in ev], _ptr(sel),
+        _tptr(V), c.c_int(V.shape[1]), _tptr(G), c.c_int(G.shape[1]), c.c_int(nbatch), _tptr(dS), c.c_int(n),
+        _tptr(dP), c.c_int(n), c.byref(info), c.c_void_p(stream))
+    if rc != 0:
+        raise RuntimeError("xtgevc3 backward failed: " + last_error())
+    if info.value != 0:
+        raise ValueError(f"xtgevc3 backward: info={info.value} (complex or infinite selected eigenvalue)")
+    return dS, dP

@@ -370,6 +370,20 @@ int xtgevc3_ztgevc3_device(char side, char howmny, const int *select, int n, con
     return 0;
 }
 
+
+int xtgevc3_dtgevc3_backward_device(int n, const double *S, int lds, const double *P, int ldp, const double *alphar,
+                                    const double *alphai, const double *beta, const int *select, const double *V,
+                                    int ldv, const double *G, int ldg, int nbatch, double *dS, int ldds, double *dP,
+                                    int lddp, int *info, void *stream)
+{
+    int rc = xtg::real_backward_device(n, S, lds, P, ldp, alphar, alphai, beta, select, V, ldv, G, ldg, nbatch, dS,
+                                       ldds, dP, lddp, info, (hipStream_t)stream);
+    if (rc != 0) {
+        return xtg::fail(info, "backward");
+    }
+    return 0;
+}
+
 } // extern "C"
 
 // Drop-in C++-linkage symbols of include/tgevc3.h:10-11.

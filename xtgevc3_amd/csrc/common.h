@@ -50,3 +50,20 @@ constexpr int WAVE = 64;
             return -1000;                                                                          \
         }                                                                                          \
     } while (0)
+
+namespace xtg {
+// XTGEVC3_DEBUG_SYNC=1 in the environment: synchronise and check after every launch.
+bool debug_sync();
+} // namespace xtg
+
+#define XTG_LAUNCH_CHECK(name)                                                                     \
+    do {                                                                                           \
+        hipError_t le_ = hipGetLastError();                                                        \
+        if (le_ == hipSuccess && xtg::debug_sync()) {                                              \
+            le_ = hipDeviceSynchronize();                                                          \
+        }                                                                                          \
+        if (le_ != hipSuccess) {                                                                   \
+            xtg::set_error(std::string("launch of ") + (name) + ": " + hipGetErrorString(le_));    \
+            return -1000;                                                                          \
+        }                                                                                          \
+    } while (0)

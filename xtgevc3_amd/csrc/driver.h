@@ -34,6 +34,11 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
                    const std::complex<double> *be, std::complex<double> *VL, long ldvl, std::complex<double> *VR,
                    long ldvr, hipStream_t st);
 
+int real_backward_device(int n, const double *S, long lds, const double *P, long ldp, const double *ar,
+                         const double *ai, const double *be, const int *select, const double *V, long ldv,
+                         const double *G, long ldg, int nbatch, double *dS, long ldds, double *dP, long lddp, int *info,
+                         hipStream_t st);
+
 // live timing of kernel launches (profile.hip); begin returns -1 when disabled
 int prof_begin(hipStream_t st);
 void prof_end(int idx, hipStream_t st, int kind, double flops, double bytes);

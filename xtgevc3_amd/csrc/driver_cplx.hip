@@ -161,10 +161,10 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
     XTG_CHECK(hipMemcpyAsync(ws.rowstep.p, rowstep.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     double2 *X = ws.X.as<double2>();
     double2 *W = ws.W.as<double2>();
-    hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, (double *)X, 2L * n * ldx);
-    hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, ws.blkexp.as<int>(), (long)nsteps * ldx);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(256), 0, st, ws.colexp.as<int>(), (long)m);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, ws.flag.as<int>(), 4L);
+    hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, (double *)X, 2L * n * ldx); XTG_LAUNCH_CHECK("k_zero");
+    hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, ws.blkexp.as<int>(), (long)nsteps * ldx); XTG_LAUNCH_CHECK("k_zero_int");
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(256), 0, st, ws.colexp.as<int>(), (long)m); XTG_LAUNCH_CHECK("k_zero_int");
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, ws.flag.as<int>(), 4L); XTG_LAUNCH_CHECK("k_zero_int");
 
     int c_begin = m;
     for (int s = 0; s < nsteps; ++s) {
@@ -179,7 +179,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
         int pr = prof_begin(st);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kz_solve<NB_Z>), dim3((N + 255) / 256), dim3(256), 0, st, Sw, ldsw, Pw,
                            ldpw, i0, nb, X, ldx, W, ldx, ws.desc.as<ZColDesc>(), c_begin, m, ws.sig.as<double>(),
-                           ws.colexp.as<int>(), ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>());
+                           ws.colexp.as<int>(), ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>()); XTG_LAUNCH_CHECK("kz_solve<NB_Z>");
         prof_end(pr, st, 0, 8.0 * nb * nb * (double)N, 16.0 * (4.0 * nb * N + 2.0 * nb * nb));
         if (i0 > 0) {
             dim3 grid((N + ZTB_H - 1) / ZTB_H, (i0 + ZTA_H - 1) / ZTA_H);
@@ -187,7 +187,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
             hipLaunchKernelGGL(kz_gemm<0>, grid, dim3(256), 0, st, i0, N, nb, nb, Sw + (long)i0 * ldsw, ldsw,
                                Pw + (long)i0 * ldpw, ldpw, (const double2 *)(W + c_begin), ldx, X + c_begin, ldx,
                                (const int *)nullptr, 1, (const double *)(ws.sig.as<double>() + c_begin),
-                               (const int *)nullptr);
+                               (const int *)nullptr); XTG_LAUNCH_CHECK("kz_gemm<0>");
             prof_end(pr, st, 1, 8.0 * i0 * N * (2.0 * nb), 16.0 * (2.0 * i0 * N + 2.0 * nb * i0 + 2.0 * nb * N));
         }
     }
@@ -195,7 +195,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
     {
         dim3 grid((m + 255) / 256, n);
         hipLaunchKernelGGL(kz_fixup_scale, grid, dim3(256), 0, st, X, ldx, n, m, ws.rowstep.as<int>(),
-                           ws.blkexp.as<int>(), ldx, ws.colexp.as<int>(), ws.flag.as<int>());
+                           ws.blkexp.as<int>(), ldx, ws.colexp.as<int>(), ws.flag.as<int>()); XTG_LAUNCH_CHECK("kz_fixup_scale");
     }
     if (back) {
         const double2 *Rp = left ? Z + (long)(n - 1) * ldz : Z;
@@ -210,15 +210,15 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
         int pr = prof_begin(st);
         hipLaunchKernelGGL(kz_gemm<1>, grid, dim3(256), 0, st, m, n, n, 0, (const double2 *)X, ldx,
                            (const double2 *)nullptr, 0L, Rp, ldr, V, ldv, ws.outcol.as<int>(), 0,
-                           (const double *)nullptr, ws.kmax.as<int>());
+                           (const double *)nullptr, ws.kmax.as<int>()); XTG_LAUNCH_CHECK("kz_gemm<1>");
         prof_end(pr, st, 2, 8.0 * n * ksum, 16.0 * ((double)n * kmx + ksum + (double)n * m));
     }
     else {
         dim3 grid((m + 31) / 32, (n + 31) / 32);
         hipLaunchKernelGGL(kz_copy_out, grid, dim3(32, 8), 0, st, (const double2 *)X, ldx, n, m, V, ldv,
-                           ws.outcol.as<int>(), left ? 1 : 0);
+                           ws.outcol.as<int>(), left ? 1 : 0); XTG_LAUNCH_CHECK("kz_copy_out");
     }
-    hipLaunchKernelGGL(kz_normalize, dim3(m), dim3(256), 0, st, V, ldv, n);
+    hipLaunchKernelGGL(kz_normalize, dim3(m), dim3(256), 0, st, V, ldv, n); XTG_LAUNCH_CHECK("kz_normalize");
     XTG_CHECK(hipGetLastError());
     return 0;
 }
@@ -245,7 +245,7 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
     const double2 *dS = reinterpret_cast<const double2 *>(S);
     const double2 *dP = reinterpret_cast<const double2 *>(P);
     ZENSURE(ws.norms, sizeof(double) * 2 * (size_t)n);
-    hipLaunchKernelGGL(kz_colnorms, dim3(n), dim3(256), 0, st, dS, lds, dP, ldp, n, ws.norms.as<double>());
+    hipLaunchKernelGGL(kz_colnorms, dim3(n), dim3(256), 0, st, dS, lds, dP, ldp, n, ws.norms.as<double>()); XTG_LAUNCH_CHECK("kz_colnorms");
     std::vector<double> nrm(2 * (size_t)n);
     XTG_CHECK(hipMemcpyAsync(nrm.data(), ws.norms.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
     XTG_CHECK(hipStreamSynchronize(st));
@@ -275,8 +275,8 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
         ZENSURE(ws.Sw, sizeof(double2) * (size_t)n * n);
         ZENSURE(ws.Pw, sizeof(double2) * (size_t)n * n);
         dim3 grid((n + 31) / 32, (n + 31) / 32);
-        hipLaunchKernelGGL(kz_reverse_conjtranspose, grid, dim3(32, 8), 0, st, dS, lds, n, ws.Sw.as<double2>(), (long)n);
-        hipLaunchKernelGGL(kz_reverse_conjtranspose, grid, dim3(32, 8), 0, st, dP, ldp, n, ws.Pw.as<double2>(), (long)n);
+        hipLaunchKernelGGL(kz_reverse_conjtranspose, grid, dim3(32, 8), 0, st, dS, lds, n, ws.Sw.as<double2>(), (long)n); XTG_LAUNCH_CHECK("kz_reverse_conjtranspose");
+        hipLaunchKernelGGL(kz_reverse_conjtranspose, grid, dim3(32, 8), 0, st, dP, ldp, n, ws.Pw.as<double2>(), (long)n); XTG_LAUNCH_CHECK("kz_reverse_conjtranspose");
         const double2 *Zp = nullptr;
         if (back) {
             ZENSURE(ws.Zt, sizeof(double2) * (size_t)n * n);

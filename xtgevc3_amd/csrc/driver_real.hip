@@ -29,7 +29,7 @@ namespace xtg {
 
 template <int MAXB>
 __global__ void k_solve_real(const double *, long, const double *, long, int, int, double *, long, double *, long,
-                             const ColDesc *, const int2 *, int, int, int, double *, int *, int *, int *);
+                             const ColDesc *, const int2 *, int, int, int, int, double *, int *, int *, int *);
 template <int TAG>
 __global__ void k_gemm_f64(int, int, int, int, const double *, long, const double *, long, const double *, long,
                            double *, long, const int *, int, const double *, const int *);
@@ -38,6 +38,12 @@ __global__ void k_copy_out(const double *, long, int, int, double *, long, const
 __global__ void k_normalize(double *, long, int, const int *, const int *);
 __global__ void k_reverse_transpose(const double *, long, int, double *, long);
 __global__ void k_colnorms(const double *, long, const double *, long, int, int, double *);
+__global__ void k_bwd_rhs(const double *, long, const double *, long, int, int, int, const int *, double *, long);
+__global__ void k_bwd_lambda_t(const double *, long, int, int, int, const int *, const double *, const int *, double *,
+                               double *);
+__global__ void k_to_rowmajor(const double *, long, int, int, double *, long);
+__global__ void k_bwd_diag(const double *, long, const double *, long, int, int, int, const int *, const double *,
+                           const int *, double *, long, double *, long);
 __global__ void k_zero(double *, long);
 __global__ void k_zero_int(int *, long);
 
@@ -209,6 +215,7 @@ struct DevBuf {
 
 struct RealWork {
     DevBuf X, W, sig, colexp, blkexp, flag, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
+    DevBuf kcol, coef, Vt, Y, LamS, LamP;
 };
 
 static std::mutex g_ws_mutex;
@@ -243,6 +250,104 @@ struct RealSideArgs {
     long ldv;
     const std::vector<double> *subw; // working subdiagonal, host
 };
+
+// ---------------------------------------------------------------------------------------
+// The right-looking sweep over block rows, bottom to top, on the working pencil.
+//   desc: packed working columns.  Forward (eigenvector) columns have d ascending and are
+//   active from the step holding their own diagonal block upward (a suffix of the list);
+//   adjoint columns (d >= n, lo ascending) are active while their row range [lo, n) meets
+//   the block (a prefix).  X (row-major, ldx) holds the right-hand sides on entry (zero for
+//   eigenvectors) and the scaled-consistent solutions on exit; colexp[p] then holds the
+//   power-of-two exponent e with X[:, p] = 2^e * (true solution).
+// ---------------------------------------------------------------------------------------
+int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *Pw, long ldpw,
+               const std::vector<double> &subw, const std::vector<ColDesc> &desc, bool adjoint, long ldx,
+               hipStream_t st)
+{
+    const int m = (int)desc.size();
+    std::vector<int2> tiles;
+    make_tiles(desc, tiles);
+    std::vector<int> i0s, nbs;
+    partition(n, subw, i0s, nbs);
+    const int nsteps = (int)i0s.size();
+    std::vector<int> rowstep(n);
+    for (int s = 0; s < nsteps; ++s) {
+        for (int r = i0s[s]; r < i0s[s] + nbs[s]; ++r) {
+            rowstep[r] = s;
+        }
+    }
+    const long ldw = ldx;
+    ENSURE(ws.W, sizeof(double) * (size_t)(2 * MAXB_REAL) * ldw);
+    ENSURE(ws.sig, sizeof(double) * (size_t)m);
+    ENSURE(ws.colexp, sizeof(int) * (size_t)m);
+    ENSURE(ws.blkexp, sizeof(int) * (size_t)nsteps * ldx);
+    ENSURE(ws.flag, sizeof(int) * 4);
+    ENSURE(ws.desc, sizeof(ColDesc) * (size_t)m);
+    ENSURE(ws.tiles, sizeof(int2) * tiles.size());
+    ENSURE(ws.rowstep, sizeof(int) * (size_t)n);
+    XTG_CHECK(hipMemcpyAsync(ws.desc.p, desc.data(), sizeof(ColDesc) * m, hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.tiles.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.rowstep.p, rowstep.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    double *X = ws.X.as<double>();
+    double *W = ws.W.as<double>();
+    hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, ws.blkexp.as<int>(), (long)nsteps * ldx); XTG_LAUNCH_CHECK("k_zero_int");
+    hipLaunchKernelGGL(k_zero_int, dim3(64), dim3(256), 0, st, ws.colexp.as<int>(), (long)m); XTG_LAUNCH_CHECK("k_zero_int");
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, ws.flag.as<int>(), 4L); XTG_LAUNCH_CHECK("k_zero_int");
+
+    int c_begin = adjoint ? 0 : m;
+    int c_end = adjoint ? 0 : m;
+    for (int s = 0; s < nsteps; ++s) {
+        const int i0 = i0s[s], nb = nbs[s];
+        if (adjoint) {
+            while (c_end < m && desc[c_end].lo < i0 + nb) {
+                ++c_end;
+            }
+        }
+        else {
+            while (c_begin > 0 && desc[c_begin - 1].d >= i0) {
+                --c_begin;
+            }
+        }
+        if (c_begin >= c_end) {
+            continue;
+        }
+        int t_begin = 0;
+        while (tiles[t_begin].x + tiles[t_begin].y <= c_begin) {
+            ++t_begin;
+        }
+        int t_end = t_begin;
+        while (t_end < (int)tiles.size() && tiles[t_end].x < c_end) {
+            ++t_end;
+        }
+        const int ntl = t_end - t_begin;
+        const int N = c_end - c_begin;
+        int pr = prof_begin(st);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_real<MAXB_REAL>), dim3((ntl + 3) / 4), dim3(256), 0, st, Sw, ldsw,
+                           Pw, ldpw, i0, nb, X, ldx, W, ldw, ws.desc.as<ColDesc>(), ws.tiles.as<int2>(), t_begin,
+                           t_end, c_begin, c_end, ws.sig.as<double>(), ws.colexp.as<int>(),
+                           ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>()); XTG_LAUNCH_CHECK("k_solve_real<MAXB_REAL>");
+        prof_end(pr, st, 0, 2.0 * nb * nb * N, 8.0 * (4.0 * nb * N + 2.0 * nb * nb));
+        // rows below every active column's range start are never used: skip them
+        const int r0 = adjoint ? std::min(desc[c_begin].lo, i0) : 0;
+        const int M = i0 - r0;
+        if (M > 0) {
+            dim3 grid((N + GEMM_TILE - 1) / GEMM_TILE, (M + GEMM_TILE - 1) / GEMM_TILE);
+            pr = prof_begin(st);
+            hipLaunchKernelGGL(k_gemm_f64<0>, grid, dim3(256), 0, st, M, N, nb, nb, Sw + r0 + (long)i0 * ldsw, ldsw,
+                               Pw + r0 + (long)i0 * ldpw, ldpw, (const double *)(W + c_begin), ldw,
+                               X + (long)r0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
+                               (const double *)(ws.sig.as<double>() + c_begin), (const int *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
+            prof_end(pr, st, 1, 2.0 * M * N * (2.0 * nb), 8.0 * (2.0 * M * N + 2.0 * nb * M + 2.0 * nb * N));
+        }
+    }
+    XTG_CHECK(hipGetLastError());
+    // apply the per-column scale history to the solved blocks
+    dim3 grid((m + 255) / 256, n);
+    hipLaunchKernelGGL(k_fixup_scale, grid, dim3(256), 0, st, X, ldx, n, m, ws.rowstep.as<int>(), ws.blkexp.as<int>(),
+                       ldx, ws.colexp.as<int>(), ws.flag.as<int>()); XTG_LAUNCH_CHECK("k_fixup_scale");
+    XTG_CHECK(hipGetLastError());
+    return 0;
+}
 
 static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<Unit> &units, int m, hipStream_t st)
 {
@@ -296,91 +401,22 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
         vcol.push_back(un.outc);
         vw.push_back(un.w);
     }
-    std::vector<int2> tiles;
-    make_tiles(desc, tiles);
-    std::vector<int> i0s, nbs;
-    partition(n, *A.subw, i0s, nbs);
-    const int nsteps = (int)i0s.size();
-    std::vector<int> rowstep(n);
-    for (int s = 0; s < nsteps; ++s) {
-        for (int r = i0s[s]; r < i0s[s] + nbs[s]; ++r) {
-            rowstep[r] = s;
-        }
-    }
-
     const long ldx = ((long)m + 7) / 8 * 8;
-    const long ldw = ldx;
-    ENSURE(ws.X, sizeof(double) * (size_t)n * ldx);
-    ENSURE(ws.W, sizeof(double) * (size_t)(2 * MAXB_REAL) * ldw);
-    ENSURE(ws.sig, sizeof(double) * (size_t)m);
-    ENSURE(ws.colexp, sizeof(int) * (size_t)m);
-    ENSURE(ws.blkexp, sizeof(int) * (size_t)nsteps * ldx);
-    ENSURE(ws.flag, sizeof(int) * 4);
-    ENSURE(ws.desc, sizeof(ColDesc) * (size_t)m);
-    ENSURE(ws.tiles, sizeof(int2) * tiles.size());
     ENSURE(ws.outcol, sizeof(int) * (size_t)m);
     ENSURE(ws.kmax, sizeof(int) * (size_t)m);
-    ENSURE(ws.rowstep, sizeof(int) * (size_t)n);
     ENSURE(ws.vcol, sizeof(int) * vcol.size());
     ENSURE(ws.vw, sizeof(int) * vw.size());
-
-    XTG_CHECK(hipMemcpyAsync(ws.desc.p, desc.data(), sizeof(ColDesc) * m, hipMemcpyHostToDevice, st));
-    XTG_CHECK(hipMemcpyAsync(ws.tiles.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, st));
     XTG_CHECK(hipMemcpyAsync(ws.outcol.p, outcol.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
     XTG_CHECK(hipMemcpyAsync(ws.kmax.p, kmaxv.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
-    XTG_CHECK(hipMemcpyAsync(ws.rowstep.p, rowstep.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     XTG_CHECK(hipMemcpyAsync(ws.vcol.p, vcol.data(), sizeof(int) * vcol.size(), hipMemcpyHostToDevice, st));
     XTG_CHECK(hipMemcpyAsync(ws.vw.p, vw.data(), sizeof(int) * vw.size(), hipMemcpyHostToDevice, st));
-
+    ENSURE(ws.X, sizeof(double) * (size_t)n * ldx);
+    hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, ws.X.as<double>(), (long)n * ldx); XTG_LAUNCH_CHECK("k_zero");
+    int rc = sweep_real(ws, n, A.Sw, A.ldsw, A.Pw, A.ldpw, *A.subw, desc, false, ldx, st);
+    if (rc) {
+        return rc;
+    }
     double *X = ws.X.as<double>();
-    double *W = ws.W.as<double>();
-    hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, X, (long)n * ldx);
-    hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, ws.blkexp.as<int>(), (long)nsteps * ldx);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(256), 0, st, ws.colexp.as<int>(), (long)m);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, ws.flag.as<int>(), 4L);
-
-    // --- right-looking sweep over block rows, bottom to top -----------------------------
-    int c_begin = m;
-    for (int s = 0; s < nsteps; ++s) {
-        const int i0 = i0s[s], nb = nbs[s];
-        // active columns: own diagonal block at or below this block (d ascending)
-        while (c_begin > 0 && desc[c_begin - 1].d >= i0) {
-            --c_begin;
-        }
-        if (c_begin >= m) {
-            continue;
-        }
-        int t_begin = 0;
-        while (tiles[t_begin].x + tiles[t_begin].y <= c_begin) {
-            ++t_begin;
-        }
-        const int ntl = (int)tiles.size() - t_begin;
-        const double Nact = (double)(m - c_begin);
-        int pr = prof_begin(st);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_real<MAXB_REAL>), dim3((ntl + 3) / 4), dim3(256), 0, st, A.Sw,
-                           A.ldsw, A.Pw, A.ldpw, i0, nb, X, ldx, W, ldw, ws.desc.as<ColDesc>(), ws.tiles.as<int2>(),
-                           t_begin, (int)tiles.size(), c_begin, ws.sig.as<double>(), ws.colexp.as<int>(),
-                           ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>());
-        prof_end(pr, st, 0, 2.0 * nb * nb * Nact, 8.0 * (4.0 * nb * Nact + 2.0 * nb * nb));
-        if (i0 > 0) {
-            const int N = m - c_begin;
-            dim3 grid((N + GEMM_TILE - 1) / GEMM_TILE, (i0 + GEMM_TILE - 1) / GEMM_TILE);
-            pr = prof_begin(st);
-            hipLaunchKernelGGL(k_gemm_f64<0>, grid, dim3(256), 0, st, i0, N, nb, nb, A.Sw + (long)i0 * A.ldsw, A.ldsw,
-                               A.Pw + (long)i0 * A.ldpw, A.ldpw, (const double *)(W + c_begin), ldw, X + c_begin, ldx,
-                               (const int *)nullptr, 1, (const double *)(ws.sig.as<double>() + c_begin),
-                               (const int *)nullptr);
-            prof_end(pr, st, 1, 2.0 * i0 * N * (2.0 * nb), 8.0 * (2.0 * i0 * N + 2.0 * nb * i0 + 2.0 * nb * N));
-        }
-    }
-    XTG_CHECK(hipGetLastError());
-
-    // --- apply the per-column scale history to the solved blocks -------------------------
-    {
-        dim3 grid((m + 255) / 256, n);
-        hipLaunchKernelGGL(k_fixup_scale, grid, dim3(256), 0, st, X, ldx, n, m, ws.rowstep.as<int>(),
-                           ws.blkexp.as<int>(), ldx, ws.colexp.as<int>(), ws.flag.as<int>());
-    }
 
     // --- output: back-transform or copy, then normalise ---------------------------------
     if (A.back) {
@@ -397,16 +433,16 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
         int pr = prof_begin(st);
         hipLaunchKernelGGL(k_gemm_f64<1>, grid, dim3(256), 0, st, m, n, n, 0, (const double *)X, ldx,
                            (const double *)nullptr, 0L, Rp, ldr, A.V, A.ldv, ws.outcol.as<int>(), 0,
-                           (const double *)nullptr, ws.kmax.as<int>());
+                           (const double *)nullptr, ws.kmax.as<int>()); XTG_LAUNCH_CHECK("k_gemm_f64<1>");
         prof_end(pr, st, 2, 2.0 * n * ksum, 8.0 * ((double)n * kmx + ksum + (double)n * m));
     }
     else {
         dim3 grid((m + 31) / 32, (n + 31) / 32);
         hipLaunchKernelGGL(k_copy_out, grid, dim3(32, 8), 0, st, (const double *)X, ldx, n, m, A.V, A.ldv,
-                           ws.outcol.as<int>(), A.left ? 1 : 0);
+                           ws.outcol.as<int>(), A.left ? 1 : 0); XTG_LAUNCH_CHECK("k_copy_out");
     }
     hipLaunchKernelGGL(k_normalize, dim3((unsigned)vcol.size()), dim3(256), 0, st, A.V, A.ldv, n, ws.vcol.as<int>(),
-                       ws.vw.as<int>());
+                       ws.vw.as<int>()); XTG_LAUNCH_CHECK("k_normalize");
     XTG_CHECK(hipGetLastError());
     return 0;
 }
@@ -436,7 +472,7 @@ int real_device(char side, char howmny, const int *select, int n, const double *
 
     // global scales from the 1-norms of S and P over the upper Hessenberg part
     ENSURE(ws.norms, sizeof(double) * 2 * (size_t)n);
-    hipLaunchKernelGGL(k_colnorms, dim3(n), dim3(256), 0, st, S, lds, P, ldp, n, 1, ws.norms.as<double>());
+    hipLaunchKernelGGL(k_colnorms, dim3(n), dim3(256), 0, st, S, lds, P, ldp, n, 1, ws.norms.as<double>()); XTG_LAUNCH_CHECK("k_colnorms");
     std::vector<double> nrm(2 * (size_t)n);
     XTG_CHECK(hipMemcpyAsync(nrm.data(), ws.norms.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
     // subdiagonal of S (host copy) for the block snapping
@@ -490,8 +526,8 @@ int real_device(char side, char howmny, const int *select, int n, const double *
         ENSURE(ws.Sw, sizeof(double) * (size_t)n * n);
         ENSURE(ws.Pw, sizeof(double) * (size_t)n * n);
         dim3 grid((n + 31) / 32, (n + 31) / 32);
-        hipLaunchKernelGGL(k_reverse_transpose, grid, dim3(32, 8), 0, st, S, lds, n, ws.Sw.as<double>(), (long)n);
-        hipLaunchKernelGGL(k_reverse_transpose, grid, dim3(32, 8), 0, st, P, ldp, n, ws.Pw.as<double>(), (long)n);
+        hipLaunchKernelGGL(k_reverse_transpose, grid, dim3(32, 8), 0, st, S, lds, n, ws.Sw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
+        hipLaunchKernelGGL(k_reverse_transpose, grid, dim3(32, 8), 0, st, P, ldp, n, ws.Pw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
         // working subdiagonal: Sw[i, i-1] = S[n-i, n-1-i]
         std::vector<double> subw(n, 0.0);
         for (int i = 1; i < n; ++i) {
@@ -525,6 +561,139 @@ int real_device(char side, char howmny, const int *select, int n, const double *
             return rc;
         }
     }
+    return 0;
+}
+
+
+// ---------------------------------------------------------------------------------------
+// Backward of the selected right eigenvectors (no back-transform): dL/dS and dL/dP for
+// nbatch cotangent batches G[:, b*m + c] of the normalised output V (n x m, as the forward
+// returns it).  alpha_k = S_kk and beta_k = P_kk for the selected real eigenvalues.
+// Returns -1001 through *info when a selected eigenvalue is complex or infinite.
+// ---------------------------------------------------------------------------------------
+int real_backward_device(int n, const double *S, long lds, const double *P, long ldp, const double *ar,
+                         const double *ai, const double *be, const int *select, const double *V, long ldv,
+                         const double *G, long ldg, int nbatch, double *dS, long ldds, double *dP, long lddp, int *info,
+                         hipStream_t st)
+{
+    int dev = 0;
+    XTG_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    RealWork &ws = g_ws[dev & 63];
+    std::vector<Unit> units;
+    int m = real_units(select ? 'S' : 'A', select, n, ai, units);
+    *info = 0;
+    for (const Unit &u : units) {
+        if (u.w != 1 || std::fabs(be[u.k]) <= std::numeric_limits<double>::min()) {
+            *info = -1001;
+            return 0;
+        }
+    }
+    if (m == 0 || nbatch <= 0) {
+        return 0;
+    }
+    ENSURE(ws.norms, sizeof(double) * 2 * (size_t)n);
+    hipLaunchKernelGGL(k_colnorms, dim3(n), dim3(256), 0, st, S, lds, P, ldp, n, 1, ws.norms.as<double>()); XTG_LAUNCH_CHECK("k_colnorms");
+    std::vector<double> nrm(2 * (size_t)n), sub(n, 0.0);
+    XTG_CHECK(hipMemcpyAsync(nrm.data(), ws.norms.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
+    if (n > 1) {
+        XTG_CHECK(hipMemcpy2DAsync(sub.data() + 1, sizeof(double), S + 1, sizeof(double) * (lds + 1), sizeof(double),
+                                   n - 1, hipMemcpyDeviceToHost, st));
+    }
+    XTG_CHECK(hipStreamSynchronize(st));
+    double an = 0.0, bn = 0.0;
+    for (int c = 0; c < n; ++c) {
+        an = std::max(an, nrm[c]);
+        bn = std::max(bn, nrm[n + c]);
+    }
+    const double safemin = std::numeric_limits<double>::min();
+    const double ascale = 1.0 / std::max(an, safemin), bscale = 1.0 / std::max(bn, safemin);
+    std::vector<double> subw(n, 0.0);
+    for (int i = 1; i < n; ++i) {
+        subw[i] = sub[n - i];
+    }
+    // per forward column: a, b, s = a / beta, b / a; working columns k-descending, batches adjacent
+    std::vector<int> kcol(m);
+    std::vector<double> coef(4 * (size_t)m);
+    const int M = m * nbatch;
+    std::vector<ColDesc> desc(M);
+    for (int c = 0; c < m; ++c) {
+        const int k = units[c].k;
+        double a, bR, bI;
+        shift_real(ar[k], 0.0, be[k], ascale, bscale, a, bR, bI, false);
+        kcol[c] = k;
+        coef[4 * c + 0] = a;
+        coef[4 * c + 1] = bR;
+        coef[4 * c + 2] = a / be[k];
+        coef[4 * c + 3] = (a != 0.0) ? bR / a : 0.0;
+        for (int b = 0; b < nbatch; ++b) {
+            int q = (m - 1 - c) * nbatch + b;
+            ColDesc d{};
+            d.a = a;
+            d.bR = bR;
+            d.bI = 0.0;
+            d.d = n;
+            d.lo = n - k;
+            d.partner = q;
+            d.flags = INIT_NONE << 2;
+            desc[q] = d;
+        }
+    }
+    const long ldx = ((long)M + 7) / 8 * 8;
+    const long ldt = ((long)m + 7) / 8 * 8;
+    ENSURE(ws.kcol, sizeof(int) * (size_t)m);
+    ENSURE(ws.coef, sizeof(double) * 4 * (size_t)m);
+    ENSURE(ws.Sw, sizeof(double) * (size_t)n * n);
+    ENSURE(ws.Pw, sizeof(double) * (size_t)n * n);
+    ENSURE(ws.X, sizeof(double) * (size_t)n * ldx);
+    ENSURE(ws.Vt, sizeof(double) * (size_t)n * ldt);
+    ENSURE(ws.Y, sizeof(double) * (size_t)n * ldt);
+    ENSURE(ws.LamS, sizeof(double) * (size_t)M * n);
+    ENSURE(ws.LamP, sizeof(double) * (size_t)M * n);
+    XTG_CHECK(hipMemcpyAsync(ws.kcol.p, kcol.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.coef.p, coef.data(), sizeof(double) * 4 * m, hipMemcpyHostToDevice, st));
+    dim3 g32((n + 31) / 32, (n + 31) / 32);
+    hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, S, lds, n, ws.Sw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
+    hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, P, ldp, n, ws.Pw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
+    double *X = ws.X.as<double>();
+    hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, X, (long)n * ldx); XTG_LAUNCH_CHECK("k_zero");
+    int pr = prof_begin(st);
+    hipLaunchKernelGGL(k_bwd_rhs, dim3(M), dim3(256), 0, st, V, ldv, G, ldg, n, m, nbatch, ws.kcol.as<int>(), X, ldx); XTG_LAUNCH_CHECK("k_bwd_rhs");
+    prof_end(pr, st, 3, 0.0, 8.0 * 3.0 * n * M);
+    int rc = sweep_real(ws, n, ws.Sw.as<double>(), n, ws.Pw.as<double>(), n, subw, desc, true, ldx, st);
+    if (rc) {
+        return rc;
+    }
+    // Y = P V  (the alpha / beta terms need lambda^T P v)
+    hipLaunchKernelGGL(k_to_rowmajor, dim3((m + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, st, V, ldv, n, m,
+                       ws.Vt.as<double>(), ldt); XTG_LAUNCH_CHECK("k_to_rowmajor");
+    {
+        dim3 grid((m + GEMM_TILE - 1) / GEMM_TILE, (n + GEMM_TILE - 1) / GEMM_TILE);
+        pr = prof_begin(st);
+        hipLaunchKernelGGL(k_gemm_f64<2>, grid, dim3(256), 0, st, n, m, n, 0, P, ldp, (const double *)nullptr, 0L,
+                           (const double *)ws.Vt.as<double>(), ldt, ws.Y.as<double>(), ldt, (const int *)nullptr, 0,
+                           (const double *)nullptr, (const int *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+        prof_end(pr, st, 3, 2.0 * n * n * (double)m, 8.0 * ((double)n * n + 2.0 * n * m));
+    }
+    hipLaunchKernelGGL(k_bwd_lambda_t, dim3((M + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, st, (const double *)X, ldx,
+                       n, m, nbatch, ws.kcol.as<int>(), ws.coef.as<double>(), ws.colexp.as<int>(),
+                       ws.LamS.as<double>(), ws.LamP.as<double>()); XTG_LAUNCH_CHECK("k_bwd_lambda_t");
+    // dS_b = V * LamS_b, dP_b = V * LamP_b  (as D[j][i] = sum_c V[j][c] Lam[c][i])
+    for (int b = 0; b < nbatch; ++b) {
+        dim3 grid((n + GEMM_TILE - 1) / GEMM_TILE, (n + GEMM_TILE - 1) / GEMM_TILE);
+        pr = prof_begin(st);
+        hipLaunchKernelGGL(k_gemm_f64<2>, grid, dim3(256), 0, st, n, n, m, 0, V, ldv, (const double *)nullptr, 0L,
+                           (const double *)(ws.LamS.as<double>() + (long)b * m * n), (long)n, dS + (long)b * n * ldds,
+                           ldds, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+        hipLaunchKernelGGL(k_gemm_f64<2>, grid, dim3(256), 0, st, n, n, m, 0, V, ldv, (const double *)nullptr, 0L,
+                           (const double *)(ws.LamP.as<double>() + (long)b * m * n), (long)n, dP + (long)b * n * lddp,
+                           lddp, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+        prof_end(pr, st, 3, 4.0 * n * n * (double)m, 8.0 * 2.0 * ((double)n * m + (double)m * n + (double)n * n));
+    }
+    hipLaunchKernelGGL(k_bwd_diag, dim3(M), dim3(64), 0, st, (const double *)X, ldx, (const double *)ws.Y.as<double>(),
+                       ldt, n, m, nbatch, ws.kcol.as<int>(), ws.coef.as<double>(), ws.colexp.as<int>(), dS, ldds, dP,
+                       lddp); XTG_LAUNCH_CHECK("k_bwd_diag");
+    XTG_CHECK(hipGetLastError());
     return 0;
 }
 

@@ -267,7 +267,7 @@ __global__ __launch_bounds__(256) void kz_gemm(int M, int N, int K1, int K2, con
     for (int i = 0; i < 4; ++i) {
 #pragma clang loop unroll(full)
         for (int vv = 0; vv < 4; ++vv) {
-            int aa = a0 + wa + i * 16 + 4 * fk + vv;
+            int aa = a0 + wa + i * 16 + fk + 4 * vv;
             if (aa >= M) {
                 continue;
             }

@@ -119,7 +119,8 @@ __global__ __launch_bounds__(256) void k_solve_real(const double *__restrict__ S
                                                     long ldp, int i0, int nb, double *__restrict__ X, long ldx,
                                                     double *__restrict__ W, long ldw, const ColDesc *__restrict__ desc,
                                                     const int2 *__restrict__ tiles, int t_begin, int t_end,
-                                                    int c_begin, double *__restrict__ sig, int *__restrict__ colexp,
+                                                    int c_begin, int c_end, double *__restrict__ sig,
+                                                    int *__restrict__ colexp,
                                                     int *__restrict__ blkexp, int *__restrict__ flag)
 {
     __shared__ double2 sp[MAXB * MAXB];
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(256) void k_solve_real(const double *__restrict__ S
     }
     const int2 tl = tiles[t];
     const int p = tl.x + lane;
-    const bool act = lane < tl.y && p >= c_begin;
+    const bool act = lane < tl.y && p >= c_begin && p < c_end;
     const ColDesc cd = desc[act ? p : tl.x];
     const bool is_im = cd.flags & 1;
     const bool is_pair = (cd.flags & 2) != 0;
@@ -415,12 +416,12 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
         __syncthreads();
     }
 
-    // epilogue: lane holds D[4*(lane/16) + v][lane%16] of each 16x16 tile
+    // epilogue: lane holds D[lane/16 + 4*v][lane%16] of each 16x16 tile (v_mfma_f64_16x16x4 layout)
 #pragma clang loop unroll(full)
     for (int i = 0; i < 4; ++i) {
 #pragma clang loop unroll(full)
         for (int vv = 0; vv < 4; ++vv) {
-            int aa = a0 + wa + i * 16 + 4 * fk + vv;
+            int aa = a0 + wa + i * 16 + fk + 4 * vv;
             if (aa >= M) {
                 continue;
             }
@@ -614,6 +615,159 @@ template __global__ void k_gemm_f64<0>(int, int, int, int, const double *, long,
 template __global__ void k_gemm_f64<1>(int, int, int, int, const double *, long, const double *, long, const double *,
                                        long, double *, long, const int *, int, const double *, const int *);
 template __global__ void k_solve_real<65>(const double *, long, const double *, long, int, int, double *, long,
-                                          double *, long, const ColDesc *, const int2 *, int, int, int, double *,
+                                          double *, long, const ColDesc *, const int2 *, int, int, int, int, double *,
                                           int *, int *, int *);
+} // namespace xtg
+
+// =======================================================================================
+// Backward (adjoint) kernels of the real path.  For a selected real eigenvalue k with
+// normalised forward vector v (largest |v_p| = 1 at p) and cotangent g = dL/dv:
+//   g'' = g - (g . v) sign(v_p) e_p           (pull-back through the normalisation)
+//   lambda = (a S - b P)[0:k,0:k]^{-T} g''    (transposed shifted solve, same pencil)
+//   dL/dS = -a lambda v^T, dL/dP = b lambda v^T (rows < k, cols <= k)
+//   dL/dalpha = s lambda^T (P v), dL/dbeta = -s (b/a) lambda^T (P v), alpha = S_kk,
+//   beta = P_kk, (a, b) = s (beta, alpha).
+// The solve runs on the reversed transpose J S^T J, so lambda_i sits at working row n-1-i.
+// =======================================================================================
+
+namespace xtg {
+
+// One workgroup per (batch b, forward column c): RHS of the adjoint solve in working rows.
+__global__ __launch_bounds__(256) void k_bwd_rhs(const double *__restrict__ V, long ldv, const double *__restrict__ G,
+                                                 long ldg, int n, int m, int nbatch, const int *__restrict__ kcol,
+                                                 double *__restrict__ X, long ldx)
+{
+    __shared__ double rv[256 / WAVE];
+    __shared__ int ri[256 / WAVE];
+    __shared__ double rd[256 / WAVE];
+    const int c = blockIdx.x % m, b = blockIdx.x / m;
+    const double *v = V + (long)c * ldv;
+    const double *g = G + (long)(b * m + c) * ldg;
+    const int k = kcol[c];
+    const int q = (m - 1 - c) * nbatch + b;
+    double best = -1.0, dot = 0.0;
+    int bi = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double a = fabs(v[i]);
+        if (a > best) {
+            best = a;
+            bi = i;
+        }
+        dot = fma(g[i], v[i], dot);
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        double ob = __shfl_xor(best, off);
+        int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
+        }
+        dot += __shfl_xor(dot, off);
+    }
+    if (threadIdx.x % WAVE == 0) {
+        rv[threadIdx.x / WAVE] = best;
+        ri[threadIdx.x / WAVE] = bi;
+        rd[threadIdx.x / WAVE] = dot;
+    }
+    __syncthreads();
+    best = rv[0];
+    bi = ri[0];
+    dot = rd[0];
+    for (int w = 1; w < (int)(blockDim.x / WAVE); ++w) {
+        if (rv[w] > best || (rv[w] == best && ri[w] < bi)) {
+            best = rv[w];
+            bi = ri[w];
+        }
+        dot += rd[w];
+    }
+    const double sp = v[bi] < 0.0 ? -1.0 : 1.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double val = 0.0;
+        if (i < k) {
+            val = g[i] - (i == bi ? dot * sp : 0.0);
+        }
+        X[(long)(n - 1 - i) * ldx + q] = val;
+    }
+}
+
+// Lam[(b*m + c) * n + i] = coef * 2^-e * lambda_i  for coef = -a_c (S part) and b_c (P part);
+// 32x32 LDS tiles over (working column q, row i).
+__global__ void k_bwd_lambda_t(const double *__restrict__ X, long ldx, int n, int m, int nbatch,
+                               const int *__restrict__ kcol, const double *__restrict__ coef,
+                               const int *__restrict__ colexp, double *__restrict__ LamS, double *__restrict__ LamP)
+{
+    __shared__ double tile[32][33];
+    const int q0 = blockIdx.x * 32, i0 = blockIdx.y * 32;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int M = m * nbatch;
+    for (int kk = ty; kk < 32; kk += 8) {
+        int i = i0 + kk, q = q0 + tx;
+        double v = 0.0;
+        if (i < n && q < M) {
+            int c = m - 1 - q / nbatch;
+            if (i < kcol[c]) {
+                v = ldexp(X[(long)(n - 1 - i) * ldx + q], -colexp[q]);
+            }
+        }
+        tile[kk][tx] = v;
+    }
+    __syncthreads();
+    for (int kk = ty; kk < 32; kk += 8) {
+        int q = q0 + kk, i = i0 + tx;
+        if (i < n && q < M) {
+            int c = m - 1 - q / nbatch, b = q % nbatch;
+            double v = tile[tx][kk];
+            long o = (long)(b * m + c) * n + i;
+            LamS[o] = -coef[4 * c + 0] * v;
+            LamP[o] = coef[4 * c + 1] * v;
+        }
+    }
+}
+
+// Vt[k * ldt + c] = V[k + c * ldv] (column-major -> row-major), 32x32 tiles.
+__global__ void k_to_rowmajor(const double *__restrict__ V, long ldv, int n, int m, double *__restrict__ Vt, long ldt)
+{
+    __shared__ double tile[32][33];
+    const int c0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int kk = ty; kk < 32; kk += 8) {
+        int c = c0 + kk, k = k0 + tx;
+        tile[kk][tx] = (c < m && k < n) ? V[k + (long)c * ldv] : 0.0;
+    }
+    __syncthreads();
+    for (int kk = ty; kk < 32; kk += 8) {
+        int k = k0 + kk, c = c0 + tx;
+        if (c < m && k < n) {
+            Vt[(long)k * ldt + c] = tile[tx][kk];
+        }
+    }
+}
+
+// One wave per (working column q): dL/dalpha and dL/dbeta folded into dS_kk, dP_kk.
+//   Y = P V (row-major, ldy), lambda_i = 2^-e X[n-1-i][q].
+__global__ __launch_bounds__(64) void k_bwd_diag(const double *__restrict__ X, long ldx, const double *__restrict__ Y,
+                                                 long ldy, int n, int m, int nbatch, const int *__restrict__ kcol,
+                                                 const double *__restrict__ coef, const int *__restrict__ colexp,
+                                                 double *__restrict__ dS, long ldds, double *__restrict__ dP, long lddp)
+{
+    const int q = blockIdx.x;
+    const int c = m - 1 - q / nbatch, b = q % nbatch;
+    const int k = kcol[c];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < k; i += WAVE) {
+        acc = fma(X[(long)(n - 1 - i) * ldx + q], Y[(long)i * ldy + c], acc);
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        acc += __shfl_xor(acc, off);
+    }
+    if (threadIdx.x == 0) {
+        double qv = ldexp(acc, -colexp[q]);
+        double s = coef[4 * c + 2], ratio = coef[4 * c + 3];
+        dS[(long)b * n * ldds + k + (long)k * ldds] += s * qv;
+        dP[(long)b * n * lddp + k + (long)k * lddp] += -s * ratio * qv;
+    }
+}
+
+template __global__ void k_gemm_f64<2>(int, int, int, int, const double *, long, const double *, long, const double *,
+                                       long, double *, long, const int *, int, const double *, const int *);
 } // namespace xtg

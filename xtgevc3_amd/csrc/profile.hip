@@ -3,13 +3,25 @@
 // launch duration and algorithmic work next to the whole-call throughput.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/xtgevc3.h"
 #include "driver.h"
 
 namespace xtg {
+
+bool debug_sync()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char *e = std::getenv("XTGEVC3_DEBUG_SYNC");
+        v = (e && e[0] == '1') ? 1 : 0;
+    }
+    return v == 1;
+}
 
 namespace {
 struct Rec {
@@ -74,6 +86,14 @@ void xtgevc3_profile_enable(int on)
 {
     std::lock_guard<std::mutex> l(xtg::g_pm);
     xtg::g_on = on != 0;
+    // create the events up front: hipEventCreate inside the timed launches would stall them
+    while (xtg::g_on && xtg::g_pool.size() < 16384) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) {
+            break;
+        }
+        xtg::g_pool.push_back(e);
+    }
 }
 
 int xtgevc3_profile_read(double *ms, long *launches, double *flops, double *bytes)

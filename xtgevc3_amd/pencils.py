@@ -7,8 +7,10 @@ triangular, and eigenvalues given explicitly, as the reference's interface expec
 * ``real_quasi_triangular`` -- real pencils with 2x2 complex-pair bumps: P diagonal
   |N(0,1)|+0.1, off-diagonal N(0,1); a fraction of the diagonal of S in 2x2 blocks whose
   eigenvalues are complex, with the matching P block diagonal and positive.  Options grade
-  the rows/columns by 10^U(-g, g) (overflow scaling) or cluster the real eigenvalues in
-  groups with relative spread 1e-6 (near-singular shifted blocks).
+  the rows/columns by 10^U(-g, g) (overflow scaling), cluster the real eigenvalues in
+  groups with relative spread 1e-6 (near-singular shifted blocks), or scale the
+  off-diagonal entries (``offdiag``; the eigenvectors of N(0,1) pencils are exponentially
+  ill-conditioned in n, a smaller scale gives well-conditioned test problems).
 * ``complex_triangular`` -- complex S, P upper triangular, off-diagonal CN(0,1), diag S
   CN(0,1), diag P |N(0,1)|+0.1 (real).
 * ``reference_style_real`` / ``reference_style_complex`` -- the reference tests' own
@@ -46,10 +48,10 @@ def pair_eigenvalue(s11, s12, s21, s22, p1, p2):
     return 0.5 * b, 0.5 * np.sqrt(-disc), p1 * p2
 
 
-def real_quasi_triangular(n, seed, pair_frac=0.4, grade=0.0, cluster_groups=0, cluster_spread=1e-6):
+def real_quasi_triangular(n, seed, pair_frac=0.4, grade=0.0, cluster_groups=0, cluster_spread=1e-6, offdiag=1.0):
     rng = np.random.default_rng(seed)
-    S = np.triu(rng.standard_normal((n, n)), 1)
-    P = np.triu(rng.standard_normal((n, n)), 1)
+    S = np.triu(rng.standard_normal((n, n)), 1) * offdiag
+    P = np.triu(rng.standard_normal((n, n)), 1) * offdiag
     ar = np.zeros(n)
     ai = np.zeros(n)
     be = np.zeros(n)
@@ -99,13 +101,11 @@ def real_quasi_triangular(n, seed, pair_frac=0.4, grade=0.0, cluster_groups=0, c
         be = be * scale
         for k in np.nonzero(is_pair)[0]:
             # pair alpha/beta were computed with beta = p1 p2 of the unscaled block
-            sc = dr[k] * dc[k]
             a_r, a_i, b_ = pair_eigenvalue(S[k, k], S[k, k + 1], S[k + 1, k], S[k + 1, k + 1], P[k, k],
                                            P[k + 1, k + 1])
             ar[k] = ar[k + 1] = a_r
             ai[k], ai[k + 1] = a_i, -a_i
             be[k] = be[k + 1] = b_
-            del sc
     return (np.asfortranarray(S), np.asfortranarray(P), ar, ai, be)
 
 
