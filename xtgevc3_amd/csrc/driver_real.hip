@@ -27,12 +27,16 @@
 
 namespace xtg {
 
-template <int MAXB>
+template <int MAXB, int CH>
 __global__ void k_solve_real(const double *, long, const double *, long, int, int, double *, long, double *, long,
-                             const ColDesc *, const int2 *, int, int, int, int, double *, int *, int *, int *);
+                             const ColDesc *, const int2 *, int, int, int, int, double *, int *, int *, int *, int,
+                             int, int, double *, int);
+constexpr int SOLVE_CH = 8;    // register chunk of the solve kernel
+constexpr int SOLVE_WAVES_H = 2; // waves per solve workgroup
 template <int TAG>
 __global__ void k_gemm_f64(int, int, int, int, const double *, long, const double *, long, const double *, long,
-                           double *, long, const int *, int, const double *, const int *);
+                           double *, long, const int *, int, const double *, const int *, const double *);
+constexpr int GROUP_STEPS = 4; // row blocks per far update (K = 2 * ~256)
 __global__ void k_fixup_scale(double *, long, int, int, const int *, const int *, long, const int *, const int *);
 __global__ void k_copy_out(const double *, long, int, int, double *, long, const int *, int);
 __global__ void k_normalize(double *, long, int, const int *, const int *);
@@ -44,6 +48,7 @@ __global__ void k_bwd_lambda_t(const double *, long, int, int, int, const int *,
 __global__ void k_to_rowmajor(const double *, long, int, int, double *, long);
 __global__ void k_bwd_diag(const double *, long, const double *, long, int, int, int, const int *, const double *,
                            const int *, double *, long, double *, long);
+__global__ void k_subdiag(const double *, long, int, double *);
 __global__ void k_zero(double *, long);
 __global__ void k_zero_int(int *, long);
 
@@ -215,7 +220,7 @@ struct DevBuf {
 
 struct RealWork {
     DevBuf X, W, sig, colexp, blkexp, flag, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
-    DevBuf kcol, coef, Vt, Y, LamS, LamP;
+    DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp;
 };
 
 static std::mutex g_ws_mutex;
@@ -277,8 +282,10 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         }
     }
     const long ldw = ldx;
-    ENSURE(ws.W, sizeof(double) * (size_t)(2 * MAXB_REAL) * ldw);
+    const int wmax = GROUP_STEPS * MAXB_REAL;
+    ENSURE(ws.W, sizeof(double) * (size_t)(2 * wmax) * ldw);
     ENSURE(ws.sig, sizeof(double) * (size_t)m);
+    ENSURE(ws.sig_grp, sizeof(double) * (size_t)m);
     ENSURE(ws.colexp, sizeof(int) * (size_t)m);
     ENSURE(ws.blkexp, sizeof(int) * (size_t)nsteps * ldx);
     ENSURE(ws.flag, sizeof(int) * 4);
@@ -294,50 +301,90 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     hipLaunchKernelGGL(k_zero_int, dim3(64), dim3(256), 0, st, ws.colexp.as<int>(), (long)m); XTG_LAUNCH_CHECK("k_zero_int");
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, ws.flag.as<int>(), 4L); XTG_LAUNCH_CHECK("k_zero_int");
 
+    // Steps go bottom-up in groups of GROUP_STEPS row blocks [g0, g1).  Within a group each
+    // step updates only the group's rows above it ("near" update, K = 2 nb); after the group
+    // one "far" update applies all of its blocks to the rows above g0 (K = 2 (g1 - g0)).
+    // The group's operand W holds rows [0, g1-g0) (S part) and [g1-g0, 2(g1-g0)) (P part).
     int c_begin = adjoint ? 0 : m;
     int c_end = adjoint ? 0 : m;
-    for (int s = 0; s < nsteps; ++s) {
-        const int i0 = i0s[s], nb = nbs[s];
-        if (adjoint) {
-            while (c_end < m && desc[c_end].lo < i0 + nb) {
-                ++c_end;
+    for (int gs = 0; gs < nsteps; gs += GROUP_STEPS) {
+        const int ge = std::min(nsteps, gs + GROUP_STEPS);
+        const int g1 = i0s[gs] + nbs[gs];
+        const int g0 = i0s[ge - 1];
+        const int wr = g1 - g0;
+        bool any_active = false;
+        bool first = true;
+        hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, st, W, (long)2 * wr * ldw); XTG_LAUNCH_CHECK("k_zero");
+        for (int s = gs; s < ge; ++s) {
+            const int i0 = i0s[s], nb = nbs[s];
+            if (adjoint) {
+                // adjoint columns (lo ascending) are active while their rows [lo, n) meet the block
+                while (c_end < m && desc[c_end].lo < i0 + nb) {
+                    ++c_end;
+                }
+                while (c_end > 0 && desc[c_end - 1].lo >= i0 + nb) {
+                    --c_end;
+                }
+            }
+            else {
+                while (c_begin > 0 && desc[c_begin - 1].d >= i0) {
+                    --c_begin;
+                }
+            }
+            if (c_begin >= c_end) {
+                continue;
+            }
+            any_active = true;
+            int t_begin = 0;
+            while (tiles[t_begin].x + tiles[t_begin].y <= c_begin) {
+                ++t_begin;
+            }
+            int t_end = t_begin;
+            while (t_end < (int)tiles.size() && tiles[t_end].x < c_end) {
+                ++t_end;
+            }
+            const int ntl = t_end - t_begin;
+            const int N = c_end - c_begin;
+            const int woff = i0 - g0;
+            int pr = prof_begin(st);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_real<MAXB_REAL, SOLVE_CH>),
+                               dim3((ntl + SOLVE_WAVES_H - 1) / SOLVE_WAVES_H), dim3(SOLVE_WAVES_H * WAVE), 0, st, Sw,
+                               ldsw, Pw, ldpw, i0, nb, X, ldx, W, ldw, ws.desc.as<ColDesc>(), ws.tiles.as<int2>(),
+                               t_begin, t_end, c_begin, c_end, ws.sig.as<double>(), ws.colexp.as<int>(),
+                               ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff, wr + woff, wr,
+                               ws.sig_grp.as<double>(), first ? 1 : 0); XTG_LAUNCH_CHECK("k_solve_real");
+            prof_end(pr, st, 0, 2.0 * nb * nb * N, 8.0 * (4.0 * nb * N + 2.0 * nb * nb));
+            first = false;
+            // near update: the group's rows above this block
+            const int r0 = adjoint ? std::max(g0, std::min(desc[c_begin].lo, i0)) : g0;
+            const int M = i0 - r0;
+            if (M > 0) {
+                dim3 grid((N + GEMM_TILE - 1) / GEMM_TILE, (M + GEMM_TILE - 1) / GEMM_TILE);
+                pr = prof_begin(st);
+                hipLaunchKernelGGL(k_gemm_f64<0>, grid, dim3(256), 0, st, M, N, nb, nb, Sw + r0 + (long)i0 * ldsw,
+                                   ldsw, Pw + r0 + (long)i0 * ldpw, ldpw, (const double *)(W + (long)woff * ldw + c_begin),
+                                   ldw, X + (long)r0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
+                                   (const double *)(ws.sig.as<double>() + c_begin), (const int *)nullptr,
+                                   (const double *)(W + (long)(wr + woff) * ldw + c_begin)); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
+                prof_end(pr, st, 1, 2.0 * M * N * (2.0 * nb), 8.0 * (2.0 * M * N + 2.0 * nb * M + 2.0 * nb * N));
             }
         }
-        else {
-            while (c_begin > 0 && desc[c_begin - 1].d >= i0) {
-                --c_begin;
-            }
-        }
-        if (c_begin >= c_end) {
+        if (!any_active) {
             continue;
         }
-        int t_begin = 0;
-        while (tiles[t_begin].x + tiles[t_begin].y <= c_begin) {
-            ++t_begin;
-        }
-        int t_end = t_begin;
-        while (t_end < (int)tiles.size() && tiles[t_end].x < c_end) {
-            ++t_end;
-        }
-        const int ntl = t_end - t_begin;
+        // far update: rows above the group, all of its blocks at once
         const int N = c_end - c_begin;
-        int pr = prof_begin(st);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_real<MAXB_REAL>), dim3((ntl + 3) / 4), dim3(256), 0, st, Sw, ldsw,
-                           Pw, ldpw, i0, nb, X, ldx, W, ldw, ws.desc.as<ColDesc>(), ws.tiles.as<int2>(), t_begin,
-                           t_end, c_begin, c_end, ws.sig.as<double>(), ws.colexp.as<int>(),
-                           ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>()); XTG_LAUNCH_CHECK("k_solve_real<MAXB_REAL>");
-        prof_end(pr, st, 0, 2.0 * nb * nb * N, 8.0 * (4.0 * nb * N + 2.0 * nb * nb));
-        // rows below every active column's range start are never used: skip them
-        const int r0 = adjoint ? std::min(desc[c_begin].lo, i0) : 0;
-        const int M = i0 - r0;
-        if (M > 0) {
+        const int r0 = adjoint ? std::min(desc[c_begin].lo, g0) : 0;
+        const int M = g0 - r0;
+        if (M > 0 && N > 0) {
             dim3 grid((N + GEMM_TILE - 1) / GEMM_TILE, (M + GEMM_TILE - 1) / GEMM_TILE);
-            pr = prof_begin(st);
-            hipLaunchKernelGGL(k_gemm_f64<0>, grid, dim3(256), 0, st, M, N, nb, nb, Sw + r0 + (long)i0 * ldsw, ldsw,
-                               Pw + r0 + (long)i0 * ldpw, ldpw, (const double *)(W + c_begin), ldw,
+            int pr = prof_begin(st);
+            hipLaunchKernelGGL(k_gemm_f64<0>, grid, dim3(256), 0, st, M, N, wr, wr, Sw + r0 + (long)g0 * ldsw, ldsw,
+                               Pw + r0 + (long)g0 * ldpw, ldpw, (const double *)(W + c_begin), ldw,
                                X + (long)r0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
-                               (const double *)(ws.sig.as<double>() + c_begin), (const int *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
-            prof_end(pr, st, 1, 2.0 * M * N * (2.0 * nb), 8.0 * (2.0 * M * N + 2.0 * nb * M + 2.0 * nb * N));
+                               (const double *)(ws.sig_grp.as<double>() + c_begin), (const int *)nullptr,
+                               (const double *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
+            prof_end(pr, st, 1, 2.0 * M * N * (2.0 * wr), 8.0 * (2.0 * M * N + 2.0 * wr * M + 2.0 * wr * N));
         }
     }
     XTG_CHECK(hipGetLastError());
@@ -433,7 +480,7 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
         int pr = prof_begin(st);
         hipLaunchKernelGGL(k_gemm_f64<1>, grid, dim3(256), 0, st, m, n, n, 0, (const double *)X, ldx,
                            (const double *)nullptr, 0L, Rp, ldr, A.V, A.ldv, ws.outcol.as<int>(), 0,
-                           (const double *)nullptr, ws.kmax.as<int>()); XTG_LAUNCH_CHECK("k_gemm_f64<1>");
+                           (const double *)nullptr, ws.kmax.as<int>(), (const double *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<1>");
         prof_end(pr, st, 2, 2.0 * n * ksum, 8.0 * ((double)n * kmx + ksum + (double)n * m));
     }
     else {
@@ -471,18 +518,15 @@ int real_device(char side, char howmny, const int *select, int n, const double *
     }
 
     // global scales from the 1-norms of S and P over the upper Hessenberg part
-    ENSURE(ws.norms, sizeof(double) * 2 * (size_t)n);
+    // column norms of S and P and the subdiagonal of S (block snapping) in one buffer
+    ENSURE(ws.norms, sizeof(double) * 3 * (size_t)n);
     hipLaunchKernelGGL(k_colnorms, dim3(n), dim3(256), 0, st, S, lds, P, ldp, n, 1, ws.norms.as<double>()); XTG_LAUNCH_CHECK("k_colnorms");
-    std::vector<double> nrm(2 * (size_t)n);
-    XTG_CHECK(hipMemcpyAsync(nrm.data(), ws.norms.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
-    // subdiagonal of S (host copy) for the block snapping
-    std::vector<double> sub(n, 0.0);
-    if (n > 1) {
-        XTG_CHECK(hipMemcpy2DAsync(sub.data() + 1, sizeof(double), S + 1, sizeof(double) * (lds + 1),
-                                   sizeof(double), n - 1, hipMemcpyDeviceToHost, st));
-    }
+    hipLaunchKernelGGL(k_subdiag, dim3((n + 255) / 256), dim3(256), 0, st, S, lds, n, ws.norms.as<double>() + 2 * (long)n); XTG_LAUNCH_CHECK("k_subdiag");
+    std::vector<double> nrm(3 * (size_t)n);
+    XTG_CHECK(hipMemcpyAsync(nrm.data(), ws.norms.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
     XTG_CHECK(hipStreamSynchronize(st));
     // sub[i] = S[i, i-1]
+    std::vector<double> sub(nrm.begin() + 2 * (long)n, nrm.end());
     double an = 0.0, bn = 0.0;
     for (int c = 0; c < n; ++c) {
         an = std::max(an, nrm[c]);
@@ -592,15 +636,13 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
     if (m == 0 || nbatch <= 0) {
         return 0;
     }
-    ENSURE(ws.norms, sizeof(double) * 2 * (size_t)n);
+    ENSURE(ws.norms, sizeof(double) * 3 * (size_t)n);
     hipLaunchKernelGGL(k_colnorms, dim3(n), dim3(256), 0, st, S, lds, P, ldp, n, 1, ws.norms.as<double>()); XTG_LAUNCH_CHECK("k_colnorms");
-    std::vector<double> nrm(2 * (size_t)n), sub(n, 0.0);
-    XTG_CHECK(hipMemcpyAsync(nrm.data(), ws.norms.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
-    if (n > 1) {
-        XTG_CHECK(hipMemcpy2DAsync(sub.data() + 1, sizeof(double), S + 1, sizeof(double) * (lds + 1), sizeof(double),
-                                   n - 1, hipMemcpyDeviceToHost, st));
-    }
+    hipLaunchKernelGGL(k_subdiag, dim3((n + 255) / 256), dim3(256), 0, st, S, lds, n, ws.norms.as<double>() + 2 * (long)n); XTG_LAUNCH_CHECK("k_subdiag");
+    std::vector<double> nrm(3 * (size_t)n);
+    XTG_CHECK(hipMemcpyAsync(nrm.data(), ws.norms.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
     XTG_CHECK(hipStreamSynchronize(st));
+    std::vector<double> sub(nrm.begin() + 2 * (long)n, nrm.end());
     double an = 0.0, bn = 0.0;
     for (int c = 0; c < n; ++c) {
         an = std::max(an, nrm[c]);
@@ -672,7 +714,7 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
         pr = prof_begin(st);
         hipLaunchKernelGGL(k_gemm_f64<2>, grid, dim3(256), 0, st, n, m, n, 0, P, ldp, (const double *)nullptr, 0L,
                            (const double *)ws.Vt.as<double>(), ldt, ws.Y.as<double>(), ldt, (const int *)nullptr, 0,
-                           (const double *)nullptr, (const int *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+                           (const double *)nullptr, (const int *)nullptr, (const double *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
         prof_end(pr, st, 3, 2.0 * n * n * (double)m, 8.0 * ((double)n * n + 2.0 * n * m));
     }
     hipLaunchKernelGGL(k_bwd_lambda_t, dim3((M + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, st, (const double *)X, ldx,
@@ -684,10 +726,10 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
         pr = prof_begin(st);
         hipLaunchKernelGGL(k_gemm_f64<2>, grid, dim3(256), 0, st, n, n, m, 0, V, ldv, (const double *)nullptr, 0L,
                            (const double *)(ws.LamS.as<double>() + (long)b * m * n), (long)n, dS + (long)b * n * ldds,
-                           ldds, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+                           ldds, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr, (const double *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
         hipLaunchKernelGGL(k_gemm_f64<2>, grid, dim3(256), 0, st, n, n, m, 0, V, ldv, (const double *)nullptr, 0L,
                            (const double *)(ws.LamP.as<double>() + (long)b * m * n), (long)n, dP + (long)b * n * lddp,
-                           lddp, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+                           lddp, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr, (const double *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
         prof_end(pr, st, 3, 4.0 * n * n * (double)m, 8.0 * 2.0 * ((double)n * m + (double)m * n + (double)n * n));
     }
     hipLaunchKernelGGL(k_bwd_diag, dim3(M), dim3(64), 0, st, (const double *)X, ldx, (const double *)ws.Y.as<double>(),

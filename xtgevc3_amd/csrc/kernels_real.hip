@@ -113,26 +113,50 @@ __device__ __noinline__ Bump2 bump_solve(cpx r1, cpx r2, cpx c11, cpx c12, cpx c
 // right-hand sides, rows of the block hold RHS on entry and the solution on exit.
 // W receives the update operands, row-major: W[k * ldw + col], k < nb: -a * x,
 // k >= nb: (b x) own part, so that X_above := sig * X_above + [S_w | P_w] * W.
+//
+// Each lane owns one column; its block of the RHS lives in LDS (vb[wave][row][lane]) and
+// is swept bottom-up in chunks of CH rows held in registers: the chunk is solved with
+// in-register updates, then the rows above it are updated in a runtime loop (CH
+// broadcast LDS reads of {S,P} and 2*CH FMAs per row).  A lane whose solution would exceed
+// 2^960 scales its whole column by a power of two (rare; recorded as an exponent).
 // ---------------------------------------------------------------------------------------
-template <int MAXB>
-__global__ __launch_bounds__(256) void k_solve_real(const double *__restrict__ S, long lds, const double *__restrict__ P,
-                                                    long ldp, int i0, int nb, double *__restrict__ X, long ldx,
-                                                    double *__restrict__ W, long ldw, const ColDesc *__restrict__ desc,
-                                                    const int2 *__restrict__ tiles, int t_begin, int t_end,
-                                                    int c_begin, int c_end, double *__restrict__ sig,
-                                                    int *__restrict__ colexp,
-                                                    int *__restrict__ blkexp, int *__restrict__ flag)
+constexpr int SOLVE_WAVES = 2;
+
+template <int MAXB, int CH>
+__global__ __launch_bounds__(SOLVE_WAVES *WAVE) void k_solve_real(
+    const double *__restrict__ S, long lds, const double *__restrict__ P, long ldp, int i0, int nb,
+    double *__restrict__ X, long ldx, double *__restrict__ W, long ldw, const ColDesc *__restrict__ desc,
+    const int2 *__restrict__ tiles, int t_begin, int t_end, int c_begin, int c_end, double *__restrict__ sig,
+    int *__restrict__ colexp, int *__restrict__ blkexp, int *__restrict__ flag, int w_s, int w_p, int w_rows,
+    double *__restrict__ sig_grp, int first_in_group)
 {
     __shared__ double2 sp[MAXB * MAXB];
-    for (int q = threadIdx.x; q < nb * nb; q += blockDim.x) {
-        int l = q % nb, r = q / nb;
-        sp[l + r * MAXB] = make_double2(S[(i0 + l) + (long)(i0 + r) * lds], P[(i0 + l) + (long)(i0 + r) * ldp]);
+    __shared__ double vb[SOLVE_WAVES][MAXB * WAVE];
+    // stage {S,P} of the diagonal block (zero outside nb x nb); 8 loads in flight per thread
+    for (int q0 = threadIdx.x; q0 < MAXB * MAXB; q0 += 8 * blockDim.x) {
+        double2 t[8];
+#pragma clang loop unroll(full)
+        for (int j = 0; j < 8; ++j) {
+            int q = q0 + j * blockDim.x;
+            int l = q % MAXB, r = q / MAXB;
+            t[j] = make_double2(0.0, 0.0);
+            if (q < MAXB * MAXB && l < nb && r < nb) {
+                t[j] = make_double2(S[(i0 + l) + (long)(i0 + r) * lds], P[(i0 + l) + (long)(i0 + r) * ldp]);
+            }
+        }
+#pragma clang loop unroll(full)
+        for (int j = 0; j < 8; ++j) {
+            int q = q0 + j * blockDim.x;
+            if (q < MAXB * MAXB) {
+                sp[q] = t[j];
+            }
+        }
     }
     __syncthreads();
 
     const int wave = threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
-    const int t = t_begin + blockIdx.x * (blockDim.x / WAVE) + wave;
+    const int t = t_begin + blockIdx.x * SOLVE_WAVES + wave;
     if (t >= t_end) {
         return;
     }
@@ -149,6 +173,7 @@ __global__ __launch_bounds__(256) void k_solve_real(const double *__restrict__ S
     const double bn1 = fabs(bR) + fabs(bI);
     const int dl = act ? cd.d - i0 : -1 - MAXB; // inactive lanes: every row is "fixed zero"
     const int lol = cd.lo - i0;
+    double *V = &vb[wave][0];
 
     // initial values of the column's own diagonal block rows (complex)
     cpx x0 = {0.0, 0.0}, x1v = {0.0, 0.0};
@@ -167,127 +192,185 @@ __global__ __launch_bounds__(256) void k_solve_real(const double *__restrict__ S
             x1v = {-a * e21.x, 0.0};
         }
     }
-
-    double v[MAXB];
+    for (int r0 = 0; r0 < nb; r0 += 16) {
+        double t[16];
 #pragma clang loop unroll(full)
-    for (int r = 0; r < MAXB; ++r) {
-        v[r] = 0.0;
-        if (r < nb) {
-            if (r < dl && r >= lol) {
-                v[r] = X[(long)(i0 + r) * ldx + p];
+        for (int j = 0; j < 16; ++j) {
+            int r = r0 + j;
+            t[j] = (r < nb && r < dl && r >= lol) ? X[(long)(i0 + r) * ldx + p] : 0.0;
+        }
+#pragma clang loop unroll(full)
+        for (int j = 0; j < 16; ++j) {
+            if (r0 + j < nb) {
+                V[(r0 + j) * WAVE + lane] = t[j];
             }
         }
     }
 
     int my_exp = 0;
-    bool pre = false;      // row r was solved together with row r+1 (2x2 bump)
-    double pre_xp = 0.0;   // partner part of that row's solution
+    int rhi = nb - 1;
+    while (rhi >= 0) {
+        int rlo = rhi - CH + 1 > 0 ? rhi - CH + 1 : 0;
+        if (rlo > 0 && sp[rlo + (rlo - 1) * MAXB].x != 0.0) {
+            ++rlo; // a 2x2 bump (rlo-1, rlo) goes whole into the next chunk
+        }
+        const int cnt = rhi - rlo + 1;
+        double xr[CH], axq[CH], bxq[CH];
 #pragma clang loop unroll(full)
-    for (int r = MAXB - 1; r >= 0; --r) {
-        if (r < nb) {
-            double xs, xp;
-            bool bump_now = false;
-            if (pre) {
-                xs = v[r];
-                xp = pre_xp;
-                pre = false;
-            }
-            else if (r >= 1 && sp[r + (r - 1) * MAXB].x != 0.0) {
-                // 2x2 bump at rows (r-1, r)
-                bump_now = true;
-                double o1 = v[r - 1], o2 = v[r];
-                double q1 = __shfl(o1, plane), q2 = __shfl(o2, plane);
-                cpx r1 = is_im ? cpx{q1, o1} : cpx{o1, is_pair ? q1 : 0.0};
-                cpx r2 = is_im ? cpx{q2, o2} : cpx{o2, is_pair ? q2 : 0.0};
-                double2 e11 = sp[(r - 1) + (r - 1) * MAXB], e12 = sp[(r - 1) + r * MAXB];
-                double2 e21 = sp[r + (r - 1) * MAXB], e22 = sp[r + r * MAXB];
-                cpx c11 = {a * e11.x - bR * e11.y, -bI * e11.y};
-                cpx c12 = {a * e12.x - bR * e12.y, -bI * e12.y};
-                cpx c21 = {a * e21.x - bR * e21.y, -bI * e21.y};
-                cpx c22 = {a * e22.x - bR * e22.y, -bI * e22.y};
-                double dm1 = fmax(kEps * (fabs(a) * (fabs(e11.x) + fabs(e21.x)) + bn1 * fabs(e11.y)), kSafeMin);
-                double dm2 = fmax(kEps * (fabs(a) * (fabs(e12.x) + fabs(e22.x)) + bn1 * fabs(e22.y)), kSafeMin);
-                const bool solve1 = (r - 1) < dl && (r - 1) >= lol;
-                const bool solve2 = r < dl && r >= lol;
-                Bump2 bs = bump_solve(r1, r2, c11, c12, c21, c22, dm1, dm2);
-                int e = (solve1 || solve2) ? bs.e : 0;
+        for (int q = 0; q < CH; ++q) {
+            xr[q] = q < cnt ? V[(rlo + q) * WAVE + lane] : 0.0;
+            axq[q] = 0.0;
+            bxq[q] = 0.0;
+        }
+        bool pre = false;    // row q was solved together with row q+1 (2x2 bump)
+        double pre_xp = 0.0; // partner part of that row's solution
+#pragma clang loop unroll(full)
+        for (int q = CH - 1; q >= 0; --q) {
+            if (q < cnt) {
+                const int r = rlo + q;
+                double xs, xp;
+                bool bump_now = false;
+                int e = 0;
+                Bump2 bs;
+                cpx xsol = {0.0, 0.0};
+                const bool from_pre = pre;
+                xs = 0.0;
+                xp = 0.0;
+                if (pre) {
+                    xs = xr[q];
+                    xp = pre_xp;
+                    pre = false;
+                }
+                else if (q >= 1 && sp[r + (r - 1) * MAXB].x != 0.0) {
+                    // 2x2 bump at rows (r-1, r)
+                    bump_now = true;
+                    double o1 = xr[q - 1], o2 = xr[q];
+                    double q1 = __shfl(o1, plane), q2 = __shfl(o2, plane);
+                    cpx r1 = is_im ? cpx{q1, o1} : cpx{o1, is_pair ? q1 : 0.0};
+                    cpx r2 = is_im ? cpx{q2, o2} : cpx{o2, is_pair ? q2 : 0.0};
+                    double2 e11 = sp[(r - 1) + (r - 1) * MAXB], e12 = sp[(r - 1) + r * MAXB];
+                    double2 e21 = sp[r + (r - 1) * MAXB], e22 = sp[r + r * MAXB];
+                    cpx c11 = {a * e11.x - bR * e11.y, -bI * e11.y};
+                    cpx c12 = {a * e12.x - bR * e12.y, -bI * e12.y};
+                    cpx c21 = {a * e21.x - bR * e21.y, -bI * e21.y};
+                    cpx c22 = {a * e22.x - bR * e22.y, -bI * e22.y};
+                    double dm1 = fmax(kEps * (fabs(a) * (fabs(e11.x) + fabs(e21.x)) + bn1 * fabs(e11.y)), kSafeMin);
+                    double dm2 = fmax(kEps * (fabs(a) * (fabs(e12.x) + fabs(e22.x)) + bn1 * fabs(e22.y)), kSafeMin);
+                    bs = bump_solve(r1, r2, c11, c12, c21, c22, dm1, dm2);
+                    const bool solve1 = (r - 1) < dl && (r - 1) >= lol;
+                    const bool solve2 = r < dl && r >= lol;
+                    e = (solve1 || solve2) ? bs.e : 0;
+                }
+                else {
+                    double o = xr[q];
+                    double qv = __shfl(o, plane);
+                    cpx rhs = is_im ? cpx{qv, o} : cpx{o, is_pair ? qv : 0.0};
+                    double2 ee = sp[r + r * MAXB];
+                    cpx c = {a * ee.x - bR * ee.y, -bI * ee.y};
+                    double dmin = fmax(kEps * (fabs(a) * fabs(ee.x) + bn1 * fabs(ee.y)), kSafeMin);
+                    if (abs1(c) < dmin) {
+                        c = {dmin, 0.0};
+                    }
+                    const bool solve = r < dl && r >= lol;
+                    double rn = abs1(rhs);
+                    if (solve && rn > 0.0 && ilogb(rn) - ilogb(abs1(c)) > kLog2Big - 2) {
+                        e = scale_exponent(rn, abs1(c), kLog2Big);
+                    }
+                    if (e != 0) {
+                        double f = ldexp(1.0, -e);
+                        rhs = {rhs.r * f, rhs.i * f};
+                    }
+                    xsol = solve ? cdiv(rhs, c) : (r == dl ? x0 : (r == dl + 1 ? x1v : cpx{0.0, 0.0}));
+                }
                 if (__any(e != 0)) {
-                    double f = ldexp(1.0, -e);
+                    // rescale the whole column of this lane by 2^-e (rare)
+                    const double f = ldexp(1.0, -e);
 #pragma clang loop unroll(full)
-                    for (int j = 0; j < MAXB; ++j) {
-                        v[j] *= f;
+                    for (int j = 0; j < CH; ++j) {
+                        xr[j] *= f;
+                        axq[j] *= f;
+                        bxq[j] *= f;
+                    }
+                    for (int rr = 0; rr < nb; ++rr) {
+                        if (rr < rlo || rr > rhi) {
+                            V[rr * WAVE + lane] *= f;
+                        }
                     }
                     my_exp -= e;
                 }
-                cpx s1 = solve1 ? bs.x1 : ((r - 1) == dl ? x0 : ((r - 1) == dl + 1 ? x1v : cpx{0.0, 0.0}));
-                cpx s2 = solve2 ? bs.x2 : (r == dl ? x0 : (r == dl + 1 ? x1v : cpx{0.0, 0.0}));
-                v[r - 1] = is_im ? s1.i : s1.r;
-                pre_xp = is_im ? s1.r : s1.i;
-                xs = is_im ? s2.i : s2.r;
-                xp = is_im ? s2.r : s2.i;
-                pre = true;
-            }
-            else {
-                double o = v[r];
-                double q = __shfl(o, plane);
-                cpx rhs = is_im ? cpx{q, o} : cpx{o, is_pair ? q : 0.0};
-                double2 e = sp[r + r * MAXB];
-                cpx c = {a * e.x - bR * e.y, -bI * e.y};
-                double dmin = fmax(kEps * (fabs(a) * fabs(e.x) + bn1 * fabs(e.y)), kSafeMin);
-                if (abs1(c) < dmin) {
-                    c = {dmin, 0.0};
+                if (bump_now) {
+                    const bool solve1 = (r - 1) < dl && (r - 1) >= lol;
+                    const bool solve2 = r < dl && r >= lol;
+                    cpx s1 = solve1 ? bs.x1 : ((r - 1) == dl ? x0 : ((r - 1) == dl + 1 ? x1v : cpx{0.0, 0.0}));
+                    cpx s2 = solve2 ? bs.x2 : (r == dl ? x0 : (r == dl + 1 ? x1v : cpx{0.0, 0.0}));
+                    xr[q - 1] = is_im ? s1.i : s1.r;
+                    pre_xp = is_im ? s1.r : s1.i;
+                    xs = is_im ? s2.i : s2.r;
+                    xp = is_im ? s2.r : s2.i;
+                    pre = true;
                 }
-                const bool solve = r < dl && r >= lol;
-                double rn = abs1(rhs);
-                int ex = 0;
-                if (solve && rn > 0.0 && ilogb(rn) - ilogb(abs1(c)) > kLog2Big - 2) {
-                    ex = scale_exponent(rn, abs1(c), kLog2Big);
+                else if (!from_pre) {
+                    xs = is_im ? xsol.i : xsol.r;
+                    xp = is_im ? xsol.r : xsol.i;
                 }
-                if (__any(ex != 0)) {
-                    double f = ldexp(1.0, -ex);
+                xr[q] = xs;
+                axq[q] = a * xs;
+                bxq[q] = bR * xs + bIs * xp;
 #pragma clang loop unroll(full)
-                    for (int j = 0; j < MAXB; ++j) {
-                        v[j] *= f;
+                for (int l = 0; l < q; ++l) {
+                    if (l == q - 1 && bump_now) {
+                        continue;
                     }
-                    rhs = {rhs.r * f, rhs.i * f};
-                    my_exp -= ex;
+                    double2 el = sp[(rlo + l) + r * MAXB];
+                    xr[l] = fma(-el.x, axq[q], xr[l]);
+                    xr[l] = fma(el.y, bxq[q], xr[l]);
                 }
-                cpx x = solve ? cdiv(rhs, c) : (r == dl ? x0 : (r == dl + 1 ? x1v : cpx{0.0, 0.0}));
-                xs = is_im ? x.i : x.r;
-                xp = is_im ? x.r : x.i;
-            }
-            v[r] = xs;
-            const double ax = a * xs;
-            const double bx = bR * xs + bIs * xp;
-#pragma clang loop unroll(full)
-            for (int l = 0; l < r; ++l) {
-                if (l == r - 1 && bump_now) {
-                    continue;
-                }
-                double2 e = sp[l + r * MAXB];
-                v[l] = fma(-e.x, ax, v[l]);
-                v[l] = fma(e.y, bx, v[l]);
             }
         }
+#pragma clang loop unroll(full)
+        for (int q = 0; q < CH; ++q) {
+            if (q < cnt) {
+                V[(rlo + q) * WAVE + lane] = xr[q];
+            }
+        }
+        // rows above the chunk: v_l -= sum_q (S[l][rlo+q] ax_q - P[l][rlo+q] bx_q)
+        for (int l = 0; l < rlo; ++l) {
+            double acc = V[l * WAVE + lane];
+#pragma clang loop unroll(full)
+            for (int q = 0; q < CH; ++q) {
+                double2 el = sp[l + (rlo + q) * MAXB];
+                acc = fma(-el.x, axq[q], acc);
+                acc = fma(el.y, bxq[q], acc);
+            }
+            V[l * WAVE + lane] = acc;
+        }
+        rhi = rlo - 1;
     }
 
-    if (!act) {
-        // keep every lane in the shuffles below
-    }
-#pragma clang loop unroll(full)
-    for (int r = 0; r < MAXB; ++r) {
-        if (r < nb) {
-            double o = v[r];
-            double q = __shfl(o, plane);
-            if (act) {
-                X[(long)(i0 + r) * ldx + p] = o;
-                W[(long)r * ldw + p] = -a * o;
-                W[(long)(nb + r) * ldw + p] = bR * o + bIs * (is_pair ? q : 0.0);
-            }
+    // W of this block at rows w_s + r (S part) and w_p + r (P part) of the group's operand
+    for (int r = 0; r < nb; ++r) {
+        double o = V[r * WAVE + lane];
+        double q = __shfl(o, plane);
+        if (act) {
+            X[(long)(i0 + r) * ldx + p] = o;
+            W[(long)(w_s + r) * ldw + p] = -a * o;
+            W[(long)(w_p + r) * ldw + p] = bR * o + bIs * (is_pair ? q : 0.0);
         }
     }
     if (act) {
-        sig[p] = ldexp(1.0, my_exp);
+        const double sg = ldexp(1.0, my_exp);
+        sig[p] = sg;
+        // cumulative scale over the group's steps (the group's far update applies it);
+        // a column starting in this block has no earlier steps in the group
+        const bool starts_here = dl >= 0 && dl < nb;
+        sig_grp[p] = (first_in_group || starts_here) ? sg : sig_grp[p] * sg;
+        if (my_exp != 0) {
+            // operands written by the group's earlier (lower) blocks carry the old scale
+            for (int k = w_s + nb; k < w_rows; ++k) {
+                W[(long)k * ldw + p] *= sg;
+                W[(long)(w_rows + k) * ldw + p] *= sg;
+            }
+        }
         int ce = colexp[p] + my_exp;
         colexp[p] = ce;
         blkexp[p] = ce;
@@ -317,12 +400,19 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
                                                   const double *__restrict__ L2, long ldl2,
                                                   const double *__restrict__ R, long ldr, double *__restrict__ C,
                                                   long ldc, const int *__restrict__ outcol, int mode,
-                                                  const double *__restrict__ sig, const int *__restrict__ kmax)
+                                                  const double *__restrict__ sig, const int *__restrict__ kmax,
+                                                  const double *__restrict__ R2)
 {
+    // K rows [0, K1) of R at R, rows [K1, K1+K2) at R2 (R + K1*ldr when R2 is null)
+    if (R2 == nullptr) {
+        R2 = R + (long)K1 * ldr;
+    }
     __shared__ double Ls[2][GK][GT];
     __shared__ double Rs[2][GK][GT];
 
-    const int a0 = blockIdx.y * GT;
+    // with a per-row K bound (triangular operand) the heaviest tiles are dispatched first
+    const int ya = kmax != nullptr ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;
+    const int a0 = ya * GT;
     const int b0 = blockIdx.x * GT;
     const int tid = threadIdx.x;
     const int wave = tid / WAVE;
@@ -332,12 +422,16 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
 
     int K = K1 + K2;
     if (kmax != nullptr) {
-        int amax = min(a0 + GT, M) - 1;
-        int kk = 0;
-        for (int q = a0; q <= amax; ++q) {
-            kk = max(kk, kmax[q]);
+        __shared__ int kred;
+        if (tid == 0) {
+            kred = 0;
         }
-        K = min(K, kk);
+        __syncthreads();
+        if (tid < GT && a0 + tid < M) {
+            atomicMax(&kred, kmax[a0 + tid]);
+        }
+        __syncthreads();
+        K = min(K, kred);
     }
 
     double4_t acc[4][4];
@@ -366,7 +460,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
                 }
                 int bb = b0 + lc;
                 if (bb < N) {
-                    rv = R[(long)k * ldr + bb];
+                    rv = (k < K1) ? R[(long)k * ldr + bb] : R2[(long)(k - K1) * ldr + bb];
                 }
             }
             lreg[s] = lv;
@@ -586,6 +680,16 @@ __global__ void k_colnorms(const double *__restrict__ S, long lds, const double 
     }
 }
 
+// out[i] = S[i, i-1] (out[0] = 0): the subdiagonal, gathered so one contiguous copy brings it
+// to the host for the block partition.
+__global__ void k_subdiag(const double *__restrict__ S, long lds, int n, double *__restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        out[i] = i > 0 ? S[i + (long)(i - 1) * lds] : 0.0;
+    }
+}
+
 // Fill a row-major block with zeros (used instead of hipMemsetAsync so the sequence can be
 // captured in a HIP graph).
 __global__ void k_zero(double *__restrict__ X, long count)
@@ -611,12 +715,14 @@ __global__ void k_zero_int(int *__restrict__ X, long count)
 namespace xtg {
 // TAG only separates the two uses in profiles: 0 = RHS update, 1 = back-transform.
 template __global__ void k_gemm_f64<0>(int, int, int, int, const double *, long, const double *, long, const double *,
-                                       long, double *, long, const int *, int, const double *, const int *);
+                                       long, double *, long, const int *, int, const double *, const int *,
+                                       const double *);
 template __global__ void k_gemm_f64<1>(int, int, int, int, const double *, long, const double *, long, const double *,
-                                       long, double *, long, const int *, int, const double *, const int *);
-template __global__ void k_solve_real<65>(const double *, long, const double *, long, int, int, double *, long,
-                                          double *, long, const ColDesc *, const int2 *, int, int, int, int, double *,
-                                          int *, int *, int *);
+                                       long, double *, long, const int *, int, const double *, const int *,
+                                       const double *);
+template __global__ void k_solve_real<65, 8>(const double *, long, const double *, long, int, int, double *, long,
+                                             double *, long, const ColDesc *, const int2 *, int, int, int, int,
+                                             double *, int *, int *, int *, int, int, int, double *, int);
 } // namespace xtg
 
 // =======================================================================================
@@ -769,5 +875,6 @@ __global__ __launch_bounds__(64) void k_bwd_diag(const double *__restrict__ X, l
 }
 
 template __global__ void k_gemm_f64<2>(int, int, int, int, const double *, long, const double *, long, const double *,
-                                       long, double *, long, const int *, int, const double *, const int *);
+                                       long, double *, long, const int *, int, const double *, const int *,
+                                       const double *);
 } // namespace xtg
