@@ -27,12 +27,12 @@
 
 namespace xtg {
 
-template <int MAXB, int CH>
+template <int MAXB>
 __global__ void k_solve_real(const double *, long, const double *, long, int, int, double *, long, double *, long,
                              const ColDesc *, const int2 *, int, int, int, int, double *, int *, int *, int *, int,
                              int, int, double *, int);
-constexpr int SOLVE_CH = 8;    // register chunk of the solve kernel
-constexpr int SOLVE_WAVES_H = 2; // waves per solve workgroup
+constexpr int SOLVE_COLS = 16;  // columns per wave of the solve kernel (4 lanes per column)
+constexpr int SOLVE_WAVES_H = 4; // maximum waves per solve workgroup
 template <int TAG>
 __global__ void k_gemm_f64(int, int, int, int, const double *, long, const double *, long, const double *, long,
                            double *, long, const int *, int, const double *, const int *, const double *);
@@ -174,15 +174,15 @@ static void partition(int n, const std::vector<double> &subw, std::vector<int> &
     }
 }
 
-static void make_tiles(const std::vector<ColDesc> &desc, std::vector<int2> &tiles)
+static void make_tiles(const std::vector<ColDesc> &desc, std::vector<int2> &tiles, int width)
 {
     tiles.clear();
     int m = (int)desc.size();
     int s = 0;
     while (s < m) {
-        int cnt = std::min(WAVE, m - s);
-        if (cnt == WAVE && (desc[s + WAVE - 1].flags & 2) && desc[s + WAVE - 1].partner == s + WAVE) {
-            cnt = WAVE - 1; // never split a pair across waves
+        int cnt = std::min(width, m - s);
+        if (cnt == width && (desc[s + width - 1].flags & 2) && desc[s + width - 1].partner == s + width) {
+            cnt = width - 1; // never split a pair across waves
         }
         tiles.push_back(make_int2(s, cnt));
         s += cnt;
@@ -271,7 +271,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
 {
     const int m = (int)desc.size();
     std::vector<int2> tiles;
-    make_tiles(desc, tiles);
+    make_tiles(desc, tiles, SOLVE_COLS);
     std::vector<int> i0s, nbs;
     partition(n, subw, i0s, nbs);
     const int nsteps = (int)i0s.size();
@@ -346,9 +346,11 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
             const int ntl = t_end - t_begin;
             const int N = c_end - c_begin;
             const int woff = i0 - g0;
+            // waves per workgroup: enough workgroups to cover the CUs, each staging the block once
+            const int wpb = std::max(1, std::min(SOLVE_WAVES_H, ntl / 256));
             int pr = prof_begin(st);
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_real<MAXB_REAL, SOLVE_CH>),
-                               dim3((ntl + SOLVE_WAVES_H - 1) / SOLVE_WAVES_H), dim3(SOLVE_WAVES_H * WAVE), 0, st, Sw,
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_real<MAXB_REAL>),
+                               dim3((ntl + wpb - 1) / wpb), dim3(wpb * WAVE), 0, st, Sw,
                                ldsw, Pw, ldpw, i0, nb, X, ldx, W, ldw, ws.desc.as<ColDesc>(), ws.tiles.as<int2>(),
                                t_begin, t_end, c_begin, c_end, ws.sig.as<double>(), ws.colexp.as<int>(),
                                ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff, wr + woff, wr,

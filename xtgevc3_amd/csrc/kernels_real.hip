@@ -111,44 +111,68 @@ __device__ __noinline__ Bump2 bump_solve(cpx r1, cpx r2, cpx c11, cpx c12, cpx c
 // X is the working matrix in row-major layout X[row * ldx + col] (rows = vector components,
 // columns = packed eigenvector columns).  Rows above the current block hold the running
 // right-hand sides, rows of the block hold RHS on entry and the solution on exit.
-// W receives the update operands, row-major: W[k * ldw + col], k < nb: -a * x,
-// k >= nb: (b x) own part, so that X_above := sig * X_above + [S_w | P_w] * W.
+// W receives the update operands of the block's group, row-major W[k * ldw + col]:
+// rows w_s + r hold -a * x_r, rows w_p + r hold (b x)_r (own part), so that
+// X_above := sig * X_above + [S_w | P_w] * W.
 //
-// Each lane owns one column; its block of the RHS lives in LDS (vb[wave][row][lane]) and
-// is swept bottom-up in chunks of CH rows held in registers: the chunk is solved with
-// in-register updates, then the rows above it are updated in a runtime loop (CH
-// broadcast LDS reads of {S,P} and 2*CH FMAs per row).  A lane whose solution would exceed
-// 2^960 scales its whole column by a power of two (rare; recorded as an exponent).
+// Four lanes (a quad) serve one column: lane j of the quad keeps rows r = 4 i + j of the
+// block in registers.  Rows are solved bottom-up; the owner of row r broadcasts its value
+// to the quad (DPP quad_perm), every lane of the quad computes x_r, and each lane applies
+// x_r to its own rows above r.  {S,P} of the diagonal block is staged in LDS.  A column
+// whose solution would exceed 2^960 is scaled by a power of two (recorded as exponent).
 // ---------------------------------------------------------------------------------------
-constexpr int SOLVE_WAVES = 2;
+constexpr int QL = 4;            // lanes per column
+constexpr int QCOLS = WAVE / QL; // columns per wave
 
-template <int MAXB, int CH>
-__global__ __launch_bounds__(SOLVE_WAVES *WAVE) void k_solve_real(
+template <int SRC> __device__ __forceinline__ double quad_bcast_c(double v)
+{
+    constexpr int ctrl = SRC * 0x55; // quad_perm [SRC, SRC, SRC, SRC]
+    int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
+    int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double quad_bcast(double v, int src)
+{
+    switch (src) {
+    case 0:
+        return quad_bcast_c<0>(v);
+    case 1:
+        return quad_bcast_c<1>(v);
+    case 2:
+        return quad_bcast_c<2>(v);
+    default:
+        return quad_bcast_c<3>(v);
+    }
+}
+
+template <int MAXB>
+__global__ __launch_bounds__(256) void k_solve_real(
     const double *__restrict__ S, long lds, const double *__restrict__ P, long ldp, int i0, int nb,
     double *__restrict__ X, long ldx, double *__restrict__ W, long ldw, const ColDesc *__restrict__ desc,
     const int2 *__restrict__ tiles, int t_begin, int t_end, int c_begin, int c_end, double *__restrict__ sig,
     int *__restrict__ colexp, int *__restrict__ blkexp, int *__restrict__ flag, int w_s, int w_p, int w_rows,
     double *__restrict__ sig_grp, int first_in_group)
 {
+    constexpr int QR = (MAXB + QL - 1) / QL; // rows per lane
     __shared__ double2 sp[MAXB * MAXB];
-    __shared__ double vb[SOLVE_WAVES][MAXB * WAVE];
     // stage {S,P} of the diagonal block (zero outside nb x nb); 8 loads in flight per thread
     for (int q0 = threadIdx.x; q0 < MAXB * MAXB; q0 += 8 * blockDim.x) {
         double2 t[8];
 #pragma clang loop unroll(full)
-        for (int j = 0; j < 8; ++j) {
-            int q = q0 + j * blockDim.x;
+        for (int jj = 0; jj < 8; ++jj) {
+            int q = q0 + jj * blockDim.x;
             int l = q % MAXB, r = q / MAXB;
-            t[j] = make_double2(0.0, 0.0);
+            t[jj] = make_double2(0.0, 0.0);
             if (q < MAXB * MAXB && l < nb && r < nb) {
-                t[j] = make_double2(S[(i0 + l) + (long)(i0 + r) * lds], P[(i0 + l) + (long)(i0 + r) * ldp]);
+                t[jj] = make_double2(S[(i0 + l) + (long)(i0 + r) * lds], P[(i0 + l) + (long)(i0 + r) * ldp]);
             }
         }
 #pragma clang loop unroll(full)
-        for (int j = 0; j < 8; ++j) {
-            int q = q0 + j * blockDim.x;
+        for (int jj = 0; jj < 8; ++jj) {
+            int q = q0 + jj * blockDim.x;
             if (q < MAXB * MAXB) {
-                sp[q] = t[j];
+                sp[q] = t[jj];
             }
         }
     }
@@ -156,24 +180,26 @@ __global__ __launch_bounds__(SOLVE_WAVES *WAVE) void k_solve_real(
 
     const int wave = threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
-    const int t = t_begin + blockIdx.x * SOLVE_WAVES + wave;
+    const int j = lane & (QL - 1);
+    const int cq = lane / QL;
+    const int t = t_begin + blockIdx.x * (blockDim.x / WAVE) + wave;
     if (t >= t_end) {
         return;
     }
     const int2 tl = tiles[t];
-    const int p = tl.x + lane;
-    const bool act = lane < tl.y && p >= c_begin && p < c_end;
+    const int p = tl.x + cq;
+    const bool act = cq < tl.y && p >= c_begin && p < c_end;
     const ColDesc cd = desc[act ? p : tl.x];
     const bool is_im = cd.flags & 1;
-    const bool is_pair = (cd.flags & 2) != 0;
+    const bool is_pair = act && (cd.flags & 2) != 0;
     const int init = (cd.flags >> 2) & 3;
-    const int plane = act ? (cd.partner - tl.x) : lane;
+    const int plane = act ? lane + QL * (cd.partner - p) : lane;
+    const bool wave_pairs = __any(is_pair);
     const double a = cd.a, bR = cd.bR, bI = cd.bI;
     const double bIs = is_im ? bI : -bI;
     const double bn1 = fabs(bR) + fabs(bI);
-    const int dl = act ? cd.d - i0 : -1 - MAXB; // inactive lanes: every row is "fixed zero"
+    const int dl = act ? cd.d - i0 : -1 - MAXB; // inactive columns: every row is "fixed zero"
     const int lol = cd.lo - i0;
-    double *V = &vb[wave][0];
 
     // initial values of the column's own diagonal block rows (complex)
     cpx x0 = {0.0, 0.0}, x1v = {0.0, 0.0};
@@ -192,172 +218,135 @@ __global__ __launch_bounds__(SOLVE_WAVES *WAVE) void k_solve_real(
             x1v = {-a * e21.x, 0.0};
         }
     }
-    for (int r0 = 0; r0 < nb; r0 += 16) {
-        double t[16];
+
+    double vr[QR];
 #pragma clang loop unroll(full)
-        for (int j = 0; j < 16; ++j) {
-            int r = r0 + j;
-            t[j] = (r < nb && r < dl && r >= lol) ? X[(long)(i0 + r) * ldx + p] : 0.0;
-        }
-#pragma clang loop unroll(full)
-        for (int j = 0; j < 16; ++j) {
-            if (r0 + j < nb) {
-                V[(r0 + j) * WAVE + lane] = t[j];
-            }
-        }
+    for (int i = 0; i < QR; ++i) {
+        const int r = QL * i + j;
+        vr[i] = (r < nb && r < dl && r >= lol) ? X[(long)(i0 + r) * ldx + p] : 0.0;
     }
 
     int my_exp = 0;
-    int rhi = nb - 1;
-    while (rhi >= 0) {
-        int rlo = rhi - CH + 1 > 0 ? rhi - CH + 1 : 0;
-        if (rlo > 0 && sp[rlo + (rlo - 1) * MAXB].x != 0.0) {
-            ++rlo; // a 2x2 bump (rlo-1, rlo) goes whole into the next chunk
-        }
-        const int cnt = rhi - rlo + 1;
-        double xr[CH], axq[CH], bxq[CH];
+    bool pre = false;    // row r was solved together with row r+1 (2x2 bump)
+    double pre_xp = 0.0; // partner part of that row's solution
 #pragma clang loop unroll(full)
-        for (int q = 0; q < CH; ++q) {
-            xr[q] = q < cnt ? V[(rlo + q) * WAVE + lane] : 0.0;
-            axq[q] = 0.0;
-            bxq[q] = 0.0;
-        }
-        bool pre = false;    // row q was solved together with row q+1 (2x2 bump)
-        double pre_xp = 0.0; // partner part of that row's solution
+    for (int r = MAXB - 1; r >= 0; --r) {
+        if (r < nb) {
+            const int jr = r & (QL - 1), ir = r / QL;
+            double xs = 0.0, xp = 0.0;
+            bool bump_now = false;
+            int e = 0;
+            Bump2 bs;
+            cpx xsol = {0.0, 0.0};
+            const bool from_pre = pre;
+            if (pre) {
+                xs = quad_bcast(vr[ir], jr);
+                xp = pre_xp;
+                pre = false;
+            }
+            else if (r >= 1 && sp[r + (r - 1) * MAXB].x != 0.0) {
+                // 2x2 bump at rows (r-1, r)
+                bump_now = true;
+                const int jr1 = (r - 1) & (QL - 1), ir1 = (r - 1) / QL;
+                double o1 = quad_bcast(vr[ir1], jr1), o2 = quad_bcast(vr[ir], jr);
+                double q1 = wave_pairs ? __shfl(o1, plane) : 0.0;
+                double q2 = wave_pairs ? __shfl(o2, plane) : 0.0;
+                cpx r1 = is_im ? cpx{q1, o1} : cpx{o1, is_pair ? q1 : 0.0};
+                cpx r2 = is_im ? cpx{q2, o2} : cpx{o2, is_pair ? q2 : 0.0};
+                double2 e11 = sp[(r - 1) + (r - 1) * MAXB], e12 = sp[(r - 1) + r * MAXB];
+                double2 e21 = sp[r + (r - 1) * MAXB], e22 = sp[r + r * MAXB];
+                cpx c11 = {a * e11.x - bR * e11.y, -bI * e11.y};
+                cpx c12 = {a * e12.x - bR * e12.y, -bI * e12.y};
+                cpx c21 = {a * e21.x - bR * e21.y, -bI * e21.y};
+                cpx c22 = {a * e22.x - bR * e22.y, -bI * e22.y};
+                double dm1 = fmax(kEps * (fabs(a) * (fabs(e11.x) + fabs(e21.x)) + bn1 * fabs(e11.y)), kSafeMin);
+                double dm2 = fmax(kEps * (fabs(a) * (fabs(e12.x) + fabs(e22.x)) + bn1 * fabs(e22.y)), kSafeMin);
+                bs = bump_solve(r1, r2, c11, c12, c21, c22, dm1, dm2);
+                const bool solve1 = (r - 1) < dl && (r - 1) >= lol;
+                const bool solve2 = r < dl && r >= lol;
+                e = (solve1 || solve2) ? bs.e : 0;
+            }
+            else {
+                double o = quad_bcast(vr[ir], jr);
+                double qv = wave_pairs ? __shfl(o, plane) : 0.0;
+                cpx rhs = is_im ? cpx{qv, o} : cpx{o, is_pair ? qv : 0.0};
+                double2 ee = sp[r + r * MAXB];
+                cpx c = {a * ee.x - bR * ee.y, -bI * ee.y};
+                double dmin = fmax(kEps * (fabs(a) * fabs(ee.x) + bn1 * fabs(ee.y)), kSafeMin);
+                if (abs1(c) < dmin) {
+                    c = {dmin, 0.0};
+                }
+                const bool solve = r < dl && r >= lol;
+                // |x| <= |rhs|_1 / |c|_1 * 2 must stay below 2^960
+                if (solve && abs1(rhs) > 0x1p958 * abs1(c)) {
+                    e = scale_exponent(abs1(rhs), abs1(c), kLog2Big);
+                }
+                if (e != 0) {
+                    double f = ldexp(1.0, -e);
+                    rhs = {rhs.r * f, rhs.i * f};
+                }
+                xsol = solve ? cdiv(rhs, c) : (r == dl ? x0 : (r == dl + 1 ? x1v : cpx{0.0, 0.0}));
+            }
+            if (__any(e != 0)) {
+                // rescale the whole column by 2^-e (rare)
+                const double f = ldexp(1.0, -e);
 #pragma clang loop unroll(full)
-        for (int q = CH - 1; q >= 0; --q) {
-            if (q < cnt) {
-                const int r = rlo + q;
-                double xs, xp;
-                bool bump_now = false;
-                int e = 0;
-                Bump2 bs;
-                cpx xsol = {0.0, 0.0};
-                const bool from_pre = pre;
-                xs = 0.0;
-                xp = 0.0;
-                if (pre) {
-                    xs = xr[q];
-                    xp = pre_xp;
-                    pre = false;
+                for (int i = 0; i < QR; ++i) {
+                    vr[i] *= f;
                 }
-                else if (q >= 1 && sp[r + (r - 1) * MAXB].x != 0.0) {
-                    // 2x2 bump at rows (r-1, r)
-                    bump_now = true;
-                    double o1 = xr[q - 1], o2 = xr[q];
-                    double q1 = __shfl(o1, plane), q2 = __shfl(o2, plane);
-                    cpx r1 = is_im ? cpx{q1, o1} : cpx{o1, is_pair ? q1 : 0.0};
-                    cpx r2 = is_im ? cpx{q2, o2} : cpx{o2, is_pair ? q2 : 0.0};
-                    double2 e11 = sp[(r - 1) + (r - 1) * MAXB], e12 = sp[(r - 1) + r * MAXB];
-                    double2 e21 = sp[r + (r - 1) * MAXB], e22 = sp[r + r * MAXB];
-                    cpx c11 = {a * e11.x - bR * e11.y, -bI * e11.y};
-                    cpx c12 = {a * e12.x - bR * e12.y, -bI * e12.y};
-                    cpx c21 = {a * e21.x - bR * e21.y, -bI * e21.y};
-                    cpx c22 = {a * e22.x - bR * e22.y, -bI * e22.y};
-                    double dm1 = fmax(kEps * (fabs(a) * (fabs(e11.x) + fabs(e21.x)) + bn1 * fabs(e11.y)), kSafeMin);
-                    double dm2 = fmax(kEps * (fabs(a) * (fabs(e12.x) + fabs(e22.x)) + bn1 * fabs(e22.y)), kSafeMin);
-                    bs = bump_solve(r1, r2, c11, c12, c21, c22, dm1, dm2);
-                    const bool solve1 = (r - 1) < dl && (r - 1) >= lol;
-                    const bool solve2 = r < dl && r >= lol;
-                    e = (solve1 || solve2) ? bs.e : 0;
+                my_exp -= e;
+            }
+            if (bump_now) {
+                const bool solve1 = (r - 1) < dl && (r - 1) >= lol;
+                const bool solve2 = r < dl && r >= lol;
+                cpx s1 = solve1 ? bs.x1 : ((r - 1) == dl ? x0 : ((r - 1) == dl + 1 ? x1v : cpx{0.0, 0.0}));
+                cpx s2 = solve2 ? bs.x2 : (r == dl ? x0 : (r == dl + 1 ? x1v : cpx{0.0, 0.0}));
+                const int jr1 = (r - 1) & (QL - 1), ir1 = (r - 1) / QL;
+                if (j == jr1) {
+                    vr[ir1] = is_im ? s1.i : s1.r;
                 }
-                else {
-                    double o = xr[q];
-                    double qv = __shfl(o, plane);
-                    cpx rhs = is_im ? cpx{qv, o} : cpx{o, is_pair ? qv : 0.0};
-                    double2 ee = sp[r + r * MAXB];
-                    cpx c = {a * ee.x - bR * ee.y, -bI * ee.y};
-                    double dmin = fmax(kEps * (fabs(a) * fabs(ee.x) + bn1 * fabs(ee.y)), kSafeMin);
-                    if (abs1(c) < dmin) {
-                        c = {dmin, 0.0};
-                    }
-                    const bool solve = r < dl && r >= lol;
-                    double rn = abs1(rhs);
-                    if (solve && rn > 0.0 && ilogb(rn) - ilogb(abs1(c)) > kLog2Big - 2) {
-                        e = scale_exponent(rn, abs1(c), kLog2Big);
-                    }
-                    if (e != 0) {
-                        double f = ldexp(1.0, -e);
-                        rhs = {rhs.r * f, rhs.i * f};
-                    }
-                    xsol = solve ? cdiv(rhs, c) : (r == dl ? x0 : (r == dl + 1 ? x1v : cpx{0.0, 0.0}));
-                }
-                if (__any(e != 0)) {
-                    // rescale the whole column of this lane by 2^-e (rare)
-                    const double f = ldexp(1.0, -e);
+                pre_xp = is_im ? s1.r : s1.i;
+                xs = is_im ? s2.i : s2.r;
+                xp = is_im ? s2.r : s2.i;
+                pre = true;
+            }
+            else if (!from_pre) {
+                xs = is_im ? xsol.i : xsol.r;
+                xp = is_im ? xsol.r : xsol.i;
+            }
+            if (j == jr) {
+                vr[ir] = xs;
+            }
+            const double ax = a * xs;
+            const double bx = bR * xs + bIs * xp;
+            // this lane's rows l = 4 i + j above r (row r-1 excluded after a bump)
 #pragma clang loop unroll(full)
-                    for (int j = 0; j < CH; ++j) {
-                        xr[j] *= f;
-                        axq[j] *= f;
-                        bxq[j] *= f;
-                    }
-                    for (int rr = 0; rr < nb; ++rr) {
-                        if (rr < rlo || rr > rhi) {
-                            V[rr * WAVE + lane] *= f;
-                        }
-                    }
-                    my_exp -= e;
-                }
-                if (bump_now) {
-                    const bool solve1 = (r - 1) < dl && (r - 1) >= lol;
-                    const bool solve2 = r < dl && r >= lol;
-                    cpx s1 = solve1 ? bs.x1 : ((r - 1) == dl ? x0 : ((r - 1) == dl + 1 ? x1v : cpx{0.0, 0.0}));
-                    cpx s2 = solve2 ? bs.x2 : (r == dl ? x0 : (r == dl + 1 ? x1v : cpx{0.0, 0.0}));
-                    xr[q - 1] = is_im ? s1.i : s1.r;
-                    pre_xp = is_im ? s1.r : s1.i;
-                    xs = is_im ? s2.i : s2.r;
-                    xp = is_im ? s2.r : s2.i;
-                    pre = true;
-                }
-                else if (!from_pre) {
-                    xs = is_im ? xsol.i : xsol.r;
-                    xp = is_im ? xsol.r : xsol.i;
-                }
-                xr[q] = xs;
-                axq[q] = a * xs;
-                bxq[q] = bR * xs + bIs * xp;
-#pragma clang loop unroll(full)
-                for (int l = 0; l < q; ++l) {
-                    if (l == q - 1 && bump_now) {
-                        continue;
-                    }
-                    double2 el = sp[(rlo + l) + r * MAXB];
-                    xr[l] = fma(-el.x, axq[q], xr[l]);
-                    xr[l] = fma(el.y, bxq[q], xr[l]);
+            for (int i = 0; i < QR; ++i) {
+                if (QL * i < r) {
+                    const int l = QL * i + j;
+                    const bool upd = l < r && !(bump_now && l == r - 1);
+                    double2 el = sp[l + r * MAXB];
+                    double nv = fma(-el.x, ax, vr[i]);
+                    nv = fma(el.y, bx, nv);
+                    vr[i] = upd ? nv : vr[i];
                 }
             }
         }
-#pragma clang loop unroll(full)
-        for (int q = 0; q < CH; ++q) {
-            if (q < cnt) {
-                V[(rlo + q) * WAVE + lane] = xr[q];
-            }
-        }
-        // rows above the chunk: v_l -= sum_q (S[l][rlo+q] ax_q - P[l][rlo+q] bx_q)
-        for (int l = 0; l < rlo; ++l) {
-            double acc = V[l * WAVE + lane];
-#pragma clang loop unroll(full)
-            for (int q = 0; q < CH; ++q) {
-                double2 el = sp[l + (rlo + q) * MAXB];
-                acc = fma(-el.x, axq[q], acc);
-                acc = fma(el.y, bxq[q], acc);
-            }
-            V[l * WAVE + lane] = acc;
-        }
-        rhi = rlo - 1;
     }
 
     // W of this block at rows w_s + r (S part) and w_p + r (P part) of the group's operand
-    for (int r = 0; r < nb; ++r) {
-        double o = V[r * WAVE + lane];
-        double q = __shfl(o, plane);
-        if (act) {
+#pragma clang loop unroll(full)
+    for (int i = 0; i < QR; ++i) {
+        const int r = QL * i + j;
+        const double o = vr[i];
+        const double q = wave_pairs ? __shfl(o, plane) : 0.0;
+        if (act && r < nb) {
             X[(long)(i0 + r) * ldx + p] = o;
             W[(long)(w_s + r) * ldw + p] = -a * o;
             W[(long)(w_p + r) * ldw + p] = bR * o + bIs * (is_pair ? q : 0.0);
         }
     }
-    if (act) {
+    if (act && j == 0) {
         const double sg = ldexp(1.0, my_exp);
         sig[p] = sg;
         // cumulative scale over the group's steps (the group's far update applies it);
@@ -720,9 +709,9 @@ template __global__ void k_gemm_f64<0>(int, int, int, int, const double *, long,
 template __global__ void k_gemm_f64<1>(int, int, int, int, const double *, long, const double *, long, const double *,
                                        long, double *, long, const int *, int, const double *, const int *,
                                        const double *);
-template __global__ void k_solve_real<65, 8>(const double *, long, const double *, long, int, int, double *, long,
-                                             double *, long, const ColDesc *, const int2 *, int, int, int, int,
-                                             double *, int *, int *, int *, int, int, int, double *, int);
+template __global__ void k_solve_real<65>(const double *, long, const double *, long, int, int, double *, long,
+                                          double *, long, const ColDesc *, const int2 *, int, int, int, int,
+                                          double *, int *, int *, int *, int, int, int, double *, int);
 } // namespace xtg
 
 // =======================================================================================
