@@ -33,9 +33,9 @@ __global__ void k_solve_real(const double *, long, const double *, long, int, in
                              int, int, double *, int);
 constexpr int SOLVE_COLS = 16;  // columns per wave of the solve kernel (4 lanes per column)
 constexpr int SOLVE_WAVES_H = 4; // maximum waves per solve workgroup
-template <int TAG>
-__global__ void k_gemm_f64(int, int, int, int, const double *, long, const double *, long, const double *, long,
-                           double *, long, const int *, int, const double *, const int *, const double *);
+void launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const double *L, long ldl, const double *L2,
+                     long ldl2, const double *R, long ldr, double *C, long ldc, const int *outcol, int mode,
+                     const double *sig, const int *kmax, const double *R2);
 constexpr int GROUP_STEPS = 4; // row blocks per far update (K = 2 * ~256)
 __global__ void k_fixup_scale(double *, long, int, int, const int *, const int *, long, const int *, const int *);
 __global__ void k_copy_out(const double *, long, int, int, double *, long, const int *, int);
@@ -54,7 +54,6 @@ __global__ void k_zero_int(int *, long);
 
 constexpr int MAXB_REAL = 65;  // nominal block 64, +1 when a boundary would split a 2x2 block
 constexpr int NB_REAL = 64;
-constexpr int GEMM_TILE = 128;
 
 // ---------------------------------------------------------------------------------------
 // argument semantics shared by every real entry point
@@ -361,9 +360,8 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
             const int r0 = adjoint ? std::max(g0, std::min(desc[c_begin].lo, i0)) : g0;
             const int M = i0 - r0;
             if (M > 0) {
-                dim3 grid((N + GEMM_TILE - 1) / GEMM_TILE, (M + GEMM_TILE - 1) / GEMM_TILE);
                 pr = prof_begin(st);
-                hipLaunchKernelGGL(k_gemm_f64<0>, grid, dim3(256), 0, st, M, N, nb, nb, Sw + r0 + (long)i0 * ldsw,
+                launch_gemm_f64(0, st, M, N, nb, nb, Sw + r0 + (long)i0 * ldsw,
                                    ldsw, Pw + r0 + (long)i0 * ldpw, ldpw, (const double *)(W + (long)woff * ldw + c_begin),
                                    ldw, X + (long)r0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
                                    (const double *)(ws.sig.as<double>() + c_begin), (const int *)nullptr,
@@ -379,9 +377,8 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         const int r0 = adjoint ? std::min(desc[c_begin].lo, g0) : 0;
         const int M = g0 - r0;
         if (M > 0 && N > 0) {
-            dim3 grid((N + GEMM_TILE - 1) / GEMM_TILE, (M + GEMM_TILE - 1) / GEMM_TILE);
             int pr = prof_begin(st);
-            hipLaunchKernelGGL(k_gemm_f64<0>, grid, dim3(256), 0, st, M, N, wr, wr, Sw + r0 + (long)g0 * ldsw, ldsw,
+            launch_gemm_f64(0, st, M, N, wr, wr, Sw + r0 + (long)g0 * ldsw, ldsw,
                                Pw + r0 + (long)g0 * ldpw, ldpw, (const double *)(W + c_begin), ldw,
                                X + (long)r0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
                                (const double *)(ws.sig_grp.as<double>() + c_begin), (const int *)nullptr,
@@ -472,7 +469,6 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
         // V[:, outcol(p)] = Zw * X[:, p]; for left eigenvectors Zw = Q J (reversed columns)
         const double *Rp = A.left ? A.Z + (long)(n - 1) * A.ldz : A.Z;
         long ldr = A.left ? -A.ldz : A.ldz;
-        dim3 grid((n + GEMM_TILE - 1) / GEMM_TILE, (m + GEMM_TILE - 1) / GEMM_TILE);
         double ksum = 0.0;
         int kmx = 0;
         for (int q = 0; q < m; ++q) {
@@ -480,7 +476,7 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
             kmx = std::max(kmx, kmaxv[q]);
         }
         int pr = prof_begin(st);
-        hipLaunchKernelGGL(k_gemm_f64<1>, grid, dim3(256), 0, st, m, n, n, 0, (const double *)X, ldx,
+        launch_gemm_f64(1, st, m, n, n, 0, (const double *)X, ldx,
                            (const double *)nullptr, 0L, Rp, ldr, A.V, A.ldv, ws.outcol.as<int>(), 0,
                            (const double *)nullptr, ws.kmax.as<int>(), (const double *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<1>");
         prof_end(pr, st, 2, 2.0 * n * ksum, 8.0 * ((double)n * kmx + ksum + (double)n * m));
@@ -712,9 +708,8 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
     hipLaunchKernelGGL(k_to_rowmajor, dim3((m + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, st, V, ldv, n, m,
                        ws.Vt.as<double>(), ldt); XTG_LAUNCH_CHECK("k_to_rowmajor");
     {
-        dim3 grid((m + GEMM_TILE - 1) / GEMM_TILE, (n + GEMM_TILE - 1) / GEMM_TILE);
         pr = prof_begin(st);
-        hipLaunchKernelGGL(k_gemm_f64<2>, grid, dim3(256), 0, st, n, m, n, 0, P, ldp, (const double *)nullptr, 0L,
+        launch_gemm_f64(2, st, n, m, n, 0, P, ldp, (const double *)nullptr, 0L,
                            (const double *)ws.Vt.as<double>(), ldt, ws.Y.as<double>(), ldt, (const int *)nullptr, 0,
                            (const double *)nullptr, (const int *)nullptr, (const double *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
         prof_end(pr, st, 3, 2.0 * n * n * (double)m, 8.0 * ((double)n * n + 2.0 * n * m));
@@ -724,12 +719,11 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
                        ws.LamS.as<double>(), ws.LamP.as<double>()); XTG_LAUNCH_CHECK("k_bwd_lambda_t");
     // dS_b = V * LamS_b, dP_b = V * LamP_b  (as D[j][i] = sum_c V[j][c] Lam[c][i])
     for (int b = 0; b < nbatch; ++b) {
-        dim3 grid((n + GEMM_TILE - 1) / GEMM_TILE, (n + GEMM_TILE - 1) / GEMM_TILE);
         pr = prof_begin(st);
-        hipLaunchKernelGGL(k_gemm_f64<2>, grid, dim3(256), 0, st, n, n, m, 0, V, ldv, (const double *)nullptr, 0L,
+        launch_gemm_f64(2, st, n, n, m, 0, V, ldv, (const double *)nullptr, 0L,
                            (const double *)(ws.LamS.as<double>() + (long)b * m * n), (long)n, dS + (long)b * n * ldds,
                            ldds, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr, (const double *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
-        hipLaunchKernelGGL(k_gemm_f64<2>, grid, dim3(256), 0, st, n, n, m, 0, V, ldv, (const double *)nullptr, 0L,
+        launch_gemm_f64(2, st, n, n, m, 0, V, ldv, (const double *)nullptr, 0L,
                            (const double *)(ws.LamP.as<double>() + (long)b * m * n), (long)n, dP + (long)b * n * lddp,
                            lddp, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr, (const double *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
         prof_end(pr, st, 3, 4.0 * n * n * (double)m, 8.0 * 2.0 * ((double)n * m + (double)m * n + (double)n * n));

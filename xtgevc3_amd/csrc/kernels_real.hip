@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <cfloat>
 #include <cmath>
+#include <type_traits>
 
 #include "common.h"
 
@@ -122,7 +123,6 @@ __device__ __noinline__ Bump2 bump_solve(cpx r1, cpx r2, cpx c11, cpx c12, cpx c
 // whose solution would exceed 2^960 is scaled by a power of two (recorded as exponent).
 // ---------------------------------------------------------------------------------------
 constexpr int QL = 4;            // lanes per column
-constexpr int QCOLS = WAVE / QL; // columns per wave
 
 template <int SRC> __device__ __forceinline__ double quad_bcast_c(double v)
 {
@@ -372,19 +372,24 @@ __global__ __launch_bounds__(256) void k_solve_real(
 // ---------------------------------------------------------------------------------------
 // k_gemm_f64: D[a][b] = sum_k L[k][a] * R[k][b] on FP64 MFMA, 16x16x4 tiles.
 //   L: two K segments, L[a + k*ldl] (segment 1, K1 rows) then L2[a + (k-K1)*ldl2]
-//   R: R[k*ldr + b]
+//   R: R[k*ldr + b] (segment 1), R2[(k-K1)*ldr + b] (segment 2; R + K1*ldr when null)
 //   C: row a starts at crow(a) = C + (outcol ? outcol[a] : a) * ldc; element b contiguous.
 //   mode 0: C = D;  mode 1: C = sig[b] * C + D.
 //   kmax (optional): per-a upper bound of k (exclusive); a tile uses the max over its rows,
 //   so triangular operands skip their zero part.
-// Workgroup: 256 threads (2x2 waves), tile 128 (a) x 128 (b), K chunks of 16 through LDS.
+// Workgroup: 256 threads (2x2 waves of 64x64), tile 128 (a) x 128 (b), K chunks of 16 rows
+// double-buffered in LDS; the next chunk is fetched into registers while the current one
+// feeds the MFMAs.  Each segment is padded to whole chunks with zero rows (masked at the
+// LDS store, so no wait on the loads precedes the MFMAs).  Columns a >= M, b >= N are
+// clamped to the last one: they only feed outputs that are never stored.
+// WIDE: 16-byte loads of column pairs (M, N >= 2); otherwise one double per load.
 // ---------------------------------------------------------------------------------------
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
 constexpr int GT = 128; // tile edge
 constexpr int GK = 16;  // K chunk
 
-template <int TAG>
+template <int TAG, bool WIDE>
 __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, const double *__restrict__ L, long ldl,
                                                   const double *__restrict__ L2, long ldl2,
                                                   const double *__restrict__ R, long ldr, double *__restrict__ C,
@@ -392,7 +397,6 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
                                                   const double *__restrict__ sig, const int *__restrict__ kmax,
                                                   const double *__restrict__ R2)
 {
-    // K rows [0, K1) of R at R, rows [K1, K1+K2) at R2 (R + K1*ldr when R2 is null)
     if (R2 == nullptr) {
         R2 = R + (long)K1 * ldr;
     }
@@ -422,6 +426,10 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
         __syncthreads();
         K = min(K, kred);
     }
+    const int rows1 = min(K1, K);
+    const int rows2 = max(0, min(K2, K - K1));
+    const int nch1 = (rows1 + GK - 1) / GK;
+    const int nchunks = nch1 + (rows2 + GK - 1) / GK;
 
     double4_t acc[4][4];
 #pragma clang loop unroll(full)
@@ -432,39 +440,68 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
         }
     }
 
-    // global -> register staging: each thread loads 8 doubles of L and 8 of R per chunk
-    // thread layout: column index c = tid % 128, k rows kr = tid / 128 + 2*s (s = 0..7)
-    const int lc = tid % GT;
-    const int lk = tid / GT;
-    double lreg[8], rreg[8];
-    auto load_chunk = [&](int k0) {
+    // register staging of one chunk: NS rows per thread (WIDE: column pair c, rows lk + 4 s;
+    // otherwise column c, rows lk + 2 s)
+    constexpr int VW = WIDE ? 2 : 1;
+    constexpr int RSTEP = 256 / (GT / VW);
+    constexpr int NS = GK / RSTEP;
+    const int c = VW * (tid % (GT / VW));
+    const int lk = tid / (GT / VW);
+    const int aa = min(a0 + c, M - VW);
+    const int bb = min(b0 + c, N - VW);
+    const bool odd_a = WIDE && a0 + c == M - 1; // pair shifted left at the last column
+    const bool odd_b = WIDE && b0 + c == N - 1;
+    typedef typename std::conditional<WIDE, double2, double>::type vec_t;
+    vec_t lreg[NS], rreg[NS];
+    int valid = 0; // rows of the staged chunk inside their segment
+    auto load_chunk = [&](int ch) {
+        const bool s1 = ch < nch1;
+        const int k0 = (s1 ? ch : ch - nch1) * GK;
+        const int rows = s1 ? rows1 : rows2;
+        const long ll = s1 ? ldl : ldl2;
+        const double *lb = (s1 ? L : L2) + aa;
+        const double *rb = (s1 ? R : R2) + bb;
+        valid = rows - k0;
+        if (valid >= GK) {
 #pragma clang loop unroll(full)
-        for (int s = 0; s < 8; ++s) {
-            int k = k0 + lk + 2 * s;
-            int aa = a0 + lc;
-            double lv = 0.0, rv = 0.0;
-            if (k < K) {
-                if (aa < M) {
-                    lv = (k < K1) ? L[aa + (long)k * ldl] : L2[aa + (long)(k - K1) * ldl2];
-                }
-                int bb = b0 + lc;
-                if (bb < N) {
-                    rv = (k < K1) ? R[(long)k * ldr + bb] : R2[(long)(k - K1) * ldr + bb];
-                }
+            for (int s = 0; s < NS; ++s) {
+                const int k = k0 + lk + RSTEP * s;
+                lreg[s] = *(const vec_t *)(lb + (long)k * ll);
+                rreg[s] = *(const vec_t *)(rb + (long)k * ldr);
             }
-            lreg[s] = lv;
-            rreg[s] = rv;
+        }
+        else {
+#pragma clang loop unroll(full)
+            for (int s = 0; s < NS; ++s) {
+                const int k = min(k0 + lk + RSTEP * s, rows - 1);
+                lreg[s] = *(const vec_t *)(lb + (long)k * ll);
+                rreg[s] = *(const vec_t *)(rb + (long)k * ldr);
+            }
         }
     };
     auto store_chunk = [&](int buf) {
 #pragma clang loop unroll(full)
-        for (int s = 0; s < 8; ++s) {
-            Ls[buf][lk + 2 * s][lc] = lreg[s];
-            Rs[buf][lk + 2 * s][lc] = rreg[s];
+        for (int s = 0; s < NS; ++s) {
+            const int r = lk + RSTEP * s;
+            const bool in = r < valid;
+            if constexpr (WIDE) {
+                double2 lv = lreg[s], rv = rreg[s];
+                lv.x = odd_a ? lv.y : lv.x;
+                rv.x = odd_b ? rv.y : rv.x;
+                if (!in) {
+                    lv = make_double2(0.0, 0.0);
+                    rv = make_double2(0.0, 0.0);
+                }
+                *(double2 *)&Ls[buf][r][c] = lv;
+                *(double2 *)&Rs[buf][r][c] = rv;
+            }
+            else {
+                Ls[buf][r][c] = in ? lreg[s] : 0.0;
+                Rs[buf][r][c] = in ? rreg[s] : 0.0;
+            }
         }
     };
 
-    const int nchunks = (K + GK - 1) / GK;
     if (nchunks > 0) {
         load_chunk(0);
         store_chunk(0);
@@ -475,7 +512,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
     for (int ch = 0; ch < nchunks; ++ch) {
         const int buf = ch & 1;
         if (ch + 1 < nchunks) {
-            load_chunk((ch + 1) * GK);
+            load_chunk(ch + 1);
         }
 #pragma clang loop unroll(full)
         for (int ks = 0; ks < GK; ks += 4) {
@@ -504,19 +541,19 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
     for (int i = 0; i < 4; ++i) {
 #pragma clang loop unroll(full)
         for (int vv = 0; vv < 4; ++vv) {
-            int aa = a0 + wa + i * 16 + fk + 4 * vv;
-            if (aa >= M) {
+            int ar = a0 + wa + i * 16 + fk + 4 * vv;
+            if (ar >= M) {
                 continue;
             }
-            long rowoff = (long)(outcol ? outcol[aa] : aa) * ldc;
+            long rowoff = (long)(outcol ? outcol[ar] : ar) * ldc;
 #pragma clang loop unroll(full)
             for (int j = 0; j < 4; ++j) {
-                int bb = b0 + wb + j * 16 + fr;
-                if (bb < N) {
+                int bc = b0 + wb + j * 16 + fr;
+                if (bc < N) {
                     double d = acc[i][j][vv];
-                    double *cp = C + rowoff + bb;
+                    double *cp = C + rowoff + bc;
                     if (mode == 1) {
-                        d = fma(sig[bb], *cp, d);
+                        d = fma(sig[bc], *cp, d);
                     }
                     *cp = d;
                 }
@@ -702,13 +739,6 @@ __global__ void k_zero_int(int *__restrict__ X, long count)
 } // namespace xtg
 
 namespace xtg {
-// TAG only separates the two uses in profiles: 0 = RHS update, 1 = back-transform.
-template __global__ void k_gemm_f64<0>(int, int, int, int, const double *, long, const double *, long, const double *,
-                                       long, double *, long, const int *, int, const double *, const int *,
-                                       const double *);
-template __global__ void k_gemm_f64<1>(int, int, int, int, const double *, long, const double *, long, const double *,
-                                       long, double *, long, const int *, int, const double *, const int *,
-                                       const double *);
 template __global__ void k_solve_real<65>(const double *, long, const double *, long, int, int, double *, long,
                                           double *, long, const ColDesc *, const int2 *, int, int, int, int,
                                           double *, int *, int *, int *, int, int, int, double *, int);
@@ -863,7 +893,29 @@ __global__ __launch_bounds__(64) void k_bwd_diag(const double *__restrict__ X, l
     }
 }
 
-template __global__ void k_gemm_f64<2>(int, int, int, int, const double *, long, const double *, long, const double *,
-                                       long, double *, long, const int *, int, const double *, const int *,
-                                       const double *);
+// Host launcher: picks the 16-byte-load form when both output extents are >= 2.
+// tag only separates the uses in profiles: 0 = RHS update, 1 = back-transform, 2 = backward.
+void launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const double *L, long ldl, const double *L2,
+                     long ldl2, const double *R, long ldr, double *C, long ldc, const int *outcol, int mode,
+                     const double *sig, const int *kmax, const double *R2)
+{
+    dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
+    const bool wide = M >= 2 && N >= 2;
+#define XTG_GEMM_CASE(T)                                                                                             \
+    if (tag == T) {                                                                                                  \
+        if (wide) {                                                                                                  \
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, true>), grid, dim3(256), 0, st, M, N, K1, K2, L, ldl, L2, \
+                               ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2);                                   \
+        }                                                                                                            \
+        else {                                                                                                       \
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, false>), grid, dim3(256), 0, st, M, N, K1, K2, L, ldl,  \
+                               L2, ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2);                               \
+        }                                                                                                            \
+        return;                                                                                                      \
+    }
+    XTG_GEMM_CASE(0)
+    XTG_GEMM_CASE(1)
+    XTG_GEMM_CASE(2)
+#undef XTG_GEMM_CASE
+}
 } // namespace xtg
