@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -29,7 +30,7 @@ namespace xtg {
 
 template <int MAXB>
 __global__ void k_solve_real(const double *, long, const double *, long, int, int, double *, long, double *, long,
-                             const ColDesc *, const int2 *, int, int, int, int, double *, int *, int *, int *, int,
+                             const ColDesc *, const int2 *, int, int, int, int, double *, int *, int *, int *, int *,
                              int, int, double *, int);
 constexpr int SOLVE_COLS = 16;  // columns per wave of the solve kernel (4 lanes per column)
 constexpr int SOLVE_WAVES_H = 4; // maximum waves per solve workgroup
@@ -50,6 +51,8 @@ __global__ void k_bwd_diag(const double *, long, const double *, long, int, int,
                            const int *, double *, long, double *, long);
 __global__ void k_subdiag(const double *, long, int, double *);
 __global__ void k_zero(double *, long);
+__global__ void k_w_rescale(double *, long, int, int, int, const int *, long, int4, int4, const int *);
+__global__ void k_zero_cols(double *, long, int, int, int);
 __global__ void k_zero_int(int *, long);
 
 constexpr int MAXB_REAL = 65;  // nominal block 64, +1 when a boundary would split a 2x2 block
@@ -218,7 +221,7 @@ struct DevBuf {
 };
 
 struct RealWork {
-    DevBuf X, W, sig, colexp, blkexp, flag, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
+    DevBuf X, W, sig, colexp, blkexp, flag, gflag, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
     DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp;
 };
 
@@ -288,6 +291,8 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     ENSURE(ws.colexp, sizeof(int) * (size_t)m);
     ENSURE(ws.blkexp, sizeof(int) * (size_t)nsteps * ldx);
     ENSURE(ws.flag, sizeof(int) * 4);
+    const int ngroups = (nsteps + GROUP_STEPS - 1) / GROUP_STEPS;
+    ENSURE(ws.gflag, sizeof(int) * (size_t)std::max(1, ngroups));
     ENSURE(ws.desc, sizeof(ColDesc) * (size_t)m);
     ENSURE(ws.tiles, sizeof(int2) * tiles.size());
     ENSURE(ws.rowstep, sizeof(int) * (size_t)n);
@@ -299,6 +304,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, ws.blkexp.as<int>(), (long)nsteps * ldx); XTG_LAUNCH_CHECK("k_zero_int");
     hipLaunchKernelGGL(k_zero_int, dim3(64), dim3(256), 0, st, ws.colexp.as<int>(), (long)m); XTG_LAUNCH_CHECK("k_zero_int");
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, ws.flag.as<int>(), 4L); XTG_LAUNCH_CHECK("k_zero_int");
+    hipLaunchKernelGGL(k_zero_int, dim3(64), dim3(256), 0, st, ws.gflag.as<int>(), (long)std::max(1, ngroups)); XTG_LAUNCH_CHECK("k_zero_int");
 
     // Steps go bottom-up in groups of GROUP_STEPS row blocks [g0, g1).  Within a group each
     // step updates only the group's rows above it ("near" update, K = 2 nb); after the group
@@ -313,7 +319,23 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         const int wr = g1 - g0;
         bool any_active = false;
         bool first = true;
-        hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, st, W, (long)2 * wr * ldw); XTG_LAUNCH_CHECK("k_zero");
+        int *gflag = ws.gflag.as<int>() + gs / GROUP_STEPS;
+        if (!adjoint) {
+            // columns whose solve starts inside the group after its first step never write the
+            // operand rows of the blocks below their start: those rows must read as zero
+            int cb_first = c_begin, cb_last = c_begin;
+            while (cb_first > 0 && desc[cb_first - 1].d >= i0s[gs]) {
+                --cb_first;
+            }
+            cb_last = cb_first;
+            while (cb_last > 0 && desc[cb_last - 1].d >= g0) {
+                --cb_last;
+            }
+            if (cb_first > cb_last) {
+                hipLaunchKernelGGL(k_zero_cols, dim3((cb_first - cb_last + 255) / 256, 2 * wr), dim3(256), 0, st, W, ldw,
+                                   2 * wr, cb_last, cb_first - cb_last); XTG_LAUNCH_CHECK("k_zero_cols");
+            }
+        }
         for (int s = gs; s < ge; ++s) {
             const int i0 = i0s[s], nb = nbs[s];
             if (adjoint) {
@@ -352,7 +374,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
                                dim3((ntl + wpb - 1) / wpb), dim3(wpb * WAVE), 0, st, Sw,
                                ldsw, Pw, ldpw, i0, nb, X, ldx, W, ldw, ws.desc.as<ColDesc>(), ws.tiles.as<int2>(),
                                t_begin, t_end, c_begin, c_end, ws.sig.as<double>(), ws.colexp.as<int>(),
-                               ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff, wr + woff, wr,
+                               ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), gflag, woff, wr + woff,
                                ws.sig_grp.as<double>(), first ? 1 : 0); XTG_LAUNCH_CHECK("k_solve_real");
             prof_end(pr, st, 0, 2.0 * nb * nb * N, 8.0 * (4.0 * nb * N + 2.0 * nb * nb));
             first = false;
@@ -377,6 +399,29 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         const int r0 = adjoint ? std::min(desc[c_begin].lo, g0) : 0;
         const int M = g0 - r0;
         if (M > 0 && N > 0) {
+            // bring the group's operands to the columns' final scale (no-op without rescaling)
+            int4 offs = make_int4(0, INT_MAX, INT_MAX, INT_MAX), steps = make_int4(ge - 1, ge - 1, ge - 1, ge - 1);
+            {
+                int off = 0;
+                for (int j = 0; j < ge - gs; ++j) {
+                    const int sj = ge - 1 - j; // blocks top-down
+                    if (j == 1) {
+                        offs.y = off;
+                        steps.y = sj;
+                    }
+                    else if (j == 2) {
+                        offs.z = off;
+                        steps.z = sj;
+                    }
+                    else if (j == 3) {
+                        offs.w = off;
+                        steps.w = sj;
+                    }
+                    off += nbs[sj];
+                }
+            }
+            hipLaunchKernelGGL(k_w_rescale, dim3((N + 255) / 256, 2 * wr), dim3(256), 0, st, W, ldw, wr, c_begin, N,
+                               (const int *)ws.blkexp.as<int>(), ldx, offs, steps, (const int *)gflag); XTG_LAUNCH_CHECK("k_w_rescale");
             int pr = prof_begin(st);
             launch_gemm_f64(0, st, M, N, wr, wr, Sw + r0 + (long)g0 * ldsw, ldsw,
                                Pw + r0 + (long)g0 * ldpw, ldpw, (const double *)(W + c_begin), ldw,

@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void k_solve_real(
     const double *__restrict__ S, long lds, const double *__restrict__ P, long ldp, int i0, int nb,
     double *__restrict__ X, long ldx, double *__restrict__ W, long ldw, const ColDesc *__restrict__ desc,
     const int2 *__restrict__ tiles, int t_begin, int t_end, int c_begin, int c_end, double *__restrict__ sig,
-    int *__restrict__ colexp, int *__restrict__ blkexp, int *__restrict__ flag, int w_s, int w_p, int w_rows,
+    int *__restrict__ colexp, int *__restrict__ blkexp, int *__restrict__ flag, int *__restrict__ gflag, int w_s, int w_p,
     double *__restrict__ sig_grp, int first_in_group)
 {
     constexpr int QR = (MAXB + QL - 1) / QL; // rows per lane
@@ -353,19 +353,54 @@ __global__ __launch_bounds__(256) void k_solve_real(
         // a column starting in this block has no earlier steps in the group
         const bool starts_here = dl >= 0 && dl < nb;
         sig_grp[p] = (first_in_group || starts_here) ? sg : sig_grp[p] * sg;
-        if (my_exp != 0) {
-            // operands written by the group's earlier (lower) blocks carry the old scale
-            for (int k = w_s + nb; k < w_rows; ++k) {
-                W[(long)k * ldw + p] *= sg;
-                W[(long)(w_rows + k) * ldw + p] *= sg;
-            }
-        }
         int ce = colexp[p] + my_exp;
         colexp[p] = ce;
         blkexp[p] = ce;
         if (my_exp != 0) {
+            // the operands of the group's earlier (lower) blocks carry the old scale: the
+            // group's rescale pass (k_w_rescale) brings them to the final one
             *flag = 1;
+            *gflag = 1;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// k_w_rescale: after the last step of a group, bring the group's update operands W (rows
+// [0, wr) S part, [wr, 2 wr) P part; local row k belongs to the block holding row g0 + k)
+// to the columns' final scale: W[k][p] *= 2^(blkexp[last][p] - blkexp[step(k)][p]).
+// Blocks are listed top-down: block j covers local rows [off_j, off_{j+1}) and was solved
+// in step steps_j; steps.x is the group's last step.  Skipped when no column of the group
+// was rescaled (*gflag == 0).
+// ---------------------------------------------------------------------------------------
+__global__ void k_w_rescale(double *__restrict__ W, long ldw, int wr, int c0, int ncols, const int *__restrict__ blkexp,
+                            long ldb, int4 offs, int4 steps, const int *__restrict__ gflag)
+{
+    if (*gflag == 0) {
+        return;
+    }
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y; // 0 .. 2 wr - 1
+    if (x >= ncols) {
+        return;
+    }
+    const int kl = k < wr ? k : k - wr;
+    const int j = (kl >= offs.y) + (kl >= offs.z) + (kl >= offs.w);
+    const int st = j == 0 ? steps.x : (j == 1 ? steps.y : (j == 2 ? steps.z : steps.w));
+    const int p = c0 + x;
+    const int e = blkexp[(long)steps.x * ldb + p] - blkexp[(long)st * ldb + p];
+    if (e != 0) {
+        double *w = W + (long)k * ldw + p;
+        *w = ldexp(*w, e);
+    }
+}
+
+// W[k][c0 + x] = 0 for k < rows, x < ncols
+__global__ void k_zero_cols(double *__restrict__ W, long ldw, int rows, int c0, int ncols)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < ncols && (int)blockIdx.y < rows) {
+        W[(long)blockIdx.y * ldw + c0 + x] = 0.0;
     }
 }
 
@@ -741,7 +776,7 @@ __global__ void k_zero_int(int *__restrict__ X, long count)
 namespace xtg {
 template __global__ void k_solve_real<65>(const double *, long, const double *, long, int, int, double *, long,
                                           double *, long, const ColDesc *, const int2 *, int, int, int, int,
-                                          double *, int *, int *, int *, int, int, int, double *, int);
+                                          double *, int *, int *, int *, int *, int, int, double *, int);
 } // namespace xtg
 
 // =======================================================================================
