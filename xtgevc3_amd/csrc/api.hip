@@ -104,7 +104,14 @@ void real_host(char side, char howmny, const int *select, int n, const double *S
     }
     int rc = real_device(side, howmny, select, n, (const double *)g_dS.p, n, (const double *)g_dP.p, n, ar, ai, be,
                          left ? (double *)g_dVL.p : nullptr, n, right ? (double *)g_dVR.p : nullptr, n, nullptr);
-    if (rc != 0 || hipDeviceSynchronize() != hipSuccess) {
+    if (rc == 0) {
+        const hipError_t se = hipDeviceSynchronize();
+        if (se != hipSuccess) {
+            set_error(std::string("kernel execution: ") + hipGetErrorString(se));
+            rc = -1;
+        }
+    }
+    if (rc != 0) {
         fail(info, "device path");
         return;
     }
@@ -159,7 +166,14 @@ void complex_host(char side, char howmny, const int *select, int n, const std::c
     }
     int rc = complex_device(side, howmny, select, n, (const zt *)g_dS.p, n, (const zt *)g_dP.p, n, al, be,
                             left ? (zt *)g_dVL.p : nullptr, n, right ? (zt *)g_dVR.p : nullptr, n, nullptr);
-    if (rc != 0 || hipDeviceSynchronize() != hipSuccess) {
+    if (rc == 0) {
+        const hipError_t se = hipDeviceSynchronize();
+        if (se != hipSuccess) {
+            set_error(std::string("kernel execution: ") + hipGetErrorString(se));
+            rc = -1;
+        }
+    }
+    if (rc != 0) {
         fail(info, "device path");
         return;
     }

@@ -30,6 +30,18 @@ enum : int {
     INIT_PAIR_ROW2 = 3, // x_{d..d+1} = null vector of the second row of the 2x2 block
 };
 
+// One eigenvalue of the real interface in the solve kernel: a real eigenvalue (one working
+// column) or a complex-conjugate pair (two adjacent columns: real part, imaginary part).
+struct UnitDesc {
+    double a, bR, bI, pad; // shift coefficients (bI = 0 for a real eigenvalue)
+    int col;               // first working column
+    int w;                 // 1 or 2 columns
+    int d;                 // first row of the unit's own diagonal block (>= n: none)
+    int lo;                // rows < lo lie outside the unit's system and stay zero
+    int init;              // INIT_* mode
+    int pad1, pad2, pad3;
+};
+
 // Same layout for the complex interface: the working vector entries are complex.
 struct ZColDesc {
     double aR, aI; // acoeff (complex for the complex interface)

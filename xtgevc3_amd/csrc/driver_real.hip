@@ -29,11 +29,11 @@
 namespace xtg {
 
 template <int MAXB>
-__global__ void k_solve_real(const double *, long, const double *, long, int, int, double *, long, double *, long,
-                             const ColDesc *, const int2 *, int, int, int, int, double *, int *, int *, int *, int *,
-                             int, int, double *, int);
-constexpr int SOLVE_COLS = 16;  // columns per wave of the solve kernel (4 lanes per column)
-constexpr int SOLVE_WAVES_H = 4; // maximum waves per solve workgroup
+__global__ void k_solve_real(const double *, long, const double *, long, int, int, unsigned long long, double *, long,
+                             double *, long, const UnitDesc *, const int2 *, int, int, int, int, double *, int *, int *,
+                             int *, int *, int, int, double *, int);
+constexpr int SOLVE_UNITS = 4;   // eigenvalue units per wave of the solve kernel (16 lanes per unit)
+constexpr int SOLVE_WAVES_H = 4; // waves per solve workgroup (they share the staged block)
 void launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const double *L, long ldl, const double *L2,
                      long ldl2, const double *R, long ldr, double *C, long ldc, const int *outcol, int mode,
                      const double *sig, const int *kmax, const double *R2);
@@ -176,18 +176,36 @@ static void partition(int n, const std::vector<double> &subw, std::vector<int> &
     }
 }
 
-static void make_tiles(const std::vector<ColDesc> &desc, std::vector<int2> &tiles, int width)
+// Units of the real solve: one per real eigenvalue column, one per complex pair (p, p+1).
+static void make_units(const std::vector<ColDesc> &desc, std::vector<UnitDesc> &units, std::vector<int> &col2unit)
+{
+    units.clear();
+    const int m = (int)desc.size();
+    col2unit.assign(m, 0);
+    for (int p = 0; p < m;) {
+        const ColDesc &c = desc[p];
+        UnitDesc u{};
+        u.a = c.a;
+        u.bR = c.bR;
+        u.bI = c.bI;
+        u.col = p;
+        u.w = ((c.flags & 2) && c.partner == p + 1) ? 2 : 1;
+        u.d = c.d;
+        u.lo = c.lo;
+        u.init = (c.flags >> 2) & 3;
+        for (int k = 0; k < u.w; ++k) {
+            col2unit[p + k] = (int)units.size();
+        }
+        units.push_back(u);
+        p += u.w;
+    }
+}
+
+static void make_tiles(int count, std::vector<int2> &tiles, int width)
 {
     tiles.clear();
-    int m = (int)desc.size();
-    int s = 0;
-    while (s < m) {
-        int cnt = std::min(width, m - s);
-        if (cnt == width && (desc[s + width - 1].flags & 2) && desc[s + width - 1].partner == s + width) {
-            cnt = width - 1; // never split a pair across waves
-        }
-        tiles.push_back(make_int2(s, cnt));
-        s += cnt;
+    for (int s = 0; s < count; s += width) {
+        tiles.push_back(make_int2(s, std::min(width, count - s)));
     }
 }
 
@@ -272,8 +290,11 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
                hipStream_t st)
 {
     const int m = (int)desc.size();
+    std::vector<UnitDesc> units;
+    std::vector<int> col2unit;
+    make_units(desc, units, col2unit);
     std::vector<int2> tiles;
-    make_tiles(desc, tiles, SOLVE_COLS);
+    make_tiles((int)units.size(), tiles, SOLVE_UNITS);
     std::vector<int> i0s, nbs;
     partition(n, subw, i0s, nbs);
     const int nsteps = (int)i0s.size();
@@ -293,10 +314,10 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     ENSURE(ws.flag, sizeof(int) * 4);
     const int ngroups = (nsteps + GROUP_STEPS - 1) / GROUP_STEPS;
     ENSURE(ws.gflag, sizeof(int) * (size_t)std::max(1, ngroups));
-    ENSURE(ws.desc, sizeof(ColDesc) * (size_t)m);
+    ENSURE(ws.desc, sizeof(UnitDesc) * units.size());
     ENSURE(ws.tiles, sizeof(int2) * tiles.size());
     ENSURE(ws.rowstep, sizeof(int) * (size_t)n);
-    XTG_CHECK(hipMemcpyAsync(ws.desc.p, desc.data(), sizeof(ColDesc) * m, hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.desc.p, units.data(), sizeof(UnitDesc) * units.size(), hipMemcpyHostToDevice, st));
     XTG_CHECK(hipMemcpyAsync(ws.tiles.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, st));
     XTG_CHECK(hipMemcpyAsync(ws.rowstep.p, rowstep.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     double *X = ws.X.as<double>();
@@ -356,24 +377,25 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
                 continue;
             }
             any_active = true;
-            int t_begin = 0;
-            while (tiles[t_begin].x + tiles[t_begin].y <= c_begin) {
-                ++t_begin;
-            }
-            int t_end = t_begin;
-            while (t_end < (int)tiles.size() && tiles[t_end].x < c_end) {
-                ++t_end;
-            }
+            const int u_begin = col2unit[c_begin], u_end = col2unit[c_end - 1] + 1;
+            const int t_begin = u_begin / SOLVE_UNITS;
+            const int t_end = (u_end + SOLVE_UNITS - 1) / SOLVE_UNITS;
             const int ntl = t_end - t_begin;
             const int N = c_end - c_begin;
             const int woff = i0 - g0;
-            // waves per workgroup: enough workgroups to cover the CUs, each staging the block once
-            const int wpb = std::max(1, std::min(SOLVE_WAVES_H, ntl / 256));
+            // 2x2 bumps of S in the block: bit r-1 <=> rows (r-1, r)
+            unsigned long long bumps = 0;
+            for (int r = 1; r < nb; ++r) {
+                if (subw[i0 + r] != 0.0) {
+                    bumps |= 1ull << (r - 1);
+                }
+            }
+            const int wpb = std::min(SOLVE_WAVES_H, ntl);
             int pr = prof_begin(st);
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_real<MAXB_REAL>),
                                dim3((ntl + wpb - 1) / wpb), dim3(wpb * WAVE), 0, st, Sw,
-                               ldsw, Pw, ldpw, i0, nb, X, ldx, W, ldw, ws.desc.as<ColDesc>(), ws.tiles.as<int2>(),
-                               t_begin, t_end, c_begin, c_end, ws.sig.as<double>(), ws.colexp.as<int>(),
+                               ldsw, Pw, ldpw, i0, nb, bumps, X, ldx, W, ldw, ws.desc.as<UnitDesc>(), ws.tiles.as<int2>(),
+                               t_begin, t_end, u_begin, u_end, ws.sig.as<double>(), ws.colexp.as<int>(),
                                ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), gflag, woff, wr + woff,
                                ws.sig_grp.as<double>(), first ? 1 : 0); XTG_LAUNCH_CHECK("k_solve_real");
             prof_end(pr, st, 0, 2.0 * nb * nb * N, 8.0 * (4.0 * nb * N + 2.0 * nb * nb));

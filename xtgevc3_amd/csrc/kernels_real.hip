@@ -31,329 +31,480 @@ __device__ __forceinline__ cpx cmul(cpx a, cpx b) { return {a.r * b.r - a.i * b.
 __device__ __forceinline__ cpx csub(cpx a, cpx b) { return {a.r - b.r, a.i - b.i}; }
 __device__ __forceinline__ double abs1(cpx a) { return fabs(a.r) + fabs(a.i); }
 
-// Smith's complex division (no overflow of |c|^2).  Real lanes have c.i == 0 and get an
-// exact real division.
-__device__ __forceinline__ cpx cdiv(cpx x, cpx c)
+// Reciprocal of a complex number, Smith's ordering (no overflow of |c|^2); c != 0.
+__device__ __forceinline__ cpx crecip(cpx c)
 {
-    if (fabs(c.r) >= fabs(c.i)) {
-        double t = c.i / c.r;
-        double den = c.r + c.i * t;
-        return {(x.r + x.i * t) / den, (x.i - x.r * t) / den};
-    }
-    double t = c.r / c.i;
-    double den = c.i + c.r * t;
-    return {(x.r * t + x.i) / den, (x.i * t - x.r) / den};
-}
-
-// Exponent e >= 0 such that 2^-e * num <= lim * den (lim = 2^k), computed without
-// forming num / den (which may overflow).
-__device__ __forceinline__ int scale_exponent(double num, double den, int log2lim)
-{
-    int e = ilogb(num) - ilogb(den) - log2lim + 2;
-    return e > 0 ? e : 0;
+    const bool sw = fabs(c.i) > fabs(c.r);
+    const double big = sw ? c.i : c.r, sml = sw ? c.r : c.i;
+    const double t = sml / big;
+    const double rd = 1.0 / (big + sml * t);
+    const double tr = t * rd;
+    return sw ? cpx{tr, -rd} : cpx{rd, -tr};
 }
 
 constexpr double kEps = 2.220446049250313e-16;     // DBL_EPSILON
 constexpr double kSafeMin = 2.2250738585072014e-308; // DBL_MIN
-constexpr int kLog2Big = 960;                        // |x| entries are kept <= 2^960
+constexpr double kScaleTrigger = 0x1p958;          // a solution entry above this triggers a rescale
+constexpr int kScaleTarget = 446;                  // rescaled entries stay below 2^446 (2^512 headroom)
 
+// Exponent e >= 0 with 2^-e * num * fac <= 2^kScaleTarget (abs1 bounds, no product formed).
+__device__ __forceinline__ int rescale_exponent(double num, double fac)
+{
+    if (!(num > 0.0) || !(fac > 0.0)) {
+        return 0;
+    }
+    const int e = ilogb(num) + ilogb(fac) + 2 - kScaleTarget;
+    return e > 0 ? e : 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// Row broadcast: every lane of each 16-lane row receives lane `src` of its row (DPP
+// row_newbcast, gfx90a and later).
+// ---------------------------------------------------------------------------------------
+constexpr int QL = 16; // lanes per eigenvalue unit
+
+template <int SRC> __device__ __forceinline__ int row_bcast_i(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + SRC, 0xF, 0xF, false);
+}
+
+template <int SRC> __device__ __forceinline__ double row_bcast_d(double v)
+{
+    int lo = row_bcast_i<SRC>(__double2loint(v));
+    int hi = row_bcast_i<SRC>(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+#define XTG_RB_CASES(F)                                                                                          \
+    switch (src) {                                                                                               \
+    case 0: return F<0>(v);                                                                                      \
+    case 1: return F<1>(v);                                                                                      \
+    case 2: return F<2>(v);                                                                                      \
+    case 3: return F<3>(v);                                                                                      \
+    case 4: return F<4>(v);                                                                                      \
+    case 5: return F<5>(v);                                                                                      \
+    case 6: return F<6>(v);                                                                                      \
+    case 7: return F<7>(v);                                                                                      \
+    case 8: return F<8>(v);                                                                                      \
+    case 9: return F<9>(v);                                                                                      \
+    case 10: return F<10>(v);                                                                                    \
+    case 11: return F<11>(v);                                                                                    \
+    case 12: return F<12>(v);                                                                                    \
+    case 13: return F<13>(v);                                                                                    \
+    case 14: return F<14>(v);                                                                                    \
+    default: return F<15>(v);                                                                                    \
+    }
+
+__device__ __forceinline__ double qbd(double v, int src) { XTG_RB_CASES(row_bcast_d) }
+__device__ __forceinline__ int qbi(int v, int src) { XTG_RB_CASES(row_bcast_i) }
+#undef XTG_RB_CASES
+
+__device__ __forceinline__ cpx qbc(cpx v, int src) { return {qbd(v.r, src), qbd(v.i, src)}; }
+
+// Robust 2x2 bump solve from its factorization (rare path: some entry would exceed the
+// rescale trigger).  Pivoted rows: rp = sw ? r2 : r1, rs = sw ? r1 : r2;
+// y = rs - mlt rp, x2 = y iu, x1 = (rp - q x2) ip.  Returns the scaled solution and the
+// exponent e of the scale 2^-e applied to the right-hand side.
 struct Bump2 {
     cpx x1, x2;
     int e;
 };
 
-// 2x2 (complex) block solve with partial pivoting on the first column and overflow guard.
-// Restates the effect of the Hessenberg GEPP of dlauhs/dlau2s (dtgevc3.cpp:132-344) on a
-// 2x2 diagonal bump of S: [c11 c12; c21 c22] [x1; x2] = [r1; r2].
-__device__ __noinline__ Bump2 bump_solve(cpx r1, cpx r2, cpx c11, cpx c12, cpx c21, cpx c22, double dmin1,
-                                         double dmin2)
+__device__ __noinline__ Bump2 bump_solve_scaled(cpx r1, cpx r2, bool sw, cpx mlt, cpx q, cpx iu, cpx ip)
 {
-    bool sw = abs1(c21) > abs1(c11);
-    cpx p = sw ? c21 : c11, q = sw ? c22 : c12;
-    cpx s = sw ? c11 : c21, t = sw ? c12 : c22;
     cpx rp = sw ? r2 : r1, rs = sw ? r1 : r2;
-    if (abs1(p) < dmin1) {
-        p = {dmin1, 0.0};
-    }
-    cpx mlt = cdiv(s, p);
-    cpx u = csub(t, cmul(mlt, q));
     cpx y = csub(rs, cmul(mlt, rp));
-    if (abs1(u) < dmin2) {
-        u = {dmin2, 0.0};
-    }
     Bump2 out;
     out.e = 0;
-    double yn = abs1(y);
-    if (yn > 0.0 && ilogb(yn) - ilogb(abs1(u)) > kLog2Big - 2) {
-        int e = scale_exponent(yn, abs1(u), kLog2Big);
-        double f = ldexp(1.0, -e);
+    int ea = rescale_exponent(abs1(y), abs1(iu));
+    if (ea > 0) {
+        const double f = ldexp(1.0, -ea);
         y = {y.r * f, y.i * f};
         rp = {rp.r * f, rp.i * f};
-        out.e += e;
+        out.e += ea;
     }
-    cpx x2 = cdiv(y, u);
+    cpx x2 = cmul(y, iu);
     cpx w = csub(rp, cmul(q, x2));
-    double wn = abs1(w);
-    if (wn > 0.0 && ilogb(wn) - ilogb(abs1(p)) > kLog2Big - 2) {
-        int e = scale_exponent(wn, abs1(p), kLog2Big);
-        double f = ldexp(1.0, -e);
+    int eb = rescale_exponent(abs1(w), abs1(ip));
+    if (eb > 0) {
+        const double f = ldexp(1.0, -eb);
         w = {w.r * f, w.i * f};
         x2 = {x2.r * f, x2.i * f};
-        out.e += e;
+        out.e += eb;
     }
-    out.x1 = cdiv(w, p);
+    out.x1 = cmul(w, ip);
     out.x2 = x2;
     return out;
 }
 
 // ---------------------------------------------------------------------------------------
-// k_solve_real: one step (row block [i0, i0+nb)) for all active columns.
+// k_solve_real: one step (row block [i0, i0+nb)) for all active eigenvalue units.
 //
 // X is the working matrix in row-major layout X[row * ldx + col] (rows = vector components,
-// columns = packed eigenvector columns).  Rows above the current block hold the running
+// columns = packed working columns).  Rows above the current block hold the running
 // right-hand sides, rows of the block hold RHS on entry and the solution on exit.
 // W receives the update operands of the block's group, row-major W[k * ldw + col]:
-// rows w_s + r hold -a * x_r, rows w_p + r hold (b x)_r (own part), so that
+// rows w_s + r hold -a * x_r, rows w_p + r hold (b x)_r, so that
 // X_above := sig * X_above + [S_w | P_w] * W.
 //
-// Four lanes (a quad) serve one column: lane j of the quad keeps rows r = 4 i + j of the
-// block in registers.  Rows are solved bottom-up; the owner of row r broadcasts its value
-// to the quad (DPP quad_perm), every lane of the quad computes x_r, and each lane applies
-// x_r to its own rows above r.  {S,P} of the diagonal block is staged in LDS.  A column
-// whose solution would exceed 2^960 is scaled by a power of two (recorded as exponent).
+// A unit is one eigenvalue: a real one (one column) or a complex pair (two columns, real
+// and imaginary part); its vector is complex in registers (imaginary part zero for real
+// units).  A 16-lane row serves one unit: lane j keeps rows r = 16 i + j of the block.
+// Before the sweep each lane factors the shifted diagonal of its own rows (independent
+// work): 1x1 rows get the reciprocal of (a S_rr - b P_rr) (perturbed to the rounding level
+// when tiny), 2x2 bumps of S (rows r-1, r: bit r-1 of `bumps`) get a pivoted LU,
+// upper row lane: A = multiplier, B = 1 / pivot; lower row lane: A = U12, B = 1 / U22.
+// Rows outside the unit's system get zero coefficients; the unit's own block gets unit
+// coefficients, its entries being preset to the starting vector.  The sweep then runs
+// bottom-up: the owner lanes turn the right-hand side into the solution with these
+// factors, broadcast it in the row (DPP), and every lane updates its rows above; the
+// {S,P} entries of the next column are fetched from LDS one row ahead.
+// A solution entry above 2^958 makes the unit rescale by a power of two (down to 2^446),
+// recorded as an exponent.
 // ---------------------------------------------------------------------------------------
-constexpr int QL = 4;            // lanes per column
-
-template <int SRC> __device__ __forceinline__ double quad_bcast_c(double v)
-{
-    constexpr int ctrl = SRC * 0x55; // quad_perm [SRC, SRC, SRC, SRC]
-    int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
-    int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-
-__device__ __forceinline__ double quad_bcast(double v, int src)
-{
-    switch (src) {
-    case 0:
-        return quad_bcast_c<0>(v);
-    case 1:
-        return quad_bcast_c<1>(v);
-    case 2:
-        return quad_bcast_c<2>(v);
-    default:
-        return quad_bcast_c<3>(v);
-    }
-}
+constexpr int SP_PITCH = 66; // LDS column pitch (rows) of the staged diagonal block
 
 template <int MAXB>
 __global__ __launch_bounds__(256) void k_solve_real(
     const double *__restrict__ S, long lds, const double *__restrict__ P, long ldp, int i0, int nb,
-    double *__restrict__ X, long ldx, double *__restrict__ W, long ldw, const ColDesc *__restrict__ desc,
-    const int2 *__restrict__ tiles, int t_begin, int t_end, int c_begin, int c_end, double *__restrict__ sig,
-    int *__restrict__ colexp, int *__restrict__ blkexp, int *__restrict__ flag, int *__restrict__ gflag, int w_s, int w_p,
-    double *__restrict__ sig_grp, int first_in_group)
+    unsigned long long bumps, double *__restrict__ X, long ldx, double *__restrict__ W, long ldw,
+    const UnitDesc *__restrict__ units, const int2 *__restrict__ tiles, int t_begin, int t_end, int u_begin,
+    int u_end, double *__restrict__ sig, int *__restrict__ colexp, int *__restrict__ blkexp, int *__restrict__ flag,
+    int *__restrict__ gflag, int w_s, int w_p, double *__restrict__ sig_grp, int first_in_group)
 {
+    static_assert(MAXB <= 65 && SP_PITCH >= MAXB + 1 && (SP_PITCH % 2) == 0, "block layout");
     constexpr int QR = (MAXB + QL - 1) / QL; // rows per lane
-    __shared__ double2 sp[MAXB * MAXB];
-    // stage {S,P} of the diagonal block (zero outside nb x nb); 8 loads in flight per thread
-    for (int q0 = threadIdx.x; q0 < MAXB * MAXB; q0 += 8 * blockDim.x) {
-        double2 t[8];
-#pragma clang loop unroll(full)
-        for (int jj = 0; jj < 8; ++jj) {
-            int q = q0 + jj * blockDim.x;
-            int l = q % MAXB, r = q / MAXB;
-            t[jj] = make_double2(0.0, 0.0);
-            if (q < MAXB * MAXB && l < nb && r < nb) {
-                t[jj] = make_double2(S[(i0 + l) + (long)(i0 + r) * lds], P[(i0 + l) + (long)(i0 + r) * ldp]);
-            }
-        }
-#pragma clang loop unroll(full)
-        for (int jj = 0; jj < 8; ++jj) {
-            int q = q0 + jj * blockDim.x;
-            if (q < MAXB * MAXB) {
-                sp[q] = t[jj];
-            }
-        }
-    }
-    __syncthreads();
+    __shared__ double2 sp[SP_PITCH * MAXB];   // sp[l + r * SP_PITCH] = {S_lr, P_lr}
 
     const int wave = threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
     const int j = lane & (QL - 1);
     const int cq = lane / QL;
-    const int t = t_begin + blockIdx.x * (blockDim.x / WAVE) + wave;
-    if (t >= t_end) {
-        return;
-    }
+    const int t = min(t_begin + (int)(blockIdx.x * (blockDim.x / WAVE)) + wave, t_end - 1);
+    const bool wave_live = t_begin + (int)(blockIdx.x * (blockDim.x / WAVE)) + wave < t_end;
     const int2 tl = tiles[t];
-    const int p = tl.x + cq;
-    const bool act = cq < tl.y && p >= c_begin && p < c_end;
-    const ColDesc cd = desc[act ? p : tl.x];
-    const bool is_im = cd.flags & 1;
-    const bool is_pair = act && (cd.flags & 2) != 0;
-    const int init = (cd.flags >> 2) & 3;
-    const int plane = act ? lane + QL * (cd.partner - p) : lane;
-    const bool wave_pairs = __any(is_pair);
-    const double a = cd.a, bR = cd.bR, bI = cd.bI;
-    const double bIs = is_im ? bI : -bI;
+    const int u = tl.x + cq;
+    const bool act = wave_live && cq < tl.y && u >= u_begin && u < u_end;
+    const UnitDesc ud = units[act ? u : tl.x];
+    const int col = ud.col;
+    const bool pair = ud.w == 2;
+    const double a = ud.a, bR = ud.bR, bI = ud.bI;
     const double bn1 = fabs(bR) + fabs(bI);
-    const int dl = act ? cd.d - i0 : -1 - MAXB; // inactive columns: every row is "fixed zero"
-    const int lol = cd.lo - i0;
+    const int dl = act ? ud.d - i0 : -1 - MAXB; // inactive units: every row is a zero row
+    const int lol = ud.lo - i0;
 
-    // initial values of the column's own diagonal block rows (complex)
-    cpx x0 = {0.0, 0.0}, x1v = {0.0, 0.0};
-    if (act && dl >= 0 && dl < nb) {
-        if (init == INIT_UNIT) {
-            x0 = {1.0, 0.0};
-        }
-        else if (init == INIT_PAIR_ROW1 && dl + 1 < nb) {
-            double2 e11 = sp[dl + dl * MAXB], e12 = sp[dl + (dl + 1) * MAXB];
-            x0 = {-(a * e12.x - bR * e12.y), bI * e12.y};
-            x1v = {a * e11.x - bR * e11.y, -bI * e11.y};
-        }
-        else if (init == INIT_PAIR_ROW2 && dl + 1 < nb) {
-            double2 e22 = sp[(dl + 1) + (dl + 1) * MAXB], e21 = sp[(dl + 1) + dl * MAXB];
-            x0 = {a * e22.x - bR * e22.y, -bI * e22.y};
-            x1v = {-a * e21.x, 0.0};
-        }
-    }
-
-    double vr[QR];
+    // right-hand side of the unit's rows in the block (loads issued before the staging)
+    cpx vr[QR];
 #pragma clang loop unroll(full)
     for (int i = 0; i < QR; ++i) {
         const int r = QL * i + j;
-        vr[i] = (r < nb && r < dl && r >= lol) ? X[(long)(i0 + r) * ldx + p] : 0.0;
+        const bool ld = r < nb && r < dl && r >= lol;
+        const double *xp = X + (long)(i0 + r) * ldx + col;
+        vr[i].r = ld ? xp[0] : 0.0;
+        vr[i].i = (ld && pair) ? xp[1] : 0.0;
     }
 
-    int my_exp = 0;
-    bool pre = false;    // row r was solved together with row r+1 (2x2 bump)
-    double pre_xp = 0.0; // partner part of that row's solution
+    // stage {S,P} of the diagonal block: item (r, pair t) holds rows 2t, 2t+1 of column r
+    {
+        constexpr int NPAIR = SP_PITCH / 2;
+        constexpr int NITEM = NPAIR * MAXB;
+        constexpr int BATCH = 12;
+        for (int q0 = threadIdx.x; q0 < NITEM; q0 += BATCH * blockDim.x) {
+            double2 ts[BATCH], tp[BATCH];
 #pragma clang loop unroll(full)
-    for (int r = MAXB - 1; r >= 0; --r) {
+            for (int k = 0; k < BATCH; ++k) {
+                const int q = q0 + k * blockDim.x;
+                const int r = q / NPAIR, l = 2 * (q % NPAIR);
+                ts[k] = make_double2(0.0, 0.0);
+                tp[k] = make_double2(0.0, 0.0);
+                if (q < NITEM && r < nb && l < nb) {
+                    const double *sa = S + (i0 + l) + (long)(i0 + r) * lds;
+                    const double *pa = P + (i0 + l) + (long)(i0 + r) * ldp;
+                    if (l + 1 < nb) {
+                        ts[k] = *(const double2 *)sa;
+                        tp[k] = *(const double2 *)pa;
+                    }
+                    else { // last row of an odd-length column
+                        ts[k] = make_double2(sa[0], 0.0);
+                        tp[k] = make_double2(pa[0], 0.0);
+                    }
+                }
+            }
+#pragma clang loop unroll(full)
+            for (int k = 0; k < BATCH; ++k) {
+                const int q = q0 + k * blockDim.x;
+                const int r = q / NPAIR, l = 2 * (q % NPAIR);
+                if (q < NITEM) {
+                    sp[l + r * SP_PITCH] = make_double2(ts[k].x, tp[k].x);
+                    sp[l + 1 + r * SP_PITCH] = make_double2(ts[k].y, tp[k].y);
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // starting vector on the unit's own block
+    if (dl >= 0 && dl < nb) {
+        cpx x0 = {0.0, 0.0}, x1v = {0.0, 0.0};
+        if (ud.init == INIT_UNIT) {
+            x0 = {1.0, 0.0};
+        }
+        else if (ud.init == INIT_PAIR_ROW1 && dl + 1 < nb) {
+            double2 e11 = sp[dl + dl * SP_PITCH], e12 = sp[dl + (dl + 1) * SP_PITCH];
+            x0 = {-(a * e12.x - bR * e12.y), bI * e12.y};
+            x1v = {a * e11.x - bR * e11.y, -bI * e11.y};
+        }
+        else if (ud.init == INIT_PAIR_ROW2 && dl + 1 < nb) {
+            double2 e22 = sp[(dl + 1) + (dl + 1) * SP_PITCH], e21 = sp[(dl + 1) + dl * SP_PITCH];
+            x0 = {a * e22.x - bR * e22.y, -bI * e22.y};
+            x1v = {-a * e21.x, 0.0};
+        }
+#pragma clang loop unroll(full)
+        for (int i = 0; i < QR; ++i) {
+            const int r = QL * i + j;
+            if (r == dl) {
+                vr[i] = x0;
+            }
+            else if (r == dl + 1 && ud.init != INIT_UNIT) {
+                vr[i] = x1v;
+            }
+        }
+    }
+
+    // factor the shifted diagonal of this lane's rows
+    cpx fa[QR], fb[QR];
+    unsigned swbits = 0;
+#pragma clang loop unroll(full)
+    for (int i = 0; i < QR; ++i) {
+        const int r = QL * i + j;
+        fa[i] = {0.0, 0.0};
+        fb[i] = {0.0, 0.0};
         if (r < nb) {
-            const int jr = r & (QL - 1), ir = r / QL;
-            double xs = 0.0, xp = 0.0;
-            bool bump_now = false;
-            int e = 0;
-            Bump2 bs;
-            cpx xsol = {0.0, 0.0};
-            const bool from_pre = pre;
-            if (pre) {
-                xs = quad_bcast(vr[ir], jr);
-                xp = pre_xp;
-                pre = false;
-            }
-            else if (r >= 1 && sp[r + (r - 1) * MAXB].x != 0.0) {
-                // 2x2 bump at rows (r-1, r)
-                bump_now = true;
-                const int jr1 = (r - 1) & (QL - 1), ir1 = (r - 1) / QL;
-                double o1 = quad_bcast(vr[ir1], jr1), o2 = quad_bcast(vr[ir], jr);
-                double q1 = wave_pairs ? __shfl(o1, plane) : 0.0;
-                double q2 = wave_pairs ? __shfl(o2, plane) : 0.0;
-                cpx r1 = is_im ? cpx{q1, o1} : cpx{o1, is_pair ? q1 : 0.0};
-                cpx r2 = is_im ? cpx{q2, o2} : cpx{o2, is_pair ? q2 : 0.0};
-                double2 e11 = sp[(r - 1) + (r - 1) * MAXB], e12 = sp[(r - 1) + r * MAXB];
-                double2 e21 = sp[r + (r - 1) * MAXB], e22 = sp[r + r * MAXB];
-                cpx c11 = {a * e11.x - bR * e11.y, -bI * e11.y};
-                cpx c12 = {a * e12.x - bR * e12.y, -bI * e12.y};
-                cpx c21 = {a * e21.x - bR * e21.y, -bI * e21.y};
-                cpx c22 = {a * e22.x - bR * e22.y, -bI * e22.y};
-                double dm1 = fmax(kEps * (fabs(a) * (fabs(e11.x) + fabs(e21.x)) + bn1 * fabs(e11.y)), kSafeMin);
-                double dm2 = fmax(kEps * (fabs(a) * (fabs(e12.x) + fabs(e22.x)) + bn1 * fabs(e22.y)), kSafeMin);
-                bs = bump_solve(r1, r2, c11, c12, c21, c22, dm1, dm2);
-                const bool solve1 = (r - 1) < dl && (r - 1) >= lol;
-                const bool solve2 = r < dl && r >= lol;
-                e = (solve1 || solve2) ? bs.e : 0;
-            }
-            else {
-                double o = quad_bcast(vr[ir], jr);
-                double qv = wave_pairs ? __shfl(o, plane) : 0.0;
-                cpx rhs = is_im ? cpx{qv, o} : cpx{o, is_pair ? qv : 0.0};
-                double2 ee = sp[r + r * MAXB];
+            const bool lower = r >= 1 && ((bumps >> (r - 1)) & 1ull);
+            const bool upper = r + 1 < nb && ((bumps >> r) & 1ull);
+            if (!lower && !upper) {
+                const double2 ee = sp[r + r * SP_PITCH];
                 cpx c = {a * ee.x - bR * ee.y, -bI * ee.y};
-                double dmin = fmax(kEps * (fabs(a) * fabs(ee.x) + bn1 * fabs(ee.y)), kSafeMin);
+                const double dmin = fmax(kEps * (fabs(a) * fabs(ee.x) + bn1 * fabs(ee.y)), kSafeMin);
                 if (abs1(c) < dmin) {
                     c = {dmin, 0.0};
                 }
                 const bool solve = r < dl && r >= lol;
-                // |x| <= |rhs|_1 / |c|_1 * 2 must stay below 2^960
-                if (solve && abs1(rhs) > 0x1p958 * abs1(c)) {
-                    e = scale_exponent(abs1(rhs), abs1(c), kLog2Big);
-                }
-                if (e != 0) {
-                    double f = ldexp(1.0, -e);
-                    rhs = {rhs.r * f, rhs.i * f};
-                }
-                xsol = solve ? cdiv(rhs, c) : (r == dl ? x0 : (r == dl + 1 ? x1v : cpx{0.0, 0.0}));
+                fa[i] = solve ? crecip(c) : (r == dl ? cpx{1.0, 0.0} : cpx{0.0, 0.0});
             }
-            if (__any(e != 0)) {
-                // rescale the whole column by 2^-e (rare)
-                const double f = ldexp(1.0, -e);
-#pragma clang loop unroll(full)
-                for (int i = 0; i < QR; ++i) {
-                    vr[i] *= f;
+            else {
+                const int r1 = lower ? r - 1 : r, r2 = r1 + 1;
+                const double2 e11 = sp[r1 + r1 * SP_PITCH], e12 = sp[r1 + r2 * SP_PITCH];
+                const double2 e21 = sp[r2 + r1 * SP_PITCH], e22 = sp[r2 + r2 * SP_PITCH];
+                cpx c11 = {a * e11.x - bR * e11.y, -bI * e11.y};
+                cpx c12 = {a * e12.x - bR * e12.y, -bI * e12.y};
+                cpx c21 = {a * e21.x - bR * e21.y, -bI * e21.y};
+                cpx c22 = {a * e22.x - bR * e22.y, -bI * e22.y};
+                const double dm1 = fmax(kEps * (fabs(a) * (fabs(e11.x) + fabs(e21.x)) + bn1 * fabs(e11.y)), kSafeMin);
+                const double dm2 = fmax(kEps * (fabs(a) * (fabs(e12.x) + fabs(e22.x)) + bn1 * fabs(e22.y)), kSafeMin);
+                const bool s1 = r1 < dl && r1 >= lol, s2 = r2 < dl && r2 >= lol;
+                bool sw = false;
+                cpx mlt = {0.0, 0.0}, q = {0.0, 0.0}, iu = {0.0, 0.0}, ip = {0.0, 0.0};
+                if (r1 == dl) {
+                    // the unit's own 2x2 block: entries preset, x = rhs
+                    iu = {1.0, 0.0};
+                    ip = {1.0, 0.0};
                 }
-                my_exp -= e;
-            }
-            if (bump_now) {
-                const bool solve1 = (r - 1) < dl && (r - 1) >= lol;
-                const bool solve2 = r < dl && r >= lol;
-                cpx s1 = solve1 ? bs.x1 : ((r - 1) == dl ? x0 : ((r - 1) == dl + 1 ? x1v : cpx{0.0, 0.0}));
-                cpx s2 = solve2 ? bs.x2 : (r == dl ? x0 : (r == dl + 1 ? x1v : cpx{0.0, 0.0}));
-                const int jr1 = (r - 1) & (QL - 1), ir1 = (r - 1) / QL;
-                if (j == jr1) {
-                    vr[ir1] = is_im ? s1.i : s1.r;
+                else if (s1) {
+                    sw = abs1(c21) > abs1(c11);
+                    cpx pv = sw ? c21 : c11, sv = sw ? c11 : c21, tv = sw ? c12 : c22;
+                    q = sw ? c22 : c12;
+                    if (abs1(pv) < dm1) {
+                        pv = {dm1, 0.0};
+                    }
+                    ip = crecip(pv);
+                    mlt = cmul(sv, ip);
+                    cpx uv = csub(tv, cmul(mlt, q));
+                    if (abs1(uv) < dm2) {
+                        uv = {dm2, 0.0};
+                    }
+                    iu = crecip(uv);
                 }
-                pre_xp = is_im ? s1.r : s1.i;
-                xs = is_im ? s2.i : s2.r;
-                xp = is_im ? s2.r : s2.i;
-                pre = true;
-            }
-            else if (!from_pre) {
-                xs = is_im ? xsol.i : xsol.r;
-                xp = is_im ? xsol.r : xsol.i;
-            }
-            if (j == jr) {
-                vr[ir] = xs;
-            }
-            const double ax = a * xs;
-            const double bx = bR * xs + bIs * xp;
-            // this lane's rows l = 4 i + j above r (row r-1 excluded after a bump)
-#pragma clang loop unroll(full)
-            for (int i = 0; i < QR; ++i) {
-                if (QL * i < r) {
-                    const int l = QL * i + j;
-                    const bool upd = l < r && !(bump_now && l == r - 1);
-                    double2 el = sp[l + r * MAXB];
-                    double nv = fma(-el.x, ax, vr[i]);
-                    nv = fma(el.y, bx, nv);
-                    vr[i] = upd ? nv : vr[i];
+                else if (s2) {
+                    // upper row outside the system (zero): x2 = rhs2 / c22
+                    if (abs1(c22) < dm2) {
+                        c22 = {dm2, 0.0};
+                    }
+                    iu = crecip(c22);
                 }
+                fa[i] = lower ? q : mlt;
+                fb[i] = lower ? iu : ip;
+                swbits |= sw ? (1u << i) : 0u;
             }
         }
     }
 
-    // W of this block at rows w_s + r (S part) and w_p + r (P part) of the group's operand
+    int my_exp = 0;
+    bool pre = false; // row r was solved with row r+1 (2x2 bump)
+    const cpx bv = {bR, bI};
+    double2 ec[QR], en[QR]; // {S,P} of this lane's rows in column r (ec) and r-1 (en)
+#pragma clang loop unroll(full)
+    for (int i = 0; i < QR; ++i) {
+        ec[i] = make_double2(0.0, 0.0);
+        en[i] = make_double2(0.0, 0.0);
+    }
+#pragma clang loop unroll(full)
+    for (int r = MAXB - 1; r >= 0; --r) {
+        if (r < nb) {
+            if (r == nb - 1) {
+#pragma clang loop unroll(full)
+                for (int i = 0; i < QR; ++i) {
+                    if (QL * i < r) {
+                        ec[i] = sp[QL * i + j + r * SP_PITCH];
+                    }
+                }
+            }
+            if (r >= 1) {
+#pragma clang loop unroll(full)
+                for (int i = 0; i < QR; ++i) {
+                    if (QL * i < r - 1) {
+                        en[i] = sp[QL * i + j + (r - 1) * SP_PITCH];
+                    }
+                }
+            }
+            if (pre) {
+                pre = false;
+            }
+            else if (r >= 1 && ((bumps >> (r - 1)) & 1ull)) {
+                // 2x2 bump at rows (r-1, r)
+                const int j1 = (r - 1) & (QL - 1), i1 = (r - 1) / QL;
+                const int j2 = r & (QL - 1), i2 = r / QL;
+                const cpx rhs1 = qbc(vr[i1], j1), rhs2 = qbc(vr[i2], j2);
+                const bool sw1 = (swbits >> i1) & 1u, sw2 = (swbits >> i2) & 1u;
+                // upper-row lane: y = rs - mlt rp
+                cpx y = csub(sw1 ? rhs1 : rhs2, cmul(fa[i1], sw1 ? rhs2 : rhs1));
+                y = qbc(y, j1);
+                // lower-row lane: x2 = y / U22, w = rp - U12 x2
+                cpx x2 = cmul(y, fb[i2]);
+                cpx w = csub(sw2 ? rhs2 : rhs1, cmul(fa[i2], x2));
+                x2 = qbc(x2, j2);
+                w = qbc(w, j2);
+                // upper-row lane: x1 = w / pivot
+                cpx x1 = qbc(cmul(w, fb[i1]), j1);
+                const bool bad = !(abs1(x1) <= kScaleTrigger && abs1(x2) <= kScaleTrigger);
+                if (__any(bad)) {
+                    const cpx mlt = qbc(fa[i1], j1), ip = qbc(fb[i1], j1);
+                    const cpx q = qbc(fa[i2], j2), iu = qbc(fb[i2], j2);
+                    const bool sw = qbi((int)sw1, j1) != 0;
+                    Bump2 bs = bump_solve_scaled(rhs1, rhs2, sw, mlt, q, iu, ip);
+                    const int e = bad ? bs.e : 0;
+                    if (bad) {
+                        x1 = bs.x1;
+                        x2 = bs.x2;
+                    }
+                    const double f = ldexp(1.0, -e);
+#pragma clang loop unroll(full)
+                    for (int i = 0; i < QR; ++i) {
+                        vr[i] = {vr[i].r * f, vr[i].i * f};
+                    }
+                    my_exp -= e;
+                }
+                if (j == j1) {
+                    vr[i1] = x1;
+                }
+                if (j == j2) {
+                    vr[i2] = x2;
+                }
+                const cpx ax1 = {a * x1.r, a * x1.i}, bx1 = cmul(bv, x1);
+                const cpx ax2 = {a * x2.r, a * x2.i}, bx2 = cmul(bv, x2);
+#pragma clang loop unroll(full)
+                for (int i = 0; i < QR; ++i) {
+                    if (QL * i < r - 1) {
+                        const int l = QL * i + j;
+                        const double2 e1 = en[i];
+                        const double2 e2 = ec[i];
+                        double nr = fma(-e1.x, ax1.r, vr[i].r);
+                        double ni = fma(-e1.x, ax1.i, vr[i].i);
+                        nr = fma(e1.y, bx1.r, nr);
+                        ni = fma(e1.y, bx1.i, ni);
+                        nr = fma(-e2.x, ax2.r, nr);
+                        ni = fma(-e2.x, ax2.i, ni);
+                        nr = fma(e2.y, bx2.r, nr);
+                        ni = fma(e2.y, bx2.i, ni);
+                        const bool upd = l < r - 1;
+                        vr[i].r = upd ? nr : vr[i].r;
+                        vr[i].i = upd ? ni : vr[i].i;
+                    }
+                }
+                pre = true;
+            }
+            else {
+                // 1x1 row
+                const int jr = r & (QL - 1), ir = r / QL;
+                cpx x = qbc(cmul(vr[ir], fa[ir]), jr);
+                const bool bad = !(abs1(x) <= kScaleTrigger);
+                if (__any(bad)) {
+                    const int eo = bad ? rescale_exponent(abs1(vr[ir]), abs1(fa[ir])) : 0;
+                    const int e = qbi(eo, jr);
+                    const double f = ldexp(1.0, -e);
+#pragma clang loop unroll(full)
+                    for (int i = 0; i < QR; ++i) {
+                        vr[i] = {vr[i].r * f, vr[i].i * f};
+                    }
+                    my_exp -= e;
+                    x = qbc(cmul(vr[ir], fa[ir]), jr);
+                }
+                if (j == jr) {
+                    vr[ir] = x;
+                }
+                const cpx ax = {a * x.r, a * x.i}, bx = cmul(bv, x);
+#pragma clang loop unroll(full)
+                for (int i = 0; i < QR; ++i) {
+                    if (QL * i < r) {
+                        const int l = QL * i + j;
+                        const double2 el = ec[i];
+                        double nr = fma(-el.x, ax.r, vr[i].r);
+                        double ni = fma(-el.x, ax.i, vr[i].i);
+                        nr = fma(el.y, bx.r, nr);
+                        ni = fma(el.y, bx.i, ni);
+                        const bool upd = l < r;
+                        vr[i].r = upd ? nr : vr[i].r;
+                        vr[i].i = upd ? ni : vr[i].i;
+                    }
+                }
+            }
+#pragma clang loop unroll(full)
+            for (int i = 0; i < QR; ++i) {
+                ec[i] = en[i];
+            }
+        }
+    }
+
+    // X and W of this block: rows w_s + r (S part: -a x) and w_p + r (P part: b x)
 #pragma clang loop unroll(full)
     for (int i = 0; i < QR; ++i) {
         const int r = QL * i + j;
-        const double o = vr[i];
-        const double q = wave_pairs ? __shfl(o, plane) : 0.0;
         if (act && r < nb) {
-            X[(long)(i0 + r) * ldx + p] = o;
-            W[(long)(w_s + r) * ldw + p] = -a * o;
-            W[(long)(w_p + r) * ldw + p] = bR * o + bIs * (is_pair ? q : 0.0);
+            const cpx v = vr[i];
+            const cpx bx = cmul(bv, v);
+            double *xr = X + (long)(i0 + r) * ldx + col;
+            double *ws = W + (long)(w_s + r) * ldw + col;
+            double *wp = W + (long)(w_p + r) * ldw + col;
+            xr[0] = v.r;
+            ws[0] = -a * v.r;
+            wp[0] = bx.r;
+            if (pair) {
+                xr[1] = v.i;
+                ws[1] = -a * v.i;
+                wp[1] = bx.i;
+            }
         }
     }
-    if (act && j == 0) {
+    if (act && j < ud.w) {
+        // lane 0: real part column, lane 1: imaginary part column
+        const int p = col + j;
         const double sg = ldexp(1.0, my_exp);
         sig[p] = sg;
         // cumulative scale over the group's steps (the group's far update applies it);
-        // a column starting in this block has no earlier steps in the group
+        // a unit starting in this block has no earlier steps in the group
         const bool starts_here = dl >= 0 && dl < nb;
         sig_grp[p] = (first_in_group || starts_here) ? sg : sig_grp[p] * sg;
-        int ce = colexp[p] + my_exp;
+        const int ce = colexp[p] + my_exp;
         colexp[p] = ce;
         blkexp[p] = ce;
         if (my_exp != 0) {
@@ -774,9 +925,9 @@ __global__ void k_zero_int(int *__restrict__ X, long count)
 } // namespace xtg
 
 namespace xtg {
-template __global__ void k_solve_real<65>(const double *, long, const double *, long, int, int, double *, long,
-                                          double *, long, const ColDesc *, const int2 *, int, int, int, int,
-                                          double *, int *, int *, int *, int *, int, int, double *, int);
+template __global__ void k_solve_real<65>(const double *, long, const double *, long, int, int, unsigned long long,
+                                          double *, long, double *, long, const UnitDesc *, const int2 *, int, int,
+                                          int, int, double *, int *, int *, int *, int *, int, int, double *, int);
 } // namespace xtg
 
 // =======================================================================================
