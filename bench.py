@@ -221,7 +221,7 @@ def main():
         step()
         torch.cuda.synchronize()
         res, ncols = residual_sample(S, P, ar, ai, be, dZ, dVR)
-        log(f"[bench] residual (metric A) over {ncols} sampled columns: {res:.3f} n*eps")
+        log(f"[bench] residual (metric A) over {ncols} sampled columns: {res:.3e} n*eps")
 
     cpu = None
     if not args.no_cpu:
@@ -246,7 +246,7 @@ def main():
         "config": {"workload": f"dtgevc3 side=R howmny=B n={n} plain FP64, quasi-triangular pencil "
                                f"(40% 2x2 pairs), Z orthogonal", "n": n, "side": "R", "howmny": "B",
                    "form": "plain-fp64", "path_tflops": round(path_tflops, 3),
-                   "max_residual_n_eps": None if res is None else round(res, 4)},
+                   "max_residual_n_eps": None if res is None else float(f"{res:.4g}")},
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},

@@ -563,19 +563,17 @@ __global__ void k_zero_cols(double *__restrict__ W, long ldw, int rows, int c0, 
 //   mode 0: C = D;  mode 1: C = sig[b] * C + D.
 //   kmax (optional): per-a upper bound of k (exclusive); a tile uses the max over its rows,
 //   so triangular operands skip their zero part.
-// Workgroup: 256 threads (2x2 waves of 64x64), tile 128 (a) x 128 (b), K chunks of 16 rows
-// double-buffered in LDS; the next chunk is fetched into registers while the current one
-// feeds the MFMAs.  Each segment is padded to whole chunks with zero rows (masked at the
+// Workgroup: 256 threads (2x2 waves), tile GT (a) x GT (b) (each wave GT/2 x GT/2), K chunks
+// of GK rows double-buffered in LDS; the next chunk is fetched into registers while the
+// current one feeds the MFMAs.  GT = 128, GK = 16 for large products; GT = 64, GK = 32 when
+// 128-tiles would leave CUs idle (the per-step updates of a few block rows).  Each segment is padded to whole chunks with zero rows (masked at the
 // LDS store, so no wait on the loads precedes the MFMAs).  Columns a >= M, b >= N are
 // clamped to the last one: they only feed outputs that are never stored.
 // WIDE: 16-byte loads of column pairs (M, N >= 2); otherwise one double per load.
 // ---------------------------------------------------------------------------------------
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-constexpr int GT = 128; // tile edge
-constexpr int GK = 16;  // K chunk
-
-template <int TAG, bool WIDE>
+template <int TAG, bool WIDE, int GT, int GK>
 __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, const double *__restrict__ L, long ldl,
                                                   const double *__restrict__ L2, long ldl2,
                                                   const double *__restrict__ R, long ldr, double *__restrict__ C,
@@ -596,8 +594,8 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
     const int tid = threadIdx.x;
     const int wave = tid / WAVE;
     const int lane = tid % WAVE;
-    const int wa = (wave >> 1) * 64; // wave offset in a
-    const int wb = (wave & 1) * 64;  // wave offset in b
+    const int wa = (wave >> 1) * (GT / 2); // wave offset in a
+    const int wb = (wave & 1) * (GT / 2);  // wave offset in b
 
     int K = K1 + K2;
     if (kmax != nullptr) {
@@ -617,11 +615,14 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
     const int nch1 = (rows1 + GK - 1) / GK;
     const int nchunks = nch1 + (rows2 + GK - 1) / GK;
 
-    double4_t acc[4][4];
+    constexpr int WT = GT / 2;  // wave tile edge
+    constexpr int FT = WT / 16; // 16x16 MFMA blocks per wave tile edge
+    static_assert(GK % 4 == 0 && (GT == 64 || GT == 128), "tile shape");
+    double4_t acc[FT][FT];
 #pragma clang loop unroll(full)
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < FT; ++i) {
 #pragma clang loop unroll(full)
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < FT; ++j) {
             acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
         }
     }
@@ -702,16 +703,16 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
         }
 #pragma clang loop unroll(full)
         for (int ks = 0; ks < GK; ks += 4) {
-            double af[4], bf[4];
+            double af[FT], bf[FT];
 #pragma clang loop unroll(full)
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < FT; ++i) {
                 af[i] = Ls[buf][ks + fk][wa + i * 16 + fr];
                 bf[i] = Rs[buf][ks + fk][wb + i * 16 + fr];
             }
 #pragma clang loop unroll(full)
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < FT; ++i) {
 #pragma clang loop unroll(full)
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < FT; ++j) {
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
                 }
             }
@@ -724,7 +725,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
 
     // epilogue: lane holds D[lane/16 + 4*v][lane%16] of each 16x16 tile (v_mfma_f64_16x16x4 layout)
 #pragma clang loop unroll(full)
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < FT; ++i) {
 #pragma clang loop unroll(full)
         for (int vv = 0; vv < 4; ++vv) {
             int ar = a0 + wa + i * 16 + fk + 4 * vv;
@@ -733,7 +734,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
             }
             long rowoff = (long)(outcol ? outcol[ar] : ar) * ldc;
 #pragma clang loop unroll(full)
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < FT; ++j) {
                 int bc = b0 + wb + j * 16 + fr;
                 if (bc < N) {
                     double d = acc[i][j][vv];
@@ -1085,17 +1086,32 @@ void launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, cons
                      long ldl2, const double *R, long ldr, double *C, long ldc, const int *outcol, int mode,
                      const double *sig, const int *kmax, const double *R2)
 {
-    dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
     const bool wide = M >= 2 && N >= 2;
+    // 128-tiles unless they would leave most CUs idle
+    const long tiles128 = (long)((M + 127) / 128) * ((N + 127) / 128);
+    const bool small = tiles128 < 256;
+    const int gt = small ? 64 : 128;
+    dim3 grid((N + gt - 1) / gt, (M + gt - 1) / gt);
+#define XTG_GEMM_LAUNCH(T, WD, G, K)                                                                                 \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, WD, G, K>), grid, dim3(256), 0, st, M, N, K1, K2, L, ldl, L2,   \
+                       ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2)
 #define XTG_GEMM_CASE(T)                                                                                             \
     if (tag == T) {                                                                                                  \
-        if (wide) {                                                                                                  \
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, true>), grid, dim3(256), 0, st, M, N, K1, K2, L, ldl, L2, \
-                               ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2);                                   \
+        if (small) {                                                                                                 \
+            if (wide) {                                                                                              \
+                XTG_GEMM_LAUNCH(T, true, 64, 32);                                                                    \
+            }                                                                                                        \
+            else {                                                                                                   \
+                XTG_GEMM_LAUNCH(T, false, 64, 32);                                                                   \
+            }                                                                                                        \
         }                                                                                                            \
         else {                                                                                                       \
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, false>), grid, dim3(256), 0, st, M, N, K1, K2, L, ldl,  \
-                               L2, ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2);                               \
+            if (wide) {                                                                                              \
+                XTG_GEMM_LAUNCH(T, true, 128, 16);                                                                   \
+            }                                                                                                        \
+            else {                                                                                                   \
+                XTG_GEMM_LAUNCH(T, false, 128, 16);                                                                  \
+            }                                                                                                        \
         }                                                                                                            \
         return;                                                                                                      \
     }
@@ -1103,5 +1119,6 @@ void launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, cons
     XTG_GEMM_CASE(1)
     XTG_GEMM_CASE(2)
 #undef XTG_GEMM_CASE
+#undef XTG_GEMM_LAUNCH
 }
 } // namespace xtg
