@@ -47,7 +47,7 @@ EXPORTED_SYMBOLS = [
     "xtgevc3_profile_enable", "xtgevc3_profile_read", "xtgevc3_dtgevc3_backward_device",
 ]
 
-PROF_KINDS = ("solve", "update", "back", "other")
+PROF_KINDS = ("solve", "update", "back", "other", "update_t128")
 
 
 def profile_enable(on=True):

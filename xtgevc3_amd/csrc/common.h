@@ -42,6 +42,16 @@ struct UnitDesc {
     int pad1, pad2, pad3;
 };
 
+// The row blocks of one update group of the real sweep, top-down: block j covers local rows
+// [off[j], off[j+1]) of the group (off[nblk] = wr, unused entries = wr) and is solved in step
+// step[j]; step[0] is the group's last step.
+struct GroupInfo {
+    int off[5];
+    int step[4];
+    int wr;
+    int last; // the launch is the group's last step
+};
+
 // Same layout for the complex interface: the working vector entries are complex.
 struct ZColDesc {
     double aR, aI; // acoeff (complex for the complex interface)

@@ -23,6 +23,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/xtgevc3.h"
 #include "common.h"
 #include "driver.h"
 
@@ -31,10 +32,10 @@ namespace xtg {
 template <int MAXB>
 __global__ void k_solve_real(const double *, long, const double *, long, int, int, unsigned long long, double *, long,
                              double *, long, const UnitDesc *, const int2 *, int, int, int, int, double *, int *, int *,
-                             int *, int *, int, int, double *, int);
+                             int *, int, int, double *, int, GroupInfo);
 constexpr int SOLVE_UNITS = 4;   // eigenvalue units per wave of the solve kernel (16 lanes per unit)
 constexpr int SOLVE_WAVES_H = 4; // waves per solve workgroup (they share the staged block)
-void launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const double *L, long ldl, const double *L2,
+int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const double *L, long ldl, const double *L2,
                      long ldl2, const double *R, long ldr, double *C, long ldc, const int *outcol, int mode,
                      const double *sig, const int *kmax, const double *R2);
 constexpr int GROUP_STEPS = 4; // row blocks per far update (K = 2 * ~256)
@@ -51,8 +52,6 @@ __global__ void k_bwd_diag(const double *, long, const double *, long, int, int,
                            const int *, double *, long, double *, long);
 __global__ void k_subdiag(const double *, long, int, double *);
 __global__ void k_zero(double *, long);
-__global__ void k_w_rescale(double *, long, int, int, int, const int *, long, int4, int4, const int *);
-__global__ void k_zero_cols(double *, long, int, int, int);
 __global__ void k_zero_int(int *, long);
 
 constexpr int MAXB_REAL = 65;  // nominal block 64, +1 when a boundary would split a 2x2 block
@@ -239,7 +238,7 @@ struct DevBuf {
 };
 
 struct RealWork {
-    DevBuf X, W, sig, colexp, blkexp, flag, gflag, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
+    DevBuf X, W, sig, colexp, blkexp, flag, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
     DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp;
 };
 
@@ -312,8 +311,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     ENSURE(ws.colexp, sizeof(int) * (size_t)m);
     ENSURE(ws.blkexp, sizeof(int) * (size_t)nsteps * ldx);
     ENSURE(ws.flag, sizeof(int) * 4);
-    const int ngroups = (nsteps + GROUP_STEPS - 1) / GROUP_STEPS;
-    ENSURE(ws.gflag, sizeof(int) * (size_t)std::max(1, ngroups));
+
     ENSURE(ws.desc, sizeof(UnitDesc) * units.size());
     ENSURE(ws.tiles, sizeof(int2) * tiles.size());
     ENSURE(ws.rowstep, sizeof(int) * (size_t)n);
@@ -325,7 +323,6 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, ws.blkexp.as<int>(), (long)nsteps * ldx); XTG_LAUNCH_CHECK("k_zero_int");
     hipLaunchKernelGGL(k_zero_int, dim3(64), dim3(256), 0, st, ws.colexp.as<int>(), (long)m); XTG_LAUNCH_CHECK("k_zero_int");
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, ws.flag.as<int>(), 4L); XTG_LAUNCH_CHECK("k_zero_int");
-    hipLaunchKernelGGL(k_zero_int, dim3(64), dim3(256), 0, st, ws.gflag.as<int>(), (long)std::max(1, ngroups)); XTG_LAUNCH_CHECK("k_zero_int");
 
     // Steps go bottom-up in groups of GROUP_STEPS row blocks [g0, g1).  Within a group each
     // step updates only the group's rows above it ("near" update, K = 2 nb); after the group
@@ -340,22 +337,20 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         const int wr = g1 - g0;
         bool any_active = false;
         bool first = true;
-        int *gflag = ws.gflag.as<int>() + gs / GROUP_STEPS;
-        if (!adjoint) {
-            // columns whose solve starts inside the group after its first step never write the
-            // operand rows of the blocks below their start: those rows must read as zero
-            int cb_first = c_begin, cb_last = c_begin;
-            while (cb_first > 0 && desc[cb_first - 1].d >= i0s[gs]) {
-                --cb_first;
+        // the group's blocks top-down
+        GroupInfo grp{};
+        grp.wr = wr;
+        {
+            int off = 0;
+            for (int jb = 0; jb < 4; ++jb) {
+                const int sj = ge - 1 - jb;
+                grp.off[jb] = off;
+                grp.step[jb] = sj >= gs ? sj : ge - 1;
+                if (sj >= gs) {
+                    off += nbs[sj];
+                }
             }
-            cb_last = cb_first;
-            while (cb_last > 0 && desc[cb_last - 1].d >= g0) {
-                --cb_last;
-            }
-            if (cb_first > cb_last) {
-                hipLaunchKernelGGL(k_zero_cols, dim3((cb_first - cb_last + 255) / 256, 2 * wr), dim3(256), 0, st, W, ldw,
-                                   2 * wr, cb_last, cb_first - cb_last); XTG_LAUNCH_CHECK("k_zero_cols");
-            }
+            grp.off[4] = off;
         }
         for (int s = gs; s < ge; ++s) {
             const int i0 = i0s[s], nb = nbs[s];
@@ -383,6 +378,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
             const int ntl = t_end - t_begin;
             const int N = c_end - c_begin;
             const int woff = i0 - g0;
+            grp.last = s == ge - 1 ? 1 : 0;
             // 2x2 bumps of S in the block: bit r-1 <=> rows (r-1, r)
             unsigned long long bumps = 0;
             for (int r = 1; r < nb; ++r) {
@@ -396,8 +392,8 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
                                dim3((ntl + wpb - 1) / wpb), dim3(wpb * WAVE), 0, st, Sw,
                                ldsw, Pw, ldpw, i0, nb, bumps, X, ldx, W, ldw, ws.desc.as<UnitDesc>(), ws.tiles.as<int2>(),
                                t_begin, t_end, u_begin, u_end, ws.sig.as<double>(), ws.colexp.as<int>(),
-                               ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), gflag, woff, wr + woff,
-                               ws.sig_grp.as<double>(), first ? 1 : 0); XTG_LAUNCH_CHECK("k_solve_real");
+                               ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff, wr + woff,
+                               ws.sig_grp.as<double>(), first ? 1 : 0, grp); XTG_LAUNCH_CHECK("k_solve_real");
             prof_end(pr, st, 0, 2.0 * nb * nb * N, 8.0 * (4.0 * nb * N + 2.0 * nb * nb));
             first = false;
             // near update: the group's rows above this block
@@ -405,12 +401,13 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
             const int M = i0 - r0;
             if (M > 0) {
                 pr = prof_begin(st);
-                launch_gemm_f64(0, st, M, N, nb, nb, Sw + r0 + (long)i0 * ldsw,
+                const int gt = launch_gemm_f64(0, st, M, N, nb, nb, Sw + r0 + (long)i0 * ldsw,
                                    ldsw, Pw + r0 + (long)i0 * ldpw, ldpw, (const double *)(W + (long)woff * ldw + c_begin),
                                    ldw, X + (long)r0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
                                    (const double *)(ws.sig.as<double>() + c_begin), (const int *)nullptr,
                                    (const double *)(W + (long)(wr + woff) * ldw + c_begin)); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
-                prof_end(pr, st, 1, 2.0 * M * N * (2.0 * nb), 8.0 * (2.0 * M * N + 2.0 * nb * M + 2.0 * nb * N));
+                prof_end(pr, st, gt == 128 ? XTGEVC3_PROF_UPDATE_T128 : XTGEVC3_PROF_UPDATE, 2.0 * M * N * (2.0 * nb),
+                         8.0 * (2.0 * M * N + 2.0 * nb * M + 2.0 * nb * N));
             }
         }
         if (!any_active) {
@@ -421,36 +418,14 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         const int r0 = adjoint ? std::min(desc[c_begin].lo, g0) : 0;
         const int M = g0 - r0;
         if (M > 0 && N > 0) {
-            // bring the group's operands to the columns' final scale (no-op without rescaling)
-            int4 offs = make_int4(0, INT_MAX, INT_MAX, INT_MAX), steps = make_int4(ge - 1, ge - 1, ge - 1, ge - 1);
-            {
-                int off = 0;
-                for (int j = 0; j < ge - gs; ++j) {
-                    const int sj = ge - 1 - j; // blocks top-down
-                    if (j == 1) {
-                        offs.y = off;
-                        steps.y = sj;
-                    }
-                    else if (j == 2) {
-                        offs.z = off;
-                        steps.z = sj;
-                    }
-                    else if (j == 3) {
-                        offs.w = off;
-                        steps.w = sj;
-                    }
-                    off += nbs[sj];
-                }
-            }
-            hipLaunchKernelGGL(k_w_rescale, dim3((N + 255) / 256, 2 * wr), dim3(256), 0, st, W, ldw, wr, c_begin, N,
-                               (const int *)ws.blkexp.as<int>(), ldx, offs, steps, (const int *)gflag); XTG_LAUNCH_CHECK("k_w_rescale");
             int pr = prof_begin(st);
-            launch_gemm_f64(0, st, M, N, wr, wr, Sw + r0 + (long)g0 * ldsw, ldsw,
+            const int gt = launch_gemm_f64(0, st, M, N, wr, wr, Sw + r0 + (long)g0 * ldsw, ldsw,
                                Pw + r0 + (long)g0 * ldpw, ldpw, (const double *)(W + c_begin), ldw,
                                X + (long)r0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
                                (const double *)(ws.sig_grp.as<double>() + c_begin), (const int *)nullptr,
                                (const double *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
-            prof_end(pr, st, 1, 2.0 * M * N * (2.0 * wr), 8.0 * (2.0 * M * N + 2.0 * wr * M + 2.0 * wr * N));
+            prof_end(pr, st, gt == 128 ? XTGEVC3_PROF_UPDATE_T128 : XTGEVC3_PROF_UPDATE, 2.0 * M * N * (2.0 * wr),
+                     8.0 * (2.0 * M * N + 2.0 * wr * M + 2.0 * wr * N));
         }
     }
     XTG_CHECK(hipGetLastError());

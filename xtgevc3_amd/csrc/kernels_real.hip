@@ -160,7 +160,13 @@ __device__ __noinline__ Bump2 bump_solve_scaled(cpx r1, cpx r2, bool sw, cpx mlt
 // factors, broadcast it in the row (DPP), and every lane updates its rows above; the
 // {S,P} entries of the next column are fetched from LDS one row ahead.
 // A solution entry above 2^958 makes the unit rescale by a power of two (down to 2^446),
-// recorded as an exponent.
+// recorded as an exponent (blkexp: the column's exponent after this step).
+// Group bookkeeping of the operands W (rows [0, wr): S part, [wr, 2 wr): P part; the group's
+// blocks top-down at local rows grp.off[j], solved in steps grp.step[j]):
+//   * a unit starting in this block (not the group's first step) zeros its operand rows of
+//     the group's lower blocks, which it never wrote;
+//   * in the group's last (top) step each unit brings its operand rows of the lower blocks
+//     to its final scale: W *= 2^(colexp - blkexp[step of the block]).
 // ---------------------------------------------------------------------------------------
 constexpr int SP_PITCH = 66; // LDS column pitch (rows) of the staged diagonal block
 
@@ -170,7 +176,7 @@ __global__ __launch_bounds__(256) void k_solve_real(
     unsigned long long bumps, double *__restrict__ X, long ldx, double *__restrict__ W, long ldw,
     const UnitDesc *__restrict__ units, const int2 *__restrict__ tiles, int t_begin, int t_end, int u_begin,
     int u_end, double *__restrict__ sig, int *__restrict__ colexp, int *__restrict__ blkexp, int *__restrict__ flag,
-    int *__restrict__ gflag, int w_s, int w_p, double *__restrict__ sig_grp, int first_in_group)
+    int w_s, int w_p, double *__restrict__ sig_grp, int first_in_group, GroupInfo grp)
 {
     static_assert(MAXB <= 65 && SP_PITCH >= MAXB + 1 && (SP_PITCH % 2) == 0, "block layout");
     constexpr int QR = (MAXB + QL - 1) / QL; // rows per lane
@@ -495,6 +501,43 @@ __global__ __launch_bounds__(256) void k_solve_real(
             }
         }
     }
+    const bool starts_here = dl >= 0 && dl < nb;
+    if (act) {
+        const int ce = colexp[col] + my_exp; // pair columns share their exponents
+        if (starts_here && !first_in_group) {
+            for (int k = w_s + nb + j; k < grp.wr; k += QL) {
+                double *w0 = W + (long)k * ldw + col;
+                double *w1 = W + (long)(grp.wr + k) * ldw + col;
+                w0[0] = 0.0;
+                w1[0] = 0.0;
+                if (pair) {
+                    w0[1] = 0.0;
+                    w1[1] = 0.0;
+                }
+            }
+        }
+        if (grp.last && !starts_here) {
+#pragma clang loop unroll(full)
+            for (int jb = 1; jb < 4; ++jb) {
+                const int k0 = grp.off[jb], k1 = grp.off[jb + 1];
+                if (k0 < k1) {
+                    const int e = ce - blkexp[(long)(grp.step[jb] - grp.step[0]) * ldx + col];
+                    if (e != 0) {
+                        for (int k = k0 + j; k < k1; k += QL) {
+                            double *w0 = W + (long)k * ldw + col;
+                            double *w1 = W + (long)(grp.wr + k) * ldw + col;
+                            w0[0] = ldexp(w0[0], e);
+                            w1[0] = ldexp(w1[0], e);
+                            if (pair) {
+                                w0[1] = ldexp(w0[1], e);
+                                w1[1] = ldexp(w1[1], e);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
     if (act && j < ud.w) {
         // lane 0: real part column, lane 1: imaginary part column
         const int p = col + j;
@@ -502,56 +545,13 @@ __global__ __launch_bounds__(256) void k_solve_real(
         sig[p] = sg;
         // cumulative scale over the group's steps (the group's far update applies it);
         // a unit starting in this block has no earlier steps in the group
-        const bool starts_here = dl >= 0 && dl < nb;
         sig_grp[p] = (first_in_group || starts_here) ? sg : sig_grp[p] * sg;
         const int ce = colexp[p] + my_exp;
         colexp[p] = ce;
         blkexp[p] = ce;
         if (my_exp != 0) {
-            // the operands of the group's earlier (lower) blocks carry the old scale: the
-            // group's rescale pass (k_w_rescale) brings them to the final one
             *flag = 1;
-            *gflag = 1;
         }
-    }
-}
-
-// ---------------------------------------------------------------------------------------
-// k_w_rescale: after the last step of a group, bring the group's update operands W (rows
-// [0, wr) S part, [wr, 2 wr) P part; local row k belongs to the block holding row g0 + k)
-// to the columns' final scale: W[k][p] *= 2^(blkexp[last][p] - blkexp[step(k)][p]).
-// Blocks are listed top-down: block j covers local rows [off_j, off_{j+1}) and was solved
-// in step steps_j; steps.x is the group's last step.  Skipped when no column of the group
-// was rescaled (*gflag == 0).
-// ---------------------------------------------------------------------------------------
-__global__ void k_w_rescale(double *__restrict__ W, long ldw, int wr, int c0, int ncols, const int *__restrict__ blkexp,
-                            long ldb, int4 offs, int4 steps, const int *__restrict__ gflag)
-{
-    if (*gflag == 0) {
-        return;
-    }
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y; // 0 .. 2 wr - 1
-    if (x >= ncols) {
-        return;
-    }
-    const int kl = k < wr ? k : k - wr;
-    const int j = (kl >= offs.y) + (kl >= offs.z) + (kl >= offs.w);
-    const int st = j == 0 ? steps.x : (j == 1 ? steps.y : (j == 2 ? steps.z : steps.w));
-    const int p = c0 + x;
-    const int e = blkexp[(long)steps.x * ldb + p] - blkexp[(long)st * ldb + p];
-    if (e != 0) {
-        double *w = W + (long)k * ldw + p;
-        *w = ldexp(*w, e);
-    }
-}
-
-// W[k][c0 + x] = 0 for k < rows, x < ncols
-__global__ void k_zero_cols(double *__restrict__ W, long ldw, int rows, int c0, int ncols)
-{
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x < ncols && (int)blockIdx.y < rows) {
-        W[(long)blockIdx.y * ldw + c0 + x] = 0.0;
     }
 }
 
@@ -928,7 +928,7 @@ __global__ void k_zero_int(int *__restrict__ X, long count)
 namespace xtg {
 template __global__ void k_solve_real<65>(const double *, long, const double *, long, int, int, unsigned long long,
                                           double *, long, double *, long, const UnitDesc *, const int2 *, int, int,
-                                          int, int, double *, int *, int *, int *, int *, int, int, double *, int);
+                                          int, int, double *, int *, int *, int *, int, int, double *, int, GroupInfo);
 } // namespace xtg
 
 // =======================================================================================
@@ -1080,9 +1080,10 @@ __global__ __launch_bounds__(64) void k_bwd_diag(const double *__restrict__ X, l
     }
 }
 
-// Host launcher: picks the 16-byte-load form when both output extents are >= 2.
-// tag only separates the uses in profiles: 0 = RHS update, 1 = back-transform, 2 = backward.
-void launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const double *L, long ldl, const double *L2,
+// Host launcher: picks the 16-byte-load form when both output extents are >= 2 and the tile
+// edge (returned) from the tile count.  tag only separates the uses in profiles:
+// 0 = RHS update, 1 = back-transform, 2 = backward.
+int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const double *L, long ldl, const double *L2,
                      long ldl2, const double *R, long ldr, double *C, long ldc, const int *outcol, int mode,
                      const double *sig, const int *kmax, const double *R2)
 {
@@ -1113,12 +1114,13 @@ void launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, cons
                 XTG_GEMM_LAUNCH(T, false, 128, 16);                                                                  \
             }                                                                                                        \
         }                                                                                                            \
-        return;                                                                                                      \
+        return gt;                                                                                                   \
     }
     XTG_GEMM_CASE(0)
     XTG_GEMM_CASE(1)
     XTG_GEMM_CASE(2)
 #undef XTG_GEMM_CASE
 #undef XTG_GEMM_LAUNCH
+    return 0;
 }
 } // namespace xtg
