@@ -25,6 +25,16 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload,
+# each scaled by the counter's reading on a plain 1 GiB device copy (FETCH x2.0, WRITE x1.0);
+# summary and commands in profiles/dtgevc3_R_B_n8192_hbm_traffic.md
+MEASURED_TRAFFIC = {
+    ("xtg::k_gemm_f64<0, true, 128, 16>", 8192): 686.2e6,
+    ("xtg::k_gemm_f64<1, true, 128, 16>", 8192): 12608.2e6,
+    ("xtg::k_gemm_f64<0, true, 64, 32>", 8192): 29.1e6,
+    ("xtg::k_solve_real<65>", 8192): 18.2e6,
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -137,6 +147,47 @@ def cpu_baseline(n, seed, budget_s=40.0):
             "seconds": ts, "n_sample": ns}
 
 
+def backward_ratio(seed, n=2048, nsel=256, nbatch=32, reps=5):
+    """Forward dtgevc3 side='R' howmny='S' (nsel random real eigenvalues, no back-transform) and
+    the hand-written backward for nbatch cotangent batches, on the device; times in ms."""
+    import torch
+
+    import xtgevc3_amd as X
+    from xtgevc3_amd import pencils as G
+
+    S, P, ar, ai, be = G.real_quasi_triangular(n, seed)
+    rng = np.random.default_rng(seed)
+    real_k = np.array([k for k in range(n) if ai[k] == 0.0 and (k == 0 or ai[k - 1] == 0.0)])
+    sel = np.zeros(n, dtype=np.int32)
+    sel[np.sort(rng.choice(real_k, size=nsel, replace=False))] = 1
+    dS = torch.from_numpy(np.ascontiguousarray(S.T)).cuda()
+    dP = torch.from_numpy(np.ascontiguousarray(P.T)).cuda()
+    Vt = torch.zeros((nsel, n), dtype=torch.float64, device="cuda")
+    Gt = torch.randn((nbatch * nsel, n), dtype=torch.float64, device="cuda")
+    gS = torch.zeros((nbatch * n, n), dtype=torch.float64, device="cuda")
+    gP = torch.zeros_like(gS)
+
+    def fwd():
+        m, info = X.dtgevc3_device("R", "S", sel, dS, dP, ar, ai, be, VR=Vt, mm=nsel)
+        assert (m, info) == (nsel, 0), (m, info)
+
+    def bwd():
+        X.dtgevc3_backward_device(dS, dP, ar, ai, be, sel, Vt, Gt, nbatch=nbatch, dS=gS, dP=gP)
+
+    out = {}
+    for name, f in (("fwd", fwd), ("bwd", bwd)):
+        f()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            f()
+        torch.cuda.synchronize()
+        out[name] = 1000.0 * (time.perf_counter() - t) / reps
+    return {"n": n, "selected": nsel, "batches": nbatch, "fwd_ms": round(out["fwd"], 3),
+            "bwd_ms": round(out["bwd"], 3), "bwd_over_fwd": round(out["bwd"] / out["fwd"], 3),
+            "bwd_per_batch_over_fwd": round(out["bwd"] / nbatch / out["fwd"], 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +197,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-residual", action="store_true")
+    ap.add_argument("--no-backward", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this path runs on one GPU (the eigenvector columns shard across GPUs without "
@@ -218,12 +270,14 @@ def main():
         # for the back-transform)
         ach = d["flops"] / d["launches"] / (avg_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": None, "kernel": names[dom],
+                "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": MEASURED_TRAFFIC.get((names[dom], n)),
+                "kernel": names[dom],
                 "avg_launch_ms": round(avg_ms, 4), "launches_per_step": d["launches"] // nprof}
     else:
         ach = d["bytes"] / d["launches"] / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "kernel": names[dom],
+                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": MEASURED_TRAFFIC.get((names[dom], n)),
+                "kernel": names[dom],
                 "avg_launch_ms": round(avg_ms, 4), "launches_per_step": d["launches"] // nprof}
     kernels = {k: {"ms_per_step": round(v["ms"] / nprof, 4), "launches_per_step": v["launches"] // nprof,
                    "tflops": round(v["flops"] / (v["ms"] * 1e9), 3) if v["ms"] else None}
@@ -235,6 +289,12 @@ def main():
         torch.cuda.synchronize()
         res, ncols = residual_sample(S, P, ar, ai, be, dZ, dVR)
         log(f"[bench] residual (metric A) over {ncols} sampled columns: {res:.3e} n*eps")
+
+    bw = None
+    if not args.no_backward:
+        bw = backward_ratio(args.seed)
+        log(f"[bench] backward n={bw['n']} sel={bw['selected']} x{bw['batches']} batches: fwd {bw['fwd_ms']} ms, "
+            f"bwd {bw['bwd_ms']} ms (all batches), ratio {bw['bwd_over_fwd']} ({bw['bwd_per_batch_over_fwd']} per batch)")
 
     cpu = None
     if not args.no_cpu:
@@ -261,6 +321,7 @@ def main():
                    "form": "plain-fp64", "path_tflops": round(path_tflops, 3),
                    "max_residual_n_eps": None if res is None else float(f"{res:.4g}")},
         "roofline": roof,
+        "backward": bw,
         "kernels": kernels,
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
     }
