@@ -198,6 +198,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-residual", action="store_true")
     ap.add_argument("--no-backward", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this path runs on one GPU (the eigenvector columns shard across GPUs without "
@@ -290,6 +291,42 @@ def main():
         res, ncols = residual_sample(S, P, ar, ai, be, dZ, dVR)
         log(f"[bench] residual (metric A) over {ncols} sampled columns: {res:.3e} n*eps")
 
+    # mixed form (FP32 MFMA products for the updates and back-transform) on the same inputs
+    mixed = None
+    if not args.no_mixed:
+        def step_mixed():
+            dVR.copy_(dZ)
+            m, info = X.dtgevc3_device("R", "B", None, dS, dP, ar, ai, be, VR=dVR, stream=stream.cuda_stream,
+                                       precision=X.PREC_MIXED_F32)
+            assert info == 0 and m == n, (m, info)
+            return m
+
+        step_mixed()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mm_ = 0
+        for _ in range(args.steps):
+            mm_ += step_mixed()
+        torch.cuda.synchronize()
+        tm = time.perf_counter() - t0
+        mixed = {"form": "FP32 MFMA products (updates, back-transform), FP64 solves/scales/normalisation",
+                 "value": round(mm_ / tm, 2), "ms_per_step": round(1000.0 * tm / args.steps, 3),
+                 "speedup_vs_plain": round(total_s / tm, 3)}
+        if not args.no_residual:
+            step()
+            torch.cuda.synchronize()
+            ref64 = dVR.clone()
+            step_mixed()
+            torch.cuda.synchronize()
+            rm, _ = residual_sample(S, P, ar, ai, be, dZ, dVR)
+            diff = float((dVR - ref64).abs().max())
+            mixed["max_residual_n_eps"] = float(f"{rm:.4g}")
+            mixed["max_abs_diff_vs_fp64"] = float(f"{diff:.4g}")
+            del ref64
+        log(f"[bench] mixed form: {mixed['ms_per_step']:.2f} ms/step, {mixed['value']:.1f} eigvecs/s, "
+            f"x{mixed['speedup_vs_plain']} vs plain; residual {mixed.get('max_residual_n_eps')} n*eps, "
+            f"max |V_mixed - V_fp64| {mixed.get('max_abs_diff_vs_fp64')}")
+
     bw = None
     if not args.no_backward:
         bw = backward_ratio(args.seed)
@@ -321,6 +358,7 @@ def main():
                    "form": "plain-fp64", "path_tflops": round(path_tflops, 3),
                    "max_residual_n_eps": None if res is None else float(f"{res:.4g}")},
         "roofline": roof,
+        "mixed": mixed,
         "backward": bw,
         "kernels": kernels,
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},

@@ -67,6 +67,16 @@ void xtgevc3_ctgevc3(char side, char howmny, const int *select, int n, const xtg
 int xtgevc3_dtgevc3_device(char side, char howmny, const int *select, int n, const double *S, int lds,
                            const double *P, int ldp, const double *alphar, const double *alphai, const double *beta,
                            double *VL, int ldvl, double *VR, int ldvr, int mm, int *m, int *info, void *stream);
+/* dtgevc3 on device with a precision selector (no counterpart in the reference: the mixed
+ * form of the update and back-transform products).  XTGEVC3_PREC_FP64 is
+ * xtgevc3_dtgevc3_device; XTGEVC3_PREC_MIXED_F32 runs the off-diagonal updates and the
+ * back-transform on FP32 MFMA products accumulated in FP32 and added to the FP64 running
+ * right-hand sides / output; the block solves, scale factors and normalisation stay FP64.
+ * Other selectors return XTGEVC3_INFO_UNSUPPORTED. */
+int xtgevc3_dtgevc3_device_ex(char side, char howmny, const int *select, int n, const double *S, int lds,
+                              const double *P, int ldp, const double *alphar, const double *alphai,
+                              const double *beta, double *VL, int ldvl, double *VR, int ldvr, int mm, int *m,
+                              int *info, void *stream, int precision);
 int xtgevc3_ztgevc3_device(char side, char howmny, const int *select, int n, const xtgevc3_zcomplex *S, int lds,
                            const xtgevc3_zcomplex *P, int ldp, const xtgevc3_zcomplex *alpha,
                            const xtgevc3_zcomplex *beta, xtgevc3_zcomplex *VL, int ldvl, xtgevc3_zcomplex *VR,

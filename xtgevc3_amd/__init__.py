@@ -38,12 +38,13 @@ _lib.xtgevc3_last_error.restype = ctypes.c_char_p
 _lib.xtgevc3_abi_version.restype = ctypes.c_int
 
 INFO_DEVICE_ERROR = -1000
+INFO_UNSUPPORTED = -1001
 PREC_FP64, PREC_MIXED_F32, PREC_MIXED_BF16X3 = 0, 1, 2
 
 EXPORTED_SYMBOLS = [
     "xtgevc3_abi_version", "xtgevc3_last_error",
     "xtgevc3_dtgevc3", "xtgevc3_stgevc3", "xtgevc3_ztgevc3", "xtgevc3_ctgevc3",
-    "xtgevc3_dtgevc3_device", "xtgevc3_ztgevc3_device",
+    "xtgevc3_dtgevc3_device", "xtgevc3_dtgevc3_device_ex", "xtgevc3_ztgevc3_device",
     "xtgevc3_profile_enable", "xtgevc3_profile_read", "xtgevc3_dtgevc3_backward_device",
 ]
 
@@ -173,16 +174,18 @@ def from_fortran_device(t):
 
 
 def tgevc3_device(prefix, side, howmny, select, S, P, alpha, beta, alphai=None, VL=None, VR=None, mm=None,
-                  stream=None):
+                  stream=None, precision=PREC_FP64):
     """Device-pointer call.  S, P, VL, VR are torch tensors holding column-major storage
     (row-major tensors of shape (cols, ld)); select and eigenvalues are host numpy arrays.
-    Returns (m, info)."""
+    precision (real interface only): PREC_FP64 or PREC_MIXED_F32.  Returns (m, info)."""
     import torch
 
     c = ctypes
     real = prefix in _REAL
     dt = _REAL[prefix] if real else _CPLX[prefix]
-    fn = getattr(_lib, f"xtgevc3_{prefix}tgevc3_device")
+    if precision != PREC_FP64 and prefix != "d":
+        raise ValueError("the mixed forms are available for dtgevc3 only")
+    fn = getattr(_lib, f"xtgevc3_{prefix}tgevc3_device" + ("_ex" if precision != PREC_FP64 else ""))
     n = S.shape[0]
     lds = S.shape[1]
     ldp = P.shape[1]
@@ -200,17 +203,23 @@ def tgevc3_device(prefix, side, howmny, select, S, P, alpha, beta, alphai=None, 
         ev = [np.ascontiguousarray(alpha, dtype=dt), np.ascontiguousarray(beta, dtype=dt)]
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
-    rc = fn(c.c_char(side.encode()), c.c_char(howmny.encode()), _ptr(sel), c.c_int(n), _tptr(S), c.c_int(lds),
+    args = [c.c_char(side.encode()), c.c_char(howmny.encode()), _ptr(sel), c.c_int(n), _tptr(S), c.c_int(lds),
             _tptr(P), c.c_int(ldp), *[_ptr(e) for e in ev], _tptr(VL), c.c_int(ldvl), _tptr(VR), c.c_int(ldvr),
-            c.c_int(mm), c.byref(m), c.byref(info), c.c_void_p(stream))
+            c.c_int(mm), c.byref(m), c.byref(info), c.c_void_p(stream)]
+    if precision != PREC_FP64:
+        args.append(c.c_int(precision))
+    rc = fn(*args)
+    if rc == INFO_UNSUPPORTED:
+        raise ValueError("xtgevc3: " + last_error())
     if rc != 0:
         raise RuntimeError("xtgevc3 device path failed: " + last_error())
     return m.value, info.value
 
 
-def dtgevc3_device(side, howmny, select, S, P, alphar, alphai, beta, VL=None, VR=None, mm=None, stream=None):
+def dtgevc3_device(side, howmny, select, S, P, alphar, alphai, beta, VL=None, VR=None, mm=None, stream=None,
+                   precision=PREC_FP64):
     return tgevc3_device("d", side, howmny, select, S, P, alphar, beta, alphai=alphai, VL=VL, VR=VR, mm=mm,
-                         stream=stream)
+                         stream=stream, precision=precision)
 
 
 def ztgevc3_device(side, howmny, select, S, P, alpha, beta, VL=None, VR=None, mm=None, stream=None):

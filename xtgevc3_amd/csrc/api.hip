@@ -271,10 +271,18 @@ void xtgevc3_stgevc3(char side, char howmny, const int *select, int n, const flo
     }
 }
 
-int xtgevc3_dtgevc3_device(char side, char howmny, const int *select, int n, const double *S, int lds,
-                           const double *P, int ldp, const double *alphar, const double *alphai, const double *beta,
-                           double *VL, int ldvl, double *VR, int ldvr, int mm, int *m, int *info, void *stream)
+int xtgevc3_dtgevc3_device_ex(char side, char howmny, const int *select, int n, const double *S, int lds,
+                              const double *P, int ldp, const double *alphar, const double *alphai,
+                              const double *beta, double *VL, int ldvl, double *VR, int ldvr, int mm, int *m,
+                              int *info, void *stream, int precision)
 {
+    if (precision != XTGEVC3_PREC_FP64 && precision != XTGEVC3_PREC_MIXED_F32) {
+        xtg::set_error("precision " + std::to_string(precision) + " is not available");
+        if (info) {
+            *info = XTGEVC3_INFO_UNSUPPORTED;
+        }
+        return XTGEVC3_INFO_UNSUPPORTED;
+    }
     std::vector<xtg::Unit> units;
     int num_sel = (howmny == 'A' || howmny == 'B' || howmny == 'S') && n > 0
                       ? xtg::real_units(howmny, select, n, alphai, units)
@@ -283,11 +291,19 @@ int xtgevc3_dtgevc3_device(char side, char howmny, const int *select, int n, con
         return 0;
     }
     int rc = xtg::real_device(side, howmny, select, n, S, lds, P, ldp, alphar, alphai, beta, VL, ldvl, VR, ldvr,
-                              (hipStream_t)stream);
+                              (hipStream_t)stream, precision);
     if (rc != 0) {
         return xtg::fail(info, "device path");
     }
     return 0;
+}
+
+int xtgevc3_dtgevc3_device(char side, char howmny, const int *select, int n, const double *S, int lds,
+                           const double *P, int ldp, const double *alphar, const double *alphai, const double *beta,
+                           double *VL, int ldvl, double *VR, int ldvr, int mm, int *m, int *info, void *stream)
+{
+    return xtgevc3_dtgevc3_device_ex(side, howmny, select, n, S, lds, P, ldp, alphar, alphai, beta, VL, ldvl, VR,
+                                     ldvr, mm, m, info, stream, XTGEVC3_PREC_FP64);
 }
 
 

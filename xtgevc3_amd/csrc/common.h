@@ -52,6 +52,17 @@ struct GroupInfo {
     int last; // the launch is the group's last step
 };
 
+// Exponent bookkeeping of the mixed (FP32-product) form, written by the real solve kernel
+// when on != 0: wblk[col] bounds |W| of the block just solved (2^wblk), wgrp[col] bounds the
+// group's W (relative to the column's exponent until the group's last step, absolute after
+// it), xrel[col] bounds the solution entries relative to the column's exponent.
+struct MixedExp {
+    int *wblk;
+    int *wgrp;
+    int *xrel;
+    int on;
+};
+
 // Same layout for the complex interface: the working vector entries are complex.
 struct ZColDesc {
     double aR, aI; // acoeff (complex for the complex interface)

@@ -32,12 +32,16 @@ namespace xtg {
 template <int MAXB>
 __global__ void k_solve_real(const double *, long, const double *, long, int, int, unsigned long long, double *, long,
                              double *, long, const UnitDesc *, const int2 *, int, int, int, int, double *, int *, int *,
-                             int *, int, int, double *, int, GroupInfo);
+                             int *, int, int, double *, int, GroupInfo, MixedExp);
+__global__ void k_rowexp(const double *, long, const double *, long, int, int, int *);
+__global__ void k_xfin(const int *, const int *, int, int *);
+__global__ void k_fill_int(int *, long, int);
 constexpr int SOLVE_UNITS = 4;   // eigenvalue units per wave of the solve kernel (16 lanes per unit)
 constexpr int SOLVE_WAVES_H = 4; // waves per solve workgroup (they share the staged block)
 int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const double *L, long ldl, const double *L2,
                      long ldl2, const double *R, long ldr, double *C, long ldc, const int *outcol, int mode,
-                     const double *sig, const int *kmax, const double *R2);
+                     const double *sig, const int *kmax, const double *R2, int prec, const int *sa,
+                     const int *sb);
 constexpr int GROUP_STEPS = 4; // row blocks per far update (K = 2 * ~256)
 __global__ void k_fixup_scale(double *, long, int, int, const int *, const int *, long, const int *, const int *);
 __global__ void k_copy_out(const double *, long, int, int, double *, long, const int *, int);
@@ -238,7 +242,7 @@ struct DevBuf {
 };
 
 struct RealWork {
-    DevBuf X, W, sig, colexp, blkexp, flag, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
+    DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wblk, wgrp, xrel, xfin, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
     DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp;
 };
 
@@ -273,6 +277,7 @@ struct RealSideArgs {
     double *V; // output, device
     long ldv;
     const std::vector<double> *subw; // working subdiagonal, host
+    int prec;                        // 0: FP64, 1: FP32-product updates and back-transform
 };
 
 // ---------------------------------------------------------------------------------------
@@ -286,7 +291,7 @@ struct RealSideArgs {
 // ---------------------------------------------------------------------------------------
 int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *Pw, long ldpw,
                const std::vector<double> &subw, const std::vector<ColDesc> &desc, bool adjoint, long ldx,
-               hipStream_t st)
+               hipStream_t st, int prec)
 {
     const int m = (int)desc.size();
     std::vector<UnitDesc> units;
@@ -316,6 +321,19 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     ENSURE(ws.tiles, sizeof(int2) * tiles.size());
     ENSURE(ws.rowstep, sizeof(int) * (size_t)n);
     XTG_CHECK(hipMemcpyAsync(ws.desc.p, units.data(), sizeof(UnitDesc) * units.size(), hipMemcpyHostToDevice, st));
+    MixedExp mx{nullptr, nullptr, nullptr, 0};
+    if (prec == 1) {
+        // mixed form: row exponents of the working pencil, per-column operand/solution bounds
+        ENSURE(ws.rowexp, sizeof(int) * (size_t)n);
+        ENSURE(ws.wblk, sizeof(int) * (size_t)m);
+        ENSURE(ws.wgrp, sizeof(int) * (size_t)m);
+        ENSURE(ws.xrel, sizeof(int) * (size_t)m);
+        hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.rowexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
+        hipLaunchKernelGGL(k_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, Sw, ldsw, Pw, ldpw, n, n,
+                           ws.rowexp.as<int>()); XTG_LAUNCH_CHECK("k_rowexp");
+        hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.xrel.as<int>(), (long)m, -1000000); XTG_LAUNCH_CHECK("k_fill_int");
+        mx = MixedExp{ws.wblk.as<int>(), ws.wgrp.as<int>(), ws.xrel.as<int>(), 1};
+    }
     XTG_CHECK(hipMemcpyAsync(ws.tiles.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, st));
     XTG_CHECK(hipMemcpyAsync(ws.rowstep.p, rowstep.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     double *X = ws.X.as<double>();
@@ -393,7 +411,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
                                ldsw, Pw, ldpw, i0, nb, bumps, X, ldx, W, ldw, ws.desc.as<UnitDesc>(), ws.tiles.as<int2>(),
                                t_begin, t_end, u_begin, u_end, ws.sig.as<double>(), ws.colexp.as<int>(),
                                ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff, wr + woff,
-                               ws.sig_grp.as<double>(), first ? 1 : 0, grp); XTG_LAUNCH_CHECK("k_solve_real");
+                               ws.sig_grp.as<double>(), first ? 1 : 0, grp, mx); XTG_LAUNCH_CHECK("k_solve_real");
             prof_end(pr, st, 0, 2.0 * nb * nb * N, 8.0 * (4.0 * nb * N + 2.0 * nb * nb));
             first = false;
             // near update: the group's rows above this block
@@ -405,7 +423,8 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
                                    ldsw, Pw + r0 + (long)i0 * ldpw, ldpw, (const double *)(W + (long)woff * ldw + c_begin),
                                    ldw, X + (long)r0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
                                    (const double *)(ws.sig.as<double>() + c_begin), (const int *)nullptr,
-                                   (const double *)(W + (long)(wr + woff) * ldw + c_begin)); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
+                                   (const double *)(W + (long)(wr + woff) * ldw + c_begin), prec,
+                                   mx.on ? ws.rowexp.as<int>() + r0 : nullptr, mx.on ? mx.wblk + c_begin : nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
                 prof_end(pr, st, gt == 128 ? XTGEVC3_PROF_UPDATE_T128 : XTGEVC3_PROF_UPDATE, 2.0 * M * N * (2.0 * nb),
                          8.0 * (2.0 * M * N + 2.0 * nb * M + 2.0 * nb * N));
             }
@@ -423,7 +442,8 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
                                Pw + r0 + (long)g0 * ldpw, ldpw, (const double *)(W + c_begin), ldw,
                                X + (long)r0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
                                (const double *)(ws.sig_grp.as<double>() + c_begin), (const int *)nullptr,
-                               (const double *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
+                               (const double *)nullptr, prec, mx.on ? ws.rowexp.as<int>() + r0 : nullptr,
+                               mx.on ? mx.wgrp + c_begin : nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
             prof_end(pr, st, gt == 128 ? XTGEVC3_PROF_UPDATE_T128 : XTGEVC3_PROF_UPDATE, 2.0 * M * N * (2.0 * wr),
                      8.0 * (2.0 * M * N + 2.0 * wr * M + 2.0 * wr * N));
         }
@@ -500,7 +520,7 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
     XTG_CHECK(hipMemcpyAsync(ws.vw.p, vw.data(), sizeof(int) * vw.size(), hipMemcpyHostToDevice, st));
     ENSURE(ws.X, sizeof(double) * (size_t)n * ldx);
     hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, ws.X.as<double>(), (long)n * ldx); XTG_LAUNCH_CHECK("k_zero");
-    int rc = sweep_real(ws, n, A.Sw, A.ldsw, A.Pw, A.ldpw, *A.subw, desc, false, ldx, st);
+    int rc = sweep_real(ws, n, A.Sw, A.ldsw, A.Pw, A.ldpw, *A.subw, desc, false, ldx, st, A.prec);
     if (rc) {
         return rc;
     }
@@ -517,10 +537,21 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
             ksum += kmaxv[q];
             kmx = std::max(kmx, kmaxv[q]);
         }
+        if (A.prec == 1) {
+            // mixed form: exponents of the final X columns and of the rows of Z (Q)
+            ENSURE(ws.xfin, sizeof(int) * (size_t)m);
+            ENSURE(ws.zexp, sizeof(int) * (size_t)n);
+            hipLaunchKernelGGL(k_xfin, dim3((m + 255) / 256), dim3(256), 0, st, (const int *)ws.colexp.as<int>(),
+                               (const int *)ws.xrel.as<int>(), m, ws.xfin.as<int>()); XTG_LAUNCH_CHECK("k_xfin");
+            hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.zexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
+            hipLaunchKernelGGL(k_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, A.Z, A.ldz,
+                               (const double *)nullptr, 0L, n, n, ws.zexp.as<int>()); XTG_LAUNCH_CHECK("k_rowexp");
+        }
         int pr = prof_begin(st);
         launch_gemm_f64(1, st, m, n, n, 0, (const double *)X, ldx,
                            (const double *)nullptr, 0L, Rp, ldr, A.V, A.ldv, ws.outcol.as<int>(), 0,
-                           (const double *)nullptr, ws.kmax.as<int>(), (const double *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<1>");
+                           (const double *)nullptr, ws.kmax.as<int>(), (const double *)nullptr, A.prec,
+                           A.prec == 1 ? ws.xfin.as<int>() : nullptr, A.prec == 1 ? ws.zexp.as<int>() : nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<1>");
         prof_end(pr, st, 2, 2.0 * n * ksum, 8.0 * ((double)n * kmx + ksum + (double)n * m));
     }
     else {
@@ -541,7 +572,7 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
 // ---------------------------------------------------------------------------------------
 int real_device(char side, char howmny, const int *select, int n, const double *S, long lds, const double *P,
                 long ldp, const double *ar, const double *ai, const double *be, double *VL, long ldvl, double *VR,
-                long ldvr, hipStream_t st)
+                long ldvr, hipStream_t st, int prec)
 {
     int dev = 0;
     XTG_CHECK(hipGetDevice(&dev));
@@ -591,6 +622,7 @@ int real_device(char side, char howmny, const int *select, int n, const double *
         A.left = false;
         A.back = back;
         A.subw = &sub;
+        A.prec = prec;
         A.V = VR;
         A.ldv = ldvr;
         if (back) {
@@ -631,6 +663,7 @@ int real_device(char side, char howmny, const int *select, int n, const double *
         A.left = true;
         A.back = back;
         A.subw = &subw;
+        A.prec = prec;
         A.V = VL;
         A.ldv = ldvl;
         if (back) {
@@ -742,7 +775,7 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
     int pr = prof_begin(st);
     hipLaunchKernelGGL(k_bwd_rhs, dim3(M), dim3(256), 0, st, V, ldv, G, ldg, n, m, nbatch, ws.kcol.as<int>(), X, ldx); XTG_LAUNCH_CHECK("k_bwd_rhs");
     prof_end(pr, st, 3, 0.0, 8.0 * 3.0 * n * M);
-    int rc = sweep_real(ws, n, ws.Sw.as<double>(), n, ws.Pw.as<double>(), n, subw, desc, true, ldx, st);
+    int rc = sweep_real(ws, n, ws.Sw.as<double>(), n, ws.Pw.as<double>(), n, subw, desc, true, ldx, st, 0);
     if (rc) {
         return rc;
     }
@@ -753,7 +786,7 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
         pr = prof_begin(st);
         launch_gemm_f64(2, st, n, m, n, 0, P, ldp, (const double *)nullptr, 0L,
                            (const double *)ws.Vt.as<double>(), ldt, ws.Y.as<double>(), ldt, (const int *)nullptr, 0,
-                           (const double *)nullptr, (const int *)nullptr, (const double *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+                           (const double *)nullptr, (const int *)nullptr, (const double *)nullptr, 0, nullptr, nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
         prof_end(pr, st, 3, 2.0 * n * n * (double)m, 8.0 * ((double)n * n + 2.0 * n * m));
     }
     hipLaunchKernelGGL(k_bwd_lambda_t, dim3((M + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, st, (const double *)X, ldx,
@@ -764,10 +797,10 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
         pr = prof_begin(st);
         launch_gemm_f64(2, st, n, n, m, 0, V, ldv, (const double *)nullptr, 0L,
                            (const double *)(ws.LamS.as<double>() + (long)b * m * n), (long)n, dS + (long)b * n * ldds,
-                           ldds, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr, (const double *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+                           ldds, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr, (const double *)nullptr, 0, nullptr, nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
         launch_gemm_f64(2, st, n, n, m, 0, V, ldv, (const double *)nullptr, 0L,
                            (const double *)(ws.LamP.as<double>() + (long)b * m * n), (long)n, dP + (long)b * n * lddp,
-                           lddp, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr, (const double *)nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+                           lddp, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr, (const double *)nullptr, 0, nullptr, nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
         prof_end(pr, st, 3, 4.0 * n * n * (double)m, 8.0 * 2.0 * ((double)n * m + (double)m * n + (double)n * n));
     }
     hipLaunchKernelGGL(k_bwd_diag, dim3(M), dim3(64), 0, st, (const double *)X, ldx, (const double *)ws.Y.as<double>(),

@@ -176,7 +176,7 @@ __global__ __launch_bounds__(256) void k_solve_real(
     unsigned long long bumps, double *__restrict__ X, long ldx, double *__restrict__ W, long ldw,
     const UnitDesc *__restrict__ units, const int2 *__restrict__ tiles, int t_begin, int t_end, int u_begin,
     int u_end, double *__restrict__ sig, int *__restrict__ colexp, int *__restrict__ blkexp, int *__restrict__ flag,
-    int w_s, int w_p, double *__restrict__ sig_grp, int first_in_group, GroupInfo grp)
+    int w_s, int w_p, double *__restrict__ sig_grp, int first_in_group, GroupInfo grp, MixedExp mx)
 {
     static_assert(MAXB <= 65 && SP_PITCH >= MAXB + 1 && (SP_PITCH % 2) == 0, "block layout");
     constexpr int QR = (MAXB + QL - 1) / QL; // rows per lane
@@ -538,6 +538,37 @@ __global__ __launch_bounds__(256) void k_solve_real(
             }
         }
     }
+    if (mx.on) {
+        // magnitude bounds of this block's solution entries and operands (mixed form)
+        double mxv = 0.0;
+#pragma clang loop unroll(full)
+        for (int i = 0; i < QR; ++i) {
+            if (QL * i + j < nb) {
+                mxv = fmax(mxv, fmax(fabs(vr[i].r), fabs(vr[i].i)));
+            }
+        }
+#pragma clang loop unroll(full)
+        for (int off = 1; off < QL; off <<= 1) {
+            mxv = fmax(mxv, __shfl_xor(mxv, off));
+        }
+        const double mw = mxv * fmax(fabs(a), 2.0 * bn1);
+        // zero blocks: a sentinel far below any exponent
+        const int ex = mxv > 0.0 ? ilogb(mxv) + 1 : -1000000;
+        const int ew = mw > 0.0 ? ilogb(mw) + 1 : -1000000;
+        if (act && j < ud.w) {
+            const int p = col + j;
+            const int ce = colexp[p] + my_exp;
+            const bool fresh = first_in_group || starts_here;
+            mx.wblk[p] = min(max(ew, -1000), 1000);
+            if (grp.last) {
+                mx.wgrp[p] = min(max(fresh ? ew : max(ew, ce + mx.wgrp[p]), -1000), 1000);
+            }
+            else {
+                mx.wgrp[p] = fresh ? ew - ce : max(mx.wgrp[p], ew - ce);
+            }
+            mx.xrel[p] = max(mx.xrel[p], ex - ce);
+        }
+    }
     if (act && j < ud.w) {
         // lane 0: real part column, lane 1: imaginary part column
         const int p = col + j;
@@ -572,20 +603,28 @@ __global__ __launch_bounds__(256) void k_solve_real(
 // WIDE: 16-byte loads of column pairs (M, N >= 2); otherwise one double per load.
 // ---------------------------------------------------------------------------------------
 typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
 
-template <int TAG, bool WIDE, int GT, int GK>
+// F32 (mixed form): operands rounded to FP32 at the LDS store, products and accumulation on
+// v_mfma_f32_16x16x4 (FP32), the running C stays FP64 (C = sig * C + D in FP64).  To keep
+// the FP32 operands in range, L[.][a] is scaled by 2^-sa[a] and R[.][b] by 2^-sb[b]
+// (power-of-two exponents bounding the row / column magnitudes; null: no scaling) and D by
+// 2^(sa[a] + sb[b]) in the FP64 epilogue.
+template <int TAG, bool WIDE, int GT, int GK, bool F32 = false>
 __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, const double *__restrict__ L, long ldl,
                                                   const double *__restrict__ L2, long ldl2,
                                                   const double *__restrict__ R, long ldr, double *__restrict__ C,
                                                   long ldc, const int *__restrict__ outcol, int mode,
                                                   const double *__restrict__ sig, const int *__restrict__ kmax,
-                                                  const double *__restrict__ R2)
+                                                  const double *__restrict__ R2, const int *__restrict__ sa,
+                                                  const int *__restrict__ sb)
 {
     if (R2 == nullptr) {
         R2 = R + (long)K1 * ldr;
     }
-    __shared__ double Ls[2][GK][GT];
-    __shared__ double Rs[2][GK][GT];
+    typedef typename std::conditional<F32, float, double>::type el_t;
+    __shared__ el_t Ls[2][GK][GT];
+    __shared__ el_t Rs[2][GK][GT];
 
     // with a per-row K bound (triangular operand) the heaviest tiles are dispatched first
     const int ya = kmax != nullptr ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;
@@ -618,12 +657,16 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
     constexpr int WT = GT / 2;  // wave tile edge
     constexpr int FT = WT / 16; // 16x16 MFMA blocks per wave tile edge
     static_assert(GK % 4 == 0 && (GT == 64 || GT == 128), "tile shape");
-    double4_t acc[FT][FT];
+    typedef typename std::conditional<F32, float4_t, double4_t>::type acc_t;
+    acc_t acc[FT][FT];
 #pragma clang loop unroll(full)
     for (int i = 0; i < FT; ++i) {
 #pragma clang loop unroll(full)
         for (int j = 0; j < FT; ++j) {
-            acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+#pragma clang loop unroll(full)
+            for (int v = 0; v < 4; ++v) {
+                acc[i][j][v] = 0;
+            }
         }
     }
 
@@ -638,6 +681,18 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
     const int bb = min(b0 + c, N - VW);
     const bool odd_a = WIDE && a0 + c == M - 1; // pair shifted left at the last column
     const bool odd_b = WIDE && b0 + c == N - 1;
+    // FP32 form: scale factors of this thread's operand columns (2^-exponent)
+    double fa0 = 1.0, fa1 = 1.0, fb0 = 1.0, fb1 = 1.0;
+    if constexpr (F32) {
+        if (sa != nullptr) {
+            fa0 = ldexp(1.0, -sa[aa]);
+            fa1 = WIDE ? ldexp(1.0, -sa[aa + VW - 1]) : fa0;
+        }
+        if (sb != nullptr) {
+            fb0 = ldexp(1.0, -sb[bb]);
+            fb1 = WIDE ? ldexp(1.0, -sb[bb + VW - 1]) : fb0;
+        }
+    }
     typedef typename std::conditional<WIDE, double2, double>::type vec_t;
     vec_t lreg[NS], rreg[NS];
     int valid = 0; // rows of the staged chunk inside their segment
@@ -679,12 +734,20 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
                     lv = make_double2(0.0, 0.0);
                     rv = make_double2(0.0, 0.0);
                 }
-                *(double2 *)&Ls[buf][r][c] = lv;
-                *(double2 *)&Rs[buf][r][c] = rv;
+                if constexpr (F32) {
+                    const double lx = lv.x * (odd_a ? fa1 : fa0), ly = lv.y * fa1;
+                    const double rx = rv.x * (odd_b ? fb1 : fb0), ry = rv.y * fb1;
+                    *(float2 *)&Ls[buf][r][c] = make_float2((float)lx, (float)ly);
+                    *(float2 *)&Rs[buf][r][c] = make_float2((float)rx, (float)ry);
+                }
+                else {
+                    *(double2 *)&Ls[buf][r][c] = lv;
+                    *(double2 *)&Rs[buf][r][c] = rv;
+                }
             }
             else {
-                Ls[buf][r][c] = in ? lreg[s] : 0.0;
-                Rs[buf][r][c] = in ? rreg[s] : 0.0;
+                Ls[buf][r][c] = (el_t)(in ? lreg[s] * fa0 : 0.0);
+                Rs[buf][r][c] = (el_t)(in ? rreg[s] * fb0 : 0.0);
             }
         }
     };
@@ -703,7 +766,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
         }
 #pragma clang loop unroll(full)
         for (int ks = 0; ks < GK; ks += 4) {
-            double af[FT], bf[FT];
+            el_t af[FT], bf[FT];
 #pragma clang loop unroll(full)
             for (int i = 0; i < FT; ++i) {
                 af[i] = Ls[buf][ks + fk][wa + i * 16 + fr];
@@ -713,7 +776,12 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
             for (int i = 0; i < FT; ++i) {
 #pragma clang loop unroll(full)
                 for (int j = 0; j < FT; ++j) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+                    if constexpr (F32) {
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+                    }
+                    else {
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+                    }
                 }
             }
         }
@@ -723,12 +791,13 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
         __syncthreads();
     }
 
-    // epilogue: lane holds D[lane/16 + 4*v][lane%16] of each 16x16 tile (v_mfma_f64_16x16x4 layout)
+    // epilogue: lane holds D[lane/16 + 4*v][lane%16] of each 16x16 tile (v_mfma_f64_16x16x4
+    // layout), D[4*(lane/16) + v][lane%16] for v_mfma_f32_16x16x4
 #pragma clang loop unroll(full)
     for (int i = 0; i < FT; ++i) {
 #pragma clang loop unroll(full)
         for (int vv = 0; vv < 4; ++vv) {
-            int ar = a0 + wa + i * 16 + fk + 4 * vv;
+            int ar = a0 + wa + i * 16 + (F32 ? 4 * fk + vv : fk + 4 * vv);
             if (ar >= M) {
                 continue;
             }
@@ -737,7 +806,10 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
             for (int j = 0; j < FT; ++j) {
                 int bc = b0 + wb + j * 16 + fr;
                 if (bc < N) {
-                    double d = acc[i][j][vv];
+                    double d = (double)acc[i][j][vv];
+                    if constexpr (F32) {
+                        d = ldexp(d, (sa != nullptr ? sa[ar] : 0) + (sb != nullptr ? sb[bc] : 0));
+                    }
                     double *cp = C + rowoff + bc;
                     if (mode == 1) {
                         d = fma(sig[bc], *cp, d);
@@ -746,6 +818,47 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
                 }
             }
         }
+    }
+}
+
+// out[r] = max(out[r], e) with e the exponent bounding max_k max(|A[r + k lda]|, |B[r + k ldb]|)
+// (B may be null) over the column chunk [64 blockIdx.y, +64) of k < ncols, clamped to
+// [-1000, 1000]; out is filled with -1000 beforehand.  Threads run over rows (coalesced).
+constexpr int ROWEXP_CHUNK = 64;
+__global__ void k_rowexp(const double *__restrict__ A, long lda, const double *__restrict__ B, long ldb, int n,
+                         int ncols, int *__restrict__ out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k0 = blockIdx.y * ROWEXP_CHUNK;
+    if (r >= n) {
+        return;
+    }
+    const int k1 = min(ncols, k0 + ROWEXP_CHUNK);
+    double mx = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        mx = fmax(mx, fabs(A[r + (long)k * lda]));
+        if (B != nullptr) {
+            mx = fmax(mx, fabs(B[r + (long)k * ldb]));
+        }
+    }
+    if (mx > 0.0) {
+        atomicMax(out + r, min(max(ilogb(mx) + 1, -1000), 1000));
+    }
+}
+
+// out[p] = clamp(colexp[p] + xrel[p]): exponent bounding the final column p of X.
+__global__ void k_xfin(const int *__restrict__ colexp, const int *__restrict__ xrel, int m, int *__restrict__ out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < m) {
+        out[p] = min(max(colexp[p] + xrel[p], -1000), 1000);
+    }
+}
+
+__global__ void k_fill_int(int *__restrict__ X, long count, int v)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x) {
+        X[i] = v;
     }
 }
 
@@ -928,7 +1041,8 @@ __global__ void k_zero_int(int *__restrict__ X, long count)
 namespace xtg {
 template __global__ void k_solve_real<65>(const double *, long, const double *, long, int, int, unsigned long long,
                                           double *, long, double *, long, const UnitDesc *, const int2 *, int, int,
-                                          int, int, double *, int *, int *, int *, int, int, double *, int, GroupInfo);
+                                          int, int, double *, int *, int *, int *, int, int, double *, int, GroupInfo,
+                                          MixedExp);
 } // namespace xtg
 
 // =======================================================================================
@@ -1082,10 +1196,12 @@ __global__ __launch_bounds__(64) void k_bwd_diag(const double *__restrict__ X, l
 
 // Host launcher: picks the 16-byte-load form when both output extents are >= 2 and the tile
 // edge (returned) from the tile count.  tag only separates the uses in profiles:
-// 0 = RHS update, 1 = back-transform, 2 = backward.
+// 0 = RHS update, 1 = back-transform, 2 = backward.  prec 1: FP32-product form (tags 0, 1)
+// with operand exponents sa (per a) and sb (per b).
 int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const double *L, long ldl, const double *L2,
                      long ldl2, const double *R, long ldr, double *C, long ldc, const int *outcol, int mode,
-                     const double *sig, const int *kmax, const double *R2)
+                     const double *sig, const int *kmax, const double *R2, int prec, const int *sa,
+                     const int *sb)
 {
     const bool wide = M >= 2 && N >= 2;
     // 128-tiles unless they would leave most CUs idle
@@ -1094,8 +1210,15 @@ int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const
     const int gt = small ? 64 : 128;
     dim3 grid((N + gt - 1) / gt, (M + gt - 1) / gt);
 #define XTG_GEMM_LAUNCH(T, WD, G, K)                                                                                 \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, WD, G, K>), grid, dim3(256), 0, st, M, N, K1, K2, L, ldl, L2,   \
-                       ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2)
+    if (prec == 1 && T != 2) {                                                                                       \
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, WD, G, K, true>), grid, dim3(256), 0, st, M, N, K1, K2, L,  \
+                           ldl, L2, ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2, sa, sb);                      \
+    }                                                                                                                \
+    else {                                                                                                           \
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, WD, G, K>), grid, dim3(256), 0, st, M, N, K1, K2, L, ldl,   \
+                           L2, ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2, (const int *)nullptr,              \
+                           (const int *)nullptr);                                                                    \
+    }
 #define XTG_GEMM_CASE(T)                                                                                             \
     if (tag == T) {                                                                                                  \
         if (small) {                                                                                                 \
