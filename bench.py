@@ -174,8 +174,15 @@ def backward_ratio(seed, n=2048, nsel=256, nbatch=32, reps=5):
     def bwd():
         X.dtgevc3_backward_device(dS, dP, ar, ai, be, sel, Vt, Gt, nbatch=nbatch, dS=gS, dP=gP)
 
+    G1 = Gt[:nsel].contiguous()
+    gS1 = gS[:n].contiguous()
+    gP1 = gP[:n].contiguous()
+
+    def bwd1():
+        X.dtgevc3_backward_device(dS, dP, ar, ai, be, sel, Vt, G1, nbatch=1, dS=gS1, dP=gP1)
+
     out = {}
-    for name, f in (("fwd", fwd), ("bwd", bwd)):
+    for name, f in (("fwd", fwd), ("bwd", bwd), ("bwd1", bwd1)):
         f()
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -185,7 +192,8 @@ def backward_ratio(seed, n=2048, nsel=256, nbatch=32, reps=5):
         out[name] = 1000.0 * (time.perf_counter() - t) / reps
     return {"n": n, "selected": nsel, "batches": nbatch, "fwd_ms": round(out["fwd"], 3),
             "bwd_ms": round(out["bwd"], 3), "bwd_over_fwd": round(out["bwd"] / out["fwd"], 3),
-            "bwd_per_batch_over_fwd": round(out["bwd"] / nbatch / out["fwd"], 4)}
+            "bwd_per_batch_over_fwd": round(out["bwd"] / nbatch / out["fwd"], 4),
+            "bwd_one_batch_ms": round(out["bwd1"], 3), "bwd_one_batch_over_fwd": round(out["bwd1"] / out["fwd"], 3)}
 
 
 def main():
@@ -248,7 +256,7 @@ def main():
     for k, v in prof.items():
         if v["launches"]:
             avg = v["ms"] / v["launches"]
-            log(f"[bench]   {k:11s} launches={v['launches'] // nprof}/step  total={v['ms'] / nprof:.3f} "
+            log(f"[bench]   {k:14s} launches={v['launches'] // nprof}/step  total={v['ms'] / nprof:.3f} "
                 f"ms/step  avg={avg * 1000:.1f} us  {v['flops'] / (v['ms'] * 1e9):.1f} TFLOP/s  "
                 f"{v['bytes'] / (v['ms'] * 1e6):.1f} GB/s")
 
@@ -260,8 +268,11 @@ def main():
 
     # dominant kernel: the kernel instance with the largest share of the timed region (names as
     # rocprofv3 prints them; at n >= 2048 the back-transform uses the 128-tile instance)
-    names = {"solve": "xtg::k_solve_real<65>", "update": "xtg::k_gemm_f64<0, true, 64, 32>",
-             "update_t128": "xtg::k_gemm_f64<0, true, 128, 16>", "back": "xtg::k_gemm_f64<1, true, 128, 16>",
+    names = {"solve": "xtg::k_solve_real<65>", "update": "xtg::k_gemm_f64<0, true, 64, 64, 32, false>",
+             "update_128x128": "xtg::k_gemm_f64<0, true, 128, 128, 16, false>",
+             "update_128x64": "xtg::k_gemm_f64<0, true, 128, 64, 16, false>",
+             "update_64x128": "xtg::k_gemm_f64<0, true, 64, 128, 16, false>",
+             "back": "xtg::k_gemm_f64<1, true, 128, 128, 16, false>",
              "other": "(copies, scale fix-up, normalisation)"}
     dom = max((k for k in prof if prof[k]["launches"] and k != "other"), key=lambda k: prof[k]["ms"])
     d = prof[dom]
@@ -331,7 +342,8 @@ def main():
     if not args.no_backward:
         bw = backward_ratio(args.seed)
         log(f"[bench] backward n={bw['n']} sel={bw['selected']} x{bw['batches']} batches: fwd {bw['fwd_ms']} ms, "
-            f"bwd {bw['bwd_ms']} ms (all batches), ratio {bw['bwd_over_fwd']} ({bw['bwd_per_batch_over_fwd']} per batch)")
+            f"bwd {bw['bwd_ms']} ms (all batches), ratio {bw['bwd_over_fwd']} ({bw['bwd_per_batch_over_fwd']} per batch); "
+            f"one batch {bw['bwd_one_batch_ms']} ms, ratio {bw['bwd_one_batch_over_fwd']}")
 
     cpu = None
     if not args.no_cpu:

@@ -100,12 +100,14 @@ int xtgevc3_dtgevc3_backward_device(int n, const double *S, int lds, const doubl
  * hot kernels is bracketed by HIP events on its stream; xtgevc3_profile_read synchronises,
  * returns per-kind totals (milliseconds, launches, algorithmic flops and bytes) and resets.
  * Kinds: */
-#define XTGEVC3_PROF_SOLVE 0       /* per-column diagonal-block solves */
-#define XTGEVC3_PROF_UPDATE 1      /* off-diagonal RHS update GEMM (64x64 tiles; complex path: all) */
-#define XTGEVC3_PROF_BACK 2        /* back-transform GEMM */
-#define XTGEVC3_PROF_OTHER 3       /* copies, scale fix-up, normalisation */
-#define XTGEVC3_PROF_UPDATE_T128 4 /* off-diagonal RHS update GEMM (128x128 tiles) */
-#define XTGEVC3_PROF_KINDS 5
+#define XTGEVC3_PROF_SOLVE 0             /* per-column diagonal-block solves */
+#define XTGEVC3_PROF_UPDATE 1            /* off-diagonal RHS update GEMM, 64x64 tiles (complex path: all) */
+#define XTGEVC3_PROF_BACK 2              /* back-transform GEMM */
+#define XTGEVC3_PROF_OTHER 3             /* copies, scale fix-up, normalisation */
+#define XTGEVC3_PROF_UPDATE_128x128 4    /* off-diagonal RHS update GEMM, 128x128 tiles */
+#define XTGEVC3_PROF_UPDATE_128x64 5     /* off-diagonal RHS update GEMM, 128x64 tiles */
+#define XTGEVC3_PROF_UPDATE_64x128 6     /* off-diagonal RHS update GEMM, 64x128 tiles */
+#define XTGEVC3_PROF_KINDS 7
 void xtgevc3_profile_enable(int on);
 int xtgevc3_profile_read(double *ms, long *launches, double *flops, double *bytes);
 

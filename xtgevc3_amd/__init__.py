@@ -48,7 +48,7 @@ EXPORTED_SYMBOLS = [
     "xtgevc3_profile_enable", "xtgevc3_profile_read", "xtgevc3_dtgevc3_backward_device",
 ]
 
-PROF_KINDS = ("solve", "update", "back", "other", "update_t128")
+PROF_KINDS = ("solve", "update", "back", "other", "update_128x128", "update_128x64", "update_64x128")
 
 
 def profile_enable(on=True):

@@ -72,6 +72,9 @@ struct ZColDesc {
 
 constexpr int WAVE = 64;
 
+// tile configurations of the FP64 GEMM (a x b)
+enum : int { GEMM_128x128 = 0, GEMM_64x128 = 1, GEMM_128x64 = 2, GEMM_64x64 = 3 };
+
 } // namespace xtg
 
 #define XTG_CHECK(x)                                                                               \

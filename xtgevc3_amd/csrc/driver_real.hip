@@ -58,6 +58,21 @@ __global__ void k_subdiag(const double *, long, int, double *);
 __global__ void k_zero(double *, long);
 __global__ void k_zero_int(int *, long);
 
+// profiling kind of an update GEMM launch by its tile configuration
+static int update_kind(int cfg)
+{
+    switch (cfg) {
+    case GEMM_128x128:
+        return XTGEVC3_PROF_UPDATE_128x128;
+    case GEMM_128x64:
+        return XTGEVC3_PROF_UPDATE_128x64;
+    case GEMM_64x128:
+        return XTGEVC3_PROF_UPDATE_64x128;
+    default:
+        return XTGEVC3_PROF_UPDATE;
+    }
+}
+
 constexpr int MAXB_REAL = 65;  // nominal block 64, +1 when a boundary would split a 2x2 block
 constexpr int NB_REAL = 64;
 
@@ -425,7 +440,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
                                    (const double *)(ws.sig.as<double>() + c_begin), (const int *)nullptr,
                                    (const double *)(W + (long)(wr + woff) * ldw + c_begin), prec,
                                    mx.on ? ws.rowexp.as<int>() + r0 : nullptr, mx.on ? mx.wblk + c_begin : nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
-                prof_end(pr, st, gt == 128 ? XTGEVC3_PROF_UPDATE_T128 : XTGEVC3_PROF_UPDATE, 2.0 * M * N * (2.0 * nb),
+                prof_end(pr, st, update_kind(gt), 2.0 * M * N * (2.0 * nb),
                          8.0 * (2.0 * M * N + 2.0 * nb * M + 2.0 * nb * N));
             }
         }
@@ -444,7 +459,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
                                (const double *)(ws.sig_grp.as<double>() + c_begin), (const int *)nullptr,
                                (const double *)nullptr, prec, mx.on ? ws.rowexp.as<int>() + r0 : nullptr,
                                mx.on ? mx.wgrp + c_begin : nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
-            prof_end(pr, st, gt == 128 ? XTGEVC3_PROF_UPDATE_T128 : XTGEVC3_PROF_UPDATE, 2.0 * M * N * (2.0 * wr),
+            prof_end(pr, st, update_kind(gt), 2.0 * M * N * (2.0 * wr),
                      8.0 * (2.0 * M * N + 2.0 * wr * M + 2.0 * wr * N));
         }
     }

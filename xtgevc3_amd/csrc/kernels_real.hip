@@ -610,7 +610,7 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 // the FP32 operands in range, L[.][a] is scaled by 2^-sa[a] and R[.][b] by 2^-sb[b]
 // (power-of-two exponents bounding the row / column magnitudes; null: no scaling) and D by
 // 2^(sa[a] + sb[b]) in the FP64 epilogue.
-template <int TAG, bool WIDE, int GT, int GK, bool F32 = false>
+template <int TAG, bool WIDE, int GTA, int GTB, int GK, bool F32 = false>
 __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, const double *__restrict__ L, long ldl,
                                                   const double *__restrict__ L2, long ldl2,
                                                   const double *__restrict__ R, long ldr, double *__restrict__ C,
@@ -623,18 +623,18 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
         R2 = R + (long)K1 * ldr;
     }
     typedef typename std::conditional<F32, float, double>::type el_t;
-    __shared__ el_t Ls[2][GK][GT];
-    __shared__ el_t Rs[2][GK][GT];
+    __shared__ el_t Ls[2][GK][GTA];
+    __shared__ el_t Rs[2][GK][GTB];
 
     // with a per-row K bound (triangular operand) the heaviest tiles are dispatched first
     const int ya = kmax != nullptr ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;
-    const int a0 = ya * GT;
-    const int b0 = blockIdx.x * GT;
+    const int a0 = ya * GTA;
+    const int b0 = blockIdx.x * GTB;
     const int tid = threadIdx.x;
     const int wave = tid / WAVE;
     const int lane = tid % WAVE;
-    const int wa = (wave >> 1) * (GT / 2); // wave offset in a
-    const int wb = (wave & 1) * (GT / 2);  // wave offset in b
+    const int wa = (wave >> 1) * (GTA / 2); // wave offset in a
+    const int wb = (wave & 1) * (GTB / 2);  // wave offset in b
 
     int K = K1 + K2;
     if (kmax != nullptr) {
@@ -643,7 +643,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
             kred = 0;
         }
         __syncthreads();
-        if (tid < GT && a0 + tid < M) {
+        if (tid < GTA && a0 + tid < M) {
             atomicMax(&kred, kmax[a0 + tid]);
         }
         __syncthreads();
@@ -654,15 +654,15 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
     const int nch1 = (rows1 + GK - 1) / GK;
     const int nchunks = nch1 + (rows2 + GK - 1) / GK;
 
-    constexpr int WT = GT / 2;  // wave tile edge
-    constexpr int FT = WT / 16; // 16x16 MFMA blocks per wave tile edge
-    static_assert(GK % 4 == 0 && (GT == 64 || GT == 128), "tile shape");
+    constexpr int FTA = GTA / 32; // 16x16 MFMA blocks per wave tile edge (wave tile GTA/2 x GTB/2)
+    constexpr int FTB = GTB / 32;
+    static_assert(GK % 4 == 0 && (GTA == 64 || GTA == 128) && (GTB == 64 || GTB == 128), "tile shape");
     typedef typename std::conditional<F32, float4_t, double4_t>::type acc_t;
-    acc_t acc[FT][FT];
+    acc_t acc[FTA][FTB];
 #pragma clang loop unroll(full)
-    for (int i = 0; i < FT; ++i) {
+    for (int i = 0; i < FTA; ++i) {
 #pragma clang loop unroll(full)
-        for (int j = 0; j < FT; ++j) {
+        for (int j = 0; j < FTB; ++j) {
 #pragma clang loop unroll(full)
             for (int v = 0; v < 4; ++v) {
                 acc[i][j][v] = 0;
@@ -670,17 +670,18 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
         }
     }
 
-    // register staging of one chunk: NS rows per thread (WIDE: column pair c, rows lk + 4 s;
-    // otherwise column c, rows lk + 2 s)
+    // register staging of one chunk.  L: column (pair) cl of the tile, rows lkl + RSL s;
+    // R: column (pair) cr, rows lkr + RSR s.
     constexpr int VW = WIDE ? 2 : 1;
-    constexpr int RSTEP = 256 / (GT / VW);
-    constexpr int NS = GK / RSTEP;
-    const int c = VW * (tid % (GT / VW));
-    const int lk = tid / (GT / VW);
-    const int aa = min(a0 + c, M - VW);
-    const int bb = min(b0 + c, N - VW);
-    const bool odd_a = WIDE && a0 + c == M - 1; // pair shifted left at the last column
-    const bool odd_b = WIDE && b0 + c == N - 1;
+    constexpr int RSL = 256 / (GTA / VW), NSL = GK / RSL;
+    constexpr int RSR = 256 / (GTB / VW), NSR = GK / RSR;
+    static_assert(NSL * RSL == GK && NSR * RSR == GK, "staging layout");
+    const int cl = VW * (tid % (GTA / VW)), lkl = tid / (GTA / VW);
+    const int cr = VW * (tid % (GTB / VW)), lkr = tid / (GTB / VW);
+    const int aa = min(a0 + cl, M - VW);
+    const int bb = min(b0 + cr, N - VW);
+    const bool odd_a = WIDE && a0 + cl == M - 1; // pair shifted left at the last column
+    const bool odd_b = WIDE && b0 + cr == N - 1;
     // FP32 form: scale factors of this thread's operand columns (2^-exponent)
     double fa0 = 1.0, fa1 = 1.0, fb0 = 1.0, fb1 = 1.0;
     if constexpr (F32) {
@@ -694,7 +695,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
         }
     }
     typedef typename std::conditional<WIDE, double2, double>::type vec_t;
-    vec_t lreg[NS], rreg[NS];
+    vec_t lreg[NSL], rreg[NSR];
     int valid = 0; // rows of the staged chunk inside their segment
     auto load_chunk = [&](int ch) {
         const bool s1 = ch < nch1;
@@ -706,48 +707,66 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
         valid = rows - k0;
         if (valid >= GK) {
 #pragma clang loop unroll(full)
-            for (int s = 0; s < NS; ++s) {
-                const int k = k0 + lk + RSTEP * s;
-                lreg[s] = *(const vec_t *)(lb + (long)k * ll);
-                rreg[s] = *(const vec_t *)(rb + (long)k * ldr);
+            for (int s = 0; s < NSL; ++s) {
+                lreg[s] = *(const vec_t *)(lb + (long)(k0 + lkl + RSL * s) * ll);
+            }
+#pragma clang loop unroll(full)
+            for (int s = 0; s < NSR; ++s) {
+                rreg[s] = *(const vec_t *)(rb + (long)(k0 + lkr + RSR * s) * ldr);
             }
         }
         else {
 #pragma clang loop unroll(full)
-            for (int s = 0; s < NS; ++s) {
-                const int k = min(k0 + lk + RSTEP * s, rows - 1);
-                lreg[s] = *(const vec_t *)(lb + (long)k * ll);
-                rreg[s] = *(const vec_t *)(rb + (long)k * ldr);
+            for (int s = 0; s < NSL; ++s) {
+                lreg[s] = *(const vec_t *)(lb + (long)min(k0 + lkl + RSL * s, rows - 1) * ll);
+            }
+#pragma clang loop unroll(full)
+            for (int s = 0; s < NSR; ++s) {
+                rreg[s] = *(const vec_t *)(rb + (long)min(k0 + lkr + RSR * s, rows - 1) * ldr);
             }
         }
     };
     auto store_chunk = [&](int buf) {
 #pragma clang loop unroll(full)
-        for (int s = 0; s < NS; ++s) {
-            const int r = lk + RSTEP * s;
+        for (int s = 0; s < NSL; ++s) {
+            const int r = lkl + RSL * s;
             const bool in = r < valid;
             if constexpr (WIDE) {
-                double2 lv = lreg[s], rv = rreg[s];
+                double2 lv = lreg[s];
                 lv.x = odd_a ? lv.y : lv.x;
-                rv.x = odd_b ? rv.y : rv.x;
                 if (!in) {
                     lv = make_double2(0.0, 0.0);
-                    rv = make_double2(0.0, 0.0);
                 }
                 if constexpr (F32) {
-                    const double lx = lv.x * (odd_a ? fa1 : fa0), ly = lv.y * fa1;
-                    const double rx = rv.x * (odd_b ? fb1 : fb0), ry = rv.y * fb1;
-                    *(float2 *)&Ls[buf][r][c] = make_float2((float)lx, (float)ly);
-                    *(float2 *)&Rs[buf][r][c] = make_float2((float)rx, (float)ry);
+                    *(float2 *)&Ls[buf][r][cl] = make_float2((float)(lv.x * (odd_a ? fa1 : fa0)), (float)(lv.y * fa1));
                 }
                 else {
-                    *(double2 *)&Ls[buf][r][c] = lv;
-                    *(double2 *)&Rs[buf][r][c] = rv;
+                    *(double2 *)&Ls[buf][r][cl] = lv;
                 }
             }
             else {
-                Ls[buf][r][c] = (el_t)(in ? lreg[s] * fa0 : 0.0);
-                Rs[buf][r][c] = (el_t)(in ? rreg[s] * fb0 : 0.0);
+                Ls[buf][r][cl] = (el_t)(in ? lreg[s] * fa0 : 0.0);
+            }
+        }
+#pragma clang loop unroll(full)
+        for (int s = 0; s < NSR; ++s) {
+            const int r = lkr + RSR * s;
+            const bool in = r < valid;
+            if constexpr (WIDE) {
+                double2 rv = rreg[s];
+                rv.x = odd_b ? rv.y : rv.x;
+                if (!in) {
+                    rv = make_double2(0.0, 0.0);
+                }
+                if constexpr (F32) {
+                    *(float2 *)&Rs[buf][r][cr] = make_float2((float)(rv.x * (odd_b ? fb1 : fb0)), (float)(rv.y * fb1));
+                }
+                else {
+                    *(double2 *)&Rs[buf][r][cr] = rv;
+                }
+            }
+            else {
+                Rs[buf][r][cr] = (el_t)(in ? rreg[s] * fb0 : 0.0);
             }
         }
     };
@@ -766,16 +785,19 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
         }
 #pragma clang loop unroll(full)
         for (int ks = 0; ks < GK; ks += 4) {
-            el_t af[FT], bf[FT];
+            el_t af[FTA], bf[FTB];
 #pragma clang loop unroll(full)
-            for (int i = 0; i < FT; ++i) {
+            for (int i = 0; i < FTA; ++i) {
                 af[i] = Ls[buf][ks + fk][wa + i * 16 + fr];
-                bf[i] = Rs[buf][ks + fk][wb + i * 16 + fr];
             }
 #pragma clang loop unroll(full)
-            for (int i = 0; i < FT; ++i) {
+            for (int j = 0; j < FTB; ++j) {
+                bf[j] = Rs[buf][ks + fk][wb + j * 16 + fr];
+            }
 #pragma clang loop unroll(full)
-                for (int j = 0; j < FT; ++j) {
+            for (int i = 0; i < FTA; ++i) {
+#pragma clang loop unroll(full)
+                for (int j = 0; j < FTB; ++j) {
                     if constexpr (F32) {
                         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
                     }
@@ -794,7 +816,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
     // epilogue: lane holds D[lane/16 + 4*v][lane%16] of each 16x16 tile (v_mfma_f64_16x16x4
     // layout), D[4*(lane/16) + v][lane%16] for v_mfma_f32_16x16x4
 #pragma clang loop unroll(full)
-    for (int i = 0; i < FT; ++i) {
+    for (int i = 0; i < FTA; ++i) {
 #pragma clang loop unroll(full)
         for (int vv = 0; vv < 4; ++vv) {
             int ar = a0 + wa + i * 16 + (F32 ? 4 * fk + vv : fk + 4 * vv);
@@ -803,7 +825,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
             }
             long rowoff = (long)(outcol ? outcol[ar] : ar) * ldc;
 #pragma clang loop unroll(full)
-            for (int j = 0; j < FT; ++j) {
+            for (int j = 0; j < FTB; ++j) {
                 int bc = b0 + wb + j * 16 + fr;
                 if (bc < N) {
                     double d = (double)acc[i][j][vv];
@@ -1194,8 +1216,41 @@ __global__ __launch_bounds__(64) void k_bwd_diag(const double *__restrict__ X, l
     }
 }
 
+// Tile configuration of a product with M x N outputs and depth K: the one with the smallest
+// estimated time, counting workgroup rounds (concurrent workgroups per CU from LDS and
+// registers; a last partial round costs about half a round plus its share) times the cost of
+// one full round measured on MI355X (K = 512: 128x64 and 64x128 tiles run three workgroups per
+// CU and beat 128x128 per unit area; for deep products 128x128 wins).
+int gemm_config(int M, int N, int K)
+{
+    struct Cfg {
+        int id, ga, gb, per_cu;
+        double cost, cost_deep; // relative time of one full round (K <= 1024, K > 1024)
+    };
+    static const Cfg cfgs[] = {
+        {GEMM_128x128, 128, 128, 2, 1.00, 1.00},
+        {GEMM_128x64, 128, 64, 3, 0.667, 0.80},
+        {GEMM_64x128, 64, 128, 3, 0.68, 0.82},
+        {GEMM_64x64, 64, 64, 2, 0.26, 0.30},
+    };
+    int best = GEMM_128x128;
+    double tbest = 1e300;
+    for (const Cfg &c : cfgs) {
+        const long tiles = (long)((M + c.ga - 1) / c.ga) * ((N + c.gb - 1) / c.gb);
+        const long conc = 256L * c.per_cu;
+        const long full = tiles / conc, rem = tiles % conc;
+        const double rounds = (double)full + (rem ? 0.5 + 0.5 * (double)rem / conc : 0.0);
+        const double t = rounds * (K > 1024 ? c.cost_deep : c.cost);
+        if (t < tbest) {
+            tbest = t;
+            best = c.id;
+        }
+    }
+    return best;
+}
+
 // Host launcher: picks the 16-byte-load form when both output extents are >= 2 and the tile
-// edge (returned) from the tile count.  tag only separates the uses in profiles:
+// configuration (returned, GEMM_*) from gemm_config.  tag only separates the uses in profiles:
 // 0 = RHS update, 1 = back-transform, 2 = backward.  prec 1: FP32-product form (tags 0, 1)
 // with operand exponents sa (per a) and sb (per b).
 int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const double *L, long ldl, const double *L2,
@@ -1204,46 +1259,50 @@ int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const
                      const int *sb)
 {
     const bool wide = M >= 2 && N >= 2;
-    // 128-tiles unless they would leave most CUs idle
-    const long tiles128 = (long)((M + 127) / 128) * ((N + 127) / 128);
-    const bool small = tiles128 < 256;
-    const int gt = small ? 64 : 128;
-    dim3 grid((N + gt - 1) / gt, (M + gt - 1) / gt);
-#define XTG_GEMM_LAUNCH(T, WD, G, K)                                                                                 \
+    const int cfg = gemm_config(M, N, K1 + K2);
+    const int gta = cfg == GEMM_64x64 || cfg == GEMM_64x128 ? 64 : 128;
+    const int gtb = cfg == GEMM_64x64 || cfg == GEMM_128x64 ? 64 : 128;
+    dim3 grid((N + gtb - 1) / gtb, (M + gta - 1) / gta);
+#define XTG_GEMM_LAUNCH(T, WD, GA, GB, K)                                                                            \
     if (prec == 1 && T != 2) {                                                                                       \
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, WD, G, K, true>), grid, dim3(256), 0, st, M, N, K1, K2, L,  \
-                           ldl, L2, ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2, sa, sb);                      \
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, WD, GA, GB, K, true>), grid, dim3(256), 0, st, M, N, K1, K2, \
+                           L, ldl, L2, ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2, sa, sb);                   \
     }                                                                                                                \
     else {                                                                                                           \
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, WD, G, K>), grid, dim3(256), 0, st, M, N, K1, K2, L, ldl,   \
-                           L2, ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2, (const int *)nullptr,              \
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, WD, GA, GB, K>), grid, dim3(256), 0, st, M, N, K1, K2, L,  \
+                           ldl, L2, ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2, (const int *)nullptr,         \
                            (const int *)nullptr);                                                                    \
+    }
+#define XTG_GEMM_WIDE(T, GA, GB, K)                                                                                  \
+    if (wide) {                                                                                                      \
+        XTG_GEMM_LAUNCH(T, true, GA, GB, K)                                                                          \
+    }                                                                                                                \
+    else {                                                                                                           \
+        XTG_GEMM_LAUNCH(T, false, GA, GB, K)                                                                         \
     }
 #define XTG_GEMM_CASE(T)                                                                                             \
     if (tag == T) {                                                                                                  \
-        if (small) {                                                                                                 \
-            if (wide) {                                                                                              \
-                XTG_GEMM_LAUNCH(T, true, 64, 32);                                                                    \
-            }                                                                                                        \
-            else {                                                                                                   \
-                XTG_GEMM_LAUNCH(T, false, 64, 32);                                                                   \
-            }                                                                                                        \
+        switch (cfg) {                                                                                               \
+        case GEMM_64x64:                                                                                             \
+            XTG_GEMM_WIDE(T, 64, 64, 32)                                                                             \
+            break;                                                                                                   \
+        case GEMM_64x128:                                                                                            \
+            XTG_GEMM_WIDE(T, 64, 128, 16)                                                                            \
+            break;                                                                                                   \
+        case GEMM_128x64:                                                                                            \
+            XTG_GEMM_WIDE(T, 128, 64, 16)                                                                            \
+            break;                                                                                                   \
+        default:                                                                                                     \
+            XTG_GEMM_WIDE(T, 128, 128, 16)                                                                           \
         }                                                                                                            \
-        else {                                                                                                       \
-            if (wide) {                                                                                              \
-                XTG_GEMM_LAUNCH(T, true, 128, 16);                                                                   \
-            }                                                                                                        \
-            else {                                                                                                   \
-                XTG_GEMM_LAUNCH(T, false, 128, 16);                                                                  \
-            }                                                                                                        \
-        }                                                                                                            \
-        return gt;                                                                                                   \
+        return cfg;                                                                                                  \
     }
     XTG_GEMM_CASE(0)
     XTG_GEMM_CASE(1)
     XTG_GEMM_CASE(2)
 #undef XTG_GEMM_CASE
+#undef XTG_GEMM_WIDE
 #undef XTG_GEMM_LAUNCH
-    return 0;
+    return cfg;
 }
 } // namespace xtg
