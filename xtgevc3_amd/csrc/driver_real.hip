@@ -31,7 +31,7 @@ namespace xtg {
 
 template <int MAXB>
 __global__ void k_solve_real(const double *, long, const double *, long, int, int, unsigned long long, double *, long,
-                             double *, long, const UnitDesc *, const int2 *, int, int, int, int, double *, int *, int *,
+                             double *, long, const UnitDesc *, const int4 *, int, int, int, int, double *, int *, int *,
                              int *, int, int, double *, int, GroupInfo, MixedExp);
 __global__ void k_rowexp(const double *, long, const double *, long, int, int, int *);
 __global__ void k_xfin(const int *, const int *, int, int *);
@@ -219,11 +219,47 @@ static void make_units(const std::vector<ColDesc> &desc, std::vector<UnitDesc> &
     }
 }
 
-static void make_tiles(int count, std::vector<int2> &tiles, int width)
+// Waves of the solve kernel: SOLVE_UNITS unit indices each (-1: empty).  Units whose own block
+// is the same row block form one group (the active units of a step are a union of groups);
+// within a group the real units and the pairs go to separate waves, so that waves of real
+// eigenvalues run in real arithmetic.  tmin/tmax: smallest / largest unit of each tile.
+static void make_tiles(const std::vector<UnitDesc> &units, const std::vector<int> &rowstep, int n,
+                       std::vector<int4> &tiles, std::vector<int> &tmin, std::vector<int> &tmax)
 {
+    static_assert(SOLVE_UNITS == 4, "tiles hold four units");
     tiles.clear();
-    for (int s = 0; s < count; s += width) {
-        tiles.push_back(make_int2(s, std::min(width, count - s)));
+    tmin.clear();
+    tmax.clear();
+    const int nu = (int)units.size();
+    auto key = [&](int u) { return units[u].d < n ? rowstep[units[u].d] : -1; };
+    auto emit = [&](const std::vector<int> &list) {
+        for (size_t s = 0; s < list.size(); s += SOLVE_UNITS) {
+            int v[SOLVE_UNITS] = {-1, -1, -1, -1};
+            int lo = INT_MAX, hi = -1;
+            for (int k = 0; k < SOLVE_UNITS && s + k < list.size(); ++k) {
+                v[k] = list[s + k];
+                lo = std::min(lo, v[k]);
+                hi = std::max(hi, v[k]);
+            }
+            tiles.push_back(make_int4(v[0], v[1], v[2], v[3]));
+            tmin.push_back(lo);
+            tmax.push_back(hi);
+        }
+    };
+    std::vector<int> reals, pairs;
+    for (int u0 = 0; u0 < nu;) {
+        int u1 = u0;
+        while (u1 < nu && key(u1) == key(u0)) {
+            ++u1;
+        }
+        reals.clear();
+        pairs.clear();
+        for (int u = u0; u < u1; ++u) {
+            (units[u].w == 2 ? pairs : reals).push_back(u);
+        }
+        emit(reals);
+        emit(pairs);
+        u0 = u1;
     }
 }
 
@@ -312,8 +348,6 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     std::vector<UnitDesc> units;
     std::vector<int> col2unit;
     make_units(desc, units, col2unit);
-    std::vector<int2> tiles;
-    make_tiles((int)units.size(), tiles, SOLVE_UNITS);
     std::vector<int> i0s, nbs;
     partition(n, subw, i0s, nbs);
     const int nsteps = (int)i0s.size();
@@ -323,6 +357,9 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
             rowstep[r] = s;
         }
     }
+    std::vector<int4> tiles;
+    std::vector<int> tmin, tmax;
+    make_tiles(units, rowstep, n, tiles, tmin, tmax);
     const long ldw = ldx;
     const int wmax = GROUP_STEPS * MAXB_REAL;
     ENSURE(ws.W, sizeof(double) * (size_t)(2 * wmax) * ldw);
@@ -333,7 +370,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     ENSURE(ws.flag, sizeof(int) * 4);
 
     ENSURE(ws.desc, sizeof(UnitDesc) * units.size());
-    ENSURE(ws.tiles, sizeof(int2) * tiles.size());
+    ENSURE(ws.tiles, sizeof(int4) * tiles.size());
     ENSURE(ws.rowstep, sizeof(int) * (size_t)n);
     XTG_CHECK(hipMemcpyAsync(ws.desc.p, units.data(), sizeof(UnitDesc) * units.size(), hipMemcpyHostToDevice, st));
     MixedExp mx{nullptr, nullptr, nullptr, 0};
@@ -349,7 +386,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.xrel.as<int>(), (long)m, -1000000); XTG_LAUNCH_CHECK("k_fill_int");
         mx = MixedExp{ws.wblk.as<int>(), ws.wgrp.as<int>(), ws.xrel.as<int>(), 1};
     }
-    XTG_CHECK(hipMemcpyAsync(ws.tiles.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.tiles.p, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice, st));
     XTG_CHECK(hipMemcpyAsync(ws.rowstep.p, rowstep.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     double *X = ws.X.as<double>();
     double *W = ws.W.as<double>();
@@ -406,8 +443,15 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
             }
             any_active = true;
             const int u_begin = col2unit[c_begin], u_end = col2unit[c_end - 1] + 1;
-            const int t_begin = u_begin / SOLVE_UNITS;
-            const int t_end = (u_end + SOLVE_UNITS - 1) / SOLVE_UNITS;
+            // tiles meeting [u_begin, u_end): groups are ordered, so the range is contiguous
+            int t_begin = 0;
+            while (t_begin < (int)tiles.size() && tmax[t_begin] < u_begin) {
+                ++t_begin;
+            }
+            int t_end = (int)tiles.size();
+            while (t_end > t_begin && tmin[t_end - 1] >= u_end) {
+                --t_end;
+            }
             const int ntl = t_end - t_begin;
             const int N = c_end - c_begin;
             const int woff = i0 - g0;
@@ -423,7 +467,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
             int pr = prof_begin(st);
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_real<MAXB_REAL>),
                                dim3((ntl + wpb - 1) / wpb), dim3(wpb * WAVE), 0, st, Sw,
-                               ldsw, Pw, ldpw, i0, nb, bumps, X, ldx, W, ldw, ws.desc.as<UnitDesc>(), ws.tiles.as<int2>(),
+                               ldsw, Pw, ldpw, i0, nb, bumps, X, ldx, W, ldw, ws.desc.as<UnitDesc>(), ws.tiles.as<int4>(),
                                t_begin, t_end, u_begin, u_end, ws.sig.as<double>(), ws.colexp.as<int>(),
                                ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff, wr + woff,
                                ws.sig_grp.as<double>(), first ? 1 : 0, grp, mx); XTG_LAUNCH_CHECK("k_solve_real");

@@ -101,253 +101,236 @@ __device__ __forceinline__ int qbi(int v, int src) { XTG_RB_CASES(row_bcast_i) }
 
 __device__ __forceinline__ cpx qbc(cpx v, int src) { return {qbd(v.r, src), qbd(v.i, src)}; }
 
+constexpr int SP_PITCH = 66; // LDS column pitch (rows) of the staged diagonal block
+
+// Arithmetic of a unit's vector entries: double for waves of real eigenvalues only, cpx when
+// the wave holds a complex pair (real units then carry a zero imaginary part).
+template <class V> struct Ar;
+template <> struct Ar<double> {
+    static __device__ __forceinline__ double mk(double r, double) { return r; }
+    static __device__ __forceinline__ double re(double v) { return v; }
+    static __device__ __forceinline__ double im(double) { return 0.0; }
+    static __device__ __forceinline__ double mul(double x, double y) { return x * y; }
+    static __device__ __forceinline__ double sub(double x, double y) { return x - y; }
+    static __device__ __forceinline__ double recip(double c) { return 1.0 / c; }
+    static __device__ __forceinline__ double abs1(double x) { return fabs(x); }
+    static __device__ __forceinline__ double amax(double x) { return fabs(x); }
+    static __device__ __forceinline__ double bc(double v, int src) { return qbd(v, src); }
+    static __device__ __forceinline__ double scal(double v, double f) { return v * f; }
+    // shifted entry a*S - b*P (b real for these waves)
+    static __device__ __forceinline__ double shift(double a, double bR, double, double2 e) { return a * e.x - bR * e.y; }
+    static __device__ __forceinline__ double bmul(double bR, double, double x) { return bR * x; }
+    // v - S_lr (a x) + P_lr (b x)
+    static __device__ __forceinline__ double upd(double v, double2 e, double ax, double bx)
+    {
+        return fma(e.y, bx, fma(-e.x, ax, v));
+    }
+};
+template <> struct Ar<cpx> {
+    static __device__ __forceinline__ cpx mk(double r, double i) { return {r, i}; }
+    static __device__ __forceinline__ double re(cpx v) { return v.r; }
+    static __device__ __forceinline__ double im(cpx v) { return v.i; }
+    static __device__ __forceinline__ cpx mul(cpx x, cpx y) { return cmul(x, y); }
+    static __device__ __forceinline__ cpx sub(cpx x, cpx y) { return csub(x, y); }
+    static __device__ __forceinline__ cpx recip(cpx c) { return crecip(c); }
+    static __device__ __forceinline__ double abs1(cpx x) { return xtg::abs1(x); }
+    static __device__ __forceinline__ double amax(cpx x) { return fmax(fabs(x.r), fabs(x.i)); }
+    static __device__ __forceinline__ cpx bc(cpx v, int src) { return qbc(v, src); }
+    static __device__ __forceinline__ cpx scal(cpx v, double f) { return {v.r * f, v.i * f}; }
+    static __device__ __forceinline__ cpx shift(double a, double bR, double bI, double2 e)
+    {
+        return {a * e.x - bR * e.y, -bI * e.y};
+    }
+    static __device__ __forceinline__ cpx bmul(double bR, double bI, cpx x)
+    {
+        return {bR * x.r - bI * x.i, bR * x.i + bI * x.r};
+    }
+    static __device__ __forceinline__ cpx upd(cpx v, double2 e, cpx ax, cpx bx)
+    {
+        return {fma(e.y, bx.r, fma(-e.x, ax.r, v.r)), fma(e.y, bx.i, fma(-e.x, ax.i, v.i))};
+    }
+};
+
+// Element-wise select (a ?: on structs would select addresses and keep arrays of them out of
+// registers).
+__device__ __forceinline__ double vsel(bool c, double x, double y) { return c ? x : y; }
+__device__ __forceinline__ cpx vsel(bool c, cpx x, cpx y) { return {c ? x.r : y.r, c ? x.i : y.i}; }
+
 // Robust 2x2 bump solve from its factorization (rare path: some entry would exceed the
 // rescale trigger).  Pivoted rows: rp = sw ? r2 : r1, rs = sw ? r1 : r2;
 // y = rs - mlt rp, x2 = y iu, x1 = (rp - q x2) ip.  Returns the scaled solution and the
 // exponent e of the scale 2^-e applied to the right-hand side.
-struct Bump2 {
-    cpx x1, x2;
+template <class V> struct Bump2 {
+    V x1, x2;
     int e;
 };
 
-__device__ __noinline__ Bump2 bump_solve_scaled(cpx r1, cpx r2, bool sw, cpx mlt, cpx q, cpx iu, cpx ip)
+template <class V>
+__device__ __forceinline__ Bump2<V> bump_solve_scaled(V r1, V r2, bool sw, V mlt, V q, V iu, V ip)
 {
-    cpx rp = sw ? r2 : r1, rs = sw ? r1 : r2;
-    cpx y = csub(rs, cmul(mlt, rp));
-    Bump2 out;
+    typedef Ar<V> A;
+    V rp = vsel(sw, r2, r1), rs = vsel(sw, r1, r2);
+    V y = A::sub(rs, A::mul(mlt, rp));
+    Bump2<V> out;
     out.e = 0;
-    int ea = rescale_exponent(abs1(y), abs1(iu));
+    int ea = rescale_exponent(A::abs1(y), A::abs1(iu));
     if (ea > 0) {
         const double f = ldexp(1.0, -ea);
-        y = {y.r * f, y.i * f};
-        rp = {rp.r * f, rp.i * f};
+        y = A::scal(y, f);
+        rp = A::scal(rp, f);
         out.e += ea;
     }
-    cpx x2 = cmul(y, iu);
-    cpx w = csub(rp, cmul(q, x2));
-    int eb = rescale_exponent(abs1(w), abs1(ip));
+    V x2 = A::mul(y, iu);
+    V w = A::sub(rp, A::mul(q, x2));
+    int eb = rescale_exponent(A::abs1(w), A::abs1(ip));
     if (eb > 0) {
         const double f = ldexp(1.0, -eb);
-        w = {w.r * f, w.i * f};
-        x2 = {x2.r * f, x2.i * f};
+        w = A::scal(w, f);
+        x2 = A::scal(x2, f);
         out.e += eb;
     }
-    out.x1 = cmul(w, ip);
+    out.x1 = A::mul(w, ip);
     out.x2 = x2;
     return out;
 }
 
-// ---------------------------------------------------------------------------------------
-// k_solve_real: one step (row block [i0, i0+nb)) for all active eigenvalue units.
-//
-// X is the working matrix in row-major layout X[row * ldx + col] (rows = vector components,
-// columns = packed working columns).  Rows above the current block hold the running
-// right-hand sides, rows of the block hold RHS on entry and the solution on exit.
-// W receives the update operands of the block's group, row-major W[k * ldw + col]:
-// rows w_s + r hold -a * x_r, rows w_p + r hold (b x)_r, so that
-// X_above := sig * X_above + [S_w | P_w] * W.
-//
-// A unit is one eigenvalue: a real one (one column) or a complex pair (two columns, real
-// and imaginary part); its vector is complex in registers (imaginary part zero for real
-// units).  A 16-lane row serves one unit: lane j keeps rows r = 16 i + j of the block.
-// Before the sweep each lane factors the shifted diagonal of its own rows (independent
-// work): 1x1 rows get the reciprocal of (a S_rr - b P_rr) (perturbed to the rounding level
-// when tiny), 2x2 bumps of S (rows r-1, r: bit r-1 of `bumps`) get a pivoted LU,
-// upper row lane: A = multiplier, B = 1 / pivot; lower row lane: A = U12, B = 1 / U22.
-// Rows outside the unit's system get zero coefficients; the unit's own block gets unit
-// coefficients, its entries being preset to the starting vector.  The sweep then runs
-// bottom-up: the owner lanes turn the right-hand side into the solution with these
-// factors, broadcast it in the row (DPP), and every lane updates its rows above; the
-// {S,P} entries of the next column are fetched from LDS one row ahead.
-// A solution entry above 2^958 makes the unit rescale by a power of two (down to 2^446),
-// recorded as an exponent (blkexp: the column's exponent after this step).
-// Group bookkeeping of the operands W (rows [0, wr): S part, [wr, 2 wr): P part; the group's
-// blocks top-down at local rows grp.off[j], solved in steps grp.step[j]):
-//   * a unit starting in this block (not the group's first step) zeros its operand rows of
-//     the group's lower blocks, which it never wrote;
-//   * in the group's last (top) step each unit brings its operand rows of the lower blocks
-//     to its final scale: W *= 2^(colexp - blkexp[step of the block]).
-// ---------------------------------------------------------------------------------------
-constexpr int SP_PITCH = 66; // LDS column pitch (rows) of the staged diagonal block
+// Per-lane description of one unit in one block step (see k_solve_real).
+struct UnitStep {
+    const double2 *sp; // staged {S,P} block, LDS
+    int nb, j, dl, lol, init;
+    unsigned long long bumps;
+    double a, bR, bI, bn1;
+};
 
-template <int MAXB>
-__global__ __launch_bounds__(256) void k_solve_real(
-    const double *__restrict__ S, long lds, const double *__restrict__ P, long ldp, int i0, int nb,
-    unsigned long long bumps, double *__restrict__ X, long ldx, double *__restrict__ W, long ldw,
-    const UnitDesc *__restrict__ units, const int2 *__restrict__ tiles, int t_begin, int t_end, int u_begin,
-    int u_end, double *__restrict__ sig, int *__restrict__ colexp, int *__restrict__ blkexp, int *__restrict__ flag,
-    int w_s, int w_p, double *__restrict__ sig_grp, int first_in_group, GroupInfo grp, MixedExp mx)
+// Right-hand side of one unit's rows in a block (real, imaginary part per row slot).
+template <int QR> struct RawRHS {
+    double2 v[QR];
+};
+
+// Writes of one unit after its block sweep: X rows and operand rows of the block, the group
+// bookkeeping of W and the column scale state (see k_solve_real).
+struct UnitOut {
+    bool act, pair;
+    int ucols, col, i0;
+    double *X;
+    long ldx;
+    double *W;
+    long ldw;
+    double *sig;
+    int *colexp, *blkexp, *flag;
+    int w_s, w_p;
+    double *sig_grp;
+    int first_in_group;
+    GroupInfo grp;
+    MixedExp mx;
+};
+
+// One unit's block step: factor the lane's rows, sweep the block bottom-up starting from the
+// right-hand side `raw`, then write the block's solution, operands and scale state.
+template <class V, int QR, int MAXB>
+__device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> raw, const UnitOut o)
 {
-    static_assert(MAXB <= 65 && SP_PITCH >= MAXB + 1 && (SP_PITCH % 2) == 0, "block layout");
-    constexpr int QR = (MAXB + QL - 1) / QL; // rows per lane
-    __shared__ double2 sp[SP_PITCH * MAXB];   // sp[l + r * SP_PITCH] = {S_lr, P_lr}
-
-    const int wave = threadIdx.x / WAVE;
-    const int lane = threadIdx.x % WAVE;
-    const int j = lane & (QL - 1);
-    const int cq = lane / QL;
-    const int t = min(t_begin + (int)(blockIdx.x * (blockDim.x / WAVE)) + wave, t_end - 1);
-    const bool wave_live = t_begin + (int)(blockIdx.x * (blockDim.x / WAVE)) + wave < t_end;
-    const int2 tl = tiles[t];
-    const int u = tl.x + cq;
-    const bool act = wave_live && cq < tl.y && u >= u_begin && u < u_end;
-    const UnitDesc ud = units[act ? u : tl.x];
-    const int col = ud.col;
-    const bool pair = ud.w == 2;
-    const double a = ud.a, bR = ud.bR, bI = ud.bI;
-    const double bn1 = fabs(bR) + fabs(bI);
-    const int dl = act ? ud.d - i0 : -1 - MAXB; // inactive units: every row is a zero row
-    const int lol = ud.lo - i0;
-
-    // right-hand side of the unit's rows in the block (loads issued before the staging)
-    cpx vr[QR];
+    typedef Ar<V> A;
+    V vr[QR];
 #pragma clang loop unroll(full)
     for (int i = 0; i < QR; ++i) {
-        const int r = QL * i + j;
-        const bool ld = r < nb && r < dl && r >= lol;
-        const double *xp = X + (long)(i0 + r) * ldx + col;
-        vr[i].r = ld ? xp[0] : 0.0;
-        vr[i].i = (ld && pair) ? xp[1] : 0.0;
+        vr[i] = A::mk(raw.v[i].x, raw.v[i].y);
     }
-
-    // stage {S,P} of the diagonal block: item (r, pair t) holds rows 2t, 2t+1 of column r
-    {
-        constexpr int NPAIR = SP_PITCH / 2;
-        constexpr int NITEM = NPAIR * MAXB;
-        constexpr int BATCH = 12;
-        for (int q0 = threadIdx.x; q0 < NITEM; q0 += BATCH * blockDim.x) {
-            double2 ts[BATCH], tp[BATCH];
-#pragma clang loop unroll(full)
-            for (int k = 0; k < BATCH; ++k) {
-                const int q = q0 + k * blockDim.x;
-                const int r = q / NPAIR, l = 2 * (q % NPAIR);
-                ts[k] = make_double2(0.0, 0.0);
-                tp[k] = make_double2(0.0, 0.0);
-                if (q < NITEM && r < nb && l < nb) {
-                    const double *sa = S + (i0 + l) + (long)(i0 + r) * lds;
-                    const double *pa = P + (i0 + l) + (long)(i0 + r) * ldp;
-                    if (l + 1 < nb) {
-                        ts[k] = *(const double2 *)sa;
-                        tp[k] = *(const double2 *)pa;
-                    }
-                    else { // last row of an odd-length column
-                        ts[k] = make_double2(sa[0], 0.0);
-                        tp[k] = make_double2(pa[0], 0.0);
-                    }
-                }
-            }
-#pragma clang loop unroll(full)
-            for (int k = 0; k < BATCH; ++k) {
-                const int q = q0 + k * blockDim.x;
-                const int r = q / NPAIR, l = 2 * (q % NPAIR);
-                if (q < NITEM) {
-                    sp[l + r * SP_PITCH] = make_double2(ts[k].x, tp[k].x);
-                    sp[l + 1 + r * SP_PITCH] = make_double2(ts[k].y, tp[k].y);
-                }
-            }
-        }
-    }
-    __syncthreads();
+    int my_exp = 0;
+    const double2 *sp = u.sp;
+    const int nb = u.nb, j = u.j, dl = u.dl, lol = u.lol;
+    const double a = u.a, bR = u.bR, bI = u.bI, bn1 = u.bn1;
 
     // starting vector on the unit's own block
     if (dl >= 0 && dl < nb) {
-        cpx x0 = {0.0, 0.0}, x1v = {0.0, 0.0};
-        if (ud.init == INIT_UNIT) {
-            x0 = {1.0, 0.0};
+        V x0 = A::mk(0.0, 0.0), x1v = A::mk(0.0, 0.0);
+        if (u.init == INIT_UNIT) {
+            x0 = A::mk(1.0, 0.0);
         }
-        else if (ud.init == INIT_PAIR_ROW1 && dl + 1 < nb) {
+        else if (u.init == INIT_PAIR_ROW1 && dl + 1 < nb) {
             double2 e11 = sp[dl + dl * SP_PITCH], e12 = sp[dl + (dl + 1) * SP_PITCH];
-            x0 = {-(a * e12.x - bR * e12.y), bI * e12.y};
-            x1v = {a * e11.x - bR * e11.y, -bI * e11.y};
+            x0 = A::mk(-(a * e12.x - bR * e12.y), bI * e12.y);
+            x1v = A::mk(a * e11.x - bR * e11.y, -bI * e11.y);
         }
-        else if (ud.init == INIT_PAIR_ROW2 && dl + 1 < nb) {
+        else if (u.init == INIT_PAIR_ROW2 && dl + 1 < nb) {
             double2 e22 = sp[(dl + 1) + (dl + 1) * SP_PITCH], e21 = sp[(dl + 1) + dl * SP_PITCH];
-            x0 = {a * e22.x - bR * e22.y, -bI * e22.y};
-            x1v = {-a * e21.x, 0.0};
+            x0 = A::mk(a * e22.x - bR * e22.y, -bI * e22.y);
+            x1v = A::mk(-a * e21.x, 0.0);
         }
 #pragma clang loop unroll(full)
         for (int i = 0; i < QR; ++i) {
             const int r = QL * i + j;
-            if (r == dl) {
-                vr[i] = x0;
-            }
-            else if (r == dl + 1 && ud.init != INIT_UNIT) {
-                vr[i] = x1v;
-            }
+            vr[i] = vsel(r == dl, x0, vsel(r == dl + 1 && u.init != INIT_UNIT, x1v, vr[i]));
         }
     }
 
     // factor the shifted diagonal of this lane's rows
-    cpx fa[QR], fb[QR];
+    V fa[QR], fb[QR];
     unsigned swbits = 0;
 #pragma clang loop unroll(full)
     for (int i = 0; i < QR; ++i) {
         const int r = QL * i + j;
-        fa[i] = {0.0, 0.0};
-        fb[i] = {0.0, 0.0};
+        fa[i] = A::mk(0.0, 0.0);
+        fb[i] = A::mk(0.0, 0.0);
         if (r < nb) {
-            const bool lower = r >= 1 && ((bumps >> (r - 1)) & 1ull);
-            const bool upper = r + 1 < nb && ((bumps >> r) & 1ull);
+            const bool lower = r >= 1 && ((u.bumps >> (r - 1)) & 1ull);
+            const bool upper = r + 1 < nb && ((u.bumps >> r) & 1ull);
             if (!lower && !upper) {
                 const double2 ee = sp[r + r * SP_PITCH];
-                cpx c = {a * ee.x - bR * ee.y, -bI * ee.y};
+                V c = A::shift(a, bR, bI, ee);
                 const double dmin = fmax(kEps * (fabs(a) * fabs(ee.x) + bn1 * fabs(ee.y)), kSafeMin);
-                if (abs1(c) < dmin) {
-                    c = {dmin, 0.0};
+                if (A::abs1(c) < dmin) {
+                    c = A::mk(dmin, 0.0);
                 }
                 const bool solve = r < dl && r >= lol;
-                fa[i] = solve ? crecip(c) : (r == dl ? cpx{1.0, 0.0} : cpx{0.0, 0.0});
+                fa[i] = vsel(solve, A::recip(c), A::mk(r == dl ? 1.0 : 0.0, 0.0));
             }
             else {
                 const int r1 = lower ? r - 1 : r, r2 = r1 + 1;
                 const double2 e11 = sp[r1 + r1 * SP_PITCH], e12 = sp[r1 + r2 * SP_PITCH];
                 const double2 e21 = sp[r2 + r1 * SP_PITCH], e22 = sp[r2 + r2 * SP_PITCH];
-                cpx c11 = {a * e11.x - bR * e11.y, -bI * e11.y};
-                cpx c12 = {a * e12.x - bR * e12.y, -bI * e12.y};
-                cpx c21 = {a * e21.x - bR * e21.y, -bI * e21.y};
-                cpx c22 = {a * e22.x - bR * e22.y, -bI * e22.y};
+                V c11 = A::shift(a, bR, bI, e11), c12 = A::shift(a, bR, bI, e12);
+                V c21 = A::shift(a, bR, bI, e21), c22 = A::shift(a, bR, bI, e22);
                 const double dm1 = fmax(kEps * (fabs(a) * (fabs(e11.x) + fabs(e21.x)) + bn1 * fabs(e11.y)), kSafeMin);
                 const double dm2 = fmax(kEps * (fabs(a) * (fabs(e12.x) + fabs(e22.x)) + bn1 * fabs(e22.y)), kSafeMin);
                 const bool s1 = r1 < dl && r1 >= lol, s2 = r2 < dl && r2 >= lol;
                 bool sw = false;
-                cpx mlt = {0.0, 0.0}, q = {0.0, 0.0}, iu = {0.0, 0.0}, ip = {0.0, 0.0};
+                V mlt = A::mk(0.0, 0.0), q = A::mk(0.0, 0.0), iu = A::mk(0.0, 0.0), ip = A::mk(0.0, 0.0);
                 if (r1 == dl) {
                     // the unit's own 2x2 block: entries preset, x = rhs
-                    iu = {1.0, 0.0};
-                    ip = {1.0, 0.0};
+                    iu = A::mk(1.0, 0.0);
+                    ip = A::mk(1.0, 0.0);
                 }
                 else if (s1) {
-                    sw = abs1(c21) > abs1(c11);
-                    cpx pv = sw ? c21 : c11, sv = sw ? c11 : c21, tv = sw ? c12 : c22;
-                    q = sw ? c22 : c12;
-                    if (abs1(pv) < dm1) {
-                        pv = {dm1, 0.0};
+                    sw = A::abs1(c21) > A::abs1(c11);
+                    V pv = vsel(sw, c21, c11), sv = vsel(sw, c11, c21), tv = vsel(sw, c12, c22);
+                    q = vsel(sw, c22, c12);
+                    if (A::abs1(pv) < dm1) {
+                        pv = A::mk(dm1, 0.0);
                     }
-                    ip = crecip(pv);
-                    mlt = cmul(sv, ip);
-                    cpx uv = csub(tv, cmul(mlt, q));
-                    if (abs1(uv) < dm2) {
-                        uv = {dm2, 0.0};
+                    ip = A::recip(pv);
+                    mlt = A::mul(sv, ip);
+                    V uv = A::sub(tv, A::mul(mlt, q));
+                    if (A::abs1(uv) < dm2) {
+                        uv = A::mk(dm2, 0.0);
                     }
-                    iu = crecip(uv);
+                    iu = A::recip(uv);
                 }
                 else if (s2) {
                     // upper row outside the system (zero): x2 = rhs2 / c22
-                    if (abs1(c22) < dm2) {
-                        c22 = {dm2, 0.0};
+                    if (A::abs1(c22) < dm2) {
+                        c22 = A::mk(dm2, 0.0);
                     }
-                    iu = crecip(c22);
+                    iu = A::recip(c22);
                 }
-                fa[i] = lower ? q : mlt;
-                fb[i] = lower ? iu : ip;
+                fa[i] = vsel(lower, q, mlt);
+                fb[i] = vsel(lower, iu, ip);
                 swbits |= sw ? (1u << i) : 0u;
             }
         }
     }
 
-    int my_exp = 0;
     bool pre = false; // row r was solved with row r+1 (2x2 bump)
-    const cpx bv = {bR, bI};
     double2 ec[QR], en[QR]; // {S,P} of this lane's rows in column r (ec) and r-1 (en)
 #pragma clang loop unroll(full)
     for (int i = 0; i < QR; ++i) {
@@ -376,65 +359,49 @@ __global__ __launch_bounds__(256) void k_solve_real(
             if (pre) {
                 pre = false;
             }
-            else if (r >= 1 && ((bumps >> (r - 1)) & 1ull)) {
+            else if (r >= 1 && ((u.bumps >> (r - 1)) & 1ull)) {
                 // 2x2 bump at rows (r-1, r)
                 const int j1 = (r - 1) & (QL - 1), i1 = (r - 1) / QL;
                 const int j2 = r & (QL - 1), i2 = r / QL;
-                const cpx rhs1 = qbc(vr[i1], j1), rhs2 = qbc(vr[i2], j2);
+                const V rhs1 = A::bc(vr[i1], j1), rhs2 = A::bc(vr[i2], j2);
                 const bool sw1 = (swbits >> i1) & 1u, sw2 = (swbits >> i2) & 1u;
                 // upper-row lane: y = rs - mlt rp
-                cpx y = csub(sw1 ? rhs1 : rhs2, cmul(fa[i1], sw1 ? rhs2 : rhs1));
-                y = qbc(y, j1);
+                V y = A::bc(A::sub(vsel(sw1, rhs1, rhs2), A::mul(fa[i1], vsel(sw1, rhs2, rhs1))), j1);
                 // lower-row lane: x2 = y / U22, w = rp - U12 x2
-                cpx x2 = cmul(y, fb[i2]);
-                cpx w = csub(sw2 ? rhs2 : rhs1, cmul(fa[i2], x2));
-                x2 = qbc(x2, j2);
-                w = qbc(w, j2);
+                V x2 = A::mul(y, fb[i2]);
+                V w = A::sub(vsel(sw2, rhs2, rhs1), A::mul(fa[i2], x2));
+                x2 = A::bc(x2, j2);
+                w = A::bc(w, j2);
                 // upper-row lane: x1 = w / pivot
-                cpx x1 = qbc(cmul(w, fb[i1]), j1);
-                const bool bad = !(abs1(x1) <= kScaleTrigger && abs1(x2) <= kScaleTrigger);
+                V x1 = A::bc(A::mul(w, fb[i1]), j1);
+                const bool bad = !(A::abs1(x1) <= kScaleTrigger && A::abs1(x2) <= kScaleTrigger);
                 if (__any(bad)) {
-                    const cpx mlt = qbc(fa[i1], j1), ip = qbc(fb[i1], j1);
-                    const cpx q = qbc(fa[i2], j2), iu = qbc(fb[i2], j2);
+                    const V mlt = A::bc(fa[i1], j1), ip = A::bc(fb[i1], j1);
+                    const V q = A::bc(fa[i2], j2), iu = A::bc(fb[i2], j2);
                     const bool sw = qbi((int)sw1, j1) != 0;
-                    Bump2 bs = bump_solve_scaled(rhs1, rhs2, sw, mlt, q, iu, ip);
+                    Bump2<V> bs = bump_solve_scaled<V>(rhs1, rhs2, sw, mlt, q, iu, ip);
                     const int e = bad ? bs.e : 0;
-                    if (bad) {
-                        x1 = bs.x1;
-                        x2 = bs.x2;
-                    }
+                    x1 = vsel(bad, bs.x1, x1);
+                    x2 = vsel(bad, bs.x2, x2);
                     const double f = ldexp(1.0, -e);
 #pragma clang loop unroll(full)
                     for (int i = 0; i < QR; ++i) {
-                        vr[i] = {vr[i].r * f, vr[i].i * f};
+                        vr[i] = A::scal(vr[i], f);
                     }
                     my_exp -= e;
                 }
-                if (j == j1) {
-                    vr[i1] = x1;
-                }
-                if (j == j2) {
-                    vr[i2] = x2;
-                }
-                const cpx ax1 = {a * x1.r, a * x1.i}, bx1 = cmul(bv, x1);
-                const cpx ax2 = {a * x2.r, a * x2.i}, bx2 = cmul(bv, x2);
+                vr[i1] = vsel(j == j1, x1, vr[i1]);
+                vr[i2] = vsel(j == j2, x2, vr[i2]);
+                const V ax1 = A::scal(x1, a), bx1 = A::bmul(bR, bI, x1);
+                const V ax2 = A::scal(x2, a), bx2 = A::bmul(bR, bI, x2);
 #pragma clang loop unroll(full)
                 for (int i = 0; i < QR; ++i) {
-                    if (QL * i < r - 1) {
-                        const int l = QL * i + j;
-                        const double2 e1 = en[i];
-                        const double2 e2 = ec[i];
-                        double nr = fma(-e1.x, ax1.r, vr[i].r);
-                        double ni = fma(-e1.x, ax1.i, vr[i].i);
-                        nr = fma(e1.y, bx1.r, nr);
-                        ni = fma(e1.y, bx1.i, ni);
-                        nr = fma(-e2.x, ax2.r, nr);
-                        ni = fma(-e2.x, ax2.i, ni);
-                        nr = fma(e2.y, bx2.r, nr);
-                        ni = fma(e2.y, bx2.i, ni);
-                        const bool upd = l < r - 1;
-                        vr[i].r = upd ? nr : vr[i].r;
-                        vr[i].i = upd ? ni : vr[i].i;
+                    if (QL * i + QL - 1 < r - 1) {
+                        vr[i] = A::upd(A::upd(vr[i], en[i], ax1, bx1), ec[i], ax2, bx2);
+                    }
+                    else if (QL * i < r - 1) {
+                        const V nv = A::upd(A::upd(vr[i], en[i], ax1, bx1), ec[i], ax2, bx2);
+                        vr[i] = vsel(QL * i + j < r - 1, nv, vr[i]);
                     }
                 }
                 pre = true;
@@ -442,35 +409,29 @@ __global__ __launch_bounds__(256) void k_solve_real(
             else {
                 // 1x1 row
                 const int jr = r & (QL - 1), ir = r / QL;
-                cpx x = qbc(cmul(vr[ir], fa[ir]), jr);
-                const bool bad = !(abs1(x) <= kScaleTrigger);
+                V x = A::bc(A::mul(vr[ir], fa[ir]), jr);
+                const bool bad = !(A::abs1(x) <= kScaleTrigger);
                 if (__any(bad)) {
-                    const int eo = bad ? rescale_exponent(abs1(vr[ir]), abs1(fa[ir])) : 0;
+                    const int eo = bad ? rescale_exponent(A::abs1(vr[ir]), A::abs1(fa[ir])) : 0;
                     const int e = qbi(eo, jr);
                     const double f = ldexp(1.0, -e);
 #pragma clang loop unroll(full)
                     for (int i = 0; i < QR; ++i) {
-                        vr[i] = {vr[i].r * f, vr[i].i * f};
+                        vr[i] = A::scal(vr[i], f);
                     }
                     my_exp -= e;
-                    x = qbc(cmul(vr[ir], fa[ir]), jr);
+                    x = A::bc(A::mul(vr[ir], fa[ir]), jr);
                 }
-                if (j == jr) {
-                    vr[ir] = x;
-                }
-                const cpx ax = {a * x.r, a * x.i}, bx = cmul(bv, x);
+                vr[ir] = vsel(j == jr, x, vr[ir]);
+                const V ax = A::scal(x, a), bx = A::bmul(bR, bI, x);
 #pragma clang loop unroll(full)
                 for (int i = 0; i < QR; ++i) {
-                    if (QL * i < r) {
-                        const int l = QL * i + j;
-                        const double2 el = ec[i];
-                        double nr = fma(-el.x, ax.r, vr[i].r);
-                        double ni = fma(-el.x, ax.i, vr[i].i);
-                        nr = fma(el.y, bx.r, nr);
-                        ni = fma(el.y, bx.i, ni);
-                        const bool upd = l < r;
-                        vr[i].r = upd ? nr : vr[i].r;
-                        vr[i].i = upd ? ni : vr[i].i;
+                    if (QL * i + QL - 1 < r) {
+                        vr[i] = A::upd(vr[i], ec[i], ax, bx);
+                    }
+                    else if (QL * i < r) {
+                        const V nv = A::upd(vr[i], ec[i], ax, bx);
+                        vr[i] = vsel(QL * i + j < r, nv, vr[i]);
                     }
                 }
             }
@@ -481,23 +442,38 @@ __global__ __launch_bounds__(256) void k_solve_real(
         }
     }
 
+    // ---- writes ----
+    const UnitStep &us = u;
+    const bool act = o.act, pair = o.pair;
+    const int ucols = o.ucols, col = o.col, i0 = o.i0;
+    double *__restrict__ X = o.X;
+    double *__restrict__ W = o.W;
+    double *__restrict__ sig = o.sig;
+    int *__restrict__ colexp = o.colexp;
+    int *__restrict__ blkexp = o.blkexp;
+    int *__restrict__ flag = o.flag;
+    double *__restrict__ sig_grp = o.sig_grp;
+    const long ldx = o.ldx, ldw = o.ldw;
+    const int w_s = o.w_s, w_p = o.w_p, first_in_group = o.first_in_group;
+    const GroupInfo &grp = o.grp;
+    const MixedExp &mx = o.mx;
     // X and W of this block: rows w_s + r (S part: -a x) and w_p + r (P part: b x)
 #pragma clang loop unroll(full)
     for (int i = 0; i < QR; ++i) {
         const int r = QL * i + j;
         if (act && r < nb) {
-            const cpx v = vr[i];
-            const cpx bx = cmul(bv, v);
+            const V v = vr[i];
+            const V bx = A::bmul(bR, bI, v);
             double *xr = X + (long)(i0 + r) * ldx + col;
             double *ws = W + (long)(w_s + r) * ldw + col;
             double *wp = W + (long)(w_p + r) * ldw + col;
-            xr[0] = v.r;
-            ws[0] = -a * v.r;
-            wp[0] = bx.r;
+            xr[0] = A::re(v);
+            ws[0] = -a * A::re(v);
+            wp[0] = A::re(bx);
             if (pair) {
-                xr[1] = v.i;
-                ws[1] = -a * v.i;
-                wp[1] = bx.i;
+                xr[1] = A::im(v);
+                ws[1] = -a * A::im(v);
+                wp[1] = A::im(bx);
             }
         }
     }
@@ -544,18 +520,18 @@ __global__ __launch_bounds__(256) void k_solve_real(
 #pragma clang loop unroll(full)
         for (int i = 0; i < QR; ++i) {
             if (QL * i + j < nb) {
-                mxv = fmax(mxv, fmax(fabs(vr[i].r), fabs(vr[i].i)));
+                mxv = fmax(mxv, A::amax(vr[i]));
             }
         }
 #pragma clang loop unroll(full)
         for (int off = 1; off < QL; off <<= 1) {
             mxv = fmax(mxv, __shfl_xor(mxv, off));
         }
-        const double mw = mxv * fmax(fabs(a), 2.0 * bn1);
+        const double mw = mxv * fmax(fabs(a), 2.0 * us.bn1);
         // zero blocks: a sentinel far below any exponent
         const int ex = mxv > 0.0 ? ilogb(mxv) + 1 : -1000000;
         const int ew = mw > 0.0 ? ilogb(mw) + 1 : -1000000;
-        if (act && j < ud.w) {
+        if (act && j < ucols) {
             const int p = col + j;
             const int ce = colexp[p] + my_exp;
             const bool fresh = first_in_group || starts_here;
@@ -569,7 +545,7 @@ __global__ __launch_bounds__(256) void k_solve_real(
             mx.xrel[p] = max(mx.xrel[p], ex - ce);
         }
     }
-    if (act && j < ud.w) {
+    if (act && j < ucols) {
         // lane 0: real part column, lane 1: imaginary part column
         const int p = col + j;
         const double sg = ldexp(1.0, my_exp);
@@ -583,6 +559,153 @@ __global__ __launch_bounds__(256) void k_solve_real(
         if (my_exp != 0) {
             *flag = 1;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// k_solve_real: one step (row block [i0, i0+nb)) for all active eigenvalue units.
+//
+// X is the working matrix in row-major layout X[row * ldx + col] (rows = vector components,
+// columns = packed working columns).  Rows above the current block hold the running
+// right-hand sides, rows of the block hold RHS on entry and the solution on exit.
+// W receives the update operands of the block's group, row-major W[k * ldw + col]:
+// rows w_s + r hold -a * x_r, rows w_p + r hold (b x)_r, so that
+// X_above := sig * X_above + [S_w | P_w] * W.
+//
+// A unit is one eigenvalue: a real one (one column) or a complex pair (two columns, real
+// and imaginary part).  A 16-lane row serves one unit: lane j keeps rows r = 16 i + j of the
+// block.  A wave holds four units (tiles[t]: unit indices, -1 = empty); waves whose active
+// units are all real sweep in real arithmetic, the others in complex arithmetic (real units
+// then carry a zero imaginary part).  Before the sweep each lane factors its own rows
+// independently: 1x1 rows get the reciprocal of the shifted diagonal (a S_rr - b P_rr,
+// perturbed to the rounding level when tiny), 2x2 bumps of S (rows r-1, r: bit r-1 of
+// `bumps`) get a pivoted LU, upper row lane: A = multiplier, B = 1 / pivot; lower row lane:
+// A = U12, B = 1 / U22.  Rows outside the unit's system get zero coefficients; the unit's own
+// block gets unit coefficients, its entries being preset to the starting vector.  The sweep
+// then runs bottom-up: the owner lanes turn the right-hand side into the solution with these
+// factors, broadcast it in the row (DPP), and every lane updates its rows above; the {S,P}
+// entries of the next column are fetched from LDS one row ahead.
+// A solution entry above 2^958 makes the unit rescale by a power of two (down to 2^446),
+// recorded as an exponent (blkexp: the column's exponent after this step).
+// Group bookkeeping of the operands W (rows [0, wr): S part, [wr, 2 wr): P part; the group's
+// blocks top-down at local rows grp.off[j], solved in steps grp.step[j]):
+//   * a unit starting in this block (not the group's first step) zeros its operand rows of
+//     the group's lower blocks, which it never wrote;
+//   * in the group's last (top) step each unit brings its operand rows of the lower blocks
+//     to its final scale: W *= 2^(colexp - blkexp[step of the block]).
+// ---------------------------------------------------------------------------------------
+template <int MAXB>
+__global__ __launch_bounds__(256) void k_solve_real(
+    const double *__restrict__ S, long lds, const double *__restrict__ P, long ldp, int i0, int nb,
+    unsigned long long bumps, double *__restrict__ X, long ldx, double *__restrict__ W, long ldw,
+    const UnitDesc *__restrict__ units, const int4 *__restrict__ tiles, int t_begin, int t_end, int u_begin,
+    int u_end, double *__restrict__ sig, int *__restrict__ colexp, int *__restrict__ blkexp, int *__restrict__ flag,
+    int w_s, int w_p, double *__restrict__ sig_grp, int first_in_group, GroupInfo grp, MixedExp mx)
+{
+    static_assert(MAXB <= 65 && SP_PITCH >= MAXB + 1 && (SP_PITCH % 2) == 0, "block layout");
+    constexpr int QR = (MAXB + QL - 1) / QL; // rows per lane
+    __shared__ double2 sp[SP_PITCH * MAXB];   // sp[l + r * SP_PITCH] = {S_lr, P_lr}
+
+    const int wave = threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    const int j = lane & (QL - 1);
+    const int cq = lane / QL;
+    const int t = min(t_begin + (int)(blockIdx.x * (blockDim.x / WAVE)) + wave, t_end - 1);
+    const bool wave_live = t_begin + (int)(blockIdx.x * (blockDim.x / WAVE)) + wave < t_end;
+    const int4 tl = tiles[t];
+    const int u = cq == 0 ? tl.x : (cq == 1 ? tl.y : (cq == 2 ? tl.z : tl.w));
+    const bool act = wave_live && u >= 0 && u >= u_begin && u < u_end;
+    const UnitDesc ud = units[act ? u : tl.x];
+    const int col = ud.col;
+    const bool pair = ud.w == 2;
+    UnitStep us;
+    us.sp = sp;
+    us.nb = nb;
+    us.j = j;
+    us.dl = act ? ud.d - i0 : -1 - MAXB; // inactive units: every row is a zero row
+    us.lol = ud.lo - i0;
+    us.init = ud.init;
+    us.bumps = bumps;
+    us.a = ud.a;
+    us.bR = ud.bR;
+    us.bI = ud.bI;
+    us.bn1 = fabs(ud.bR) + fabs(ud.bI);
+
+    // right-hand side of the unit's rows in the block (loads issued before the staging)
+    RawRHS<QR> raw;
+#pragma clang loop unroll(full)
+    for (int i = 0; i < QR; ++i) {
+        const int r = QL * i + j;
+        const bool ld = r < nb && r < us.dl && r >= us.lol;
+        const double *xp = X + (long)(i0 + r) * ldx + col;
+        raw.v[i].x = ld ? xp[0] : 0.0;
+        raw.v[i].y = (ld && pair) ? xp[1] : 0.0;
+    }
+
+    // stage {S,P} of the diagonal block: item (r, pair t) holds rows 2t, 2t+1 of column r
+    {
+        constexpr int NPAIR = SP_PITCH / 2;
+        constexpr int NITEM = NPAIR * MAXB;
+        constexpr int BATCH = 12;
+        for (int q0 = threadIdx.x; q0 < NITEM; q0 += BATCH * blockDim.x) {
+            double2 ts[BATCH], tp[BATCH];
+#pragma clang loop unroll(full)
+            for (int k = 0; k < BATCH; ++k) {
+                const int q = q0 + k * blockDim.x;
+                const int r = q / NPAIR, l = 2 * (q % NPAIR);
+                ts[k] = make_double2(0.0, 0.0);
+                tp[k] = make_double2(0.0, 0.0);
+                if (q < NITEM && r < nb && l < nb) {
+                    const double *sa = S + (i0 + l) + (long)(i0 + r) * lds;
+                    const double *pa = P + (i0 + l) + (long)(i0 + r) * ldp;
+                    if (l + 1 < nb) {
+                        ts[k] = *(const double2 *)sa;
+                        tp[k] = *(const double2 *)pa;
+                    }
+                    else { // last row of an odd-length column
+                        ts[k] = make_double2(sa[0], 0.0);
+                        tp[k] = make_double2(pa[0], 0.0);
+                    }
+                }
+            }
+#pragma clang loop unroll(full)
+            for (int k = 0; k < BATCH; ++k) {
+                const int q = q0 + k * blockDim.x;
+                const int r = q / NPAIR, l = 2 * (q % NPAIR);
+                if (q < NITEM) {
+                    sp[l + r * SP_PITCH] = make_double2(ts[k].x, tp[k].x);
+                    sp[l + 1 + r * SP_PITCH] = make_double2(ts[k].y, tp[k].y);
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    UnitOut o;
+    o.act = act;
+    o.pair = pair;
+    o.ucols = ud.w;
+    o.col = col;
+    o.i0 = i0;
+    o.X = X;
+    o.ldx = ldx;
+    o.W = W;
+    o.ldw = ldw;
+    o.sig = sig;
+    o.colexp = colexp;
+    o.blkexp = blkexp;
+    o.flag = flag;
+    o.w_s = w_s;
+    o.w_p = w_p;
+    o.sig_grp = sig_grp;
+    o.first_in_group = first_in_group;
+    o.grp = grp;
+    o.mx = mx;
+    if (!__any(act && pair)) {
+        unit_block<double, QR, MAXB>(us, raw, o);
+    }
+    else {
+        unit_block<cpx, QR, MAXB>(us, raw, o);
     }
 }
 
@@ -1062,7 +1185,7 @@ __global__ void k_zero_int(int *__restrict__ X, long count)
 
 namespace xtg {
 template __global__ void k_solve_real<65>(const double *, long, const double *, long, int, int, unsigned long long,
-                                          double *, long, double *, long, const UnitDesc *, const int2 *, int, int,
+                                          double *, long, double *, long, const UnitDesc *, const int4 *, int, int,
                                           int, int, double *, int *, int *, int *, int, int, double *, int, GroupInfo,
                                           MixedExp);
 } // namespace xtg
