@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from tests.helpers import check_cplx_parity, run_cplx
+from tests.helpers import EPS, check_cplx_parity, run_cplx
 from xtgevc3_amd import pencils as G
 from oracle import oracle as O
 
@@ -65,3 +65,48 @@ def test_float_interface():
     O.normalize(oR, kind)
     assert (m, info) == (n, 0)
     assert np.max(np.abs(VR - oR)) < 1e-4 and np.max(np.abs(VL - oL)) < 1e-4
+
+
+@pytest.mark.parametrize("n", [300, 520])
+@pytest.mark.parametrize("gen", ["ref", "qz"])
+def test_complex_groups(gen, n):
+    """Several update groups of 8 row blocks of 32 (n = 520: the last group has one block),
+    near and far updates, group operand rescaling."""
+    if gen == "ref":
+        S, P, al, be = G.reference_style_complex(n, 90 + n)
+        Q, Z = G.random_unitary(n, 5), G.random_unitary(n, 6)
+    else:
+        S, P, Q, Z, al, be = G.qz_pencil(n, 91 + n, complex_=True)
+    check_cplx_parity(S, P, al, be, "B", "B", None, Q, Z, cond=gen == "ref")
+
+
+@pytest.mark.parametrize("side", ["R", "L"])
+def test_complex_select_mid_group_after_full(side):
+    """A call that leaves the group operands of the workspace filled, then a selection whose
+    only column starts in the middle of the first update group: the operand rows of the
+    group's lower blocks must be cleared, not reused."""
+    n = 300
+    S, P, al, be = G.reference_style_complex(n, 17)
+    run_cplx("gpu", side, "A", None, S, P, al, be)
+    sel = np.zeros(n, dtype=np.int32)
+    sel[[120, 121]] = 1
+    check_cplx_parity(S, P, al, be, side, "S", sel, cond=True)
+
+
+@pytest.mark.parametrize("side", ["R", "L"])
+def test_complex_large_residual(side):
+    """n = 2304 N(0,1) complex Schur pencil (entries of the solutions grow far past 2^958
+    before normalisation): backward-error residual and normalisation."""
+    n = 2304
+    S, P, al, be = G.complex_triangular(n, 43)
+    Z = G.random_unitary(n, 44)
+    gL, gR, m, info = run_cplx("gpu", side, "B", None, S, P, al, be, Z, Z)
+    assert (m, info) == (n, 0)
+    g = gR if side == "R" else gL
+    assert np.all(np.isfinite(g))
+    mx = np.max(np.abs(g.real) + np.abs(g.imag), axis=0)
+    assert np.allclose(mx, 1.0, rtol=0, atol=64 * EPS)
+    A = Z @ S @ Z.conj().T
+    B = Z @ P @ Z.conj().T
+    res = O.residual_A(A, B, g, al, be, left=side == "L")
+    assert res.max() <= 10.0 * n * EPS, f"residual {res.max() / (n * EPS):.3g} n*eps"

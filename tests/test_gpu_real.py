@@ -49,6 +49,23 @@ def test_real_select_edge():
     check_real_parity(S, P, ar, ai, be, "B", "S", sel, cond=True)
 
 
+@pytest.mark.parametrize("side", ["R", "L"])
+def test_real_select_mid_group_after_full(side):
+    """A call that leaves the group operands of the workspace filled, then a selection whose
+    columns start in the middle of the first update group (row blocks of 64, groups of 4):
+    the operand rows of the group's lower blocks must be cleared, not reused."""
+    n = 300
+    S, P, ar, ai, be = G.real_quasi_triangular(n, 19)
+    run_real("gpu", side, "A", None, S, P, ar, ai, be)
+    sel = np.zeros(n, dtype=np.int32)
+    # one real eigenvalue and one pair near row 100 (block [44, 108) of the first group)
+    kr = next(k for k in range(90, n) if ai[k] == 0.0)
+    kp = next(k for k in range(100, n - 1) if ai[k] > 0.0)
+    sel[kr] = 1
+    sel[kp] = 1
+    check_real_parity(S, P, ar, ai, be, side, "S", sel, cond=True)
+
+
 def test_real_info_codes_gpu():
     n = 20
     S, P, ar, ai, be = G.real_quasi_triangular(n, 3)

@@ -42,12 +42,13 @@ struct UnitDesc {
     int pad1, pad2, pad3;
 };
 
-// The row blocks of one update group of the real sweep, top-down: block j covers local rows
+// The row blocks of one update group of a sweep, top-down: block j covers local rows
 // [off[j], off[j+1]) of the group (off[nblk] = wr, unused entries = wr) and is solved in step
-// step[j]; step[0] is the group's last step.
+// step[j]; step[0] is the group's last step.  At most GROUP_MAX_BLOCKS blocks per group.
+constexpr int GROUP_MAX_BLOCKS = 8;
 struct GroupInfo {
-    int off[5];
-    int step[4];
+    int off[GROUP_MAX_BLOCKS + 1];
+    int step[GROUP_MAX_BLOCKS];
     int wr;
     int last; // the launch is the group's last step
 };

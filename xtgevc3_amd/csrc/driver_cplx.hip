@@ -21,11 +21,12 @@
 namespace xtg {
 
 template <int MAXB>
-__global__ void kz_solve(const double2 *, long, const double2 *, long, int, int, double2 *, long, double2 *, long,
-                         const ZColDesc *, int, int, double *, int *, int *, int *);
+__global__ void kz_solve_units(const double2 *, long, const double2 *, long, int, int, double2 *, long, double2 *, long,
+                               const ZColDesc *, int, int, double *, int *, int *, int *, int, int, double *, int,
+                               GroupInfo);
 template <int TAG>
 __global__ void kz_gemm(int, int, int, int, const double2 *, long, const double2 *, long, const double2 *, long,
-                        double2 *, long, const int *, int, const double *, const int *);
+                        double2 *, long, const int *, int, const double *, const int *, const double2 *);
 __global__ void kz_fixup_scale(double2 *, long, int, int, const int *, const int *, long, const int *, const int *);
 __global__ void kz_copy_out(const double2 *, long, int, int, double2 *, long, const int *, int);
 __global__ void kz_normalize(double2 *, long, int);
@@ -35,7 +36,10 @@ __global__ void k_zero(double *, long);
 __global__ void k_zero_int(int *, long);
 
 constexpr int NB_Z = 32;
+constexpr int ZGROUP = 8;        // row blocks per far update (K = 2 * 256)
+constexpr int ZCOLS_PER_BLOCK = 16; // solve kernel: 16 lanes per column, 256 threads
 constexpr int ZTA_H = 128, ZTB_H = 64;
+static_assert(ZGROUP <= GROUP_MAX_BLOCKS, "group size");
 
 int complex_units(char howmny, const int *select, int n, std::vector<Unit> &units)
 {
@@ -81,7 +85,7 @@ struct DBuf {
 };
 
 struct ZWork {
-    DBuf X, W, sig, colexp, blkexp, flag, desc, outcol, kmax, rowstep, Sw, Pw, Zt, norms;
+    DBuf X, W, sig, sig_grp, colexp, blkexp, flag, desc, outcol, kmax, rowstep, Sw, Pw, Zt, norms;
 };
 
 std::mutex g_zmutex;
@@ -146,8 +150,10 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
     }
     const long ldx = ((long)m + 7) / 8 * 8;
     ZENSURE(ws.X, sizeof(double2) * (size_t)n * ldx);
-    ZENSURE(ws.W, sizeof(double2) * (size_t)(2 * NB_Z) * ldx);
+    const long ldw = ldx;
+    ZENSURE(ws.W, sizeof(double2) * (size_t)(2 * ZGROUP * NB_Z) * ldw);
     ZENSURE(ws.sig, sizeof(double) * (size_t)m);
+    ZENSURE(ws.sig_grp, sizeof(double) * (size_t)m);
     ZENSURE(ws.colexp, sizeof(int) * (size_t)m);
     ZENSURE(ws.blkexp, sizeof(int) * (size_t)nsteps * ldx);
     ZENSURE(ws.flag, sizeof(int) * 4);
@@ -166,30 +172,75 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(256), 0, st, ws.colexp.as<int>(), (long)m); XTG_LAUNCH_CHECK("k_zero_int");
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, ws.flag.as<int>(), 4L); XTG_LAUNCH_CHECK("k_zero_int");
 
+    // Steps go bottom-up in groups of ZGROUP row blocks [g0, g1).  Within a group each step
+    // updates only the group's rows above it ("near" update, K = 2 nb); after the group one
+    // "far" update applies all of its blocks to the rows above g0 (K = 2 (g1 - g0)).  The
+    // group's operand W holds rows [0, g1-g0) (S part: -a x) and [g1-g0, 2(g1-g0)) (P part: b x).
     int c_begin = m;
-    for (int s = 0; s < nsteps; ++s) {
-        const int i0 = i0s[s], nb = nbs[s];
-        while (c_begin > 0 && desc[c_begin - 1].d >= i0) {
-            --c_begin;
+    for (int gs = 0; gs < nsteps; gs += ZGROUP) {
+        const int ge = std::min(nsteps, gs + ZGROUP);
+        const int g1 = i0s[gs] + nbs[gs];
+        const int g0 = i0s[ge - 1];
+        const int wr = g1 - g0;
+        GroupInfo grp{};
+        grp.wr = wr;
+        {
+            int off = 0;
+            for (int jb = 0; jb < GROUP_MAX_BLOCKS; ++jb) {
+                const int sj = ge - 1 - jb;
+                grp.off[jb] = off;
+                grp.step[jb] = sj >= gs ? sj : ge - 1;
+                if (sj >= gs) {
+                    off += nbs[sj];
+                }
+            }
+            grp.off[GROUP_MAX_BLOCKS] = off;
         }
-        if (c_begin >= m) {
+        bool first = true;
+        for (int s = gs; s < ge; ++s) {
+            const int i0 = i0s[s], nb = nbs[s];
+            while (c_begin > 0 && desc[c_begin - 1].d >= i0) {
+                --c_begin;
+            }
+            if (c_begin >= m) {
+                continue;
+            }
+            const int N = m - c_begin;
+            const int woff = i0 - g0;
+            grp.last = s == ge - 1 ? 1 : 0;
+            int pr = prof_begin(st);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(kz_solve_units<NB_Z>),
+                               dim3((N + ZCOLS_PER_BLOCK - 1) / ZCOLS_PER_BLOCK), dim3(256), 0, st, Sw, ldsw, Pw,
+                               ldpw, i0, nb, X, ldx, W, ldw, ws.desc.as<ZColDesc>(), c_begin, m, ws.sig.as<double>(),
+                               ws.colexp.as<int>(), ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff,
+                               wr + woff, ws.sig_grp.as<double>(), s == gs ? 1 : 0, grp); XTG_LAUNCH_CHECK("kz_solve_units");
+            prof_end(pr, st, 0, 8.0 * nb * nb * (double)N, 16.0 * (4.0 * nb * N + 2.0 * nb * nb));
+            first = false;
+            // near update: the group's rows above this block
+            const int M = i0 - g0;
+            if (M > 0) {
+                dim3 grid((N + ZTB_H - 1) / ZTB_H, (M + ZTA_H - 1) / ZTA_H);
+                pr = prof_begin(st);
+                hipLaunchKernelGGL(kz_gemm<0>, grid, dim3(256), 0, st, M, N, nb, nb, Sw + g0 + (long)i0 * ldsw, ldsw,
+                                   Pw + g0 + (long)i0 * ldpw, ldpw, (const double2 *)(W + (long)woff * ldw + c_begin),
+                                   ldw, X + (long)g0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
+                                   (const double *)(ws.sig.as<double>() + c_begin), (const int *)nullptr,
+                                   (const double2 *)(W + (long)(wr + woff) * ldw + c_begin)); XTG_LAUNCH_CHECK("kz_gemm<0>");
+                prof_end(pr, st, 1, 8.0 * M * N * (2.0 * nb), 16.0 * (2.0 * M * N + 2.0 * nb * M + 2.0 * nb * N));
+            }
+        }
+        if (first || g0 == 0) {
             continue;
         }
+        // far update: rows above the group, all of its blocks at once
         const int N = m - c_begin;
+        dim3 grid((N + ZTB_H - 1) / ZTB_H, (g0 + ZTA_H - 1) / ZTA_H);
         int pr = prof_begin(st);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(kz_solve<NB_Z>), dim3((N + 255) / 256), dim3(256), 0, st, Sw, ldsw, Pw,
-                           ldpw, i0, nb, X, ldx, W, ldx, ws.desc.as<ZColDesc>(), c_begin, m, ws.sig.as<double>(),
-                           ws.colexp.as<int>(), ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>()); XTG_LAUNCH_CHECK("kz_solve<NB_Z>");
-        prof_end(pr, st, 0, 8.0 * nb * nb * (double)N, 16.0 * (4.0 * nb * N + 2.0 * nb * nb));
-        if (i0 > 0) {
-            dim3 grid((N + ZTB_H - 1) / ZTB_H, (i0 + ZTA_H - 1) / ZTA_H);
-            pr = prof_begin(st);
-            hipLaunchKernelGGL(kz_gemm<0>, grid, dim3(256), 0, st, i0, N, nb, nb, Sw + (long)i0 * ldsw, ldsw,
-                               Pw + (long)i0 * ldpw, ldpw, (const double2 *)(W + c_begin), ldx, X + c_begin, ldx,
-                               (const int *)nullptr, 1, (const double *)(ws.sig.as<double>() + c_begin),
-                               (const int *)nullptr); XTG_LAUNCH_CHECK("kz_gemm<0>");
-            prof_end(pr, st, 1, 8.0 * i0 * N * (2.0 * nb), 16.0 * (2.0 * i0 * N + 2.0 * nb * i0 + 2.0 * nb * N));
-        }
+        hipLaunchKernelGGL(kz_gemm<0>, grid, dim3(256), 0, st, g0, N, wr, wr, Sw + (long)g0 * ldsw, ldsw,
+                           Pw + (long)g0 * ldpw, ldpw, (const double2 *)(W + c_begin), ldw, X + c_begin, ldx,
+                           (const int *)nullptr, 1, (const double *)(ws.sig_grp.as<double>() + c_begin),
+                           (const int *)nullptr, (const double2 *)(W + (long)wr * ldw + c_begin)); XTG_LAUNCH_CHECK("kz_gemm<0>");
+        prof_end(pr, st, 1, 8.0 * g0 * N * (2.0 * wr), 16.0 * (2.0 * g0 * N + 2.0 * wr * g0 + 2.0 * wr * N));
     }
     XTG_CHECK(hipGetLastError());
     {
@@ -210,7 +261,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
         int pr = prof_begin(st);
         hipLaunchKernelGGL(kz_gemm<1>, grid, dim3(256), 0, st, m, n, n, 0, (const double2 *)X, ldx,
                            (const double2 *)nullptr, 0L, Rp, ldr, V, ldv, ws.outcol.as<int>(), 0,
-                           (const double *)nullptr, ws.kmax.as<int>()); XTG_LAUNCH_CHECK("kz_gemm<1>");
+                           (const double *)nullptr, ws.kmax.as<int>(), (const double2 *)nullptr); XTG_LAUNCH_CHECK("kz_gemm<1>");
         prof_end(pr, st, 2, 8.0 * n * ksum, 16.0 * ((double)n * kmx + ksum + (double)n * m));
     }
     else {

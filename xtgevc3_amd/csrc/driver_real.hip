@@ -406,7 +406,6 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         const int g0 = i0s[ge - 1];
         const int wr = g1 - g0;
         bool any_active = false;
-        bool first = true;
         // the group's blocks top-down
         GroupInfo grp{};
         grp.wr = wr;
@@ -420,7 +419,9 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
                     off += nbs[sj];
                 }
             }
-            grp.off[4] = off;
+            for (int jb = 4; jb <= GROUP_MAX_BLOCKS; ++jb) {
+                grp.off[jb] = off;
+            }
         }
         for (int s = gs; s < ge; ++s) {
             const int i0 = i0s[s], nb = nbs[s];
@@ -470,9 +471,8 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
                                ldsw, Pw, ldpw, i0, nb, bumps, X, ldx, W, ldw, ws.desc.as<UnitDesc>(), ws.tiles.as<int4>(),
                                t_begin, t_end, u_begin, u_end, ws.sig.as<double>(), ws.colexp.as<int>(),
                                ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff, wr + woff,
-                               ws.sig_grp.as<double>(), first ? 1 : 0, grp, mx); XTG_LAUNCH_CHECK("k_solve_real");
+                               ws.sig_grp.as<double>(), s == gs ? 1 : 0, grp, mx); XTG_LAUNCH_CHECK("k_solve_real");
             prof_end(pr, st, 0, 2.0 * nb * nb * N, 8.0 * (4.0 * nb * N + 2.0 * nb * nb));
-            first = false;
             // near update: the group's rows above this block
             const int r0 = adjoint ? std::max(g0, std::min(desc[c_begin].lo, i0)) : g0;
             const int M = i0 - r0;

@@ -1,9 +1,10 @@
 // gfx950 kernels of the complex128 xTGEVC3 path (ztgevc3 / ctgevc3 interface).
 //
-//   kz_solve     -- one block-row step: per-column shifted triangular solve of a diagonal
-//                   block (complex, no 2x2 bumps), one lane per eigenvector column, S/P
-//                   block in LDS, per-column power-of-two overflow scaling
-//                   (replaces zlalsr/zlalsl + ztrsm_, reference src/ztgevc3.cpp:116-358)
+//   kz_solve_units -- one block-row step: per-column shifted triangular solve of a diagonal
+//                   block (complex, no 2x2 bumps), 16 lanes per eigenvector column, S/P
+//                   block in LDS, per-column power-of-two overflow scaling, group operands
+//                   of the near/far updates (replaces zlalsr/zlalsl + ztrsm_, reference
+//                   src/ztgevc3.cpp:116-358)
 //   kz_gemm      -- complex FP64 MFMA GEMM (four v_mfma_f64_16x16x4 per complex product) for
 //                   the RHS update (ztgevc3.cpp:664-665, :808-809) and the back-transform
 //                   (ztgevc3.cpp:676, :820)
@@ -22,39 +23,84 @@ struct zc {
 
 __device__ __forceinline__ zc zmul(zc a, zc b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
 __device__ __forceinline__ double zabs1(zc a) { return fabs(a.r) + fabs(a.i); }
-__device__ __forceinline__ zc zdiv(zc x, zc c)
-{
-    if (fabs(c.r) >= fabs(c.i)) {
-        double t = c.i / c.r;
-        double den = c.r + c.i * t;
-        return {(x.r + x.i * t) / den, (x.i - x.r * t) / den};
-    }
-    double t = c.r / c.i;
-    double den = c.i + c.r * t;
-    return {(x.r * t + x.i) / den, (x.i * t - x.r) / den};
-}
 
 constexpr double kzEps = 2.220446049250313e-16;
 constexpr double kzSafeMin = 2.2250738585072014e-308;
-constexpr int kzLog2Big = 960;
 
-// X, W: interleaved complex (re, im) doubles, row-major: X[(row * ldx + col) * 2 + {0,1}].
-template <int MAXB>
-__global__ __launch_bounds__(256) void kz_solve(const double2 *__restrict__ S, long lds, const double2 *__restrict__ P,
-                                                long ldp, int i0, int nb, double2 *__restrict__ X, long ldx,
-                                                double2 *__restrict__ W, long ldw, const ZColDesc *__restrict__ desc,
-                                                int c_begin, int m, double *__restrict__ sig, int *__restrict__ colexp,
-                                                int *__restrict__ blkexp, int *__restrict__ flag)
+// ---------------------------------------------------------------------------------------
+// kz_solve_units: one block-row step of the complex sweep, 16 lanes per eigenvector column.
+// Lane j of a column keeps rows 16 i + j of the block.  Before the sweep every lane
+// computes the reciprocal of the shifted diagonal of its rows (a S_rr - b P_rr, perturbed to
+// the rounding level when tiny; rows outside the column's system get 0, its own row 1).  The
+// sweep runs bottom-up: the owner lane forms x_r = rhs_r / c_r, broadcasts it in the 16-lane
+// row (DPP row_newbcast), and every lane updates its rows above with the {S,P} entries of
+// column r, fetched from LDS one row ahead.  A solution entry above 2^958 rescales the column
+// by a power of two down to 2^446 (exponent bookkeeping as in the real sweep).
+// Operands of the group (rows w_s + r: -a x, rows w_p + r: b x); a column starting in this
+// block (not the group's first step) zeros its operand rows of the group's lower blocks; the
+// group's last step brings the lower blocks' operand rows to the column's final scale.
+// X, W: complex (double2) row-major, X[row * ldx + col].
+// ---------------------------------------------------------------------------------------
+constexpr int ZQL = 16;
+constexpr int ZSP = 33; // LDS column pitch of the staged block (rows)
+
+template <int SRC> __device__ __forceinline__ double zrow_bcast_c(double v)
 {
-    __shared__ double2 ss[MAXB * MAXB];
-    __shared__ double2 pp[MAXB * MAXB];
-    for (int q = threadIdx.x; q < nb * nb; q += blockDim.x) {
-        int l = q % nb, r = q / nb;
-        ss[l + r * MAXB] = S[(i0 + l) + (long)(i0 + r) * lds];
-        pp[l + r * MAXB] = P[(i0 + l) + (long)(i0 + r) * ldp];
+    int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x150 + SRC, 0xF, 0xF, false);
+    int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x150 + SRC, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+template <int SRC> __device__ __forceinline__ int zrow_bcast_i(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + SRC, 0xF, 0xF, false);
+}
+#define XTG_ZRB_CASES(F)                                                                                         \
+    switch (src) {                                                                                               \
+    case 0: return F<0>(v);                                                                                      \
+    case 1: return F<1>(v);                                                                                      \
+    case 2: return F<2>(v);                                                                                      \
+    case 3: return F<3>(v);                                                                                      \
+    case 4: return F<4>(v);                                                                                      \
+    case 5: return F<5>(v);                                                                                      \
+    case 6: return F<6>(v);                                                                                      \
+    case 7: return F<7>(v);                                                                                      \
+    case 8: return F<8>(v);                                                                                      \
+    case 9: return F<9>(v);                                                                                      \
+    case 10: return F<10>(v);                                                                                    \
+    case 11: return F<11>(v);                                                                                    \
+    case 12: return F<12>(v);                                                                                    \
+    case 13: return F<13>(v);                                                                                    \
+    case 14: return F<14>(v);                                                                                    \
+    default: return F<15>(v);                                                                                    \
     }
-    __syncthreads();
-    const int p = c_begin + blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ double zbd(double v, int src) { XTG_ZRB_CASES(zrow_bcast_c) }
+__device__ __forceinline__ int zbi(int v, int src) { XTG_ZRB_CASES(zrow_bcast_i) }
+#undef XTG_ZRB_CASES
+__device__ __forceinline__ zc zbc(zc v, int src) { return {zbd(v.r, src), zbd(v.i, src)}; }
+__device__ __forceinline__ zc zsel(bool c, zc x, zc y) { return {c ? x.r : y.r, c ? x.i : y.i}; }
+__device__ __forceinline__ zc zrecip(zc c)
+{
+    const bool sw = fabs(c.i) > fabs(c.r);
+    const double big = sw ? c.i : c.r, sml = sw ? c.r : c.i;
+    const double t = sml / big;
+    const double rd = 1.0 / (big + sml * t);
+    const double tr = t * rd;
+    return sw ? zc{tr, -rd} : zc{rd, -tr};
+}
+
+template <int MAXB>
+__global__ __launch_bounds__(256) void kz_solve_units(
+    const double2 *__restrict__ S, long lds, const double2 *__restrict__ P, long ldp, int i0, int nb,
+    double2 *__restrict__ X, long ldx, double2 *__restrict__ W, long ldw, const ZColDesc *__restrict__ desc,
+    int c_begin, int m, double *__restrict__ sig, int *__restrict__ colexp, int *__restrict__ blkexp,
+    int *__restrict__ flag, int w_s, int w_p, double *__restrict__ sig_grp, int first_in_group, GroupInfo grp)
+{
+    static_assert(MAXB <= 32 && ZSP > MAXB, "block layout");
+    constexpr int QR = (MAXB + ZQL - 1) / ZQL;
+    __shared__ double2 ss[ZSP * MAXB], pp[ZSP * MAXB]; // ss[l + r * ZSP] = S_lr
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int j = lane & (ZQL - 1);
+    const int p = c_begin + (int)(blockIdx.x * (blockDim.x / WAVE) + wave) * (WAVE / ZQL) + lane / ZQL;
     const bool act = p < m;
     const ZColDesc cd = desc[act ? p : c_begin];
     const zc a = {cd.aR, cd.aI}, b = {cd.bR, cd.bI};
@@ -63,90 +109,177 @@ __global__ __launch_bounds__(256) void kz_solve(const double2 *__restrict__ S, l
     const int lol = cd.lo - i0;
     const int init = (cd.flags >> 2) & 3;
 
-    double vr[MAXB], vi[MAXB];
+    // right-hand side of the column's rows (issued before the staging)
+    zc vr[QR];
 #pragma clang loop unroll(full)
-    for (int r = 0; r < MAXB; ++r) {
-        vr[r] = 0.0;
-        vi[r] = 0.0;
+    for (int i = 0; i < QR; ++i) {
+        const int r = ZQL * i + j;
+        vr[i] = {0.0, 0.0};
         if (r < nb && r < dl && r >= lol) {
-            double2 t = X[(long)(i0 + r) * ldx + p];
-            vr[r] = t.x;
-            vi[r] = t.y;
+            const double2 t = X[(long)(i0 + r) * ldx + p];
+            vr[i] = {t.x, t.y};
         }
     }
-    int my_exp = 0;
+    for (int q = threadIdx.x; q < nb * nb; q += blockDim.x) {
+        const int l = q % nb, r = q / nb;
+        ss[l + r * ZSP] = S[(i0 + l) + (long)(i0 + r) * lds];
+        pp[l + r * ZSP] = P[(i0 + l) + (long)(i0 + r) * ldp];
+    }
+    __syncthreads();
+
+    // reciprocals of the shifted diagonal of this lane's rows
+    zc fa[QR];
 #pragma clang loop unroll(full)
-    for (int r = MAXB - 1; r >= 0; --r) {
+    for (int i = 0; i < QR; ++i) {
+        const int r = ZQL * i + j;
+        fa[i] = {0.0, 0.0};
         if (r < nb) {
-            double2 es = ss[r + r * MAXB], ep = pp[r + r * MAXB];
+            const double2 es = ss[r + r * ZSP], ep = pp[r + r * ZSP];
             zc c = {a.r * es.x - a.i * es.y - (b.r * ep.x - b.i * ep.y), a.r * es.y + a.i * es.x - (b.r * ep.y + b.i * ep.x)};
-            double dmin = fmax(kzEps * (an1 * (fabs(es.x) + fabs(es.y)) + bn1 * (fabs(ep.x) + fabs(ep.y))), kzSafeMin);
+            const double dmin =
+                fmax(kzEps * (an1 * (fabs(es.x) + fabs(es.y)) + bn1 * (fabs(ep.x) + fabs(ep.y))), kzSafeMin);
             if (zabs1(c) < dmin) {
                 c = {dmin, 0.0};
             }
-            zc rhs = {vr[r], vi[r]};
             const bool solve = r < dl && r >= lol;
-            double rn = zabs1(rhs);
-            int ex = 0;
-            if (solve && rn > 0.0 && ilogb(rn) - ilogb(zabs1(c)) > kzLog2Big - 2) {
-                ex = ilogb(rn) - ilogb(zabs1(c)) - kzLog2Big + 2;
-            }
-            if (__any(ex != 0)) {
-                double f = ldexp(1.0, -ex);
-#pragma clang loop unroll(full)
-                for (int j = 0; j < MAXB; ++j) {
-                    vr[j] *= f;
-                    vi[j] *= f;
-                }
-                rhs = {rhs.r * f, rhs.i * f};
-                my_exp -= ex;
-            }
-            zc x = solve ? zdiv(rhs, c) : ((r == dl && init == INIT_UNIT) ? zc{1.0, 0.0} : zc{0.0, 0.0});
-            vr[r] = x.r;
-            vi[r] = x.i;
-            const zc ax = zmul(a, x), bx = zmul(b, x);
-#pragma clang loop unroll(full)
-            for (int l = 0; l < r; ++l) {
-                double2 s2 = ss[l + r * MAXB], p2 = pp[l + r * MAXB];
-                // v_l -= S_lr * ax - P_lr * bx
-                vr[l] = fma(-s2.x, ax.r, vr[l]);
-                vr[l] = fma(s2.y, ax.i, vr[l]);
-                vr[l] = fma(p2.x, bx.r, vr[l]);
-                vr[l] = fma(-p2.y, bx.i, vr[l]);
-                vi[l] = fma(-s2.x, ax.i, vi[l]);
-                vi[l] = fma(-s2.y, ax.r, vi[l]);
-                vi[l] = fma(p2.x, bx.i, vi[l]);
-                vi[l] = fma(p2.y, bx.r, vi[l]);
+            const bool own = r == dl && init == INIT_UNIT;
+            fa[i] = zsel(solve, zrecip(c), zc{own ? 1.0 : 0.0, 0.0});
+            if (own) {
+                vr[i] = {1.0, 0.0};
             }
         }
     }
+
+    int my_exp = 0;
+    double2 es_c[QR], ep_c[QR], es_n[QR], ep_n[QR]; // column r (c) and r-1 (n) of S, P at this lane's rows
+#pragma clang loop unroll(full)
+    for (int i = 0; i < QR; ++i) {
+        es_c[i] = es_n[i] = ep_c[i] = ep_n[i] = make_double2(0.0, 0.0);
+    }
+#pragma clang loop unroll(full)
+    for (int r = MAXB - 1; r >= 0; --r) {
+        if (r < nb) {
+            if (r == nb - 1) {
+#pragma clang loop unroll(full)
+                for (int i = 0; i < QR; ++i) {
+                    if (ZQL * i < r) {
+                        es_c[i] = ss[ZQL * i + j + r * ZSP];
+                        ep_c[i] = pp[ZQL * i + j + r * ZSP];
+                    }
+                }
+            }
+            if (r >= 1) {
+#pragma clang loop unroll(full)
+                for (int i = 0; i < QR; ++i) {
+                    if (ZQL * i < r - 1) {
+                        es_n[i] = ss[ZQL * i + j + (r - 1) * ZSP];
+                        ep_n[i] = pp[ZQL * i + j + (r - 1) * ZSP];
+                    }
+                }
+            }
+            const int jr = r & (ZQL - 1), ir = r / ZQL;
+            zc x = zbc(zmul(vr[ir], fa[ir]), jr);
+            const bool bad = !(zabs1(x) <= 0x1p958);
+            if (__any(bad)) {
+                int eo = 0;
+                const double rn = zabs1(vr[ir]), fn = zabs1(fa[ir]);
+                if (bad && rn > 0.0 && fn > 0.0) {
+                    eo = max(0, ilogb(rn) + ilogb(fn) + 2 - 446);
+                }
+                const int e = zbi(eo, jr);
+                const double f = ldexp(1.0, -e);
+#pragma clang loop unroll(full)
+                for (int i = 0; i < QR; ++i) {
+                    vr[i] = {vr[i].r * f, vr[i].i * f};
+                }
+                my_exp -= e;
+                x = zbc(zmul(vr[ir], fa[ir]), jr);
+            }
+            vr[ir] = zsel(j == jr, x, vr[ir]);
+            const zc ax = zmul(a, x), bx = zmul(b, x);
+#pragma clang loop unroll(full)
+            for (int i = 0; i < QR; ++i) {
+                if (ZQL * i < r) {
+                    const double2 s2 = es_c[i], p2 = ep_c[i];
+                    // v_l -= S_lr * ax - P_lr * bx
+                    double nr = fma(-s2.x, ax.r, vr[i].r);
+                    nr = fma(s2.y, ax.i, nr);
+                    nr = fma(p2.x, bx.r, nr);
+                    nr = fma(-p2.y, bx.i, nr);
+                    double ni = fma(-s2.x, ax.i, vr[i].i);
+                    ni = fma(-s2.y, ax.r, ni);
+                    ni = fma(p2.x, bx.i, ni);
+                    ni = fma(p2.y, bx.r, ni);
+                    const bool upd = ZQL * i + ZQL - 1 < r || ZQL * i + j < r;
+                    vr[i] = zsel(upd, zc{nr, ni}, vr[i]);
+                }
+            }
+#pragma clang loop unroll(full)
+            for (int i = 0; i < QR; ++i) {
+                es_c[i] = es_n[i];
+                ep_c[i] = ep_n[i];
+            }
+        }
+    }
+
     if (!act) {
         return;
     }
 #pragma clang loop unroll(full)
-    for (int r = 0; r < MAXB; ++r) {
+    for (int i = 0; i < QR; ++i) {
+        const int r = ZQL * i + j;
         if (r < nb) {
-            zc x = {vr[r], vi[r]};
+            const zc x = vr[i];
             X[(long)(i0 + r) * ldx + p] = make_double2(x.r, x.i);
-            zc ax = zmul(a, x), bx = zmul(b, x);
-            W[(long)r * ldw + p] = make_double2(-ax.r, -ax.i);
-            W[(long)(nb + r) * ldw + p] = make_double2(bx.r, bx.i);
+            const zc ax = zmul(a, x), bx = zmul(b, x);
+            W[(long)(w_s + r) * ldw + p] = make_double2(-ax.r, -ax.i);
+            W[(long)(w_p + r) * ldw + p] = make_double2(bx.r, bx.i);
         }
     }
-    sig[p] = ldexp(1.0, my_exp);
-    int ce = colexp[p] + my_exp;
-    colexp[p] = ce;
-    blkexp[p] = ce;
-    if (my_exp != 0) {
-        *flag = 1;
+    const bool starts_here = dl >= 0 && dl < nb;
+    const int ce = colexp[p] + my_exp;
+    if (starts_here && !first_in_group) {
+        for (int k = w_s + nb + j; k < grp.wr; k += ZQL) {
+            W[(long)k * ldw + p] = make_double2(0.0, 0.0);
+            W[(long)(grp.wr + k) * ldw + p] = make_double2(0.0, 0.0);
+        }
+    }
+    if (grp.last && !starts_here) {
+        for (int jb = 1; jb < 8; ++jb) {
+            const int k0 = grp.off[jb], k1 = grp.off[jb + 1];
+            if (k0 < k1) {
+                const int e = ce - blkexp[(long)(grp.step[jb] - grp.step[0]) * ldx + p];
+                if (e != 0) {
+                    for (int k = k0 + j; k < k1; k += ZQL) {
+                        double2 *w0 = W + (long)k * ldw + p;
+                        double2 *w1 = W + (long)(grp.wr + k) * ldw + p;
+                        const double2 v0 = *w0, v1 = *w1;
+                        *w0 = make_double2(ldexp(v0.x, e), ldexp(v0.y, e));
+                        *w1 = make_double2(ldexp(v1.x, e), ldexp(v1.y, e));
+                    }
+                }
+            }
+        }
+    }
+    if (j == 0) {
+        const double sg = ldexp(1.0, my_exp);
+        sig[p] = sg;
+        sig_grp[p] = (first_in_group || starts_here) ? sg : sig_grp[p] * sg;
+        colexp[p] = ce;
+        blkexp[p] = ce;
+        if (my_exp != 0) {
+            *flag = 1;
+        }
     }
 }
 
-template __global__ void kz_solve<32>(const double2 *, long, const double2 *, long, int, int, double2 *, long,
-                                      double2 *, long, const ZColDesc *, int, int, double *, int *, int *, int *);
+template __global__ void kz_solve_units<32>(const double2 *, long, const double2 *, long, int, int, double2 *, long,
+                                            double2 *, long, const ZColDesc *, int, int, double *, int *, int *,
+                                            int *, int, int, double *, int, GroupInfo);
 
 // ---------------------------------------------------------------------------------------
-// kz_gemm: D[a][b] = sum_k L[k][a] * R[k][b], complex, FP64 MFMA.
+// kz_gemm: D[a][b] = sum_k L[k][a] * R[k][b], complex, FP64 MFMA; rows k >= K1 of L and R
+// come from the second segments L2 and R2 (row k - K1).
 // Tile 128 (a) x 64 (b) per 256-thread workgroup (waves 2x2, each 64 x 32), K chunks of 8.
 // Same operand / output conventions as k_gemm_f64 (element = complex double2).
 // ---------------------------------------------------------------------------------------
@@ -158,7 +291,7 @@ __global__ __launch_bounds__(256) void kz_gemm(int M, int N, int K1, int K2, con
                                                const double2 *__restrict__ L2, long ldl2, const double2 *__restrict__ R,
                                                long ldr, double2 *__restrict__ C, long ldc,
                                                const int *__restrict__ outcol, int mode, const double *__restrict__ sig,
-                                               const int *__restrict__ kmax)
+                                               const int *__restrict__ kmax, const double2 *__restrict__ R2)
 {
     __shared__ double Lr[2][ZGK][ZTA], Li[2][ZGK][ZTA];
     __shared__ double Rr[2][ZGK][ZTB], Ri[2][ZGK][ZTB];
@@ -204,7 +337,7 @@ __global__ __launch_bounds__(256) void kz_gemm(int M, int N, int K1, int K2, con
             int bb = b0 + rc;
             double2 v = make_double2(0.0, 0.0);
             if (k < K && bb < N) {
-                v = R[(long)k * ldr + bb];
+                v = (k < K1) ? R[(long)k * ldr + bb] : R2[(long)(k - K1) * ldr + bb];
             }
             rreg[s] = v;
         }
@@ -292,9 +425,11 @@ __global__ __launch_bounds__(256) void kz_gemm(int M, int N, int K1, int K2, con
 }
 
 template __global__ void kz_gemm<0>(int, int, int, int, const double2 *, long, const double2 *, long, const double2 *,
-                                    long, double2 *, long, const int *, int, const double *, const int *);
+                                    long, double2 *, long, const int *, int, const double *, const int *,
+                                    const double2 *);
 template __global__ void kz_gemm<1>(int, int, int, int, const double2 *, long, const double2 *, long, const double2 *,
-                                    long, double2 *, long, const int *, int, const double *, const int *);
+                                    long, double2 *, long, const int *, int, const double *, const int *,
+                                    const double2 *);
 
 __global__ void kz_fixup_scale(double2 *__restrict__ X, long ldx, int n, int m, const int *__restrict__ rowstep,
                                const int *__restrict__ blkexp, long ldb, const int *__restrict__ colexp,
