@@ -302,41 +302,60 @@ def main():
         res, ncols = residual_sample(S, P, ar, ai, be, dZ, dVR)
         log(f"[bench] residual (metric A) over {ncols} sampled columns: {res:.3e} n*eps")
 
-    # mixed form (FP32 MFMA products for the updates and back-transform) on the same inputs
-    mixed = None
-    if not args.no_mixed:
-        def step_mixed():
+    # mixed forms on the same inputs: split-int8 digit products (FP64-level residual) and FP32
+    # products; each timed over the same K steps, then one profiled step and one residual step
+    def run_form(precision, label):
+        def step_form():
             dVR.copy_(dZ)
             m, info = X.dtgevc3_device("R", "B", None, dS, dP, ar, ai, be, VR=dVR, stream=stream.cuda_stream,
-                                       precision=X.PREC_MIXED_F32)
+                                       precision=precision)
             assert info == 0 and m == n, (m, info)
             return m
 
-        step_mixed()
+        step_form()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         mm_ = 0
         for _ in range(args.steps):
-            mm_ += step_mixed()
+            mm_ += step_form()
         torch.cuda.synchronize()
         tm = time.perf_counter() - t0
-        mixed = {"form": "FP32 MFMA products (updates, back-transform), FP64 solves/scales/normalisation",
-                 "value": round(mm_ / tm, 2), "ms_per_step": round(1000.0 * tm / args.steps, 3),
-                 "speedup_vs_plain": round(total_s / tm, 3)}
+        out = {"form": label, "value": round(mm_ / tm, 2), "ms_per_step": round(1000.0 * tm / args.steps, 3),
+               "speedup_vs_plain": round(total_s / tm, 3)}
+        X.profile_enable(True)
+        step_form()
+        torch.cuda.synchronize()
+        X.profile_enable(False)
+        pf = X.profile_read()
+        out["kernels"] = {k: {"ms_per_step": round(v["ms"], 4), "launches_per_step": v["launches"],
+                              "tflops": round(v["flops"] / (v["ms"] * 1e9), 3) if v["ms"] else None}
+                          for k, v in pf.items() if v["launches"]}
+        for k, v in out["kernels"].items():
+            log(f"[bench]   {label[:10]:10s} {k:14s} launches={v['launches_per_step']}  {v['ms_per_step']:.3f} ms  "
+                f"{v['tflops']} TFLOP/s")
         if not args.no_residual:
             step()
             torch.cuda.synchronize()
             ref64 = dVR.clone()
-            step_mixed()
+            step_form()
             torch.cuda.synchronize()
             rm, _ = residual_sample(S, P, ar, ai, be, dZ, dVR)
             diff = float((dVR - ref64).abs().max())
-            mixed["max_residual_n_eps"] = float(f"{rm:.4g}")
-            mixed["max_abs_diff_vs_fp64"] = float(f"{diff:.4g}")
+            out["max_residual_n_eps"] = float(f"{rm:.4g}")
+            out["max_abs_diff_vs_fp64"] = float(f"{diff:.4g}")
             del ref64
-        log(f"[bench] mixed form: {mixed['ms_per_step']:.2f} ms/step, {mixed['value']:.1f} eigvecs/s, "
-            f"x{mixed['speedup_vs_plain']} vs plain; residual {mixed.get('max_residual_n_eps')} n*eps, "
-            f"max |V_mixed - V_fp64| {mixed.get('max_abs_diff_vs_fp64')}")
+        log(f"[bench] {label}: {out['ms_per_step']:.2f} ms/step, {out['value']:.1f} eigvecs/s, "
+            f"x{out['speedup_vs_plain']} vs plain; residual {out.get('max_residual_n_eps')} n*eps, "
+            f"max |V - V_fp64| {out.get('max_abs_diff_vs_fp64')}")
+        return out
+
+    mixed = mixed_f32 = None
+    if not args.no_mixed:
+        mixed = run_form(X.PREC_SPLIT_I8, "split-int8 digit products (5 base-255 digits, exact int32 "
+                                          "accumulation; far updates, back-transform), FP64 solves/near "
+                                          "updates/scales/normalisation")
+        mixed_f32 = run_form(X.PREC_MIXED_F32, "FP32 MFMA products (updates, back-transform), FP64 "
+                                               "solves/scales/normalisation")
 
     bw = None
     if not args.no_backward:
@@ -371,6 +390,7 @@ def main():
                    "max_residual_n_eps": None if res is None else float(f"{res:.4g}")},
         "roofline": roof,
         "mixed": mixed,
+        "mixed_f32": mixed_f32,
         "backward": bw,
         "kernels": kernels,
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},

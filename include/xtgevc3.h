@@ -36,7 +36,8 @@ typedef float _Complex xtgevc3_ccomplex;
 /* precision selector of the *_ex entry points */
 #define XTGEVC3_PREC_FP64 0      /* all arithmetic in FP64 (complex128) */
 #define XTGEVC3_PREC_MIXED_F32 1 /* updates/back-transform on FP32 MFMA, FP64 solves */
-#define XTGEVC3_PREC_MIXED_BF16X3 2 /* updates/back-transform on split-BF16 MFMA */
+#define XTGEVC3_PREC_MIXED_BF16X3 2 /* updates/back-transform on split-BF16 MFMA (not built) */
+#define XTGEVC3_PREC_SPLIT_I8 3     /* updates/back-transform on exact int8 MFMA digit products */
 
 int xtgevc3_abi_version(void);
 const char *xtgevc3_last_error(void);
@@ -71,8 +72,11 @@ int xtgevc3_dtgevc3_device(char side, char howmny, const int *select, int n, con
  * form of the update and back-transform products).  XTGEVC3_PREC_FP64 is
  * xtgevc3_dtgevc3_device; XTGEVC3_PREC_MIXED_F32 runs the off-diagonal updates and the
  * back-transform on FP32 MFMA products accumulated in FP32 and added to the FP64 running
- * right-hand sides / output; the block solves, scale factors and normalisation stay FP64.
- * Other selectors return XTGEVC3_INFO_UNSUPPORTED. */
+ * right-hand sides / output; XTGEVC3_PREC_SPLIT_I8 runs them on int8 MFMA products of
+ * five-digit splits of the operands (7-bit digits, exponent per row of the pencil / of Z and
+ * per column of the operands), accumulated exactly in int32 and combined in FP64 (error about
+ * 2^-34 of the row-maximum times column-maximum products).  The block solves, scale factors
+ * and normalisation stay FP64 in both.  Other selectors return XTGEVC3_INFO_UNSUPPORTED. */
 int xtgevc3_dtgevc3_device_ex(char side, char howmny, const int *select, int n, const double *S, int lds,
                               const double *P, int ldp, const double *alphar, const double *alphai,
                               const double *beta, double *VL, int ldvl, double *VR, int ldvr, int mm, int *m,
@@ -107,7 +111,10 @@ int xtgevc3_dtgevc3_backward_device(int n, const double *S, int lds, const doubl
 #define XTGEVC3_PROF_UPDATE_128x128 4    /* off-diagonal RHS update GEMM, 128x128 tiles */
 #define XTGEVC3_PROF_UPDATE_128x64 5     /* off-diagonal RHS update GEMM, 128x64 tiles */
 #define XTGEVC3_PROF_UPDATE_64x128 6     /* off-diagonal RHS update GEMM, 64x128 tiles */
-#define XTGEVC3_PROF_KINDS 7
+#define XTGEVC3_PROF_UPDATE_SPLIT 7      /* far RHS update (whole group), split-int8 digit products */
+#define XTGEVC3_PROF_BACK_SPLIT 8        /* back-transform, split-int8 digit products */
+#define XTGEVC3_PROF_SPLIT_DIGITS 9      /* digit splitting of operands (split-int8 form) */
+#define XTGEVC3_PROF_KINDS 10
 void xtgevc3_profile_enable(int on);
 int xtgevc3_profile_read(double *ms, long *launches, double *flops, double *bytes);
 

@@ -81,3 +81,89 @@ def test_mixed_unsupported_selector():
     S, P, ar, ai, be = G.real_quasi_triangular(20, 3)
     with pytest.raises(ValueError):
         _run(S, P, ar, ai, be, np.eye(20, order="F"), 7)
+
+
+# --- split-int8 form: exact int8 digit products, FP64 residual level ------------------------
+
+
+def _check_split(S, P, ar, ai, be, Z, side, select=None, howmny="B"):
+    """Runs the split-int8 form and the FP64 oracle; returns (max err / cond, residual / (n eps))."""
+    import torch
+
+    import xtgevc3_amd as X
+
+    n = S.shape[0]
+    dS = X.fortran_device(S)
+    dP = X.fortran_device(P)
+    dV = X.fortran_device(Z)
+    kw = {"VR": dV} if side == "R" else {"VL": dV}
+    m, info = X.dtgevc3_device(side, howmny, select, dS, dP, ar, ai, be, precision=X.PREC_SPLIT_I8, **kw)
+    torch.cuda.synchronize()
+    g = X.from_fortran_device(dV)[:, :m]
+    V = Z.copy(order="F")
+    kwo = {"VR": V} if side == "R" else {"VL": V}
+    om, oinfo = O.tgevc3("d", side, howmny, select, S, P, ar, be, alphai=ai, **kwo)[:2]
+    assert (m, info) == (om, oinfo)
+    kind, idx = O.real_layout(howmny, select, ai)
+    V = O.normalize(V[:, :m].copy(order="F"), kind)
+    assert np.all(np.isfinite(g))
+    gc = O.complex_columns(g, kind)
+    mx = np.max(np.abs(gc.real) + np.abs(gc.imag), axis=0)
+    assert np.allclose(mx, 1.0, rtol=0, atol=64 * EPS)
+    err = np.max(np.abs(g - V), axis=0)
+    al, bb = O.eig_alpha_beta(ar, ai, be, idx)
+    cond = O.eigvec_cond(S, P, al, bb, kind, idx, left=side == "L")
+    A, B = (Z @ S @ Z.T, Z @ P @ Z.T) if howmny == "B" else (S, P)
+    res = O.residual_A(A, B, gc, al, bb, left=side == "L")
+    return (err / np.maximum(cond, 1.0)).max(), res.max() / (n * EPS)
+
+
+@pytest.mark.parametrize("side", ["R", "L"])
+@pytest.mark.parametrize("n", [70, 300])
+def test_split_i8_against_oracle(side, n):
+    S, P, ar, ai, be = G.real_quasi_triangular(n, 60 + n, offdiag=0.2)
+    Z = G.random_orthogonal(n, 61 + n)
+    rel, res = _check_split(S, P, ar, ai, be, Z, side)
+    print(f"\n[split-i8 n={n} {side}] max err/cond vs FP64 oracle {rel:.3e}, max residual {res:.3g} n*eps "
+          f"= {res * n * EPS:.3e}")
+    # digit products carry about 2^-39 of max|row| * max|column|: at these small n that level,
+    # not 10 n eps, bounds the residual
+    assert rel <= 1e-10
+    assert res * n * EPS <= 1e-11
+
+
+@pytest.mark.parametrize("side", ["R", "L"])
+def test_split_i8_selected_mid_group(side):
+    """Selected columns starting inside an update group (near-update digits, group digits)."""
+    n = 300
+    S, P, ar, ai, be = G.real_quasi_triangular(n, 19, offdiag=0.2)
+    sel = np.zeros(n, dtype=np.int32)
+    sel[[k for k in range(90, 140) if ai[k] == 0.0][:3]] = 1
+    sel[next(k for k in range(100, n - 1) if ai[k] > 0.0)] = 1
+    Z = G.random_orthogonal(n, 20)
+    rel, res = _check_split(S, P, ar, ai, be, Z, side, sel, "S")
+    print(f"\n[split-i8 select n={n} {side}] max err/cond {rel:.3e}, max residual {res:.3g} n*eps "
+          f"= {res * n * EPS:.3e}")
+    assert rel <= 1e-10
+    assert res * n * EPS <= 1e-11
+
+
+@pytest.mark.parametrize("side", ["R", "L"])
+def test_split_i8_large_residual(side):
+    """n = 2304, N(0,1) pencil (overflow rescaling active): the split-int8 form keeps the
+    FP64 residual bound 10 n eps."""
+    import xtgevc3_amd as X
+
+    n = 2304
+    S, P, ar, ai, be = G.real_quasi_triangular(n, 41)
+    Z = G.random_orthogonal(n, 42)
+    g = _run(S, P, ar, ai, be, Z, X.PREC_SPLIT_I8, side)
+    assert np.all(np.isfinite(g))
+    kind, idx = O.real_layout("B", None, ai)
+    al, bb = O.eig_alpha_beta(ar, ai, be, idx)
+    gc = O.complex_columns(g, kind)
+    mx = np.max(np.abs(gc.real) + np.abs(gc.imag), axis=0)
+    assert np.allclose(mx, 1.0, rtol=0, atol=64 * EPS)
+    res = O.residual_A(Z @ S @ Z.T, Z @ P @ Z.T, gc, al, bb, left=side == "L")
+    print(f"\n[split-i8 n={n} {side} N(0,1)] max residual {res.max():.3e} = {res.max() / (n * EPS):.3g} n*eps")
+    assert res.max() <= 10.0 * n * EPS

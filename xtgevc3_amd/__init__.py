@@ -39,7 +39,7 @@ _lib.xtgevc3_abi_version.restype = ctypes.c_int
 
 INFO_DEVICE_ERROR = -1000
 INFO_UNSUPPORTED = -1001
-PREC_FP64, PREC_MIXED_F32, PREC_MIXED_BF16X3 = 0, 1, 2
+PREC_FP64, PREC_MIXED_F32, PREC_MIXED_BF16X3, PREC_SPLIT_I8 = 0, 1, 2, 3
 
 EXPORTED_SYMBOLS = [
     "xtgevc3_abi_version", "xtgevc3_last_error",
@@ -48,7 +48,8 @@ EXPORTED_SYMBOLS = [
     "xtgevc3_profile_enable", "xtgevc3_profile_read", "xtgevc3_dtgevc3_backward_device",
 ]
 
-PROF_KINDS = ("solve", "update", "back", "other", "update_128x128", "update_128x64", "update_64x128")
+PROF_KINDS = ("solve", "update", "back", "other", "update_128x128", "update_128x64", "update_64x128",
+              "update_split", "back_split", "split_digits")
 
 
 def profile_enable(on=True):
@@ -177,7 +178,7 @@ def tgevc3_device(prefix, side, howmny, select, S, P, alpha, beta, alphai=None, 
                   stream=None, precision=PREC_FP64):
     """Device-pointer call.  S, P, VL, VR are torch tensors holding column-major storage
     (row-major tensors of shape (cols, ld)); select and eigenvalues are host numpy arrays.
-    precision (real interface only): PREC_FP64 or PREC_MIXED_F32.  Returns (m, info)."""
+    precision (real interface only): PREC_FP64, PREC_MIXED_F32 or PREC_SPLIT_I8.  Returns (m, info)."""
     import torch
 
     c = ctypes

@@ -276,7 +276,8 @@ int xtgevc3_dtgevc3_device_ex(char side, char howmny, const int *select, int n, 
                               const double *beta, double *VL, int ldvl, double *VR, int ldvr, int mm, int *m,
                               int *info, void *stream, int precision)
 {
-    if (precision != XTGEVC3_PREC_FP64 && precision != XTGEVC3_PREC_MIXED_F32) {
+    if (precision != XTGEVC3_PREC_FP64 && precision != XTGEVC3_PREC_MIXED_F32 &&
+        precision != XTGEVC3_PREC_SPLIT_I8) {
         xtg::set_error("precision " + std::to_string(precision) + " is not available");
         if (info) {
             *info = XTGEVC3_INFO_UNSUPPORTED;

@@ -73,6 +73,34 @@ struct ZColDesc {
 
 constexpr int WAVE = 64;
 
+// Split-int8 form: digits per value and digit-product levels kept (i + j < SPLIT_LEVELS).
+constexpr int SPLIT_DIGITS = 5;
+constexpr int SPLIT_LEVELS = 5;
+
+// Balanced base-255 digits of u (|u| <= 1/2): x = 255 u_s, d_s = x rounded to nearest with
+// ties toward zero, u_{s+1} = x - d_s.  |x| <= 127.5, so d_s lies in [-127, 127] (the int8
+// range) and |u_{s+1}| <= 1/2; u = sum_s d_s 255^-(s+1) up to 255^-D / 2 and the FP64
+// rounding of the products 255 u_s (relative 2^-53 each).
+constexpr double SPLIT_BASE = 255.0;
+__device__ __forceinline__ void split_digits(double u, int d[SPLIT_DIGITS])
+{
+#pragma clang loop unroll(full)
+    for (int s = 0; s < SPLIT_DIGITS; ++s) {
+        const double x = u * SPLIT_BASE;
+        const double r = x >= 0.0 ? ceil(x - 0.5) : floor(x + 0.5);
+        d[s] = (int)fmin(fmax(r, -127.0), 127.0);
+        u = x - r;
+    }
+}
+
+// Split-int8 bookkeeping of the real solve kernel (bexp == nullptr: off): bexp (offset to
+// the step, row pitch ldx like blkexp) receives E of each column's block operand (2^E >
+// max |operand|, -1000 for a zero block); the group's last step writes gexp[p] = E of the
+// column's whole group operand at its final scale.
+struct SplitOut {
+    int *bexp;
+    int *gexp;
+};
 // tile configurations of the FP64 GEMM (a x b)
 enum : int { GEMM_128x128 = 0, GEMM_64x128 = 1, GEMM_128x64 = 2, GEMM_64x64 = 3 };
 

@@ -32,8 +32,13 @@ namespace xtg {
 template <int MAXB>
 __global__ void k_solve_real(const double *, long, const double *, long, int, int, unsigned long long, double *, long,
                              double *, long, const UnitDesc *, const int4 *, int, int, int, int, double *, int *, int *,
-                             int *, int, int, double *, int, GroupInfo, MixedExp);
+                             int *, int, int, double *, int, GroupInfo, MixedExp, SplitOut);
 __global__ void k_rowexp(const double *, long, const double *, long, int, int, int *);
+__global__ void k_slice_digits(const double *, long, const double *, long, int, int, int, int, int, const int *,
+                               signed char *, long, long);
+int launch_gemm_i8(hipStream_t st, int M, int N, int K1, int K2, const signed char *L1, const signed char *L2,
+                   long lslice, long ldl, const signed char *R, long rslice, long ldr, const int *EL, const int *ER,
+                   double *C, long ldc, const int *outcol, int mode, const double *sig, const int *kmax);
 __global__ void k_xfin(const int *, const int *, int, int *);
 __global__ void k_fill_int(int *, long, int);
 constexpr int SOLVE_UNITS = 4;   // eigenvalue units per wave of the solve kernel (16 lanes per unit)
@@ -75,6 +80,12 @@ static int update_kind(int cfg)
 
 constexpr int MAXB_REAL = 65;  // nominal block 64, +1 when a boundary would split a 2x2 block
 constexpr int NB_REAL = 64;
+// split-int8 form: K step of the digit products; operand row pitch of the far update (two
+// segments of up to 288 rows)
+constexpr int SPLIT_KSTEP = 32;
+constexpr long WG_LD = 2 * 288;
+static_assert(GROUP_STEPS * MAXB_REAL <= 288, "split operand pitch");
+static inline int round_up(int v, int q) { return (v + q - 1) / q * q; }
 
 // ---------------------------------------------------------------------------------------
 // argument semantics shared by every real entry point
@@ -295,6 +306,7 @@ struct DevBuf {
 struct RealWork {
     DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wblk, wgrp, xrel, xfin, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
     DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp;
+    DevBuf Sq, Pq, Wg, wgexp, wbexp, Xq, Zq, xexp; // split-int8 digits and exponents
 };
 
 static std::mutex g_ws_mutex;
@@ -374,6 +386,29 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     ENSURE(ws.rowstep, sizeof(int) * (size_t)n);
     XTG_CHECK(hipMemcpyAsync(ws.desc.p, units.data(), sizeof(UnitDesc) * units.size(), hipMemcpyHostToDevice, st));
     MixedExp mx{nullptr, nullptr, nullptr, 0};
+    const bool split = prec == XTGEVC3_PREC_SPLIT_I8;
+    long qld = 0, qsl = 0;
+    if (split) {
+        // digits of the working pencil rows (exponent per row over S and P), once per sweep
+        qld = round_up(n, SPLIT_KSTEP) + SPLIT_KSTEP;
+        qsl = (long)n * qld;
+        ENSURE(ws.rowexp, sizeof(int) * (size_t)n);
+        ENSURE(ws.Sq, (size_t)SPLIT_DIGITS * qsl);
+        ENSURE(ws.Pq, (size_t)SPLIT_DIGITS * qsl);
+        ENSURE(ws.Wg, (size_t)SPLIT_DIGITS * m * WG_LD);
+        ENSURE(ws.wgexp, sizeof(int) * (size_t)m);
+        ENSURE(ws.wbexp, sizeof(int) * (size_t)nsteps * ldx);
+        hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.rowexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
+        hipLaunchKernelGGL(k_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, Sw, ldsw, Pw, ldpw, n, n,
+                           ws.rowexp.as<int>()); XTG_LAUNCH_CHECK("k_rowexp");
+        const dim3 grid((n + 63) / 64, (unsigned)((qld + 63) / 64));
+        hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, Sw, ldsw, (const double *)nullptr, 0L, n, n,
+                           (int)qld, 0, (int)qld, (const int *)ws.rowexp.as<int>(), ws.Sq.as<signed char>(), qsl,
+                           qld); XTG_LAUNCH_CHECK("k_slice_digits");
+        hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, Pw, ldpw, (const double *)nullptr, 0L, n, n,
+                           (int)qld, 0, (int)qld, (const int *)ws.rowexp.as<int>(), ws.Pq.as<signed char>(), qsl,
+                           qld); XTG_LAUNCH_CHECK("k_slice_digits");
+    }
     if (prec == 1) {
         // mixed form: row exponents of the working pencil, per-column operand/solution bounds
         ENSURE(ws.rowexp, sizeof(int) * (size_t)n);
@@ -471,18 +506,21 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
                                ldsw, Pw, ldpw, i0, nb, bumps, X, ldx, W, ldw, ws.desc.as<UnitDesc>(), ws.tiles.as<int4>(),
                                t_begin, t_end, u_begin, u_end, ws.sig.as<double>(), ws.colexp.as<int>(),
                                ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff, wr + woff,
-                               ws.sig_grp.as<double>(), s == gs ? 1 : 0, grp, mx); XTG_LAUNCH_CHECK("k_solve_real");
+                               ws.sig_grp.as<double>(), s == gs ? 1 : 0, grp, mx,
+                               split ? SplitOut{ws.wbexp.as<int>() + (long)s * ldx, ws.wgexp.as<int>()}
+                                     : SplitOut{nullptr, nullptr}); XTG_LAUNCH_CHECK("k_solve_real");
             prof_end(pr, st, 0, 2.0 * nb * nb * N, 8.0 * (4.0 * nb * N + 2.0 * nb * nb));
             // near update: the group's rows above this block
             const int r0 = adjoint ? std::max(g0, std::min(desc[c_begin].lo, i0)) : g0;
             const int M = i0 - r0;
+            // (FP64 also in the split-int8 form: its short products run faster there)
             if (M > 0) {
                 pr = prof_begin(st);
                 const int gt = launch_gemm_f64(0, st, M, N, nb, nb, Sw + r0 + (long)i0 * ldsw,
                                    ldsw, Pw + r0 + (long)i0 * ldpw, ldpw, (const double *)(W + (long)woff * ldw + c_begin),
                                    ldw, X + (long)r0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
                                    (const double *)(ws.sig.as<double>() + c_begin), (const int *)nullptr,
-                                   (const double *)(W + (long)(wr + woff) * ldw + c_begin), prec,
+                                   (const double *)(W + (long)(wr + woff) * ldw + c_begin), split ? 0 : prec,
                                    mx.on ? ws.rowexp.as<int>() + r0 : nullptr, mx.on ? mx.wblk + c_begin : nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
                 prof_end(pr, st, update_kind(gt), 2.0 * M * N * (2.0 * nb),
                          8.0 * (2.0 * M * N + 2.0 * nb * M + 2.0 * nb * N));
@@ -495,7 +533,27 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         const int N = c_end - c_begin;
         const int r0 = adjoint ? std::min(desc[c_begin].lo, g0) : 0;
         const int M = g0 - r0;
-        if (M > 0 && N > 0) {
+        if (M > 0 && N > 0 && split) {
+            // digits of the group operand (exponent per column over both parts, from the
+            // group's last solve), then the product
+            const int wrp = round_up(wr, SPLIT_KSTEP);
+            int pr = prof_begin(st);
+            signed char *wg = ws.Wg.as<signed char>() + (long)c_begin * WG_LD;
+            hipLaunchKernelGGL(k_slice_digits, dim3((N + 63) / 64, (2 * wrp + 63) / 64), dim3(256), 0, st,
+                               (const double *)(W + c_begin), ldw, (const double *)(W + (long)wr * ldw + c_begin), ldw,
+                               N, wr, wrp, wr, 2 * wrp, (const int *)(ws.wgexp.as<int>() + c_begin), wg,
+                               (long)m * WG_LD, WG_LD); XTG_LAUNCH_CHECK("k_slice_digits");
+            prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * (2.0 * wr * N) + SPLIT_DIGITS * 2.0 * wrp * N);
+            pr = prof_begin(st);
+            XTG_CHECK((hipError_t)launch_gemm_i8(
+                st, M, N, wrp, wrp, ws.Sq.as<signed char>() + (long)r0 * qld + g0,
+                ws.Pq.as<signed char>() + (long)r0 * qld + g0, qsl, qld, wg, (long)m * WG_LD, WG_LD,
+                ws.rowexp.as<int>() + r0, ws.wgexp.as<int>() + c_begin, X + (long)r0 * ldx + c_begin, ldx, nullptr,
+                1, ws.sig_grp.as<double>() + c_begin, nullptr));
+            prof_end(pr, st, XTGEVC3_PROF_UPDATE_SPLIT, 2.0 * M * N * (2.0 * wr),
+                     8.0 * (2.0 * M * N) + SPLIT_DIGITS * (2.0 * wrp * M + 2.0 * wrp * N));
+        }
+        else if (M > 0 && N > 0) {
             int pr = prof_begin(st);
             const int gt = launch_gemm_f64(0, st, M, N, wr, wr, Sw + r0 + (long)g0 * ldsw, ldsw,
                                Pw + r0 + (long)g0 * ldpw, ldpw, (const double *)(W + c_begin), ldw,
@@ -606,12 +664,43 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
             hipLaunchKernelGGL(k_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, A.Z, A.ldz,
                                (const double *)nullptr, 0L, n, n, ws.zexp.as<int>()); XTG_LAUNCH_CHECK("k_rowexp");
         }
-        int pr = prof_begin(st);
-        launch_gemm_f64(1, st, m, n, n, 0, (const double *)X, ldx,
-                           (const double *)nullptr, 0L, Rp, ldr, A.V, A.ldv, ws.outcol.as<int>(), 0,
-                           (const double *)nullptr, ws.kmax.as<int>(), (const double *)nullptr, A.prec,
-                           A.prec == 1 ? ws.xfin.as<int>() : nullptr, A.prec == 1 ? ws.zexp.as<int>() : nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<1>");
-        prof_end(pr, st, 2, 2.0 * n * ksum, 8.0 * ((double)n * kmx + ksum + (double)n * m));
+        if (A.prec == XTGEVC3_PREC_SPLIT_I8) {
+            // digits of the columns of X and of the rows of Z (exponents from exact maxima)
+            const int kb = round_up(n, SPLIT_KSTEP);
+            ENSURE(ws.xexp, sizeof(int) * (size_t)m);
+            ENSURE(ws.zexp, sizeof(int) * (size_t)n);
+            ENSURE(ws.Xq, (size_t)SPLIT_DIGITS * m * kb);
+            ENSURE(ws.Zq, (size_t)SPLIT_DIGITS * n * kb);
+            int pr = prof_begin(st);
+            hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.xexp.as<int>(), (long)m, -1000); XTG_LAUNCH_CHECK("k_fill_int");
+            hipLaunchKernelGGL(k_rowexp, dim3((m + 255) / 256, (n + 63) / 64), dim3(256), 0, st, (const double *)X, ldx,
+                               (const double *)nullptr, 0L, m, n, ws.xexp.as<int>()); XTG_LAUNCH_CHECK("k_rowexp");
+            hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.zexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
+            hipLaunchKernelGGL(k_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, A.Z, A.ldz,
+                               (const double *)nullptr, 0L, n, n, ws.zexp.as<int>()); XTG_LAUNCH_CHECK("k_rowexp");
+            hipLaunchKernelGGL(k_slice_digits, dim3((m + 63) / 64, (kb + 63) / 64), dim3(256), 0, st, (const double *)X,
+                               ldx, (const double *)nullptr, 0L, m, n, kb, 0, kb, (const int *)ws.xexp.as<int>(),
+                               ws.Xq.as<signed char>(), (long)m * kb, (long)kb); XTG_LAUNCH_CHECK("k_slice_digits");
+            hipLaunchKernelGGL(k_slice_digits, dim3((n + 63) / 64, (kb + 63) / 64), dim3(256), 0, st, Rp, ldr,
+                               (const double *)nullptr, 0L, n, n, kb, 0, kb, (const int *)ws.zexp.as<int>(),
+                               ws.Zq.as<signed char>(), (long)n * kb, (long)kb); XTG_LAUNCH_CHECK("k_slice_digits");
+            prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * 2.0 * ((double)n * m + (double)n * n));
+            pr = prof_begin(st);
+            XTG_CHECK((hipError_t)launch_gemm_i8(st, m, n, kb, 0, ws.Xq.as<signed char>(), ws.Xq.as<signed char>(),
+                                                 (long)m * kb, kb, ws.Zq.as<signed char>(), (long)n * kb, kb,
+                                                 ws.xexp.as<int>(), ws.zexp.as<int>(), A.V, A.ldv,
+                                                 ws.outcol.as<int>(), 0, nullptr, ws.kmax.as<int>()));
+            prof_end(pr, st, XTGEVC3_PROF_BACK_SPLIT, 2.0 * n * ksum,
+                     8.0 * (double)n * m + SPLIT_DIGITS * ((double)n * kmx + ksum));
+        }
+        else {
+            int pr = prof_begin(st);
+            launch_gemm_f64(1, st, m, n, n, 0, (const double *)X, ldx, (const double *)nullptr, 0L, Rp, ldr, A.V,
+                            A.ldv, ws.outcol.as<int>(), 0, (const double *)nullptr, ws.kmax.as<int>(),
+                            (const double *)nullptr, A.prec, A.prec == 1 ? ws.xfin.as<int>() : nullptr,
+                            A.prec == 1 ? ws.zexp.as<int>() : nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<1>");
+            prof_end(pr, st, 2, 2.0 * n * ksum, 8.0 * ((double)n * kmx + ksum + (double)n * m));
+        }
     }
     else {
         dim3 grid((m + 31) / 32, (n + 31) / 32);

@@ -223,6 +223,7 @@ struct UnitOut {
     int first_in_group;
     GroupInfo grp;
     MixedExp mx;
+    SplitOut sq;
 };
 
 // One unit's block step: factor the lane's rows, sweep the block bottom-up starting from the
@@ -545,6 +546,49 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
             mx.xrel[p] = max(mx.xrel[p], ex - ce);
         }
     }
+    if (o.sq.bexp != nullptr) {
+        // split-int8 form: exponent of each column's block operand (exact maximum over the
+        // block's S and P parts) and, at the group's last step, of its group operand
+        const SplitOut &sq = o.sq;
+        double m0 = 0.0, m1 = 0.0;
+#pragma clang loop unroll(full)
+        for (int i = 0; i < QR; ++i) {
+            if (QL * i + j < nb) {
+                const V v = vr[i];
+                const V bx = A::bmul(bR, bI, v);
+                m0 = fmax(m0, fmax(fabs(a * A::re(v)), fabs(A::re(bx))));
+                m1 = fmax(m1, fmax(fabs(a * A::im(v)), fabs(A::im(bx))));
+            }
+        }
+#pragma clang loop unroll(full)
+        for (int off = 1; off < QL; off <<= 1) {
+            m0 = fmax(m0, __shfl_xor(m0, off));
+            m1 = fmax(m1, __shfl_xor(m1, off));
+        }
+        if (act && j < ucols) {
+            const int p = col + j;
+            const double mj = j == 0 ? m0 : m1;
+            const int ej = mj > 0.0 ? min(max(ilogb(mj) + 1, -1000), 1000) : -1000;
+            sq.bexp[p] = ej;
+            if (grp.last) {
+                // the lower blocks the column reaches, each brought to the final scale by
+                // 2^(ce - blkexp) above
+                const int ce = colexp[p] + my_exp;
+                const int d = dl + i0;
+                int eg = ej;
+                for (int jb = 1; jb < GROUP_MAX_BLOCKS; ++jb) {
+                    if (grp.off[jb] < grp.off[jb + 1] && i0 + grp.off[jb] <= d) {
+                        const long so = (long)(grp.step[jb] - grp.step[0]) * ldx + p;
+                        const int eb = sq.bexp[so];
+                        if (eb > -1000) {
+                            eg = max(eg, eb + ce - blkexp[so]);
+                        }
+                    }
+                }
+                sq.gexp[p] = min(max(eg, -1000), 1000);
+            }
+        }
+    }
     if (act && j < ucols) {
         // lane 0: real part column, lane 1: imaginary part column
         const int p = col + j;
@@ -600,7 +644,7 @@ __global__ __launch_bounds__(256) void k_solve_real(
     unsigned long long bumps, double *__restrict__ X, long ldx, double *__restrict__ W, long ldw,
     const UnitDesc *__restrict__ units, const int4 *__restrict__ tiles, int t_begin, int t_end, int u_begin,
     int u_end, double *__restrict__ sig, int *__restrict__ colexp, int *__restrict__ blkexp, int *__restrict__ flag,
-    int w_s, int w_p, double *__restrict__ sig_grp, int first_in_group, GroupInfo grp, MixedExp mx)
+    int w_s, int w_p, double *__restrict__ sig_grp, int first_in_group, GroupInfo grp, MixedExp mx, SplitOut sq)
 {
     static_assert(MAXB <= 65 && SP_PITCH >= MAXB + 1 && (SP_PITCH % 2) == 0, "block layout");
     constexpr int QR = (MAXB + QL - 1) / QL; // rows per lane
@@ -701,6 +745,7 @@ __global__ __launch_bounds__(256) void k_solve_real(
     o.first_in_group = first_in_group;
     o.grp = grp;
     o.mx = mx;
+    o.sq = sq;
     if (!__any(act && pair)) {
         unit_block<double, QR, MAXB>(us, raw, o);
     }
@@ -1187,7 +1232,7 @@ namespace xtg {
 template __global__ void k_solve_real<65>(const double *, long, const double *, long, int, int, unsigned long long,
                                           double *, long, double *, long, const UnitDesc *, const int4 *, int, int,
                                           int, int, double *, int *, int *, int *, int, int, double *, int, GroupInfo,
-                                          MixedExp);
+                                          MixedExp, SplitOut);
 } // namespace xtg
 
 // =======================================================================================

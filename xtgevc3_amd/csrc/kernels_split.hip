@@ -1,0 +1,325 @@
+// gfx950 kernels of the split-int8 form of the real path (XTGEVC3_PREC_SPLIT_I8).
+//
+// A value v of a row (or column) with exponent E (2^E > max |row|) is written as five
+// balanced base-255 digits d_0..d_4 in [-127, 127] (split_digits):
+//     v = 2^(E+1) * sum_s d_s 255^-(s+1) + r,   |r| <= 2^(E+1) 255^-5 / 2  (about 2^-41 2^E).
+// A product sum_k L(a,k) R(b,k) then becomes, with exact int32 MFMA accumulation,
+//     2^(EL_a + ER_b + 2) * sum_t 255^-(t+2) acc_t,    acc_t = sum_{i+j=t} sum_k dL_i(a,k) dR_j(b,k),
+// keeping the levels t = i + j <= 4 (15 digit products).  The dropped levels and the digit
+// remainders bound the error by about 2^-39 * K * max|L(a,:)| * max|R(b,:)|; |acc_t| stays
+// below 5 * 127^2 * K, exact in int32 for K <= SPLIT_KMAX (longer products run in chunks).
+//
+//   k_slice_digits -- digits of a matrix, rows a with their exponents, k-contiguous
+//   k_gemm_i8      -- C = sig * C + sum_k L(a,k) R(b,k) on v_mfma_i32_32x32x32_i8, digits
+//                     staged through LDS, FP64 epilogue
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+
+#include "common.h"
+
+namespace xtg {
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x16 __attribute__((ext_vector_type(16)));
+
+// Q[s * qs + a * ldq + k] = digit s of A(a, k) * 2^-(E[a] + 1), A(a, k) = A1[a + k * lda1] for
+// k < K1, A2[a + (k - K1p) * lda2] for K1p <= k < K1p + K2, zero elsewhere in [0, Kp).
+// Block: 64 rows a x 64 k, thread t: row t / 4, k chunk 16 (t % 4).
+__global__ __launch_bounds__(256) void k_slice_digits(const double *__restrict__ A1, long lda1,
+                                                      const double *__restrict__ A2, long lda2, int M, int K1,
+                                                      int K1p, int K2, int Kp, const int *__restrict__ E,
+                                                      signed char *__restrict__ Q, long qs, long ldq)
+{
+    const int a = blockIdx.x * 64 + threadIdx.x / 4;
+    const int k0 = blockIdx.y * 64 + (threadIdx.x % 4) * 16;
+    if (a >= M || k0 >= Kp) {
+        return;
+    }
+    const int e = E[a] + 1;
+    int dg[SPLIT_DIGITS][16];
+#pragma clang loop unroll(full)
+    for (int q = 0; q < 16; ++q) {
+        const int k = k0 + q;
+        double v = 0.0;
+        if (k < K1) {
+            v = A1[a + (long)k * lda1];
+        }
+        else if (k >= K1p && k < K1p + K2) {
+            v = A2[a + (long)(k - K1p) * lda2];
+        }
+        int d[SPLIT_DIGITS];
+        split_digits(ldexp(v, -e), d);
+#pragma clang loop unroll(full)
+        for (int s = 0; s < SPLIT_DIGITS; ++s) {
+            dg[s][q] = d[s];
+        }
+    }
+#pragma clang loop unroll(full)
+    for (int s = 0; s < SPLIT_DIGITS; ++s) {
+        i32x4 w;
+#pragma clang loop unroll(full)
+        for (int c = 0; c < 4; ++c) {
+            w[c] = (dg[s][4 * c] & 0xff) | ((dg[s][4 * c + 1] & 0xff) << 8) | ((dg[s][4 * c + 2] & 0xff) << 16) |
+                   ((dg[s][4 * c + 3] & 0xff) << 24);
+        }
+        *reinterpret_cast<i32x4 *>(Q + s * qs + (long)a * ldq + k0) = w;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// k_gemm_i8: C[out(a)][b] = sig[b] * C + 2^(EL[a] + ER[b] + 2) sum_t 255^-(t+2) acc_t(a, b)
+// (mode 0: no sig * C term; sig == nullptr: sig = 1).  L digits: rows a, k-contiguous; k < K1 from L1, k in
+// [K1, K1 + K2) from L2 at k - K1; R digits: rows b, k-contiguous over [0, K1 + K2).  K1, K2
+// are multiples of 32; kmax (optional) bounds k + kofs per row a (rounded up to 32 per tile).
+// Block tile BM x BN, waves WMW x WNW, each wave (BM/WMW) x (BN/WNW) in 32x32 MFMA blocks.
+// K step 32.  A stage holds the step's digits of the tile, per digit BM (then BN) rows of
+// 32 bytes; NSTAGE stages rotate, filled by direct global-to-LDS loads (16 bytes per lane,
+// consecutive lanes to consecutive LDS addresses) two steps ahead of the MFMA work.  Rows
+// beyond M (N) load row M-1 (N-1): they only feed outputs that are not written.
+// Tiles: the launch is 1-D; workgroup w runs on XCD w % 8, so each XCD takes a contiguous
+// range of tiles: a group of 8 column tiles over all row tiles (even work per XCD also for a
+// triangular kmax; the XCD's concurrent tiles share L2 panels of L and R).
+// MFMA operand bytes: lane l, byte q <-> (row l % 32, k = 16 (l / 32) + q) for A and B alike;
+// result register v of lane l <-> (row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32).
+// ---------------------------------------------------------------------------------------
+constexpr int QK = 32;
+constexpr int NSTAGE = 3;
+constexpr int SPLIT_KMAX = 26624; // 5 * 127^2 * K < 2^31
+constexpr int NXCD = 8;
+
+// s_waitcnt with vmcnt = N (0..63), other counters at their maximum (gfx9 encoding)
+template <int N> __device__ __forceinline__ void wait_vm()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+template <int BM, int BN, int WMW, int WNW>
+__global__ __launch_bounds__(WMW *WNW * 64) void k_gemm_i8(
+    int M, int N, int K1, int K2, const signed char *__restrict__ L1, const signed char *__restrict__ L2, long lslice,
+    long ldl, const signed char *__restrict__ R, long rslice, long ldr, const int *__restrict__ EL,
+    const int *__restrict__ ER, double *__restrict__ C, long ldc, const int *__restrict__ outcol, int mode,
+    const double *__restrict__ sig, const int *__restrict__ kmax, int kofs)
+{
+    constexpr int NW = WMW * WNW;
+    constexpr int WTM = BM / WMW, WTN = BN / WNW;
+    constexpr int AM = WTM / 32, AN = WTN / 32;
+    static_assert(AM * 32 == WTM && AN * 32 == WTN, "wave tile");
+    constexpr int CA = SPLIT_DIGITS * BM * 2, CB = SPLIT_DIGITS * BN * 2; // 16-byte chunks per step
+    constexpr int CH = CA + CB;
+    static_assert(CA % 64 == 0 && CH % 64 == 0, "load instructions cover one operand each");
+    constexpr int IPW = (CH + 64 * NW - 1) / (64 * NW); // load instructions per wave and step
+    constexpr int STAGE = IPW * NW * 64 * 16;            // bytes per stage (chunks past CH: padding)
+    constexpr int LV = SPLIT_LEVELS;
+    __shared__ __attribute__((aligned(16))) signed char lds[NSTAGE * STAGE];
+
+    // tile of this workgroup: position p in the order (groups of 8 column tiles; within a
+    // group the row tiles from the last (largest kmax) down; the group's columns fastest),
+    // each XCD taking a contiguous range of p
+    const int tm_n = (M + BM - 1) / BM, tn_n = (N + BN - 1) / BN;
+    const int nblk = tm_n * tn_n;
+    int p;
+    {
+        const int l = blockIdx.x, x = l % NXCD, slot = l / NXCD;
+        const int q = nblk / NXCD, r = nblk % NXCD;
+        p = x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+    }
+    const int grp = p / (8 * tm_n), gw = min(8, tn_n - grp * 8), rem = p - grp * 8 * tm_n;
+    const int a0 = (tm_n - 1 - rem / gw) * BM, b0 = (grp * 8 + rem % gw) * BN;
+
+    const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+    const int wa = (wave / WNW) * WTM, wb = (wave % WNW) * WTN;
+    int K = K1 + K2;
+    if (kmax != nullptr) {
+        int kk = 0;
+        const int amax = min(a0 + BM, M);
+        for (int q = a0; q < amax; ++q) {
+            kk = max(kk, kmax[q]);
+        }
+        K = min(K, max(0, (kk - kofs + QK - 1) / QK * QK));
+    }
+    // per-lane source offsets of its chunks (fixed over the steps); chunks past CH reload
+    // chunk CA (a valid R row) into the stage's padding
+    long off[IPW];
+#pragma clang loop unroll(full)
+    for (int u = 0; u < IPW; ++u) {
+        const int c = (u * NW + wave) * 64 + lane;
+        if (c < CA) {
+            const int sd = c / (BM * 2), rr = c % (BM * 2);
+            off[u] = sd * lslice + (long)min(a0 + rr / 2, M - 1) * ldl + (rr % 2) * 16;
+        }
+        else {
+            const int cc = c < CH ? c - CA : 0;
+            const int sd = cc / (BN * 2), rr = cc % (BN * 2);
+            off[u] = sd * rslice + (long)min(b0 + rr / 2, N - 1) * ldr + (rr % 2) * 16;
+        }
+    }
+    auto issue = [&](int st) {
+        const int k0 = st * QK;
+        const signed char *lbase = k0 < K1 ? L1 + k0 : L2 + (k0 - K1);
+        signed char *stage = lds + (st % NSTAGE) * STAGE;
+#pragma clang loop unroll(full)
+        for (int u = 0; u < IPW; ++u) {
+            const int cbase = (u * NW + wave) * 64; // wave-uniform; an instruction loads one operand
+            const signed char *g = (cbase < CA ? lbase : R + k0) + off[u];
+            __builtin_amdgcn_global_load_lds((const void *)g,
+                                             (__attribute__((address_space(3))) void *)(stage + cbase * 16), 16, 0, 0);
+        }
+    };
+
+    i32x16 acc[LV][AM][AN];
+#pragma clang loop unroll(full)
+    for (int t = 0; t < LV; ++t) {
+#pragma clang loop unroll(full)
+        for (int i = 0; i < AM; ++i) {
+#pragma clang loop unroll(full)
+            for (int j = 0; j < AN; ++j) {
+#pragma clang loop unroll(full)
+                for (int v = 0; v < 16; ++v) {
+                    acc[t][i][j][v] = 0;
+                }
+            }
+        }
+    }
+    const int nst = K / QK;
+    if (nst > 0) {
+        issue(0);
+    }
+    if (nst > 1) {
+        issue(1);
+    }
+    const int fr = lane % 32, fh = (lane / 32) * 16;
+    for (int st = 0; st < nst; ++st) {
+        if (st + 1 < nst) {
+            wait_vm<IPW>();
+        }
+        else {
+            wait_vm<0>();
+        }
+        // every wave's loads of this stage have landed (its own vmcnt wait) and every wave is
+        // done with the stage refilled next; no memory fence, so the loads two steps ahead
+        // stay in flight
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_sched_barrier(0);
+        if (st + 2 < nst) {
+            issue(st + 2);
+        }
+        const signed char *sa = lds + (st % NSTAGE) * STAGE;
+        const signed char *sb = sa + CA * 16;
+        i32x4 bf[SPLIT_DIGITS][AN];
+#pragma clang loop unroll(full)
+        for (int sd = 0; sd < SPLIT_DIGITS; ++sd) {
+#pragma clang loop unroll(full)
+            for (int j = 0; j < AN; ++j) {
+                bf[sd][j] = *reinterpret_cast<const i32x4 *>(sb + sd * (BN * 32) + (wb + 32 * j + fr) * 32 + fh);
+            }
+        }
+#pragma clang loop unroll(full)
+        for (int sd = 0; sd < SPLIT_DIGITS; ++sd) {
+            i32x4 af[AM];
+#pragma clang loop unroll(full)
+            for (int i = 0; i < AM; ++i) {
+                af[i] = *reinterpret_cast<const i32x4 *>(sa + sd * (BM * 32) + (wa + 32 * i + fr) * 32 + fh);
+            }
+#pragma clang loop unroll(full)
+            for (int r = 0; r + sd < LV && r < SPLIT_DIGITS; ++r) {
+#pragma clang loop unroll(full)
+                for (int i = 0; i < AM; ++i) {
+#pragma clang loop unroll(full)
+                    for (int j = 0; j < AN; ++j) {
+                        acc[sd + r][i][j] =
+                            __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[r][j], acc[sd + r][i][j], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+#pragma clang loop unroll(full)
+    for (int i = 0; i < AM; ++i) {
+#pragma clang loop unroll(full)
+        for (int v = 0; v < 16; ++v) {
+            const int a = a0 + wa + 32 * i + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+            if (a >= M) {
+                continue;
+            }
+            const long rowoff = (long)(outcol ? outcol[a] : a) * ldc;
+            const int ea = EL[a];
+#pragma clang loop unroll(full)
+            for (int j = 0; j < AN; ++j) {
+                const int b = b0 + wb + 32 * j + fr;
+                if (b < N) {
+                    double sum = (double)acc[LV - 1][i][j][v];
+#pragma clang loop unroll(full)
+                    for (int t = LV - 2; t >= 0; --t) {
+                        sum = fma(sum, 1.0 / SPLIT_BASE, (double)acc[t][i][j][v]);
+                    }
+                    double d = ldexp(sum * (1.0 / (SPLIT_BASE * SPLIT_BASE)), ea + ER[b] + 2);
+                    double *cp = C + rowoff + b;
+                    if (mode == 1) {
+                        d = fma(sig != nullptr ? sig[b] : 1.0, *cp, d);
+                    }
+                    *cp = d;
+                }
+            }
+        }
+    }
+}
+
+#define XTG_I8_INST(BM, BN, WMW, WNW)                                                                               \
+    template __global__ void k_gemm_i8<BM, BN, WMW, WNW>(int, int, int, int, const signed char *,                   \
+                                                         const signed char *, long, long, const signed char *, long, \
+                                                         long, const int *, const int *, double *, long,             \
+                                                         const int *, int, const double *, const int *, int);
+XTG_I8_INST(128, 128, 2, 4)
+XTG_I8_INST(64, 128, 1, 4)
+#undef XTG_I8_INST
+
+// One launch; the tile is 64 x 128 when M is small.
+static int launch_i8(hipStream_t st, int M, int N, int K1, int K2, const signed char *L1, const signed char *L2,
+                     long lslice, long ldl, const signed char *R, long rslice, long ldr, const int *EL, const int *ER,
+                     double *C, long ldc, const int *outcol, int mode, const double *sig, const int *kmax, int kofs)
+{
+    if (M <= 64) {
+        const dim3 grid(((N + 127) / 128) * ((M + 63) / 64));
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_i8<64, 128, 1, 4>), grid, dim3(256), 0, st, M, N, K1, K2, L1, L2,
+                           lslice, ldl, R, rslice, ldr, EL, ER, C, ldc, outcol, mode, sig, kmax, kofs);
+    }
+    else {
+        const dim3 grid(((N + 127) / 128) * ((M + 127) / 128));
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_i8<128, 128, 2, 4>), grid, dim3(512), 0, st, M, N, K1, K2, L1, L2,
+                           lslice, ldl, R, rslice, ldr, EL, ER, C, ldc, outcol, mode, sig, kmax, kofs);
+    }
+    return (int)hipGetLastError();
+}
+
+// Launches C = sig C + L R^T on the digits; returns 0 or a HIP error code.  A product longer
+// than SPLIT_KMAX (single segment only) runs in K chunks accumulated into C.
+int launch_gemm_i8(hipStream_t st, int M, int N, int K1, int K2, const signed char *L1, const signed char *L2,
+                   long lslice, long ldl, const signed char *R, long rslice, long ldr, const int *EL, const int *ER,
+                   double *C, long ldc, const int *outcol, int mode, const double *sig, const int *kmax)
+{
+    if (M <= 0 || N <= 0) {
+        return 0;
+    }
+    if (K1 % QK != 0 || K2 % QK != 0 || K2 > SPLIT_KMAX) {
+        return (int)hipErrorInvalidValue;
+    }
+    if (K1 + K2 > SPLIT_KMAX) {
+        if (K2 != 0) {
+            return (int)hipErrorInvalidValue;
+        }
+        for (int k0 = 0; k0 < K1; k0 += SPLIT_KMAX) {
+            const int kc = std::min(SPLIT_KMAX, K1 - k0);
+            const int rc = launch_i8(st, M, N, kc, 0, L1 + k0, L1 + k0, lslice, ldl, R + k0, rslice, ldr, EL, ER, C,
+                                     ldc, outcol, k0 == 0 ? mode : 1, k0 == 0 ? sig : nullptr, kmax, k0);
+            if (rc != 0) {
+                return rc;
+            }
+        }
+        return 0;
+    }
+    return launch_i8(st, M, N, K1, K2, L1, L2, lslice, ldl, R, rslice, ldr, EL, ER, C, ldc, outcol, mode, sig, kmax, 0);
+}
+
+} // namespace xtg
