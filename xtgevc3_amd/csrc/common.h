@@ -93,13 +93,14 @@ __device__ __forceinline__ void split_digits(double u, int d[SPLIT_DIGITS])
     }
 }
 
-// Split-int8 bookkeeping of the real solve kernel (bexp == nullptr: off): bexp (offset to
-// the step, row pitch ldx like blkexp) receives E of each column's block operand (2^E >
-// max |operand|, -1000 for a zero block); the group's last step writes gexp[p] = E of the
-// column's whole group operand at its final scale.
+// Split-int8 bookkeeping of the real solve kernel (bexp == nullptr: off): bexp and xbexp
+// (offset to the step, row pitch ldx like blkexp) receive E of each column's block operand
+// and of its block of X (2^E > max |.|, -1000 for a zero block); the group's last step
+// writes gexp[p] = E of the column's whole group operand at its final scale.
 struct SplitOut {
     int *bexp;
     int *gexp;
+    int *xbexp;
 };
 // tile configurations of the FP64 GEMM (a x b)
 enum : int { GEMM_128x128 = 0, GEMM_64x128 = 1, GEMM_128x64 = 2, GEMM_64x64 = 3 };

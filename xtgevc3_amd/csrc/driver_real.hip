@@ -34,10 +34,11 @@ __global__ void k_solve_real(const double *, long, const double *, long, int, in
                              double *, long, const UnitDesc *, const int4 *, int, int, int, int, double *, int *, int *,
                              int *, int, int, double *, int, GroupInfo, MixedExp, SplitOut);
 __global__ void k_rowexp(const double *, long, const double *, long, int, int, int *);
-__global__ void k_slice_digits(const double *, long, const double *, long, int, int, int, int, int, const int *,
+__global__ void k_slice_digits(const double *, long, int, int, const double *, long, int, int, int, int, const int *,
                                signed char *, long, long);
+__global__ void k_xexp_final(const int *, const int *, long, const int *, const int *, int, int, int *);
 int launch_gemm_i8(hipStream_t st, int M, int N, int K1, int K2, const signed char *L1, const signed char *L2,
-                   long lslice, long ldl, const signed char *R, long rslice, long ldr, const int *EL, const int *ER,
+                   long lslice, long kbl, const signed char *R, long rslice, long kbr, const int *EL, const int *ER,
                    double *C, long ldc, const int *outcol, int mode, const double *sig, const int *kmax);
 __global__ void k_xfin(const int *, const int *, int, int *);
 __global__ void k_fill_int(int *, long, int);
@@ -80,12 +81,15 @@ static int update_kind(int cfg)
 
 constexpr int MAXB_REAL = 65;  // nominal block 64, +1 when a boundary would split a 2x2 block
 constexpr int NB_REAL = 64;
-// split-int8 form: K step of the digit products; operand row pitch of the far update (two
-// segments of up to 288 rows)
+// split-int8 form: K step of the digit products (digits are k-blocked by it); K extent of the
+// far update's group operand (two segments, each the group's rows shifted to the K step
+// boundary below the group and rounded up: at most 320)
 constexpr int SPLIT_KSTEP = 32;
-constexpr long WG_LD = 2 * 288;
-static_assert(GROUP_STEPS * MAXB_REAL <= 288, "split operand pitch");
+constexpr long WG_K = 2 * 320;
+static_assert(GROUP_STEPS * MAXB_REAL + SPLIT_KSTEP - 1 <= 320, "split operand extent");
 static inline int round_up(int v, int q) { return (v + q - 1) / q * q; }
+// row pitch of the pencil digits: the far update reads up to SPLIT_KSTEP - 1 columns past n
+static inline long split_qld(int n) { return round_up(n, SPLIT_KSTEP) + SPLIT_KSTEP; }
 
 // ---------------------------------------------------------------------------------------
 // argument semantics shared by every real entry point
@@ -306,7 +310,11 @@ struct DevBuf {
 struct RealWork {
     DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wblk, wgrp, xrel, xfin, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
     DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp;
-    DevBuf Sq, Pq, Wg, wgexp, wbexp, Xq, Zq, xexp; // split-int8 digits and exponents
+    DevBuf Sq, Pq, Wg, wgexp, wbexp, xbexp, sstart, Xq, Zq, xexp; // split-int8 digits and exponents
+    // split-int8 form: side stream (default priority) for the digit splitting of the pencil
+    // and of Z, overlapped with the sweep; events order it against the caller's stream
+    hipStream_t side = nullptr;
+    hipEvent_t ev_start = nullptr, ev_pencil = nullptr, ev_z = nullptr;
 };
 
 static std::mutex g_ws_mutex;
@@ -354,7 +362,7 @@ struct RealSideArgs {
 // ---------------------------------------------------------------------------------------
 int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *Pw, long ldpw,
                const std::vector<double> &subw, const std::vector<ColDesc> &desc, bool adjoint, long ldx,
-               hipStream_t st, int prec)
+               hipStream_t st, int prec, hipEvent_t pencil_ready)
 {
     const int m = (int)desc.size();
     std::vector<UnitDesc> units;
@@ -389,25 +397,21 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     const bool split = prec == XTGEVC3_PREC_SPLIT_I8;
     long qld = 0, qsl = 0;
     if (split) {
-        // digits of the working pencil rows (exponent per row over S and P), once per sweep
-        qld = round_up(n, SPLIT_KSTEP) + SPLIT_KSTEP;
+        // pencil digits (ws.Sq, ws.Pq, ws.rowexp) come from the side stream (pencil_ready);
+        // the sweep records block exponents of the operands and of X
+        qld = split_qld(n);
         qsl = (long)n * qld;
-        ENSURE(ws.rowexp, sizeof(int) * (size_t)n);
-        ENSURE(ws.Sq, (size_t)SPLIT_DIGITS * qsl);
-        ENSURE(ws.Pq, (size_t)SPLIT_DIGITS * qsl);
-        ENSURE(ws.Wg, (size_t)SPLIT_DIGITS * m * WG_LD);
+        ENSURE(ws.Wg, (size_t)SPLIT_DIGITS * m * WG_K);
         ENSURE(ws.wgexp, sizeof(int) * (size_t)m);
         ENSURE(ws.wbexp, sizeof(int) * (size_t)nsteps * ldx);
-        hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.rowexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
-        hipLaunchKernelGGL(k_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, Sw, ldsw, Pw, ldpw, n, n,
-                           ws.rowexp.as<int>()); XTG_LAUNCH_CHECK("k_rowexp");
-        const dim3 grid((n + 63) / 64, (unsigned)((qld + 63) / 64));
-        hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, Sw, ldsw, (const double *)nullptr, 0L, n, n,
-                           (int)qld, 0, (int)qld, (const int *)ws.rowexp.as<int>(), ws.Sq.as<signed char>(), qsl,
-                           qld); XTG_LAUNCH_CHECK("k_slice_digits");
-        hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, Pw, ldpw, (const double *)nullptr, 0L, n, n,
-                           (int)qld, 0, (int)qld, (const int *)ws.rowexp.as<int>(), ws.Pq.as<signed char>(), qsl,
-                           qld); XTG_LAUNCH_CHECK("k_slice_digits");
+        ENSURE(ws.xbexp, sizeof(int) * (size_t)nsteps * ldx);
+        ENSURE(ws.sstart, sizeof(int) * (size_t)m);
+        ENSURE(ws.xexp, sizeof(int) * (size_t)m);
+        std::vector<int> sstart(m);
+        for (int p = 0; p < m; ++p) {
+            sstart[p] = desc[p].d < n ? rowstep[desc[p].d] : 0;
+        }
+        XTG_CHECK(hipMemcpyAsync(ws.sstart.p, sstart.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
     }
     if (prec == 1) {
         // mixed form: row exponents of the working pencil, per-column operand/solution bounds
@@ -507,8 +511,9 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
                                t_begin, t_end, u_begin, u_end, ws.sig.as<double>(), ws.colexp.as<int>(),
                                ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff, wr + woff,
                                ws.sig_grp.as<double>(), s == gs ? 1 : 0, grp, mx,
-                               split ? SplitOut{ws.wbexp.as<int>() + (long)s * ldx, ws.wgexp.as<int>()}
-                                     : SplitOut{nullptr, nullptr}); XTG_LAUNCH_CHECK("k_solve_real");
+                               split ? SplitOut{ws.wbexp.as<int>() + (long)s * ldx, ws.wgexp.as<int>(),
+                                                ws.xbexp.as<int>() + (long)s * ldx}
+                                     : SplitOut{nullptr, nullptr, nullptr}); XTG_LAUNCH_CHECK("k_solve_real");
             prof_end(pr, st, 0, 2.0 * nb * nb * N, 8.0 * (4.0 * nb * N + 2.0 * nb * nb));
             // near update: the group's rows above this block
             const int r0 = adjoint ? std::max(g0, std::min(desc[c_begin].lo, i0)) : g0;
@@ -535,23 +540,31 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         const int M = g0 - r0;
         if (M > 0 && N > 0 && split) {
             // digits of the group operand (exponent per column over both parts, from the
-            // group's last solve), then the product
-            const int wrp = round_up(wr, SPLIT_KSTEP);
+            // group's last solve), each part shifted by off so the K range starts at the K step
+            // boundary klo <= g0; then the product
+            const int klo = g0 / SPLIT_KSTEP * SPLIT_KSTEP, off = g0 - klo;
+            const int kp1 = round_up(off + wr, SPLIT_KSTEP);
+            if (pencil_ready != nullptr) {
+                XTG_CHECK(hipStreamWaitEvent(st, pencil_ready, 0));
+                pencil_ready = nullptr;
+            }
             int pr = prof_begin(st);
-            signed char *wg = ws.Wg.as<signed char>() + (long)c_begin * WG_LD;
-            hipLaunchKernelGGL(k_slice_digits, dim3((N + 63) / 64, (2 * wrp + 63) / 64), dim3(256), 0, st,
-                               (const double *)(W + c_begin), ldw, (const double *)(W + (long)wr * ldw + c_begin), ldw,
-                               N, wr, wrp, wr, 2 * wrp, (const int *)(ws.wgexp.as<int>() + c_begin), wg,
-                               (long)m * WG_LD, WG_LD); XTG_LAUNCH_CHECK("k_slice_digits");
-            prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * (2.0 * wr * N) + SPLIT_DIGITS * 2.0 * wrp * N);
+            signed char *wg = ws.Wg.as<signed char>() + (long)c_begin * SPLIT_KSTEP;
+            const long wslice = (long)m * WG_K, wkb = (long)m * SPLIT_KSTEP;
+            hipLaunchKernelGGL(k_slice_digits, dim3((N + 63) / 64, (2 * kp1 + 63) / 64), dim3(256), 0, st,
+                               (const double *)(W + c_begin), ldw, off, wr, (const double *)(W + (long)wr * ldw + c_begin),
+                               ldw, kp1 + off, wr, N, 2 * kp1, (const int *)(ws.wgexp.as<int>() + c_begin), wg, wslice,
+                               wkb); XTG_LAUNCH_CHECK("k_slice_digits");
+            prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * (2.0 * wr * N) + SPLIT_DIGITS * 2.0 * kp1 * N);
             pr = prof_begin(st);
-            XTG_CHECK((hipError_t)launch_gemm_i8(
-                st, M, N, wrp, wrp, ws.Sq.as<signed char>() + (long)r0 * qld + g0,
-                ws.Pq.as<signed char>() + (long)r0 * qld + g0, qsl, qld, wg, (long)m * WG_LD, WG_LD,
-                ws.rowexp.as<int>() + r0, ws.wgexp.as<int>() + c_begin, X + (long)r0 * ldx + c_begin, ldx, nullptr,
-                1, ws.sig_grp.as<double>() + c_begin, nullptr));
+            const long kbs = (long)n * SPLIT_KSTEP, ko = (long)(klo / SPLIT_KSTEP) * kbs + (long)r0 * SPLIT_KSTEP;
+            XTG_CHECK((hipError_t)launch_gemm_i8(st, M, N, kp1, kp1, ws.Sq.as<signed char>() + ko,
+                                                 ws.Pq.as<signed char>() + ko, qsl, kbs, wg, wslice, wkb,
+                                                 ws.rowexp.as<int>() + r0, ws.wgexp.as<int>() + c_begin,
+                                                 X + (long)r0 * ldx + c_begin, ldx, nullptr, 1,
+                                                 ws.sig_grp.as<double>() + c_begin, nullptr));
             prof_end(pr, st, XTGEVC3_PROF_UPDATE_SPLIT, 2.0 * M * N * (2.0 * wr),
-                     8.0 * (2.0 * M * N) + SPLIT_DIGITS * (2.0 * wrp * M + 2.0 * wrp * N));
+                     8.0 * (2.0 * M * N) + SPLIT_DIGITS * (2.0 * kp1 * M + 2.0 * kp1 * N));
         }
         else if (M > 0 && N > 0) {
             int pr = prof_begin(st);
@@ -566,12 +579,59 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         }
     }
     XTG_CHECK(hipGetLastError());
+    if (pencil_ready != nullptr) {
+        XTG_CHECK(hipStreamWaitEvent(st, pencil_ready, 0)); // the side stream's digits are in use until here
+    }
     // apply the per-column scale history to the solved blocks
     dim3 grid((m + 255) / 256, n);
     hipLaunchKernelGGL(k_fixup_scale, grid, dim3(256), 0, st, X, ldx, n, m, ws.rowstep.as<int>(), ws.blkexp.as<int>(),
                        ldx, ws.colexp.as<int>(), ws.flag.as<int>()); XTG_LAUNCH_CHECK("k_fixup_scale");
+    if (split) {
+        // exponents of the final columns of X from the per-step block exponents
+        hipLaunchKernelGGL(k_xexp_final, dim3((m + 255) / 256), dim3(256), 0, st, (const int *)ws.xbexp.as<int>(),
+                           (const int *)ws.blkexp.as<int>(), ldx, (const int *)ws.colexp.as<int>(),
+                           (const int *)ws.sstart.as<int>(), nsteps, m, ws.xexp.as<int>()); XTG_LAUNCH_CHECK("k_xexp_final");
+    }
     XTG_CHECK(hipGetLastError());
     return 0;
+}
+
+// Creates the split form's side stream and events once per device workspace.
+static hipError_t split_side_init(RealWork &ws)
+{
+    if (ws.side == nullptr) {
+        hipError_t e = hipStreamCreateWithFlags(&ws.side, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            ws.side = nullptr;
+            return e;
+        }
+        for (hipEvent_t *ev : {&ws.ev_start, &ws.ev_pencil, &ws.ev_z}) {
+            e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+            if (e != hipSuccess) {
+                return e;
+            }
+        }
+    }
+    return hipSuccess;
+}
+
+// Digits of the rows of A (and B, with the exponent of each row taken over both): E[r] from the
+// exact row maxima, then QA (QB) = k-blocked digits over [0, kp) (k-block pitch 32 rows).
+static hipError_t split_rows(hipStream_t st, int rows, int cols, int kp, const double *A, long lda, const double *B,
+                             long ldb, int *E, signed char *QA, signed char *QB)
+{
+    hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, E, (long)rows, -1000);
+    hipLaunchKernelGGL(k_rowexp, dim3((rows + 255) / 256, (cols + 63) / 64), dim3(256), 0, st, A, lda, B, ldb, rows,
+                       cols, E);
+    const dim3 grid((rows + 63) / 64, (kp + 63) / 64);
+    const long qs = (long)rows * kp, kbp = (long)rows * SPLIT_KSTEP;
+    hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, A, lda, 0, cols, (const double *)nullptr, 0L, 0, 0, rows,
+                       kp, (const int *)E, QA, qs, kbp);
+    if (B != nullptr) {
+        hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, B, ldb, 0, cols, (const double *)nullptr, 0L, 0, 0,
+                           rows, kp, (const int *)E, QB, qs, kbp);
+    }
+    return hipGetLastError();
 }
 
 static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<Unit> &units, int m, hipStream_t st)
@@ -637,7 +697,39 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
     XTG_CHECK(hipMemcpyAsync(ws.vw.p, vw.data(), sizeof(int) * vw.size(), hipMemcpyHostToDevice, st));
     ENSURE(ws.X, sizeof(double) * (size_t)n * ldx);
     hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, ws.X.as<double>(), (long)n * ldx); XTG_LAUNCH_CHECK("k_zero");
-    int rc = sweep_real(ws, n, A.Sw, A.ldsw, A.Pw, A.ldpw, *A.subw, desc, false, ldx, st, A.prec);
+    const bool split = A.prec == XTGEVC3_PREC_SPLIT_I8;
+    if (split) {
+        // digits of the working pencil (and of Z for the back-transform) on the side stream,
+        // after everything the caller's stream holds so far
+        const long qld = split_qld(n), qsl = (long)n * qld;
+        const int kb = round_up(n, SPLIT_KSTEP);
+        ENSURE(ws.rowexp, sizeof(int) * (size_t)n);
+        ENSURE(ws.Sq, (size_t)SPLIT_DIGITS * qsl);
+        ENSURE(ws.Pq, (size_t)SPLIT_DIGITS * qsl);
+        if (A.back) {
+            ENSURE(ws.zexp, sizeof(int) * (size_t)n);
+            ENSURE(ws.Zq, (size_t)SPLIT_DIGITS * n * kb);
+        }
+        XTG_CHECK(split_side_init(ws));
+        XTG_CHECK(hipEventRecord(ws.ev_start, st));
+        XTG_CHECK(hipStreamWaitEvent(ws.side, ws.ev_start, 0));
+        int pr = prof_begin(ws.side);
+        XTG_CHECK(split_rows(ws.side, n, n, (int)qld, A.Sw, A.ldsw, A.Pw, A.ldpw, ws.rowexp.as<int>(),
+                             ws.Sq.as<signed char>(), ws.Pq.as<signed char>()));
+        prof_end(pr, ws.side, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * 4.0 * n * (double)n + 2.0 * SPLIT_DIGITS * qsl);
+        XTG_CHECK(hipEventRecord(ws.ev_pencil, ws.side));
+        if (A.back) {
+            const double *Rp = A.left ? A.Z + (long)(n - 1) * A.ldz : A.Z;
+            const long ldr = A.left ? -A.ldz : A.ldz;
+            pr = prof_begin(ws.side);
+            XTG_CHECK(split_rows(ws.side, n, n, kb, Rp, ldr, nullptr, 0, ws.zexp.as<int>(), ws.Zq.as<signed char>(),
+                                 nullptr));
+            prof_end(pr, ws.side, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * 2.0 * n * (double)n + SPLIT_DIGITS * (double)n * kb);
+            XTG_CHECK(hipEventRecord(ws.ev_z, ws.side));
+        }
+    }
+    int rc = sweep_real(ws, n, A.Sw, A.ldsw, A.Pw, A.ldpw, *A.subw, desc, false, ldx, st, A.prec,
+                        split ? ws.ev_pencil : nullptr);
     if (rc) {
         return rc;
     }
@@ -664,30 +756,20 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
             hipLaunchKernelGGL(k_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, A.Z, A.ldz,
                                (const double *)nullptr, 0L, n, n, ws.zexp.as<int>()); XTG_LAUNCH_CHECK("k_rowexp");
         }
-        if (A.prec == XTGEVC3_PREC_SPLIT_I8) {
-            // digits of the columns of X and of the rows of Z (exponents from exact maxima)
+        if (split) {
+            // digits of the columns of X (exponents from the sweep); Z's come from the side stream
             const int kb = round_up(n, SPLIT_KSTEP);
-            ENSURE(ws.xexp, sizeof(int) * (size_t)m);
-            ENSURE(ws.zexp, sizeof(int) * (size_t)n);
             ENSURE(ws.Xq, (size_t)SPLIT_DIGITS * m * kb);
-            ENSURE(ws.Zq, (size_t)SPLIT_DIGITS * n * kb);
             int pr = prof_begin(st);
-            hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.xexp.as<int>(), (long)m, -1000); XTG_LAUNCH_CHECK("k_fill_int");
-            hipLaunchKernelGGL(k_rowexp, dim3((m + 255) / 256, (n + 63) / 64), dim3(256), 0, st, (const double *)X, ldx,
-                               (const double *)nullptr, 0L, m, n, ws.xexp.as<int>()); XTG_LAUNCH_CHECK("k_rowexp");
-            hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.zexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
-            hipLaunchKernelGGL(k_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, A.Z, A.ldz,
-                               (const double *)nullptr, 0L, n, n, ws.zexp.as<int>()); XTG_LAUNCH_CHECK("k_rowexp");
             hipLaunchKernelGGL(k_slice_digits, dim3((m + 63) / 64, (kb + 63) / 64), dim3(256), 0, st, (const double *)X,
-                               ldx, (const double *)nullptr, 0L, m, n, kb, 0, kb, (const int *)ws.xexp.as<int>(),
-                               ws.Xq.as<signed char>(), (long)m * kb, (long)kb); XTG_LAUNCH_CHECK("k_slice_digits");
-            hipLaunchKernelGGL(k_slice_digits, dim3((n + 63) / 64, (kb + 63) / 64), dim3(256), 0, st, Rp, ldr,
-                               (const double *)nullptr, 0L, n, n, kb, 0, kb, (const int *)ws.zexp.as<int>(),
-                               ws.Zq.as<signed char>(), (long)n * kb, (long)kb); XTG_LAUNCH_CHECK("k_slice_digits");
-            prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * 2.0 * ((double)n * m + (double)n * n));
+                               ldx, 0, n, (const double *)nullptr, 0L, 0, 0, m, kb, (const int *)ws.xexp.as<int>(),
+                               ws.Xq.as<signed char>(), (long)m * kb, (long)m * SPLIT_KSTEP); XTG_LAUNCH_CHECK("k_slice_digits");
+            prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * (double)n * m + SPLIT_DIGITS * (double)m * kb);
+            XTG_CHECK(hipStreamWaitEvent(st, ws.ev_z, 0));
             pr = prof_begin(st);
             XTG_CHECK((hipError_t)launch_gemm_i8(st, m, n, kb, 0, ws.Xq.as<signed char>(), ws.Xq.as<signed char>(),
-                                                 (long)m * kb, kb, ws.Zq.as<signed char>(), (long)n * kb, kb,
+                                                 (long)m * kb, (long)m * SPLIT_KSTEP, ws.Zq.as<signed char>(),
+                                                 (long)n * kb, (long)n * SPLIT_KSTEP,
                                                  ws.xexp.as<int>(), ws.zexp.as<int>(), A.V, A.ldv,
                                                  ws.outcol.as<int>(), 0, nullptr, ws.kmax.as<int>()));
             prof_end(pr, st, XTGEVC3_PROF_BACK_SPLIT, 2.0 * n * ksum,
@@ -773,7 +855,12 @@ int real_device(char side, char howmny, const int *select, int n, const double *
         A.prec = prec;
         A.V = VR;
         A.ldv = ldvr;
-        if (back) {
+        if (back && prec == XTGEVC3_PREC_SPLIT_I8) {
+            // the split form reads Z only through its digits, made before the output is written
+            A.Z = VR;
+            A.ldz = ldvr;
+        }
+        else if (back) {
             // the input VR holds Z; keep a copy since the output overwrites it
             ENSURE(ws.Zt, sizeof(double) * (size_t)n * n);
             XTG_CHECK(hipMemcpy2DAsync(ws.Zt.p, sizeof(double) * n, VR, sizeof(double) * ldvr, sizeof(double) * n, n,
@@ -814,7 +901,11 @@ int real_device(char side, char howmny, const int *select, int n, const double *
         A.prec = prec;
         A.V = VL;
         A.ldv = ldvl;
-        if (back) {
+        if (back && prec == XTGEVC3_PREC_SPLIT_I8) {
+            A.Z = VL;
+            A.ldz = ldvl;
+        }
+        else if (back) {
             ENSURE(ws.Zt, sizeof(double) * (size_t)n * n);
             XTG_CHECK(hipMemcpy2DAsync(ws.Zt.p, sizeof(double) * n, VL, sizeof(double) * ldvl, sizeof(double) * n, n,
                                        hipMemcpyDeviceToDevice, st));
@@ -923,7 +1014,7 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
     int pr = prof_begin(st);
     hipLaunchKernelGGL(k_bwd_rhs, dim3(M), dim3(256), 0, st, V, ldv, G, ldg, n, m, nbatch, ws.kcol.as<int>(), X, ldx); XTG_LAUNCH_CHECK("k_bwd_rhs");
     prof_end(pr, st, 3, 0.0, 8.0 * 3.0 * n * M);
-    int rc = sweep_real(ws, n, ws.Sw.as<double>(), n, ws.Pw.as<double>(), n, subw, desc, true, ldx, st, 0);
+    int rc = sweep_real(ws, n, ws.Sw.as<double>(), n, ws.Pw.as<double>(), n, subw, desc, true, ldx, st, 0, nullptr);
     if (rc) {
         return rc;
     }

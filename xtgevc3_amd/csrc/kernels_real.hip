@@ -550,7 +550,7 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
         // split-int8 form: exponent of each column's block operand (exact maximum over the
         // block's S and P parts) and, at the group's last step, of its group operand
         const SplitOut &sq = o.sq;
-        double m0 = 0.0, m1 = 0.0;
+        double m0 = 0.0, m1 = 0.0, x0 = 0.0, x1 = 0.0;
 #pragma clang loop unroll(full)
         for (int i = 0; i < QR; ++i) {
             if (QL * i + j < nb) {
@@ -558,18 +558,24 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
                 const V bx = A::bmul(bR, bI, v);
                 m0 = fmax(m0, fmax(fabs(a * A::re(v)), fabs(A::re(bx))));
                 m1 = fmax(m1, fmax(fabs(a * A::im(v)), fabs(A::im(bx))));
+                x0 = fmax(x0, fabs(A::re(v)));
+                x1 = fmax(x1, fabs(A::im(v)));
             }
         }
 #pragma clang loop unroll(full)
         for (int off = 1; off < QL; off <<= 1) {
             m0 = fmax(m0, __shfl_xor(m0, off));
             m1 = fmax(m1, __shfl_xor(m1, off));
+            x0 = fmax(x0, __shfl_xor(x0, off));
+            x1 = fmax(x1, __shfl_xor(x1, off));
         }
         if (act && j < ucols) {
             const int p = col + j;
             const double mj = j == 0 ? m0 : m1;
+            const double xj = j == 0 ? x0 : x1;
             const int ej = mj > 0.0 ? min(max(ilogb(mj) + 1, -1000), 1000) : -1000;
             sq.bexp[p] = ej;
+            sq.xbexp[p] = xj > 0.0 ? min(max(ilogb(xj) + 1, -1000), 1000) : -1000;
             if (grp.last) {
                 // the lower blocks the column reaches, each brought to the final scale by
                 // 2^(ce - blkexp) above

@@ -23,13 +23,17 @@ namespace xtg {
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x16 __attribute__((ext_vector_type(16)));
 
-// Q[s * qs + a * ldq + k] = digit s of A(a, k) * 2^-(E[a] + 1), A(a, k) = A1[a + k * lda1] for
-// k < K1, A2[a + (k - K1p) * lda2] for K1p <= k < K1p + K2, zero elsewhere in [0, Kp).
+// Digit layout ("k-blocked"): digit s of element (a, k) at Q[s * qs + (k / 32) * kbp + a * 32 +
+// k % 32], kbp = 32 * (rows of the array): a K step of 32 over consecutive rows is one
+// contiguous run of 32-byte rows.
+//
+// Q = digits of A(a, k) * 2^-(E[a] + 1) for k in [0, Kp), A(a, k) = A1[a + (k - o1) * lda1] for
+// o1 <= k < o1 + K1, A2[a + (k - o2) * lda2] for o2 <= k < o2 + K2, zero elsewhere.
 // Block: 64 rows a x 64 k, thread t: row t / 4, k chunk 16 (t % 4).
-__global__ __launch_bounds__(256) void k_slice_digits(const double *__restrict__ A1, long lda1,
-                                                      const double *__restrict__ A2, long lda2, int M, int K1,
-                                                      int K1p, int K2, int Kp, const int *__restrict__ E,
-                                                      signed char *__restrict__ Q, long qs, long ldq)
+__global__ __launch_bounds__(256) void k_slice_digits(const double *__restrict__ A1, long lda1, int o1, int K1,
+                                                      const double *__restrict__ A2, long lda2, int o2, int K2, int M,
+                                                      int Kp, const int *__restrict__ E, signed char *__restrict__ Q,
+                                                      long qs, long kbp)
 {
     const int a = blockIdx.x * 64 + threadIdx.x / 4;
     const int k0 = blockIdx.y * 64 + (threadIdx.x % 4) * 16;
@@ -42,11 +46,11 @@ __global__ __launch_bounds__(256) void k_slice_digits(const double *__restrict__
     for (int q = 0; q < 16; ++q) {
         const int k = k0 + q;
         double v = 0.0;
-        if (k < K1) {
-            v = A1[a + (long)k * lda1];
+        if (k >= o1 && k < o1 + K1) {
+            v = A1[a + (long)(k - o1) * lda1];
         }
-        else if (k >= K1p && k < K1p + K2) {
-            v = A2[a + (long)(k - K1p) * lda2];
+        else if (k >= o2 && k < o2 + K2) {
+            v = A2[a + (long)(k - o2) * lda2];
         }
         int d[SPLIT_DIGITS];
         split_digits(ldexp(v, -e), d);
@@ -55,6 +59,7 @@ __global__ __launch_bounds__(256) void k_slice_digits(const double *__restrict__
             dg[s][q] = d[s];
         }
     }
+    signed char *q0 = Q + (long)(k0 / 32) * kbp + (long)a * 32 + (k0 % 32);
 #pragma clang loop unroll(full)
     for (int s = 0; s < SPLIT_DIGITS; ++s) {
         i32x4 w;
@@ -63,15 +68,38 @@ __global__ __launch_bounds__(256) void k_slice_digits(const double *__restrict__
             w[c] = (dg[s][4 * c] & 0xff) | ((dg[s][4 * c + 1] & 0xff) << 8) | ((dg[s][4 * c + 2] & 0xff) << 16) |
                    ((dg[s][4 * c + 3] & 0xff) << 24);
         }
-        *reinterpret_cast<i32x4 *>(Q + s * qs + (long)a * ldq + k0) = w;
+        *reinterpret_cast<i32x4 *>(q0 + s * qs) = w;
     }
+}
+
+// out[p] = E of the final column p of X: the maximum over the steps s >= sstart[p] (the
+// column's first block) of xbexp[s][p] + colexp[p] - blkexp[s][p] (the block's exponent at
+// the column's final scale); -1000 for a zero column.
+__global__ void k_xexp_final(const int *__restrict__ xbexp, const int *__restrict__ blkexp, long ldb,
+                             const int *__restrict__ colexp, const int *__restrict__ sstart, int nsteps, int m,
+                             int *__restrict__ out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m) {
+        return;
+    }
+    int e = -1000;
+    const int ce = colexp[p];
+    for (int s = sstart[p]; s < nsteps; ++s) {
+        const int eb = xbexp[(long)s * ldb + p];
+        if (eb > -1000) {
+            e = max(e, eb + ce - blkexp[(long)s * ldb + p]);
+        }
+    }
+    out[p] = min(max(e, -1000), 1000);
 }
 
 // ---------------------------------------------------------------------------------------
 // k_gemm_i8: C[out(a)][b] = sig[b] * C + 2^(EL[a] + ER[b] + 2) sum_t 255^-(t+2) acc_t(a, b)
-// (mode 0: no sig * C term; sig == nullptr: sig = 1).  L digits: rows a, k-contiguous; k < K1 from L1, k in
-// [K1, K1 + K2) from L2 at k - K1; R digits: rows b, k-contiguous over [0, K1 + K2).  K1, K2
-// are multiples of 32; kmax (optional) bounds k + kofs per row a (rounded up to 32 per tile).
+// (mode 0: no sig * C term; sig == nullptr: sig = 1).  L digits (k-blocked, k-block pitch kbl,
+// digit pitch lslice): k < K1 from L1, k in [K1, K1 + K2) from L2 at k - K1; R digits (pitches
+// kbr, rslice) over [0, K1 + K2).  K1, K2 are multiples of 32; kmax (optional) bounds k + kofs
+// per row a (rounded up to 32 per tile).
 // Block tile BM x BN, waves WMW x WNW, each wave (BM/WMW) x (BN/WNW) in 32x32 MFMA blocks.
 // K step 32.  A stage holds the step's digits of the tile, per digit BM (then BN) rows of
 // 32 bytes; NSTAGE stages rotate, filled by direct global-to-LDS loads (16 bytes per lane,
@@ -98,7 +126,7 @@ template <int N> __device__ __forceinline__ void wait_vm()
 template <int BM, int BN, int WMW, int WNW>
 __global__ __launch_bounds__(WMW *WNW * 64) void k_gemm_i8(
     int M, int N, int K1, int K2, const signed char *__restrict__ L1, const signed char *__restrict__ L2, long lslice,
-    long ldl, const signed char *__restrict__ R, long rslice, long ldr, const int *__restrict__ EL,
+    long kbl, const signed char *__restrict__ R, long rslice, long kbr, const int *__restrict__ EL,
     const int *__restrict__ ER, double *__restrict__ C, long ldc, const int *__restrict__ outcol, int mode,
     const double *__restrict__ sig, const int *__restrict__ kmax, int kofs)
 {
@@ -140,29 +168,34 @@ __global__ __launch_bounds__(WMW *WNW * 64) void k_gemm_i8(
         K = min(K, max(0, (kk - kofs + QK - 1) / QK * QK));
     }
     // per-lane source offsets of its chunks (fixed over the steps); chunks past CH reload
-    // chunk CA (a valid R row) into the stage's padding
+    // chunk CA (a valid R row) into the stage's padding.  LDS position 2 r + h' of a row pair
+    // holds half h = h' ^ ((r >> 3) & 1) of row r, so the 16 lanes of a b128 fragment read
+    // (rows r..r+15, one half) cover all 64 banks.
     long off[IPW];
 #pragma clang loop unroll(full)
     for (int u = 0; u < IPW; ++u) {
         const int c = (u * NW + wave) * 64 + lane;
-        if (c < CA) {
-            const int sd = c / (BM * 2), rr = c % (BM * 2);
-            off[u] = sd * lslice + (long)min(a0 + rr / 2, M - 1) * ldl + (rr % 2) * 16;
+        const int cs = c < CH ? c : CA;
+        const int cc = cs < CA ? cs : cs - CA;
+        const int rows2 = cs < CA ? BM * 2 : BN * 2;
+        const int sd = cc / rows2, rr = (cc % rows2) / 2;
+        const int h = (cc % 2) ^ ((rr >> 3) & 1);
+        if (cs < CA) {
+            off[u] = sd * lslice + (long)min(a0 + rr, M - 1) * 32 + h * 16;
         }
         else {
-            const int cc = c < CH ? c - CA : 0;
-            const int sd = cc / (BN * 2), rr = cc % (BN * 2);
-            off[u] = sd * rslice + (long)min(b0 + rr / 2, N - 1) * ldr + (rr % 2) * 16;
+            off[u] = sd * rslice + (long)min(b0 + rr, N - 1) * 32 + h * 16;
         }
     }
     auto issue = [&](int st) {
         const int k0 = st * QK;
-        const signed char *lbase = k0 < K1 ? L1 + k0 : L2 + (k0 - K1);
+        const signed char *lbase = k0 < K1 ? L1 + (long)(k0 / QK) * kbl : L2 + (long)((k0 - K1) / QK) * kbl;
+        const signed char *rbase = R + (long)(k0 / QK) * kbr;
         signed char *stage = lds + (st % NSTAGE) * STAGE;
 #pragma clang loop unroll(full)
         for (int u = 0; u < IPW; ++u) {
             const int cbase = (u * NW + wave) * 64; // wave-uniform; an instruction loads one operand
-            const signed char *g = (cbase < CA ? lbase : R + k0) + off[u];
+            const signed char *g = (cbase < CA ? lbase : rbase) + off[u];
             __builtin_amdgcn_global_load_lds((const void *)g,
                                              (__attribute__((address_space(3))) void *)(stage + cbase * 16), 16, 0, 0);
         }
@@ -189,7 +222,8 @@ __global__ __launch_bounds__(WMW *WNW * 64) void k_gemm_i8(
     if (nst > 1) {
         issue(1);
     }
-    const int fr = lane % 32, fh = (lane / 32) * 16;
+    // fragment rows start at multiples of 32, so a row's swizzle bit is that of fr
+    const int fr = lane % 32, fsw = (((lane / 32) ^ ((fr >> 3) & 1))) * 16;
     for (int st = 0; st < nst; ++st) {
         if (st + 1 < nst) {
             wait_vm<IPW>();
@@ -212,7 +246,7 @@ __global__ __launch_bounds__(WMW *WNW * 64) void k_gemm_i8(
         for (int sd = 0; sd < SPLIT_DIGITS; ++sd) {
 #pragma clang loop unroll(full)
             for (int j = 0; j < AN; ++j) {
-                bf[sd][j] = *reinterpret_cast<const i32x4 *>(sb + sd * (BN * 32) + (wb + 32 * j + fr) * 32 + fh);
+                bf[sd][j] = *reinterpret_cast<const i32x4 *>(sb + sd * (BN * 32) + (wb + 32 * j + fr) * 32 + fsw);
             }
         }
 #pragma clang loop unroll(full)
@@ -220,7 +254,7 @@ __global__ __launch_bounds__(WMW *WNW * 64) void k_gemm_i8(
             i32x4 af[AM];
 #pragma clang loop unroll(full)
             for (int i = 0; i < AM; ++i) {
-                af[i] = *reinterpret_cast<const i32x4 *>(sa + sd * (BM * 32) + (wa + 32 * i + fr) * 32 + fh);
+                af[i] = *reinterpret_cast<const i32x4 *>(sa + sd * (BM * 32) + (wa + 32 * i + fr) * 32 + fsw);
             }
 #pragma clang loop unroll(full)
             for (int r = 0; r + sd < LV && r < SPLIT_DIGITS; ++r) {
@@ -277,18 +311,18 @@ XTG_I8_INST(64, 128, 1, 4)
 
 // One launch; the tile is 64 x 128 when M is small.
 static int launch_i8(hipStream_t st, int M, int N, int K1, int K2, const signed char *L1, const signed char *L2,
-                     long lslice, long ldl, const signed char *R, long rslice, long ldr, const int *EL, const int *ER,
+                     long lslice, long kbl, const signed char *R, long rslice, long kbr, const int *EL, const int *ER,
                      double *C, long ldc, const int *outcol, int mode, const double *sig, const int *kmax, int kofs)
 {
     if (M <= 64) {
         const dim3 grid(((N + 127) / 128) * ((M + 63) / 64));
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_i8<64, 128, 1, 4>), grid, dim3(256), 0, st, M, N, K1, K2, L1, L2,
-                           lslice, ldl, R, rslice, ldr, EL, ER, C, ldc, outcol, mode, sig, kmax, kofs);
+                           lslice, kbl, R, rslice, kbr, EL, ER, C, ldc, outcol, mode, sig, kmax, kofs);
     }
     else {
         const dim3 grid(((N + 127) / 128) * ((M + 127) / 128));
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_i8<128, 128, 2, 4>), grid, dim3(512), 0, st, M, N, K1, K2, L1, L2,
-                           lslice, ldl, R, rslice, ldr, EL, ER, C, ldc, outcol, mode, sig, kmax, kofs);
+                           lslice, kbl, R, rslice, kbr, EL, ER, C, ldc, outcol, mode, sig, kmax, kofs);
     }
     return (int)hipGetLastError();
 }
@@ -296,7 +330,7 @@ static int launch_i8(hipStream_t st, int M, int N, int K1, int K2, const signed 
 // Launches C = sig C + L R^T on the digits; returns 0 or a HIP error code.  A product longer
 // than SPLIT_KMAX (single segment only) runs in K chunks accumulated into C.
 int launch_gemm_i8(hipStream_t st, int M, int N, int K1, int K2, const signed char *L1, const signed char *L2,
-                   long lslice, long ldl, const signed char *R, long rslice, long ldr, const int *EL, const int *ER,
+                   long lslice, long kbl, const signed char *R, long rslice, long kbr, const int *EL, const int *ER,
                    double *C, long ldc, const int *outcol, int mode, const double *sig, const int *kmax)
 {
     if (M <= 0 || N <= 0) {
@@ -311,7 +345,8 @@ int launch_gemm_i8(hipStream_t st, int M, int N, int K1, int K2, const signed ch
         }
         for (int k0 = 0; k0 < K1; k0 += SPLIT_KMAX) {
             const int kc = std::min(SPLIT_KMAX, K1 - k0);
-            const int rc = launch_i8(st, M, N, kc, 0, L1 + k0, L1 + k0, lslice, ldl, R + k0, rslice, ldr, EL, ER, C,
+            const long lo = (long)(k0 / QK) * kbl, ro = (long)(k0 / QK) * kbr;
+            const int rc = launch_i8(st, M, N, kc, 0, L1 + lo, L1 + lo, lslice, kbl, R + ro, rslice, kbr, EL, ER, C,
                                      ldc, outcol, k0 == 0 ? mode : 1, k0 == 0 ? sig : nullptr, kmax, k0);
             if (rc != 0) {
                 return rc;
@@ -319,7 +354,7 @@ int launch_gemm_i8(hipStream_t st, int M, int N, int K1, int K2, const signed ch
         }
         return 0;
     }
-    return launch_i8(st, M, N, K1, K2, L1, L2, lslice, ldl, R, rslice, ldr, EL, ER, C, ldc, outcol, mode, sig, kmax, 0);
+    return launch_i8(st, M, N, K1, K2, L1, L2, lslice, kbl, R, rslice, kbr, EL, ER, C, ldc, outcol, mode, sig, kmax, 0);
 }
 
 } // namespace xtg
