@@ -24,15 +24,18 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+PEAK_INT8_TOPS = 5000.0
+SPLIT_PRODUCTS = 15  # digit products per multiply-add in the split-int8 form (levels i + j <= 4 of 5 digits)
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload,
 # each scaled by the counter's reading on a plain 1 GiB device copy (FETCH x2.0, WRITE x1.0);
 # summary and commands in profiles/dtgevc3_R_B_n8192_hbm_traffic.md
 MEASURED_TRAFFIC = {
-    ("xtg::k_gemm_f64<0, true, 128, 16>", 8192): 686.2e6,
-    ("xtg::k_gemm_f64<1, true, 128, 16>", 8192): 12608.2e6,
-    ("xtg::k_gemm_f64<0, true, 64, 32>", 8192): 29.1e6,
-    ("xtg::k_solve_real<65>", 8192): 18.2e6,
+    ("xtg::k_gemm_f64<1, true, 128, 128, 16, false>", 8192): 12327.9e6,
+    ("xtg::k_gemm_f64<0, true, 128, 64, 16, false>", 8192): 823.3e6,
+    ("xtg::k_gemm_f64<0, true, 128, 128, 16, false>", 8192): 403.1e6,
+    ("xtg::k_gemm_f64<0, true, 64, 64, 32, false>", 8192): 25.0e6,
+    ("xtg::k_solve_real<65>", 8192): 27.5e6,
 }
 
 
@@ -333,6 +336,16 @@ def main():
         for k, v in out["kernels"].items():
             log(f"[bench]   {label[:10]:10s} {k:14s} launches={v['launches_per_step']}  {v['ms_per_step']:.3f} ms  "
                 f"{v['tflops']} TFLOP/s")
+        if "back_split" in pf and pf["back_split"]["launches"]:
+            # the split form's back-transform kernel against the int8 MFMA peak: each FP64-equivalent
+            # multiply-add costs SPLIT_PRODUCTS int8 digit products
+            v = pf["back_split"]
+            ach = v["flops"] * SPLIT_PRODUCTS / (v["ms"] * 1e9)
+            out["roofline"] = {"bound": "mfma", "achieved": round(ach, 1), "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
+                               "frac": round(ach / PEAK_INT8_TOPS, 4), "traffic": None,
+                               "kernel": "xtg::k_gemm_i8<128, 128, 2, 4> (back-transform)",
+                               "avg_launch_ms": round(v["ms"] / v["launches"], 4),
+                               "fp64_equivalent_tflops": round(v["flops"] / (v["ms"] * 1e9), 2)}
         if not args.no_residual:
             step()
             torch.cuda.synchronize()
