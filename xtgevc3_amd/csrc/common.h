@@ -73,6 +73,23 @@ struct ZColDesc {
 
 constexpr int WAVE = 64;
 
+// Tile of workgroup l in a 1-D GEMM launch over tm_n x tn_n tiles.  Workgroup l runs on
+// XCD l % 8, so each XCD takes a contiguous range of positions p; positions run over groups
+// of 8 column tiles, within a group the row tiles from the last down (largest per-row K bound
+// first), the group's columns fastest.  An XCD so works through one group of column tiles over
+// all row tiles (even work for a triangular K bound) and its concurrent tiles share L2 panels.
+__device__ __forceinline__ void xcd_tile(int l, int tm_n, int tn_n, int &tm, int &tn)
+{
+    constexpr int NX = 8;
+    const int nblk = tm_n * tn_n;
+    const int x = l % NX, slot = l / NX;
+    const int q = nblk / NX, r = nblk % NX;
+    const int p = x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+    const int grp = p / (8 * tm_n), gw = min(8, tn_n - grp * 8), rem = p - grp * 8 * tm_n;
+    tm = tm_n - 1 - rem / gw;
+    tn = grp * 8 + rem % gw;
+}
+
 // Split-int8 form: digits per value and digit-product levels kept (i + j < SPLIT_LEVELS).
 constexpr int SPLIT_DIGITS = 5;
 constexpr int SPLIT_LEVELS = 5;

@@ -219,7 +219,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
             // near update: the group's rows above this block
             const int M = i0 - g0;
             if (M > 0) {
-                dim3 grid((N + ZTB_H - 1) / ZTB_H, (M + ZTA_H - 1) / ZTA_H);
+                dim3 grid(((N + ZTB_H - 1) / ZTB_H) * ((M + ZTA_H - 1) / ZTA_H));
                 pr = prof_begin(st);
                 hipLaunchKernelGGL(kz_gemm<0>, grid, dim3(256), 0, st, M, N, nb, nb, Sw + g0 + (long)i0 * ldsw, ldsw,
                                    Pw + g0 + (long)i0 * ldpw, ldpw, (const double2 *)(W + (long)woff * ldw + c_begin),
@@ -234,7 +234,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
         }
         // far update: rows above the group, all of its blocks at once
         const int N = m - c_begin;
-        dim3 grid((N + ZTB_H - 1) / ZTB_H, (g0 + ZTA_H - 1) / ZTA_H);
+        dim3 grid(((N + ZTB_H - 1) / ZTB_H) * ((g0 + ZTA_H - 1) / ZTA_H));
         int pr = prof_begin(st);
         hipLaunchKernelGGL(kz_gemm<0>, grid, dim3(256), 0, st, g0, N, wr, wr, Sw + (long)g0 * ldsw, ldsw,
                            Pw + (long)g0 * ldpw, ldpw, (const double2 *)(W + c_begin), ldw, X + c_begin, ldx,
@@ -251,7 +251,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
     if (back) {
         const double2 *Rp = left ? Z + (long)(n - 1) * ldz : Z;
         long ldr = left ? -ldz : ldz;
-        dim3 grid((n + ZTB_H - 1) / ZTB_H, (m + ZTA_H - 1) / ZTA_H);
+        dim3 grid(((n + ZTB_H - 1) / ZTB_H) * ((m + ZTA_H - 1) / ZTA_H));
         double ksum = 0.0;
         int kmx = 0;
         for (int q = 0; q < m; ++q) {

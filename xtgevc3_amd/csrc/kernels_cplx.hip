@@ -295,7 +295,9 @@ __global__ __launch_bounds__(256) void kz_gemm(int M, int N, int K1, int K2, con
 {
     __shared__ double Lr[2][ZGK][ZTA], Li[2][ZGK][ZTA];
     __shared__ double Rr[2][ZGK][ZTB], Ri[2][ZGK][ZTB];
-    const int a0 = blockIdx.y * ZTA, b0 = blockIdx.x * ZTB;
+    int tm, tn;
+    xcd_tile(blockIdx.x, (M + ZTA - 1) / ZTA, (N + ZTB - 1) / ZTB, tm, tn);
+    const int a0 = tm * ZTA, b0 = tn * ZTB;
     const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
     const int wa = (wave >> 1) * 64, wb = (wave & 1) * 32;
     int K = K1 + K2;

@@ -800,10 +800,11 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
     __shared__ el_t Ls[2][GK][GTA];
     __shared__ el_t Rs[2][GK][GTB];
 
-    // with a per-row K bound (triangular operand) the heaviest tiles are dispatched first
-    const int ya = kmax != nullptr ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;
-    const int a0 = ya * GTA;
-    const int b0 = blockIdx.x * GTB;
+    // XCD-aware tile order; with a per-row K bound the heaviest row tiles come first
+    int tm, tn;
+    xcd_tile(blockIdx.x, (M + GTA - 1) / GTA, (N + GTB - 1) / GTB, tm, tn);
+    const int a0 = tm * GTA;
+    const int b0 = tn * GTB;
     const int tid = threadIdx.x;
     const int wave = tid / WAVE;
     const int lane = tid % WAVE;
@@ -1436,7 +1437,7 @@ int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const
     const int cfg = gemm_config(M, N, K1 + K2);
     const int gta = cfg == GEMM_64x64 || cfg == GEMM_64x128 ? 64 : 128;
     const int gtb = cfg == GEMM_64x64 || cfg == GEMM_128x64 ? 64 : 128;
-    dim3 grid((N + gtb - 1) / gtb, (M + gta - 1) / gta);
+    const dim3 grid(((N + gtb - 1) / gtb) * ((M + gta - 1) / gta));
 #define XTG_GEMM_LAUNCH(T, WD, GA, GB, K)                                                                            \
     if (prec == 1 && T != 2) {                                                                                       \
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, WD, GA, GB, K, true>), grid, dim3(256), 0, st, M, N, K1, K2, \

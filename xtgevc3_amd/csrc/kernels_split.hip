@@ -114,7 +114,6 @@ __global__ void k_xexp_final(const int *__restrict__ xbexp, const int *__restric
 constexpr int QK = 32;
 constexpr int NSTAGE = 3;
 constexpr int SPLIT_KMAX = 26624; // 5 * 127^2 * K < 2^31
-constexpr int NXCD = 8;
 
 // s_waitcnt with vmcnt = N (0..63), other counters at their maximum (gfx9 encoding)
 template <int N> __device__ __forceinline__ void wait_vm()
@@ -142,19 +141,9 @@ __global__ __launch_bounds__(WMW *WNW * 64) void k_gemm_i8(
     constexpr int LV = SPLIT_LEVELS;
     __shared__ __attribute__((aligned(16))) signed char lds[NSTAGE * STAGE];
 
-    // tile of this workgroup: position p in the order (groups of 8 column tiles; within a
-    // group the row tiles from the last (largest kmax) down; the group's columns fastest),
-    // each XCD taking a contiguous range of p
-    const int tm_n = (M + BM - 1) / BM, tn_n = (N + BN - 1) / BN;
-    const int nblk = tm_n * tn_n;
-    int p;
-    {
-        const int l = blockIdx.x, x = l % NXCD, slot = l / NXCD;
-        const int q = nblk / NXCD, r = nblk % NXCD;
-        p = x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
-    }
-    const int grp = p / (8 * tm_n), gw = min(8, tn_n - grp * 8), rem = p - grp * 8 * tm_n;
-    const int a0 = (tm_n - 1 - rem / gw) * BM, b0 = (grp * 8 + rem % gw) * BN;
+    int tm, tn;
+    xcd_tile(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
+    const int a0 = tm * BM, b0 = tn * BN;
 
     const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
     const int wa = (wave / WNW) * WTM, wb = (wave % WNW) * WTN;
