@@ -86,6 +86,16 @@ int xtgevc3_ztgevc3_device(char side, char howmny, const int *select, int n, con
                            const xtgevc3_zcomplex *beta, xtgevc3_zcomplex *VL, int ldvl, xtgevc3_zcomplex *VR,
                            int ldvr, int mm, int *m, int *info, void *stream);
 
+/* ztgevc3 on device with a precision selector (no counterpart in the reference):
+ * XTGEVC3_PREC_FP64 is xtgevc3_ztgevc3_device; XTGEVC3_PREC_SPLIT_I8 runs the far updates and
+ * the back-transform as two real int8 digit products each (Re = [Lr, -Li].[Rr, Ri],
+ * Im = [Li, Lr].[Rr, Ri] over (re, im)-interleaved K), as in xtgevc3_dtgevc3_device_ex.
+ * Other selectors return XTGEVC3_INFO_UNSUPPORTED. */
+int xtgevc3_ztgevc3_device_ex(char side, char howmny, const int *select, int n, const xtgevc3_zcomplex *S, int lds,
+                              const xtgevc3_zcomplex *P, int ldp, const xtgevc3_zcomplex *alpha,
+                              const xtgevc3_zcomplex *beta, xtgevc3_zcomplex *VL, int ldvl, xtgevc3_zcomplex *VR,
+                              int ldvr, int mm, int *m, int *info, void *stream, int precision);
+
 /* Backward (adjoint) of the selected right eigenvectors without back-transform (no
  * counterpart in the reference): for the normalised output V (n x m) of
  * xtgevc3_dtgevc3_device(side='R', howmny = select ? 'S' : 'A') and nbatch cotangent batches

@@ -167,3 +167,76 @@ def test_split_i8_large_residual(side):
     res = O.residual_A(Z @ S @ Z.T, Z @ P @ Z.T, gc, al, bb, left=side == "L")
     print(f"\n[split-i8 n={n} {side} N(0,1)] max residual {res.max():.3e} = {res.max() / (n * EPS):.3g} n*eps")
     assert res.max() <= 10.0 * n * EPS
+
+
+def _run_cplx_split(S, P, al, be, Q, Z, side, select=None, howmny="B"):
+    """ztgevc3 in the split-int8 form through the device C ABI; returns (VL, VR, m, info)."""
+    import torch
+
+    import xtgevc3_amd as X
+
+    n = S.shape[0]
+    dS = X.fortran_device(np.asfortranarray(S.astype(np.complex128)))
+    dP = X.fortran_device(np.asfortranarray(P.astype(np.complex128)))
+    if howmny == "B":
+        dVL, dVR = X.fortran_device(Q), X.fortran_device(Z)
+    else:
+        dVL = X.fortran_device(np.zeros((n, n), dtype=np.complex128, order="F"))
+        dVR = X.fortran_device(np.zeros((n, n), dtype=np.complex128, order="F"))
+    kw = {}
+    if side in "RB":
+        kw["VR"] = dVR
+    if side in "LB":
+        kw["VL"] = dVL
+    m, info = X.ztgevc3_device(side, howmny, select, dS, dP, al, be, precision=X.PREC_SPLIT_I8, **kw)
+    torch.cuda.synchronize()
+    return X.from_fortran_device(dVL), X.from_fortran_device(dVR), m, info
+
+
+@pytest.mark.parametrize("side", ["R", "L", "B"])
+@pytest.mark.parametrize("n", [70, 300])
+def test_split_i8_complex_against_oracle(side, n):
+    """Complex products as two real digit products over (re, im)-interleaved K."""
+    from tests.helpers import run_cplx
+
+    S, P, al, be = G.reference_style_complex(n, 80 + n)
+    Q, Z = G.random_unitary(n, 81 + n), G.random_unitary(n, 82 + n)
+    gL, gR, m, info = _run_cplx_split(S, P, al, be, Q, Z, side)
+    oL, oR, om, oinfo = run_cplx("oracle", side, "B", None, S, P, al, be, Q, Z)
+    assert (m, info) == (om, oinfo) == (n, 0)
+    kind = np.zeros(n, dtype=np.int32)
+    idx = np.arange(n)
+    for name, g, o, left, M in (("R", gR, oR, False, Z), ("L", gL, oL, True, Q)):
+        if name not in side and side != "B":
+            continue
+        assert np.all(np.isfinite(g))
+        mx = np.max(np.abs(g.real) + np.abs(g.imag), axis=0)
+        assert np.allclose(mx, 1.0, rtol=0, atol=64 * EPS)
+        o = O.normalize(o.copy(order="F"))
+        err = np.max(np.abs(g - o), axis=0)
+        cond = O.eigvec_cond(S, P, al, be, kind, idx, left=left)
+        rel = (err / np.maximum(cond, 1.0)).max()
+        Qm, Zm = Q, Z
+        res = O.residual_A(Qm @ S @ Zm.conj().T, Qm @ P @ Zm.conj().T, g, al, be, left=left).max()
+        print(f"\n[split-i8 z n={n} {name}] max err/cond {rel:.3e}, max residual {res / (n * EPS):.3g} n*eps "
+              f"= {res:.3e}")
+        assert rel <= 1e-10
+        assert res <= 1e-11
+
+
+def test_split_i8_complex_large_residual():
+    """n = 2304 complex N(0,1) Schur pencil, both sides: residual within 10 n eps."""
+    n = 2304
+    S, P, al, be = G.complex_triangular(n, 43)
+    Z = G.random_unitary(n, 44)
+    gL, gR, m, info = _run_cplx_split(S, P, al, be, Z, Z, "B")
+    assert (m, info) == (n, 0)
+    A = Z @ S @ Z.conj().T
+    B = Z @ P @ Z.conj().T
+    for g, left in ((gR, False), (gL, True)):
+        assert np.all(np.isfinite(g))
+        mx = np.max(np.abs(g.real) + np.abs(g.imag), axis=0)
+        assert np.allclose(mx, 1.0, rtol=0, atol=64 * EPS)
+        res = O.residual_A(A, B, g, al, be, left=left)
+        print(f"\n[split-i8 z n={n} {'L' if left else 'R'}] max residual {res.max() / (n * EPS):.3g} n*eps")
+        assert res.max() <= 10.0 * n * EPS

@@ -44,7 +44,7 @@ PREC_FP64, PREC_MIXED_F32, PREC_MIXED_BF16X3, PREC_SPLIT_I8 = 0, 1, 2, 3
 EXPORTED_SYMBOLS = [
     "xtgevc3_abi_version", "xtgevc3_last_error",
     "xtgevc3_dtgevc3", "xtgevc3_stgevc3", "xtgevc3_ztgevc3", "xtgevc3_ctgevc3",
-    "xtgevc3_dtgevc3_device", "xtgevc3_dtgevc3_device_ex", "xtgevc3_ztgevc3_device",
+    "xtgevc3_dtgevc3_device", "xtgevc3_dtgevc3_device_ex", "xtgevc3_ztgevc3_device", "xtgevc3_ztgevc3_device_ex",
     "xtgevc3_profile_enable", "xtgevc3_profile_read", "xtgevc3_dtgevc3_backward_device",
 ]
 
@@ -184,8 +184,8 @@ def tgevc3_device(prefix, side, howmny, select, S, P, alpha, beta, alphai=None, 
     c = ctypes
     real = prefix in _REAL
     dt = _REAL[prefix] if real else _CPLX[prefix]
-    if precision != PREC_FP64 and prefix != "d":
-        raise ValueError("the mixed forms are available for dtgevc3 only")
+    if precision != PREC_FP64 and prefix not in ("d", "z"):
+        raise ValueError("the mixed forms are available for dtgevc3 and ztgevc3 only")
     fn = getattr(_lib, f"xtgevc3_{prefix}tgevc3_device" + ("_ex" if precision != PREC_FP64 else ""))
     n = S.shape[0]
     lds = S.shape[1]
@@ -223,8 +223,10 @@ def dtgevc3_device(side, howmny, select, S, P, alphar, alphai, beta, VL=None, VR
                          stream=stream, precision=precision)
 
 
-def ztgevc3_device(side, howmny, select, S, P, alpha, beta, VL=None, VR=None, mm=None, stream=None):
-    return tgevc3_device("z", side, howmny, select, S, P, alpha, beta, VL=VL, VR=VR, mm=mm, stream=stream)
+def ztgevc3_device(side, howmny, select, S, P, alpha, beta, VL=None, VR=None, mm=None, stream=None,
+                   precision=PREC_FP64):
+    return tgevc3_device("z", side, howmny, select, S, P, alpha, beta, VL=VL, VR=VR, mm=mm, stream=stream,
+                         precision=precision)
 
 
 def dtgevc3_backward_device(S, P, alphar, alphai, beta, select, V, G, nbatch=1, dS=None, dP=None, stream=None):

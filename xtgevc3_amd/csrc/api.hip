@@ -386,6 +386,22 @@ int xtgevc3_ztgevc3_device(char side, char howmny, const int *select, int n, con
                            const xtgevc3_zcomplex *beta, xtgevc3_zcomplex *VL, int ldvl, xtgevc3_zcomplex *VR,
                            int ldvr, int mm, int *m, int *info, void *stream)
 {
+    return xtgevc3_ztgevc3_device_ex(side, howmny, select, n, S, lds, P, ldp, alpha, beta, VL, ldvl, VR, ldvr, mm, m,
+                                     info, stream, XTGEVC3_PREC_FP64);
+}
+
+int xtgevc3_ztgevc3_device_ex(char side, char howmny, const int *select, int n, const xtgevc3_zcomplex *S, int lds,
+                              const xtgevc3_zcomplex *P, int ldp, const xtgevc3_zcomplex *alpha,
+                              const xtgevc3_zcomplex *beta, xtgevc3_zcomplex *VL, int ldvl, xtgevc3_zcomplex *VR,
+                              int ldvr, int mm, int *m, int *info, void *stream, int precision)
+{
+    if (precision != XTGEVC3_PREC_FP64 && precision != XTGEVC3_PREC_SPLIT_I8) {
+        xtg::set_error("precision " + std::to_string(precision) + " is not available for ztgevc3");
+        if (info) {
+            *info = XTGEVC3_INFO_UNSUPPORTED;
+        }
+        return XTGEVC3_INFO_UNSUPPORTED;
+    }
     std::vector<xtg::Unit> units;
     int num_sel = (howmny == 'A' || howmny == 'B' || howmny == 'S') && n > 0
                       ? xtg::complex_units(howmny, select, n, units)
@@ -394,7 +410,7 @@ int xtgevc3_ztgevc3_device(char side, char howmny, const int *select, int n, con
         return 0;
     }
     int rc = xtg::complex_device(side, howmny, select, n, S, lds, P, ldp, alpha, beta, VL, ldvl, VR, ldvr,
-                                 (hipStream_t)stream);
+                                 (hipStream_t)stream, precision);
     if (rc != 0) {
         return xtg::fail(info, "device path");
     }

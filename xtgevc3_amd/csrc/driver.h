@@ -25,7 +25,8 @@ bool check_args(char howmny, int n, int num_sel, int mm, int *m, T *work, int lw
 int real_units(char howmny, const int *select, int n, const double *ai, std::vector<Unit> &units);
 int complex_units(char howmny, const int *select, int n, std::vector<Unit> &units);
 
-// prec: 0 = FP64 throughout; 1 = FP32-product updates and back-transform (mixed form)
+// prec (XTGEVC3_PREC_*): FP64 throughout; FP32-product or split-int8 far updates and
+// back-transform (mixed forms)
 int real_device(char side, char howmny, const int *select, int n, const double *S, long lds, const double *P,
                 long ldp, const double *ar, const double *ai, const double *be, double *VL, long ldvl, double *VR,
                 long ldvr, hipStream_t st, int prec = 0);
@@ -33,7 +34,7 @@ int real_device(char side, char howmny, const int *select, int n, const double *
 int complex_device(char side, char howmny, const int *select, int n, const std::complex<double> *S, long lds,
                    const std::complex<double> *P, long ldp, const std::complex<double> *al,
                    const std::complex<double> *be, std::complex<double> *VL, long ldvl, std::complex<double> *VR,
-                   long ldvr, hipStream_t st);
+                   long ldvr, hipStream_t st, int prec = 0);
 
 int real_backward_device(int n, const double *S, long lds, const double *P, long ldp, const double *ar,
                          const double *ai, const double *be, const int *select, const double *V, long ldv,

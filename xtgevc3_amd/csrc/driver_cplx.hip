@@ -15,6 +15,7 @@
 #include <mutex>
 #include <vector>
 
+#include "../../include/xtgevc3.h"
 #include "common.h"
 #include "driver.h"
 
@@ -23,7 +24,15 @@ namespace xtg {
 template <int MAXB>
 __global__ void kz_solve_units(const double2 *, long, const double2 *, long, int, int, double2 *, long, double2 *, long,
                                const ZColDesc *, int, int, double *, int *, int *, int *, int, int, double *, int,
-                               GroupInfo);
+                               GroupInfo, SplitOut);
+__global__ void k_slice_digits(const double *, long, int, int, const double *, long, int, int, int, int, const int *,
+                               signed char *, long, long, int);
+__global__ void kz_rowexp(const double2 *, long, const double2 *, long, int, int, int *);
+__global__ void k_fill_int(int *, long, int);
+__global__ void k_xexp_final(const int *, const int *, long, const int *, const int *, int, int, int *);
+int launch_gemm_i8(hipStream_t st, int M, int N, int K1, int K2, const signed char *L1, const signed char *L2,
+                   long lslice, long kbl, const signed char *R, long rslice, long kbr, const int *EL, const int *ER,
+                   double *C, long ldc, int cst, const int *outcol, int mode, const double *sig, const int *kmax);
 template <int TAG>
 __global__ void kz_gemm(int, int, int, int, const double2 *, long, const double2 *, long, const double2 *, long,
                         double2 *, long, const int *, int, const double *, const int *, const double2 *);
@@ -40,6 +49,13 @@ constexpr int ZGROUP = 8;        // row blocks per far update (K = 2 * 256)
 constexpr int ZCOLS_PER_BLOCK = 16; // solve kernel: 16 lanes per column, 256 threads
 constexpr int ZTA_H = 128, ZTB_H = 64;
 static_assert(ZGROUP <= GROUP_MAX_BLOCKS, "group size");
+// split-int8 form: complex entries expand to (re, im) pairs along K; K step 32 of the
+// expanded index; K extent of the group operand (two segments of up to 2 * 256 expanded rows
+// shifted to the K step boundary: 544 each)
+constexpr int ZSPLIT_KSTEP = 32;
+constexpr long ZWG_K = 2 * 544;
+static_assert(2 * ZGROUP * NB_Z + ZSPLIT_KSTEP - 2 <= 544, "split operand extent");
+static inline int zround_up(int v, int q) { return (v + q - 1) / q * q; }
 
 int complex_units(char howmny, const int *select, int n, std::vector<Unit> &units)
 {
@@ -86,6 +102,9 @@ struct DBuf {
 
 struct ZWork {
     DBuf X, W, sig, sig_grp, colexp, blkexp, flag, desc, outcol, kmax, rowstep, Sw, Pw, Zt, norms;
+    // split-int8 form: pencil digits expanded as (re, -im) and (im, re), group operand, X and Z
+    // digits, exponents
+    DBuf Sx, Sy, Px, Py, rowexp, Wg, wgexp, wbexp, xbexp, sstart, xexp, Xx, Xy, Zq, zexp, kmax2;
 };
 
 std::mutex g_zmutex;
@@ -110,8 +129,9 @@ void shift_cplx(zt al, zt bv, double ascale, double bscale, zt &a, zt &b)
 
 int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 *Pw, long ldpw, const zt *al,
                   const zt *be, double ascale, double bscale, bool left, bool back, const double2 *Z, long ldz,
-                  double2 *V, long ldv, const std::vector<Unit> &units, int m, hipStream_t st)
+                  double2 *V, long ldv, const std::vector<Unit> &units, int m, hipStream_t st, int prec)
 {
+    const bool split = prec == XTGEVC3_PREC_SPLIT_I8;
     std::vector<ZColDesc> desc(m);
     std::vector<int> outcol(m), kmaxv(m);
     for (int p = 0; p < m; ++p) {
@@ -171,6 +191,46 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
     hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, ws.blkexp.as<int>(), (long)nsteps * ldx); XTG_LAUNCH_CHECK("k_zero_int");
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(256), 0, st, ws.colexp.as<int>(), (long)m); XTG_LAUNCH_CHECK("k_zero_int");
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, ws.flag.as<int>(), 4L); XTG_LAUNCH_CHECK("k_zero_int");
+    long qld = 0, qsl = 0, kbs = 0;
+    if (split) {
+        // digits of the pencil rows, complex entries expanded along K (exponent per row over
+        // |Re|, |Im| of S and P)
+        qld = zround_up(2 * n, ZSPLIT_KSTEP) + ZSPLIT_KSTEP;
+        qsl = (long)n * qld;
+        kbs = (long)n * ZSPLIT_KSTEP;
+        for (DBuf *b : {&ws.Sx, &ws.Sy, &ws.Px, &ws.Py}) {
+            ZENSURE(*b, (size_t)SPLIT_DIGITS * qsl);
+        }
+        ZENSURE(ws.rowexp, sizeof(int) * (size_t)n);
+        ZENSURE(ws.Wg, (size_t)SPLIT_DIGITS * m * ZWG_K);
+        ZENSURE(ws.wgexp, sizeof(int) * (size_t)m);
+        ZENSURE(ws.wbexp, sizeof(int) * (size_t)nsteps * ldx);
+        ZENSURE(ws.xbexp, sizeof(int) * (size_t)nsteps * ldx);
+        ZENSURE(ws.sstart, sizeof(int) * (size_t)m);
+        ZENSURE(ws.xexp, sizeof(int) * (size_t)m);
+        std::vector<int> sstart(m);
+        for (int p = 0; p < m; ++p) {
+            sstart[p] = rowstep[desc[p].d];
+        }
+        XTG_CHECK(hipMemcpyAsync(ws.sstart.p, sstart.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+        int pr = prof_begin(st);
+        hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.rowexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
+        hipLaunchKernelGGL(kz_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, Sw, ldsw, Pw, ldpw, n, n,
+                           ws.rowexp.as<int>()); XTG_LAUNCH_CHECK("kz_rowexp");
+        const dim3 grid((n + 63) / 64, (unsigned)((qld + 63) / 64));
+        const struct {
+            const double2 *A;
+            long lda;
+            DBuf *Q;
+            int cmode;
+        } parts[4] = {{Sw, ldsw, &ws.Sx, 1}, {Sw, ldsw, &ws.Sy, 2}, {Pw, ldpw, &ws.Px, 1}, {Pw, ldpw, &ws.Py, 2}};
+        for (const auto &q : parts) {
+            hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, (const double *)q.A, q.lda, 0, n,
+                               (const double *)nullptr, 0L, 0, 0, n, (int)qld, (const int *)ws.rowexp.as<int>(),
+                               q.Q->as<signed char>(), qsl, kbs, q.cmode); XTG_LAUNCH_CHECK("k_slice_digits");
+        }
+        prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 16.0 * 3.0 * n * (double)n + 4.0 * SPLIT_DIGITS * qsl);
+    }
 
     // Steps go bottom-up in groups of ZGROUP row blocks [g0, g1).  Within a group each step
     // updates only the group's rows above it ("near" update, K = 2 nb); after the group one
@@ -213,7 +273,10 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
                                dim3((N + ZCOLS_PER_BLOCK - 1) / ZCOLS_PER_BLOCK), dim3(256), 0, st, Sw, ldsw, Pw,
                                ldpw, i0, nb, X, ldx, W, ldw, ws.desc.as<ZColDesc>(), c_begin, m, ws.sig.as<double>(),
                                ws.colexp.as<int>(), ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff,
-                               wr + woff, ws.sig_grp.as<double>(), s == gs ? 1 : 0, grp); XTG_LAUNCH_CHECK("kz_solve_units");
+                               wr + woff, ws.sig_grp.as<double>(), s == gs ? 1 : 0, grp,
+                               split ? SplitOut{ws.wbexp.as<int>() + (long)s * ldx, ws.wgexp.as<int>(),
+                                                ws.xbexp.as<int>() + (long)s * ldx}
+                                     : SplitOut{nullptr, nullptr, nullptr}); XTG_LAUNCH_CHECK("kz_solve_units");
             prof_end(pr, st, 0, 8.0 * nb * nb * (double)N, 16.0 * (4.0 * nb * N + 2.0 * nb * nb));
             first = false;
             // near update: the group's rows above this block
@@ -234,6 +297,33 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
         }
         // far update: rows above the group, all of its blocks at once
         const int N = m - c_begin;
+        if (split) {
+            // digits of the group operand, (re, im) expanded and shifted by off so the expanded K
+            // range starts at the K step boundary klo <= 2 g0; Re and Im as two real products
+            const int klo = (2 * g0) / ZSPLIT_KSTEP * ZSPLIT_KSTEP, off = 2 * g0 - klo;
+            const int kp1 = zround_up(off + 2 * wr, ZSPLIT_KSTEP);
+            signed char *wg = ws.Wg.as<signed char>() + (long)c_begin * ZSPLIT_KSTEP;
+            const long wslice = (long)m * ZWG_K, wkb = (long)m * ZSPLIT_KSTEP;
+            int pr = prof_begin(st);
+            hipLaunchKernelGGL(k_slice_digits, dim3((N + 63) / 64, (2 * kp1 + 63) / 64), dim3(256), 0, st,
+                               (const double *)(W + c_begin), ldw, off / 2, wr,
+                               (const double *)(W + (long)wr * ldw + c_begin), ldw, (kp1 + off) / 2, wr, N, 2 * kp1,
+                               (const int *)(ws.wgexp.as<int>() + c_begin), wg, wslice, wkb, 3); XTG_LAUNCH_CHECK("k_slice_digits");
+            prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 16.0 * (2.0 * wr * N) + SPLIT_DIGITS * 2.0 * kp1 * N);
+            const long ko = (long)(klo / ZSPLIT_KSTEP) * kbs;
+            pr = prof_begin(st);
+            for (int part = 0; part < 2; ++part) {
+                const signed char *ls = (part == 0 ? ws.Sx : ws.Sy).as<signed char>() + ko;
+                const signed char *lp = (part == 0 ? ws.Px : ws.Py).as<signed char>() + ko;
+                XTG_CHECK((hipError_t)launch_gemm_i8(st, g0, N, kp1, kp1, ls, lp, qsl, kbs, wg, wslice, wkb,
+                                                     ws.rowexp.as<int>(), ws.wgexp.as<int>() + c_begin,
+                                                     (double *)(X + c_begin) + part, 2 * ldx, 2, nullptr, 1,
+                                                     ws.sig_grp.as<double>() + c_begin, nullptr));
+            }
+            prof_end(pr, st, XTGEVC3_PROF_UPDATE_SPLIT, 8.0 * g0 * N * (2.0 * wr),
+                     16.0 * (2.0 * g0 * N) + 2.0 * SPLIT_DIGITS * (2.0 * kp1 * g0 + 2.0 * kp1 * N));
+            continue;
+        }
         dim3 grid(((N + ZTB_H - 1) / ZTB_H) * ((g0 + ZTA_H - 1) / ZTA_H));
         int pr = prof_begin(st);
         hipLaunchKernelGGL(kz_gemm<0>, grid, dim3(256), 0, st, g0, N, wr, wr, Sw + (long)g0 * ldsw, ldsw,
@@ -248,7 +338,58 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
         hipLaunchKernelGGL(kz_fixup_scale, grid, dim3(256), 0, st, X, ldx, n, m, ws.rowstep.as<int>(),
                            ws.blkexp.as<int>(), ldx, ws.colexp.as<int>(), ws.flag.as<int>()); XTG_LAUNCH_CHECK("kz_fixup_scale");
     }
-    if (back) {
+    if (split) {
+        hipLaunchKernelGGL(k_xexp_final, dim3((m + 255) / 256), dim3(256), 0, st, (const int *)ws.xbexp.as<int>(),
+                           (const int *)ws.blkexp.as<int>(), ldx, (const int *)ws.colexp.as<int>(),
+                           (const int *)ws.sstart.as<int>(), nsteps, m, ws.xexp.as<int>()); XTG_LAUNCH_CHECK("k_xexp_final");
+    }
+    if (back && split) {
+        // digits of X's columns as (re, -im) and (im, re), of Z's rows as (re, im); V = Re + i Im
+        const double2 *Rp = left ? Z + (long)(n - 1) * ldz : Z;
+        const long ldr = left ? -ldz : ldz;
+        const int kb = zround_up(2 * n, ZSPLIT_KSTEP);
+        ZENSURE(ws.Xx, (size_t)SPLIT_DIGITS * m * kb);
+        ZENSURE(ws.Xy, (size_t)SPLIT_DIGITS * m * kb);
+        ZENSURE(ws.Zq, (size_t)SPLIT_DIGITS * n * kb);
+        ZENSURE(ws.zexp, sizeof(int) * (size_t)n);
+        ZENSURE(ws.kmax2, sizeof(int) * (size_t)m);
+        std::vector<int> kmax2(m);
+        double ksum = 0.0;
+        int kmx = 0;
+        for (int q = 0; q < m; ++q) {
+            kmax2[q] = 2 * kmaxv[q];
+            ksum += kmaxv[q];
+            kmx = std::max(kmx, kmaxv[q]);
+        }
+        XTG_CHECK(hipMemcpyAsync(ws.kmax2.p, kmax2.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+        int pr = prof_begin(st);
+        hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.zexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
+        hipLaunchKernelGGL(kz_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, Rp, ldr,
+                           (const double2 *)nullptr, 0L, n, n, ws.zexp.as<int>()); XTG_LAUNCH_CHECK("kz_rowexp");
+        const dim3 gx((m + 63) / 64, (kb + 63) / 64), gz((n + 63) / 64, (kb + 63) / 64);
+        hipLaunchKernelGGL(k_slice_digits, gx, dim3(256), 0, st, (const double *)X, ldx, 0, n, (const double *)nullptr,
+                           0L, 0, 0, m, kb, (const int *)ws.xexp.as<int>(), ws.Xx.as<signed char>(), (long)m * kb,
+                           (long)m * ZSPLIT_KSTEP, 1); XTG_LAUNCH_CHECK("k_slice_digits");
+        hipLaunchKernelGGL(k_slice_digits, gx, dim3(256), 0, st, (const double *)X, ldx, 0, n, (const double *)nullptr,
+                           0L, 0, 0, m, kb, (const int *)ws.xexp.as<int>(), ws.Xy.as<signed char>(), (long)m * kb,
+                           (long)m * ZSPLIT_KSTEP, 2); XTG_LAUNCH_CHECK("k_slice_digits");
+        hipLaunchKernelGGL(k_slice_digits, gz, dim3(256), 0, st, (const double *)Rp, ldr, 0, n, (const double *)nullptr,
+                           0L, 0, 0, n, kb, (const int *)ws.zexp.as<int>(), ws.Zq.as<signed char>(), (long)n * kb,
+                           (long)n * ZSPLIT_KSTEP, 3); XTG_LAUNCH_CHECK("k_slice_digits");
+        prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 16.0 * 2.0 * ((double)n * m + (double)n * n));
+        pr = prof_begin(st);
+        for (int part = 0; part < 2; ++part) {
+            XTG_CHECK((hipError_t)launch_gemm_i8(st, m, n, kb, 0, (part == 0 ? ws.Xx : ws.Xy).as<signed char>(),
+                                                 (part == 0 ? ws.Xx : ws.Xy).as<signed char>(), (long)m * kb,
+                                                 (long)m * ZSPLIT_KSTEP, ws.Zq.as<signed char>(), (long)n * kb,
+                                                 (long)n * ZSPLIT_KSTEP, ws.xexp.as<int>(), ws.zexp.as<int>(),
+                                                 (double *)V + part, 2 * ldv, 2, ws.outcol.as<int>(), 0, nullptr,
+                                                 ws.kmax2.as<int>()));
+        }
+        prof_end(pr, st, XTGEVC3_PROF_BACK_SPLIT, 8.0 * n * ksum,
+                 16.0 * (double)n * m + 2.0 * SPLIT_DIGITS * (2.0 * (double)n * kmx + 2.0 * ksum));
+    }
+    else if (back) {
         const double2 *Rp = left ? Z + (long)(n - 1) * ldz : Z;
         long ldr = left ? -ldz : ldz;
         dim3 grid(((n + ZTB_H - 1) / ZTB_H) * ((m + ZTA_H - 1) / ZTA_H));
@@ -279,7 +420,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
 int complex_device(char side, char howmny, const int *select, int n, const std::complex<double> *S, long lds,
                    const std::complex<double> *P, long ldp, const std::complex<double> *al,
                    const std::complex<double> *be, std::complex<double> *VL, long ldvl, std::complex<double> *VR,
-                   long ldvr, hipStream_t st)
+                   long ldvr, hipStream_t st, int prec)
 {
     int dev = 0;
     XTG_CHECK(hipGetDevice(&dev));
@@ -310,14 +451,20 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
     double bscale = 1.0 / std::max(bn, safemin);
     if (right && VR) {
         const double2 *Zp = nullptr;
-        if (back) {
+        long zld = n;
+        if (back && prec == XTGEVC3_PREC_SPLIT_I8) {
+            // the split form reads Z only through its digits, made before the output is written
+            Zp = reinterpret_cast<const double2 *>(VR);
+            zld = ldvr;
+        }
+        else if (back) {
             ZENSURE(ws.Zt, sizeof(double2) * (size_t)n * n);
             XTG_CHECK(hipMemcpy2DAsync(ws.Zt.p, sizeof(double2) * n, VR, sizeof(double2) * ldvr, sizeof(double2) * n,
                                        n, hipMemcpyDeviceToDevice, st));
             Zp = ws.Zt.as<double2>();
         }
-        int rc = run_cplx_side(ws, n, dS, lds, dP, ldp, al, be, ascale, bscale, false, back, Zp, n,
-                               reinterpret_cast<double2 *>(VR), ldvr, units, m, st);
+        int rc = run_cplx_side(ws, n, dS, lds, dP, ldp, al, be, ascale, bscale, false, back, Zp, zld,
+                               reinterpret_cast<double2 *>(VR), ldvr, units, m, st, prec);
         if (rc) {
             return rc;
         }
@@ -329,14 +476,20 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
         hipLaunchKernelGGL(kz_reverse_conjtranspose, grid, dim3(32, 8), 0, st, dS, lds, n, ws.Sw.as<double2>(), (long)n); XTG_LAUNCH_CHECK("kz_reverse_conjtranspose");
         hipLaunchKernelGGL(kz_reverse_conjtranspose, grid, dim3(32, 8), 0, st, dP, ldp, n, ws.Pw.as<double2>(), (long)n); XTG_LAUNCH_CHECK("kz_reverse_conjtranspose");
         const double2 *Zp = nullptr;
-        if (back) {
+        long zld = n;
+        if (back && prec == XTGEVC3_PREC_SPLIT_I8) {
+            // the split form reads Z only through its digits, made before the output is written
+            Zp = reinterpret_cast<const double2 *>(VL);
+            zld = ldvl;
+        }
+        else if (back) {
             ZENSURE(ws.Zt, sizeof(double2) * (size_t)n * n);
             XTG_CHECK(hipMemcpy2DAsync(ws.Zt.p, sizeof(double2) * n, VL, sizeof(double2) * ldvl, sizeof(double2) * n,
                                        n, hipMemcpyDeviceToDevice, st));
             Zp = ws.Zt.as<double2>();
         }
         int rc = run_cplx_side(ws, n, ws.Sw.as<double2>(), n, ws.Pw.as<double2>(), n, al, be, ascale, bscale, true,
-                               back, Zp, n, reinterpret_cast<double2 *>(VL), ldvl, units, m, st);
+                               back, Zp, zld, reinterpret_cast<double2 *>(VL), ldvl, units, m, st, prec);
         if (rc) {
             return rc;
         }

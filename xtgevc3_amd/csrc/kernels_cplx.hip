@@ -39,7 +39,9 @@ constexpr double kzSafeMin = 2.2250738585072014e-308;
 // Operands of the group (rows w_s + r: -a x, rows w_p + r: b x); a column starting in this
 // block (not the group's first step) zeros its operand rows of the group's lower blocks; the
 // group's last step brings the lower blocks' operand rows to the column's final scale.
-// X, W: complex (double2) row-major, X[row * ldx + col].
+// X, W: complex (double2) row-major, X[row * ldx + col].  sq (split-int8 form, bexp ==
+// nullptr: off): block exponents of the operand and of X per step, the group operand's at the
+// group's last step, as in the real solve kernel.
 // ---------------------------------------------------------------------------------------
 constexpr int ZQL = 16;
 constexpr int ZSP = 33; // LDS column pitch of the staged block (rows)
@@ -93,7 +95,8 @@ __global__ __launch_bounds__(256) void kz_solve_units(
     const double2 *__restrict__ S, long lds, const double2 *__restrict__ P, long ldp, int i0, int nb,
     double2 *__restrict__ X, long ldx, double2 *__restrict__ W, long ldw, const ZColDesc *__restrict__ desc,
     int c_begin, int m, double *__restrict__ sig, int *__restrict__ colexp, int *__restrict__ blkexp,
-    int *__restrict__ flag, int w_s, int w_p, double *__restrict__ sig_grp, int first_in_group, GroupInfo grp)
+    int *__restrict__ flag, int w_s, int w_p, double *__restrict__ sig_grp, int first_in_group, GroupInfo grp,
+    SplitOut sq)
 {
     static_assert(MAXB <= 32 && ZSP > MAXB, "block layout");
     constexpr int QR = (MAXB + ZQL - 1) / ZQL;
@@ -222,6 +225,25 @@ __global__ __launch_bounds__(256) void kz_solve_units(
         }
     }
 
+    // split-int8 form: exponents of the block's operand and solution (exact maxima of |Re|,
+    // |Im| over the column's 16 lanes)
+    double mw = 0.0, mxv = 0.0;
+    if (sq.bexp != nullptr) {
+#pragma clang loop unroll(full)
+        for (int i = 0; i < QR; ++i) {
+            if (ZQL * i + j < nb) {
+                const zc x = vr[i];
+                const zc ax = zmul(a, x), bx = zmul(b, x);
+                mxv = fmax(mxv, fmax(fabs(x.r), fabs(x.i)));
+                mw = fmax(mw, fmax(fmax(fabs(ax.r), fabs(ax.i)), fmax(fabs(bx.r), fabs(bx.i))));
+            }
+        }
+#pragma clang loop unroll(full)
+        for (int off = 1; off < ZQL; off <<= 1) {
+            mw = fmax(mw, __shfl_xor(mw, off));
+            mxv = fmax(mxv, __shfl_xor(mxv, off));
+        }
+    }
     if (!act) {
         return;
     }
@@ -261,6 +283,26 @@ __global__ __launch_bounds__(256) void kz_solve_units(
             }
         }
     }
+    if (j == 0 && sq.bexp != nullptr) {
+        const int ew = mw > 0.0 ? min(max(ilogb(mw) + 1, -1000), 1000) : -1000;
+        sq.bexp[p] = ew;
+        sq.xbexp[p] = mxv > 0.0 ? min(max(ilogb(mxv) + 1, -1000), 1000) : -1000;
+        if (grp.last) {
+            // group operand exponent: this block and the lower blocks the column reaches, each
+            // at the final scale (lower blocks were brought there by 2^(ce - blkexp) above)
+            int eg = ew;
+            for (int jb = 1; jb < GROUP_MAX_BLOCKS; ++jb) {
+                if (grp.off[jb] < grp.off[jb + 1] && i0 + grp.off[jb] <= cd.d) {
+                    const long so = (long)(grp.step[jb] - grp.step[0]) * ldx + p;
+                    const int eb = sq.bexp[so];
+                    if (eb > -1000) {
+                        eg = max(eg, eb + ce - blkexp[so]);
+                    }
+                }
+            }
+            sq.gexp[p] = min(max(eg, -1000), 1000);
+        }
+    }
     if (j == 0) {
         const double sg = ldexp(1.0, my_exp);
         sig[p] = sg;
@@ -275,7 +317,7 @@ __global__ __launch_bounds__(256) void kz_solve_units(
 
 template __global__ void kz_solve_units<32>(const double2 *, long, const double2 *, long, int, int, double2 *, long,
                                             double2 *, long, const ZColDesc *, int, int, double *, int *, int *,
-                                            int *, int, int, double *, int, GroupInfo);
+                                            int *, int, int, double *, int, GroupInfo, SplitOut);
 
 // ---------------------------------------------------------------------------------------
 // kz_gemm: D[a][b] = sum_k L[k][a] * R[k][b], complex, FP64 MFMA; rows k >= K1 of L and R

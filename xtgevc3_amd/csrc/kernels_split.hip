@@ -9,7 +9,8 @@
 // remainders bound the error by about 2^-39 * K * max|L(a,:)| * max|R(b,:)|; |acc_t| stays
 // below 5 * 127^2 * K, exact in int32 for K <= SPLIT_KMAX (longer products run in chunks).
 //
-//   k_slice_digits -- digits of a matrix, rows a with their exponents, k-contiguous
+//   k_slice_digits -- digits of a matrix (real, or complex expanded along k), k-blocked
+//   kz_rowexp      -- row exponents of complex matrices
 //   k_gemm_i8      -- C = sig * C + sum_k L(a,k) R(b,k) on v_mfma_i32_32x32x32_i8, digits
 //                     staged through LDS, FP64 epilogue
 #include <hip/hip_runtime.h>
@@ -29,11 +30,16 @@ typedef int i32x16 __attribute__((ext_vector_type(16)));
 //
 // Q = digits of A(a, k) * 2^-(E[a] + 1) for k in [0, Kp), A(a, k) = A1[a + (k - o1) * lda1] for
 // o1 <= k < o1 + K1, A2[a + (k - o2) * lda2] for o2 <= k < o2 + K2, zero elsewhere.
+// Complex operands (cmode != 0; A1, A2 hold double2 and o1, K1, o2, K2 count complex
+// entries) are expanded along k: position k holds part k % 2 of complex entry k / 2 as
+//     cmode 1: (re, -im),   cmode 2: (im, re),   cmode 3: (re, im),
+// so that a complex product becomes two real ones, Re = [Lr, -Li] . [Rr, Ri] and
+// Im = [Li, Lr] . [Rr, Ri].
 // Block: 64 rows a x 64 k, thread t: row t / 4, k chunk 16 (t % 4).
 __global__ __launch_bounds__(256) void k_slice_digits(const double *__restrict__ A1, long lda1, int o1, int K1,
                                                       const double *__restrict__ A2, long lda2, int o2, int K2, int M,
                                                       int Kp, const int *__restrict__ E, signed char *__restrict__ Q,
-                                                      long qs, long kbp)
+                                                      long qs, long kbp, int cmode)
 {
     const int a = blockIdx.x * 64 + threadIdx.x / 4;
     const int k0 = blockIdx.y * 64 + (threadIdx.x % 4) * 16;
@@ -44,13 +50,26 @@ __global__ __launch_bounds__(256) void k_slice_digits(const double *__restrict__
     int dg[SPLIT_DIGITS][16];
 #pragma clang loop unroll(full)
     for (int q = 0; q < 16; ++q) {
-        const int k = k0 + q;
+        const int kk = k0 + q;
         double v = 0.0;
-        if (k >= o1 && k < o1 + K1) {
-            v = A1[a + (long)(k - o1) * lda1];
+        if (cmode == 0) {
+            if (kk >= o1 && kk < o1 + K1) {
+                v = A1[a + (long)(kk - o1) * lda1];
+            }
+            else if (kk >= o2 && kk < o2 + K2) {
+                v = A2[a + (long)(kk - o2) * lda2];
+            }
         }
-        else if (k >= o2 && k < o2 + K2) {
-            v = A2[a + (long)(k - o2) * lda2];
+        else {
+            const int k = kk >> 1, part = kk & 1;
+            double2 z = make_double2(0.0, 0.0);
+            if (k >= o1 && k < o1 + K1) {
+                z = reinterpret_cast<const double2 *>(A1)[a + (long)(k - o1) * lda1];
+            }
+            else if (k >= o2 && k < o2 + K2) {
+                z = reinterpret_cast<const double2 *>(A2)[a + (long)(k - o2) * lda2];
+            }
+            v = cmode == 1 ? (part ? -z.y : z.x) : (cmode == 2 ? (part ? z.x : z.y) : (part ? z.y : z.x));
         }
         int d[SPLIT_DIGITS];
         split_digits(ldexp(v, -e), d);
@@ -69,6 +88,31 @@ __global__ __launch_bounds__(256) void k_slice_digits(const double *__restrict__
                    ((dg[s][4 * c + 3] & 0xff) << 24);
         }
         *reinterpret_cast<i32x4 *>(q0 + s * qs) = w;
+    }
+}
+
+// out[r] = max(out[r], ilogb(max |Re|, |Im| over the chunk) + 1) for the complex rows r of A
+// and B (B may be null), A(r, k) = A[r + k * lda]; chunks of 64 columns per blockIdx.y.
+__global__ __launch_bounds__(256) void kz_rowexp(const double2 *__restrict__ A, long lda, const double2 *__restrict__ B,
+                                                 long ldb, int n, int ncols, int *__restrict__ out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k0 = blockIdx.y * 64;
+    if (r >= n) {
+        return;
+    }
+    const int k1 = min(ncols, k0 + 64);
+    double mx = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        const double2 z = A[r + (long)k * lda];
+        mx = fmax(mx, fmax(fabs(z.x), fabs(z.y)));
+        if (B != nullptr) {
+            const double2 w = B[r + (long)k * ldb];
+            mx = fmax(mx, fmax(fabs(w.x), fabs(w.y)));
+        }
+    }
+    if (mx > 0.0) {
+        atomicMax(out + r, min(max(ilogb(mx) + 1, -1000), 1000));
     }
 }
 
@@ -95,7 +139,7 @@ __global__ void k_xexp_final(const int *__restrict__ xbexp, const int *__restric
 }
 
 // ---------------------------------------------------------------------------------------
-// k_gemm_i8: C[out(a)][b] = sig[b] * C + 2^(EL[a] + ER[b] + 2) sum_t 255^-(t+2) acc_t(a, b)
+// k_gemm_i8: C[out(a) * ldc + b * cst] = sig[b] * C + 2^(EL[a] + ER[b] + 2) sum_t 255^-(t+2) acc_t(a, b)
 // (mode 0: no sig * C term; sig == nullptr: sig = 1).  L digits (k-blocked, k-block pitch kbl,
 // digit pitch lslice): k < K1 from L1, k in [K1, K1 + K2) from L2 at k - K1; R digits (pitches
 // kbr, rslice) over [0, K1 + K2).  K1, K2 are multiples of 32; kmax (optional) bounds k + kofs
@@ -126,7 +170,7 @@ template <int BM, int BN, int WMW, int WNW>
 __global__ __launch_bounds__(WMW *WNW * 64) void k_gemm_i8(
     int M, int N, int K1, int K2, const signed char *__restrict__ L1, const signed char *__restrict__ L2, long lslice,
     long kbl, const signed char *__restrict__ R, long rslice, long kbr, const int *__restrict__ EL,
-    const int *__restrict__ ER, double *__restrict__ C, long ldc, const int *__restrict__ outcol, int mode,
+    const int *__restrict__ ER, double *__restrict__ C, long ldc, int cst, const int *__restrict__ outcol, int mode,
     const double *__restrict__ sig, const int *__restrict__ kmax, int kofs)
 {
     constexpr int NW = WMW * WNW;
@@ -278,7 +322,7 @@ __global__ __launch_bounds__(WMW *WNW * 64) void k_gemm_i8(
                         sum = fma(sum, 1.0 / SPLIT_BASE, (double)acc[t][i][j][v]);
                     }
                     double d = ldexp(sum * (1.0 / (SPLIT_BASE * SPLIT_BASE)), ea + ER[b] + 2);
-                    double *cp = C + rowoff + b;
+                    double *cp = C + rowoff + (long)b * cst;
                     if (mode == 1) {
                         d = fma(sig != nullptr ? sig[b] : 1.0, *cp, d);
                     }
@@ -292,7 +336,7 @@ __global__ __launch_bounds__(WMW *WNW * 64) void k_gemm_i8(
 #define XTG_I8_INST(BM, BN, WMW, WNW)                                                                               \
     template __global__ void k_gemm_i8<BM, BN, WMW, WNW>(int, int, int, int, const signed char *,                   \
                                                          const signed char *, long, long, const signed char *, long, \
-                                                         long, const int *, const int *, double *, long,             \
+                                                         long, const int *, const int *, double *, long, int,        \
                                                          const int *, int, const double *, const int *, int);
 XTG_I8_INST(128, 128, 2, 4)
 XTG_I8_INST(64, 128, 1, 4)
@@ -301,17 +345,18 @@ XTG_I8_INST(64, 128, 1, 4)
 // One launch; the tile is 64 x 128 when M is small.
 static int launch_i8(hipStream_t st, int M, int N, int K1, int K2, const signed char *L1, const signed char *L2,
                      long lslice, long kbl, const signed char *R, long rslice, long kbr, const int *EL, const int *ER,
-                     double *C, long ldc, const int *outcol, int mode, const double *sig, const int *kmax, int kofs)
+                     double *C, long ldc, int cst, const int *outcol, int mode, const double *sig, const int *kmax,
+                     int kofs)
 {
     if (M <= 64) {
         const dim3 grid(((N + 127) / 128) * ((M + 63) / 64));
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_i8<64, 128, 1, 4>), grid, dim3(256), 0, st, M, N, K1, K2, L1, L2,
-                           lslice, kbl, R, rslice, kbr, EL, ER, C, ldc, outcol, mode, sig, kmax, kofs);
+                           lslice, kbl, R, rslice, kbr, EL, ER, C, ldc, cst, outcol, mode, sig, kmax, kofs);
     }
     else {
         const dim3 grid(((N + 127) / 128) * ((M + 127) / 128));
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_i8<128, 128, 2, 4>), grid, dim3(512), 0, st, M, N, K1, K2, L1, L2,
-                           lslice, kbl, R, rslice, kbr, EL, ER, C, ldc, outcol, mode, sig, kmax, kofs);
+                           lslice, kbl, R, rslice, kbr, EL, ER, C, ldc, cst, outcol, mode, sig, kmax, kofs);
     }
     return (int)hipGetLastError();
 }
@@ -320,7 +365,7 @@ static int launch_i8(hipStream_t st, int M, int N, int K1, int K2, const signed 
 // than SPLIT_KMAX (single segment only) runs in K chunks accumulated into C.
 int launch_gemm_i8(hipStream_t st, int M, int N, int K1, int K2, const signed char *L1, const signed char *L2,
                    long lslice, long kbl, const signed char *R, long rslice, long kbr, const int *EL, const int *ER,
-                   double *C, long ldc, const int *outcol, int mode, const double *sig, const int *kmax)
+                   double *C, long ldc, int cst, const int *outcol, int mode, const double *sig, const int *kmax)
 {
     if (M <= 0 || N <= 0) {
         return 0;
@@ -336,14 +381,15 @@ int launch_gemm_i8(hipStream_t st, int M, int N, int K1, int K2, const signed ch
             const int kc = std::min(SPLIT_KMAX, K1 - k0);
             const long lo = (long)(k0 / QK) * kbl, ro = (long)(k0 / QK) * kbr;
             const int rc = launch_i8(st, M, N, kc, 0, L1 + lo, L1 + lo, lslice, kbl, R + ro, rslice, kbr, EL, ER, C,
-                                     ldc, outcol, k0 == 0 ? mode : 1, k0 == 0 ? sig : nullptr, kmax, k0);
+                                     ldc, cst, outcol, k0 == 0 ? mode : 1, k0 == 0 ? sig : nullptr, kmax, k0);
             if (rc != 0) {
                 return rc;
             }
         }
         return 0;
     }
-    return launch_i8(st, M, N, K1, K2, L1, L2, lslice, kbl, R, rslice, kbr, EL, ER, C, ldc, outcol, mode, sig, kmax, 0);
+    return launch_i8(st, M, N, K1, K2, L1, L2, lslice, kbl, R, rslice, kbr, EL, ER, C, ldc, cst, outcol, mode, sig, kmax,
+                     0);
 }
 
 } // namespace xtg
