@@ -26,14 +26,14 @@ __global__ void kz_solve_units(const double2 *, long, const double2 *, long, int
                                const ZColDesc *, int, int, double *, int *, int *, int *, int, int, double *, int,
                                GroupInfo, SplitOut);
 __global__ void k_slice_digits(const double *, long, int, int, const double *, long, int, int, int, int, const int *,
-                               signed char *, long, long, int);
+                               signed char *, long, long, int, signed char *);
 __global__ void kz_rowexp(const double2 *, long, const double2 *, long, int, int, int *);
 __global__ void k_fill_int(int *, long, int);
 __global__ void k_xexp_final(const int *, const int *, long, const int *, const int *, int, int, int *);
 int launch_gemm_i8(hipStream_t st, int M, int N, int K1, int K2, const signed char *L1, const signed char *L2,
                    long lslice, long kbl, const signed char *R, long rslice, long kbr, const int *EL, const int *ER,
                    double *C, long ldc, int cst, const int *outcol, int mode, const double *sig, const int *kmax);
-template <int TAG>
+template <int TAG, int TA>
 __global__ void kz_gemm(int, int, int, int, const double2 *, long, const double2 *, long, const double2 *, long,
                         double2 *, long, const int *, int, const double *, const int *, const double2 *);
 __global__ void kz_fixup_scale(double2 *, long, int, int, const int *, const int *, long, const int *, const int *);
@@ -218,17 +218,12 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
         hipLaunchKernelGGL(kz_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, Sw, ldsw, Pw, ldpw, n, n,
                            ws.rowexp.as<int>()); XTG_LAUNCH_CHECK("kz_rowexp");
         const dim3 grid((n + 63) / 64, (unsigned)((qld + 63) / 64));
-        const struct {
-            const double2 *A;
-            long lda;
-            DBuf *Q;
-            int cmode;
-        } parts[4] = {{Sw, ldsw, &ws.Sx, 1}, {Sw, ldsw, &ws.Sy, 2}, {Pw, ldpw, &ws.Px, 1}, {Pw, ldpw, &ws.Py, 2}};
-        for (const auto &q : parts) {
-            hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, (const double *)q.A, q.lda, 0, n,
-                               (const double *)nullptr, 0L, 0, 0, n, (int)qld, (const int *)ws.rowexp.as<int>(),
-                               q.Q->as<signed char>(), qsl, kbs, q.cmode); XTG_LAUNCH_CHECK("k_slice_digits");
-        }
+        hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, (const double *)Sw, ldsw, 0, n,
+                           (const double *)nullptr, 0L, 0, 0, n, (int)qld, (const int *)ws.rowexp.as<int>(),
+                           ws.Sx.as<signed char>(), qsl, kbs, 1, ws.Sy.as<signed char>()); XTG_LAUNCH_CHECK("k_slice_digits");
+        hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, (const double *)Pw, ldpw, 0, n,
+                           (const double *)nullptr, 0L, 0, 0, n, (int)qld, (const int *)ws.rowexp.as<int>(),
+                           ws.Px.as<signed char>(), qsl, kbs, 1, ws.Py.as<signed char>()); XTG_LAUNCH_CHECK("k_slice_digits");
         prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 16.0 * 3.0 * n * (double)n + 4.0 * SPLIT_DIGITS * qsl);
     }
 
@@ -282,9 +277,13 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
             // near update: the group's rows above this block
             const int M = i0 - g0;
             if (M > 0) {
-                dim3 grid(((N + ZTB_H - 1) / ZTB_H) * ((M + ZTA_H - 1) / ZTA_H));
+                // few rows: 32-row tiles keep more CUs busy than 128-row tiles
+                const bool small = ((N + ZTB_H - 1) / ZTB_H) * ((M + ZTA_H - 1) / ZTA_H) < 256;
+                const int ta = small ? 32 : ZTA_H;
+                dim3 grid(((N + ZTB_H - 1) / ZTB_H) * ((M + ta - 1) / ta));
                 pr = prof_begin(st);
-                hipLaunchKernelGGL(kz_gemm<0>, grid, dim3(256), 0, st, M, N, nb, nb, Sw + g0 + (long)i0 * ldsw, ldsw,
+                auto kern = small ? kz_gemm<0, 32> : kz_gemm<0, ZTA_H>;
+                hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, M, N, nb, nb, Sw + g0 + (long)i0 * ldsw, ldsw,
                                    Pw + g0 + (long)i0 * ldpw, ldpw, (const double2 *)(W + (long)woff * ldw + c_begin),
                                    ldw, X + (long)g0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
                                    (const double *)(ws.sig.as<double>() + c_begin), (const int *)nullptr,
@@ -308,7 +307,8 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
             hipLaunchKernelGGL(k_slice_digits, dim3((N + 63) / 64, (2 * kp1 + 63) / 64), dim3(256), 0, st,
                                (const double *)(W + c_begin), ldw, off / 2, wr,
                                (const double *)(W + (long)wr * ldw + c_begin), ldw, (kp1 + off) / 2, wr, N, 2 * kp1,
-                               (const int *)(ws.wgexp.as<int>() + c_begin), wg, wslice, wkb, 3); XTG_LAUNCH_CHECK("k_slice_digits");
+                               (const int *)(ws.wgexp.as<int>() + c_begin), wg, wslice, wkb, 3,
+                               (signed char *)nullptr); XTG_LAUNCH_CHECK("k_slice_digits");
             prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 16.0 * (2.0 * wr * N) + SPLIT_DIGITS * 2.0 * kp1 * N);
             const long ko = (long)(klo / ZSPLIT_KSTEP) * kbs;
             pr = prof_begin(st);
@@ -326,7 +326,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
         }
         dim3 grid(((N + ZTB_H - 1) / ZTB_H) * ((g0 + ZTA_H - 1) / ZTA_H));
         int pr = prof_begin(st);
-        hipLaunchKernelGGL(kz_gemm<0>, grid, dim3(256), 0, st, g0, N, wr, wr, Sw + (long)g0 * ldsw, ldsw,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kz_gemm<0, ZTA_H>), grid, dim3(256), 0, st, g0, N, wr, wr, Sw + (long)g0 * ldsw, ldsw,
                            Pw + (long)g0 * ldpw, ldpw, (const double2 *)(W + c_begin), ldw, X + c_begin, ldx,
                            (const int *)nullptr, 1, (const double *)(ws.sig_grp.as<double>() + c_begin),
                            (const int *)nullptr, (const double2 *)(W + (long)wr * ldw + c_begin)); XTG_LAUNCH_CHECK("kz_gemm<0>");
@@ -369,13 +369,10 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
         const dim3 gx((m + 63) / 64, (kb + 63) / 64), gz((n + 63) / 64, (kb + 63) / 64);
         hipLaunchKernelGGL(k_slice_digits, gx, dim3(256), 0, st, (const double *)X, ldx, 0, n, (const double *)nullptr,
                            0L, 0, 0, m, kb, (const int *)ws.xexp.as<int>(), ws.Xx.as<signed char>(), (long)m * kb,
-                           (long)m * ZSPLIT_KSTEP, 1); XTG_LAUNCH_CHECK("k_slice_digits");
-        hipLaunchKernelGGL(k_slice_digits, gx, dim3(256), 0, st, (const double *)X, ldx, 0, n, (const double *)nullptr,
-                           0L, 0, 0, m, kb, (const int *)ws.xexp.as<int>(), ws.Xy.as<signed char>(), (long)m * kb,
-                           (long)m * ZSPLIT_KSTEP, 2); XTG_LAUNCH_CHECK("k_slice_digits");
+                           (long)m * ZSPLIT_KSTEP, 1, ws.Xy.as<signed char>()); XTG_LAUNCH_CHECK("k_slice_digits");
         hipLaunchKernelGGL(k_slice_digits, gz, dim3(256), 0, st, (const double *)Rp, ldr, 0, n, (const double *)nullptr,
                            0L, 0, 0, n, kb, (const int *)ws.zexp.as<int>(), ws.Zq.as<signed char>(), (long)n * kb,
-                           (long)n * ZSPLIT_KSTEP, 3); XTG_LAUNCH_CHECK("k_slice_digits");
+                           (long)n * ZSPLIT_KSTEP, 3, (signed char *)nullptr); XTG_LAUNCH_CHECK("k_slice_digits");
         prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 16.0 * 2.0 * ((double)n * m + (double)n * n));
         pr = prof_begin(st);
         for (int part = 0; part < 2; ++part) {
@@ -400,7 +397,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
             kmx = std::max(kmx, kmaxv[q]);
         }
         int pr = prof_begin(st);
-        hipLaunchKernelGGL(kz_gemm<1>, grid, dim3(256), 0, st, m, n, n, 0, (const double2 *)X, ldx,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kz_gemm<1, ZTA_H>), grid, dim3(256), 0, st, m, n, n, 0, (const double2 *)X, ldx,
                            (const double2 *)nullptr, 0L, Rp, ldr, V, ldv, ws.outcol.as<int>(), 0,
                            (const double *)nullptr, ws.kmax.as<int>(), (const double2 *)nullptr); XTG_LAUNCH_CHECK("kz_gemm<1>");
         prof_end(pr, st, 2, 8.0 * n * ksum, 16.0 * ((double)n * kmx + ksum + (double)n * m));

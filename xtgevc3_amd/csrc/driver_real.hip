@@ -35,7 +35,7 @@ __global__ void k_solve_real(const double *, long, const double *, long, int, in
                              int *, int, int, double *, int, GroupInfo, MixedExp, SplitOut);
 __global__ void k_rowexp(const double *, long, const double *, long, int, int, int *);
 __global__ void k_slice_digits(const double *, long, int, int, const double *, long, int, int, int, int, const int *,
-                               signed char *, long, long, int);
+                               signed char *, long, long, int, signed char *);
 __global__ void k_xexp_final(const int *, const int *, long, const int *, const int *, int, int, int *);
 int launch_gemm_i8(hipStream_t st, int M, int N, int K1, int K2, const signed char *L1, const signed char *L2,
                    long lslice, long kbl, const signed char *R, long rslice, long kbr, const int *EL, const int *ER,
@@ -554,7 +554,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
             hipLaunchKernelGGL(k_slice_digits, dim3((N + 63) / 64, (2 * kp1 + 63) / 64), dim3(256), 0, st,
                                (const double *)(W + c_begin), ldw, off, wr, (const double *)(W + (long)wr * ldw + c_begin),
                                ldw, kp1 + off, wr, N, 2 * kp1, (const int *)(ws.wgexp.as<int>() + c_begin), wg, wslice,
-                               wkb, 0); XTG_LAUNCH_CHECK("k_slice_digits");
+                               wkb, 0, (signed char *)nullptr); XTG_LAUNCH_CHECK("k_slice_digits");
             prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * (2.0 * wr * N) + SPLIT_DIGITS * 2.0 * kp1 * N);
             pr = prof_begin(st);
             const long kbs = (long)n * SPLIT_KSTEP, ko = (long)(klo / SPLIT_KSTEP) * kbs + (long)r0 * SPLIT_KSTEP;
@@ -626,10 +626,10 @@ static hipError_t split_rows(hipStream_t st, int rows, int cols, int kp, const d
     const dim3 grid((rows + 63) / 64, (kp + 63) / 64);
     const long qs = (long)rows * kp, kbp = (long)rows * SPLIT_KSTEP;
     hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, A, lda, 0, cols, (const double *)nullptr, 0L, 0, 0, rows,
-                       kp, (const int *)E, QA, qs, kbp, 0);
+                       kp, (const int *)E, QA, qs, kbp, 0, (signed char *)nullptr);
     if (B != nullptr) {
         hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, B, ldb, 0, cols, (const double *)nullptr, 0L, 0, 0,
-                           rows, kp, (const int *)E, QB, qs, kbp, 0);
+                           rows, kp, (const int *)E, QB, qs, kbp, 0, (signed char *)nullptr);
     }
     return hipGetLastError();
 }
@@ -763,7 +763,7 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
             int pr = prof_begin(st);
             hipLaunchKernelGGL(k_slice_digits, dim3((m + 63) / 64, (kb + 63) / 64), dim3(256), 0, st, (const double *)X,
                                ldx, 0, n, (const double *)nullptr, 0L, 0, 0, m, kb, (const int *)ws.xexp.as<int>(),
-                               ws.Xq.as<signed char>(), (long)m * kb, (long)m * SPLIT_KSTEP, 0); XTG_LAUNCH_CHECK("k_slice_digits");
+                               ws.Xq.as<signed char>(), (long)m * kb, (long)m * SPLIT_KSTEP, 0, (signed char *)nullptr); XTG_LAUNCH_CHECK("k_slice_digits");
             prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * (double)n * m + SPLIT_DIGITS * (double)m * kb);
             XTG_CHECK(hipStreamWaitEvent(st, ws.ev_z, 0));
             pr = prof_begin(st);

@@ -322,52 +322,55 @@ template __global__ void kz_solve_units<32>(const double2 *, long, const double2
 // ---------------------------------------------------------------------------------------
 // kz_gemm: D[a][b] = sum_k L[k][a] * R[k][b], complex, FP64 MFMA; rows k >= K1 of L and R
 // come from the second segments L2 and R2 (row k - K1).
-// Tile 128 (a) x 64 (b) per 256-thread workgroup (waves 2x2, each 64 x 32), K chunks of 8.
+// Tile TA (a) x 64 (b) per 256-thread workgroup (waves 2x2, each TA/2 x 32), K chunks of 8.
 // Same operand / output conventions as k_gemm_f64 (element = complex double2).
 // ---------------------------------------------------------------------------------------
 typedef double zdouble4_t __attribute__((ext_vector_type(4)));
-constexpr int ZTA = 128, ZTB = 64, ZGK = 8;
+constexpr int ZTB = 64, ZGK = 8;
 
-template <int TAG>
+// TA: tile rows (128, or 32 for short products over few rows)
+template <int TAG, int TA>
 __global__ __launch_bounds__(256) void kz_gemm(int M, int N, int K1, int K2, const double2 *__restrict__ L, long ldl,
                                                const double2 *__restrict__ L2, long ldl2, const double2 *__restrict__ R,
                                                long ldr, double2 *__restrict__ C, long ldc,
                                                const int *__restrict__ outcol, int mode, const double *__restrict__ sig,
                                                const int *__restrict__ kmax, const double2 *__restrict__ R2)
 {
-    __shared__ double Lr[2][ZGK][ZTA], Li[2][ZGK][ZTA];
+    constexpr int AI = TA / 32;       // 16-row MFMA blocks per wave (waves 2 x 2)
+    constexpr int LS = 256 / TA;      // k rows of an L chunk loaded per pass
+    __shared__ double Lr[2][ZGK][TA], Li[2][ZGK][TA];
     __shared__ double Rr[2][ZGK][ZTB], Ri[2][ZGK][ZTB];
     int tm, tn;
-    xcd_tile(blockIdx.x, (M + ZTA - 1) / ZTA, (N + ZTB - 1) / ZTB, tm, tn);
-    const int a0 = tm * ZTA, b0 = tn * ZTB;
+    xcd_tile(blockIdx.x, (M + TA - 1) / TA, (N + ZTB - 1) / ZTB, tm, tn);
+    const int a0 = tm * TA, b0 = tn * ZTB;
     const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
-    const int wa = (wave >> 1) * 64, wb = (wave & 1) * 32;
+    const int wa = (wave >> 1) * (TA / 2), wb = (wave & 1) * 32;
     int K = K1 + K2;
     if (kmax != nullptr) {
         int kk = 0;
-        int amax = min(a0 + ZTA, M) - 1;
+        int amax = min(a0 + TA, M) - 1;
         for (int q = a0; q <= amax; ++q) {
             kk = max(kk, kmax[q]);
         }
         K = min(K, kk);
     }
-    zdouble4_t accr[4][2], acci[4][2];
+    zdouble4_t accr[AI][2], acci[AI][2];
 #pragma clang loop unroll(full)
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < AI; ++i) {
 #pragma clang loop unroll(full)
         for (int j = 0; j < 2; ++j) {
             accr[i][j] = (zdouble4_t){0, 0, 0, 0};
             acci[i][j] = (zdouble4_t){0, 0, 0, 0};
         }
     }
-    // staging: L chunk 8 x 128 complex (4 per thread), R chunk 8 x 64 complex (2 per thread)
-    const int lc = tid % ZTA, lk = tid / ZTA;  // lk in {0,1}
+    // staging: L chunk 8 x TA complex (TA/32 per thread), R chunk 8 x 64 complex (2 per thread)
+    const int lc = tid % TA, lk = tid / TA;
     const int rc = tid % ZTB, rk = tid / ZTB;  // rk in {0..3}
-    double2 lreg[4], rreg[2];
+    double2 lreg[AI], rreg[2];
     auto load_chunk = [&](int k0) {
 #pragma clang loop unroll(full)
-        for (int s = 0; s < 4; ++s) {
-            int k = k0 + lk + 2 * s;
+        for (int s = 0; s < AI; ++s) {
+            int k = k0 + lk + LS * s;
             int aa = a0 + lc;
             double2 v = make_double2(0.0, 0.0);
             if (k < K && aa < M) {
@@ -388,9 +391,9 @@ __global__ __launch_bounds__(256) void kz_gemm(int M, int N, int K1, int K2, con
     };
     auto store_chunk = [&](int buf) {
 #pragma clang loop unroll(full)
-        for (int s = 0; s < 4; ++s) {
-            Lr[buf][lk + 2 * s][lc] = lreg[s].x;
-            Li[buf][lk + 2 * s][lc] = lreg[s].y;
+        for (int s = 0; s < AI; ++s) {
+            Lr[buf][lk + LS * s][lc] = lreg[s].x;
+            Li[buf][lk + LS * s][lc] = lreg[s].y;
         }
 #pragma clang loop unroll(full)
         for (int s = 0; s < 2; ++s) {
@@ -412,9 +415,9 @@ __global__ __launch_bounds__(256) void kz_gemm(int M, int N, int K1, int K2, con
         }
 #pragma clang loop unroll(full)
         for (int ks = 0; ks < ZGK; ks += 4) {
-            double ar_[4], ai_[4], br_[2], bi_[2];
+            double ar_[AI], ai_[AI], br_[2], bi_[2];
 #pragma clang loop unroll(full)
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < AI; ++i) {
                 ar_[i] = Lr[buf][ks + fk][wa + i * 16 + fr];
                 ai_[i] = Li[buf][ks + fk][wa + i * 16 + fr];
             }
@@ -424,7 +427,7 @@ __global__ __launch_bounds__(256) void kz_gemm(int M, int N, int K1, int K2, con
                 bi_[j] = Ri[buf][ks + fk][wb + j * 16 + fr];
             }
 #pragma clang loop unroll(full)
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < AI; ++i) {
                 double nai = -ai_[i];
 #pragma clang loop unroll(full)
                 for (int j = 0; j < 2; ++j) {
@@ -441,7 +444,7 @@ __global__ __launch_bounds__(256) void kz_gemm(int M, int N, int K1, int K2, con
         __syncthreads();
     }
 #pragma clang loop unroll(full)
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < AI; ++i) {
 #pragma clang loop unroll(full)
         for (int vv = 0; vv < 4; ++vv) {
             int aa = a0 + wa + i * 16 + fk + 4 * vv;
@@ -468,12 +471,14 @@ __global__ __launch_bounds__(256) void kz_gemm(int M, int N, int K1, int K2, con
     }
 }
 
-template __global__ void kz_gemm<0>(int, int, int, int, const double2 *, long, const double2 *, long, const double2 *,
-                                    long, double2 *, long, const int *, int, const double *, const int *,
-                                    const double2 *);
-template __global__ void kz_gemm<1>(int, int, int, int, const double2 *, long, const double2 *, long, const double2 *,
-                                    long, double2 *, long, const int *, int, const double *, const int *,
-                                    const double2 *);
+#define XTG_KZ_INST(T, TA)                                                                                       \
+    template __global__ void kz_gemm<T, TA>(int, int, int, int, const double2 *, long, const double2 *, long,          \
+                                            const double2 *, long, double2 *, long, const int *, int, const double *,  \
+                                            const int *, const double2 *);
+XTG_KZ_INST(0, 128)
+XTG_KZ_INST(0, 32)
+XTG_KZ_INST(1, 128)
+#undef XTG_KZ_INST
 
 __global__ void kz_fixup_scale(double2 *__restrict__ X, long ldx, int n, int m, const int *__restrict__ rowstep,
                                const int *__restrict__ blkexp, long ldb, const int *__restrict__ colexp,
