@@ -43,13 +43,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_inputs(n, seed, device):
+def make_inputs(n, seed, device, cluster=False):
     import torch
 
     from xtgevc3_amd import pencils as G
 
     t0 = time.time()
-    S, P, ar, ai, be = G.real_quasi_triangular(n, seed)
+    # cluster: diagonal ratios s_kk / p_kk drawn from 64 centres with relative spread 1e-6
+    S, P, ar, ai, be = G.real_quasi_triangular(n, seed, cluster_groups=64 if cluster else 0)
     g = torch.Generator(device=device)
     g.manual_seed(seed + 1)
     A = torch.randn(n, n, dtype=torch.float64, device=device, generator=g)
@@ -64,9 +65,9 @@ def make_inputs(n, seed, device):
     return S, P, ar, ai, be, dS, dP, dZ
 
 
-def residual_sample(S, P, ar, ai, be, dZ, dVR, ncols=64, seed=0):
-    """Metric A residual of a sample of back-transformed columns, in units of n*eps, computed
-    on the device in FP64: x = Z^T v, r = (beta S - alpha P) x."""
+def residual_sample(S, P, ar, ai, be, dZ, dVR, ncols=64, seed=0, back=True):
+    """Metric A residual of a sample of columns, in units of n*eps, computed on the device in
+    FP64: x = Z^T v (back-transformed output) or x = v, r = (beta S - alpha P) x."""
     import torch
 
     from oracle import oracle as O
@@ -86,8 +87,7 @@ def residual_sample(S, P, ar, ai, be, dZ, dVR, ncols=64, seed=0):
             full.add(c)
     cols = np.array(sorted(full))
     V = dVR[cols, :].t()  # n x len(cols), column c = VR[:, c]
-    Zm = dZ.t()           # Z (n x n)
-    X = Zm.t() @ V        # back to the Schur basis
+    X = dZ @ V if back else V  # back to the Schur basis: Z^T v (dZ holds Z^T row-major)
     Sd = torch.from_numpy(np.ascontiguousarray(S)).cuda()
     Pd = torch.from_numpy(np.ascontiguousarray(P)).cuda()
     SX = (Sd @ X).cpu().numpy()
@@ -210,6 +210,10 @@ def main():
     ap.add_argument("--no-residual", action="store_true")
     ap.add_argument("--no-backward", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--cluster", action="store_true",
+                    help="eigenvalues clustered in 64 groups (relative spread 1e-6), as the n=16384 configuration")
+    ap.add_argument("--howmny", default="B", choices=["A", "B"],
+                    help="B: back-transformed (the metric); A: eigenvectors of (S, P), no back-transform")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this path runs on one GPU (the eigenvector columns shard across GPUs without "
@@ -222,13 +226,16 @@ def main():
 
     n = args.n
     dev = "cuda"
-    S, P, ar, ai, be, dS, dP, dZ = make_inputs(n, args.seed, dev)
+    S, P, ar, ai, be, dS, dP, dZ = make_inputs(n, args.seed, dev, cluster=args.cluster)
     dVR = torch.empty_like(dZ)
     stream = torch.cuda.current_stream()
 
+    back = args.howmny == "B"
+
     def step():
-        dVR.copy_(dZ)
-        m, info = X.dtgevc3_device("R", "B", None, dS, dP, ar, ai, be, VR=dVR, stream=stream.cuda_stream)
+        if back:
+            dVR.copy_(dZ)
+        m, info = X.dtgevc3_device("R", args.howmny, None, dS, dP, ar, ai, be, VR=dVR, stream=stream.cuda_stream)
         assert info == 0 and m == n, (m, info)
         return m
 
@@ -266,7 +273,7 @@ def main():
     # algorithmic flops of the whole path: sum_k 2k^2 (solve) + 2n(k+1) (back-transform), pairs on
     # both columns
     kk = np.arange(n, dtype=np.float64)
-    flops = float(np.sum(2 * kk * kk) + np.sum(2 * n * (kk + 1)))
+    flops = float(np.sum(2 * kk * kk) + (np.sum(2 * n * (kk + 1)) if back else 0.0))
     path_tflops = flops * args.steps / total_s / 1e12
 
     # dominant kernel: the kernel instance with the largest share of the timed region (names as
@@ -302,15 +309,16 @@ def main():
     if not args.no_residual:
         step()
         torch.cuda.synchronize()
-        res, ncols = residual_sample(S, P, ar, ai, be, dZ, dVR)
+        res, ncols = residual_sample(S, P, ar, ai, be, dZ, dVR, back=back)
         log(f"[bench] residual (metric A) over {ncols} sampled columns: {res:.3e} n*eps")
 
     # mixed forms on the same inputs: split-int8 digit products (FP64-level residual) and FP32
     # products; each timed over the same K steps, then one profiled step and one residual step
     def run_form(precision, label):
         def step_form():
-            dVR.copy_(dZ)
-            m, info = X.dtgevc3_device("R", "B", None, dS, dP, ar, ai, be, VR=dVR, stream=stream.cuda_stream,
+            if back:
+                dVR.copy_(dZ)
+            m, info = X.dtgevc3_device("R", args.howmny, None, dS, dP, ar, ai, be, VR=dVR, stream=stream.cuda_stream,
                                        precision=precision)
             assert info == 0 and m == n, (m, info)
             return m
@@ -352,7 +360,7 @@ def main():
             ref64 = dVR.clone()
             step_form()
             torch.cuda.synchronize()
-            rm, _ = residual_sample(S, P, ar, ai, be, dZ, dVR)
+            rm, _ = residual_sample(S, P, ar, ai, be, dZ, dVR, back=back)
             diff = float((dVR - ref64).abs().max())
             out["max_residual_n_eps"] = float(f"{rm:.4g}")
             out["max_abs_diff_vs_fp64"] = float(f"{diff:.4g}")
@@ -397,8 +405,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"dtgevc3 side=R howmny=B n={n} plain FP64, quasi-triangular pencil "
-                               f"(40% 2x2 pairs), Z orthogonal", "n": n, "side": "R", "howmny": "B",
+        "config": {"workload": f"dtgevc3 side=R howmny={args.howmny} n={n} plain FP64, quasi-triangular pencil "
+                               f"(40% 2x2 pairs)" + (", 64 eigenvalue clusters" if args.cluster else "")
+                               + (", Z orthogonal" if back else ""), "n": n, "side": "R",
+                   "howmny": args.howmny,
                    "form": "plain-fp64", "path_tflops": round(path_tflops, 3),
                    "max_residual_n_eps": None if res is None else float(f"{res:.4g}")},
         "roofline": roof,
