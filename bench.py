@@ -210,6 +210,7 @@ def main():
     ap.add_argument("--no-residual", action="store_true")
     ap.add_argument("--no-backward", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-host-api", action="store_true")
     ap.add_argument("--cluster", action="store_true",
                     help="eigenvalues clustered in 64 groups (relative spread 1e-6), as the n=16384 configuration")
     ap.add_argument("--howmny", default="B", choices=["A", "B"],
@@ -378,6 +379,28 @@ def main():
         mixed_f32 = run_form(X.PREC_MIXED_F32, "FP32 MFMA products (updates, back-transform), FP64 "
                                                "solves/scales/normalisation")
 
+    # the drop-in host-pointer interface (reference signature): S, P, Z staged over PCIe, VR
+    # copied back; PCIe-inclusive rate of one call (not the reported value), also for the FP32
+    # interface stgevc3 on the same pencil
+    host = None
+    if not args.no_host_api and back:
+        Sf = np.asfortranarray(S)
+        Pf = np.asfortranarray(P)
+        Zf = np.asfortranarray(dZ.t().cpu().numpy())
+        host = {}
+        for name, fn, dt in (("dtgevc3", X.dtgevc3, np.float64), ("stgevc3", X.stgevc3, np.float32)):
+            Sx, Px = np.asfortranarray(Sf.astype(dt)), np.asfortranarray(Pf.astype(dt))
+            Vx = np.asfortranarray(Zf.astype(dt))
+            fn("R", "B", None, Sx, Px, ar.astype(dt), ai.astype(dt), be.astype(dt), VR=Vx.copy(order="F"))
+            Vx2 = Vx.copy(order="F")
+            t0 = time.perf_counter()
+            mh, ih, _ = fn("R", "B", None, Sx, Px, ar.astype(dt), ai.astype(dt), be.astype(dt), VR=Vx2)
+            th = time.perf_counter() - t0
+            assert ih == 0 and mh == n, (mh, ih)
+            host[name] = {"ms_per_call": round(1000.0 * th, 2), "eigvecs_per_s": round(mh / th, 1)}
+            log(f"[bench] host-pointer {name} (PCIe staging included): {1000.0 * th:.1f} ms, {mh / th:.1f} eigvecs/s")
+        del Sf, Pf, Zf
+
     bw = None
     if not args.no_backward:
         bw = backward_ratio(args.seed)
@@ -414,6 +437,7 @@ def main():
         "roofline": roof,
         "mixed": mixed,
         "mixed_f32": mixed_f32,
+        "host_api": host,
         "backward": bw,
         "kernels": kernels,
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
