@@ -120,7 +120,7 @@ struct SplitOut {
     int *xbexp;
 };
 // tile configurations of the FP64 GEMM (a x b)
-enum : int { GEMM_128x128 = 0, GEMM_64x128 = 1, GEMM_128x64 = 2, GEMM_64x64 = 3 };
+enum : int { GEMM_128x128 = 0, GEMM_64x128 = 1, GEMM_128x64 = 2, GEMM_64x64 = 3, GEMM_32x64 = 4 };
 
 } // namespace xtg
 

@@ -831,7 +831,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
 
     constexpr int FTA = GTA / 32; // 16x16 MFMA blocks per wave tile edge (wave tile GTA/2 x GTB/2)
     constexpr int FTB = GTB / 32;
-    static_assert(GK % 4 == 0 && (GTA == 64 || GTA == 128) && (GTB == 64 || GTB == 128), "tile shape");
+    static_assert(GK % 4 == 0 && (GTA == 32 || GTA == 64 || GTA == 128) && (GTB == 64 || GTB == 128), "tile shape");
     typedef typename std::conditional<F32, float4_t, double4_t>::type acc_t;
     acc_t acc[FTA][FTB];
 #pragma clang loop unroll(full)
@@ -1407,10 +1407,14 @@ int gemm_config(int M, int N, int K)
         {GEMM_128x64, 128, 64, 3, 0.667, 0.80},
         {GEMM_64x128, 64, 128, 3, 0.68, 0.82},
         {GEMM_64x64, 64, 64, 2, 0.26, 0.30},
+        {GEMM_32x64, 32, 64, 3, 0.15, 0.17},
     };
     int best = GEMM_128x128;
     double tbest = 1e300;
     for (const Cfg &c : cfgs) {
+        if (c.id == GEMM_32x64 && (M > 256 || K > 256)) {
+            continue; // measured only on the short near-update products
+        }
         const long tiles = (long)((M + c.ga - 1) / c.ga) * ((N + c.gb - 1) / c.gb);
         const long conc = 256L * c.per_cu;
         const long full = tiles / conc, rem = tiles % conc;
@@ -1435,8 +1439,8 @@ int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const
 {
     const bool wide = M >= 2 && N >= 2;
     const int cfg = gemm_config(M, N, K1 + K2);
-    const int gta = cfg == GEMM_64x64 || cfg == GEMM_64x128 ? 64 : 128;
-    const int gtb = cfg == GEMM_64x64 || cfg == GEMM_128x64 ? 64 : 128;
+    const int gta = cfg == GEMM_32x64 ? 32 : (cfg == GEMM_64x64 || cfg == GEMM_64x128 ? 64 : 128);
+    const int gtb = cfg == GEMM_64x64 || cfg == GEMM_128x64 || cfg == GEMM_32x64 ? 64 : 128;
     const dim3 grid(((N + gtb - 1) / gtb) * ((M + gta - 1) / gta));
 #define XTG_GEMM_LAUNCH(T, WD, GA, GB, K)                                                                            \
     if (prec == 1 && T != 2) {                                                                                       \
@@ -1460,6 +1464,9 @@ int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const
         switch (cfg) {                                                                                               \
         case GEMM_64x64:                                                                                             \
             XTG_GEMM_WIDE(T, 64, 64, 32)                                                                             \
+            break;                                                                                                   \
+        case GEMM_32x64:                                                                                             \
+            XTG_GEMM_WIDE(T, 32, 64, 32)                                                                             \
             break;                                                                                                   \
         case GEMM_64x128:                                                                                            \
             XTG_GEMM_WIDE(T, 64, 128, 16)                                                                            \
