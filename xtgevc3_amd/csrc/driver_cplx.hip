@@ -217,7 +217,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
         hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.rowexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
         hipLaunchKernelGGL(kz_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, Sw, ldsw, Pw, ldpw, n, n,
                            ws.rowexp.as<int>()); XTG_LAUNCH_CHECK("kz_rowexp");
-        const dim3 grid((n + 63) / 64, (unsigned)((qld + 63) / 64));
+        const dim3 grid((n + 63) / 64, (unsigned)((qld + 15) / 16));
         hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, (const double *)Sw, ldsw, 0, n,
                            (const double *)nullptr, 0L, 0, 0, n, (int)qld, (const int *)ws.rowexp.as<int>(),
                            ws.Sx.as<signed char>(), qsl, kbs, 1, ws.Sy.as<signed char>()); XTG_LAUNCH_CHECK("k_slice_digits");
@@ -304,7 +304,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
             signed char *wg = ws.Wg.as<signed char>() + (long)c_begin * ZSPLIT_KSTEP;
             const long wslice = (long)m * ZWG_K, wkb = (long)m * ZSPLIT_KSTEP;
             int pr = prof_begin(st);
-            hipLaunchKernelGGL(k_slice_digits, dim3((N + 63) / 64, (2 * kp1 + 63) / 64), dim3(256), 0, st,
+            hipLaunchKernelGGL(k_slice_digits, dim3((N + 63) / 64, (2 * kp1 + 15) / 16), dim3(256), 0, st,
                                (const double *)(W + c_begin), ldw, off / 2, wr,
                                (const double *)(W + (long)wr * ldw + c_begin), ldw, (kp1 + off) / 2, wr, N, 2 * kp1,
                                (const int *)(ws.wgexp.as<int>() + c_begin), wg, wslice, wkb, 3,
@@ -366,7 +366,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
         hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.zexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
         hipLaunchKernelGGL(kz_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, Rp, ldr,
                            (const double2 *)nullptr, 0L, n, n, ws.zexp.as<int>()); XTG_LAUNCH_CHECK("kz_rowexp");
-        const dim3 gx((m + 63) / 64, (kb + 63) / 64), gz((n + 63) / 64, (kb + 63) / 64);
+        const dim3 gx((m + 63) / 64, (kb + 15) / 16), gz((n + 63) / 64, (kb + 15) / 16);
         hipLaunchKernelGGL(k_slice_digits, gx, dim3(256), 0, st, (const double *)X, ldx, 0, n, (const double *)nullptr,
                            0L, 0, 0, m, kb, (const int *)ws.xexp.as<int>(), ws.Xx.as<signed char>(), (long)m * kb,
                            (long)m * ZSPLIT_KSTEP, 1, ws.Xy.as<signed char>()); XTG_LAUNCH_CHECK("k_slice_digits");

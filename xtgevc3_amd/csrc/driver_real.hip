@@ -551,7 +551,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
             int pr = prof_begin(st);
             signed char *wg = ws.Wg.as<signed char>() + (long)c_begin * SPLIT_KSTEP;
             const long wslice = (long)m * WG_K, wkb = (long)m * SPLIT_KSTEP;
-            hipLaunchKernelGGL(k_slice_digits, dim3((N + 63) / 64, (2 * kp1 + 63) / 64), dim3(256), 0, st,
+            hipLaunchKernelGGL(k_slice_digits, dim3((N + 63) / 64, (2 * kp1 + 15) / 16), dim3(256), 0, st,
                                (const double *)(W + c_begin), ldw, off, wr, (const double *)(W + (long)wr * ldw + c_begin),
                                ldw, kp1 + off, wr, N, 2 * kp1, (const int *)(ws.wgexp.as<int>() + c_begin), wg, wslice,
                                wkb, 0, (signed char *)nullptr); XTG_LAUNCH_CHECK("k_slice_digits");
@@ -623,7 +623,7 @@ static hipError_t split_rows(hipStream_t st, int rows, int cols, int kp, const d
     hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, E, (long)rows, -1000);
     hipLaunchKernelGGL(k_rowexp, dim3((rows + 255) / 256, (cols + 63) / 64), dim3(256), 0, st, A, lda, B, ldb, rows,
                        cols, E);
-    const dim3 grid((rows + 63) / 64, (kp + 63) / 64);
+    const dim3 grid((rows + 63) / 64, (kp + 15) / 16);
     const long qs = (long)rows * kp, kbp = (long)rows * SPLIT_KSTEP;
     hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, A, lda, 0, cols, (const double *)nullptr, 0L, 0, 0, rows,
                        kp, (const int *)E, QA, qs, kbp, 0, (signed char *)nullptr);
@@ -761,7 +761,7 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
             const int kb = round_up(n, SPLIT_KSTEP);
             ENSURE(ws.Xq, (size_t)SPLIT_DIGITS * m * kb);
             int pr = prof_begin(st);
-            hipLaunchKernelGGL(k_slice_digits, dim3((m + 63) / 64, (kb + 63) / 64), dim3(256), 0, st, (const double *)X,
+            hipLaunchKernelGGL(k_slice_digits, dim3((m + 63) / 64, (kb + 15) / 16), dim3(256), 0, st, (const double *)X,
                                ldx, 0, n, (const double *)nullptr, 0L, 0, 0, m, kb, (const int *)ws.xexp.as<int>(),
                                ws.Xq.as<signed char>(), (long)m * kb, (long)m * SPLIT_KSTEP, 0, (signed char *)nullptr); XTG_LAUNCH_CHECK("k_slice_digits");
             prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * (double)n * m + SPLIT_DIGITS * (double)m * kb);

@@ -38,21 +38,22 @@ typedef int i32x16 __attribute__((ext_vector_type(16)));
 // With Q2 != nullptr (cmode 1 only) the same pass also writes the (im, re) expansion to Q2:
 // its digits at positions (2k, 2k+1) are (-d(2k+1), d(2k)) of the (re, -im) digits, exact
 // because the digits of -v are those of v negated.
-// Block: 64 rows a x 64 k, thread t: row t / 4, k chunk 16 (t % 4).
+// Block: 64 rows a x 16 k, thread t: row t / 4, k quad 4 (t % 4) (many small blocks: the
+// kernel is latency-bound at the sizes of the group operands).
 __global__ __launch_bounds__(256) void k_slice_digits(const double *__restrict__ A1, long lda1, int o1, int K1,
                                                       const double *__restrict__ A2, long lda2, int o2, int K2, int M,
                                                       int Kp, const int *__restrict__ E, signed char *__restrict__ Q,
                                                       long qs, long kbp, int cmode, signed char *__restrict__ Q2)
 {
     const int a = blockIdx.x * 64 + threadIdx.x / 4;
-    const int k0 = blockIdx.y * 64 + (threadIdx.x % 4) * 16;
+    const int k0 = blockIdx.y * 16 + (threadIdx.x % 4) * 4;
     if (a >= M || k0 >= Kp) {
         return;
     }
     const int e = E[a] + 1;
-    int dg[SPLIT_DIGITS][16];
+    int dg[SPLIT_DIGITS][4];
 #pragma clang loop unroll(full)
-    for (int q = 0; q < 16; ++q) {
+    for (int q = 0; q < 4; ++q) {
         const int kk = k0 + q;
         double v = 0.0;
         if (cmode == 0) {
@@ -84,24 +85,14 @@ __global__ __launch_bounds__(256) void k_slice_digits(const double *__restrict__
     const long qo = (long)(k0 / 32) * kbp + (long)a * 32 + (k0 % 32);
 #pragma clang loop unroll(full)
     for (int s = 0; s < SPLIT_DIGITS; ++s) {
-        i32x4 w;
-#pragma clang loop unroll(full)
-        for (int c = 0; c < 4; ++c) {
-            w[c] = (dg[s][4 * c] & 0xff) | ((dg[s][4 * c + 1] & 0xff) << 8) | ((dg[s][4 * c + 2] & 0xff) << 16) |
-                   ((dg[s][4 * c + 3] & 0xff) << 24);
-        }
-        *reinterpret_cast<i32x4 *>(Q + qo + s * qs) = w;
+        *reinterpret_cast<int *>(Q + qo + s * qs) =
+            (dg[s][0] & 0xff) | ((dg[s][1] & 0xff) << 8) | ((dg[s][2] & 0xff) << 16) | ((dg[s][3] & 0xff) << 24);
     }
     if (Q2 != nullptr) {
 #pragma clang loop unroll(full)
         for (int s = 0; s < SPLIT_DIGITS; ++s) {
-            i32x4 w;
-#pragma clang loop unroll(full)
-            for (int c = 0; c < 4; ++c) {
-                w[c] = ((-dg[s][4 * c + 1]) & 0xff) | ((dg[s][4 * c] & 0xff) << 8) |
-                       (((-dg[s][4 * c + 3]) & 0xff) << 16) | ((dg[s][4 * c + 2] & 0xff) << 24);
-            }
-            *reinterpret_cast<i32x4 *>(Q2 + qo + s * qs) = w;
+            *reinterpret_cast<int *>(Q2 + qo + s * qs) = ((-dg[s][1]) & 0xff) | ((dg[s][0] & 0xff) << 8) |
+                                                         (((-dg[s][3]) & 0xff) << 16) | ((dg[s][2] & 0xff) << 24);
         }
     }
 }
