@@ -348,13 +348,15 @@ XTG_I8_INST(128, 128, 2, 4)
 XTG_I8_INST(64, 128, 1, 4)
 #undef XTG_I8_INST
 
-// One launch; the tile is 64 x 128 when M is small.
+// One launch; the tile is 64 x 128 when M is small or 128 x 128 tiles would not fill the GPU.
 static int launch_i8(hipStream_t st, int M, int N, int K1, int K2, const signed char *L1, const signed char *L2,
                      long lslice, long kbl, const signed char *R, long rslice, long kbr, const int *EL, const int *ER,
                      double *C, long ldc, int cst, const int *outcol, int mode, const double *sig, const int *kmax,
                      int kofs)
 {
-    if (M <= 64) {
+    // 64-row tiles when 128-row tiles would leave CUs idle (few rows, or few columns)
+    const long t128 = (long)((N + 127) / 128) * ((M + 127) / 128);
+    if (M <= 64 || t128 < 256) {
         const dim3 grid(((N + 127) / 128) * ((M + 63) / 64));
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_i8<64, 128, 1, 4>), grid, dim3(256), 0, st, M, N, K1, K2, L1, L2,
                            lslice, kbl, R, rslice, kbr, EL, ER, C, ldc, cst, outcol, mode, sig, kmax, kofs);
