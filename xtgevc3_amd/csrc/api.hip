@@ -6,10 +6,13 @@
 // entry points take device arrays and a HIP stream and skip the workspace arguments.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <complex>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/xtgevc3.h"
@@ -20,7 +23,8 @@ namespace xtg {
 
 namespace {
 
-struct HostStage {
+// Grow-only device array of the host-pointer entry points.
+struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
     bool ensure(size_t bytes)
@@ -41,8 +45,78 @@ struct HostStage {
     }
 };
 
+// Matrices move between user memory and the device in column chunks of at most STAGE_CHUNK
+// bytes through two pinned host buffers: packing one chunk on the host overlaps the DMA of the
+// other.  FP32 chunks land in one of two device buffers and are widened (or narrowed, on the way
+// back) by a kernel, so PCIe carries FP32 bytes for stgevc3/ctgevc3.
+constexpr size_t STAGE_CHUNK = (size_t)32 << 20;
+
+struct Ring {
+    void *host[2] = {nullptr, nullptr};
+    void *dev[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool armed[2] = {false, false};
+    size_t cap = 0;
+
+    // waits until the last copy through buffer b has finished
+    bool wait(int b)
+    {
+        if (!armed[b]) {
+            return true;
+        }
+        armed[b] = false;
+        return hipEventSynchronize(ev[b]) == hipSuccess;
+    }
+    bool mark(int b, hipStream_t st)
+    {
+        if (hipEventRecord(ev[b], st) != hipSuccess) {
+            return false;
+        }
+        armed[b] = true;
+        return true;
+    }
+    bool ensure(size_t colbytes)
+    {
+        const size_t want = std::max(STAGE_CHUNK, colbytes);
+        if (want <= cap) {
+            return true;
+        }
+        if (!wait(0) || !wait(1)) {
+            return false;
+        }
+        cap = 0;
+        for (int b = 0; b < 2; ++b) {
+            if (host[b]) {
+                (void)hipHostFree(host[b]);
+                host[b] = nullptr;
+            }
+            if (dev[b]) {
+                (void)hipFree(dev[b]);
+                dev[b] = nullptr;
+            }
+            if (!ev[b] && hipEventCreateWithFlags(&ev[b], hipEventDisableTiming) != hipSuccess) {
+                ev[b] = nullptr;
+                return false;
+            }
+        }
+        for (int b = 0; b < 2; ++b) {
+            if (hipHostMalloc(&host[b], want, hipHostMallocDefault) != hipSuccess) {
+                host[b] = nullptr;
+                return false;
+            }
+            if (hipMalloc(&dev[b], want) != hipSuccess) {
+                dev[b] = nullptr;
+                return false;
+            }
+        }
+        cap = want;
+        return true;
+    }
+};
+
 std::mutex g_stage_mutex;
-HostStage g_dS, g_dP, g_dVL, g_dVR;
+DevBuf g_dS, g_dP, g_dVL, g_dVR;
+Ring g_ring;
 
 int fail(int *info, const char *what)
 {
@@ -54,25 +128,163 @@ int fail(int *info, const char *what)
     return XTGEVC3_INFO_DEVICE_ERROR;
 }
 
-template <class T> hipError_t put(void *dst, const T *src, int rows, int cols, int ld)
+bool hip_ok(hipError_t e, const char *what)
 {
-    return hipMemcpy2D(dst, sizeof(T) * rows, src, sizeof(T) * ld, sizeof(T) * rows, cols, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error(std::string(what) + ": " + hipGetErrorString(e));
+        return false;
+    }
+    return true;
 }
 
-template <class T> hipError_t get(T *dst, int ld, const void *src, int rows, int cols)
+__global__ void k_widen(const float *__restrict__ src, double *__restrict__ dst, long len)
 {
-    return hipMemcpy2D(dst, sizeof(T) * ld, src, sizeof(T) * rows, sizeof(T) * rows, cols, hipMemcpyDeviceToHost);
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < len; i += (long)gridDim.x * blockDim.x) {
+        dst[i] = (double)src[i];
+    }
 }
 
-// Host-pointer driver for the double-precision real interface.
-void real_host(char side, char howmny, const int *select, int n, const double *S, int lds, const double *P, int ldp,
-               const double *ar, const double *ai, const double *be, double *VL, int ldvl, double *VR, int ldvr,
-               int mm, int *m, double *work, int lwork, int *info, bool have_work)
+__global__ void k_narrow(const double *__restrict__ src, float *__restrict__ dst, long len)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < len; i += (long)gridDim.x * blockDim.x) {
+        dst[i] = (float)src[i];
+    }
+}
+
+unsigned flat_grid(long len) { return (unsigned)std::min<long>((len + 255) / 256, 4096); }
+
+// Copies `cols` columns of `colbytes` bytes between strided host arrays, on up to 8 threads
+// when the chunk is large (packing pageable memory is bound by per-thread copy bandwidth).
+void copy_cols(char *dst, size_t dstride, const char *src, size_t sstride, size_t colbytes, int cols)
+{
+    const int nt = colbytes * cols >= ((size_t)4 << 20) ? std::min(cols, 8) : 1;
+    auto part = [=](int t) {
+        const int c1 = (int)((long)cols * (t + 1) / nt);
+        for (int c = (int)((long)cols * t / nt); c < c1; ++c) {
+            std::memcpy(dst + c * dstride, src + c * sstride, colbytes);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) {
+        try {
+            pool.emplace_back(part, t);
+        } catch (...) {
+            part(t);
+        }
+    }
+    part(0);
+    for (auto &th : pool) {
+        th.join();
+    }
+}
+
+// Moves a rows x cols column-major host matrix (leading dimension ld, elements of w scalars of
+// type T = double or float) into a packed device matrix of w doubles per element, queued on st.
+template <class T> bool upload(double *dev, const T *host, int rows, int cols, int ld, int w, hipStream_t st)
+{
+    if (rows <= 0 || cols <= 0) {
+        return true;
+    }
+    const size_t colb = (size_t)rows * w * sizeof(T);
+    if (!g_ring.ensure(colb)) {
+        set_error("pinned staging allocation failed");
+        return false;
+    }
+    const int cc = (int)std::min<size_t>(cols, g_ring.cap / colb);
+    for (int c0 = 0, i = 0; c0 < cols; c0 += cc, ++i) {
+        const int b = i & 1, nc = std::min(cc, cols - c0);
+        if (!hip_ok(g_ring.wait(b) ? hipSuccess : hipErrorUnknown, "staging wait")) {
+            return false;
+        }
+        copy_cols((char *)g_ring.host[b], colb, (const char *)(host + (size_t)c0 * ld * w), (size_t)ld * w * sizeof(T),
+                  colb, nc);
+        double *d = dev + (size_t)c0 * rows * w;
+        if (std::is_same<T, double>::value) {
+            if (!hip_ok(hipMemcpyAsync(d, g_ring.host[b], colb * nc, hipMemcpyHostToDevice, st), "host to device copy")) {
+                return false;
+            }
+        } else {
+            if (!hip_ok(hipMemcpyAsync(g_ring.dev[b], g_ring.host[b], colb * nc, hipMemcpyHostToDevice, st),
+                        "host to device copy")) {
+                return false;
+            }
+            const long len = (long)nc * rows * w;
+            hipLaunchKernelGGL(k_widen, dim3(flat_grid(len)), dim3(256), 0, st, (const float *)g_ring.dev[b], d, len);
+            if (!hip_ok(hipGetLastError(), "k_widen launch")) {
+                return false;
+            }
+        }
+        if (!hip_ok(g_ring.mark(b, st) ? hipSuccess : hipErrorUnknown, "staging event")) {
+            return false;
+        }
+    }
+    return true;
+}
+
+// Moves the first cols columns of a packed device matrix (w doubles per element) into a host
+// matrix with leading dimension ld and scalars of type T, rounding to FP32 on the device for
+// T = float; returns when the host copy is complete.
+template <class T> bool download(T *host, int ld, const double *dev, int rows, int cols, int w, hipStream_t st)
+{
+    if (rows <= 0 || cols <= 0) {
+        return true;
+    }
+    const size_t colb = (size_t)rows * w * sizeof(T);
+    if (!g_ring.ensure(colb)) {
+        set_error("pinned staging allocation failed");
+        return false;
+    }
+    const int cc = (int)std::min<size_t>(cols, g_ring.cap / colb);
+    const int nchunks = (cols + cc - 1) / cc;
+    auto issue = [&](int i) -> bool {
+        const int b = i & 1, c0 = i * cc, nc = std::min(cc, cols - c0);
+        if (!hip_ok(g_ring.wait(b) ? hipSuccess : hipErrorUnknown, "staging wait")) {
+            return false;
+        }
+        const double *d = dev + (size_t)c0 * rows * w;
+        if (std::is_same<T, double>::value) {
+            if (!hip_ok(hipMemcpyAsync(g_ring.host[b], d, colb * nc, hipMemcpyDeviceToHost, st), "device to host copy")) {
+                return false;
+            }
+        } else {
+            const long len = (long)nc * rows * w;
+            hipLaunchKernelGGL(k_narrow, dim3(flat_grid(len)), dim3(256), 0, st, d, (float *)g_ring.dev[b], len);
+            if (!hip_ok(hipGetLastError(), "k_narrow launch") ||
+                !hip_ok(hipMemcpyAsync(g_ring.host[b], g_ring.dev[b], colb * nc, hipMemcpyDeviceToHost, st),
+                        "device to host copy")) {
+                return false;
+            }
+        }
+        return hip_ok(g_ring.mark(b, st) ? hipSuccess : hipErrorUnknown, "staging event");
+    };
+    if (!issue(0) || (nchunks > 1 && !issue(1))) {
+        return false;
+    }
+    for (int i = 0; i < nchunks; ++i) {
+        const int b = i & 1, c0 = i * cc, nc = std::min(cc, cols - c0);
+        if (!hip_ok(g_ring.wait(b) ? hipSuccess : hipErrorUnknown, "staging wait")) {
+            return false;
+        }
+        copy_cols((char *)(host + (size_t)c0 * ld * w), (size_t)ld * w * sizeof(T), (const char *)g_ring.host[b], colb,
+                  colb, nc);
+        if (i + 2 < nchunks && !issue(i + 2)) {
+            return false;
+        }
+    }
+    return true;
+}
+
+// Host-pointer driver for the real interfaces.  T (double, or float for stgevc3) is the host
+// scalar type; the eigenvalue arrays are FP64 and the device computes in FP64.
+template <class T>
+void real_host(char side, char howmny, const int *select, int n, const T *S, int lds, const T *P, int ldp,
+               const double *ar, const double *ai, const double *be, T *VL, int ldvl, T *VR, int ldvr, int mm, int *m,
+               T *work, int lwork, int *info)
 {
     std::vector<Unit> units;
     int num_sel = (howmny == 'A' || howmny == 'B' || howmny == 'S') && n > 0 ? real_units(howmny, select, n, ai, units)
                                                                              : 0;
-    if (!check_args<double>(howmny, n, num_sel, mm, m, work, lwork, info, have_work)) {
+    if (!check_args<T>(howmny, n, num_sel, mm, m, work, lwork, info, true)) {
         return;
     }
     bool right = (side == 'R' || side == 'B') && VR != nullptr;
@@ -82,6 +294,7 @@ void real_host(char side, char howmny, const int *select, int n, const double *S
     }
     bool back = howmny == 'B';
     std::lock_guard<std::mutex> lock(g_stage_mutex);
+    const hipStream_t st = nullptr;
     size_t nn = (size_t)n * n * sizeof(double);
     size_t nm = (size_t)n * (back ? n : num_sel) * sizeof(double);
     if (!g_dS.ensure(nn) || !g_dP.ensure(nn) || (right && !g_dVR.ensure(nm)) || (left && !g_dVL.ensure(nm))) {
@@ -89,52 +302,40 @@ void real_host(char side, char howmny, const int *select, int n, const double *S
         fail(info, "staging");
         return;
     }
-    if (put(g_dS.p, S, n, n, lds) != hipSuccess || put(g_dP.p, P, n, n, ldp) != hipSuccess) {
-        set_error("host to device copy failed");
+    double *dS = (double *)g_dS.p, *dP = (double *)g_dP.p, *dVL = (double *)g_dVL.p, *dVR = (double *)g_dVR.p;
+    if (!upload(dS, S, n, n, lds, 1, st) || !upload(dP, P, n, n, ldp, 1, st) ||
+        (back && right && !upload(dVR, VR, n, n, ldvr, 1, st)) || (back && left && !upload(dVL, VL, n, n, ldvl, 1, st))) {
         fail(info, "staging");
         return;
     }
-    if (back) {
-        if ((right && put(g_dVR.p, VR, n, n, ldvr) != hipSuccess) ||
-            (left && put(g_dVL.p, VL, n, n, ldvl) != hipSuccess)) {
-            set_error("host to device copy failed");
-            fail(info, "staging");
-            return;
-        }
-    }
-    int rc = real_device(side, howmny, select, n, (const double *)g_dS.p, n, (const double *)g_dP.p, n, ar, ai, be,
-                         left ? (double *)g_dVL.p : nullptr, n, right ? (double *)g_dVR.p : nullptr, n, nullptr);
-    if (rc == 0) {
-        const hipError_t se = hipDeviceSynchronize();
-        if (se != hipSuccess) {
-            set_error(std::string("kernel execution: ") + hipGetErrorString(se));
-            rc = -1;
-        }
+    int rc = real_device(side, howmny, select, n, dS, n, dP, n, ar, ai, be, left ? dVL : nullptr, n,
+                         right ? dVR : nullptr, n, st);
+    if (rc == 0 && !hip_ok(hipDeviceSynchronize(), "kernel execution")) {
+        rc = -1;
     }
     if (rc != 0) {
         fail(info, "device path");
         return;
     }
-    if ((right && get(VR, ldvr, g_dVR.p, n, num_sel) != hipSuccess) ||
-        (left && get(VL, ldvl, g_dVL.p, n, num_sel) != hipSuccess)) {
-        set_error("device to host copy failed");
+    if ((right && !download(VR, ldvr, dVR, n, num_sel, 1, st)) || (left && !download(VL, ldvl, dVL, n, num_sel, 1, st))) {
         fail(info, "staging");
         return;
     }
 }
 
-
-// Host-pointer driver for the complex128 interface.
-void complex_host(char side, char howmny, const int *select, int n, const std::complex<double> *S, int lds,
-                  const std::complex<double> *P, int ldp, const std::complex<double> *al,
-                  const std::complex<double> *be, std::complex<double> *VL, int ldvl, std::complex<double> *VR,
-                  int ldvr, int mm, int *m, std::complex<double> *work, int lwork, int *info, bool have_work)
+// Host-pointer driver for the complex interfaces.  T (double, or float for ctgevc3) is the
+// scalar type of the host std::complex<T> arrays; the device computes in complex128.
+template <class T>
+void complex_host(char side, char howmny, const int *select, int n, const std::complex<T> *S, int lds,
+                  const std::complex<T> *P, int ldp, const std::complex<double> *al, const std::complex<double> *be,
+                  std::complex<T> *VL, int ldvl, std::complex<T> *VR, int ldvr, int mm, int *m, std::complex<T> *work,
+                  int lwork, int *info)
 {
     typedef std::complex<double> zt;
     std::vector<Unit> units;
     int num_sel = (howmny == 'A' || howmny == 'B' || howmny == 'S') && n > 0 ? complex_units(howmny, select, n, units)
                                                                              : 0;
-    if (!check_args<zt>(howmny, n, num_sel, mm, m, work, lwork, info, have_work)) {
+    if (!check_args<std::complex<T>>(howmny, n, num_sel, mm, m, work, lwork, info, true)) {
         return;
     }
     bool right = (side == 'R' || side == 'B') && VR != nullptr;
@@ -144,6 +345,7 @@ void complex_host(char side, char howmny, const int *select, int n, const std::c
     }
     bool back = howmny == 'B';
     std::lock_guard<std::mutex> lock(g_stage_mutex);
+    const hipStream_t st = nullptr;
     size_t nn = (size_t)n * n * sizeof(zt);
     size_t nm = (size_t)n * (back ? n : num_sel) * sizeof(zt);
     if (!g_dS.ensure(nn) || !g_dP.ensure(nn) || (right && !g_dVR.ensure(nm)) || (left && !g_dVL.ensure(nm))) {
@@ -151,35 +353,26 @@ void complex_host(char side, char howmny, const int *select, int n, const std::c
         fail(info, "staging");
         return;
     }
-    if (put(g_dS.p, S, n, n, lds) != hipSuccess || put(g_dP.p, P, n, n, ldp) != hipSuccess) {
-        set_error("host to device copy failed");
+    double *dS = (double *)g_dS.p, *dP = (double *)g_dP.p, *dVL = (double *)g_dVL.p, *dVR = (double *)g_dVR.p;
+    const T *hS = reinterpret_cast<const T *>(S), *hP = reinterpret_cast<const T *>(P);
+    T *hVL = reinterpret_cast<T *>(VL), *hVR = reinterpret_cast<T *>(VR);
+    if (!upload(dS, hS, n, n, lds, 2, st) || !upload(dP, hP, n, n, ldp, 2, st) ||
+        (back && right && !upload(dVR, hVR, n, n, ldvr, 2, st)) ||
+        (back && left && !upload(dVL, hVL, n, n, ldvl, 2, st))) {
         fail(info, "staging");
         return;
     }
-    if (back) {
-        if ((right && put(g_dVR.p, VR, n, n, ldvr) != hipSuccess) ||
-            (left && put(g_dVL.p, VL, n, n, ldvl) != hipSuccess)) {
-            set_error("host to device copy failed");
-            fail(info, "staging");
-            return;
-        }
-    }
-    int rc = complex_device(side, howmny, select, n, (const zt *)g_dS.p, n, (const zt *)g_dP.p, n, al, be,
-                            left ? (zt *)g_dVL.p : nullptr, n, right ? (zt *)g_dVR.p : nullptr, n, nullptr);
-    if (rc == 0) {
-        const hipError_t se = hipDeviceSynchronize();
-        if (se != hipSuccess) {
-            set_error(std::string("kernel execution: ") + hipGetErrorString(se));
-            rc = -1;
-        }
+    int rc = complex_device(side, howmny, select, n, (const zt *)dS, n, (const zt *)dP, n, al, be,
+                            left ? (zt *)dVL : nullptr, n, right ? (zt *)dVR : nullptr, n, st);
+    if (rc == 0 && !hip_ok(hipDeviceSynchronize(), "kernel execution")) {
+        rc = -1;
     }
     if (rc != 0) {
         fail(info, "device path");
         return;
     }
-    if ((right && get(VR, ldvr, g_dVR.p, n, num_sel) != hipSuccess) ||
-        (left && get(VL, ldvl, g_dVL.p, n, num_sel) != hipSuccess)) {
-        set_error("device to host copy failed");
+    if ((right && !download(hVR, ldvr, dVR, n, num_sel, 2, st)) ||
+        (left && !download(hVL, ldvl, dVL, n, num_sel, 2, st))) {
         fail(info, "staging");
         return;
     }
@@ -201,8 +394,8 @@ void xtgevc3_dtgevc3(char side, char howmny, const int *select, int n, const dou
                      int ldp, const double *alphar, const double *alphai, const double *beta, double *VL, int ldvl,
                      double *VR, int ldvr, int mm, int *m, double *work, int lwork, int *info)
 {
-    real_host(side, howmny, select, n, S, lds, P, ldp, alphar, alphai, beta, VL, ldvl, VR, ldvr, mm, m, work, lwork,
-              info, true);
+    real_host<double>(side, howmny, select, n, S, lds, P, ldp, alphar, alphai, beta, VL, ldvl, VR, ldvr, mm, m, work,
+                      lwork, info);
 }
 
 void xtgevc3_stgevc3(char side, char howmny, const int *select, int n, const float *S, int lds, const float *P,
@@ -211,64 +404,16 @@ void xtgevc3_stgevc3(char side, char howmny, const int *select, int n, const flo
 {
     // FP32 interface: same argument semantics (stgevc3.cpp:1122-1242); computed on the FP64
     // path and rounded back, so its vectors are at least as accurate as the FP32 reference's.
-    std::vector<xtg::Unit> units;
-    std::vector<double> ai_d(n > 0 ? n : 0);
-    for (int k = 0; k < n; ++k) {
+    // Matrices cross PCIe in FP32 and are widened and rounded on the device.
+    const int nv = n > 0 ? n : 0;
+    std::vector<double> ar_d(nv), ai_d(nv), be_d(nv);
+    for (int k = 0; k < nv; ++k) {
+        ar_d[k] = alphar[k];
         ai_d[k] = alphai[k];
+        be_d[k] = beta[k];
     }
-    int num_sel = (howmny == 'A' || howmny == 'B' || howmny == 'S') && n > 0
-                      ? xtg::real_units(howmny, select, n, ai_d.data(), units)
-                      : 0;
-    if (!xtg::check_args<float>(howmny, n, num_sel, mm, m, work, lwork, info, true)) {
-        return;
-    }
-    bool right = (side == 'R' || side == 'B') && VR != nullptr;
-    bool left = (side == 'L' || side == 'B') && VL != nullptr;
-    if (num_sel == 0 || (!right && !left)) {
-        return;
-    }
-    bool back = howmny == 'B';
-    std::vector<double> Sd((size_t)n * n), Pd((size_t)n * n), ar_d(n), be_d(n);
-    for (int c = 0; c < n; ++c) {
-        for (int r = 0; r < n; ++r) {
-            Sd[r + (size_t)c * n] = S[r + (size_t)c * lds];
-            Pd[r + (size_t)c * n] = P[r + (size_t)c * ldp];
-        }
-        ar_d[c] = alphar[c];
-        be_d[c] = beta[c];
-    }
-    int vcols = back ? n : num_sel;
-    std::vector<double> VLd(left ? (size_t)n * vcols : 0), VRd(right ? (size_t)n * vcols : 0);
-    if (back) {
-        for (int c = 0; c < n; ++c) {
-            for (int r = 0; r < n; ++r) {
-                if (left) {
-                    VLd[r + (size_t)c * n] = VL[r + (size_t)c * ldvl];
-                }
-                if (right) {
-                    VRd[r + (size_t)c * n] = VR[r + (size_t)c * ldvr];
-                }
-            }
-        }
-    }
-    int md = 0, infod = 0;
-    double wq = 0.0;
-    real_host(side, howmny, select, n, Sd.data(), n, Pd.data(), n, ar_d.data(), ai_d.data(), be_d.data(),
-              left ? VLd.data() : nullptr, n, right ? VRd.data() : nullptr, n, vcols, &md, &wq, 0, &infod, false);
-    if (infod != 0) {
-        *info = infod;
-        return;
-    }
-    for (int c = 0; c < num_sel; ++c) {
-        for (int r = 0; r < n; ++r) {
-            if (left) {
-                VL[r + (size_t)c * ldvl] = (float)VLd[r + (size_t)c * n];
-            }
-            if (right) {
-                VR[r + (size_t)c * ldvr] = (float)VRd[r + (size_t)c * n];
-            }
-        }
-    }
+    real_host<float>(side, howmny, select, n, S, lds, P, ldp, ar_d.data(), ai_d.data(), be_d.data(), VL, ldvl, VR,
+                     ldvr, mm, m, work, lwork, info);
 }
 
 int xtgevc3_dtgevc3_device_ex(char side, char howmny, const int *select, int n, const double *S, int lds,
@@ -313,8 +458,8 @@ void xtgevc3_ztgevc3(char side, char howmny, const int *select, int n, const xtg
                      xtgevc3_zcomplex *VL, int ldvl, xtgevc3_zcomplex *VR, int ldvr, int mm, int *m,
                      xtgevc3_zcomplex *work, int lwork, int *info)
 {
-    complex_host(side, howmny, select, n, S, lds, P, ldp, alpha, beta, VL, ldvl, VR, ldvr, mm, m, work, lwork, info,
-                 true);
+    complex_host<double>(side, howmny, select, n, S, lds, P, ldp, alpha, beta, VL, ldvl, VR, ldvr, mm, m, work, lwork,
+                         info);
 }
 
 void xtgevc3_ctgevc3(char side, char howmny, const int *select, int n, const xtgevc3_ccomplex *S, int lds,
@@ -323,62 +468,16 @@ void xtgevc3_ctgevc3(char side, char howmny, const int *select, int n, const xtg
                      xtgevc3_ccomplex *work, int lwork, int *info)
 {
     // complex64 interface: argument semantics of ctgevc3.cpp:435-516, computed on the
-    // complex128 path and rounded back.
+    // complex128 path and rounded back; matrices cross PCIe in complex64.
     typedef std::complex<double> zt;
-    std::vector<xtg::Unit> units;
-    int num_sel = (howmny == 'A' || howmny == 'B' || howmny == 'S') && n > 0
-                      ? xtg::complex_units(howmny, select, n, units)
-                      : 0;
-    if (!xtg::check_args<xtgevc3_ccomplex>(howmny, n, num_sel, mm, m, work, lwork, info, true)) {
-        return;
+    const int nv = n > 0 ? n : 0;
+    std::vector<zt> al(nv), be(nv);
+    for (int k = 0; k < nv; ++k) {
+        al[k] = zt(alpha[k]);
+        be[k] = zt(beta[k]);
     }
-    bool right = (side == 'R' || side == 'B') && VR != nullptr;
-    bool left = (side == 'L' || side == 'B') && VL != nullptr;
-    if (num_sel == 0 || (!right && !left)) {
-        return;
-    }
-    bool back = howmny == 'B';
-    std::vector<zt> Sd((size_t)n * n), Pd((size_t)n * n), al(n), be(n);
-    for (int c = 0; c < n; ++c) {
-        for (int r = 0; r < n; ++r) {
-            Sd[r + (size_t)c * n] = zt(S[r + (size_t)c * lds]);
-            Pd[r + (size_t)c * n] = zt(P[r + (size_t)c * ldp]);
-        }
-        al[c] = zt(alpha[c]);
-        be[c] = zt(beta[c]);
-    }
-    int vcols = back ? n : num_sel;
-    std::vector<zt> VLd(left ? (size_t)n * vcols : 0), VRd(right ? (size_t)n * vcols : 0);
-    if (back) {
-        for (int c = 0; c < n; ++c) {
-            for (int r = 0; r < n; ++r) {
-                if (left) {
-                    VLd[r + (size_t)c * n] = zt(VL[r + (size_t)c * ldvl]);
-                }
-                if (right) {
-                    VRd[r + (size_t)c * n] = zt(VR[r + (size_t)c * ldvr]);
-                }
-            }
-        }
-    }
-    int md = 0, infod = 0;
-    zt wq;
-    complex_host(side, howmny, select, n, Sd.data(), n, Pd.data(), n, al.data(), be.data(), left ? VLd.data() : nullptr,
-                 n, right ? VRd.data() : nullptr, n, vcols, &md, &wq, 0, &infod, false);
-    if (infod != 0) {
-        *info = infod;
-        return;
-    }
-    for (int c = 0; c < num_sel; ++c) {
-        for (int r = 0; r < n; ++r) {
-            if (left) {
-                VL[r + (size_t)c * ldvl] = xtgevc3_ccomplex(VLd[r + (size_t)c * n]);
-            }
-            if (right) {
-                VR[r + (size_t)c * ldvr] = xtgevc3_ccomplex(VRd[r + (size_t)c * n]);
-            }
-        }
-    }
+    complex_host<float>(side, howmny, select, n, S, lds, P, ldp, al.data(), be.data(), VL, ldvl, VR, ldvr, mm, m, work,
+                        lwork, info);
 }
 
 int xtgevc3_ztgevc3_device(char side, char howmny, const int *select, int n, const xtgevc3_zcomplex *S, int lds,
