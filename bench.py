@@ -184,8 +184,20 @@ def backward_ratio(seed, n=2048, nsel=256, nbatch=32, reps=5):
     def bwd1():
         X.dtgevc3_backward_device(dS, dP, ar, ai, be, sel, Vt, G1, nbatch=1, dS=gS1, dP=gP1)
 
+    Vm = torch.zeros_like(Vt)
+    gSm = torch.zeros_like(gS)
+    gPm = torch.zeros_like(gP)
+
+    def fwd_mixed():
+        m, info = X.dtgevc3_device("R", "S", sel, dS, dP, ar, ai, be, VR=Vm, mm=nsel, precision=X.PREC_SPLIT_I8)
+        assert (m, info) == (nsel, 0), (m, info)
+
+    def bwd_mixed():
+        X.dtgevc3_backward_device(dS, dP, ar, ai, be, sel, Vm, Gt, nbatch=nbatch, dS=gSm, dP=gPm,
+                                  precision=X.PREC_SPLIT_I8)
+
     out = {}
-    for name, f in (("fwd", fwd), ("bwd", bwd), ("bwd1", bwd1)):
+    for name, f in (("fwd", fwd), ("bwd", bwd), ("bwd1", bwd1), ("fwd_mixed", fwd_mixed), ("bwd_mixed", bwd_mixed)):
         f()
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -193,10 +205,58 @@ def backward_ratio(seed, n=2048, nsel=256, nbatch=32, reps=5):
             f()
         torch.cuda.synchronize()
         out[name] = 1000.0 * (time.perf_counter() - t) / reps
+    # on N(0,1) off-diagonals the exact gradients of most selected eigenvectors exceed the
+    # FP64 range at this size (triangular-solve growth): count what is representable
+    nonfinite = float((~torch.isfinite(gS)).sum() + (~torch.isfinite(gP)).sum()) / (2.0 * gS.numel())
     return {"n": n, "selected": nsel, "batches": nbatch, "fwd_ms": round(out["fwd"], 3),
             "bwd_ms": round(out["bwd"], 3), "bwd_over_fwd": round(out["bwd"] / out["fwd"], 3),
             "bwd_per_batch_over_fwd": round(out["bwd"] / nbatch / out["fwd"], 4),
-            "bwd_one_batch_ms": round(out["bwd1"], 3), "bwd_one_batch_over_fwd": round(out["bwd1"] / out["fwd"], 3)}
+            "bwd_one_batch_ms": round(out["bwd1"], 3), "bwd_one_batch_over_fwd": round(out["bwd1"] / out["fwd"], 3),
+            "fp64_grad_nonfinite_fraction": round(nonfinite, 4),
+            "mixed": {"form": "split-int8", "fwd_ms": round(out["fwd_mixed"], 3),
+                      "bwd_ms": round(out["bwd_mixed"], 3),
+                      "bwd_over_fwd": round(out["bwd_mixed"] / out["fwd_mixed"], 3)},
+            "accuracy": backward_accuracy(seed, n, nsel)}
+
+
+def backward_accuracy(seed, n=2048, nsel=256, nbatch=4, offdiag=0.003):
+    """Gradient error of the mixed forms (forward and backward both mixed) against the FP64
+    forward + backward, on a pencil whose FP64 gradients carry digits: the same construction
+    with off-diagonals scaled by `offdiag`, the nsel real eigenvalues farthest from all others
+    selected.  Errors are max |g - g_fp64| / max |g_fp64| over dL/dS and dL/dP."""
+    import torch
+
+    import xtgevc3_amd as X
+    from xtgevc3_amd import pencils as G
+
+    S, P, ar, ai, be = G.real_quasi_triangular(n, seed, offdiag=offdiag)
+    lam = (ar + 1j * ai) / be
+    real_k = np.array([k for k in range(n) if ai[k] == 0.0 and (k == 0 or ai[k - 1] == 0.0)])
+    gap = np.array([np.partition(np.abs(lam - lam[k]), 1)[1] for k in real_k])
+    pick = np.sort(real_k[np.argsort(-gap)[:nsel]])
+    sel = np.zeros(n, dtype=np.int32)
+    sel[pick] = 1
+    dS = torch.from_numpy(np.ascontiguousarray(S.T)).cuda()
+    dP = torch.from_numpy(np.ascontiguousarray(P.T)).cuda()
+    gen = torch.Generator("cuda").manual_seed(seed + 1)
+    Gt = torch.randn((nbatch * nsel, n), dtype=torch.float64, device="cuda", generator=gen)
+    res = {}
+    for p in (X.PREC_FP64, X.PREC_SPLIT_I8, X.PREC_MIXED_F32):
+        Vt = torch.zeros((nsel, n), dtype=torch.float64, device="cuda")
+        m, info = X.dtgevc3_device("R", "S", sel, dS, dP, ar, ai, be, VR=Vt, mm=nsel, precision=p)
+        assert (m, info) == (nsel, 0), (m, info)
+        gS, gP = X.dtgevc3_backward_device(dS, dP, ar, ai, be, sel, Vt, Gt, nbatch=nbatch, precision=p)
+        res[p] = (Vt, torch.cat([gS.flatten(), gP.flatten()]))
+    V0, g0 = res[X.PREC_FP64]
+    scale = float(g0.abs().max())
+    out = {"pencil": f"configs construction, off-diagonals x{offdiag}, {nsel} best-separated real eigenvalues, "
+                     f"{nbatch} cotangent batches", "fp64_grad_max": float("%.4g" % scale),
+           "fp64_grad_finite": bool(torch.isfinite(g0).all())}
+    for p, name in ((X.PREC_SPLIT_I8, "split_int8"), (X.PREC_MIXED_F32, "fp32_products")):
+        Vm, gm = res[p]
+        out[name] = {"max_rel_grad_err_vs_fp64": float("%.3g" % (float((gm - g0).abs().max()) / scale)),
+                     "max_abs_V_err_vs_fp64": float("%.3g" % float((Vm - V0).abs().max()))}
+    return out
 
 
 def main():
@@ -406,7 +466,9 @@ def main():
         bw = backward_ratio(args.seed)
         log(f"[bench] backward n={bw['n']} sel={bw['selected']} x{bw['batches']} batches: fwd {bw['fwd_ms']} ms, "
             f"bwd {bw['bwd_ms']} ms (all batches), ratio {bw['bwd_over_fwd']} ({bw['bwd_per_batch_over_fwd']} per batch); "
-            f"one batch {bw['bwd_one_batch_ms']} ms, ratio {bw['bwd_one_batch_over_fwd']}")
+            f"one batch {bw['bwd_one_batch_ms']} ms, ratio {bw['bwd_one_batch_over_fwd']}; "
+            f"split-int8 fwd {bw['mixed']['fwd_ms']} ms, bwd {bw['mixed']['bwd_ms']} ms; "
+            f"FP64 gradient non-finite fraction {bw['fp64_grad_nonfinite_fraction']}; accuracy {bw['accuracy']}")
 
     cpu = None
     if not args.no_cpu:

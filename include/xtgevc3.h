@@ -110,6 +110,17 @@ int xtgevc3_dtgevc3_backward_device(int n, const double *S, int lds, const doubl
                                     int ldv, const double *G, int ldg, int nbatch, double *dS, int ldds, double *dP,
                                     int lddp, int *info, void *stream);
 
+/* xtgevc3_dtgevc3_backward_device with a precision argument (no counterpart in the reference):
+ * XTGEVC3_PREC_FP64 as above; XTGEVC3_PREC_SPLIT_I8 / XTGEVC3_PREC_MIXED_F32 run the far
+ * updates of the adjoint block sweep as in the corresponding forms of
+ * xtgevc3_dtgevc3_device_ex (the diagonal-block solves, near updates, and the outer products
+ * that assemble dS and dP stay FP64).  Other values: XTGEVC3_INFO_UNSUPPORTED, nothing written. */
+int xtgevc3_dtgevc3_backward_device_ex(int n, const double *S, int lds, const double *P, int ldp,
+                                       const double *alphar, const double *alphai, const double *beta,
+                                       const int *select, const double *V, int ldv, const double *G, int ldg,
+                                       int nbatch, double *dS, int ldds, double *dP, int lddp, int *info, void *stream,
+                                       int precision);
+
 /* Live kernel timing (no counterpart in the reference).  When enabled, every launch of the
  * hot kernels is bracketed by HIP events on its stream; xtgevc3_profile_read synchronises,
  * returns per-kind totals (milliseconds, launches, algorithmic flops and bytes) and resets.

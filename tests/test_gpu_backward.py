@@ -126,3 +126,59 @@ def test_backward_rejects_complex_selection():
     Vt = torch.zeros((2, n), dtype=torch.float64, device="cuda")
     with pytest.raises(ValueError):
         X.dtgevc3_backward_device(dS, dP, ar, ai, be, sel, Vt, Vt.clone(), nbatch=1)
+
+
+@pytest.mark.parametrize("prec,tolf", [(3, 100.0), (1, 1e9)])
+def test_backward_mixed_matches_fp64(prec, tolf):
+    """Mixed forward + mixed backward (far updates of both sweeps as split-int8 digit products,
+    or as FP32 products) against the FP64 forward + backward, at a size with far updates
+    (n = 400: one group of four 64-row blocks, then far updates), for the 32 best separated
+    real eigenvalues.  Bound: tolf * eps * cond_max of the selected eigenvectors, relative to
+    the gradient's largest entry."""
+    import torch
+
+    import xtgevc3_amd as X
+
+    n, nb, nsel = 400, 3, 32
+    S, P, ar, ai, be = G.real_quasi_triangular(n, 12, offdiag=0.02)
+    lam = (ar + 1j * ai) / be
+    real_k = np.array([k for k in range(n) if ai[k] == 0.0 and (k == 0 or ai[k - 1] == 0.0)])
+    gap = np.array([np.min(np.abs(np.delete(lam, k) - lam[k])) for k in real_k])
+    pick = np.sort(real_k[np.argsort(-gap)[:nsel]])
+    sel = np.zeros(n, dtype=np.int32)
+    sel[pick] = 1
+    kind, idx = O.real_layout("S", sel, ai)
+    al, bb = O.eig_alpha_beta(ar, ai, be, idx)
+    cond = O.eigvec_cond(S, P, al, bb, kind, idx).max()
+    dS = X.fortran_device(S)
+    dP = X.fortran_device(P)
+    rng = np.random.default_rng(17)
+    Gt = torch.tensor(rng.standard_normal((nb * nsel, n)), device="cuda")
+    out = {}
+    for p in (X.PREC_FP64, prec):
+        Vt = torch.zeros((nsel, n), dtype=torch.float64, device="cuda")
+        m, info = X.dtgevc3_device("R", "S", sel, dS, dP, ar, ai, be, VR=Vt, mm=nsel, precision=p)
+        assert (m, info) == (nsel, 0)
+        gS, gP = X.dtgevc3_backward_device(dS, dP, ar, ai, be, sel, Vt, Gt, nbatch=nb, precision=p)
+        out[p] = (Vt.cpu().numpy(), gS.cpu().numpy(), gP.cpu().numpy())
+    V64, gS64, gP64 = out[X.PREC_FP64]
+    Vm, gSm, gPm = out[prec]
+    scale = max(np.abs(gS64).max(), np.abs(gP64).max())
+    err = max(np.abs(gSm - gS64).max(), np.abs(gPm - gP64).max()) / scale
+    bound = tolf * np.finfo(float).eps * cond
+    assert np.isfinite(err) and err <= bound, (err, bound, cond)
+    assert np.abs(Vm - V64).max() <= bound, (np.abs(Vm - V64).max(), bound)
+
+
+def test_backward_rejects_unknown_precision():
+    import torch
+
+    import xtgevc3_amd as X
+
+    n = 20
+    S, P, ar, ai, be, sel, pick = _problem(n, 3, 5)
+    dS = X.fortran_device(S)
+    dP = X.fortran_device(P)
+    Vt = torch.zeros((len(pick), n), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        X.dtgevc3_backward_device(dS, dP, ar, ai, be, sel, Vt, Vt.clone(), nbatch=1, precision=7)

@@ -169,6 +169,33 @@ def test_split_i8_large_residual(side):
     assert res.max() <= 10.0 * n * EPS
 
 
+# Backward-error floor of the split-int8 form: digit products carry ~40 bits relative to the
+# operands' row / column maxima, so metric A residuals reach ~1e-11 on pencils whose norm is
+# small next to those maxima (measured 4e-12 .. 1e-11 for diagonally dominant pencils,
+# n = 1100 .. 4096); 10 n eps is below that floor up to n ~ 4500.
+SPLIT_RES_FLOOR = 2.5e-11
+
+
+@pytest.mark.parametrize("side", ["R", "L"])
+def test_split_i8_small_offdiag_residual(side):
+    """Off-diagonals 100x below the diagonal (n = 1100, several far updates).  The digits of a
+    pencil row are scaled to the columns its far updates read, not to the diagonal: scaling by
+    the whole row gave ~5e-11 (110-230 n eps at n = 2048) from the far updates alone."""
+    import xtgevc3_amd as X
+
+    n = 1100
+    S, P, ar, ai, be = G.real_quasi_triangular(n, 45, offdiag=0.01)
+    Z = G.random_orthogonal(n, 46)
+    g = _run(S, P, ar, ai, be, Z, X.PREC_SPLIT_I8, side)
+    assert np.all(np.isfinite(g))
+    kind, idx = O.real_layout("B", None, ai)
+    al, bb = O.eig_alpha_beta(ar, ai, be, idx)
+    gc = O.complex_columns(g, kind)
+    res = O.residual_A(Z @ S @ Z.T, Z @ P @ Z.T, gc, al, bb, left=side == "L")
+    print(f"\n[split-i8 n={n} {side} offdiag 0.01] max residual {res.max():.3e} = {res.max() / (n * EPS):.3g} n*eps")
+    assert res.max() <= max(10.0 * n * EPS, SPLIT_RES_FLOOR)
+
+
 def _run_cplx_split(S, P, al, be, Q, Z, side, select=None, howmny="B"):
     """ztgevc3 in the split-int8 form through the device C ABI; returns (VL, VR, m, info)."""
     import torch
@@ -240,3 +267,23 @@ def test_split_i8_complex_large_residual():
         res = O.residual_A(A, B, g, al, be, left=left)
         print(f"\n[split-i8 z n={n} {'L' if left else 'R'}] max residual {res.max() / (n * EPS):.3g} n*eps")
         assert res.max() <= 10.0 * n * EPS
+
+
+def test_split_i8_complex_small_offdiag_residual():
+    """Complex pencil with off-diagonals 100x below the diagonal, n = 1100, both sides, row
+    digits scaled to each row's far-update columns: residual within the split form's floor."""
+    n = 1100
+    S, P, al, be = G.complex_triangular(n, 47)
+    S = np.diag(np.diag(S)) + 0.01 * np.triu(S, 1)
+    P = np.diag(np.diag(P)) + 0.01 * np.triu(P, 1)
+    Z = G.random_unitary(n, 48)
+    gL, gR, m, info = _run_cplx_split(S, P, al, be, Z, Z, "B")
+    assert (m, info) == (n, 0)
+    A = Z @ S @ Z.conj().T
+    B = Z @ P @ Z.conj().T
+    for g, left in ((gR, False), (gL, True)):
+        assert np.all(np.isfinite(g))
+        res = O.residual_A(A, B, g, al, be, left=left)
+        print(f"\n[split-i8 z n={n} {'L' if left else 'R'} offdiag 0.01] max residual {res.max():.3e} = "
+              f"{res.max() / (n * EPS):.3g} n*eps")
+        assert res.max() <= max(10.0 * n * EPS, SPLIT_RES_FLOOR)

@@ -46,6 +46,7 @@ EXPORTED_SYMBOLS = [
     "xtgevc3_dtgevc3", "xtgevc3_stgevc3", "xtgevc3_ztgevc3", "xtgevc3_ctgevc3",
     "xtgevc3_dtgevc3_device", "xtgevc3_dtgevc3_device_ex", "xtgevc3_ztgevc3_device", "xtgevc3_ztgevc3_device_ex",
     "xtgevc3_profile_enable", "xtgevc3_profile_read", "xtgevc3_dtgevc3_backward_device",
+    "xtgevc3_dtgevc3_backward_device_ex",
 ]
 
 PROF_KINDS = ("solve", "update", "back", "other", "update_128x128", "update_128x64", "update_64x128",
@@ -229,11 +230,13 @@ def ztgevc3_device(side, howmny, select, S, P, alpha, beta, VL=None, VR=None, mm
                          precision=precision)
 
 
-def dtgevc3_backward_device(S, P, alphar, alphai, beta, select, V, G, nbatch=1, dS=None, dP=None, stream=None):
+def dtgevc3_backward_device(S, P, alphar, alphai, beta, select, V, G, nbatch=1, dS=None, dP=None, stream=None,
+                            precision=PREC_FP64):
     """Backward of ``dtgevc3_device('R', 'S' or 'A', ...)`` for real selected eigenvalues.
     S, P, V, G, dS, dP: torch tensors holding column-major storage (shape (cols, ld)).
     G holds nbatch * m columns.  Returns (dS, dP) with nbatch stacked n x n matrices each
-    (tensors of shape (nbatch * n, n) in column-major storage)."""
+    (tensors of shape (nbatch * n, n) in column-major storage).  precision: PREC_FP64,
+    PREC_SPLIT_I8 or PREC_MIXED_F32 (far updates of the adjoint sweep in that form)."""
     import torch
 
     c = ctypes
@@ -247,10 +250,12 @@ def dtgevc3_backward_device(S, P, alphar, alphai, beta, select, V, G, nbatch=1, 
     info = c.c_int(-99)
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
-    rc = _lib.xtgevc3_dtgevc3_backward_device(
+    rc = _lib.xtgevc3_dtgevc3_backward_device_ex(
         c.c_int(n), _tptr(S), c.c_int(S.shape[1]), _tptr(P), c.c_int(P.shape[1]), *[_ptr(e) for e in ev], _ptr(sel),
         _tptr(V), c.c_int(V.shape[1]), _tptr(G), c.c_int(G.shape[1]), c.c_int(nbatch), _tptr(dS), c.c_int(n),
-        _tptr(dP), c.c_int(n), c.byref(info), c.c_void_p(stream))
+        _tptr(dP), c.c_int(n), c.byref(info), c.c_void_p(stream), c.c_int(precision))
+    if rc == INFO_UNSUPPORTED:
+        raise ValueError("xtgevc3: " + last_error())
     if rc != 0:
         raise RuntimeError("xtgevc3 backward failed: " + last_error())
     if info.value != 0:

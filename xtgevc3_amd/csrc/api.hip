@@ -522,8 +522,26 @@ int xtgevc3_dtgevc3_backward_device(int n, const double *S, int lds, const doubl
                                     int ldv, const double *G, int ldg, int nbatch, double *dS, int ldds, double *dP,
                                     int lddp, int *info, void *stream)
 {
+    return xtgevc3_dtgevc3_backward_device_ex(n, S, lds, P, ldp, alphar, alphai, beta, select, V, ldv, G, ldg, nbatch,
+                                              dS, ldds, dP, lddp, info, stream, XTGEVC3_PREC_FP64);
+}
+
+int xtgevc3_dtgevc3_backward_device_ex(int n, const double *S, int lds, const double *P, int ldp,
+                                       const double *alphar, const double *alphai, const double *beta,
+                                       const int *select, const double *V, int ldv, const double *G, int ldg,
+                                       int nbatch, double *dS, int ldds, double *dP, int lddp, int *info, void *stream,
+                                       int precision)
+{
+    if (precision != XTGEVC3_PREC_FP64 && precision != XTGEVC3_PREC_MIXED_F32 &&
+        precision != XTGEVC3_PREC_SPLIT_I8) {
+        xtg::set_error("precision " + std::to_string(precision) + " is not available for the backward");
+        if (info) {
+            *info = XTGEVC3_INFO_UNSUPPORTED;
+        }
+        return XTGEVC3_INFO_UNSUPPORTED;
+    }
     int rc = xtg::real_backward_device(n, S, lds, P, ldp, alphar, alphai, beta, select, V, ldv, G, ldg, nbatch, dS,
-                                       ldds, dP, lddp, info, (hipStream_t)stream);
+                                       ldds, dP, lddp, info, (hipStream_t)stream, precision);
     if (rc != 0) {
         return xtg::fail(info, "backward");
     }

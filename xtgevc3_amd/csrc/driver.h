@@ -39,7 +39,7 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
 int real_backward_device(int n, const double *S, long lds, const double *P, long ldp, const double *ar,
                          const double *ai, const double *be, const int *select, const double *V, long ldv,
                          const double *G, long ldg, int nbatch, double *dS, long ldds, double *dP, long lddp, int *info,
-                         hipStream_t st);
+                         hipStream_t st, int prec = 0);
 
 // live timing of kernel launches (profile.hip); begin returns -1 when disabled
 int prof_begin(hipStream_t st);

@@ -27,7 +27,7 @@ __global__ void kz_solve_units(const double2 *, long, const double2 *, long, int
                                GroupInfo, SplitOut);
 __global__ void k_slice_digits(const double *, long, int, int, const double *, long, int, int, int, int, const int *,
                                signed char *, long, long, int, signed char *);
-__global__ void kz_rowexp(const double2 *, long, const double2 *, long, int, int, int *);
+__global__ void kz_rowexp(const double2 *, long, const double2 *, long, int, int, int *, const int *);
 __global__ void k_fill_int(int *, long, int);
 __global__ void k_xexp_final(const int *, const int *, long, const int *, const int *, int, int, int *);
 int launch_gemm_i8(hipStream_t st, int M, int N, int K1, int K2, const signed char *L1, const signed char *L2,
@@ -104,7 +104,7 @@ struct ZWork {
     DBuf X, W, sig, sig_grp, colexp, blkexp, flag, desc, outcol, kmax, rowstep, Sw, Pw, Zt, norms;
     // split-int8 form: pencil digits expanded as (re, -im) and (im, re), group operand, X and Z
     // digits, exponents
-    DBuf Sx, Sy, Px, Py, rowexp, Wg, wgexp, wbexp, xbexp, sstart, xexp, Xx, Xy, Zq, zexp, kmax2;
+    DBuf Sx, Sy, Px, Py, rowexp, Wg, wgexp, wbexp, xbexp, sstart, xexp, Xx, Xy, Zq, zexp, kmax2, cfar;
 };
 
 std::mutex g_zmutex;
@@ -213,10 +213,22 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
             sstart[p] = rowstep[desc[p].d];
         }
         XTG_CHECK(hipMemcpyAsync(ws.sstart.p, sstart.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+        // row exponents over each row's far-update region only: columns from the bottom edge of
+        // the row's group on (the far updates of a row read no other columns; digits left of
+        // it may saturate and only meet zero digits of the group operand)
+        std::vector<int> cfar(n, n);
+        for (int s = 0; s < nsteps; ++s) {
+            const int gs = s / ZGROUP * ZGROUP;
+            for (int r = i0s[s]; r < i0s[s] + nbs[s]; ++r) {
+                cfar[r] = i0s[gs] + nbs[gs];
+            }
+        }
+        ZENSURE(ws.cfar, sizeof(int) * (size_t)n);
+        XTG_CHECK(hipMemcpyAsync(ws.cfar.p, cfar.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
         int pr = prof_begin(st);
         hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.rowexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
         hipLaunchKernelGGL(kz_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, Sw, ldsw, Pw, ldpw, n, n,
-                           ws.rowexp.as<int>()); XTG_LAUNCH_CHECK("kz_rowexp");
+                           ws.rowexp.as<int>(), (const int *)ws.cfar.as<int>()); XTG_LAUNCH_CHECK("kz_rowexp");
         const dim3 grid((n + 63) / 64, (unsigned)((qld + 15) / 16));
         hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, (const double *)Sw, ldsw, 0, n,
                            (const double *)nullptr, 0L, 0, 0, n, (int)qld, (const int *)ws.rowexp.as<int>(),
@@ -365,7 +377,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
         int pr = prof_begin(st);
         hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.zexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
         hipLaunchKernelGGL(kz_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, Rp, ldr,
-                           (const double2 *)nullptr, 0L, n, n, ws.zexp.as<int>()); XTG_LAUNCH_CHECK("kz_rowexp");
+                           (const double2 *)nullptr, 0L, n, n, ws.zexp.as<int>(), (const int *)nullptr); XTG_LAUNCH_CHECK("kz_rowexp");
         const dim3 gx((m + 63) / 64, (kb + 15) / 16), gz((n + 63) / 64, (kb + 15) / 16);
         hipLaunchKernelGGL(k_slice_digits, gx, dim3(256), 0, st, (const double *)X, ldx, 0, n, (const double *)nullptr,
                            0L, 0, 0, m, kb, (const int *)ws.xexp.as<int>(), ws.Xx.as<signed char>(), (long)m * kb,

@@ -33,7 +33,7 @@ template <int MAXB>
 __global__ void k_solve_real(const double *, long, const double *, long, int, int, unsigned long long, double *, long,
                              double *, long, const UnitDesc *, const int4 *, int, int, int, int, double *, int *, int *,
                              int *, int, int, double *, int, GroupInfo, MixedExp, SplitOut);
-__global__ void k_rowexp(const double *, long, const double *, long, int, int, int *);
+__global__ void k_rowexp(const double *, long, const double *, long, int, int, int *, const int *);
 __global__ void k_slice_digits(const double *, long, int, int, const double *, long, int, int, int, int, const int *,
                                signed char *, long, long, int, signed char *);
 __global__ void k_xexp_final(const int *, const int *, long, const int *, const int *, int, int, int *);
@@ -209,6 +209,23 @@ static void partition(int n, const std::vector<double> &subw, std::vector<int> &
     }
 }
 
+// First column of the far-update region of each row of the working pencil: the bottom edge g1 of
+// the row's group of GROUP_STEPS blocks.  A row only meets far updates of groups below its own,
+// whose K columns lie in [g1, n).
+static void far_columns(int n, const std::vector<double> &subw, std::vector<int> &cfar)
+{
+    std::vector<int> i0s, nbs;
+    partition(n, subw, i0s, nbs);
+    cfar.assign(n, n);
+    for (int s = 0; s < (int)i0s.size(); ++s) {
+        const int gs = s / GROUP_STEPS * GROUP_STEPS;
+        const int g1 = i0s[gs] + nbs[gs];
+        for (int r = i0s[s]; r < i0s[s] + nbs[s]; ++r) {
+            cfar[r] = g1;
+        }
+    }
+}
+
 // Units of the real solve: one per real eigenvalue column, one per complex pair (p, p+1).
 static void make_units(const std::vector<ColDesc> &desc, std::vector<UnitDesc> &units, std::vector<int> &col2unit)
 {
@@ -310,7 +327,7 @@ struct DevBuf {
 struct RealWork {
     DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wblk, wgrp, xrel, xfin, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
     DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp;
-    DevBuf Sq, Pq, Wg, wgexp, wbexp, xbexp, sstart, Xq, Zq, xexp; // split-int8 digits and exponents
+    DevBuf Sq, Pq, Wg, wgexp, wbexp, xbexp, sstart, Xq, Zq, xexp, cfar; // split-int8 digits and exponents
     // split-int8 form: side stream (default priority) for the digit splitting of the pencil
     // and of Z, overlapped with the sweep; events order it against the caller's stream
     hipStream_t side = nullptr;
@@ -421,7 +438,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         ENSURE(ws.xrel, sizeof(int) * (size_t)m);
         hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.rowexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
         hipLaunchKernelGGL(k_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, Sw, ldsw, Pw, ldpw, n, n,
-                           ws.rowexp.as<int>()); XTG_LAUNCH_CHECK("k_rowexp");
+                           ws.rowexp.as<int>(), (const int *)nullptr); XTG_LAUNCH_CHECK("k_rowexp");
         hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.xrel.as<int>(), (long)m, -1000000); XTG_LAUNCH_CHECK("k_fill_int");
         mx = MixedExp{ws.wblk.as<int>(), ws.wgrp.as<int>(), ws.xrel.as<int>(), 1};
     }
@@ -617,12 +634,14 @@ static hipError_t split_side_init(RealWork &ws)
 
 // Digits of the rows of A (and B, with the exponent of each row taken over both): E[r] from the
 // exact row maxima, then QA (QB) = k-blocked digits over [0, kp) (k-block pitch 32 rows).
+// With cfirst, E[r] bounds only the columns k >= cfirst[r] (the far-update region of a pencil
+// row); digits left of it may saturate and must only meet zero digits of the other operand.
 static hipError_t split_rows(hipStream_t st, int rows, int cols, int kp, const double *A, long lda, const double *B,
-                             long ldb, int *E, signed char *QA, signed char *QB)
+                             long ldb, int *E, signed char *QA, signed char *QB, const int *cfirst = nullptr)
 {
     hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, E, (long)rows, -1000);
     hipLaunchKernelGGL(k_rowexp, dim3((rows + 255) / 256, (cols + 63) / 64), dim3(256), 0, st, A, lda, B, ldb, rows,
-                       cols, E);
+                       cols, E, cfirst);
     const dim3 grid((rows + 63) / 64, (kp + 15) / 16);
     const long qs = (long)rows * kp, kbp = (long)rows * SPLIT_KSTEP;
     hipLaunchKernelGGL(k_slice_digits, grid, dim3(256), 0, st, A, lda, 0, cols, (const double *)nullptr, 0L, 0, 0, rows,
@@ -710,12 +729,16 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
             ENSURE(ws.zexp, sizeof(int) * (size_t)n);
             ENSURE(ws.Zq, (size_t)SPLIT_DIGITS * n * kb);
         }
+        std::vector<int> cfar;
+        far_columns(n, *A.subw, cfar);
+        ENSURE(ws.cfar, sizeof(int) * (size_t)n);
+        XTG_CHECK(hipMemcpyAsync(ws.cfar.p, cfar.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
         XTG_CHECK(split_side_init(ws));
         XTG_CHECK(hipEventRecord(ws.ev_start, st));
         XTG_CHECK(hipStreamWaitEvent(ws.side, ws.ev_start, 0));
         int pr = prof_begin(ws.side);
         XTG_CHECK(split_rows(ws.side, n, n, (int)qld, A.Sw, A.ldsw, A.Pw, A.ldpw, ws.rowexp.as<int>(),
-                             ws.Sq.as<signed char>(), ws.Pq.as<signed char>()));
+                             ws.Sq.as<signed char>(), ws.Pq.as<signed char>(), ws.cfar.as<int>()));
         prof_end(pr, ws.side, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * 4.0 * n * (double)n + 2.0 * SPLIT_DIGITS * qsl);
         XTG_CHECK(hipEventRecord(ws.ev_pencil, ws.side));
         if (A.back) {
@@ -754,7 +777,7 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
                                (const int *)ws.xrel.as<int>(), m, ws.xfin.as<int>()); XTG_LAUNCH_CHECK("k_xfin");
             hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.zexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
             hipLaunchKernelGGL(k_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, A.Z, A.ldz,
-                               (const double *)nullptr, 0L, n, n, ws.zexp.as<int>()); XTG_LAUNCH_CHECK("k_rowexp");
+                               (const double *)nullptr, 0L, n, n, ws.zexp.as<int>(), (const int *)nullptr); XTG_LAUNCH_CHECK("k_rowexp");
         }
         if (split) {
             // digits of the columns of X (exponents from the sweep); Z's come from the side stream
@@ -930,7 +953,7 @@ int real_device(char side, char howmny, const int *select, int n, const double *
 int real_backward_device(int n, const double *S, long lds, const double *P, long ldp, const double *ar,
                          const double *ai, const double *be, const int *select, const double *V, long ldv,
                          const double *G, long ldg, int nbatch, double *dS, long ldds, double *dP, long lddp, int *info,
-                         hipStream_t st)
+                         hipStream_t st, int prec)
 {
     int dev = 0;
     XTG_CHECK(hipGetDevice(&dev));
@@ -1014,7 +1037,28 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
     int pr = prof_begin(st);
     hipLaunchKernelGGL(k_bwd_rhs, dim3(M), dim3(256), 0, st, V, ldv, G, ldg, n, m, nbatch, ws.kcol.as<int>(), X, ldx); XTG_LAUNCH_CHECK("k_bwd_rhs");
     prof_end(pr, st, 3, 0.0, 8.0 * 3.0 * n * M);
-    int rc = sweep_real(ws, n, ws.Sw.as<double>(), n, ws.Pw.as<double>(), n, subw, desc, true, ldx, st, 0, nullptr);
+    const bool split = prec == XTGEVC3_PREC_SPLIT_I8;
+    if (split) {
+        // digits of the working pencil on the side stream, after its reversed transposes
+        const long qld = split_qld(n), qsl = (long)n * qld;
+        ENSURE(ws.rowexp, sizeof(int) * (size_t)n);
+        ENSURE(ws.Sq, (size_t)SPLIT_DIGITS * qsl);
+        ENSURE(ws.Pq, (size_t)SPLIT_DIGITS * qsl);
+        std::vector<int> cfar;
+        far_columns(n, subw, cfar);
+        ENSURE(ws.cfar, sizeof(int) * (size_t)n);
+        XTG_CHECK(hipMemcpyAsync(ws.cfar.p, cfar.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+        XTG_CHECK(split_side_init(ws));
+        XTG_CHECK(hipEventRecord(ws.ev_start, st));
+        XTG_CHECK(hipStreamWaitEvent(ws.side, ws.ev_start, 0));
+        pr = prof_begin(ws.side);
+        XTG_CHECK(split_rows(ws.side, n, n, (int)qld, ws.Sw.as<double>(), (long)n, ws.Pw.as<double>(), (long)n,
+                             ws.rowexp.as<int>(), ws.Sq.as<signed char>(), ws.Pq.as<signed char>(), ws.cfar.as<int>()));
+        prof_end(pr, ws.side, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * 4.0 * n * (double)n + 2.0 * SPLIT_DIGITS * qsl);
+        XTG_CHECK(hipEventRecord(ws.ev_pencil, ws.side));
+    }
+    int rc = sweep_real(ws, n, ws.Sw.as<double>(), n, ws.Pw.as<double>(), n, subw, desc, true, ldx, st, prec,
+                        split ? ws.ev_pencil : nullptr);
     if (rc) {
         return rc;
     }

@@ -1019,11 +1019,12 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
 }
 
 // out[r] = max(out[r], e) with e the exponent bounding max_k max(|A[r + k lda]|, |B[r + k ldb]|)
-// (B may be null) over the column chunk [64 blockIdx.y, +64) of k < ncols, clamped to
-// [-1000, 1000]; out is filled with -1000 beforehand.  Threads run over rows (coalesced).
+// (B may be null) over the column chunk [64 blockIdx.y, +64) of k < ncols, and k >= cfirst[r]
+// when cfirst is given, clamped to [-1000, 1000]; out is filled with -1000 beforehand.
+// Threads run over rows (coalesced).
 constexpr int ROWEXP_CHUNK = 64;
 __global__ void k_rowexp(const double *__restrict__ A, long lda, const double *__restrict__ B, long ldb, int n,
-                         int ncols, int *__restrict__ out)
+                         int ncols, int *__restrict__ out, const int *__restrict__ cfirst)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int k0 = blockIdx.y * ROWEXP_CHUNK;
@@ -1032,7 +1033,7 @@ __global__ void k_rowexp(const double *__restrict__ A, long lda, const double *_
     }
     const int k1 = min(ncols, k0 + ROWEXP_CHUNK);
     double mx = 0.0;
-    for (int k = k0; k < k1; ++k) {
+    for (int k = cfirst != nullptr ? max(k0, cfirst[r]) : k0; k < k1; ++k) {
         mx = fmax(mx, fabs(A[r + (long)k * lda]));
         if (B != nullptr) {
             mx = fmax(mx, fabs(B[r + (long)k * ldb]));

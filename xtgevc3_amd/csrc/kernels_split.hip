@@ -98,9 +98,11 @@ __global__ __launch_bounds__(256) void k_slice_digits(const double *__restrict__
 }
 
 // out[r] = max(out[r], ilogb(max |Re|, |Im| over the chunk) + 1) for the complex rows r of A
-// and B (B may be null), A(r, k) = A[r + k * lda]; chunks of 64 columns per blockIdx.y.
+// and B (B may be null), A(r, k) = A[r + k * lda]; chunks of 64 columns per blockIdx.y, and
+// only columns k >= cfirst[r] when cfirst is given.
 __global__ __launch_bounds__(256) void kz_rowexp(const double2 *__restrict__ A, long lda, const double2 *__restrict__ B,
-                                                 long ldb, int n, int ncols, int *__restrict__ out)
+                                                 long ldb, int n, int ncols, int *__restrict__ out,
+                                                 const int *__restrict__ cfirst)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int k0 = blockIdx.y * 64;
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(256) void kz_rowexp(const double2 *__restrict__ A, 
     }
     const int k1 = min(ncols, k0 + 64);
     double mx = 0.0;
-    for (int k = k0; k < k1; ++k) {
+    for (int k = cfirst != nullptr ? max(k0, cfirst[r]) : k0; k < k1; ++k) {
         const double2 z = A[r + (long)k * lda];
         mx = fmax(mx, fmax(fabs(z.x), fabs(z.y)));
         if (B != nullptr) {
