@@ -31,11 +31,13 @@ SPLIT_PRODUCTS = 15  # digit products per multiply-add in the split-int8 form (l
 # each scaled by the counter's reading on a plain 1 GiB device copy (FETCH x2.0, WRITE x1.0);
 # summary and commands in profiles/dtgevc3_R_B_n8192_hbm_traffic.md
 MEASURED_TRAFFIC = {
-    ("xtg::k_gemm_f64<1, true, 128, 128, 16, false>", 8192): 12327.9e6,
-    ("xtg::k_gemm_f64<0, true, 128, 64, 16, false>", 8192): 823.3e6,
-    ("xtg::k_gemm_f64<0, true, 128, 128, 16, false>", 8192): 403.1e6,
-    ("xtg::k_gemm_f64<0, true, 64, 64, 32, false>", 8192): 25.0e6,
-    ("xtg::k_solve_real<65>", 8192): 27.5e6,
+    ("xtg::k_gemm_f64<1, true, 128, 128, 16, false>", 8192): 11611.4e6,
+    ("xtg::k_gemm_f64<0, true, 128, 64, 16, false>", 8192): 432.1e6,
+    ("xtg::k_gemm_f64<0, true, 128, 128, 16, false>", 8192): 221.6e6,
+    ("xtg::k_gemm_f64<0, true, 64, 64, 32, false>", 8192): 104.1e6,
+    ("xtg::k_gemm_f64<0, true, 32, 64, 32, false>", 8192): 16.3e6,
+    ("xtg::k_solve_real<65>", 8192): 27.4e6,
+    ("xtg::k_gemm_i8<128, 128, 2, 4> (back-transform)", 8192): 4866.3e6,
 }
 
 
@@ -411,7 +413,8 @@ def main():
             v = pf["back_split"]
             ach = v["flops"] * SPLIT_PRODUCTS / (v["ms"] * 1e9)
             out["roofline"] = {"bound": "mfma", "achieved": round(ach, 1), "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
-                               "frac": round(ach / PEAK_INT8_TOPS, 4), "traffic": None,
+                               "frac": round(ach / PEAK_INT8_TOPS, 4),
+                               "traffic": MEASURED_TRAFFIC.get(("xtg::k_gemm_i8<128, 128, 2, 4> (back-transform)", n)),
                                "kernel": "xtg::k_gemm_i8<128, 128, 2, 4> (back-transform)",
                                "avg_launch_ms": round(v["ms"] / v["launches"], 4),
                                "fp64_equivalent_tflops": round(v["flops"] / (v["ms"] * 1e9), 2)}
