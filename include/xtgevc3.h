@@ -121,6 +121,18 @@ int xtgevc3_dtgevc3_backward_device_ex(int n, const double *S, int lds, const do
                                        int nbatch, double *dS, int ldds, double *dP, int lddp, int *info, void *stream,
                                        int precision);
 
+/* Eigenvalue condition numbers (no counterpart in the reference; the semantics of LAPACK
+ * dtgsna with JOB = 'E'): for the eigenvalues chosen by howmny ('A' all, 'S' by select, with
+ * dtgevc3's selection rule, complex pairs from alphai) and their left and right eigenvectors
+ * VL, VR (n x m device arrays, the columns dtgevc3 returns for side = 'B', in the same basis as
+ * the device arrays S and P), writes s[c] = |(u^H S v, u^H P v)| / (|u| |v|) for every column
+ * c (both columns of a pair get the pair's value) to the device array s (m values).  m must be
+ * the column count of the selection.  Returns 0, or a negative code with
+ * xtgevc3_last_error() set: -1 bad howmny, -3 bad sizes, XTGEVC3_INFO_DEVICE_ERROR. */
+int xtgevc3_dtgsna_e_device(char howmny, const int *select, int n, const double *S, int lds, const double *P,
+                            int ldp, const double *alphai, const double *VL, int ldvl, const double *VR, int ldvr,
+                            int m, double *s, int *info, void *stream);
+
 /* Live kernel timing (no counterpart in the reference).  When enabled, every launch of the
  * hot kernels is bracketed by HIP events on its stream; xtgevc3_profile_read synchronises,
  * returns per-kind totals (milliseconds, launches, algorithmic flops and bytes) and resets.

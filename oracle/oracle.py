@@ -158,6 +158,33 @@ def complex_columns(V, kind):
     return out
 
 
+def tgsna_e(S, P, VL, VR, kind):
+    """Eigenvalue condition numbers, restating LAPACK 3.x dtgsna (JOB = 'E', the reciprocal
+    condition number S(ks) of the eigenvalue branch): for a real eigenvalue with left / right
+    vectors u, v, s = dlapy2(u^T S v, u^T P v) / (|u| |v|); for a complex pair with columns
+    (u0, u1), (v0, v1): uhav = u0'S v0 + u1'S v1, uhavi = u0'S v1 - u1'S v0 (the same with P),
+    s = dlapy2(dlapy2(uhav, uhavi), dlapy2(uhbv, uhbvi)) / (dlapy2(|v0|, |v1|) dlapy2(|u0|, |u1|)),
+    stored in both columns.  kind: column kinds from real_layout."""
+    n, m = VR.shape
+    out = np.zeros(m)
+    c = 0
+    while c < m:
+        if kind[c] == 1:
+            u0, u1, v0, v1 = VL[:, c], VL[:, c + 1], VR[:, c], VR[:, c + 1]
+            sa0, sa1, pb0, pb1 = S @ v0, S @ v1, P @ v0, P @ v1
+            av, avi = u0 @ sa0 + u1 @ sa1, u0 @ sa1 - u1 @ sa0
+            bv, bvi = u0 @ pb0 + u1 @ pb1, u0 @ pb1 - u1 @ pb0
+            rn = np.hypot(np.linalg.norm(v0), np.linalg.norm(v1))
+            ln = np.hypot(np.linalg.norm(u0), np.linalg.norm(u1))
+            out[c] = out[c + 1] = np.hypot(np.hypot(av, avi), np.hypot(bv, bvi)) / (rn * ln)
+            c += 2
+        else:
+            u, v = VL[:, c], VR[:, c]
+            out[c] = np.hypot(u @ (S @ v), u @ (P @ v)) / (np.linalg.norm(u) * np.linalg.norm(v))
+            c += 1
+    return out
+
+
 def eig_alpha_beta(alpha, alphai, beta, idx):
     al = np.asarray(alpha, dtype=np.float64)[idx] + 1j * (np.zeros(len(idx)) if alphai is None
                                                           else np.asarray(alphai, dtype=np.float64)[idx])

@@ -1,0 +1,61 @@
+"""GPU eigenvalue condition numbers (xtgevc3_dtgsna_e_device, LAPACK dtgsna JOB='E' semantics)
+against the oracle's restatement on the same eigenvectors, and the C entry's error paths."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from xtgevc3_amd import pencils as G
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(float).eps
+
+
+def _vectors(S, P, ar, ai, be, howmny, sel):
+    import torch
+
+    import xtgevc3_amd as X
+
+    n = S.shape[0]
+    dS, dP = X.fortran_device(S), X.fortran_device(P)
+    dVL = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    dVR = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    m, info = X.dtgevc3_device("B", howmny, sel, dS, dP, ar, ai, be, VL=dVL, VR=dVR, mm=n)
+    assert info == 0
+    return dS, dP, dVL[:m], dVR[:m], m
+
+
+@pytest.mark.parametrize("n,howmny,offdiag", [(300, "A", 0.2), (700, "S", 0.1), (90, "S", 1.0)])
+def test_tgsna_e_matches_oracle(n, howmny, offdiag):
+    import xtgevc3_amd as X
+
+    S, P, ar, ai, be = G.real_quasi_triangular(n, 60 + n, offdiag=offdiag)
+    sel = None
+    if howmny == "S":
+        sel = (np.random.default_rng(n).random(n) < 0.3).astype(np.int32)
+    dS, dP, dVL, dVR, m = _vectors(S, P, ar, ai, be, howmny, sel)
+    s = X.dtgsna_e_device(howmny, sel, dS, dP, ai, dVL, dVR, m=m).cpu().numpy()
+    kind, _ = O.real_layout(howmny, sel, ai)
+    assert len(kind) == m and kind[kind == 1].size > 0
+    VL = np.asfortranarray(dVL.cpu().numpy().T)
+    VR = np.asfortranarray(dVR.cpu().numpy().T)
+    ref = O.tgsna_e(S, P, VL, VR, kind)
+    # dot products of normalised vectors: |ds| <= ~n eps (|S| + |P|)
+    tol = 8 * n * EPS * (np.linalg.norm(S) + np.linalg.norm(P))
+    assert np.all(np.isfinite(s)) and np.max(np.abs(s - ref)) <= tol, (np.max(np.abs(s - ref)), tol)
+    pairs = np.nonzero(kind == 1)[0]
+    assert np.array_equal(s[pairs], s[pairs + 1])
+
+
+def test_tgsna_e_rejects_bad_arguments():
+    import torch
+
+    import xtgevc3_amd as X
+
+    n = 40
+    S, P, ar, ai, be = G.real_quasi_triangular(n, 7)
+    dS, dP, dVL, dVR, m = _vectors(S, P, ar, ai, be, "A", None)
+    with pytest.raises(RuntimeError, match="does not match"):
+        X.dtgsna_e_device("A", None, dS, dP, ai, dVL, dVR, m=m - 1)
+    with pytest.raises(RuntimeError, match="howmny"):
+        X.dtgsna_e_device("B", None, dS, dP, ai, dVL, dVR, m=m)
+    torch.cuda.synchronize()

@@ -233,3 +233,45 @@ def test_implicit_gradient_reference_matches_finite_differences(seed):
             fd = (Lp - Lm) / (2 * h)
             ref = (rS if which == "S" else rP)[i, j]
             assert abs(fd - ref) <= 1e-7 * scale, (which, i, j, fd, ref)
+
+
+def test_tgsna_e_diagonal_closed_form():
+    """Diagonal pencil: e_k are the eigenvectors, s_k = hypot(S_kk, P_kk) whatever their scale."""
+    rng = np.random.default_rng(5)
+    n = 9
+    d_s, d_p = rng.standard_normal(n), np.abs(rng.standard_normal(n)) + 0.1
+    S, P = np.diag(d_s), np.diag(d_p)
+    s = O.tgsna_e(S, P, 3.0 * np.eye(n), -0.5 * np.eye(n), np.zeros(n, dtype=np.int32))
+    assert np.allclose(s, np.hypot(d_s, d_p), rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("n", [6, 15])
+def test_tgsna_e_matches_independent_eigensolver(n):
+    """s_k from the oracle's real-layout vectors of (A, B) (QZ + tgevc3, back-transformed, pair
+    formula) against s_k from LAPACK xGGEV's complex left/right eigenvectors (scipy.linalg.eig)
+    with the complex formula |(u^H A v, u^H B v)| / (|u| |v|)."""
+    import scipy.linalg as sl
+
+    rng = np.random.default_rng(40 + n)
+    A = rng.standard_normal((n, n))
+    B = rng.standard_normal((n, n))
+    S, P, al, be, Q, Z = sl.ordqz(A, B, sort="lhp", output="real")
+    ar, ai, be = al.real.copy(), al.imag.copy(), be.real.copy()
+    S, P, Q, Z = (np.asfortranarray(x) for x in (S, P, Q, Z))
+    kind, idx = O.real_layout("A", None, ai)
+    VL = np.asfortranarray(Q.copy())
+    VR = np.asfortranarray(Z.copy())
+    O.tgevc3("d", "B", "B", None, S, P, ar, be, alphai=ai, VL=VL, VR=VR)
+    mine = O.tgsna_e(A, B, VL, VR, kind)
+    # the same quantity in the Schur basis: the value does not depend on the basis
+    VLs = np.zeros((n, n), order="F")
+    VRs = np.zeros((n, n), order="F")
+    O.tgevc3("d", "B", "A", None, S, P, ar, be, alphai=ai, VL=VLs, VR=VRs)
+    assert np.allclose(O.tgsna_e(S, P, VLs, VRs, kind), mine, rtol=1e-10, atol=0)
+    w, vl, vr = sl.eig(A, B, left=True, right=True)
+    lam = (ar + 1j * ai)[idx] / be[idx]
+    for c in range(n):
+        k = int(np.argmin(np.abs(w - lam[c])))
+        u, v = vl[:, k], vr[:, k]
+        ref = np.hypot(abs(np.vdot(u, A @ v)), abs(np.vdot(u, B @ v))) / (np.linalg.norm(u) * np.linalg.norm(v))
+        assert abs(mine[c] - ref) <= 1e-9 * ref, (c, mine[c], ref)

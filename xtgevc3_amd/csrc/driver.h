@@ -41,6 +41,10 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
                          const double *G, long ldg, int nbatch, double *dS, long ldds, double *dP, long lddp, int *info,
                          hipStream_t st, int prec = 0);
 
+int real_tgsna_device(char howmny, const int *select, int n, const double *S, long lds, const double *P, long ldp,
+                      const double *ai, const double *VL, long ldvl, const double *VR, long ldvr, int m, double *s,
+                      hipStream_t st);
+
 // live timing of kernel launches (profile.hip); begin returns -1 when disabled
 int prof_begin(hipStream_t st);
 void prof_end(int idx, hipStream_t st, int kind, double flops, double bytes);

@@ -34,6 +34,9 @@ __global__ void k_solve_real(const double *, long, const double *, long, int, in
                              double *, long, const UnitDesc *, const int4 *, int, int, int, int, double *, int *, int *,
                              int *, int, int, double *, int, GroupInfo, MixedExp, SplitOut);
 __global__ void k_rowexp(const double *, long, const double *, long, int, int, int *, const int *);
+__global__ void k_tgsna_part(const double *, const double *, const double *, const double *, long, int, int,
+                             const int *, const int *, int, double *);
+__global__ void k_tgsna_fin(const double *, int, int, const int *, const int *, double *);
 __global__ void k_slice_digits(const double *, long, int, int, const double *, long, int, int, int, int, const int *,
                                signed char *, long, long, int, signed char *);
 __global__ void k_xexp_final(const int *, const int *, long, const int *, const int *, int, int, int *);
@@ -327,6 +330,7 @@ struct DevBuf {
 struct RealWork {
     DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wblk, wgrp, xrel, xfin, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
     DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp;
+    DevBuf Ut, YB, part, tcol, tw; // eigenvalue condition numbers
     DevBuf Sq, Pq, Wg, wgexp, wbexp, xbexp, sstart, Xq, Zq, xexp, cfar; // split-int8 digits and exponents
     // split-int8 form: side stream (default priority) for the digit splitting of the pencil
     // and of Z, overlapped with the sweep; events order it against the caller's stream
@@ -1093,4 +1097,67 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
     return 0;
 }
 
+// ---------------------------------------------------------------------------------------
+// Eigenvalue condition numbers s_k (LAPACK dtgsna, JOB = 'E', HOWMNY = 'S' / 'A') from the
+// left and right eigenvectors of the pencil (S, P) as dtgevc3 returns them (device arrays, the
+// same basis for S, P and the vectors): s_k = |(u^H S v, u^H P v)| / (|u| |v|), a complex pair
+// sharing one value over its two columns.  Two FP64 products (S V, P V) and a column reduction
+// in fixed order.  s: device array of m values.
+// ---------------------------------------------------------------------------------------
+int real_tgsna_device(char howmny, const int *select, int n, const double *S, long lds, const double *P, long ldp,
+                      const double *ai, const double *VL, long ldvl, const double *VR, long ldvr, int m, double *s,
+                      hipStream_t st)
+{
+    int dev = 0;
+    XTG_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    RealWork &ws = g_ws[dev & 63];
+    std::vector<Unit> units;
+    const int mc = real_units(howmny, select, n, ai, units);
+    if (mc != m) {
+        set_error("m = " + std::to_string(m) + " does not match the " + std::to_string(mc) +
+                  " columns the selection describes");
+        return -1;
+    }
+    if (m == 0) {
+        return 0;
+    }
+    const long ldt = ((long)m + 7) / 8 * 8;
+    ENSURE(ws.Vt, sizeof(double) * (size_t)n * ldt);
+    ENSURE(ws.Ut, sizeof(double) * (size_t)n * ldt);
+    ENSURE(ws.Y, sizeof(double) * (size_t)n * ldt);
+    ENSURE(ws.YB, sizeof(double) * (size_t)n * ldt);
+    const int nu = (int)units.size();
+    std::vector<int> ucol(nu), uw(nu);
+    for (int q = 0; q < nu; ++q) {
+        ucol[q] = units[q].outc;
+        uw[q] = units[q].w;
+    }
+    ENSURE(ws.tcol, sizeof(int) * (size_t)nu);
+    ENSURE(ws.tw, sizeof(int) * (size_t)nu);
+    XTG_CHECK(hipMemcpyAsync(ws.tcol.p, ucol.data(), sizeof(int) * nu, hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.tw.p, uw.data(), sizeof(int) * nu, hipMemcpyHostToDevice, st));
+    const dim3 gt((m + 31) / 32, (n + 31) / 32);
+    hipLaunchKernelGGL(k_to_rowmajor, gt, dim3(32, 8), 0, st, VR, ldvr, n, m, ws.Vt.as<double>(), ldt); XTG_LAUNCH_CHECK("k_to_rowmajor");
+    hipLaunchKernelGGL(k_to_rowmajor, gt, dim3(32, 8), 0, st, VL, ldvl, n, m, ws.Ut.as<double>(), ldt); XTG_LAUNCH_CHECK("k_to_rowmajor");
+    for (int part = 0; part < 2; ++part) {
+        launch_gemm_f64(2, st, n, m, n, 0, part == 0 ? S : P, part == 0 ? lds : ldp, (const double *)nullptr, 0L,
+                        (const double *)ws.Vt.as<double>(), ldt, part == 0 ? ws.Y.as<double>() : ws.YB.as<double>(),
+                        ldt, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr,
+                        (const double *)nullptr, 0, nullptr, nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+    }
+    constexpr int ROWS = 128;
+    const int nchunk = (n + ROWS - 1) / ROWS;
+    ENSURE(ws.part, sizeof(double) * 12 * (size_t)nchunk * nu);
+    hipLaunchKernelGGL(k_tgsna_part, dim3((nu + 255) / 256, nchunk), dim3(256), 0, st,
+                       (const double *)ws.Ut.as<double>(), (const double *)ws.Vt.as<double>(),
+                       (const double *)ws.Y.as<double>(), (const double *)ws.YB.as<double>(), ldt, n, nu,
+                       (const int *)ws.tcol.as<int>(), (const int *)ws.tw.as<int>(), ROWS, ws.part.as<double>()); XTG_LAUNCH_CHECK("k_tgsna_part");
+    hipLaunchKernelGGL(k_tgsna_fin, dim3((nu + 255) / 256), dim3(256), 0, st, (const double *)ws.part.as<double>(),
+                       nchunk, nu, (const int *)ws.tcol.as<int>(), (const int *)ws.tw.as<int>(), s); XTG_LAUNCH_CHECK("k_tgsna_fin");
+    XTG_CHECK(hipGetLastError());
+    return 0;
+}
+
 } // namespace xtg
+

@@ -1489,3 +1489,97 @@ int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const
     return cfg;
 }
 } // namespace xtg
+
+namespace xtg {
+
+// ---------------------------------------------------------------------------------------
+// Eigenvalue condition numbers (LAPACK dtgsna, JOB = 'E').  U, V, YA, YB row-major n x m
+// (element (i, c) at i * ld + c): U the left vectors, V the right vectors, YA = S V and
+// YB = P V.  Unit q covers column ucol[q] (uw[q] = 1) or the pair ucol[q], ucol[q] + 1
+// (real and imaginary part, uw[q] = 2).  k_tgsna_part: thread (q, chunk) sums rows
+// [chunk * rows, +rows) into part[(chunk * nu + q) * TGSNA_SUMS + t]:
+//   t = 0..3: u0.a0, u1.a0, u0.a1, u1.a1 (a = YA columns), 4..7 the same with YB,
+//   8, 9: |u0|^2, |u1|^2, 10, 11: |v0|^2, |v1|^2.
+// k_tgsna_fin sums the chunks in order and forms s (dtgsna's real and complex-pair formulas).
+// ---------------------------------------------------------------------------------------
+constexpr int TGSNA_SUMS = 12;
+
+__global__ __launch_bounds__(256) void k_tgsna_part(const double *__restrict__ U, const double *__restrict__ V,
+                                                    const double *__restrict__ YA, const double *__restrict__ YB,
+                                                    long ld, int n, int nu, const int *__restrict__ ucol,
+                                                    const int *__restrict__ uw, int rows, double *__restrict__ part)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = blockIdx.y;
+    if (q >= nu) {
+        return;
+    }
+    const int c = ucol[q];
+    const bool pair = uw[q] == 2;
+    double acc[TGSNA_SUMS];
+#pragma clang loop unroll(full)
+    for (int t = 0; t < TGSNA_SUMS; ++t) {
+        acc[t] = 0.0;
+    }
+    const int i1 = min(n, (chunk + 1) * rows);
+    for (int i = chunk * rows; i < i1; ++i) {
+        const long o = (long)i * ld + c;
+        const double u0 = U[o], v0 = V[o], a0 = YA[o], b0 = YB[o];
+        acc[0] = fma(u0, a0, acc[0]);
+        acc[4] = fma(u0, b0, acc[4]);
+        acc[8] = fma(u0, u0, acc[8]);
+        acc[10] = fma(v0, v0, acc[10]);
+        if (pair) {
+            const double u1 = U[o + 1], v1 = V[o + 1], a1 = YA[o + 1], b1 = YB[o + 1];
+            acc[1] = fma(u1, a0, acc[1]);
+            acc[2] = fma(u0, a1, acc[2]);
+            acc[3] = fma(u1, a1, acc[3]);
+            acc[5] = fma(u1, b0, acc[5]);
+            acc[6] = fma(u0, b1, acc[6]);
+            acc[7] = fma(u1, b1, acc[7]);
+            acc[9] = fma(u1, u1, acc[9]);
+            acc[11] = fma(v1, v1, acc[11]);
+        }
+    }
+    double *p = part + ((long)chunk * nu + q) * TGSNA_SUMS;
+#pragma clang loop unroll(full)
+    for (int t = 0; t < TGSNA_SUMS; ++t) {
+        p[t] = acc[t];
+    }
+}
+
+__global__ void k_tgsna_fin(const double *__restrict__ part, int nchunk, int nu, const int *__restrict__ ucol,
+                            const int *__restrict__ uw, double *__restrict__ s)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nu) {
+        return;
+    }
+    double t[TGSNA_SUMS];
+#pragma clang loop unroll(full)
+    for (int k = 0; k < TGSNA_SUMS; ++k) {
+        t[k] = 0.0;
+    }
+    for (int ch = 0; ch < nchunk; ++ch) {
+        const double *p = part + ((long)ch * nu + q) * TGSNA_SUMS;
+#pragma clang loop unroll(full)
+        for (int k = 0; k < TGSNA_SUMS; ++k) {
+            t[k] += p[k];
+        }
+    }
+    const int c = ucol[q];
+    if (uw[q] == 1) {
+        // real eigenvalue: |(u'Sv, u'Pv)| / (|u| |v|)
+        s[c] = hypot(t[0], t[4]) / (sqrt(t[8]) * sqrt(t[10]));
+    }
+    else {
+        // complex pair u = u0 + i u1, v = v0 + i v1: |u^H S v| from the four real products
+        const double av = t[0] + t[3], avi = t[2] - t[1];
+        const double bv = t[4] + t[7], bvi = t[6] - t[5];
+        const double cond = hypot(hypot(av, avi), hypot(bv, bvi));
+        const double rn = hypot(sqrt(t[10]), sqrt(t[11])), ln = hypot(sqrt(t[8]), sqrt(t[9]));
+        s[c] = s[c + 1] = cond / (rn * ln);
+    }
+}
+
+} // namespace xtg
