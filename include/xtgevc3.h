@@ -133,6 +133,13 @@ int xtgevc3_dtgsna_e_device(char howmny, const int *select, int n, const double 
                             int ldp, const double *alphai, const double *VL, int ldvl, const double *VR, int ldvr,
                             int m, double *s, int *info, void *stream);
 
+/* The complex128 counterpart (LAPACK ztgsna, JOB = 'E'): one value per selected eigenvalue
+ * column, s[c] = |(u^H S v, u^H P v)| / (|u| |v|), from the columns of
+ * xtgevc3_ztgevc3_device(side = 'B', ...). */
+int xtgevc3_ztgsna_e_device(char howmny, const int *select, int n, const xtgevc3_zcomplex *S, int lds,
+                            const xtgevc3_zcomplex *P, int ldp, const xtgevc3_zcomplex *VL, int ldvl,
+                            const xtgevc3_zcomplex *VR, int ldvr, int m, double *s, int *info, void *stream);
+
 /* Live kernel timing (no counterpart in the reference).  When enabled, every launch of the
  * hot kernels is bracketed by HIP events on its stream; xtgevc3_profile_read synchronises,
  * returns per-kind totals (milliseconds, launches, algorithmic flops and bytes) and resets.

@@ -185,6 +185,16 @@ def tgsna_e(S, P, VL, VR, kind):
     return out
 
 
+def tgsna_e_complex(S, P, VL, VR):
+    """LAPACK 3.x ztgsna (JOB = 'E') restated: s = dlapy2(|u^H S v|, |u^H P v|) / (|u| |v|)
+    per column."""
+    out = np.zeros(VR.shape[1])
+    for c in range(VR.shape[1]):
+        u, v = VL[:, c], VR[:, c]
+        out[c] = np.hypot(abs(np.vdot(u, S @ v)), abs(np.vdot(u, P @ v))) / (np.linalg.norm(u) * np.linalg.norm(v))
+    return out
+
+
 def eig_alpha_beta(alpha, alphai, beta, idx):
     al = np.asarray(alpha, dtype=np.float64)[idx] + 1j * (np.zeros(len(idx)) if alphai is None
                                                           else np.asarray(alphai, dtype=np.float64)[idx])

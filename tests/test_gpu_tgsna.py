@@ -59,3 +59,26 @@ def test_tgsna_e_rejects_bad_arguments():
     with pytest.raises(RuntimeError, match="howmny"):
         X.dtgsna_e_device("B", None, dS, dP, ai, dVL, dVR, m=m)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("n,howmny", [(300, "A"), (200, "S")])
+def test_ztgsna_e_matches_oracle(n, howmny):
+    import torch
+
+    import xtgevc3_amd as X
+
+    S, P, al, be = G.complex_triangular(n, 80 + n)
+    sel = None
+    if howmny == "S":
+        sel = (np.random.default_rng(n).random(n) < 0.3).astype(np.int32)
+    dS, dP = X.fortran_device(S), X.fortran_device(P)
+    dVL = torch.zeros((n, n), dtype=torch.complex128, device="cuda")
+    dVR = torch.zeros((n, n), dtype=torch.complex128, device="cuda")
+    m, info = X.ztgevc3_device("B", howmny, sel, dS, dP, al, be, VL=dVL, VR=dVR, mm=n)
+    assert info == 0
+    s = X.ztgsna_e_device(howmny, sel, dS, dP, dVL[:m], dVR[:m], m=m).cpu().numpy()
+    VL = np.asfortranarray(dVL[:m].cpu().numpy().T)
+    VR = np.asfortranarray(dVR[:m].cpu().numpy().T)
+    ref = O.tgsna_e_complex(S, P, VL, VR)
+    tol = 8 * n * EPS * (np.linalg.norm(S) + np.linalg.norm(P))
+    assert np.all(np.isfinite(s)) and np.max(np.abs(s - ref)) <= tol, (np.max(np.abs(s - ref)), tol)

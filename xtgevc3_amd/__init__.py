@@ -46,7 +46,7 @@ EXPORTED_SYMBOLS = [
     "xtgevc3_dtgevc3", "xtgevc3_stgevc3", "xtgevc3_ztgevc3", "xtgevc3_ctgevc3",
     "xtgevc3_dtgevc3_device", "xtgevc3_dtgevc3_device_ex", "xtgevc3_ztgevc3_device", "xtgevc3_ztgevc3_device_ex",
     "xtgevc3_profile_enable", "xtgevc3_profile_read", "xtgevc3_dtgevc3_backward_device",
-    "xtgevc3_dtgevc3_backward_device_ex", "xtgevc3_dtgsna_e_device",
+    "xtgevc3_dtgevc3_backward_device_ex", "xtgevc3_dtgsna_e_device", "xtgevc3_ztgsna_e_device",
 ]
 
 PROF_KINDS = ("solve", "update", "back", "other", "update_128x128", "update_128x64", "update_64x128",
@@ -254,6 +254,30 @@ def dtgsna_e_device(howmny, select, S, P, alphai, VL, VR, m=None, s=None, stream
         c.byref(info), c.c_void_p(stream))
     if rc != 0:
         raise RuntimeError("xtgevc3 dtgsna_e failed: " + last_error())
+    return s[:m]
+
+
+def ztgsna_e_device(howmny, select, S, P, VL, VR, m=None, s=None, stream=None):
+    """Eigenvalue condition numbers of a complex pencil (LAPACK ztgsna, JOB='E'); arguments as
+    dtgsna_e_device with complex128 tensors and no alphai."""
+    import torch
+
+    c = ctypes
+    n = S.shape[0]
+    if m is None:
+        m = VR.shape[0]
+    if s is None:
+        s = torch.empty(max(m, 1), dtype=torch.float64, device=S.device)
+    sel = None if select is None else np.ascontiguousarray(select, dtype=np.int32)
+    info = c.c_int(-99)
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    rc = _lib.xtgevc3_ztgsna_e_device(
+        c.c_char(howmny.encode()), _ptr(sel), c.c_int(n), _tptr(S), c.c_int(S.shape[1]), _tptr(P), c.c_int(P.shape[1]),
+        _tptr(VL), c.c_int(VL.shape[1]), _tptr(VR), c.c_int(VR.shape[1]), c.c_int(m), _tptr(s), c.byref(info),
+        c.c_void_p(stream))
+    if rc != 0:
+        raise RuntimeError("xtgevc3 ztgsna_e failed: " + last_error())
     return s[:m]
 
 

@@ -580,6 +580,38 @@ int xtgevc3_dtgsna_e_device(char howmny, const int *select, int n, const double 
     return 0;
 }
 
+int xtgevc3_ztgsna_e_device(char howmny, const int *select, int n, const xtgevc3_zcomplex *S, int lds,
+                            const xtgevc3_zcomplex *P, int ldp, const xtgevc3_zcomplex *VL, int ldvl,
+                            const xtgevc3_zcomplex *VR, int ldvr, int m, double *s, int *info, void *stream)
+{
+    if (info) {
+        *info = 0;
+    }
+    if (howmny != 'A' && howmny != 'S') {
+        xtg::set_error("howmny must be 'A' or 'S'");
+        if (info) {
+            *info = -1;
+        }
+        return -1;
+    }
+    if (n < 0 || m < 0 || (n > 0 && (lds < n || ldp < n || ldvl < n || ldvr < n))) {
+        xtg::set_error("invalid sizes");
+        if (info) {
+            *info = -3;
+        }
+        return -3;
+    }
+    if (n == 0) {
+        return 0;
+    }
+    int rc = xtg::complex_tgsna_device(howmny, select, n, S, lds, P, ldp, VL, ldvl, VR, ldvr, m, s,
+                                       (hipStream_t)stream);
+    if (rc != 0) {
+        return xtg::fail(info, "ztgsna_e");
+    }
+    return 0;
+}
+
 } // extern "C"
 
 // Drop-in C++-linkage symbols of include/tgevc3.h:10-11.

@@ -45,6 +45,10 @@ int real_tgsna_device(char howmny, const int *select, int n, const double *S, lo
                       const double *ai, const double *VL, long ldvl, const double *VR, long ldvr, int m, double *s,
                       hipStream_t st);
 
+int complex_tgsna_device(char howmny, const int *select, int n, const std::complex<double> *S, long lds,
+                         const std::complex<double> *P, long ldp, const std::complex<double> *VL, long ldvl,
+                         const std::complex<double> *VR, long ldvr, int m, double *s, hipStream_t st);
+
 // live timing of kernel launches (profile.hip); begin returns -1 when disabled
 int prof_begin(hipStream_t st);
 void prof_end(int idx, hipStream_t st, int kind, double flops, double bytes);

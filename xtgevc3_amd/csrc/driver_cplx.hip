@@ -41,6 +41,9 @@ __global__ void kz_copy_out(const double2 *, long, int, int, double2 *, long, co
 __global__ void kz_normalize(double2 *, long, int);
 __global__ void kz_reverse_conjtranspose(const double2 *, long, int, double2 *, long);
 __global__ void kz_colnorms(const double2 *, long, const double2 *, long, int, double *);
+__global__ void kz_to_rowmajor(const double2 *, long, int, int, double2 *, long);
+__global__ void kz_tgsna(const double2 *, long, const double2 *, long, const double2 *, const double2 *, long, int,
+                         double *);
 __global__ void k_zero(double *, long);
 __global__ void k_zero_int(int *, long);
 
@@ -105,6 +108,7 @@ struct ZWork {
     // split-int8 form: pencil digits expanded as (re, -im) and (im, re), group operand, X and Z
     // digits, exponents
     DBuf Sx, Sy, Px, Py, rowexp, Wg, wgexp, wbexp, xbexp, sstart, xexp, Xx, Xy, Zq, zexp, kmax2, cfar;
+    DBuf Vt, YA, YB; // eigenvalue condition numbers
 };
 
 std::mutex g_zmutex;
@@ -506,4 +510,48 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
     return 0;
 }
 
+// Eigenvalue condition numbers of the complex pencil (LAPACK ztgsna, JOB = 'E'): for the
+// columns howmny / select describes (dtgevc3's rule: one column per eigenvalue), with left and
+// right vectors VL, VR (device, the basis of S and P): s[c] = |(u^H S v, u^H P v)| / (|u| |v|).
+// YA = S V and YB = P V on the complex back-transform GEMM, then one wave per column.
+int complex_tgsna_device(char howmny, const int *select, int n, const std::complex<double> *S, long lds,
+                         const std::complex<double> *P, long ldp, const std::complex<double> *VL, long ldvl,
+                         const std::complex<double> *VR, long ldvr, int m, double *s, hipStream_t st)
+{
+    int dev = 0;
+    XTG_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_zmutex);
+    ZWork &ws = g_zws[dev & 63];
+    std::vector<Unit> units;
+    const int mc = complex_units(howmny, select, n, units);
+    if (mc != m) {
+        set_error("m = " + std::to_string(m) + " does not match the " + std::to_string(mc) +
+                  " columns the selection describes");
+        return -1;
+    }
+    if (m == 0) {
+        return 0;
+    }
+    const long ldt = ((long)m + 7) / 8 * 8;
+    ZENSURE(ws.Vt, sizeof(double2) * (size_t)n * ldt);
+    ZENSURE(ws.YA, sizeof(double2) * (size_t)n * m);
+    ZENSURE(ws.YB, sizeof(double2) * (size_t)n * m);
+    hipLaunchKernelGGL(kz_to_rowmajor, dim3((m + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, st, (const double2 *)VR,
+                       ldvr, n, m, ws.Vt.as<double2>(), ldt); XTG_LAUNCH_CHECK("kz_to_rowmajor");
+    // D[a][b] = sum_k V(k, a) M(b, k): column a of Y = M v_a (TAG 1 form, full K, no column map)
+    const dim3 grid(((n + ZTB_H - 1) / ZTB_H) * ((m + ZTA_H - 1) / ZTA_H));
+    for (int part = 0; part < 2; ++part) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kz_gemm<1, ZTA_H>), grid, dim3(256), 0, st, m, n, n, 0,
+                           (const double2 *)ws.Vt.as<double2>(), ldt, (const double2 *)nullptr, 0L,
+                           (const double2 *)(part == 0 ? S : P), part == 0 ? lds : ldp,
+                           part == 0 ? ws.YA.as<double2>() : ws.YB.as<double2>(), (long)n, (const int *)nullptr, 0,
+                           (const double *)nullptr, (const int *)nullptr, (const double2 *)nullptr); XTG_LAUNCH_CHECK("kz_gemm<1>");
+    }
+    hipLaunchKernelGGL(kz_tgsna, dim3(m), dim3(64), 0, st, (const double2 *)VL, ldvl, (const double2 *)VR, ldvr,
+                       (const double2 *)ws.YA.as<double2>(), (const double2 *)ws.YB.as<double2>(), (long)n, n, s); XTG_LAUNCH_CHECK("kz_tgsna");
+    XTG_CHECK(hipGetLastError());
+    return 0;
+}
+
 } // namespace xtg
+

@@ -609,4 +609,63 @@ __global__ void kz_colnorms(const double2 *__restrict__ S, long lds, const doubl
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Eigenvalue condition numbers of the complex pencil (LAPACK ztgsna, JOB = 'E').
+// kz_to_rowmajor: Vt[k * ldt + c] = V[k + c * ldv] (32 x 32 tiles).
+// kz_tgsna: one wave per column c of the column-major n x m arrays VL, VR, YA = S VR,
+// YB = P VR: s[c] = hypot(|VL_c^H YA_c|, |VL_c^H YB_c|) / (|VL_c| |VR_c|).
+// ---------------------------------------------------------------------------------------
+__global__ void kz_to_rowmajor(const double2 *__restrict__ V, long ldv, int n, int m, double2 *__restrict__ Vt,
+                               long ldt)
+{
+    __shared__ double2 tile[32][33];
+    const int c0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int kk = ty; kk < 32; kk += 8) {
+        const int c = c0 + kk, k = k0 + tx;
+        tile[kk][tx] = (c < m && k < n) ? V[k + (long)c * ldv] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int kk = ty; kk < 32; kk += 8) {
+        const int k = k0 + kk, c = c0 + tx;
+        if (c < m && k < n) {
+            Vt[(long)k * ldt + c] = tile[tx][kk];
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void kz_tgsna(const double2 *__restrict__ VL, long ldvl,
+                                               const double2 *__restrict__ VR, long ldvr,
+                                               const double2 *__restrict__ YA, const double2 *__restrict__ YB,
+                                               long ldy, int n, double *__restrict__ s)
+{
+    const int c = blockIdx.x;
+    const int lane = threadIdx.x;
+    double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0, nu = 0.0, nv = 0.0;
+    for (int i = lane; i < n; i += 64) {
+        const double2 u = VL[i + (long)c * ldvl], v = VR[i + (long)c * ldvr];
+        const double2 ya = YA[i + (long)c * ldy], yb = YB[i + (long)c * ldy];
+        // conj(u) * y
+        ar = fma(u.x, ya.x, fma(u.y, ya.y, ar));
+        ai = fma(u.x, ya.y, fma(-u.y, ya.x, ai));
+        br = fma(u.x, yb.x, fma(u.y, yb.y, br));
+        bi = fma(u.x, yb.y, fma(-u.y, yb.x, bi));
+        nu = fma(u.x, u.x, fma(u.y, u.y, nu));
+        nv = fma(v.x, v.x, fma(v.y, v.y, nv));
+    }
+#pragma clang loop unroll(full)
+    for (int off = 32; off > 0; off >>= 1) {
+        ar += __shfl_xor(ar, off);
+        ai += __shfl_xor(ai, off);
+        br += __shfl_xor(br, off);
+        bi += __shfl_xor(bi, off);
+        nu += __shfl_xor(nu, off);
+        nv += __shfl_xor(nv, off);
+    }
+    if (lane == 0) {
+        s[c] = hypot(hypot(ar, ai), hypot(br, bi)) / (sqrt(nu) * sqrt(nv));
+    }
+}
+
 } // namespace xtg
+
