@@ -94,18 +94,13 @@ def real_quasi_triangular(n, seed, pair_frac=0.4, grade=0.0, cluster_groups=0, c
             dc[k + 1] = dc[k]
         S = dr[:, None] * S * dc[None, :]
         P = dr[:, None] * P * dc[None, :]
-        # eigenvalues of the graded pencil: alpha_k / beta_k unchanged; rescale consistently
+        # eigenvalues of the graded pencil: alpha_k / beta_k unchanged.  A pair's block is scaled
+        # by one factor f = dr_k dc_k, so its (alpha, beta) from the unscaled block times f
+        # stay consistent (recomputing them from the graded entries would overflow)
         scale = dr * dc
         ar = ar * scale
         ai = ai * scale
         be = be * scale
-        for k in np.nonzero(is_pair)[0]:
-            # pair alpha/beta were computed with beta = p1 p2 of the unscaled block
-            a_r, a_i, b_ = pair_eigenvalue(S[k, k], S[k, k + 1], S[k + 1, k], S[k + 1, k + 1], P[k, k],
-                                           P[k + 1, k + 1])
-            ar[k] = ar[k + 1] = a_r
-            ai[k], ai[k + 1] = a_i, -a_i
-            be[k] = be[k + 1] = b_
     return (np.asfortranarray(S), np.asfortranarray(P), ar, ai, be)
 
 
