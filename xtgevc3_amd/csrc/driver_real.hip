@@ -17,6 +17,7 @@
 #include <climits>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <mutex>
@@ -336,6 +337,9 @@ struct RealWork {
     // and of Z, overlapped with the sweep; events order it against the caller's stream
     hipStream_t side = nullptr;
     hipEvent_t ev_start = nullptr, ev_pencil = nullptr, ev_z = nullptr;
+    // far-update lookahead: the rows above the next group run on `fs`
+    hipStream_t fs = nullptr;
+    hipEvent_t ev_wready[2] = {nullptr, nullptr}, ev_fdone[2] = {nullptr, nullptr};
 };
 
 static std::mutex g_ws_mutex;
@@ -381,6 +385,8 @@ struct RealSideArgs {
 //   eigenvectors) and the scaled-consistent solutions on exit; colexp[p] then holds the
 //   power-of-two exponent e with X[:, p] = 2^e * (true solution).
 // ---------------------------------------------------------------------------------------
+static hipError_t far_stream_init(RealWork &ws);
+
 int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *Pw, long ldpw,
                const std::vector<double> &subw, const std::vector<ColDesc> &desc, bool adjoint, long ldx,
                hipStream_t st, int prec, hipEvent_t pencil_ready)
@@ -403,9 +409,13 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     make_tiles(units, rowstep, n, tiles, tmin, tmax);
     const long ldw = ldx;
     const int wmax = GROUP_STEPS * MAXB_REAL;
-    ENSURE(ws.W, sizeof(double) * (size_t)(2 * wmax) * ldw);
+    // group operands, group scales (and their digits / exponents below) alternate between two
+    // copies by group parity: the lookahead far update of one group reads its copy while the
+    // next group's steps write the other
+    const size_t wsz = (size_t)(2 * wmax) * ldw;
+    ENSURE(ws.W, sizeof(double) * 2 * wsz);
     ENSURE(ws.sig, sizeof(double) * (size_t)m);
-    ENSURE(ws.sig_grp, sizeof(double) * (size_t)m);
+    ENSURE(ws.sig_grp, sizeof(double) * 2 * (size_t)m);
     ENSURE(ws.colexp, sizeof(int) * (size_t)m);
     ENSURE(ws.blkexp, sizeof(int) * (size_t)nsteps * ldx);
     ENSURE(ws.flag, sizeof(int) * 4);
@@ -422,8 +432,8 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         // the sweep records block exponents of the operands and of X
         qld = split_qld(n);
         qsl = (long)n * qld;
-        ENSURE(ws.Wg, (size_t)SPLIT_DIGITS * m * WG_K);
-        ENSURE(ws.wgexp, sizeof(int) * (size_t)m);
+        ENSURE(ws.Wg, 2 * (size_t)SPLIT_DIGITS * m * WG_K);
+        ENSURE(ws.wgexp, sizeof(int) * 2 * (size_t)m);
         ENSURE(ws.wbexp, sizeof(int) * (size_t)nsteps * ldx);
         ENSURE(ws.xbexp, sizeof(int) * (size_t)nsteps * ldx);
         ENSURE(ws.sstart, sizeof(int) * (size_t)m);
@@ -438,7 +448,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         // mixed form: row exponents of the working pencil, per-column operand/solution bounds
         ENSURE(ws.rowexp, sizeof(int) * (size_t)n);
         ENSURE(ws.wblk, sizeof(int) * (size_t)m);
-        ENSURE(ws.wgrp, sizeof(int) * (size_t)m);
+        ENSURE(ws.wgrp, sizeof(int) * 2 * (size_t)m);
         ENSURE(ws.xrel, sizeof(int) * (size_t)m);
         hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.rowexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
         hipLaunchKernelGGL(k_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, Sw, ldsw, Pw, ldpw, n, n,
@@ -460,12 +470,32 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     // The group's operand W holds rows [0, g1-g0) (S part) and [g1-g0, 2(g1-g0)) (P part).
     int c_begin = adjoint ? 0 : m;
     int c_end = adjoint ? 0 : m;
+    static const bool lookahead = std::getenv("XTGEVC3_NO_LOOKAHEAD") == nullptr;
+    if (lookahead) {
+        XTG_CHECK(far_stream_init(ws));
+    }
+    bool f_pending = false; // a far update on ws.fs has not been waited for yet
+    int last_par = 0;       // its group parity
     for (int gs = 0; gs < nsteps; gs += GROUP_STEPS) {
         const int ge = std::min(nsteps, gs + GROUP_STEPS);
         const int g1 = i0s[gs] + nbs[gs];
         const int g0 = i0s[ge - 1];
         const int wr = g1 - g0;
         bool any_active = false;
+        // this group's copies of the operand, group scales and their digits / exponents
+        const int par = (gs / GROUP_STEPS) & 1;
+        if (f_pending && last_par == par) {
+            XTG_CHECK(hipStreamWaitEvent(st, ws.ev_fdone[last_par], 0));
+            f_pending = false;
+        }
+        double *Wb = W + (size_t)par * wsz;
+        double *sgb = ws.sig_grp.as<double>() + (size_t)par * m;
+        int *wgexpb = split ? ws.wgexp.as<int>() + (size_t)par * m : nullptr;
+        signed char *Wgb = split ? ws.Wg.as<signed char>() + (size_t)par * SPLIT_DIGITS * m * WG_K : nullptr;
+        MixedExp mxb = mx;
+        if (mx.on) {
+            mxb.wgrp = mx.wgrp + (size_t)par * m;
+        }
         // the group's blocks top-down
         GroupInfo grp{};
         grp.wr = wr;
@@ -528,11 +558,11 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
             int pr = prof_begin(st);
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_real<MAXB_REAL>),
                                dim3((ntl + wpb - 1) / wpb), dim3(wpb * WAVE), 0, st, Sw,
-                               ldsw, Pw, ldpw, i0, nb, bumps, X, ldx, W, ldw, ws.desc.as<UnitDesc>(), ws.tiles.as<int4>(),
+                               ldsw, Pw, ldpw, i0, nb, bumps, X, ldx, Wb, ldw, ws.desc.as<UnitDesc>(), ws.tiles.as<int4>(),
                                t_begin, t_end, u_begin, u_end, ws.sig.as<double>(), ws.colexp.as<int>(),
                                ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff, wr + woff,
-                               ws.sig_grp.as<double>(), s == gs ? 1 : 0, grp, mx,
-                               split ? SplitOut{ws.wbexp.as<int>() + (long)s * ldx, ws.wgexp.as<int>(),
+                               sgb, s == gs ? 1 : 0, grp, mxb,
+                               split ? SplitOut{ws.wbexp.as<int>() + (long)s * ldx, wgexpb,
                                                 ws.xbexp.as<int>() + (long)s * ldx}
                                      : SplitOut{nullptr, nullptr, nullptr}); XTG_LAUNCH_CHECK("k_solve_real");
             prof_end(pr, st, 0, 2.0 * nb * nb * N, 8.0 * (4.0 * nb * N + 2.0 * nb * nb));
@@ -543,10 +573,10 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
             if (M > 0) {
                 pr = prof_begin(st);
                 const int gt = launch_gemm_f64(0, st, M, N, nb, nb, Sw + r0 + (long)i0 * ldsw,
-                                   ldsw, Pw + r0 + (long)i0 * ldpw, ldpw, (const double *)(W + (long)woff * ldw + c_begin),
+                                   ldsw, Pw + r0 + (long)i0 * ldpw, ldpw, (const double *)(Wb + (long)woff * ldw + c_begin),
                                    ldw, X + (long)r0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
                                    (const double *)(ws.sig.as<double>() + c_begin), (const int *)nullptr,
-                                   (const double *)(W + (long)(wr + woff) * ldw + c_begin), split ? 0 : prec,
+                                   (const double *)(Wb + (long)(wr + woff) * ldw + c_begin), split ? 0 : prec,
                                    mx.on ? ws.rowexp.as<int>() + r0 : nullptr, mx.on ? mx.wblk + c_begin : nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
                 prof_end(pr, st, update_kind(gt), 2.0 * M * N * (2.0 * nb),
                          8.0 * (2.0 * M * N + 2.0 * nb * M + 2.0 * nb * N));
@@ -555,49 +585,89 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         if (!any_active) {
             continue;
         }
-        // far update: rows above the group, all of its blocks at once
+        // far update: rows above the group, all of its blocks at once.  With the lookahead the
+        // rows of the next group (which its solves need next) run on st and the rows above it
+        // on ws.fs, overlapping the next group's steps; the next far update, which writes those
+        // rows again, waits for them.
         const int N = c_end - c_begin;
         const int r0 = adjoint ? std::min(desc[c_begin].lo, g0) : 0;
         const int M = g0 - r0;
-        if (M > 0 && N > 0 && split) {
+        if (f_pending) {
+            XTG_CHECK(hipStreamWaitEvent(st, ws.ev_fdone[last_par], 0));
+            f_pending = false;
+        }
+        if (M <= 0 || N <= 0) {
+            continue;
+        }
+        const int gen = std::min(nsteps, ge + GROUP_STEPS);
+        const int rsplit = lookahead && ge < nsteps ? std::max(r0, i0s[gen - 1]) : r0;
+        int klo = 0, kp1 = 0;
+        signed char *wg = nullptr;
+        const long wslice = (long)m * WG_K, wkb = (long)m * SPLIT_KSTEP;
+        if (split) {
             // digits of the group operand (exponent per column over both parts, from the
             // group's last solve), each part shifted by off so the K range starts at the K step
-            // boundary klo <= g0; then the product
-            const int klo = g0 / SPLIT_KSTEP * SPLIT_KSTEP, off = g0 - klo;
-            const int kp1 = round_up(off + wr, SPLIT_KSTEP);
+            // boundary klo <= g0
+            klo = g0 / SPLIT_KSTEP * SPLIT_KSTEP;
+            const int off = g0 - klo;
+            kp1 = round_up(off + wr, SPLIT_KSTEP);
             if (pencil_ready != nullptr) {
                 XTG_CHECK(hipStreamWaitEvent(st, pencil_ready, 0));
                 pencil_ready = nullptr;
             }
             int pr = prof_begin(st);
-            signed char *wg = ws.Wg.as<signed char>() + (long)c_begin * SPLIT_KSTEP;
-            const long wslice = (long)m * WG_K, wkb = (long)m * SPLIT_KSTEP;
+            wg = Wgb + (long)c_begin * SPLIT_KSTEP;
             hipLaunchKernelGGL(k_slice_digits, dim3((N + 63) / 64, (2 * kp1 + 15) / 16), dim3(256), 0, st,
-                               (const double *)(W + c_begin), ldw, off, wr, (const double *)(W + (long)wr * ldw + c_begin),
-                               ldw, kp1 + off, wr, N, 2 * kp1, (const int *)(ws.wgexp.as<int>() + c_begin), wg, wslice,
+                               (const double *)(Wb + c_begin), ldw, off, wr, (const double *)(Wb + (long)wr * ldw + c_begin),
+                               ldw, kp1 + off, wr, N, 2 * kp1, (const int *)(wgexpb + c_begin), wg, wslice,
                                wkb, 0, (signed char *)nullptr); XTG_LAUNCH_CHECK("k_slice_digits");
             prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * (2.0 * wr * N) + SPLIT_DIGITS * 2.0 * kp1 * N);
-            pr = prof_begin(st);
-            const long kbs = (long)n * SPLIT_KSTEP, ko = (long)(klo / SPLIT_KSTEP) * kbs + (long)r0 * SPLIT_KSTEP;
-            XTG_CHECK((hipError_t)launch_gemm_i8(st, M, N, kp1, kp1, ws.Sq.as<signed char>() + ko,
-                                                 ws.Pq.as<signed char>() + ko, qsl, kbs, wg, wslice, wkb,
-                                                 ws.rowexp.as<int>() + r0, ws.wgexp.as<int>() + c_begin,
-                                                 X + (long)r0 * ldx + c_begin, ldx, 1, nullptr, 1,
-                                                 ws.sig_grp.as<double>() + c_begin, nullptr));
-            prof_end(pr, st, XTGEVC3_PROF_UPDATE_SPLIT, 2.0 * M * N * (2.0 * wr),
-                     8.0 * (2.0 * M * N) + SPLIT_DIGITS * (2.0 * kp1 * M + 2.0 * kp1 * N));
         }
-        else if (M > 0 && N > 0) {
-            int pr = prof_begin(st);
-            const int gt = launch_gemm_f64(0, st, M, N, wr, wr, Sw + r0 + (long)g0 * ldsw, ldsw,
-                               Pw + r0 + (long)g0 * ldpw, ldpw, (const double *)(W + c_begin), ldw,
-                               X + (long)r0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
-                               (const double *)(ws.sig_grp.as<double>() + c_begin), (const int *)nullptr,
-                               (const double *)nullptr, prec, mx.on ? ws.rowexp.as<int>() + r0 : nullptr,
-                               mx.on ? mx.wgrp + c_begin : nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
-            prof_end(pr, st, update_kind(gt), 2.0 * M * N * (2.0 * wr),
-                     8.0 * (2.0 * M * N + 2.0 * wr * M + 2.0 * wr * N));
+        // the product for rows [ra, rb) on stream fst
+        auto far = [&](int ra, int rb, hipStream_t fst) -> int {
+            const int Mp = rb - ra;
+            if (Mp <= 0) {
+                return 0;
+            }
+            int pr = prof_begin(fst);
+            if (split) {
+                const long kbs = (long)n * SPLIT_KSTEP, ko = (long)(klo / SPLIT_KSTEP) * kbs + (long)ra * SPLIT_KSTEP;
+                XTG_CHECK((hipError_t)launch_gemm_i8(fst, Mp, N, kp1, kp1, ws.Sq.as<signed char>() + ko,
+                                                     ws.Pq.as<signed char>() + ko, qsl, kbs, wg, wslice, wkb,
+                                                     ws.rowexp.as<int>() + ra, wgexpb + c_begin,
+                                                     X + (long)ra * ldx + c_begin, ldx, 1, nullptr, 1, sgb + c_begin,
+                                                     nullptr));
+                prof_end(pr, fst, XTGEVC3_PROF_UPDATE_SPLIT, 2.0 * Mp * N * (2.0 * wr),
+                         8.0 * (2.0 * Mp * N) + SPLIT_DIGITS * (2.0 * kp1 * Mp + 2.0 * kp1 * N));
+            }
+            else {
+                const int gt = launch_gemm_f64(0, fst, Mp, N, wr, wr, Sw + ra + (long)g0 * ldsw, ldsw,
+                                   Pw + ra + (long)g0 * ldpw, ldpw, (const double *)(Wb + c_begin), ldw,
+                                   X + (long)ra * ldx + c_begin, ldx, (const int *)nullptr, 1,
+                                   (const double *)(sgb + c_begin), (const int *)nullptr,
+                                   (const double *)nullptr, prec, mx.on ? ws.rowexp.as<int>() + ra : nullptr,
+                                   mx.on ? mxb.wgrp + c_begin : nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
+                prof_end(pr, fst, update_kind(gt), 2.0 * Mp * N * (2.0 * wr),
+                         8.0 * (2.0 * Mp * N + 2.0 * wr * Mp + 2.0 * wr * N));
+            }
+            return 0;
+        };
+        if (const int frc = far(rsplit, g0, st)) {
+            return frc;
         }
+        if (rsplit > r0) {
+            XTG_CHECK(hipEventRecord(ws.ev_wready[par], st));
+            XTG_CHECK(hipStreamWaitEvent(ws.fs, ws.ev_wready[par], 0));
+            if (const int frc = far(r0, rsplit, ws.fs)) {
+                return frc;
+            }
+            XTG_CHECK(hipEventRecord(ws.ev_fdone[par], ws.fs));
+            f_pending = true;
+            last_par = par;
+        }
+    }
+    if (f_pending) {
+        XTG_CHECK(hipStreamWaitEvent(st, ws.ev_fdone[last_par], 0));
     }
     XTG_CHECK(hipGetLastError());
     if (pencil_ready != nullptr) {
@@ -630,6 +700,27 @@ static hipError_t split_side_init(RealWork &ws)
             e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
             if (e != hipSuccess) {
                 return e;
+            }
+        }
+    }
+    return hipSuccess;
+}
+
+// Creates the far-update lookahead stream and its events once per device workspace.
+static hipError_t far_stream_init(RealWork &ws)
+{
+    if (ws.fs == nullptr) {
+        hipError_t e = hipStreamCreateWithFlags(&ws.fs, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            ws.fs = nullptr;
+            return e;
+        }
+        for (int b = 0; b < 2; ++b) {
+            for (hipEvent_t *ev : {&ws.ev_wready[b], &ws.ev_fdone[b]}) {
+                e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+                if (e != hipSuccess) {
+                    return e;
+                }
             }
         }
     }
