@@ -59,21 +59,19 @@ struct Ring {
     size_t cap = 0;
 
     // waits until the last copy through buffer b has finished
-    bool wait(int b)
+    hipError_t wait(int b)
     {
         if (!armed[b]) {
-            return true;
+            return hipSuccess;
         }
         armed[b] = false;
-        return hipEventSynchronize(ev[b]) == hipSuccess;
+        return hipEventSynchronize(ev[b]);
     }
-    bool mark(int b, hipStream_t st)
+    hipError_t mark(int b, hipStream_t st)
     {
-        if (hipEventRecord(ev[b], st) != hipSuccess) {
-            return false;
-        }
-        armed[b] = true;
-        return true;
+        const hipError_t e = hipEventRecord(ev[b], st);
+        armed[b] = e == hipSuccess;
+        return e;
     }
     bool ensure(size_t colbytes)
     {
@@ -81,7 +79,7 @@ struct Ring {
         if (want <= cap) {
             return true;
         }
-        if (!wait(0) || !wait(1)) {
+        if (wait(0) != hipSuccess || wait(1) != hipSuccess) {
             return false;
         }
         cap = 0;
@@ -193,7 +191,7 @@ template <class T> bool upload(double *dev, const T *host, int rows, int cols, i
     const int cc = (int)std::min<size_t>(cols, g_ring.cap / colb);
     for (int c0 = 0, i = 0; c0 < cols; c0 += cc, ++i) {
         const int b = i & 1, nc = std::min(cc, cols - c0);
-        if (!hip_ok(g_ring.wait(b) ? hipSuccess : hipErrorUnknown, "staging wait")) {
+        if (!hip_ok(g_ring.wait(b), "staging wait")) {
             return false;
         }
         copy_cols((char *)g_ring.host[b], colb, (const char *)(host + (size_t)c0 * ld * w), (size_t)ld * w * sizeof(T),
@@ -214,7 +212,7 @@ template <class T> bool upload(double *dev, const T *host, int rows, int cols, i
                 return false;
             }
         }
-        if (!hip_ok(g_ring.mark(b, st) ? hipSuccess : hipErrorUnknown, "staging event")) {
+        if (!hip_ok(g_ring.mark(b, st), "staging event")) {
             return false;
         }
     }
@@ -238,7 +236,7 @@ template <class T> bool download(T *host, int ld, const double *dev, int rows, i
     const int nchunks = (cols + cc - 1) / cc;
     auto issue = [&](int i) -> bool {
         const int b = i & 1, c0 = i * cc, nc = std::min(cc, cols - c0);
-        if (!hip_ok(g_ring.wait(b) ? hipSuccess : hipErrorUnknown, "staging wait")) {
+        if (!hip_ok(g_ring.wait(b), "staging wait")) {
             return false;
         }
         const double *d = dev + (size_t)c0 * rows * w;
@@ -255,14 +253,14 @@ template <class T> bool download(T *host, int ld, const double *dev, int rows, i
                 return false;
             }
         }
-        return hip_ok(g_ring.mark(b, st) ? hipSuccess : hipErrorUnknown, "staging event");
+        return hip_ok(g_ring.mark(b, st), "staging event");
     };
     if (!issue(0) || (nchunks > 1 && !issue(1))) {
         return false;
     }
     for (int i = 0; i < nchunks; ++i) {
         const int b = i & 1, c0 = i * cc, nc = std::min(cc, cols - c0);
-        if (!hip_ok(g_ring.wait(b) ? hipSuccess : hipErrorUnknown, "staging wait")) {
+        if (!hip_ok(g_ring.wait(b), "staging wait")) {
             return false;
         }
         copy_cols((char *)(host + (size_t)c0 * ld * w), (size_t)ld * w * sizeof(T), (const char *)g_ring.host[b], colb,
