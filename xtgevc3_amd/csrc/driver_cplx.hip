@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <limits>
 #include <mutex>
 #include <vector>
@@ -109,6 +110,9 @@ struct ZWork {
     // digits, exponents
     DBuf Sx, Sy, Px, Py, rowexp, Wg, wgexp, wbexp, xbexp, sstart, xexp, Xx, Xy, Zq, zexp, kmax2, cfar;
     DBuf Vt, YA, YB; // eigenvalue condition numbers
+    // far-update lookahead: the rows above the next group run on `fs`
+    hipStream_t fs = nullptr;
+    hipEvent_t ev_wready[2] = {nullptr, nullptr}, ev_fdone[2] = {nullptr, nullptr};
 };
 
 std::mutex g_zmutex;
@@ -130,6 +134,27 @@ void shift_cplx(zt al, zt bv, double ascale, double bscale, zt &a, zt &b)
 }
 
 #define ZENSURE(buf, bytes) XTG_CHECK((buf).ensure(bytes))
+
+// Creates the far-update lookahead stream and its events once per device workspace.
+static hipError_t zfar_stream_init(ZWork &ws)
+{
+    if (ws.fs == nullptr) {
+        hipError_t e = hipStreamCreateWithFlags(&ws.fs, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            ws.fs = nullptr;
+            return e;
+        }
+        for (int b = 0; b < 2; ++b) {
+            for (hipEvent_t *ev : {&ws.ev_wready[b], &ws.ev_fdone[b]}) {
+                e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+                if (e != hipSuccess) {
+                    return e;
+                }
+            }
+        }
+    }
+    return hipSuccess;
+}
 
 int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 *Pw, long ldpw, const zt *al,
                   const zt *be, double ascale, double bscale, bool left, bool back, const double2 *Z, long ldz,
@@ -175,9 +200,13 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
     const long ldx = ((long)m + 7) / 8 * 8;
     ZENSURE(ws.X, sizeof(double2) * (size_t)n * ldx);
     const long ldw = ldx;
-    ZENSURE(ws.W, sizeof(double2) * (size_t)(2 * ZGROUP * NB_Z) * ldw);
+    // group operands, group scales (and their digits / exponents) alternate between two copies
+    // by group parity: the lookahead far update of one group reads its copy while the next
+    // group's steps write the other
+    const size_t wsz = (size_t)(2 * ZGROUP * NB_Z) * ldw;
+    ZENSURE(ws.W, sizeof(double2) * 2 * wsz);
     ZENSURE(ws.sig, sizeof(double) * (size_t)m);
-    ZENSURE(ws.sig_grp, sizeof(double) * (size_t)m);
+    ZENSURE(ws.sig_grp, sizeof(double) * 2 * (size_t)m);
     ZENSURE(ws.colexp, sizeof(int) * (size_t)m);
     ZENSURE(ws.blkexp, sizeof(int) * (size_t)nsteps * ldx);
     ZENSURE(ws.flag, sizeof(int) * 4);
@@ -206,8 +235,8 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
             ZENSURE(*b, (size_t)SPLIT_DIGITS * qsl);
         }
         ZENSURE(ws.rowexp, sizeof(int) * (size_t)n);
-        ZENSURE(ws.Wg, (size_t)SPLIT_DIGITS * m * ZWG_K);
-        ZENSURE(ws.wgexp, sizeof(int) * (size_t)m);
+        ZENSURE(ws.Wg, 2 * (size_t)SPLIT_DIGITS * m * ZWG_K);
+        ZENSURE(ws.wgexp, sizeof(int) * 2 * (size_t)m);
         ZENSURE(ws.wbexp, sizeof(int) * (size_t)nsteps * ldx);
         ZENSURE(ws.xbexp, sizeof(int) * (size_t)nsteps * ldx);
         ZENSURE(ws.sstart, sizeof(int) * (size_t)m);
@@ -248,11 +277,27 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
     // "far" update applies all of its blocks to the rows above g0 (K = 2 (g1 - g0)).  The
     // group's operand W holds rows [0, g1-g0) (S part: -a x) and [g1-g0, 2(g1-g0)) (P part: b x).
     int c_begin = m;
+    static const bool lookahead = std::getenv("XTGEVC3_NO_LOOKAHEAD") == nullptr;
+    if (lookahead) {
+        XTG_CHECK(zfar_stream_init(ws));
+    }
+    bool f_pending = false; // a far update on ws.fs has not been waited for yet
+    int last_par = 0;       // its group parity
     for (int gs = 0; gs < nsteps; gs += ZGROUP) {
         const int ge = std::min(nsteps, gs + ZGROUP);
         const int g1 = i0s[gs] + nbs[gs];
         const int g0 = i0s[ge - 1];
         const int wr = g1 - g0;
+        // this group's copies of the operand, group scales and their digits / exponents
+        const int par = (gs / ZGROUP) & 1;
+        if (f_pending && last_par == par) {
+            XTG_CHECK(hipStreamWaitEvent(st, ws.ev_fdone[last_par], 0));
+            f_pending = false;
+        }
+        double2 *Wb = W + (size_t)par * wsz;
+        double *sgb = ws.sig_grp.as<double>() + (size_t)par * m;
+        int *wgexpb = split ? ws.wgexp.as<int>() + (size_t)par * m : nullptr;
+        signed char *Wgb = split ? ws.Wg.as<signed char>() + (size_t)par * SPLIT_DIGITS * m * ZWG_K : nullptr;
         GroupInfo grp{};
         grp.wr = wr;
         {
@@ -282,10 +327,10 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
             int pr = prof_begin(st);
             hipLaunchKernelGGL(HIP_KERNEL_NAME(kz_solve_units<NB_Z>),
                                dim3((N + ZCOLS_PER_BLOCK - 1) / ZCOLS_PER_BLOCK), dim3(256), 0, st, Sw, ldsw, Pw,
-                               ldpw, i0, nb, X, ldx, W, ldw, ws.desc.as<ZColDesc>(), c_begin, m, ws.sig.as<double>(),
+                               ldpw, i0, nb, X, ldx, Wb, ldw, ws.desc.as<ZColDesc>(), c_begin, m, ws.sig.as<double>(),
                                ws.colexp.as<int>(), ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff,
-                               wr + woff, ws.sig_grp.as<double>(), s == gs ? 1 : 0, grp,
-                               split ? SplitOut{ws.wbexp.as<int>() + (long)s * ldx, ws.wgexp.as<int>(),
+                               wr + woff, sgb, s == gs ? 1 : 0, grp,
+                               split ? SplitOut{ws.wbexp.as<int>() + (long)s * ldx, wgexpb,
                                                 ws.xbexp.as<int>() + (long)s * ldx}
                                      : SplitOut{nullptr, nullptr, nullptr}); XTG_LAUNCH_CHECK("kz_solve_units");
             prof_end(pr, st, 0, 8.0 * nb * nb * (double)N, 16.0 * (4.0 * nb * N + 2.0 * nb * nb));
@@ -300,53 +345,92 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
                 pr = prof_begin(st);
                 auto kern = small ? kz_gemm<0, 32> : kz_gemm<0, ZTA_H>;
                 hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, M, N, nb, nb, Sw + g0 + (long)i0 * ldsw, ldsw,
-                                   Pw + g0 + (long)i0 * ldpw, ldpw, (const double2 *)(W + (long)woff * ldw + c_begin),
+                                   Pw + g0 + (long)i0 * ldpw, ldpw, (const double2 *)(Wb + (long)woff * ldw + c_begin),
                                    ldw, X + (long)g0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
                                    (const double *)(ws.sig.as<double>() + c_begin), (const int *)nullptr,
-                                   (const double2 *)(W + (long)(wr + woff) * ldw + c_begin)); XTG_LAUNCH_CHECK("kz_gemm<0>");
+                                   (const double2 *)(Wb + (long)(wr + woff) * ldw + c_begin)); XTG_LAUNCH_CHECK("kz_gemm<0>");
                 prof_end(pr, st, 1, 8.0 * M * N * (2.0 * nb), 16.0 * (2.0 * M * N + 2.0 * nb * M + 2.0 * nb * N));
             }
+        }
+        if (f_pending && !(first || g0 == 0)) {
+            // the previous group's rows above this group are written again below
+            XTG_CHECK(hipStreamWaitEvent(st, ws.ev_fdone[last_par], 0));
+            f_pending = false;
         }
         if (first || g0 == 0) {
             continue;
         }
-        // far update: rows above the group, all of its blocks at once
+        // far update: rows above the group, all of its blocks at once.  With the lookahead the
+        // rows of the next group run on st and the rows above it on ws.fs, overlapping the
+        // next group's steps; the next far update waits for them.
         const int N = m - c_begin;
+        const int gen = std::min(nsteps, ge + ZGROUP);
+        const int rsplit = lookahead && ge < nsteps ? i0s[gen - 1] : 0;
+        int klo = 0, kp1 = 0;
+        signed char *wg = nullptr;
+        const long wslice = (long)m * ZWG_K, wkb = (long)m * ZSPLIT_KSTEP;
         if (split) {
             // digits of the group operand, (re, im) expanded and shifted by off so the expanded K
             // range starts at the K step boundary klo <= 2 g0; Re and Im as two real products
-            const int klo = (2 * g0) / ZSPLIT_KSTEP * ZSPLIT_KSTEP, off = 2 * g0 - klo;
-            const int kp1 = zround_up(off + 2 * wr, ZSPLIT_KSTEP);
-            signed char *wg = ws.Wg.as<signed char>() + (long)c_begin * ZSPLIT_KSTEP;
-            const long wslice = (long)m * ZWG_K, wkb = (long)m * ZSPLIT_KSTEP;
+            klo = (2 * g0) / ZSPLIT_KSTEP * ZSPLIT_KSTEP;
+            const int off = 2 * g0 - klo;
+            kp1 = zround_up(off + 2 * wr, ZSPLIT_KSTEP);
+            wg = Wgb + (long)c_begin * ZSPLIT_KSTEP;
             int pr = prof_begin(st);
             hipLaunchKernelGGL(k_slice_digits, dim3((N + 63) / 64, (2 * kp1 + 15) / 16), dim3(256), 0, st,
-                               (const double *)(W + c_begin), ldw, off / 2, wr,
-                               (const double *)(W + (long)wr * ldw + c_begin), ldw, (kp1 + off) / 2, wr, N, 2 * kp1,
-                               (const int *)(ws.wgexp.as<int>() + c_begin), wg, wslice, wkb, 3,
+                               (const double *)(Wb + c_begin), ldw, off / 2, wr,
+                               (const double *)(Wb + (long)wr * ldw + c_begin), ldw, (kp1 + off) / 2, wr, N, 2 * kp1,
+                               (const int *)(wgexpb + c_begin), wg, wslice, wkb, 3,
                                (signed char *)nullptr); XTG_LAUNCH_CHECK("k_slice_digits");
             prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 16.0 * (2.0 * wr * N) + SPLIT_DIGITS * 2.0 * kp1 * N);
-            const long ko = (long)(klo / ZSPLIT_KSTEP) * kbs;
-            pr = prof_begin(st);
-            for (int part = 0; part < 2; ++part) {
-                const signed char *ls = (part == 0 ? ws.Sx : ws.Sy).as<signed char>() + ko;
-                const signed char *lp = (part == 0 ? ws.Px : ws.Py).as<signed char>() + ko;
-                XTG_CHECK((hipError_t)launch_gemm_i8(st, g0, N, kp1, kp1, ls, lp, qsl, kbs, wg, wslice, wkb,
-                                                     ws.rowexp.as<int>(), ws.wgexp.as<int>() + c_begin,
-                                                     (double *)(X + c_begin) + part, 2 * ldx, 2, nullptr, 1,
-                                                     ws.sig_grp.as<double>() + c_begin, nullptr));
-            }
-            prof_end(pr, st, XTGEVC3_PROF_UPDATE_SPLIT, 8.0 * g0 * N * (2.0 * wr),
-                     16.0 * (2.0 * g0 * N) + 2.0 * SPLIT_DIGITS * (2.0 * kp1 * g0 + 2.0 * kp1 * N));
-            continue;
         }
-        dim3 grid(((N + ZTB_H - 1) / ZTB_H) * ((g0 + ZTA_H - 1) / ZTA_H));
-        int pr = prof_begin(st);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(kz_gemm<0, ZTA_H>), grid, dim3(256), 0, st, g0, N, wr, wr, Sw + (long)g0 * ldsw, ldsw,
-                           Pw + (long)g0 * ldpw, ldpw, (const double2 *)(W + c_begin), ldw, X + c_begin, ldx,
-                           (const int *)nullptr, 1, (const double *)(ws.sig_grp.as<double>() + c_begin),
-                           (const int *)nullptr, (const double2 *)(W + (long)wr * ldw + c_begin)); XTG_LAUNCH_CHECK("kz_gemm<0>");
-        prof_end(pr, st, 1, 8.0 * g0 * N * (2.0 * wr), 16.0 * (2.0 * g0 * N + 2.0 * wr * g0 + 2.0 * wr * N));
+        // the product for rows [ra, rb) on stream fst
+        auto far = [&](int ra, int rb, hipStream_t fst) -> int {
+            const int Mp = rb - ra;
+            if (Mp <= 0) {
+                return 0;
+            }
+            int pr = prof_begin(fst);
+            if (split) {
+                const long ko = (long)(klo / ZSPLIT_KSTEP) * kbs + (long)ra * ZSPLIT_KSTEP;
+                for (int part = 0; part < 2; ++part) {
+                    const signed char *ls = (part == 0 ? ws.Sx : ws.Sy).as<signed char>() + ko;
+                    const signed char *lp = (part == 0 ? ws.Px : ws.Py).as<signed char>() + ko;
+                    XTG_CHECK((hipError_t)launch_gemm_i8(fst, Mp, N, kp1, kp1, ls, lp, qsl, kbs, wg, wslice, wkb,
+                                                         ws.rowexp.as<int>() + ra, wgexpb + c_begin,
+                                                         (double *)(X + (long)ra * ldx + c_begin) + part, 2 * ldx, 2,
+                                                         nullptr, 1, sgb + c_begin, nullptr));
+                }
+                prof_end(pr, fst, XTGEVC3_PROF_UPDATE_SPLIT, 8.0 * Mp * N * (2.0 * wr),
+                         16.0 * (2.0 * Mp * N) + 2.0 * SPLIT_DIGITS * (2.0 * kp1 * Mp + 2.0 * kp1 * N));
+            }
+            else {
+                dim3 grid(((N + ZTB_H - 1) / ZTB_H) * ((Mp + ZTA_H - 1) / ZTA_H));
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(kz_gemm<0, ZTA_H>), grid, dim3(256), 0, fst, Mp, N, wr, wr,
+                                   Sw + ra + (long)g0 * ldsw, ldsw, Pw + ra + (long)g0 * ldpw, ldpw,
+                                   (const double2 *)(Wb + c_begin), ldw, X + (long)ra * ldx + c_begin, ldx,
+                                   (const int *)nullptr, 1, (const double *)(sgb + c_begin), (const int *)nullptr,
+                                   (const double2 *)(Wb + (long)wr * ldw + c_begin)); XTG_LAUNCH_CHECK("kz_gemm<0>");
+                prof_end(pr, fst, 1, 8.0 * Mp * N * (2.0 * wr), 16.0 * (2.0 * Mp * N + 2.0 * wr * Mp + 2.0 * wr * N));
+            }
+            return 0;
+        };
+        if (const int frc = far(rsplit, g0, st)) {
+            return frc;
+        }
+        if (rsplit > 0) {
+            XTG_CHECK(hipEventRecord(ws.ev_wready[par], st));
+            XTG_CHECK(hipStreamWaitEvent(ws.fs, ws.ev_wready[par], 0));
+            if (const int frc = far(0, rsplit, ws.fs)) {
+                return frc;
+            }
+            XTG_CHECK(hipEventRecord(ws.ev_fdone[par], ws.fs));
+            f_pending = true;
+            last_par = par;
+        }
+    }
+    if (f_pending) {
+        XTG_CHECK(hipStreamWaitEvent(st, ws.ev_fdone[last_par], 0));
     }
     XTG_CHECK(hipGetLastError());
     {
