@@ -251,7 +251,8 @@ def backward_accuracy(seed, n=2048, nsel=256, nbatch=4, offdiag=0.003):
         res[p] = (Vt, torch.cat([gS.flatten(), gP.flatten()]))
     V0, g0 = res[X.PREC_FP64]
     scale = float(g0.abs().max())
-    out = {"pencil": f"configs construction, off-diagonals x{offdiag}, {nsel} best-separated real eigenvalues, "
+    out = {"pencil": f"quasi-triangular pencil (40% pairs), off-diagonals x{offdiag}, {nsel} best-separated real "
+                     f"eigenvalues, "
                      f"{nbatch} cotangent batches", "fp64_grad_max": float("%.4g" % scale),
            "fp64_grad_finite": bool(torch.isfinite(g0).all())}
     for p, name in ((X.PREC_SPLIT_I8, "split_int8"), (X.PREC_MIXED_F32, "fp32_products")):
