@@ -154,3 +154,30 @@ def test_real_graded_stays_finite(howmny):
     assert not np.any(dg & ~do)
     assert np.all(kind[dg] > 0) and dg.sum() < n // 4
     assert np.allclose(mg[~dg], 1.0, rtol=0, atol=64 * EPS)
+
+
+def test_calls_on_two_streams_share_the_workspace():
+    """Two device calls issued back to back on different streams (no synchronisation between
+    them) share the device's workspace: each call waits for the previous call's work, so both
+    results equal the sequential ones bit for bit."""
+    import torch
+
+    import xtgevc3_amd as X
+
+    n = 600
+    pencils = [G.real_quasi_triangular(n, 70 + i) for i in range(2)]
+    inputs = [(X.fortran_device(S), X.fortran_device(P), ar, ai, be) for S, P, ar, ai, be in pencils]
+    refs = []
+    for dS, dP, ar, ai, be in inputs:
+        V = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+        X.dtgevc3_device("R", "A", None, dS, dP, ar, ai, be, VR=V)
+        torch.cuda.synchronize()
+        refs.append(V)
+    outs = [torch.zeros((n, n), dtype=torch.float64, device="cuda") for _ in inputs]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    torch.cuda.synchronize()
+    for (dS, dP, ar, ai, be), s, V in zip(inputs, streams, outs):
+        X.dtgevc3_device("R", "A", None, dS, dP, ar, ai, be, VR=V, stream=s.cuda_stream)
+    torch.cuda.synchronize()
+    for V, R in zip(outs, refs):
+        assert torch.equal(V, R)

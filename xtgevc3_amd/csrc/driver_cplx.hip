@@ -113,6 +113,33 @@ struct ZWork {
     // far-update lookahead: the rows above the next group run on `fs`
     hipStream_t fs = nullptr;
     hipEvent_t ev_wready[2] = {nullptr, nullptr}, ev_fdone[2] = {nullptr, nullptr};
+    hipEvent_t ev_last = nullptr; // end of the previous call's work on this workspace
+};
+
+// Orders a call after the previous call's work on the device's workspace, whatever stream
+// that ran on, and records the end of this call when it goes out of scope.
+struct ZWsOrder {
+    hipEvent_t *ev;
+    hipStream_t st;
+    bool on = false; // off while st is being captured into a graph (the graph orders itself)
+    hipError_t begin()
+    {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+            return hipSuccess;
+        }
+        on = true;
+        if (*ev == nullptr) {
+            return hipEventCreateWithFlags(ev, hipEventDisableTiming);
+        }
+        return hipStreamWaitEvent(st, *ev, 0);
+    }
+    ~ZWsOrder()
+    {
+        if (on && *ev != nullptr) {
+            (void)hipEventRecord(*ev, st);
+        }
+    }
 };
 
 std::mutex g_zmutex;
@@ -523,6 +550,8 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
     XTG_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_zmutex);
     ZWork &ws = g_zws[dev & 63];
+    ZWsOrder order{&ws.ev_last, st};
+    XTG_CHECK(order.begin());
     bool right = side == 'R' || side == 'B';
     bool left = side == 'L' || side == 'B';
     bool back = howmny == 'B';
@@ -606,6 +635,8 @@ int complex_tgsna_device(char howmny, const int *select, int n, const std::compl
     XTG_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_zmutex);
     ZWork &ws = g_zws[dev & 63];
+    ZWsOrder order{&ws.ev_last, st};
+    XTG_CHECK(order.begin());
     std::vector<Unit> units;
     const int mc = complex_units(howmny, select, n, units);
     if (mc != m) {

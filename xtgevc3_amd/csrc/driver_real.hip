@@ -340,6 +340,34 @@ struct RealWork {
     // far-update lookahead: the rows above the next group run on `fs`
     hipStream_t fs = nullptr;
     hipEvent_t ev_wready[2] = {nullptr, nullptr}, ev_fdone[2] = {nullptr, nullptr};
+    hipEvent_t ev_last = nullptr; // end of the previous call's work on this workspace
+};
+
+// Orders a call after the previous call's work on the device's workspace, whatever stream
+// that ran on (calls on one device then share it safely), and records the end of this call
+// when it goes out of scope.
+struct WsOrder {
+    hipEvent_t *ev;
+    hipStream_t st;
+    bool on = false; // off while st is being captured into a graph (the graph orders itself)
+    hipError_t begin()
+    {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+            return hipSuccess;
+        }
+        on = true;
+        if (*ev == nullptr) {
+            return hipEventCreateWithFlags(ev, hipEventDisableTiming);
+        }
+        return hipStreamWaitEvent(st, *ev, 0);
+    }
+    ~WsOrder()
+    {
+        if (on && *ev != nullptr) {
+            (void)hipEventRecord(*ev, st);
+        }
+    }
 };
 
 static std::mutex g_ws_mutex;
@@ -926,6 +954,8 @@ int real_device(char side, char howmny, const int *select, int n, const double *
     XTG_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_ws_mutex);
     RealWork &ws = g_ws[dev & 63];
+    WsOrder order{&ws.ev_last, st};
+    XTG_CHECK(order.begin());
 
     bool right = side == 'R' || side == 'B';
     bool left = side == 'L' || side == 'B';
@@ -1054,6 +1084,8 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
     XTG_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_ws_mutex);
     RealWork &ws = g_ws[dev & 63];
+    WsOrder order{&ws.ev_last, st};
+    XTG_CHECK(order.begin());
     std::vector<Unit> units;
     int m = real_units(select ? 'S' : 'A', select, n, ai, units);
     *info = 0;
@@ -1203,6 +1235,8 @@ int real_tgsna_device(char howmny, const int *select, int n, const double *S, lo
     XTG_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_ws_mutex);
     RealWork &ws = g_ws[dev & 63];
+    WsOrder order{&ws.ev_last, st};
+    XTG_CHECK(order.begin());
     std::vector<Unit> units;
     const int mc = real_units(howmny, select, n, ai, units);
     if (mc != m) {
