@@ -51,7 +51,7 @@ constexpr int SOLVE_WAVES_H = 4; // waves per solve workgroup (they share the st
 int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const double *L, long ldl, const double *L2,
                      long ldl2, const double *R, long ldr, double *C, long ldc, const int *outcol, int mode,
                      const double *sig, const int *kmax, const double *R2, int prec, const int *sa,
-                     const int *sb);
+                     const int *sb, int batch = 1, long bsr = 0, long bsc = 0);
 constexpr int GROUP_STEPS = 4; // row blocks per far update (K = 2 * ~256)
 __global__ void k_fixup_scale(double *, long, int, int, const int *, const int *, long, const int *, const int *);
 __global__ void k_copy_out(const double *, long, int, int, double *, long, const int *, int);
@@ -330,7 +330,7 @@ struct DevBuf {
 
 struct RealWork {
     DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wblk, wgrp, xrel, xfin, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
-    DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp;
+    DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp, bkpre;
     DevBuf Ut, YB, part, tcol, tw; // eigenvalue condition numbers
     DevBuf Sq, Pq, Wg, wgexp, wbexp, xbexp, sstart, Xq, Zq, xexp, cfar; // split-int8 digits and exponents
     // split-int8 form: side stream (default priority) for the digit splitting of the pencil
@@ -1202,16 +1202,33 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
     hipLaunchKernelGGL(k_bwd_lambda_t, dim3((M + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, st, (const double *)X, ldx,
                        n, m, nbatch, ws.kcol.as<int>(), ws.coef.as<double>(), ws.colexp.as<int>(),
                        ws.LamS.as<double>(), ws.LamP.as<double>()); XTG_LAUNCH_CHECK("k_bwd_lambda_t");
-    // dS_b = V * LamS_b, dP_b = V * LamP_b  (as D[j][i] = sum_c V[j][c] Lam[c][i])
-    for (int b = 0; b < nbatch; ++b) {
+    // dS_b = V * LamS_b, dP_b = V * LamP_b  (as D[j][i] = sum_c V[j][c] Lam[c][i]), all batches
+    // in one launch each.  v_c is zero below row kcol[c] and kcol ascends, so with the columns
+    // taken in reverse order (negative strides) row j of the product needs only the first
+    // kpre[j] = #{c : kcol[c] >= j} of them: a per-row K bound.
+    std::vector<int> kpre(n);
+    double kwork = 0.0;
+    for (int j = 0, c = 0; j < n; ++j) {
+        while (c < m && kcol[c] < j) {
+            ++c;
+        }
+        kpre[j] = m - c;
+        kwork += kpre[j];
+    }
+    ENSURE(ws.bkpre, sizeof(int) * (size_t)n);
+    XTG_CHECK(hipMemcpyAsync(ws.bkpre.p, kpre.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    {
+        const double *Vrev = V + (long)(m - 1) * ldv;
+        const long lrev = (long)(m - 1) * n;
         pr = prof_begin(st);
-        launch_gemm_f64(2, st, n, n, m, 0, V, ldv, (const double *)nullptr, 0L,
-                           (const double *)(ws.LamS.as<double>() + (long)b * m * n), (long)n, dS + (long)b * n * ldds,
-                           ldds, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr, (const double *)nullptr, 0, nullptr, nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
-        launch_gemm_f64(2, st, n, n, m, 0, V, ldv, (const double *)nullptr, 0L,
-                           (const double *)(ws.LamP.as<double>() + (long)b * m * n), (long)n, dP + (long)b * n * lddp,
-                           lddp, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr, (const double *)nullptr, 0, nullptr, nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
-        prof_end(pr, st, 3, 4.0 * n * n * (double)m, 8.0 * 2.0 * ((double)n * m + (double)m * n + (double)n * n));
+        launch_gemm_f64(2, st, n, n, m, 0, Vrev, -ldv, (const double *)nullptr, 0L, ws.LamS.as<double>() + lrev,
+                        -(long)n, dS, ldds, (const int *)nullptr, 0, (const double *)nullptr, ws.bkpre.as<int>(),
+                        (const double *)nullptr, 0, nullptr, nullptr, nbatch, (long)m * n, (long)n * ldds); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+        launch_gemm_f64(2, st, n, n, m, 0, Vrev, -ldv, (const double *)nullptr, 0L, ws.LamP.as<double>() + lrev,
+                        -(long)n, dP, lddp, (const int *)nullptr, 0, (const double *)nullptr, ws.bkpre.as<int>(),
+                        (const double *)nullptr, 0, nullptr, nullptr, nbatch, (long)m * n, (long)n * lddp); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+        prof_end(pr, st, 3, 4.0 * n * kwork * nbatch,
+                 8.0 * 2.0 * nbatch * ((double)n * m + (double)m * n + (double)n * n));
     }
     hipLaunchKernelGGL(k_bwd_diag, dim3(M), dim3(64), 0, st, (const double *)X, ldx, (const double *)ws.Y.as<double>(),
                        ldt, n, m, nbatch, ws.kcol.as<int>(), ws.coef.as<double>(), ws.colexp.as<int>(), dS, ldds, dP,

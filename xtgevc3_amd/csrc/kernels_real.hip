@@ -768,6 +768,8 @@ __global__ __launch_bounds__(256) void k_solve_real(
 //   mode 0: C = D;  mode 1: C = sig[b] * C + D.
 //   kmax (optional): per-a upper bound of k (exclusive); a tile uses the max over its rows,
 //   so triangular operands skip their zero part.
+//   Batches (gridDim.y > 1): batch y reads R + y*bsr (and R2 + y*bsr) and writes C + y*bsc;
+//   L, kmax and the other arguments are shared.
 // Workgroup: 256 threads (2x2 waves), tile GT (a) x GT (b) (each wave GT/2 x GT/2), K chunks
 // of GK rows double-buffered in LDS; the next chunk is fetched into registers while the
 // current one feeds the MFMAs.  GT = 128, GK = 16 for large products; GT = 64, GK = 32 when
@@ -791,10 +793,15 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
                                                   long ldc, const int *__restrict__ outcol, int mode,
                                                   const double *__restrict__ sig, const int *__restrict__ kmax,
                                                   const double *__restrict__ R2, const int *__restrict__ sa,
-                                                  const int *__restrict__ sb)
+                                                  const int *__restrict__ sb, long bsr, long bsc)
 {
+    R += (long)blockIdx.y * bsr;
+    C += (long)blockIdx.y * bsc;
     if (R2 == nullptr) {
         R2 = R + (long)K1 * ldr;
+    }
+    else {
+        R2 += (long)blockIdx.y * bsr;
     }
     typedef typename std::conditional<F32, float, double>::type el_t;
     __shared__ el_t Ls[2][GK][GTA];
@@ -1396,8 +1403,9 @@ __global__ __launch_bounds__(64) void k_bwd_diag(const double *__restrict__ X, l
 // estimated time, counting workgroup rounds (concurrent workgroups per CU from LDS and
 // registers; a last partial round costs about half a round plus its share) times the cost of
 // one full round measured on MI355X (K = 512: 128x64 and 64x128 tiles run three workgroups per
-// CU and beat 128x128 per unit area; for deep products 128x128 wins).
-int gemm_config(int M, int N, int K)
+// CU and beat 128x128 per unit area; for deep products 128x128 wins).  batch products of the
+// same shape share the grid.
+int gemm_config(int M, int N, int K, int batch)
 {
     struct Cfg {
         int id, ga, gb, per_cu;
@@ -1416,7 +1424,7 @@ int gemm_config(int M, int N, int K)
         if (c.id == GEMM_32x64 && (M > 256 || K > 256)) {
             continue; // measured only on the short near-update products
         }
-        const long tiles = (long)((M + c.ga - 1) / c.ga) * ((N + c.gb - 1) / c.gb);
+        const long tiles = (long)((M + c.ga - 1) / c.ga) * ((N + c.gb - 1) / c.gb) * batch;
         const long conc = 256L * c.per_cu;
         const long full = tiles / conc, rem = tiles % conc;
         const double rounds = (double)full + (rem ? 0.5 + 0.5 * (double)rem / conc : 0.0);
@@ -1436,22 +1444,22 @@ int gemm_config(int M, int N, int K)
 int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const double *L, long ldl, const double *L2,
                      long ldl2, const double *R, long ldr, double *C, long ldc, const int *outcol, int mode,
                      const double *sig, const int *kmax, const double *R2, int prec, const int *sa,
-                     const int *sb)
+                     const int *sb, int batch, long bsr, long bsc)
 {
     const bool wide = M >= 2 && N >= 2;
-    const int cfg = gemm_config(M, N, K1 + K2);
+    const int cfg = gemm_config(M, N, K1 + K2, batch);
     const int gta = cfg == GEMM_32x64 ? 32 : (cfg == GEMM_64x64 || cfg == GEMM_64x128 ? 64 : 128);
     const int gtb = cfg == GEMM_64x64 || cfg == GEMM_128x64 || cfg == GEMM_32x64 ? 64 : 128;
-    const dim3 grid(((N + gtb - 1) / gtb) * ((M + gta - 1) / gta));
+    const dim3 grid(((N + gtb - 1) / gtb) * ((M + gta - 1) / gta), batch);
 #define XTG_GEMM_LAUNCH(T, WD, GA, GB, K)                                                                            \
     if (prec == 1 && T != 2) {                                                                                       \
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, WD, GA, GB, K, true>), grid, dim3(256), 0, st, M, N, K1, K2, \
-                           L, ldl, L2, ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2, sa, sb);                   \
+                           L, ldl, L2, ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2, sa, sb, bsr, bsc);         \
     }                                                                                                                \
     else {                                                                                                           \
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, WD, GA, GB, K>), grid, dim3(256), 0, st, M, N, K1, K2, L,  \
                            ldl, L2, ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2, (const int *)nullptr,         \
-                           (const int *)nullptr);                                                                    \
+                           (const int *)nullptr, bsr, bsc);                                                          \
     }
 #define XTG_GEMM_WIDE(T, GA, GB, K)                                                                                  \
     if (wide) {                                                                                                      \
