@@ -10,7 +10,9 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <future>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -113,8 +115,16 @@ struct Ring {
 };
 
 std::mutex g_stage_mutex;
-DevBuf g_dS, g_dP, g_dVL, g_dVR;
+DevBuf g_dS, g_dP, g_dVL, g_dVR, g_dZR, g_dZL;
 Ring g_ring;
+
+// Per-device copy stream (non-blocking, default priority) and the events that mark the end of
+// the right and left Z uploads on it.
+struct CopyLane {
+    hipStream_t s = nullptr;
+    hipEvent_t done[2] = {nullptr, nullptr};
+};
+CopyLane g_lane[64];
 
 int fail(int *info, const char *what)
 {
@@ -151,11 +161,11 @@ __global__ void k_narrow(const double *__restrict__ src, float *__restrict__ dst
 
 unsigned flat_grid(long len) { return (unsigned)std::min<long>((len + 255) / 256, 4096); }
 
-// Copies `cols` columns of `colbytes` bytes between strided host arrays, on up to 8 threads
+// Copies `cols` columns of `colbytes` bytes between strided host arrays, on up to maxt threads
 // when the chunk is large (packing pageable memory is bound by per-thread copy bandwidth).
-void copy_cols(char *dst, size_t dstride, const char *src, size_t sstride, size_t colbytes, int cols)
+void copy_cols(char *dst, size_t dstride, const char *src, size_t sstride, size_t colbytes, int cols, int maxt = 8)
 {
-    const int nt = colbytes * cols >= ((size_t)4 << 20) ? std::min(cols, 8) : 1;
+    const int nt = colbytes * cols >= ((size_t)4 << 20) ? std::min(cols, maxt) : 1;
     auto part = [=](int t) {
         const int c1 = (int)((long)cols * (t + 1) / nt);
         for (int c = (int)((long)cols * t / nt); c < c1; ++c) {
@@ -178,7 +188,8 @@ void copy_cols(char *dst, size_t dstride, const char *src, size_t sstride, size_
 
 // Moves a rows x cols column-major host matrix (leading dimension ld, elements of w scalars of
 // type T = double or float) into a packed device matrix of w doubles per element, queued on st.
-template <class T> bool upload(double *dev, const T *host, int rows, int cols, int ld, int w, hipStream_t st)
+template <class T>
+bool upload(double *dev, const T *host, int rows, int cols, int ld, int w, hipStream_t st, int maxt = 8)
 {
     if (rows <= 0 || cols <= 0) {
         return true;
@@ -195,7 +206,7 @@ template <class T> bool upload(double *dev, const T *host, int rows, int cols, i
             return false;
         }
         copy_cols((char *)g_ring.host[b], colb, (const char *)(host + (size_t)c0 * ld * w), (size_t)ld * w * sizeof(T),
-                  colb, nc);
+                  colb, nc, maxt);
         double *d = dev + (size_t)c0 * rows * w;
         if (std::is_same<T, double>::value) {
             if (!hip_ok(hipMemcpyAsync(d, g_ring.host[b], colb * nc, hipMemcpyHostToDevice, st), "host to device copy")) {
@@ -272,6 +283,77 @@ template <class T> bool download(T *host, int ld, const double *dev, int rows, i
     return true;
 }
 
+// Z of the back-transform (the input VR / VL of howmny = 'B') crosses PCIe on the device's copy
+// stream from a worker thread while the sweep runs on the calling stream: the right matrix
+// first, then the left, each followed by an event.  feed.ready(side) blocks until the worker
+// has queued that side's copies and returns its event, so the back-transform waits on the
+// device only for its own matrix.  finish() joins the worker; every path calls it before the
+// host arrays are released to the caller.
+constexpr int ZSTAGE_THREADS = 4; // packing threads beside the sweep's launching thread
+template <class T> struct ZStage {
+    std::thread th;
+    std::promise<bool> done[2];
+    std::shared_future<bool> fut[2];
+    std::string err;
+    CopyLane *lane = nullptr;
+    ZFeed feed{};
+
+    bool start(int dev, double *dzr, const T *zr, int ldzr, double *dzl, const T *zl, int ldzl, int n, int w)
+    {
+        CopyLane &l = g_lane[dev & 63];
+        if (l.s == nullptr) {
+            if (!hip_ok(hipStreamCreateWithFlags(&l.s, hipStreamNonBlocking), "copy stream")) {
+                return false;
+            }
+            for (hipEvent_t &e : l.done) {
+                if (!hip_ok(hipEventCreateWithFlags(&e, hipEventDisableTiming), "copy event")) {
+                    return false;
+                }
+            }
+        }
+        lane = &l;
+        for (int i = 0; i < 2; ++i) {
+            fut[i] = done[i].get_future().share();
+        }
+        feed.zr = dzr;
+        feed.zl = dzl;
+        feed.ready = [this](bool left) -> hipEvent_t {
+            if (!fut[left ? 1 : 0].get()) {
+                set_error(err);
+                return nullptr;
+            }
+            return lane->done[left ? 1 : 0];
+        };
+        auto body = [this, dev, zr, zl, dzr, dzl, ldzr, ldzl, n, w]() {
+            bool ok = hip_ok(hipSetDevice(dev), "copy thread device");
+            const T *src[2] = {zr, zl};
+            double *dst[2] = {dzr, dzl};
+            const int ld[2] = {ldzr, ldzl};
+            for (int i = 0; i < 2; ++i) {
+                ok = ok && (src[i] == nullptr || upload(dst[i], src[i], n, n, ld[i], w, lane->s, ZSTAGE_THREADS)) &&
+                     hip_ok(hipEventRecord(lane->done[i], lane->s), "copy event");
+                if (!ok && err.empty()) {
+                    err = last_error();
+                }
+                done[i].set_value(ok);
+            }
+        };
+        try {
+            th = std::thread(body);
+        } catch (...) {
+            body();
+        }
+        return true;
+    }
+    void finish()
+    {
+        if (th.joinable()) {
+            th.join();
+        }
+    }
+    ~ZStage() { finish(); }
+};
+
 // Host-pointer driver for the real interfaces.  T (double, or float for stgevc3) is the host
 // scalar type; the eigenvalue arrays are FP64 and the device computes in FP64.
 template <class T>
@@ -293,21 +375,33 @@ void real_host(char side, char howmny, const int *select, int n, const T *S, int
     bool back = howmny == 'B';
     std::lock_guard<std::mutex> lock(g_stage_mutex);
     const hipStream_t st = nullptr;
+    int dev = 0;
     size_t nn = (size_t)n * n * sizeof(double);
-    size_t nm = (size_t)n * (back ? n : num_sel) * sizeof(double);
-    if (!g_dS.ensure(nn) || !g_dP.ensure(nn) || (right && !g_dVR.ensure(nm)) || (left && !g_dVL.ensure(nm))) {
+    size_t nm = (size_t)n * num_sel * sizeof(double);
+    if (!hip_ok(hipGetDevice(&dev), "device query")) {
+        fail(info, "staging");
+        return;
+    }
+    if (!g_dS.ensure(nn) || !g_dP.ensure(nn) || (right && !g_dVR.ensure(nm)) || (left && !g_dVL.ensure(nm)) ||
+        (back && right && !g_dZR.ensure(nn)) || (back && left && !g_dZL.ensure(nn))) {
         set_error("device allocation failed");
         fail(info, "staging");
         return;
     }
     double *dS = (double *)g_dS.p, *dP = (double *)g_dP.p, *dVL = (double *)g_dVL.p, *dVR = (double *)g_dVR.p;
-    if (!upload(dS, S, n, n, lds, 1, st) || !upload(dP, P, n, n, ldp, 1, st) ||
-        (back && right && !upload(dVR, VR, n, n, ldvr, 1, st)) || (back && left && !upload(dVL, VL, n, n, ldvl, 1, st))) {
+    if (!upload(dS, S, n, n, lds, 1, st) || !upload(dP, P, n, n, ldp, 1, st)) {
+        fail(info, "staging");
+        return;
+    }
+    ZStage<T> zs;
+    if (back && !zs.start(dev, (double *)g_dZR.p, right ? VR : nullptr, ldvr, (double *)g_dZL.p, left ? VL : nullptr,
+                          ldvl, n, 1)) {
         fail(info, "staging");
         return;
     }
     int rc = real_device(side, howmny, select, n, dS, n, dP, n, ar, ai, be, left ? dVL : nullptr, n,
-                         right ? dVR : nullptr, n, st);
+                         right ? dVR : nullptr, n, st, XTGEVC3_PREC_FP64, back ? &zs.feed : nullptr);
+    zs.finish();
     if (rc == 0 && !hip_ok(hipDeviceSynchronize(), "kernel execution")) {
         rc = -1;
     }
@@ -344,9 +438,15 @@ void complex_host(char side, char howmny, const int *select, int n, const std::c
     bool back = howmny == 'B';
     std::lock_guard<std::mutex> lock(g_stage_mutex);
     const hipStream_t st = nullptr;
+    int dev = 0;
     size_t nn = (size_t)n * n * sizeof(zt);
-    size_t nm = (size_t)n * (back ? n : num_sel) * sizeof(zt);
-    if (!g_dS.ensure(nn) || !g_dP.ensure(nn) || (right && !g_dVR.ensure(nm)) || (left && !g_dVL.ensure(nm))) {
+    size_t nm = (size_t)n * num_sel * sizeof(zt);
+    if (!hip_ok(hipGetDevice(&dev), "device query")) {
+        fail(info, "staging");
+        return;
+    }
+    if (!g_dS.ensure(nn) || !g_dP.ensure(nn) || (right && !g_dVR.ensure(nm)) || (left && !g_dVL.ensure(nm)) ||
+        (back && right && !g_dZR.ensure(nn)) || (back && left && !g_dZL.ensure(nn))) {
         set_error("device allocation failed");
         fail(info, "staging");
         return;
@@ -354,14 +454,20 @@ void complex_host(char side, char howmny, const int *select, int n, const std::c
     double *dS = (double *)g_dS.p, *dP = (double *)g_dP.p, *dVL = (double *)g_dVL.p, *dVR = (double *)g_dVR.p;
     const T *hS = reinterpret_cast<const T *>(S), *hP = reinterpret_cast<const T *>(P);
     T *hVL = reinterpret_cast<T *>(VL), *hVR = reinterpret_cast<T *>(VR);
-    if (!upload(dS, hS, n, n, lds, 2, st) || !upload(dP, hP, n, n, ldp, 2, st) ||
-        (back && right && !upload(dVR, hVR, n, n, ldvr, 2, st)) ||
-        (back && left && !upload(dVL, hVL, n, n, ldvl, 2, st))) {
+    if (!upload(dS, hS, n, n, lds, 2, st) || !upload(dP, hP, n, n, ldp, 2, st)) {
+        fail(info, "staging");
+        return;
+    }
+    ZStage<T> zs;
+    if (back && !zs.start(dev, (double *)g_dZR.p, right ? hVR : nullptr, ldvr, (double *)g_dZL.p, left ? hVL : nullptr,
+                          ldvl, n, 2)) {
         fail(info, "staging");
         return;
     }
     int rc = complex_device(side, howmny, select, n, (const zt *)dS, n, (const zt *)dP, n, al, be,
-                            left ? (zt *)dVL : nullptr, n, right ? (zt *)dVR : nullptr, n, st);
+                            left ? (zt *)dVL : nullptr, n, right ? (zt *)dVR : nullptr, n, st, XTGEVC3_PREC_FP64,
+                            back ? &zs.feed : nullptr);
+    zs.finish();
     if (rc == 0 && !hip_ok(hipDeviceSynchronize(), "kernel execution")) {
         rc = -1;
     }

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <complex>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -25,16 +26,27 @@ bool check_args(char howmny, int n, int num_sel, int mm, int *m, T *work, int lw
 int real_units(char howmny, const int *select, int n, const double *ai, std::vector<Unit> &units);
 int complex_units(char howmny, const int *select, int n, std::vector<Unit> &units);
 
+// Z for the back-transform (howmny = 'B') delivered beside the sweep instead of in VL / VR:
+// zr / zl are device matrices (leading dimension n, the path's element type) that the caller
+// fills on another stream.  ready(left) is called before that side's back-transform is queued;
+// it returns an event recorded after the side's matrix is written, or null (error message set)
+// if that failed.  FP64 and FP32-product forms only (the split form reads Z from the start of
+// the sweep).
+struct ZFeed {
+    const void *zr, *zl;
+    std::function<hipEvent_t(bool left)> ready;
+};
+
 // prec (XTGEVC3_PREC_*): FP64 throughout; FP32-product or split-int8 far updates and
 // back-transform (mixed forms)
 int real_device(char side, char howmny, const int *select, int n, const double *S, long lds, const double *P,
                 long ldp, const double *ar, const double *ai, const double *be, double *VL, long ldvl, double *VR,
-                long ldvr, hipStream_t st, int prec = 0);
+                long ldvr, hipStream_t st, int prec = 0, const ZFeed *zf = nullptr);
 
 int complex_device(char side, char howmny, const int *select, int n, const std::complex<double> *S, long lds,
                    const std::complex<double> *P, long ldp, const std::complex<double> *al,
                    const std::complex<double> *be, std::complex<double> *VL, long ldvl, std::complex<double> *VR,
-                   long ldvr, hipStream_t st, int prec = 0);
+                   long ldvr, hipStream_t st, int prec = 0, const ZFeed *zf = nullptr);
 
 int real_backward_device(int n, const double *S, long lds, const double *P, long ldp, const double *ar,
                          const double *ai, const double *be, const int *select, const double *V, long ldv,

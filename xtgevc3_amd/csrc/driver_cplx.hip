@@ -185,7 +185,8 @@ static hipError_t zfar_stream_init(ZWork &ws)
 
 int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 *Pw, long ldpw, const zt *al,
                   const zt *be, double ascale, double bscale, bool left, bool back, const double2 *Z, long ldz,
-                  double2 *V, long ldv, const std::vector<Unit> &units, int m, hipStream_t st, int prec)
+                  double2 *V, long ldv, const std::vector<Unit> &units, int m, hipStream_t st, int prec,
+                  const ZFeed *zf)
 {
     const bool split = prec == XTGEVC3_PREC_SPLIT_I8;
     std::vector<ZColDesc> desc(m);
@@ -470,6 +471,13 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
                            (const int *)ws.blkexp.as<int>(), ldx, (const int *)ws.colexp.as<int>(),
                            (const int *)ws.sstart.as<int>(), nsteps, m, ws.xexp.as<int>()); XTG_LAUNCH_CHECK("k_xexp_final");
     }
+    if (back && zf != nullptr) {
+        const hipEvent_t zready = zf->ready(left);
+        if (zready == nullptr) {
+            return -1000;
+        }
+        XTG_CHECK(hipStreamWaitEvent(st, zready, 0));
+    }
     if (back && split) {
         // digits of X's columns as (re, -im) and (im, re), of Z's rows as (re, im); V = Re + i Im
         const double2 *Rp = left ? Z + (long)(n - 1) * ldz : Z;
@@ -544,8 +552,12 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
 int complex_device(char side, char howmny, const int *select, int n, const std::complex<double> *S, long lds,
                    const std::complex<double> *P, long ldp, const std::complex<double> *al,
                    const std::complex<double> *be, std::complex<double> *VL, long ldvl, std::complex<double> *VR,
-                   long ldvr, hipStream_t st, int prec)
+                   long ldvr, hipStream_t st, int prec, const ZFeed *zf)
 {
+    if (zf != nullptr && prec == XTGEVC3_PREC_SPLIT_I8) {
+        set_error("Z delivered beside the sweep is not supported in the split form");
+        return -1000;
+    }
     int dev = 0;
     XTG_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_zmutex);
@@ -578,7 +590,10 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
     if (right && VR) {
         const double2 *Zp = nullptr;
         long zld = n;
-        if (back && prec == XTGEVC3_PREC_SPLIT_I8) {
+        if (back && zf != nullptr) {
+            Zp = static_cast<const double2 *>(zf->zr);
+        }
+        else if (back && prec == XTGEVC3_PREC_SPLIT_I8) {
             // the split form reads Z only through its digits, made before the output is written
             Zp = reinterpret_cast<const double2 *>(VR);
             zld = ldvr;
@@ -590,7 +605,7 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
             Zp = ws.Zt.as<double2>();
         }
         int rc = run_cplx_side(ws, n, dS, lds, dP, ldp, al, be, ascale, bscale, false, back, Zp, zld,
-                               reinterpret_cast<double2 *>(VR), ldvr, units, m, st, prec);
+                               reinterpret_cast<double2 *>(VR), ldvr, units, m, st, prec, zf);
         if (rc) {
             return rc;
         }
@@ -603,7 +618,10 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
         hipLaunchKernelGGL(kz_reverse_conjtranspose, grid, dim3(32, 8), 0, st, dP, ldp, n, ws.Pw.as<double2>(), (long)n); XTG_LAUNCH_CHECK("kz_reverse_conjtranspose");
         const double2 *Zp = nullptr;
         long zld = n;
-        if (back && prec == XTGEVC3_PREC_SPLIT_I8) {
+        if (back && zf != nullptr) {
+            Zp = static_cast<const double2 *>(zf->zl);
+        }
+        else if (back && prec == XTGEVC3_PREC_SPLIT_I8) {
             // the split form reads Z only through its digits, made before the output is written
             Zp = reinterpret_cast<const double2 *>(VL);
             zld = ldvl;
@@ -615,7 +633,7 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
             Zp = ws.Zt.as<double2>();
         }
         int rc = run_cplx_side(ws, n, ws.Sw.as<double2>(), n, ws.Pw.as<double2>(), n, al, be, ascale, bscale, true,
-                               back, Zp, zld, reinterpret_cast<double2 *>(VL), ldvl, units, m, st, prec);
+                               back, Zp, zld, reinterpret_cast<double2 *>(VL), ldvl, units, m, st, prec, zf);
         if (rc) {
             return rc;
         }

@@ -402,6 +402,7 @@ struct RealSideArgs {
     long ldv;
     const std::vector<double> *subw; // working subdiagonal, host
     int prec;                        // 0: FP64, 1: FP32-product updates and back-transform
+    const ZFeed *zf;                 // Z arriving on another stream (null: Z is ready)
 };
 
 // ---------------------------------------------------------------------------------------
@@ -882,6 +883,13 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
     double *X = ws.X.as<double>();
 
     // --- output: back-transform or copy, then normalise ---------------------------------
+    if (A.back && A.zf != nullptr) {
+        const hipEvent_t zready = A.zf->ready(A.left);
+        if (zready == nullptr) {
+            return -1000;
+        }
+        XTG_CHECK(hipStreamWaitEvent(st, zready, 0));
+    }
     if (A.back) {
         // V[:, outcol(p)] = Zw * X[:, p]; for left eigenvectors Zw = Q J (reversed columns)
         const double *Rp = A.left ? A.Z + (long)(n - 1) * A.ldz : A.Z;
@@ -948,8 +956,12 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
 // ---------------------------------------------------------------------------------------
 int real_device(char side, char howmny, const int *select, int n, const double *S, long lds, const double *P,
                 long ldp, const double *ar, const double *ai, const double *be, double *VL, long ldvl, double *VR,
-                long ldvr, hipStream_t st, int prec)
+                long ldvr, hipStream_t st, int prec, const ZFeed *zf)
 {
+    if (zf != nullptr && prec == XTGEVC3_PREC_SPLIT_I8) {
+        set_error("Z delivered beside the sweep is not supported in the split form");
+        return -1000;
+    }
     int dev = 0;
     XTG_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_ws_mutex);
@@ -1003,7 +1015,12 @@ int real_device(char side, char howmny, const int *select, int n, const double *
         A.prec = prec;
         A.V = VR;
         A.ldv = ldvr;
-        if (back && prec == XTGEVC3_PREC_SPLIT_I8) {
+        A.zf = zf;
+        if (back && zf != nullptr) {
+            A.Z = static_cast<const double *>(zf->zr);
+            A.ldz = n;
+        }
+        else if (back && prec == XTGEVC3_PREC_SPLIT_I8) {
             // the split form reads Z only through its digits, made before the output is written
             A.Z = VR;
             A.ldz = ldvr;
@@ -1049,7 +1066,12 @@ int real_device(char side, char howmny, const int *select, int n, const double *
         A.prec = prec;
         A.V = VL;
         A.ldv = ldvl;
-        if (back && prec == XTGEVC3_PREC_SPLIT_I8) {
+        A.zf = zf;
+        if (back && zf != nullptr) {
+            A.Z = static_cast<const double *>(zf->zl);
+            A.ldz = n;
+        }
+        else if (back && prec == XTGEVC3_PREC_SPLIT_I8) {
             A.Z = VL;
             A.ldz = ldvl;
         }
