@@ -123,7 +123,9 @@ Ring g_ring;
 struct CopyLane {
     hipStream_t s = nullptr;
     hipEvent_t done[2] = {nullptr, nullptr};
+    hipEvent_t piece[2 * 8] = {}; // end of each back-transform piece (right, left)
 };
+constexpr int OUT_PIECES = 8; // back-transform pieces per side whose copy-out overlaps the rest
 CopyLane g_lane[64];
 
 int fail(int *info, const char *what)
@@ -288,7 +290,10 @@ template <class T> bool download(T *host, int ld, const double *dev, int rows, i
 // first, then the left, each followed by an event.  feed.ready(side) blocks until the worker
 // has queued that side's copies and returns its event, so the back-transform waits on the
 // device only for its own matrix.  finish() joins the worker; every path calls it before the
-// host arrays are released to the caller.
+// host arrays are released to the caller.  With pieces_on (real path) the back-transform runs
+// in OUT_PIECES pieces per side and feed.done records an event after each; copy_out() then
+// brings the pieces back in device order on the copy stream, each behind its event, so the
+// PCIe copies of finished columns overlap the device work that is still queued.
 constexpr int ZSTAGE_THREADS = 4; // packing threads beside the sweep's launching thread
 template <class T> struct ZStage {
     std::thread th;
@@ -297,8 +302,16 @@ template <class T> struct ZStage {
     std::string err;
     CopyLane *lane = nullptr;
     ZFeed feed{};
+    struct Piece {
+        bool left;
+        int c0, c1;
+        hipEvent_t ev;
+    };
+    std::vector<Piece> pieces; // back-transform pieces queued so far, in device order
+    int npiece[2] = {0, 0};
 
-    bool start(int dev, double *dzr, const T *zr, int ldzr, double *dzl, const T *zl, int ldzl, int n, int w)
+    bool start(int dev, double *dzr, const T *zr, int ldzr, double *dzl, const T *zl, int ldzl, int n, int w,
+               bool pieces_on)
     {
         CopyLane &l = g_lane[dev & 63];
         if (l.s == nullptr) {
@@ -306,6 +319,11 @@ template <class T> struct ZStage {
                 return false;
             }
             for (hipEvent_t &e : l.done) {
+                if (!hip_ok(hipEventCreateWithFlags(&e, hipEventDisableTiming), "copy event")) {
+                    return false;
+                }
+            }
+            for (hipEvent_t &e : l.piece) {
                 if (!hip_ok(hipEventCreateWithFlags(&e, hipEventDisableTiming), "copy event")) {
                     return false;
                 }
@@ -323,6 +341,20 @@ template <class T> struct ZStage {
                 return nullptr;
             }
             return lane->done[left ? 1 : 0];
+        };
+        feed.chunks = pieces_on ? OUT_PIECES : 1;
+        feed.done = [this](bool left, int c0, int c1, hipStream_t st) -> int {
+            int &np = npiece[left ? 1 : 0];
+            if (np >= OUT_PIECES) {
+                set_error("more back-transform pieces than events");
+                return -1000;
+            }
+            const hipEvent_t ev = lane->piece[(left ? OUT_PIECES : 0) + np++];
+            if (!hip_ok(hipEventRecord(ev, st), "piece event")) {
+                return -1000;
+            }
+            pieces.push_back(Piece{left, c0, c1, ev});
+            return 0;
         };
         auto body = [this, dev, zr, zl, dzr, dzl, ldzr, ldzl, n, w]() {
             bool ok = hip_ok(hipSetDevice(dev), "copy thread device");
@@ -350,6 +382,32 @@ template <class T> struct ZStage {
         if (th.joinable()) {
             th.join();
         }
+    }
+    // Copies the queued pieces of side `left` (device matrix dv, n rows, leading dimension n)
+    // into the host matrix hv; false if none were queued for that side (the caller copies the
+    // whole matrix) or on error (ok cleared).
+    bool copy_out(bool left, T *hv, int ldv, const double *dv, int n, int w, int cols, bool &ok)
+    {
+        int covered = 0;
+        for (const Piece &pc : pieces) {
+            if (pc.left == left) {
+                covered += pc.c1 - pc.c0;
+            }
+        }
+        if (covered != cols) {
+            return false;
+        }
+        for (const Piece &pc : pieces) {
+            if (pc.left != left) {
+                continue;
+            }
+            if (!hip_ok(hipStreamWaitEvent(lane->s, pc.ev, 0), "piece wait") ||
+                !download(hv + (size_t)pc.c0 * ldv * w, ldv, dv + (size_t)pc.c0 * n * w, n, pc.c1 - pc.c0, w, lane->s)) {
+                ok = false;
+                return true;
+            }
+        }
+        return true;
     }
     ~ZStage() { finish(); }
 };
@@ -394,22 +452,35 @@ void real_host(char side, char howmny, const int *select, int n, const T *S, int
         return;
     }
     ZStage<T> zs;
+    // pieces for FP64 host arrays only: the FP32 copy-out narrows on the device, and those small
+    // kernels wait for CUs behind the back-transform pieces still running (slower overall)
     if (back && !zs.start(dev, (double *)g_dZR.p, right ? VR : nullptr, ldvr, (double *)g_dZL.p, left ? VL : nullptr,
-                          ldvl, n, 1)) {
+                          ldvl, n, 1, std::is_same<T, double>::value)) {
         fail(info, "staging");
         return;
     }
     int rc = real_device(side, howmny, select, n, dS, n, dP, n, ar, ai, be, left ? dVL : nullptr, n,
                          right ? dVR : nullptr, n, st, XTGEVC3_PREC_FP64, back ? &zs.feed : nullptr);
     zs.finish();
-    if (rc == 0 && !hip_ok(hipDeviceSynchronize(), "kernel execution")) {
-        rc = -1;
-    }
     if (rc != 0) {
+        (void)hipDeviceSynchronize();
         fail(info, "device path");
         return;
     }
-    if ((right && !download(VR, ldvr, dVR, n, num_sel, 1, st)) || (left && !download(VL, ldvl, dVL, n, num_sel, 1, st))) {
+    // back-transform pieces first (their copies overlap the queued device work), then the rest
+    bool ok = true, got_r = false, got_l = false;
+    if (back && right) {
+        got_r = zs.copy_out(false, VR, ldvr, dVR, n, 1, num_sel, ok);
+    }
+    if (ok && back && left) {
+        got_l = zs.copy_out(true, VL, ldvl, dVL, n, 1, num_sel, ok);
+    }
+    if (!ok || !hip_ok(hipDeviceSynchronize(), "kernel execution")) {
+        fail(info, "device path");
+        return;
+    }
+    if ((right && !got_r && !download(VR, ldvr, dVR, n, num_sel, 1, st)) ||
+        (left && !got_l && !download(VL, ldvl, dVL, n, num_sel, 1, st))) {
         fail(info, "staging");
         return;
     }
@@ -460,7 +531,7 @@ void complex_host(char side, char howmny, const int *select, int n, const std::c
     }
     ZStage<T> zs;
     if (back && !zs.start(dev, (double *)g_dZR.p, right ? hVR : nullptr, ldvr, (double *)g_dZL.p, left ? hVL : nullptr,
-                          ldvl, n, 2)) {
+                          ldvl, n, 2, false)) {
         fail(info, "staging");
         return;
     }

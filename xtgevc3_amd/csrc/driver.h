@@ -32,9 +32,15 @@ int complex_units(char howmny, const int *select, int n, std::vector<Unit> &unit
 // it returns an event recorded after the side's matrix is written, or null (error message set)
 // if that failed.  FP64 and FP32-product forms only (the split form reads Z from the start of
 // the sweep).
+// Optionally (real path, chunks > 1) the back-transform runs in up to `chunks` pieces of
+// consecutive output columns; after each piece's vectors are final on st, done(left, c0, c1, st)
+// is called (output columns [c0, c1)) so that the caller can queue their copy-out behind an
+// event of its own.  A nonzero return aborts the call with that code.
 struct ZFeed {
     const void *zr, *zl;
     std::function<hipEvent_t(bool left)> ready;
+    int chunks;
+    std::function<int(bool left, int c0, int c1, hipStream_t st)> done;
 };
 
 // prec (XTGEVC3_PREC_*): FP64 throughout; FP32-product or split-int8 far updates and

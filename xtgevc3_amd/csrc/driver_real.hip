@@ -929,6 +929,44 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
             prof_end(pr, st, XTGEVC3_PROF_BACK_SPLIT, 2.0 * n * ksum,
                      8.0 * (double)n * m + SPLIT_DIGITS * ((double)n * kmx + ksum));
         }
+        else if (A.zf != nullptr && A.zf->done && A.zf->chunks > 1) {
+            // pieces of consecutive working units (about equal column counts), each normalised
+            // and handed to the caller as soon as it is queued; a piece's output columns are
+            // contiguous (the units of a side are in output order, reversed for the left side)
+            const int U = (int)units.size(), C = std::min(A.zf->chunks, U);
+            std::vector<int> ustart(U + 1, 0);
+            for (int u = 0; u < U; ++u) {
+                ustart[u + 1] = ustart[u] + (A.left ? units[U - 1 - u] : units[u]).w;
+            }
+            for (int i = 0; i < C; ++i) {
+                const int u0 = (int)((long)U * i / C), u1 = (int)((long)U * (i + 1) / C);
+                const int p0 = ustart[u0], p1 = ustart[u1];
+                int c0 = n, c1 = 0;
+                double ks = 0.0;
+                int kx = 0;
+                for (int q = p0; q < p1; ++q) {
+                    c0 = std::min(c0, outcol[q]);
+                    c1 = std::max(c1, outcol[q] + 1);
+                    ks += kmaxv[q];
+                    kx = std::max(kx, kmaxv[q]);
+                }
+                int pr = prof_begin(st);
+                launch_gemm_f64(1, st, p1 - p0, n, n, 0, (const double *)X + p0, ldx, (const double *)nullptr, 0L, Rp,
+                                ldr, A.V, A.ldv, ws.outcol.as<int>() + p0, 0, (const double *)nullptr,
+                                ws.kmax.as<int>() + p0, (const double *)nullptr, A.prec,
+                                A.prec == 1 ? ws.xfin.as<int>() + p0 : nullptr,
+                                A.prec == 1 ? ws.zexp.as<int>() : nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<1>");
+                prof_end(pr, st, 2, 2.0 * n * ks, 8.0 * ((double)n * kx + ks + (double)n * (p1 - p0)));
+                const int v0 = A.left ? U - u1 : u0; // the pieces' units in vcol / vw order
+                hipLaunchKernelGGL(k_normalize, dim3((unsigned)(u1 - u0)), dim3(256), 0, st, A.V, A.ldv, n,
+                                   ws.vcol.as<int>() + v0, ws.vw.as<int>() + v0); XTG_LAUNCH_CHECK("k_normalize");
+                if (const int drc = A.zf->done(A.left, c0, c1, st)) {
+                    return drc;
+                }
+            }
+            XTG_CHECK(hipGetLastError());
+            return 0;
+        }
         else {
             int pr = prof_begin(st);
             launch_gemm_f64(1, st, m, n, n, 0, (const double *)X, ldx, (const double *)nullptr, 0L, Rp, ldr, A.V,
