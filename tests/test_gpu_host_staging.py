@@ -108,3 +108,11 @@ def test_host_staging_padded(prefix, side, howmny):
         assert np.all(h[N:, :] == SENT)
         assert np.array_equal(h[:N, m:], init[:, m:])
     assert np.all(hS[N:] == SENT) and np.array_equal(hS[:N], S)
+
+
+@pytest.mark.parametrize("side", ["R", "L"])
+@pytest.mark.parametrize("prefix", ["d", "z"])
+def test_host_back_transform_pieces_one_side(prefix, side):
+    """dtgevc3 / ztgevc3 copy their back-transformed vectors out in pieces per side (right
+    pieces first, then left); one side alone takes only that side's pieces and events."""
+    test_host_staging_padded(prefix, side, "B")

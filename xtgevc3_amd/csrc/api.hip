@@ -290,10 +290,10 @@ template <class T> bool download(T *host, int ld, const double *dev, int rows, i
 // first, then the left, each followed by an event.  feed.ready(side) blocks until the worker
 // has queued that side's copies and returns its event, so the back-transform waits on the
 // device only for its own matrix.  finish() joins the worker; every path calls it before the
-// host arrays are released to the caller.  With pieces_on (real path) the back-transform runs
-// in OUT_PIECES pieces per side and feed.done records an event after each; copy_out() then
-// brings the pieces back in device order on the copy stream, each behind its event, so the
-// PCIe copies of finished columns overlap the device work that is still queued.
+// host arrays are released to the caller.  With pieces_on (FP64 / complex128 host arrays) the
+// back-transform runs in OUT_PIECES pieces per side and feed.done records an event after each;
+// copy_out() then brings the pieces back in device order on the copy stream, each behind its
+// event, so the PCIe copies of finished columns overlap the device work that is still queued.
 constexpr int ZSTAGE_THREADS = 4; // packing threads beside the sweep's launching thread
 template <class T> struct ZStage {
     std::thread th;
@@ -530,8 +530,9 @@ void complex_host(char side, char howmny, const int *select, int n, const std::c
         return;
     }
     ZStage<T> zs;
+    // pieces for complex128 host arrays only (complex64 narrows on the device, as in real_host)
     if (back && !zs.start(dev, (double *)g_dZR.p, right ? hVR : nullptr, ldvr, (double *)g_dZL.p, left ? hVL : nullptr,
-                          ldvl, n, 2, false)) {
+                          ldvl, n, 2, std::is_same<T, double>::value)) {
         fail(info, "staging");
         return;
     }
@@ -539,15 +540,24 @@ void complex_host(char side, char howmny, const int *select, int n, const std::c
                             left ? (zt *)dVL : nullptr, n, right ? (zt *)dVR : nullptr, n, st, XTGEVC3_PREC_FP64,
                             back ? &zs.feed : nullptr);
     zs.finish();
-    if (rc == 0 && !hip_ok(hipDeviceSynchronize(), "kernel execution")) {
-        rc = -1;
-    }
     if (rc != 0) {
+        (void)hipDeviceSynchronize();
         fail(info, "device path");
         return;
     }
-    if ((right && !download(hVR, ldvr, dVR, n, num_sel, 2, st)) ||
-        (left && !download(hVL, ldvl, dVL, n, num_sel, 2, st))) {
+    bool ok = true, got_r = false, got_l = false;
+    if (back && right) {
+        got_r = zs.copy_out(false, hVR, ldvr, dVR, n, 2, num_sel, ok);
+    }
+    if (ok && back && left) {
+        got_l = zs.copy_out(true, hVL, ldvl, dVL, n, 2, num_sel, ok);
+    }
+    if (!ok || !hip_ok(hipDeviceSynchronize(), "kernel execution")) {
+        fail(info, "device path");
+        return;
+    }
+    if ((right && !got_r && !download(hVR, ldvr, dVR, n, num_sel, 2, st)) ||
+        (left && !got_l && !download(hVL, ldvl, dVL, n, num_sel, 2, st))) {
         fail(info, "staging");
         return;
     }

@@ -32,7 +32,7 @@ int complex_units(char howmny, const int *select, int n, std::vector<Unit> &unit
 // it returns an event recorded after the side's matrix is written, or null (error message set)
 // if that failed.  FP64 and FP32-product forms only (the split form reads Z from the start of
 // the sweep).
-// Optionally (real path, chunks > 1) the back-transform runs in up to `chunks` pieces of
+// Optionally (FP64 form, chunks > 1) the back-transform runs in up to `chunks` pieces of
 // consecutive output columns; after each piece's vectors are final on st, done(left, c0, c1, st)
 // is called (output columns [c0, c1)) so that the caller can queue their copy-out behind an
 // event of its own.  A nonzero return aborts the call with that code.

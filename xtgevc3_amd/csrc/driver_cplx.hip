@@ -521,6 +521,39 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
         prof_end(pr, st, XTGEVC3_PROF_BACK_SPLIT, 8.0 * n * ksum,
                  16.0 * (double)n * m + 2.0 * SPLIT_DIGITS * (2.0 * (double)n * kmx + 2.0 * ksum));
     }
+    else if (back && zf != nullptr && zf->done && zf->chunks > 1) {
+        // pieces of consecutive working columns, each normalised and handed to the caller as
+        // soon as it is queued; a piece's output columns are contiguous (output order, reversed
+        // for the left side)
+        const double2 *Rp = left ? Z + (long)(n - 1) * ldz : Z;
+        const long ldr = left ? -ldz : ldz;
+        const int C = std::min(zf->chunks, m);
+        for (int i = 0; i < C; ++i) {
+            const int p0 = (int)((long)m * i / C), p1 = (int)((long)m * (i + 1) / C);
+            int c0 = n, c1 = 0;
+            double ks = 0.0;
+            int kx = 0;
+            for (int q = p0; q < p1; ++q) {
+                c0 = std::min(c0, outcol[q]);
+                c1 = std::max(c1, outcol[q] + 1);
+                ks += kmaxv[q];
+                kx = std::max(kx, kmaxv[q]);
+            }
+            const dim3 grid(((n + ZTB_H - 1) / ZTB_H) * ((p1 - p0 + ZTA_H - 1) / ZTA_H));
+            int pr = prof_begin(st);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(kz_gemm<1, ZTA_H>), grid, dim3(256), 0, st, p1 - p0, n, n, 0,
+                               (const double2 *)X + p0, ldx, (const double2 *)nullptr, 0L, Rp, ldr, V, ldv,
+                               ws.outcol.as<int>() + p0, 0, (const double *)nullptr, ws.kmax.as<int>() + p0,
+                               (const double2 *)nullptr); XTG_LAUNCH_CHECK("kz_gemm<1>");
+            prof_end(pr, st, 2, 8.0 * n * ks, 16.0 * ((double)n * kx + ks + (double)n * (p1 - p0)));
+            hipLaunchKernelGGL(kz_normalize, dim3(c1 - c0), dim3(256), 0, st, V + (long)c0 * ldv, ldv, n); XTG_LAUNCH_CHECK("kz_normalize");
+            if (const int drc = zf->done(left, c0, c1, st)) {
+                return drc;
+            }
+        }
+        XTG_CHECK(hipGetLastError());
+        return 0;
+    }
     else if (back) {
         const double2 *Rp = left ? Z + (long)(n - 1) * ldz : Z;
         long ldr = left ? -ldz : ldz;
