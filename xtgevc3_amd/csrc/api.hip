@@ -384,8 +384,9 @@ template <class T> struct ZStage {
         }
     }
     // Copies the queued pieces of side `left` (device matrix dv, n rows, leading dimension n)
-    // into the host matrix hv; false if none were queued for that side (the caller copies the
-    // whole matrix) or on error (ok cleared).
+    // into the host matrix hv.  Returns false, copying nothing, unless the pieces cover all
+    // cols columns (the caller then copies the whole matrix); on a copy error it returns true
+    // with ok cleared.
     bool copy_out(bool left, T *hv, int ldv, const double *dv, int n, int w, int cols, bool &ok)
     {
         int covered = 0;
