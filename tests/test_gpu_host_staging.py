@@ -111,8 +111,9 @@ def test_host_staging_padded(prefix, side, howmny):
 
 
 @pytest.mark.parametrize("side", ["R", "L"])
-@pytest.mark.parametrize("prefix", ["d", "z"])
+@pytest.mark.parametrize("prefix", ["d", "s", "z", "c"])
 def test_host_back_transform_pieces_one_side(prefix, side):
-    """dtgevc3 / ztgevc3 copy their back-transformed vectors out in pieces per side (right
-    pieces first, then left); one side alone takes only that side's pieces and events."""
+    """The host entries copy their back-transformed vectors out in pieces per side (right
+    pieces first, then left; rounded to FP32 on the device for s / c); one side alone takes
+    only that side's pieces and events."""
     test_host_staging_padded(prefix, side, "B")

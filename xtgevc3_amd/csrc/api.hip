@@ -115,7 +115,7 @@ struct Ring {
 };
 
 std::mutex g_stage_mutex;
-DevBuf g_dS, g_dP, g_dVL, g_dVR, g_dZR, g_dZL;
+DevBuf g_dS, g_dP, g_dVL, g_dVR, g_dZR, g_dZL, g_dVL32, g_dVR32;
 Ring g_ring;
 
 // Per-device copy stream (non-blocking, default priority) and the events that mark the end of
@@ -232,10 +232,10 @@ bool upload(double *dev, const T *host, int rows, int cols, int ld, int w, hipSt
     return true;
 }
 
-// Moves the first cols columns of a packed device matrix (w doubles per element) into a host
-// matrix with leading dimension ld and scalars of type T, rounding to FP32 on the device for
-// T = float; returns when the host copy is complete.
-template <class T> bool download(T *host, int ld, const double *dev, int rows, int cols, int w, hipStream_t st)
+// Moves the first cols columns of a packed device matrix (w scalars of type D per element)
+// into a host matrix with leading dimension ld and scalars of type T, rounding to FP32 on the
+// device for T = float, D = double; returns when the host copy is complete.
+template <class T, class D> bool download(T *host, int ld, const D *dev, int rows, int cols, int w, hipStream_t st)
 {
     if (rows <= 0 || cols <= 0) {
         return true;
@@ -252,12 +252,13 @@ template <class T> bool download(T *host, int ld, const double *dev, int rows, i
         if (!hip_ok(g_ring.wait(b), "staging wait")) {
             return false;
         }
-        const double *d = dev + (size_t)c0 * rows * w;
-        if (std::is_same<T, double>::value) {
+        const D *d = dev + (size_t)c0 * rows * w;
+        if constexpr (std::is_same<T, D>::value) {
             if (!hip_ok(hipMemcpyAsync(g_ring.host[b], d, colb * nc, hipMemcpyDeviceToHost, st), "device to host copy")) {
                 return false;
             }
         } else {
+            static_assert(std::is_same<T, float>::value && std::is_same<D, double>::value, "narrowing copy");
             const long len = (long)nc * rows * w;
             hipLaunchKernelGGL(k_narrow, dim3(flat_grid(len)), dim3(256), 0, st, d, (float *)g_ring.dev[b], len);
             if (!hip_ok(hipGetLastError(), "k_narrow launch") ||
@@ -290,8 +291,9 @@ template <class T> bool download(T *host, int ld, const double *dev, int rows, i
 // first, then the left, each followed by an event.  feed.ready(side) blocks until the worker
 // has queued that side's copies and returns its event, so the back-transform waits on the
 // device only for its own matrix.  finish() joins the worker; every path calls it before the
-// host arrays are released to the caller.  With pieces_on (FP64 / complex128 host arrays) the
-// back-transform runs in OUT_PIECES pieces per side and feed.done records an event after each;
+// host arrays are released to the caller.  With pieces_on the back-transform runs in
+// OUT_PIECES pieces per side (rounded to FP32 on the device when feed.out32 is set) and
+// feed.done records an event after each;
 // copy_out() then brings the pieces back in device order on the copy stream, each behind its
 // event, so the PCIe copies of finished columns overlap the device work that is still queued.
 constexpr int ZSTAGE_THREADS = 4; // packing threads beside the sweep's launching thread
@@ -383,11 +385,11 @@ template <class T> struct ZStage {
             th.join();
         }
     }
-    // Copies the queued pieces of side `left` (device matrix dv, n rows, leading dimension n)
-    // into the host matrix hv.  Returns false, copying nothing, unless the pieces cover all
-    // cols columns (the caller then copies the whole matrix); on a copy error it returns true
-    // with ok cleared.
-    bool copy_out(bool left, T *hv, int ldv, const double *dv, int n, int w, int cols, bool &ok)
+    // Copies the queued pieces of side `left` (device matrix dv of FP64 or, rounded by the
+    // device path, of T; n rows, leading dimension n) into the host matrix hv.  Returns false,
+    // copying nothing, unless the pieces cover all cols columns (the caller then copies the
+    // whole matrix); on a copy error it returns true with ok cleared.
+    template <class D> bool copy_out(bool left, T *hv, int ldv, const D *dv, int n, int w, int cols, bool &ok)
     {
         int covered = 0;
         for (const Piece &pc : pieces) {
@@ -453,12 +455,24 @@ void real_host(char side, char howmny, const int *select, int n, const T *S, int
         return;
     }
     ZStage<T> zs;
-    // pieces for FP64 host arrays only: the FP32 copy-out narrows on the device, and those small
-    // kernels wait for CUs behind the back-transform pieces still running (slower overall)
-    if (back && !zs.start(dev, (double *)g_dZR.p, right ? VR : nullptr, ldvr, (double *)g_dZL.p, left ? VL : nullptr,
-                          ldvl, n, 1, std::is_same<T, double>::value)) {
+    // FP32 host arrays: the device path rounds each piece into an FP32 matrix on its own stream
+    // (narrowing kernels on the copy stream would wait for CUs behind the running pieces), and
+    // the copy stream moves FP32 bytes only
+    constexpr bool narrow = !std::is_same<T, double>::value;
+    const size_t n32 = (size_t)n * num_sel * sizeof(float);
+    if (narrow && back && ((right && !g_dVR32.ensure(n32)) || (left && !g_dVL32.ensure(n32)))) {
+        set_error("device allocation failed");
         fail(info, "staging");
         return;
+    }
+    if (back && !zs.start(dev, (double *)g_dZR.p, right ? VR : nullptr, ldvr, (double *)g_dZL.p, left ? VL : nullptr,
+                          ldvl, n, 1, true)) {
+        fail(info, "staging");
+        return;
+    }
+    if (narrow) {
+        zs.feed.out32[0] = right ? g_dVR32.p : nullptr;
+        zs.feed.out32[1] = left ? g_dVL32.p : nullptr;
     }
     int rc = real_device(side, howmny, select, n, dS, n, dP, n, ar, ai, be, left ? dVL : nullptr, n,
                          right ? dVR : nullptr, n, st, XTGEVC3_PREC_FP64, back ? &zs.feed : nullptr);
@@ -471,10 +485,12 @@ void real_host(char side, char howmny, const int *select, int n, const T *S, int
     // back-transform pieces first (their copies overlap the queued device work), then the rest
     bool ok = true, got_r = false, got_l = false;
     if (back && right) {
-        got_r = zs.copy_out(false, VR, ldvr, dVR, n, 1, num_sel, ok);
+        got_r = narrow ? zs.copy_out(false, VR, ldvr, (const T *)g_dVR32.p, n, 1, num_sel, ok)
+                       : zs.copy_out(false, VR, ldvr, dVR, n, 1, num_sel, ok);
     }
     if (ok && back && left) {
-        got_l = zs.copy_out(true, VL, ldvl, dVL, n, 1, num_sel, ok);
+        got_l = narrow ? zs.copy_out(true, VL, ldvl, (const T *)g_dVL32.p, n, 1, num_sel, ok)
+                       : zs.copy_out(true, VL, ldvl, dVL, n, 1, num_sel, ok);
     }
     if (!ok || !hip_ok(hipDeviceSynchronize(), "kernel execution")) {
         fail(info, "device path");
@@ -531,11 +547,22 @@ void complex_host(char side, char howmny, const int *select, int n, const std::c
         return;
     }
     ZStage<T> zs;
-    // pieces for complex128 host arrays only (complex64 narrows on the device, as in real_host)
-    if (back && !zs.start(dev, (double *)g_dZR.p, right ? hVR : nullptr, ldvr, (double *)g_dZL.p, left ? hVL : nullptr,
-                          ldvl, n, 2, std::is_same<T, double>::value)) {
+    // complex64 host arrays: pieces rounded on the device path's stream, as in real_host
+    constexpr bool narrow = !std::is_same<T, double>::value;
+    const size_t n32 = (size_t)n * num_sel * 2 * sizeof(float);
+    if (narrow && back && ((right && !g_dVR32.ensure(n32)) || (left && !g_dVL32.ensure(n32)))) {
+        set_error("device allocation failed");
         fail(info, "staging");
         return;
+    }
+    if (back && !zs.start(dev, (double *)g_dZR.p, right ? hVR : nullptr, ldvr, (double *)g_dZL.p, left ? hVL : nullptr,
+                          ldvl, n, 2, true)) {
+        fail(info, "staging");
+        return;
+    }
+    if (narrow) {
+        zs.feed.out32[0] = right ? g_dVR32.p : nullptr;
+        zs.feed.out32[1] = left ? g_dVL32.p : nullptr;
     }
     int rc = complex_device(side, howmny, select, n, (const zt *)dS, n, (const zt *)dP, n, al, be,
                             left ? (zt *)dVL : nullptr, n, right ? (zt *)dVR : nullptr, n, st, XTGEVC3_PREC_FP64,
@@ -548,10 +575,12 @@ void complex_host(char side, char howmny, const int *select, int n, const std::c
     }
     bool ok = true, got_r = false, got_l = false;
     if (back && right) {
-        got_r = zs.copy_out(false, hVR, ldvr, dVR, n, 2, num_sel, ok);
+        got_r = narrow ? zs.copy_out(false, hVR, ldvr, (const T *)g_dVR32.p, n, 2, num_sel, ok)
+                       : zs.copy_out(false, hVR, ldvr, dVR, n, 2, num_sel, ok);
     }
     if (ok && back && left) {
-        got_l = zs.copy_out(true, hVL, ldvl, dVL, n, 2, num_sel, ok);
+        got_l = narrow ? zs.copy_out(true, hVL, ldvl, (const T *)g_dVL32.p, n, 2, num_sel, ok)
+                       : zs.copy_out(true, hVL, ldvl, dVL, n, 2, num_sel, ok);
     }
     if (!ok || !hip_ok(hipDeviceSynchronize(), "kernel execution")) {
         fail(info, "device path");

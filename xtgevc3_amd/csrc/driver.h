@@ -35,12 +35,15 @@ int complex_units(char howmny, const int *select, int n, std::vector<Unit> &unit
 // Optionally (FP64 form, chunks > 1) the back-transform runs in up to `chunks` pieces of
 // consecutive output columns; after each piece's vectors are final on st, done(left, c0, c1, st)
 // is called (output columns [c0, c1)) so that the caller can queue their copy-out behind an
-// event of its own.  A nonzero return aborts the call with that code.
+// event of its own.  A nonzero return aborts the call with that code.  out32[side] (right,
+// left; optional): before done(), the piece's vectors are also rounded to FP32 into this
+// device matrix (same columns, leading dimension n, 1 or 2 floats per element).
 struct ZFeed {
     const void *zr, *zl;
     std::function<hipEvent_t(bool left)> ready;
     int chunks;
     std::function<int(bool left, int c0, int c1, hipStream_t st)> done;
+    void *out32[2];
 };
 
 // prec (XTGEVC3_PREC_*): FP64 throughout; FP32-product or split-int8 far updates and

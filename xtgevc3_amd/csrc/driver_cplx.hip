@@ -22,6 +22,7 @@
 
 namespace xtg {
 
+__global__ void k_round_f32(const double *, long, long, float *, long);
 template <int MAXB>
 __global__ void kz_solve_units(const double2 *, long, const double2 *, long, int, int, double2 *, long, double2 *, long,
                                const ZColDesc *, int, int, double *, int *, int *, int *, int, int, double *, int,
@@ -547,6 +548,11 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
                                (const double2 *)nullptr); XTG_LAUNCH_CHECK("kz_gemm<1>");
             prof_end(pr, st, 2, 8.0 * n * ks, 16.0 * ((double)n * kx + ks + (double)n * (p1 - p0)));
             hipLaunchKernelGGL(kz_normalize, dim3(c1 - c0), dim3(256), 0, st, V + (long)c0 * ldv, ldv, n); XTG_LAUNCH_CHECK("kz_normalize");
+            if (float *o = static_cast<float *>(zf->out32[left ? 1 : 0])) {
+                // (re, im) pairs as 2n scalars per column
+                hipLaunchKernelGGL(k_round_f32, dim3(c1 - c0), dim3(256), 0, st, (const double *)(V + (long)c0 * ldv),
+                                   2 * ldv, 2L * n, o + 2L * c0 * n, 2L * n); XTG_LAUNCH_CHECK("k_round_f32");
+            }
             if (const int drc = zf->done(left, c0, c1, st)) {
                 return drc;
             }

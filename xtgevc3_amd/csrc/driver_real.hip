@@ -56,6 +56,7 @@ constexpr int GROUP_STEPS = 4; // row blocks per far update (K = 2 * ~256)
 __global__ void k_fixup_scale(double *, long, int, int, const int *, const int *, long, const int *, const int *);
 __global__ void k_copy_out(const double *, long, int, int, double *, long, const int *, int);
 __global__ void k_normalize(double *, long, int, const int *, const int *);
+__global__ void k_round_f32(const double *, long, long, float *, long);
 __global__ void k_reverse_transpose(const double *, long, int, double *, long);
 __global__ void k_colnorms(const double *, long, const double *, long, int, int, double *);
 __global__ void k_bwd_rhs(const double *, long, const double *, long, int, int, int, const int *, double *, long);
@@ -960,6 +961,11 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
                 const int v0 = A.left ? U - u1 : u0; // the pieces' units in vcol / vw order
                 hipLaunchKernelGGL(k_normalize, dim3((unsigned)(u1 - u0)), dim3(256), 0, st, A.V, A.ldv, n,
                                    ws.vcol.as<int>() + v0, ws.vw.as<int>() + v0); XTG_LAUNCH_CHECK("k_normalize");
+                if (float *o = static_cast<float *>(A.zf->out32[A.left ? 1 : 0])) {
+                    hipLaunchKernelGGL(k_round_f32, dim3((unsigned)(c1 - c0)), dim3(256), 0, st,
+                                       (const double *)A.V + (long)c0 * A.ldv, A.ldv, (long)n, o + (long)c0 * n,
+                                       (long)n); XTG_LAUNCH_CHECK("k_round_f32");
+                }
                 if (const int drc = A.zf->done(A.left, c0, c1, st)) {
                     return drc;
                 }

@@ -1154,6 +1154,18 @@ __global__ __launch_bounds__(256) void k_normalize(double *__restrict__ V, long 
     }
 }
 
+// out[r + c*ldo] = (float) V[r + c*ldv] for r < rows, c < cols (one workgroup per column):
+// FP32 copies of finished vector columns for FP32 host arrays.
+__global__ __launch_bounds__(256) void k_round_f32(const double *__restrict__ V, long ldv, long rows,
+                                                   float *__restrict__ out, long ldo)
+{
+    const double *src = V + (long)blockIdx.x * ldv;
+    float *dst = out + (long)blockIdx.x * ldo;
+    for (long r = threadIdx.x; r < rows; r += blockDim.x) {
+        dst[r] = (float)src[r];
+    }
+}
+
 // D[i + j*ldd] = A[(n-1-j) + (n-1-i)*lda]  (D = J A^T J), 32x32 LDS tiles.
 __global__ void k_reverse_transpose(const double *__restrict__ A, long lda, int n, double *__restrict__ D, long ldd)
 {
