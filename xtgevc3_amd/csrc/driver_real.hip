@@ -1228,7 +1228,7 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
     double *X = ws.X.as<double>();
     hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, X, (long)n * ldx); XTG_LAUNCH_CHECK("k_zero");
     int pr = prof_begin(st);
-    hipLaunchKernelGGL(k_bwd_rhs, dim3(M), dim3(256), 0, st, V, ldv, G, ldg, n, m, nbatch, ws.kcol.as<int>(), X, ldx); XTG_LAUNCH_CHECK("k_bwd_rhs");
+    hipLaunchKernelGGL(k_bwd_rhs, dim3((M + 3) / 4), dim3(256), 0, st, V, ldv, G, ldg, n, m, nbatch, ws.kcol.as<int>(), X, ldx); XTG_LAUNCH_CHECK("k_bwd_rhs");
     prof_end(pr, st, 3, 0.0, 8.0 * 3.0 * n * M);
     const bool split = prec == XTGEVC3_PREC_SPLIT_I8;
     if (split) {

@@ -1275,61 +1275,75 @@ template __global__ void k_solve_real<65>(const double *, long, const double *, 
 
 namespace xtg {
 
-// One workgroup per (batch b, forward column c): RHS of the adjoint solve in working rows.
+// RHS of the adjoint solve in working rows: X[(n-1-i)*ldx + q] = g_i - [i == i*] (g.v) sign(v_i*)
+// for i < kcol[c] (zero elsewhere), where working column q = (m-1-c)*nbatch + b holds batch b of
+// forward column c and i* is the first index of max |v_i|.  One workgroup per 4 working
+// columns: each wave reduces one column (lane stride 64, fixed order, so a column's result does
+// not depend on nbatch), then 4 x 512 tiles through LDS write rows of X as 4 contiguous doubles
+// while g is read along i (one column per workgroup wrote X at a full row's stride per value;
+// 8 and 32 columns per workgroup left too few workgroups for one cotangent batch).
 __global__ __launch_bounds__(256) void k_bwd_rhs(const double *__restrict__ V, long ldv, const double *__restrict__ G,
                                                  long ldg, int n, int m, int nbatch, const int *__restrict__ kcol,
                                                  double *__restrict__ X, long ldx)
 {
-    __shared__ double rv[256 / WAVE];
-    __shared__ int ri[256 / WAVE];
-    __shared__ double rd[256 / WAVE];
-    const int c = blockIdx.x % m, b = blockIdx.x / m;
-    const double *v = V + (long)c * ldv;
-    const double *g = G + (long)(b * m + c) * ldg;
-    const int k = kcol[c];
-    const int q = (m - 1 - c) * nbatch + b;
-    double best = -1.0, dot = 0.0;
-    int bi = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double a = fabs(v[i]);
-        if (a > best) {
-            best = a;
-            bi = i;
+    constexpr int TQ = 4, TI = 512;
+    __shared__ double tile[TQ][TI + 1];
+    __shared__ double s_corr[TQ];
+    __shared__ int s_bi[TQ], s_k[TQ];
+    const int M = m * nbatch, q0 = blockIdx.x * TQ;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    for (int j = wave; j < TQ; j += 256 / WAVE) {
+        const int q = q0 + j;
+        if (q >= M) {
+            continue;
         }
-        dot = fma(g[i], v[i], dot);
-    }
-    for (int off = WAVE / 2; off > 0; off >>= 1) {
-        double ob = __shfl_xor(best, off);
-        int oi = __shfl_xor(bi, off);
-        if (ob > best || (ob == best && oi < bi)) {
-            best = ob;
-            bi = oi;
+        const int c = m - 1 - q / nbatch, b = q % nbatch;
+        const double *v = V + (long)c * ldv;
+        const double *g = G + (long)(b * m + c) * ldg;
+        double best = -1.0, dot = 0.0;
+        int bi = 0;
+        for (int i = lane; i < n; i += WAVE) {
+            const double a = fabs(v[i]);
+            if (a > best) {
+                best = a;
+                bi = i;
+            }
+            dot = fma(g[i], v[i], dot);
         }
-        dot += __shfl_xor(dot, off);
-    }
-    if (threadIdx.x % WAVE == 0) {
-        rv[threadIdx.x / WAVE] = best;
-        ri[threadIdx.x / WAVE] = bi;
-        rd[threadIdx.x / WAVE] = dot;
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(best, off);
+            const int oi = __shfl_xor(bi, off);
+            if (ob > best || (ob == best && oi < bi)) {
+                best = ob;
+                bi = oi;
+            }
+            dot += __shfl_xor(dot, off);
+        }
+        if (lane == 0) {
+            s_bi[j] = bi;
+            s_corr[j] = v[bi] < 0.0 ? -dot : dot;
+            s_k[j] = kcol[c];
+        }
     }
     __syncthreads();
-    best = rv[0];
-    bi = ri[0];
-    dot = rd[0];
-    for (int w = 1; w < (int)(blockDim.x / WAVE); ++w) {
-        if (rv[w] > best || (rv[w] == best && ri[w] < bi)) {
-            best = rv[w];
-            bi = ri[w];
+    for (int i0 = 0; i0 < n; i0 += TI) {
+        for (int t = threadIdx.x; t < TQ * TI; t += 256) {
+            const int j = t / TI, ii = t % TI, i = i0 + ii, q = q0 + j;
+            double val = 0.0;
+            if (q < M && i < s_k[j]) {
+                const int c = m - 1 - q / nbatch, b = q % nbatch;
+                val = G[(long)(b * m + c) * ldg + i] - (i == s_bi[j] ? s_corr[j] : 0.0);
+            }
+            tile[j][ii] = val;
         }
-        dot += rd[w];
-    }
-    const double sp = v[bi] < 0.0 ? -1.0 : 1.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double val = 0.0;
-        if (i < k) {
-            val = g[i] - (i == bi ? dot * sp : 0.0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < TQ * TI; t += 256) {
+            const int ii = t / TQ, j = t % TQ, i = i0 + ii, q = q0 + j;
+            if (q < M && i < n) {
+                X[(long)(n - 1 - i) * ldx + q] = tile[j][ii];
+            }
         }
-        X[(long)(n - 1 - i) * ldx + q] = val;
+        __syncthreads();
     }
 }
 
