@@ -341,6 +341,7 @@ struct RealWork {
     // far-update lookahead: the rows above the next group run on `fs`
     hipStream_t fs = nullptr;
     hipEvent_t ev_wready[2] = {nullptr, nullptr}, ev_fdone[2] = {nullptr, nullptr};
+    hipEvent_t ev_zsrc = nullptr, ev_zcopy = nullptr; // copy of Z on `fs`: its start and end
     hipEvent_t ev_last = nullptr; // end of the previous call's work on this workspace
 };
 
@@ -753,8 +754,35 @@ static hipError_t far_stream_init(RealWork &ws)
                 }
             }
         }
+        for (hipEvent_t *ev : {&ws.ev_zsrc, &ws.ev_zcopy}) {
+            e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+            if (e != hipSuccess) {
+                return e;
+            }
+        }
     }
     return hipSuccess;
+}
+
+// Copies Z (n x n, leading dimension ldz) into ws.Zt on ws.fs after the work queued so far on
+// st, so that the copy runs beside the sweep; ws.ev_zcopy marks its end.
+static hipError_t zcopy_async(RealWork &ws, const double *Z, long ldz, int n, hipStream_t st)
+{
+    hipError_t e = far_stream_init(ws);
+    if (e == hipSuccess) {
+        e = hipEventRecord(ws.ev_zsrc, st);
+    }
+    if (e == hipSuccess) {
+        e = hipStreamWaitEvent(ws.fs, ws.ev_zsrc, 0);
+    }
+    if (e == hipSuccess) {
+        e = hipMemcpy2DAsync(ws.Zt.p, sizeof(double) * n, Z, sizeof(double) * ldz, sizeof(double) * n, n,
+                             hipMemcpyDeviceToDevice, ws.fs);
+    }
+    if (e == hipSuccess) {
+        e = hipEventRecord(ws.ev_zcopy, ws.fs);
+    }
+    return e;
 }
 
 // Digits of the rows of A (and B, with the exponent of each row taken over both): E[r] from the
@@ -1040,6 +1068,9 @@ int real_device(char side, char howmny, const int *select, int n, const double *
     const double safemin = std::numeric_limits<double>::min();
     double ascale = 1.0 / std::max(an, safemin);
     double bscale = 1.0 / std::max(bn, safemin);
+    // the back-transform's own copy of Z (the output overwrites VR / VL) arrives on ws.fs
+    ZFeed zown{};
+    zown.ready = [&ws](bool) { return ws.ev_zcopy; };
 
     if (right && VR != nullptr) {
         RealSideArgs A{};
@@ -1072,10 +1103,10 @@ int real_device(char side, char howmny, const int *select, int n, const double *
         else if (back) {
             // the input VR holds Z; keep a copy since the output overwrites it
             ENSURE(ws.Zt, sizeof(double) * (size_t)n * n);
-            XTG_CHECK(hipMemcpy2DAsync(ws.Zt.p, sizeof(double) * n, VR, sizeof(double) * ldvr, sizeof(double) * n, n,
-                                       hipMemcpyDeviceToDevice, st));
+            XTG_CHECK(zcopy_async(ws, VR, ldvr, n, st));
             A.Z = ws.Zt.as<double>();
             A.ldz = n;
+            A.zf = &zown;
         }
         int rc = run_real_side(ws, A, units, m, st);
         if (rc) {
@@ -1120,11 +1151,12 @@ int real_device(char side, char howmny, const int *select, int n, const double *
             A.ldz = ldvl;
         }
         else if (back) {
+            // queued after the right side's back-transform, which reads the same workspace
             ENSURE(ws.Zt, sizeof(double) * (size_t)n * n);
-            XTG_CHECK(hipMemcpy2DAsync(ws.Zt.p, sizeof(double) * n, VL, sizeof(double) * ldvl, sizeof(double) * n, n,
-                                       hipMemcpyDeviceToDevice, st));
+            XTG_CHECK(zcopy_async(ws, VL, ldvl, n, st));
             A.Z = ws.Zt.as<double>();
             A.ldz = n;
+            A.zf = &zown;
         }
         int rc = run_real_side(ws, A, units, m, st);
         if (rc) {
