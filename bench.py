@@ -31,13 +31,13 @@ SPLIT_PRODUCTS = 15  # digit products per multiply-add in the split-int8 form (l
 # each scaled by the counter's reading on a plain 1 GiB device copy (FETCH x2.0, WRITE x1.0);
 # summary and commands in profiles/dtgevc3_R_B_n8192_hbm_traffic.md
 MEASURED_TRAFFIC = {
-    ("xtg::k_gemm_f64<1, true, 128, 128, 16, false>", 8192): 11611.4e6,
-    ("xtg::k_gemm_f64<0, true, 128, 64, 16, false>", 8192): 432.1e6,
-    ("xtg::k_gemm_f64<0, true, 128, 128, 16, false>", 8192): 221.6e6,
-    ("xtg::k_gemm_f64<0, true, 64, 64, 32, false>", 8192): 104.1e6,
+    ("xtg::k_gemm_f64<1, true, 128, 128, 16, false>", 8192): 13349.8e6,
+    ("xtg::k_gemm_f64<0, true, 128, 64, 16, false>", 8192): 405.8e6,
+    ("xtg::k_gemm_f64<0, true, 128, 128, 16, false>", 8192): 214.4e6,
+    ("xtg::k_gemm_f64<0, true, 64, 64, 32, false>", 8192): 59.2e6,
     ("xtg::k_gemm_f64<0, true, 32, 64, 32, false>", 8192): 16.3e6,
-    ("xtg::k_solve_real<65>", 8192): 27.4e6,
-    ("xtg::k_gemm_i8<128, 128, 2, 4> (back-transform)", 8192): 4866.3e6,
+    ("xtg::k_solve_real<65>", 8192): 27.3e6,
+    ("xtg::k_gemm_i8<128, 128, 2, 4> (back-transform)", 8192): 5112.3e6,
 }
 
 
