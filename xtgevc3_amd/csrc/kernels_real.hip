@@ -1233,8 +1233,8 @@ __global__ void k_subdiag(const double *__restrict__ S, long lds, int n, double 
     }
 }
 
-// Fill a row-major block with zeros (used instead of hipMemsetAsync so the sequence can be
-// captured in a HIP graph).
+// Fill a row-major block with zeros (a kernel rather than hipMemsetAsync, so that the sweep's
+// launch sequence stays replayable from a HIP graph).
 __global__ void k_zero(double *__restrict__ X, long count)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
