@@ -181,3 +181,39 @@ def test_calls_on_two_streams_share_the_workspace():
     torch.cuda.synchronize()
     for V, R in zip(outs, refs):
         assert torch.equal(V, R)
+
+
+def test_back_transform_on_a_side_stream_is_ordered():
+    """howmny = 'B' on a non-default stream, side 'B': Z and Q are written on that stream just
+    before the call, and the vectors are read (and the buffers reused) on it right after, with
+    no device synchronisation in between.  The call's own copy of Z and its lookahead products
+    run on a second stream ordered by events, so the vectors equal the default-stream call's
+    bit for bit."""
+    import torch
+
+    import xtgevc3_amd as X
+
+    n = 500
+    S, P, ar, ai, be = G.real_quasi_triangular(n, 81)
+    dS, dP = X.fortran_device(S), X.fortran_device(P)
+    rng = np.random.default_rng(5)
+    Z = torch.tensor(rng.standard_normal((n, n)), device="cuda")
+    Q = torch.tensor(rng.standard_normal((n, n)), device="cuda")
+    refR, refL = Z.clone(), Q.clone()
+    m, info = X.dtgevc3_device("B", "B", None, dS, dP, ar, ai, be, VL=refL, VR=refR, mm=n)
+    assert (m, info) == (n, 0)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        VR = torch.empty_like(Z)
+        VL = torch.empty_like(Q)
+        VR.copy_(Z)
+        VL.copy_(Q)
+        m2, info2 = X.dtgevc3_device("B", "B", None, dS, dP, ar, ai, be, VL=VL, VR=VR, mm=n, stream=s.cuda_stream)
+        outR = VR.clone()
+        outL = VL.clone()
+        VR.fill_(7.0)
+        VL.fill_(7.0)
+    torch.cuda.synchronize()
+    assert (m2, info2) == (n, 0)
+    assert torch.equal(outR, refR) and torch.equal(outL, refL)
