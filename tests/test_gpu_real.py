@@ -183,15 +183,18 @@ def test_calls_on_two_streams_share_the_workspace():
         assert torch.equal(V, R)
 
 
-def test_back_transform_on_a_side_stream_is_ordered():
+@pytest.mark.parametrize("precision", ["fp64", "split_i8", "mixed_f32"])
+def test_back_transform_on_a_side_stream_is_ordered(precision):
     """howmny = 'B' on a non-default stream, side 'B': Z and Q are written on that stream just
     before the call, and the vectors are read (and the buffers reused) on it right after, with
-    no device synchronisation in between.  The call's own copy of Z and its lookahead products
-    run on a second stream ordered by events, so the vectors equal the default-stream call's
-    bit for bit."""
+    no device synchronisation in between.  The call's own copy of Z (FP64 and FP32 forms) or
+    Z's digits (split form) and its lookahead products run on other streams ordered by events,
+    so the vectors equal the default-stream call's bit for bit."""
     import torch
 
     import xtgevc3_amd as X
+
+    prec = {"fp64": X.PREC_FP64, "split_i8": X.PREC_SPLIT_I8, "mixed_f32": X.PREC_MIXED_F32}[precision]
 
     n = 500
     S, P, ar, ai, be = G.real_quasi_triangular(n, 81)
@@ -200,7 +203,7 @@ def test_back_transform_on_a_side_stream_is_ordered():
     Z = torch.tensor(rng.standard_normal((n, n)), device="cuda")
     Q = torch.tensor(rng.standard_normal((n, n)), device="cuda")
     refR, refL = Z.clone(), Q.clone()
-    m, info = X.dtgevc3_device("B", "B", None, dS, dP, ar, ai, be, VL=refL, VR=refR, mm=n)
+    m, info = X.dtgevc3_device("B", "B", None, dS, dP, ar, ai, be, VL=refL, VR=refR, mm=n, precision=prec)
     assert (m, info) == (n, 0)
     torch.cuda.synchronize()
     s = torch.cuda.Stream()
@@ -209,7 +212,8 @@ def test_back_transform_on_a_side_stream_is_ordered():
         VL = torch.empty_like(Q)
         VR.copy_(Z)
         VL.copy_(Q)
-        m2, info2 = X.dtgevc3_device("B", "B", None, dS, dP, ar, ai, be, VL=VL, VR=VR, mm=n, stream=s.cuda_stream)
+        m2, info2 = X.dtgevc3_device("B", "B", None, dS, dP, ar, ai, be, VL=VL, VR=VR, mm=n, stream=s.cuda_stream,
+                                     precision=prec)
         outR = VR.clone()
         outL = VL.clone()
         VR.fill_(7.0)
