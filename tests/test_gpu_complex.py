@@ -110,3 +110,40 @@ def test_complex_large_residual(side):
     B = Z @ P @ Z.conj().T
     res = O.residual_A(A, B, g, al, be, left=side == "L")
     assert res.max() <= 10.0 * n * EPS, f"residual {res.max() / (n * EPS):.3g} n*eps"
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split_i8"])
+def test_complex_back_transform_on_a_side_stream_is_ordered(precision):
+    """ztgevc3 howmny = 'B', side 'B', on a non-default stream: Z and Q written on it just
+    before the call, the vectors read and the buffers reused on it right after, with no device
+    synchronisation in between; equal bit for bit to the default-stream call."""
+    import torch
+
+    import xtgevc3_amd as X
+
+    prec = {"fp64": X.PREC_FP64, "split_i8": X.PREC_SPLIT_I8}[precision]
+    n = 400
+    S, P, al, be = G.complex_triangular(n, 83)
+    dS, dP = X.fortran_device(S), X.fortran_device(P)
+    rng = np.random.default_rng(6)
+    Z = torch.tensor(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)), device="cuda")
+    Q = torch.tensor(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)), device="cuda")
+    refR, refL = Z.clone(), Q.clone()
+    m, info = X.ztgevc3_device("B", "B", None, dS, dP, al, be, VL=refL, VR=refR, mm=n, precision=prec)
+    assert (m, info) == (n, 0)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        VR = torch.empty_like(Z)
+        VL = torch.empty_like(Q)
+        VR.copy_(Z)
+        VL.copy_(Q)
+        m2, info2 = X.ztgevc3_device("B", "B", None, dS, dP, al, be, VL=VL, VR=VR, mm=n, stream=s.cuda_stream,
+                                     precision=prec)
+        outR = VR.clone()
+        outL = VL.clone()
+        VR.fill_(7.0)
+        VL.fill_(7.0)
+    torch.cuda.synchronize()
+    assert (m2, info2) == (n, 0)
+    assert torch.equal(outR, refR) and torch.equal(outL, refL)
