@@ -182,3 +182,42 @@ def test_backward_rejects_unknown_precision():
     Vt = torch.zeros((len(pick), n), dtype=torch.float64, device="cuda")
     with pytest.raises(ValueError):
         X.dtgevc3_backward_device(dS, dP, ar, ai, be, sel, Vt, Vt.clone(), nbatch=1, precision=7)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split_i8"])
+def test_backward_on_a_side_stream_is_ordered(precision):
+    """The backward on a non-default stream: the cotangents are written on it just before the
+    call, the gradients read and the buffers reused on it right after, with no device
+    synchronisation in between (the adjoint sweep's lookahead products and, in the split form,
+    the pencil digits run on other streams ordered by events); equal bit for bit to the
+    default-stream call."""
+    import torch
+
+    import xtgevc3_amd as X
+
+    prec = {"fp64": X.PREC_FP64, "split_i8": X.PREC_SPLIT_I8}[precision]
+    n, nsel, nb = 700, 64, 4
+    S, P, ar, ai, be, sel, pick = _problem(n, nsel, 12)
+    m = len(pick)
+    dS = X.fortran_device(S)
+    dP = X.fortran_device(P)
+    Vt = torch.zeros((m, n), dtype=torch.float64, device="cuda")
+    X.dtgevc3_device("R", "S", sel, dS, dP, ar, ai, be, VR=Vt, mm=m)
+    Gh = torch.tensor(np.random.default_rng(8).standard_normal((m * nb, n)), device="cuda")
+    refS, refP = X.dtgevc3_backward_device(dS, dP, ar, ai, be, sel, Vt, Gh, nbatch=nb, precision=prec)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        Gt = torch.empty_like(Gh)
+        Gt.copy_(Gh)
+        gS = torch.empty_like(refS)
+        gP = torch.empty_like(refP)
+        X.dtgevc3_backward_device(dS, dP, ar, ai, be, sel, Vt, Gt, nbatch=nb, dS=gS, dP=gP, stream=s.cuda_stream,
+                                  precision=prec)
+        outS = gS.clone()
+        outP = gP.clone()
+        Gt.fill_(7.0)
+        gS.fill_(7.0)
+        gP.fill_(7.0)
+    torch.cuda.synchronize()
+    assert torch.equal(outS, refS) and torch.equal(outP, refP)
