@@ -1,8 +1,11 @@
 """Writes the golden fixtures of tests/golden/*.npz.
 
 Inputs are the reference tests' own known-answer pencils (vlovero/xTGEVC3
-tests/test_dtgevc3.cpp:318-434, tests/test_ztgevc3.cpp:146-201) and seeded random pencils in
-the reference tests' construction (tests/test_lapack_d.cpp:170-237, test_lapack_z.cpp).
+tests/test_dtgevc3.cpp:318-434, tests/test_stgevc3.cpp:236-313, tests/test_ztgevc3.cpp:146-201),
+the same pencils with an infinite eigenvalue (beta = 0, or subnormal in the FP32 interfaces),
+seeded random pencils in the reference tests' construction (tests/test_lapack_d.cpp:170-237,
+test_lapack_z.cpp) and the reference tests' six overflow / underflow scalings of them
+(tests/test_lapack_d.cpp:328-344, tests/test_lapack_z.cpp:248-264).
 Expected outputs are the raw (unnormalised) VL/VR, m and info returned by the reference
 library itself, compiled from its sources by ``make -C oracle ref`` (oracle/_ref, never
 committed).  Run from the repository root:  python tests/golden/make_golden.py
@@ -91,10 +94,30 @@ def main():
         ar, ai, be = (KAT_AR * s).astype(np.float32), (KAT_AI * s).astype(np.float32), (KAT_BE * s).astype(np.float32)
         VL, VR, m, info = run_ref_real("s", "B", "B", None, S, P, ar, ai, be, eye4)
         save(tag, S=S, P=P, ar=ar, ai=ai, be=be, side="B", howmny="B", VL=VL, VR=VR, m=m, info=info)
+    # float KAT with an infinite eigenvalue: beta_3 = 0, and beta_3 subnormal in FP32
+    # (0 < |beta| <= FLT_MIN counts as infinite in stgevc3, stgevc3.cpp:619-624, :1219)
+    for tag, b3 in (("kat_s_inf", 0.0), ("kat_s_sub", 1e-40)):
+        S, P = KAT_S.astype(np.float32), KAT_P.astype(np.float32)
+        P[3, 3] = np.float32(b3)
+        ar, ai, be = KAT_AR.astype(np.float32), KAT_AI.astype(np.float32), KAT_BE.astype(np.float32)
+        be[3] = np.float32(b3)
+        VL, VR, m, info = run_ref_real("s", "B", "B", None, S, P, ar, ai, be, eye4)
+        save(tag, S=S, P=P, ar=ar, ai=ai, be=be, side="B", howmny="B", VL=VL, VR=VR, m=m, info=info)
     # complex KAT
     S, P, al, be = kat_complex()
     VL, VR, m, info = run_ref_cplx("z", "B", "B", None, S, P, al, be, eye4)
     save("kat_z", S=S, P=P, al=al, be=be, side="B", howmny="B", VL=VL, VR=VR, m=m, info=info)
+    # complex KAT with beta_3 = 0 (complex128) and beta_3 subnormal in complex64
+    Sz, Pz, alz, bez = kat_complex()
+    Pz[3, 3] = 0.0
+    bez[3] = 0.0
+    VL, VR, m, info = run_ref_cplx("z", "B", "B", None, Sz, Pz, alz, bez, eye4)
+    save("kat_z_inf", S=Sz, P=Pz, al=alz, be=bez, side="B", howmny="B", VL=VL, VR=VR, m=m, info=info)
+    Sc, Pc, alc, bec = (a.astype(np.complex64) for a in kat_complex())
+    Pc[3, 3] = np.complex64(1e-40)
+    bec[3] = np.complex64(1e-40)
+    VL, VR, m, info = run_ref_cplx("c", "B", "B", None, Sc, Pc, alc, bec, eye4)
+    save("kat_c_sub", S=Sc, P=Pc, al=alc, be=bec, side="B", howmny="B", VL=VL, VR=VR, m=m, info=info)
 
     # seeded random pencils in the reference tests' construction
     for n, seed in ((10, 100), (40, 101), (75, 102)):
@@ -131,6 +154,15 @@ def main():
         ar, ai, be = ar0 * sS, ai0 * sS, be0 * sP
         VL, VR, m, info = run_ref_real("d", "B", "B", None, S, P, ar, ai, be, np.eye(n))
         save(f"scale_d_{tag}", S=S, P=P, ar=ar, ai=ai, be=be, side="B", howmny="B", VL=VL, VR=VR, m=m, info=info)
+    # the same six scalings of a complex pencil (tests/test_lapack_z.cpp:142-153, :248-264)
+    Sz0, Pz0, al0, bz0 = G.reference_style_complex(n, 201)
+    for tag, sS, sP in (("ovS", bignum, 1.0), ("unS", smlnum, 1.0), ("ovP", 1.0, bignum), ("unP", 1.0, smlnum),
+                        ("ovSP", bignum, bignum), ("unSP", smlnum, smlnum)):
+        S = np.asfortranarray(Sz0 * sS)
+        P = np.asfortranarray(Pz0 * sP)
+        al, be = al0 * sS, bz0 * sP
+        VL, VR, m, info = run_ref_cplx("z", "B", "B", None, S, P, al, be, np.eye(n, dtype=np.complex128))
+        save(f"scale_z_{tag}", S=S, P=P, al=al, be=be, side="B", howmny="B", VL=VL, VR=VR, m=m, info=info)
 
 
 if __name__ == "__main__":

@@ -124,3 +124,59 @@ def check_cplx_parity(S, P, al, be, side="B", howmny="B", select=None, Q=None, Z
             assert bad.size == 0, f"{name}: cols {bad[:8]} err {err[bad[:8]]} tol {tol[bad[:8]]}"
         out[name] = (res.max() / (n * EPS), err.max())
     return out
+
+
+def eigvec_cond_sampled(S, P, alpha, beta, kind, idx, left=False, iters=24):
+    """cond_k for the parity tolerance of selected columns at sizes where a dense SVD per
+    column is too slow: (|beta_k| ||S||_F + |alpha_k| ||P||_F) / sigma_min(T_k), with T_k the
+    part of beta_k S - alpha_k P that the column's substitution inverts (rows / columns [0, k)
+    for right vectors, [k + w, n) for left ones, as in oracle.eigvec_cond).  sigma_min comes
+    from inverse power iteration on T^H T with one LU of T, which approaches it from above,
+    so the estimate never exceeds the true cond_k (the tolerance errs on the strict side).
+    S and P are scaled to unit maximum first; the value is invariant under separate scalings
+    of S and P (and of alpha and beta with them)."""
+    import scipy.linalg as sl
+
+    S = np.asarray(S, dtype=np.complex128)
+    P = np.asarray(P, dtype=np.complex128)
+    fs = 1.0 / max(np.abs(S).max(), np.finfo(np.float64).tiny)
+    fp = 1.0 / max(np.abs(P).max(), np.finfo(np.float64).tiny)
+    S = S * fs
+    P = P * fp
+    nS, nP = np.linalg.norm(S), np.linalg.norm(P)
+    rng = np.random.default_rng(0)
+    out = np.zeros(len(idx))
+    for c, k in enumerate(idx):
+        w = 2 if kind[c] in (1, 2) else 1
+        k0 = k if kind[c] != 2 else k - 1
+        al, be = alpha[c] * fs, beta[c] * fp
+        sl_ = slice(k0 + w, None) if left else slice(0, k0)
+        T = be * S[sl_, sl_] - al * P[sl_, sl_]
+        if T.shape[0] == 0:
+            out[c] = 1.0
+            continue
+        lu = sl.lu_factor(T, check_finite=False)
+        if np.any(np.diag(lu[0]) == 0):
+            out[c] = np.inf
+            continue
+        x = rng.standard_normal(T.shape[0]) + 0j
+        x /= np.linalg.norm(x)
+        est = 0.0
+        for _ in range(iters):
+            y = sl.lu_solve(lu, x, check_finite=False)          # T^-1 x
+            z = sl.lu_solve(lu, y, trans=2, check_finite=False)  # T^-H T^-1 x
+            est = np.linalg.norm(z)
+            if not np.isfinite(est) or est == 0.0:
+                break
+            x = z / est
+        smin = 1.0 / np.sqrt(est) if est > 0 and np.isfinite(est) else 0.0
+        out[c] = np.inf if smin == 0 else (abs(be) * nS + abs(al) * nP) / smin
+    return out
+
+
+def compare_sampled(g, o, cond, tol_factor=100.0):
+    """Per-column max |g - o| of normalised vectors against tol_factor * eps * max(cond, 1);
+    returns the offending column positions."""
+    err = np.max(np.abs(g - o), axis=0)
+    tol = tol_factor * EPS * np.maximum(cond, 1.0)
+    return np.nonzero(~(err <= tol))[0], err, tol

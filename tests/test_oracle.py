@@ -50,7 +50,7 @@ def test_oracle_real_matches_golden_raw(name):
             assert _rel(mine[:, c], ref[:, c]) < 1e-9, (name, c)
 
 
-@pytest.mark.parametrize("name", _golden_names("kat_z") + _golden_names("rand_z"))
+@pytest.mark.parametrize("name", _golden_names("kat_z") + _golden_names("rand_z") + _golden_names("scale_z"))
 def test_oracle_complex_matches_golden_raw(name):
     g = _golden(name)
     n = g["S"].shape[0]
@@ -65,12 +65,22 @@ def test_oracle_complex_matches_golden_raw(name):
             assert _rel(mine[:, c], ref[:, c]) < 1e-9, (name, c)
 
 
-@pytest.mark.parametrize("name", ["kat_s_big", "kat_s_small"])
+@pytest.mark.parametrize("name", ["kat_s_big", "kat_s_small", "kat_s_inf", "kat_s_sub"])
 def test_oracle_float_matches_golden(name):
     g = _golden(name)
     VL = np.asfortranarray(np.eye(4, dtype=np.float32))
     VR = np.asfortranarray(np.eye(4, dtype=np.float32))
     m, info, _ = O.tgevc3("s", "B", "B", None, g["S"], g["P"], g["ar"], g["be"], alphai=g["ai"], VL=VL, VR=VR)
+    assert (m, info) == (int(g["m"]), int(g["info"]))
+    assert _rel(VL, g["VL"]) < 1e-5 and _rel(VR, g["VR"]) < 1e-5
+
+
+def test_oracle_complex64_subnormal_beta_matches_golden():
+    """complex64 KAT with beta_3 = 1e-40: the reference (ctgevc3) counts it as infinite."""
+    g = _golden("kat_c_sub")
+    VL = np.asfortranarray(np.eye(4, dtype=np.complex64))
+    VR = np.asfortranarray(np.eye(4, dtype=np.complex64))
+    m, info, _ = O.tgevc3("c", "B", "B", None, g["S"], g["P"], g["al"], g["be"], VL=VL, VR=VR)
     assert (m, info) == (int(g["m"]), int(g["info"]))
     assert _rel(VL, g["VL"]) < 1e-5 and _rel(VR, g["VR"]) < 1e-5
 

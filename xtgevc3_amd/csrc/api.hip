@@ -475,7 +475,8 @@ void real_host(char side, char howmny, const int *select, int n, const T *S, int
         zs.feed.out32[1] = left ? g_dVL32.p : nullptr;
     }
     int rc = real_device(side, howmny, select, n, dS, n, dP, n, ar, ai, be, left ? dVL : nullptr, n,
-                         right ? dVR : nullptr, n, st, XTGEVC3_PREC_FP64, back ? &zs.feed : nullptr);
+                         right ? dVR : nullptr, n, st, XTGEVC3_PREC_FP64, back ? &zs.feed : nullptr,
+                         narrow ? kFltMin : kDblMin);
     zs.finish();
     if (rc != 0) {
         (void)hipDeviceSynchronize();
@@ -566,7 +567,7 @@ void complex_host(char side, char howmny, const int *select, int n, const std::c
     }
     int rc = complex_device(side, howmny, select, n, (const zt *)dS, n, (const zt *)dP, n, al, be,
                             left ? (zt *)dVL : nullptr, n, right ? (zt *)dVR : nullptr, n, st, XTGEVC3_PREC_FP64,
-                            back ? &zs.feed : nullptr);
+                            back ? &zs.feed : nullptr, narrow ? kFltMin : kDblMin);
     zs.finish();
     if (rc != 0) {
         (void)hipDeviceSynchronize();

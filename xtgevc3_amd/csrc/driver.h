@@ -47,15 +47,19 @@ struct ZFeed {
 };
 
 // prec (XTGEVC3_PREC_*): FP64 throughout; FP32-product or split-int8 far updates and
-// back-transform (mixed forms)
+// back-transform (mixed forms).  safemin: smallest normal number of the interface precision
+// (FLT_MIN for the FP32 / complex64 interfaces), the floor of the global scales and shift
+// normalisers and the threshold below which |beta| makes an eigenvalue infinite.
+constexpr double kDblMin = 2.2250738585072014e-308;
+constexpr double kFltMin = 1.1754943508222875e-38;
 int real_device(char side, char howmny, const int *select, int n, const double *S, long lds, const double *P,
                 long ldp, const double *ar, const double *ai, const double *be, double *VL, long ldvl, double *VR,
-                long ldvr, hipStream_t st, int prec = 0, const ZFeed *zf = nullptr);
+                long ldvr, hipStream_t st, int prec = 0, const ZFeed *zf = nullptr, double safemin = kDblMin);
 
 int complex_device(char side, char howmny, const int *select, int n, const std::complex<double> *S, long lds,
                    const std::complex<double> *P, long ldp, const std::complex<double> *al,
                    const std::complex<double> *be, std::complex<double> *VL, long ldvl, std::complex<double> *VR,
-                   long ldvr, hipStream_t st, int prec = 0, const ZFeed *zf = nullptr);
+                   long ldvr, hipStream_t st, int prec = 0, const ZFeed *zf = nullptr, double safemin = kDblMin);
 
 int real_backward_device(int n, const double *S, long lds, const double *P, long ldp, const double *ar,
                          const double *ai, const double *be, const int *select, const double *V, long ldv,

@@ -149,9 +149,9 @@ ZWork g_zws[64];
 typedef std::complex<double> zt;
 
 // complex shift coefficients, in the reference's operation order (ztgevc3.cpp:145-153)
-void shift_cplx(zt al, zt bv, double ascale, double bscale, zt &a, zt &b)
+// safemin: smallest normal number of the interface precision (FLT_MIN for ctgevc3)
+void shift_cplx(zt al, zt bv, double ascale, double bscale, zt &a, zt &b, double safemin)
 {
-    const double safemin = std::numeric_limits<double>::min();
     double t = 1.0 / std::max(std::max(std::abs(al) * ascale, std::abs(bv) * bscale), safemin);
     a = zt(t * ascale, 0.0) * bv * zt(bscale, 0.0);
     b = zt(t * bscale, 0.0) * al * zt(ascale, 0.0);
@@ -187,7 +187,7 @@ static hipError_t zfar_stream_init(ZWork &ws)
 int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 *Pw, long ldpw, const zt *al,
                   const zt *be, double ascale, double bscale, bool left, bool back, const double2 *Z, long ldz,
                   double2 *V, long ldv, const std::vector<Unit> &units, int m, hipStream_t st, int prec,
-                  const ZFeed *zf)
+                  const ZFeed *zf, double safemin)
 {
     const bool split = prec == XTGEVC3_PREC_SPLIT_I8;
     std::vector<ZColDesc> desc(m);
@@ -195,7 +195,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
     for (int p = 0; p < m; ++p) {
         const Unit &un = left ? units[m - 1 - p] : units[p];
         zt a, b;
-        shift_cplx(al[un.k], be[un.k], ascale, bscale, a, b);
+        shift_cplx(al[un.k], be[un.k], ascale, bscale, a, b, safemin);
         if (left) {
             a = std::conj(a);
             b = std::conj(b);
@@ -591,7 +591,7 @@ int run_cplx_side(ZWork &ws, int n, const double2 *Sw, long ldsw, const double2 
 int complex_device(char side, char howmny, const int *select, int n, const std::complex<double> *S, long lds,
                    const std::complex<double> *P, long ldp, const std::complex<double> *al,
                    const std::complex<double> *be, std::complex<double> *VL, long ldvl, std::complex<double> *VR,
-                   long ldvr, hipStream_t st, int prec, const ZFeed *zf)
+                   long ldvr, hipStream_t st, int prec, const ZFeed *zf, double safemin)
 {
     if (zf != nullptr && prec == XTGEVC3_PREC_SPLIT_I8) {
         set_error("Z delivered beside the sweep is not supported in the split form");
@@ -623,7 +623,6 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
         an = std::max(an, nrm[c]);
         bn = std::max(bn, nrm[n + c]);
     }
-    const double safemin = std::numeric_limits<double>::min();
     double ascale = 1.0 / std::max(an, safemin);
     double bscale = 1.0 / std::max(bn, safemin);
     if (right && VR) {
@@ -644,7 +643,7 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
             Zp = ws.Zt.as<double2>();
         }
         int rc = run_cplx_side(ws, n, dS, lds, dP, ldp, al, be, ascale, bscale, false, back, Zp, zld,
-                               reinterpret_cast<double2 *>(VR), ldvr, units, m, st, prec, zf);
+                               reinterpret_cast<double2 *>(VR), ldvr, units, m, st, prec, zf, safemin);
         if (rc) {
             return rc;
         }
@@ -672,7 +671,7 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
             Zp = ws.Zt.as<double2>();
         }
         int rc = run_cplx_side(ws, n, ws.Sw.as<double2>(), n, ws.Pw.as<double2>(), n, al, be, ascale, bscale, true,
-                               back, Zp, zld, reinterpret_cast<double2 *>(VL), ldvl, units, m, st, prec, zf);
+                               back, Zp, zld, reinterpret_cast<double2 *>(VL), ldvl, units, m, st, prec, zf, safemin);
         if (rc) {
             return rc;
         }

@@ -179,10 +179,11 @@ template bool check_args<std::complex<double>>(char, int, int, int, int *, std::
 template bool check_args<std::complex<float>>(char, int, int, int, int *, std::complex<float> *, int, int *, bool);
 
 // Shift coefficients for one eigenvalue (dtgevc3.cpp:716-725; with aI = 0 this is :647-655).
+// safemin is the smallest normal number of the interface precision (FLT_MIN for stgevc3,
+// stgevc3.cpp:619-624, :688-693, :1219), which also decides that an eigenvalue is infinite.
 static void shift_real(double aR, double aI, double bv, double ascale, double bscale, double &a, double &bR,
-                       double &bI, bool pairform)
+                       double &bI, bool pairform, double safemin = std::numeric_limits<double>::min())
 {
-    const double safemin = std::numeric_limits<double>::min();
     double num = pairform ? std::max(std::max(std::fabs(aR) * ascale + std::fabs(aI) * ascale, std::fabs(bv) * bscale), safemin)
                           : std::max(std::max(std::fabs(aR) * ascale, std::fabs(bv) * bscale), safemin);
     double t = 1.0 / num;
@@ -405,6 +406,7 @@ struct RealSideArgs {
     const std::vector<double> *subw; // working subdiagonal, host
     int prec;                        // 0: FP64, 1: FP32-product updates and back-transform
     const ZFeed *zf;                 // Z arriving on another stream (null: Z is ready)
+    double safemin;                  // smallest normal number of the interface precision
 };
 
 // ---------------------------------------------------------------------------------------
@@ -819,7 +821,7 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
         const int k = un.k;
         double a, bR, bI;
         if (un.w == 1) {
-            shift_real(A.ar[k], 0.0, A.be[k], A.ascale, A.bscale, a, bR, bI, false);
+            shift_real(A.ar[k], 0.0, A.be[k], A.ascale, A.bscale, a, bR, bI, false, A.safemin);
             ColDesc c{};
             c.a = a;
             c.bR = bR;
@@ -834,7 +836,7 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
             ++p;
         }
         else {
-            shift_real(A.ar[k], A.ai[k], A.be[k], A.ascale, A.bscale, a, bR, bI, true);
+            shift_real(A.ar[k], A.ai[k], A.be[k], A.ascale, A.bscale, a, bR, bI, true, A.safemin);
             ColDesc c{};
             c.a = a;
             c.bR = bR;
@@ -1028,7 +1030,7 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
 // ---------------------------------------------------------------------------------------
 int real_device(char side, char howmny, const int *select, int n, const double *S, long lds, const double *P,
                 long ldp, const double *ar, const double *ai, const double *be, double *VL, long ldvl, double *VR,
-                long ldvr, hipStream_t st, int prec, const ZFeed *zf)
+                long ldvr, hipStream_t st, int prec, const ZFeed *zf, double safemin)
 {
     if (zf != nullptr && prec == XTGEVC3_PREC_SPLIT_I8) {
         set_error("Z delivered beside the sweep is not supported in the split form");
@@ -1065,7 +1067,6 @@ int real_device(char side, char howmny, const int *select, int n, const double *
         an = std::max(an, nrm[c]);
         bn = std::max(bn, nrm[n + c]);
     }
-    const double safemin = std::numeric_limits<double>::min();
     double ascale = 1.0 / std::max(an, safemin);
     double bscale = 1.0 / std::max(bn, safemin);
     // the back-transform's own copy of Z (the output overwrites VR / VL) arrives on ws.fs
@@ -1091,6 +1092,7 @@ int real_device(char side, char howmny, const int *select, int n, const double *
         A.V = VR;
         A.ldv = ldvr;
         A.zf = zf;
+        A.safemin = safemin;
         if (back && zf != nullptr) {
             A.Z = static_cast<const double *>(zf->zr);
             A.ldz = n;
@@ -1142,6 +1144,7 @@ int real_device(char side, char howmny, const int *select, int n, const double *
         A.V = VL;
         A.ldv = ldvl;
         A.zf = zf;
+        A.safemin = safemin;
         if (back && zf != nullptr) {
             A.Z = static_cast<const double *>(zf->zl);
             A.ldz = n;

@@ -11,8 +11,8 @@ triangular, and eigenvalues given explicitly, as the reference's interface expec
   groups with relative spread 1e-6 (near-singular shifted blocks), or scale the
   off-diagonal entries (``offdiag``; the eigenvectors of N(0,1) pencils are exponentially
   ill-conditioned in n, a smaller scale gives well-conditioned test problems).
-* ``complex_triangular`` -- complex S, P upper triangular, off-diagonal CN(0,1), diag S
-  CN(0,1), diag P |N(0,1)|+0.1 (real).
+* ``complex_triangular`` -- complex S, P upper triangular, off-diagonal CN(0,1) (times
+  ``offdiag``), diag S CN(0,1), diag P |N(0,1)|+0.1 (real).
 * ``reference_style_real`` / ``reference_style_complex`` -- the reference tests' own
   construction (tests/test_lapack_d.cpp:170-220, tests/test_lapack_z.cpp:115-140): U(-1,1)
   entries, P diagonal pushed away from zero by +-1, pairs [[a, b], [-b, a]] with P = I.
@@ -104,14 +104,14 @@ def real_quasi_triangular(n, seed, pair_frac=0.4, grade=0.0, cluster_groups=0, c
     return (np.asfortranarray(S), np.asfortranarray(P), ar, ai, be)
 
 
-def complex_triangular(n, seed):
+def complex_triangular(n, seed, offdiag=1.0):
     rng = np.random.default_rng(seed)
 
     def cn(shape):
         return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2.0)
 
-    S = np.triu(cn((n, n)), 1)
-    P = np.triu(cn((n, n)), 1)
+    S = np.triu(cn((n, n)), 1) * offdiag
+    P = np.triu(cn((n, n)), 1) * offdiag
     ds = cn(n)
     dp = np.abs(rng.standard_normal(n)) + 0.1
     S[np.arange(n), np.arange(n)] = ds
