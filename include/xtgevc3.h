@@ -121,6 +121,23 @@ int xtgevc3_dtgevc3_backward_device_ex(int n, const double *S, int lds, const do
                                        int nbatch, double *dS, int ldds, double *dP, int lddp, int *info, void *stream,
                                        int precision);
 
+/* Allocation-free variant of xtgevc3_dtgevc3_device (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * xtgevc3_dtgevc3_device (FP64), for n <= XTGEVC3_NOALLOC_MAX_N; larger n returns
+ * XTGEVC3_INFO_UNSUPPORTED with a message and touches nothing.  The call makes no hipMalloc /
+ * hipFree and creates no stream or event: its workspace is one static device array per GPU of
+ * at most XTGEVC3_NOALLOC_SCRATCH_BYTES (held until the process ends) and every launch and copy
+ * runs on `stream`.  All calls on one GPU share that array: calls that overlap in time on one
+ * GPU (other streams, other host threads without ordering) race.  Like the other device
+ * entries it synchronises `stream` once per call to read the pencil's column norms. */
+#define XTGEVC3_NOALLOC_MAX_N 512
+#define XTGEVC3_NOALLOC_SCRATCH_BYTES (13u << 20)
+int xtgevc3_dtgevc3_device_noalloc(char side, char howmny, const int *select, int n, const double *S, int lds,
+                                   const double *P, int ldp, const double *alphar, const double *alphai,
+                                   const double *beta, double *VL, int ldvl, double *VR, int ldvr, int mm, int *m,
+                                   int *info, void *stream);
+
 /* Eigenvalue condition numbers (no counterpart in the reference; the semantics of LAPACK
  * dtgsna with JOB = 'E'): for the eigenvalues chosen by howmny ('A' all, 'S' by select, with
  * dtgevc3's selection rule, complex pairs from alphai) and their left and right eigenvectors

@@ -47,7 +47,10 @@ EXPORTED_SYMBOLS = [
     "xtgevc3_dtgevc3_device", "xtgevc3_dtgevc3_device_ex", "xtgevc3_ztgevc3_device", "xtgevc3_ztgevc3_device_ex",
     "xtgevc3_profile_enable", "xtgevc3_profile_read", "xtgevc3_dtgevc3_backward_device",
     "xtgevc3_dtgevc3_backward_device_ex", "xtgevc3_dtgsna_e_device", "xtgevc3_ztgsna_e_device",
+    "xtgevc3_dtgevc3_device_noalloc",
 ]
+
+NOALLOC_MAX_N = 512
 
 PROF_KINDS = ("solve", "update", "back", "other", "update_128x128", "update_128x64", "update_64x128",
               "update_split", "back_split", "split_digits")
@@ -211,6 +214,35 @@ def tgevc3_device(prefix, side, howmny, select, S, P, alpha, beta, alphai=None, 
     if precision != PREC_FP64:
         args.append(c.c_int(precision))
     rc = fn(*args)
+    if rc == INFO_UNSUPPORTED:
+        raise ValueError("xtgevc3: " + last_error())
+    if rc != 0:
+        raise RuntimeError("xtgevc3 device path failed: " + last_error())
+    return m.value, info.value
+
+
+def dtgevc3_device_noalloc(side, howmny, select, S, P, alphar, alphai, beta, VL=None, VR=None, mm=None,
+                           stream=None):
+    """``xtgevc3_dtgevc3_device_noalloc``: the FP64 device call with no device allocation (a
+    static per-GPU workspace, n <= NOALLOC_MAX_N); arguments and result as dtgevc3_device.
+    Raises ValueError past the size limit."""
+    import torch
+
+    c = ctypes
+    n = S.shape[0]
+    if mm is None:
+        mm = n
+    sel = None if select is None else np.ascontiguousarray(select, dtype=np.int32)
+    ev = [np.ascontiguousarray(x, dtype=np.float64) for x in (alphar, alphai, beta)]
+    m = c.c_int(-1)
+    info = c.c_int(-99)
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    rc = _lib.xtgevc3_dtgevc3_device_noalloc(
+        c.c_char(side.encode()), c.c_char(howmny.encode()), _ptr(sel), c.c_int(n), _tptr(S), c.c_int(S.shape[1]),
+        _tptr(P), c.c_int(P.shape[1]), *[_ptr(e) for e in ev], _tptr(VL), c.c_int(VL.shape[1] if VL is not None else 1),
+        _tptr(VR), c.c_int(VR.shape[1] if VR is not None else 1), c.c_int(mm), c.byref(m), c.byref(info),
+        c.c_void_p(stream))
     if rc == INFO_UNSUPPORTED:
         raise ValueError("xtgevc3: " + last_error())
     if rc != 0:

@@ -668,6 +668,40 @@ int xtgevc3_dtgevc3_device(char side, char howmny, const int *select, int n, con
                                      ldvr, mm, m, info, stream, XTGEVC3_PREC_FP64);
 }
 
+int xtgevc3_dtgevc3_device_noalloc(char side, char howmny, const int *select, int n, const double *S, int lds,
+                                   const double *P, int ldp, const double *alphar, const double *alphai,
+                                   const double *beta, double *VL, int ldvl, double *VR, int ldvr, int mm, int *m,
+                                   int *info, void *stream)
+{
+    if (n > XTGEVC3_NOALLOC_MAX_N) {
+        xtg::set_error("n = " + std::to_string(n) + " exceeds XTGEVC3_NOALLOC_MAX_N = " +
+                       std::to_string(XTGEVC3_NOALLOC_MAX_N) + " of the allocation-free entry");
+        if (info) {
+            *info = XTGEVC3_INFO_UNSUPPORTED;
+        }
+        return XTGEVC3_INFO_UNSUPPORTED;
+    }
+    std::vector<xtg::Unit> units;
+    int num_sel = (howmny == 'A' || howmny == 'B' || howmny == 'S') && n > 0
+                      ? xtg::real_units(howmny, select, n, alphai, units)
+                      : 0;
+    if (!xtg::check_args<double>(howmny, n, num_sel, mm, m, (double *)nullptr, 0, info, false)) {
+        return 0;
+    }
+    int rc = xtg::real_device_noalloc(side, howmny, select, n, S, lds, P, ldp, alphar, alphai, beta, VL, ldvl, VR,
+                                      ldvr, (hipStream_t)stream);
+    if (rc == XTGEVC3_INFO_UNSUPPORTED) {
+        if (info) {
+            *info = XTGEVC3_INFO_UNSUPPORTED;
+        }
+        return rc;
+    }
+    if (rc != 0) {
+        return xtg::fail(info, "device path");
+    }
+    return 0;
+}
+
 
 void xtgevc3_ztgevc3(char side, char howmny, const int *select, int n, const xtgevc3_zcomplex *S, int lds,
                      const xtgevc3_zcomplex *P, int ldp, const xtgevc3_zcomplex *alpha, const xtgevc3_zcomplex *beta,

@@ -61,6 +61,11 @@ int complex_device(char side, char howmny, const int *select, int n, const std::
                    const std::complex<double> *be, std::complex<double> *VL, long ldvl, std::complex<double> *VR,
                    long ldvr, hipStream_t st, int prec = 0, const ZFeed *zf = nullptr, double safemin = kDblMin);
 
+// allocation-free FP64 variant of real_device (static per-device workspace, n <= XTGEVC3_NOALLOC_MAX_N)
+int real_device_noalloc(char side, char howmny, const int *select, int n, const double *S, long lds,
+                        const double *P, long ldp, const double *ar, const double *ai, const double *be, double *VL,
+                        long ldvl, double *VR, long ldvr, hipStream_t st);
+
 int real_backward_device(int n, const double *S, long lds, const double *P, long ldp, const double *ar,
                          const double *ai, const double *be, const int *select, const double *V, long ldv,
                          const double *G, long ldg, int nbatch, double *dS, long ldds, double *dP, long lddp, int *info,

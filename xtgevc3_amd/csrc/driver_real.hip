@@ -307,10 +307,14 @@ static void make_tiles(const std::vector<UnitDesc> &units, const std::vector<int
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    bool fixed = false; // a slot of the allocation-free workspace: never reallocated
     hipError_t ensure(size_t bytes)
     {
         if (bytes <= cap) {
             return hipSuccess;
+        }
+        if (fixed) {
+            return hipErrorOutOfMemory;
         }
         if (p) {
             hipError_t e = hipFree(p);
@@ -344,6 +348,9 @@ struct RealWork {
     hipEvent_t ev_wready[2] = {nullptr, nullptr}, ev_fdone[2] = {nullptr, nullptr};
     hipEvent_t ev_zsrc = nullptr, ev_zcopy = nullptr; // copy of Z on `fs`: its start and end
     hipEvent_t ev_last = nullptr; // end of the previous call's work on this workspace
+    // allocation-free workspace (buffers are fixed slots of a static device array): every
+    // launch and copy stays on the caller's stream, no stream or event is created
+    bool fixed = false;
 };
 
 // Orders a call after the previous call's work on the device's workspace, whatever stream
@@ -503,7 +510,8 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     // The group's operand W holds rows [0, g1-g0) (S part) and [g1-g0, 2(g1-g0)) (P part).
     int c_begin = adjoint ? 0 : m;
     int c_end = adjoint ? 0 : m;
-    static const bool lookahead = std::getenv("XTGEVC3_NO_LOOKAHEAD") == nullptr;
+    static const bool lookahead_env = std::getenv("XTGEVC3_NO_LOOKAHEAD") == nullptr;
+    const bool lookahead = lookahead_env && !ws.fixed;
     if (lookahead) {
         XTG_CHECK(far_stream_init(ws));
     }
@@ -1028,6 +1036,11 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
 // alphar, alphai, beta are host arrays.  No argument checks beyond the reference's: call
 // check_args first.
 // ---------------------------------------------------------------------------------------
+static int real_device_ws(RealWork &ws, char side, char howmny, const int *select, int n, const double *S, long lds,
+                          const double *P, long ldp, const double *ar, const double *ai, const double *be, double *VL,
+                          long ldvl, double *VR, long ldvr, hipStream_t st, int prec, const ZFeed *zf,
+                          double safemin);
+
 int real_device(char side, char howmny, const int *select, int n, const double *S, long lds, const double *P,
                 long ldp, const double *ar, const double *ai, const double *be, double *VL, long ldvl, double *VR,
                 long ldvr, hipStream_t st, int prec, const ZFeed *zf, double safemin)
@@ -1042,7 +1055,121 @@ int real_device(char side, char howmny, const int *select, int n, const double *
     RealWork &ws = g_ws[dev & 63];
     WsOrder order{&ws.ev_last, st};
     XTG_CHECK(order.begin());
+    return real_device_ws(ws, side, howmny, select, n, S, lds, P, ldp, ar, ai, be, VL, ldvl, VR, ldvr, st, prec, zf,
+                          safemin);
+}
 
+// ---------------------------------------------------------------------------------------
+// Allocation-free variant of the FP64 device entry: the workspace is carved from one static
+// device array per GPU (`scratch`, NOALLOC_SCRATCH_BYTES, held until the process ends), so a
+// call makes no hipMalloc / hipFree and creates no stream or event; everything runs on the
+// caller's stream.  Sizes up to NOALLOC_MAX_N.  The array's device address comes from
+// hipGetSymbolAddress once per device (first call on it, with the device current); host code
+// uses only that address.  Calls that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------------------
+constexpr int NOALLOC_MAX_N = XTGEVC3_NOALLOC_MAX_N;
+constexpr size_t na_slot(size_t bytes) { return (bytes + 255) / 256 * 256; }
+constexpr size_t NA_N = NOALLOC_MAX_N;
+constexpr size_t NA_LDX = (NA_N + 7) / 8 * 8;
+constexpr size_t NA_STEPS = NA_N / 32 + 2; // row blocks have at least 63 rows
+// slot sizes (bytes) of the buffers the FP64 path uses, at the maximum size
+constexpr size_t NA_BYTES[] = {
+    sizeof(double) * 3 * NA_N,                                 // norms
+    sizeof(double) * 2 * (2 * GROUP_STEPS * 65) * NA_LDX,      // W (two group copies)
+    sizeof(double) * NA_N,                                     // sig
+    sizeof(double) * 2 * NA_N,                                 // sig_grp
+    sizeof(int) * NA_N,                                        // colexp
+    sizeof(int) * NA_STEPS * NA_LDX,                           // blkexp
+    sizeof(int) * 4,                                           // flag
+    sizeof(UnitDesc) * NA_N,                                   // desc
+    sizeof(int4) * 2 * NA_N,                                   // tiles
+    sizeof(int) * NA_N,                                        // rowstep
+    sizeof(int) * NA_N,                                        // outcol
+    sizeof(int) * NA_N,                                        // kmax
+    sizeof(int) * NA_N,                                        // vcol
+    sizeof(int) * NA_N,                                        // vw
+    sizeof(double) * NA_N * NA_LDX,                            // X
+    sizeof(double) * NA_N * NA_N,                              // Zt
+    sizeof(double) * NA_N * NA_N,                              // Sw
+    sizeof(double) * NA_N * NA_N,                              // Pw
+};
+constexpr size_t na_total()
+{
+    size_t t = 0;
+    for (size_t b : NA_BYTES) {
+        t += na_slot(b);
+    }
+    return t;
+}
+constexpr size_t NOALLOC_SCRATCH_BYTES = na_total();
+static_assert(NOALLOC_SCRATCH_BYTES <= (size_t)XTGEVC3_NOALLOC_SCRATCH_BYTES, "documented scratch size");
+
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_na_mutex;
+static RealWork g_na_ws[64];
+static void *g_na_ptr[64]; // device address of `scratch` on each device
+
+static hipError_t noalloc_setup(int dev)
+{
+    if (g_na_ptr[dev] != nullptr) {
+        return hipSuccess;
+    }
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        return e;
+    }
+    if (ptr == nullptr) {
+        return hipErrorInvalidSymbol;
+    }
+    RealWork &ws = g_na_ws[dev];
+    DevBuf *slots[] = {&ws.norms, &ws.W,      &ws.sig,  &ws.sig_grp, &ws.colexp, &ws.blkexp,
+                       &ws.flag,  &ws.desc,   &ws.tiles, &ws.rowstep, &ws.outcol, &ws.kmax,
+                       &ws.vcol,  &ws.vw,     &ws.X,    &ws.Zt,      &ws.Sw,     &ws.Pw};
+    static_assert(sizeof(slots) / sizeof(slots[0]) == sizeof(NA_BYTES) / sizeof(NA_BYTES[0]), "one slot per buffer");
+    size_t off = 0;
+    for (size_t i = 0; i < sizeof(NA_BYTES) / sizeof(NA_BYTES[0]); ++i) {
+        slots[i]->p = static_cast<unsigned char *>(ptr) + off;
+        slots[i]->cap = NA_BYTES[i];
+        slots[i]->fixed = true;
+        off += na_slot(NA_BYTES[i]);
+    }
+    // every other buffer of the workspace is a zero-capacity fixed slot: using one is an error
+    DevBuf *unused[] = {&ws.rowexp, &ws.zexp, &ws.wblk, &ws.wgrp, &ws.xrel, &ws.xfin, &ws.kcol, &ws.coef,
+                        &ws.Vt,     &ws.Y,    &ws.LamS, &ws.LamP, &ws.bkpre, &ws.Ut,  &ws.YB,   &ws.part,
+                        &ws.tcol,   &ws.tw,   &ws.Sq,   &ws.Pq,   &ws.Wg,    &ws.wgexp, &ws.wbexp, &ws.xbexp,
+                        &ws.sstart, &ws.Xq,   &ws.Zq,   &ws.xexp, &ws.cfar};
+    for (DevBuf *b : unused) {
+        b->fixed = true;
+    }
+    ws.fixed = true;
+    g_na_ptr[dev] = ptr;
+    return hipSuccess;
+}
+
+int real_device_noalloc(char side, char howmny, const int *select, int n, const double *S, long lds,
+                        const double *P, long ldp, const double *ar, const double *ai, const double *be, double *VL,
+                        long ldvl, double *VR, long ldvr, hipStream_t st)
+{
+    if (n > NOALLOC_MAX_N) {
+        set_error("n = " + std::to_string(n) + " exceeds the allocation-free entry's maximum of " +
+                  std::to_string(NOALLOC_MAX_N));
+        return XTGEVC3_INFO_UNSUPPORTED;
+    }
+    int dev = 0;
+    XTG_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    XTG_CHECK(noalloc_setup(dev & 63));
+    return real_device_ws(g_na_ws[dev & 63], side, howmny, select, n, S, lds, P, ldp, ar, ai, be, VL, ldvl, VR, ldvr,
+                          st, XTGEVC3_PREC_FP64, nullptr, kDblMin);
+}
+
+static int real_device_ws(RealWork &ws, char side, char howmny, const int *select, int n, const double *S, long lds,
+                          const double *P, long ldp, const double *ar, const double *ai, const double *be, double *VL,
+                          long ldvl, double *VR, long ldvr, hipStream_t st, int prec, const ZFeed *zf,
+                          double safemin)
+{
     bool right = side == 'R' || side == 'B';
     bool left = side == 'L' || side == 'B';
     bool back = howmny == 'B';
@@ -1105,10 +1232,16 @@ int real_device(char side, char howmny, const int *select, int n, const double *
         else if (back) {
             // the input VR holds Z; keep a copy since the output overwrites it
             ENSURE(ws.Zt, sizeof(double) * (size_t)n * n);
-            XTG_CHECK(zcopy_async(ws, VR, ldvr, n, st));
+            if (ws.fixed) {
+                XTG_CHECK(hipMemcpy2DAsync(ws.Zt.p, sizeof(double) * n, VR, sizeof(double) * ldvr,
+                                           sizeof(double) * n, n, hipMemcpyDeviceToDevice, st));
+            }
+            else {
+                XTG_CHECK(zcopy_async(ws, VR, ldvr, n, st));
+                A.zf = &zown;
+            }
             A.Z = ws.Zt.as<double>();
             A.ldz = n;
-            A.zf = &zown;
         }
         int rc = run_real_side(ws, A, units, m, st);
         if (rc) {
@@ -1156,10 +1289,16 @@ int real_device(char side, char howmny, const int *select, int n, const double *
         else if (back) {
             // queued after the right side's back-transform, which reads the same workspace
             ENSURE(ws.Zt, sizeof(double) * (size_t)n * n);
-            XTG_CHECK(zcopy_async(ws, VL, ldvl, n, st));
+            if (ws.fixed) {
+                XTG_CHECK(hipMemcpy2DAsync(ws.Zt.p, sizeof(double) * n, VL, sizeof(double) * ldvl,
+                                           sizeof(double) * n, n, hipMemcpyDeviceToDevice, st));
+            }
+            else {
+                XTG_CHECK(zcopy_async(ws, VL, ldvl, n, st));
+                A.zf = &zown;
+            }
             A.Z = ws.Zt.as<double>();
             A.ldz = n;
-            A.zf = &zown;
         }
         int rc = run_real_side(ws, A, units, m, st);
         if (rc) {
