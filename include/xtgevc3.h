@@ -103,7 +103,10 @@ int xtgevc3_ztgevc3_device_ex(char side, char howmny, const int *select, int n, 
  * (n x n column-major matrices at dS + b*n*ldds, dP + b*n*lddp).  alpha_k = S_kk and
  * beta_k = P_kk are differentiated as functions of the diagonals.  Supports real, finite
  * selected eigenvalues only: info = XTGEVC3_INFO_UNSUPPORTED otherwise.  Device arrays,
- * host eigenvalues/select. */
+ * host eigenvalues/select.  info > 0: that many (batch, column) adjoint vectors exceed the
+ * double range (ill-conditioned eigenvectors: the exact gradient overflows); the entries of
+ * dS / dP that depend on them are NaN and every other entry is finite and exact.  The call
+ * synchronises `stream` at its end to read that count. */
 #define XTGEVC3_INFO_UNSUPPORTED (-1001)
 int xtgevc3_dtgevc3_backward_device(int n, const double *S, int lds, const double *P, int ldp, const double *alphar,
                                     const double *alphai, const double *beta, const int *select, const double *V,

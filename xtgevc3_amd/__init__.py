@@ -314,12 +314,14 @@ def ztgsna_e_device(howmny, select, S, P, VL, VR, m=None, s=None, stream=None):
 
 
 def dtgevc3_backward_device(S, P, alphar, alphai, beta, select, V, G, nbatch=1, dS=None, dP=None, stream=None,
-                            precision=PREC_FP64):
+                            precision=PREC_FP64, return_info=False):
     """Backward of ``dtgevc3_device('R', 'S' or 'A', ...)`` for real selected eigenvalues.
     S, P, V, G, dS, dP: torch tensors holding column-major storage (shape (cols, ld)).
     G holds nbatch * m columns.  Returns (dS, dP) with nbatch stacked n x n matrices each
     (tensors of shape (nbatch * n, n) in column-major storage).  precision: PREC_FP64,
-    PREC_SPLIT_I8 or PREC_MIXED_F32 (far updates of the adjoint sweep in that form)."""
+    PREC_SPLIT_I8 or PREC_MIXED_F32 (far updates of the adjoint sweep in that form).  A positive info
+    (adjoints past the double range, their gradient entries NaN) gives a RuntimeWarning, and
+    with return_info=True the result is (dS, dP, info)."""
     import torch
 
     c = ctypes
@@ -341,6 +343,13 @@ def dtgevc3_backward_device(S, P, alphar, alphai, beta, select, V, G, nbatch=1, 
         raise ValueError("xtgevc3: " + last_error())
     if rc != 0:
         raise RuntimeError("xtgevc3 backward failed: " + last_error())
-    if info.value != 0:
+    if info.value < 0:
         raise ValueError(f"xtgevc3 backward: info={info.value} (complex or infinite selected eigenvalue)")
+    if info.value > 0:
+        import warnings
+
+        warnings.warn(f"xtgevc3 backward: {info.value} adjoint vectors exceed the double range; the gradient "
+                      "entries that depend on them are NaN", RuntimeWarning, stacklevel=2)
+    if return_info:
+        return dS, dP, info.value
     return dS, dP

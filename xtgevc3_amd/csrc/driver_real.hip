@@ -61,7 +61,7 @@ __global__ void k_reverse_transpose(const double *, long, int, double *, long);
 __global__ void k_colnorms(const double *, long, const double *, long, int, int, double *);
 __global__ void k_bwd_rhs(const double *, long, const double *, long, int, int, int, const int *, double *, long);
 __global__ void k_bwd_lambda_t(const double *, long, int, int, int, const int *, const double *, const int *, double *,
-                               double *);
+                               double *, int *);
 __global__ void k_to_rowmajor(const double *, long, int, int, double *, long);
 __global__ void k_bwd_diag(const double *, long, const double *, long, int, int, int, const int *, const double *,
                            const int *, double *, long, double *, long);
@@ -336,7 +336,7 @@ struct DevBuf {
 
 struct RealWork {
     DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wblk, wgrp, xrel, xfin, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
-    DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp, bkpre;
+    DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp, bkpre, bovf;
     DevBuf Ut, YB, part, tcol, tw; // eigenvalue condition numbers
     DevBuf Sq, Pq, Wg, wgexp, wbexp, xbexp, sstart, Xq, Zq, xexp, cfar; // split-int8 digits and exponents
     // split-int8 form: side stream (default priority) for the digit splitting of the pencil
@@ -1439,9 +1439,11 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
                            (const double *)nullptr, (const int *)nullptr, (const double *)nullptr, 0, nullptr, nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
         prof_end(pr, st, 3, 2.0 * n * n * (double)m, 8.0 * ((double)n * n + 2.0 * n * m));
     }
+    ENSURE(ws.bovf, sizeof(int) * (size_t)M);
+    hipLaunchKernelGGL(k_zero_int, dim3((M + 255) / 256), dim3(256), 0, st, ws.bovf.as<int>(), (long)M); XTG_LAUNCH_CHECK("k_zero_int");
     hipLaunchKernelGGL(k_bwd_lambda_t, dim3((M + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, st, (const double *)X, ldx,
                        n, m, nbatch, ws.kcol.as<int>(), ws.coef.as<double>(), ws.colexp.as<int>(),
-                       ws.LamS.as<double>(), ws.LamP.as<double>()); XTG_LAUNCH_CHECK("k_bwd_lambda_t");
+                       ws.LamS.as<double>(), ws.LamP.as<double>(), ws.bovf.as<int>()); XTG_LAUNCH_CHECK("k_bwd_lambda_t");
     // dS_b = V * LamS_b, dP_b = V * LamP_b  (as D[j][i] = sum_c V[j][c] Lam[c][i]), all batches
     // in one launch each.  v_c is zero below row kcol[c] and kcol ascends, so with the columns
     // taken in reverse order (negative strides) row j of the product needs only the first
@@ -1474,6 +1476,16 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
                        ldt, n, m, nbatch, ws.kcol.as<int>(), ws.coef.as<double>(), ws.colexp.as<int>(), dS, ldds, dP,
                        lddp); XTG_LAUNCH_CHECK("k_bwd_diag");
     XTG_CHECK(hipGetLastError());
+    // info > 0: the number of (batch, column) adjoints whose lambda exceeds the double range;
+    // the gradient entries that depend on them are NaN, every other entry is finite and exact
+    std::vector<int> ovf(M);
+    XTG_CHECK(hipMemcpyAsync(ovf.data(), ws.bovf.p, sizeof(int) * M, hipMemcpyDeviceToHost, st));
+    XTG_CHECK(hipStreamSynchronize(st));
+    int novf = 0;
+    for (int q = 0; q < M; ++q) {
+        novf += ovf[q] != 0;
+    }
+    *info = novf;
     return 0;
 }
 

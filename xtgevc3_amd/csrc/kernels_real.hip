@@ -16,6 +16,7 @@
 //   k_colnorms      -- 1-norm column sums of S and P over the upper Hessenberg part.
 #include <hip/hip_runtime.h>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <type_traits>
 
@@ -819,6 +820,17 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
     const int wb = (wave & 1) * (GTB / 2);  // wave offset in b
 
     int K = K1 + K2;
+    // TAG 2 with a per-row K bound (the backward's outer products): R entries that are not
+    // finite are staged as zero, and nfk[b] keeps the first K index of a non-finite entry of
+    // column b; an output whose row's exact K range [0, kmax[a]) includes it becomes NaN in
+    // the epilogue.  Outputs whose terms are all finite are then exact, and the zeros of L past
+    // a row's own bound never multiply an Inf of R (the tile's K bound is its rows' maximum).
+    constexpr bool NFG = TAG == 2;
+    __shared__ int nfk[NFG ? GTB : 1];
+    const bool guard = NFG && kmax != nullptr;
+    if (guard && tid < GTB) {
+        nfk[tid] = INT_MAX;
+    }
     if (kmax != nullptr) {
         __shared__ int kred;
         if (tid == 0) {
@@ -879,6 +891,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
     typedef typename std::conditional<WIDE, double2, double>::type vec_t;
     vec_t lreg[NSL], rreg[NSR];
     int valid = 0; // rows of the staged chunk inside their segment
+    int kbase = 0; // K index of the staged chunk's first row
     auto load_chunk = [&](int ch) {
         const bool s1 = ch < nch1;
         const int k0 = (s1 ? ch : ch - nch1) * GK;
@@ -887,6 +900,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
         const double *lb = (s1 ? L : L2) + aa;
         const double *rb = (s1 ? R : R2) + bb;
         valid = rows - k0;
+        kbase = (s1 ? 0 : K1) + k0;
         if (valid >= GK) {
 #pragma clang loop unroll(full)
             for (int s = 0; s < NSL; ++s) {
@@ -940,6 +954,18 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
                 if (!in) {
                     rv = make_double2(0.0, 0.0);
                 }
+                if constexpr (NFG) {
+                    if (guard && !(isfinite(rv.x) && isfinite(rv.y))) {
+                        if (!isfinite(rv.x)) {
+                            atomicMin(&nfk[cr], kbase + r);
+                            rv.x = 0.0;
+                        }
+                        if (!isfinite(rv.y)) {
+                            atomicMin(&nfk[cr + 1], kbase + r);
+                            rv.y = 0.0;
+                        }
+                    }
+                }
                 if constexpr (F32) {
                     *(float2 *)&Rs[buf][r][cr] = make_float2((float)(rv.x * (odd_b ? fb1 : fb0)), (float)(rv.y * fb1));
                 }
@@ -948,7 +974,14 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
                 }
             }
             else {
-                Rs[buf][r][cr] = (el_t)(in ? rreg[s] * fb0 : 0.0);
+                double rv = in ? rreg[s] * fb0 : 0.0;
+                if constexpr (NFG) {
+                    if (guard && !isfinite(rv)) {
+                        atomicMin(&nfk[cr], kbase + r);
+                        rv = 0.0;
+                    }
+                }
+                Rs[buf][r][cr] = (el_t)rv;
             }
         }
     };
@@ -1017,6 +1050,11 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
                     double *cp = C + rowoff + bc;
                     if (mode == 1) {
                         d = fma(sig[bc], *cp, d);
+                    }
+                    if constexpr (NFG) {
+                        if (guard && nfk[bc - b0] < kmax[ar]) {
+                            d = __builtin_nan("");
+                        }
                     }
                     *cp = d;
                 }
@@ -1349,9 +1387,11 @@ __global__ __launch_bounds__(256) void k_bwd_rhs(const double *__restrict__ V, l
 
 // Lam[(b*m + c) * n + i] = coef * 2^-e * lambda_i  for coef = -a_c (S part) and b_c (P part);
 // 32x32 LDS tiles over (working column q, row i).
+// ovf[q] = 1 when lambda of working column q is not representable (2^-e overflows it).
 __global__ void k_bwd_lambda_t(const double *__restrict__ X, long ldx, int n, int m, int nbatch,
                                const int *__restrict__ kcol, const double *__restrict__ coef,
-                               const int *__restrict__ colexp, double *__restrict__ LamS, double *__restrict__ LamP)
+                               const int *__restrict__ colexp, double *__restrict__ LamS, double *__restrict__ LamP,
+                               int *__restrict__ ovf)
 {
     __shared__ double tile[32][33];
     const int q0 = blockIdx.x * 32, i0 = blockIdx.y * 32;
@@ -1364,6 +1404,9 @@ __global__ void k_bwd_lambda_t(const double *__restrict__ X, long ldx, int n, in
             int c = m - 1 - q / nbatch;
             if (i < kcol[c]) {
                 v = ldexp(X[(long)(n - 1 - i) * ldx + q], -colexp[q]);
+                if (!isfinite(v)) {
+                    ovf[q] = 1;
+                }
             }
         }
         tile[kk][tx] = v;
