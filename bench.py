@@ -67,55 +67,16 @@ def make_inputs(n, seed, device, cluster=False):
     return S, P, ar, ai, be, dS, dP, dZ
 
 
-def residual_sample(S, P, ar, ai, be, dZ, dVR, ncols=64, seed=0, back=True):
-    """Metric A residual of a sample of columns, in units of n*eps, computed on the device in
-    FP64: x = Z^T v (back-transformed output) or x = v, r = (beta S - alpha P) x."""
-    import torch
+def residual_every_vector(dS, dP, ar, ai, be, dV, howmny="B", dZ=None):
+    """Metric A residual of EVERY column of a right-eigenvector output, in units of n*eps,
+    computed on the device in FP64 (xtgevc3_amd.residuals): x = Z^T v for back-transformed
+    output, r = (beta S - alpha P) x.  Returns (max, number of columns)."""
+    from xtgevc3_amd.residuals import real_column_kinds, residuals_real
 
-    from oracle import oracle as O
-
-    n = S.shape[0]
-    kind, idx = O.real_layout("B", None, ai)
-    rng = np.random.default_rng(seed)
-    cols = np.unique(np.concatenate([rng.choice(n, ncols, replace=False), [0, n - 1]]))
-    # include both columns of any sampled pair
-    full = set()
-    for c in cols:
-        if kind[c] == 1:
-            full.update([c, c + 1])
-        elif kind[c] == 2:
-            full.update([c - 1, c])
-        else:
-            full.add(c)
-    cols = np.array(sorted(full))
-    V = dVR[cols, :].t()  # n x len(cols), column c = VR[:, c]
-    X = dZ @ V if back else V  # back to the Schur basis: Z^T v (dZ holds Z^T row-major)
-    Sd = torch.from_numpy(np.ascontiguousarray(S)).cuda()
-    Pd = torch.from_numpy(np.ascontiguousarray(P)).cuda()
-    SX = (Sd @ X).cpu().numpy()
-    PX = (Pd @ X).cpu().numpy()
-    Xh = X.cpu().numpy()
-    nS = float(torch.linalg.norm(Sd))
-    nP = float(torch.linalg.norm(Pd))
-    worst = 0.0
-    eps = np.finfo(np.float64).eps
-    for j, c in enumerate(cols):
-        k = idx[c]
-        if kind[c] == 2:
-            continue
-        if kind[c] == 1:
-            jr = j
-            ji = list(cols).index(c + 1)
-            x = Xh[:, jr] + 1j * Xh[:, ji]
-            sx = SX[:, jr] + 1j * SX[:, ji]
-            px = PX[:, jr] + 1j * PX[:, ji]
-            a = ar[k] + 1j * ai[k]
-        else:
-            x, sx, px, a = Xh[:, j], SX[:, j], PX[:, j], ar[k]
-        b = be[k]
-        r = np.linalg.norm(b * sx - a * px) / ((abs(b) * nS + abs(a) * nP) * np.linalg.norm(x))
-        worst = max(worst, r / (n * eps))
-    return worst, len(cols)
+    n = dS.shape[0]
+    kinds, idx = real_column_kinds(ai, howmny=howmny)
+    res = residuals_real(dS, dP, dV, ar, ai, be, kinds, idx, Z_t=dZ if howmny == "B" else None)
+    return float(res.max()) / (n * np.finfo(np.float64).eps), len(kinds)
 
 
 def cpu_baseline(n, seed, budget_s=40.0):
@@ -150,6 +111,244 @@ def cpu_baseline(n, seed, budget_s=40.0):
               f"construction at n={ns}: {ts:.2f} s" + ("" if ns == n else f"; eigvecs/s scaled by (n_s/n)^2 to n={n}"))
     return {"value": value, "unit": "eigvecs/s", "cores": cores, "kind": "reference", "sample": sample,
             "seconds": ts, "n_sample": ns}
+
+
+def _lapack_dtgevc(side, howmny, S, P, VL, VR):
+    """LAPACK dtgevc_ of the image's OpenBLAS (the reference tests' comparison point,
+    tests/test_lapack_d.cpp:293-297) on column-major arrays, in place.  Returns (m, info)."""
+    import ctypes
+
+    from oracle import oracle as O
+
+    n = S.shape[0]
+    c = ctypes
+    p = lambda a: a.ctypes.data_as(c.c_void_p)
+    m, info = c.c_int(0), c.c_int(0)
+    work = np.zeros(6 * max(n, 1))
+    O._openblas().scipy_dtgevc_(c.byref(c.c_char(side.encode())), c.byref(c.c_char(howmny.encode())), None,
+                                c.byref(c.c_int(n)), p(S), c.byref(c.c_int(n)), p(P), c.byref(c.c_int(n)), p(VL),
+                                c.byref(c.c_int(VL.shape[0])), p(VR), c.byref(c.c_int(VR.shape[0])),
+                                c.byref(c.c_int(n)), c.byref(m), p(work), c.byref(info), c.c_size_t(1), c.c_size_t(1))
+    return m.value, info.value
+
+
+def _timed(fn, reps=3):
+    """Median wall time of reps calls (seconds)."""
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return sorted(ts)[len(ts) // 2]
+
+
+def cpu_legs(n, seed):
+    """The CPU comparison points beside the all-cores reference (BASELINE.md plan): the
+    reference library single-threaded and LAPACK dtgevc_ (all cores), side R, howmny B on the
+    same construction at n_s = 2048, eigvecs/s scaled by (n_s / n)^2 to n (the work grows as
+    n^3)."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import oracle as O
+    from xtgevc3_amd import pencils as G
+
+    if not O.have_ref():
+        return None
+    ns = 2048
+    S, P, ar, ai, be = G.real_quasi_triangular(ns, seed)
+    Z = G.random_orthogonal(ns, seed + 1)
+    out = {}
+    with threadpool_limits(1):
+        t1 = _timed(lambda: O.tgevc3("d", "R", "B", None, S, P, ar, be, alphai=ai, VR=Z.copy(order="F"),
+                                     impl="ref"), reps=1)
+    out["reference_1_thread"] = {"value": round(ns / t1 * (ns / n) ** 2, 2), "unit": "eigvecs/s", "cores": 1,
+                                 "seconds_at_n_sample": round(t1, 3), "n_sample": ns}
+    Sf, Pf = np.asfortranarray(S), np.asfortranarray(P)
+    tl = _timed(lambda: _lapack_dtgevc("R", "B", Sf, Pf, np.zeros((1, 1), order="F"), Z.copy(order="F")), reps=1)
+    out["lapack_dtgevc"] = {"value": round(ns / tl * (ns / n) ** 2, 2), "unit": "eigvecs/s",
+                            "cores": len(os.sched_getaffinity(0)), "seconds_at_n_sample": round(tl, 3),
+                            "n_sample": ns}
+    return out
+
+
+def config0_cpu(seed):
+    """configs[0] on the host cores (the reference's own CPU test path): the reference library
+    (all threads and one) and LAPACK dtgevc_ beside it, on the inputs of config0."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import oracle as O
+    from xtgevc3_amd import pencils as G
+
+    if not O.have_ref():
+        return {}
+    S, P, Q, Z, ar, ai, be = G.qz_pencil(512, seed)
+    out = {}
+    run = lambda: O.tgevc3("d", "B", "B", None, S, P, ar, be, alphai=ai, VL=Q.copy(order="F"), VR=Z.copy(order="F"),
+                           impl="ref")
+    run()
+    out["cpu_reference_ms"] = round(1000 * _timed(run), 3)
+    with threadpool_limits(1):
+        out["cpu_reference_1_thread_ms"] = round(1000 * _timed(run), 3)
+    out["cpu_cores"] = len(os.sched_getaffinity(0))
+    Sf, Pf = np.asfortranarray(S), np.asfortranarray(P)
+    out["cpu_lapack_dtgevc_ms"] = round(1000 * _timed(lambda: _lapack_dtgevc("B", "B", Sf, Pf, Q.copy(order="F"),
+                                                                               Z.copy(order="F"))), 3)
+    return out
+
+
+def config0(seed):
+    """dtgevc3 side B, howmny B at n = 512 on the QZ form of an N(0,1) pencil
+    (LAPACK dgges via SciPy), VL = Q, VR = Z: the GPU device entry, with the residual of every
+    vector (both sides); config0_cpu times the host legs on the same inputs."""
+    import torch
+
+    import xtgevc3_amd as X
+    from xtgevc3_amd import pencils as G
+    from xtgevc3_amd.residuals import real_column_kinds, residuals_real
+
+    n = 512
+    S, P, Q, Z, ar, ai, be = G.qz_pencil(n, seed)
+    out = {"workload": f"dtgevc3 side=B howmny=B n={n}, QZ form of N(0,1) A, B (dgges), VL=Q, VR=Z"}
+    dS, dP = X.fortran_device(S), X.fortran_device(P)
+    dQ, dZ = X.fortran_device(Q), X.fortran_device(Z)
+    VL, VR = dQ.clone(), dZ.clone()
+
+    def gpu():
+        VL.copy_(dQ)
+        VR.copy_(dZ)
+        X.dtgevc3_device("B", "B", None, dS, dP, ar, ai, be, VL=VL, VR=VR)
+        torch.cuda.synchronize()
+
+    gpu()
+    t = _timed(gpu, reps=5)
+    out["gpu_ms"] = round(1000 * t, 3)
+    out["gpu_eigvecs_per_s"] = round(2 * n / t, 1)
+    kinds, idx = real_column_kinds(ai, howmny="B")
+    eps = np.finfo(np.float64).eps
+    out["gpu_max_residual_n_eps"] = {
+        side: float("%.4g" % (residuals_real(dS, dP, V, ar, ai, be, kinds, idx, Z_t=W, left=side == "L").max() /
+                              (n * eps))) for side, V, W in (("R", VR, dZ), ("L", VL, dQ))}
+    return out
+
+
+def _orth_device(n, seed, complex_=False):
+    import torch
+
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    A = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    if complex_:
+        A = torch.complex(A, torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g))
+    Qm, R = torch.linalg.qr(A)
+    d = torch.diagonal(R)
+    return (Qm * (d / d.abs())[None, :]).t().contiguous()
+
+
+def _time_device(fn, steps):
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) / steps
+
+
+def config2(seed, steps):
+    """ztgevc3 side B, howmny B at n = 8192 (complex triangular pencil, unitary Q
+    and Z), plain complex128 and the split-int8 form, with the residual of every vector."""
+    import torch
+
+    import xtgevc3_amd as X
+    from xtgevc3_amd import pencils as G
+    from xtgevc3_amd.residuals import residuals_complex
+
+    n = 8192
+    dS, dP, al, be = G.complex_triangular_device(n, seed)
+    dQ, dZ = _orth_device(n, seed + 1, True), _orth_device(n, seed + 2, True)
+    VL, VR = torch.empty_like(dQ), torch.empty_like(dZ)
+    eps = np.finfo(np.float64).eps
+    out = {"workload": f"ztgevc3 side=B howmny=B n={n}, complex triangular pencil (CN(0,1)), unitary Q, Z"}
+    for name, prec in (("plain", X.PREC_FP64), ("split_int8", X.PREC_SPLIT_I8)):
+        def step():
+            VL.copy_(dQ)
+            VR.copy_(dZ)
+            m, info = X.ztgevc3_device("B", "B", None, dS, dP, al, be, VL=VL, VR=VR, precision=prec)
+            assert (m, info) == (n, 0)
+
+        t = _time_device(step, steps)
+        r = {"ms_per_call": round(1000 * t, 3), "eigvecs_per_s": round(2 * n / t, 1)}
+        r["max_residual_n_eps"] = {
+            side: float("%.4g" % (residuals_complex(dS, dP, V, al, be, np.arange(n), Z_t=W, left=side == "L").max()
+                                  / (n * eps))) for side, V, W in (("R", VR, dZ), ("L", VL, dQ))}
+        out[name] = r
+    out["split_speedup"] = round(out["plain"]["ms_per_call"] / out["split_int8"]["ms_per_call"], 3)
+    del dS, dP, dQ, dZ, VL, VR
+    torch.cuda.empty_cache()
+    return out
+
+
+def config3(seed, steps):
+    """dtgevc3 side R at n = 16384, eigenvalues clustered in 64 groups (relative
+    spread 1e-6), howmny A and B, plain FP64 and split-int8 on the same inputs, plus the FP32
+    interface stgevc3 (host pointers, PCIe staging included) on the FP32-rounded pencil; the
+    residual of every vector for each."""
+    import torch
+
+    import xtgevc3_amd as X
+    from xtgevc3_amd import pencils as G
+    from xtgevc3_amd.residuals import real_column_kinds, residuals_real
+
+    n = 16384
+    dS, dP, ar, ai, be = G.real_quasi_triangular_device(n, seed, cluster_groups=64)
+    dZ = _orth_device(n, seed + 3)
+    V = torch.empty_like(dZ)
+    eps = np.finfo(np.float64).eps
+    out = {"workload": f"dtgevc3 side=R n={n}, quasi-triangular pencil (40% pairs), 64 eigenvalue clusters "
+                       f"(spread 1e-6), Z orthogonal"}
+    for howmny in ("A", "B"):
+        kinds, idx = real_column_kinds(ai, howmny=howmny)
+        for name, prec in (("plain", X.PREC_FP64), ("split_int8", X.PREC_SPLIT_I8)):
+            def step():
+                if howmny == "B":
+                    V.copy_(dZ)
+                m, info = X.dtgevc3_device("R", howmny, None, dS, dP, ar, ai, be, VR=V, precision=prec)
+                assert (m, info) == (n, 0)
+
+            t = _time_device(step, steps)
+            res = residuals_real(dS, dP, V, ar, ai, be, kinds, idx, Z_t=dZ if howmny == "B" else None)
+            out[f"R_{howmny}_{name}"] = {"ms_per_call": round(1000 * t, 3), "eigvecs_per_s": round(n / t, 1),
+                                         "max_residual_n_eps": float("%.4g" % (res.max() / (n * eps)))}
+        out[f"R_{howmny}_split_speedup"] = round(out[f"R_{howmny}_plain"]["ms_per_call"] /
+                                                 out[f"R_{howmny}_split_int8"]["ms_per_call"], 3)
+    # FP32 interface through the host-pointer drop-in
+    f = np.float32
+    S32 = np.asfortranarray(dS.to(torch.float32).t().cpu().numpy())
+    P32 = np.asfortranarray(dP.to(torch.float32).t().cpu().numpy())
+    Z32d = dZ.to(torch.float32)
+    Z32 = np.asfortranarray(Z32d.t().cpu().numpy())
+    ar32, ai32, be32 = ar.astype(f), ai.astype(f), be.astype(f)
+    VRh = Z32.copy(order="F")
+
+    def host():
+        VRh[...] = Z32
+        m, info, _ = X.stgevc3("R", "B", None, S32, P32, ar32, ai32, be32, VR=VRh)
+        assert (m, info) == (n, 0)
+
+    host()
+    t = _timed(host, reps=3)
+    kinds, idx = real_column_kinds(ai32, howmny="B")
+    res = residuals_real(X.fortran_device(S32.astype(np.float64)), X.fortran_device(P32.astype(np.float64)),
+                         X.fortran_device(VRh.astype(np.float64)), ar32.astype(np.float64), ai32.astype(np.float64),
+                         be32.astype(np.float64), kinds, idx, Z_t=Z32d.to(torch.float64))
+    e32 = np.finfo(f).eps
+    out["stgevc3_R_B_host_api"] = {"ms_per_call": round(1000 * t, 2), "eigvecs_per_s": round(n / t, 1),
+                                   "max_residual_n_eps32": float("%.4g" % (res.max() / (n * e32))),
+                                   "note": "FP32 host arrays, PCIe staging included (median of 3 calls)"}
+    del dS, dP, dZ, V
+    torch.cuda.empty_cache()
+    return out
 
 
 def backward_ratio(seed, n=2048, nsel=256, nbatch=32, reps=5):
@@ -274,6 +473,8 @@ def main():
     ap.add_argument("--no-backward", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-host-api", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the other configurations (n=512 QZ on CPU and GPU, ztgevc3 n=8192, n=16384 clustered)")
     ap.add_argument("--cluster", action="store_true",
                     help="eigenvalues clustered in 64 groups (relative spread 1e-6), as the n=16384 configuration")
     ap.add_argument("--howmny", default="B", choices=["A", "B"],
@@ -373,8 +574,8 @@ def main():
     if not args.no_residual:
         step()
         torch.cuda.synchronize()
-        res, ncols = residual_sample(S, P, ar, ai, be, dZ, dVR, back=back)
-        log(f"[bench] residual (metric A) over {ncols} sampled columns: {res:.3e} n*eps")
+        res, ncols = residual_every_vector(dS, dP, ar, ai, be, dVR, args.howmny, dZ)
+        log(f"[bench] residual (metric A) of every one of the {ncols} vectors: max {res:.3e} n*eps")
 
     # mixed forms on the same inputs: split-int8 digit products (FP64-level residual) and FP32
     # products; each timed over the same K steps, then one profiled step and one residual step
@@ -425,7 +626,7 @@ def main():
             ref64 = dVR.clone()
             step_form()
             torch.cuda.synchronize()
-            rm, _ = residual_sample(S, P, ar, ai, be, dZ, dVR, back=back)
+            rm, _ = residual_every_vector(dS, dP, ar, ai, be, dVR, args.howmny, dZ)
             diff = float((dVR - ref64).abs().max())
             out["max_residual_n_eps"] = float(f"{rm:.4g}")
             out["max_abs_diff_vs_fp64"] = float(f"{diff:.4g}")
@@ -487,6 +688,21 @@ def main():
         cpu = cpu_baseline(n, args.seed)
         if cpu:
             log(f"[bench] cpu baseline {cpu['value']:.1f} eigvecs/s ({cpu['sample']})")
+            legs = cpu_legs(n, args.seed)
+            if legs:
+                legs["cfg0_dtgevc3_B_B_n512_qz"] = config0_cpu(args.seed)
+                cpu["legs"] = legs
+                log(f"[bench] cpu legs {legs}")
+
+    extra = None
+    if not args.no_extra:
+        extra = {}
+        for key, fn in (("cfg0_dtgevc3_B_B_n512_qz", lambda: config0(args.seed)),
+                        ("cfg2_ztgevc3_B_B_n8192", lambda: config2(args.seed, max(2, min(args.steps, 3)))),
+                        ("cfg3_dtgevc3_R_n16384_clustered", lambda: config3(args.seed, max(2, min(args.steps, 3))))):
+            t0 = time.perf_counter()
+            extra[key] = fn()
+            log(f"[bench] {key} ({time.perf_counter() - t0:.1f} s): {json.dumps(extra[key])}")
 
     out = {
         "metric": "eigvecs/s at n=8192: mixed vs plain FP64 GPU at fixed residual; bwd/fwd ratio",
@@ -506,14 +722,17 @@ def main():
                                + (", Z orthogonal" if back else ""), "n": n, "side": "R",
                    "howmny": args.howmny,
                    "form": "plain-fp64", "path_tflops": round(path_tflops, 3),
-                   "max_residual_n_eps": None if res is None else float(f"{res:.4g}")},
+                   "max_residual_n_eps": None if res is None else float(f"{res:.4g}"),
+                   "residual_columns": "every vector"},
         "roofline": roof,
         "mixed": mixed,
         "mixed_f32": mixed_f32,
         "host_api": host,
         "backward": bw,
         "kernels": kernels,
-        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample",
+                                                                      "legs") if k in cpu},
+        "other_configs": extra,
     }
     print(json.dumps(out), flush=True)
 

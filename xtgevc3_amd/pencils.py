@@ -194,3 +194,85 @@ def qz_pencil(n, seed, complex_=False):
     S, P, alpha, beta, Q, Z = sl.ordqz(A, B, sort="lhp", output="real")
     return (np.asfortranarray(S), np.asfortranarray(P), np.asfortranarray(Q), np.asfortranarray(Z),
             alpha.real.copy(), alpha.imag.copy(), beta.real.copy())
+
+
+def _diagonal_structure(n, rng, pair_frac, cluster_groups, cluster_spread):
+    """The diagonal of ``real_quasi_triangular``: 1x1 entries and 2x2 pair blocks, with their
+    eigenvalues.  Returns (entries, ar, ai, be): entries is a list of (row, col, s, p) for S and
+    P inside the diagonal blocks."""
+    ar, ai, be = np.zeros(n), np.zeros(n), np.zeros(n)
+    want = pair_frac / (2.0 - pair_frac)
+    coin = rng.random(n)
+    centres = rng.standard_normal(cluster_groups) if cluster_groups else None
+    rows, cols, sv, pv = [], [], [], []
+    k = 0
+    while k < n:
+        if k + 1 < n and coin[k] < want:
+            s11, s12, s21, s22, p1, p2 = _pair_block(rng)
+            rows += [k, k, k + 1, k + 1]
+            cols += [k, k + 1, k, k + 1]
+            sv += [s11, s12, s21, s22]
+            pv += [p1, 0.0, 0.0, p2]
+            a_r, a_i, b_ = pair_eigenvalue(s11, s12, s21, s22, p1, p2)
+            ar[k] = ar[k + 1] = a_r
+            ai[k], ai[k + 1] = a_i, -a_i
+            be[k] = be[k + 1] = b_
+            k += 2
+        else:
+            p = abs(rng.standard_normal()) + 0.1
+            if centres is not None:
+                s = p * centres[rng.integers(cluster_groups)] * (1.0 + cluster_spread * rng.standard_normal())
+            else:
+                s = rng.standard_normal()
+            rows.append(k)
+            cols.append(k)
+            sv.append(s)
+            pv.append(p)
+            ar[k], be[k] = s, p
+            k += 1
+    return (np.array(rows), np.array(cols), np.array(sv), np.array(pv)), ar, ai, be
+
+
+def real_quasi_triangular_device(n, seed, pair_frac=0.4, cluster_groups=0, cluster_spread=1e-6, device="cuda"):
+    """The construction of ``real_quasi_triangular`` (without grading) with the off-diagonal
+    entries drawn on the device (torch generator), for large n.  Returns (S_t, P_t, ar, ai, be):
+    S_t, P_t are device tensors holding the column-major storage of S and P (S = S_t.T), the
+    eigenvalues host arrays.  Same distribution as the host generator, other random numbers."""
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    # column-major storage of an upper-triangular matrix = row-major lower-triangular tensor
+    S_t = torch.tril(torch.randn(n, n, dtype=torch.float64, device=device, generator=g), -1)
+    P_t = torch.tril(torch.randn(n, n, dtype=torch.float64, device=device, generator=g), -1)
+    (r, c, sv, pv), ar, ai, be = _diagonal_structure(n, np.random.default_rng(seed), pair_frac, cluster_groups,
+                                                     cluster_spread)
+    ri = torch.as_tensor(r, device=device)
+    ci = torch.as_tensor(c, device=device)
+    S_t[ci, ri] = torch.as_tensor(sv, device=device)
+    P_t[ci, ri] = torch.as_tensor(pv, device=device)
+    return S_t, P_t, ar, ai, be
+
+
+def complex_triangular_device(n, seed, device="cuda"):
+    """``complex_triangular`` with the entries drawn on the device: (S_t, P_t, alpha, beta),
+    S_t and P_t device tensors holding column-major storage, alpha and beta host arrays."""
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+
+    def cn():
+        re = torch.randn(n, n, dtype=torch.float64, device=device, generator=g)
+        im = torch.randn(n, n, dtype=torch.float64, device=device, generator=g)
+        return torch.complex(re, im) / np.sqrt(2.0)
+
+    S_t = torch.tril(cn(), -1)
+    P_t = torch.tril(cn(), -1)
+    rng = np.random.default_rng(seed)
+    ds = (rng.standard_normal(n) + 1j * rng.standard_normal(n)) / np.sqrt(2.0)
+    dp = np.abs(rng.standard_normal(n)) + 0.1
+    d = torch.arange(n, device=device)
+    S_t[d, d] = torch.as_tensor(ds, device=device)
+    P_t[d, d] = torch.as_tensor(dp.astype(np.complex128), device=device)
+    return S_t, P_t, ds, dp.astype(np.complex128)
