@@ -495,20 +495,40 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
             }
         }
         if (grp.last && !starts_here) {
+            // lower blocks' operand rows to the final scale: all loads of a block are issued
+            // before its stores (a load-store chain per element serialised the round trips)
+            constexpr int KR = (MAXB + QL - 1) / QL;
 #pragma clang loop unroll(full)
             for (int jb = 1; jb < 4; ++jb) {
                 const int k0 = grp.off[jb], k1 = grp.off[jb + 1];
                 if (k0 < k1) {
                     const int e = ce - blkexp[(long)(grp.step[jb] - grp.step[0]) * ldx + col];
                     if (e != 0) {
-                        for (int k = k0 + j; k < k1; k += QL) {
-                            double *w0 = W + (long)k * ldw + col;
-                            double *w1 = W + (long)(grp.wr + k) * ldw + col;
-                            w0[0] = ldexp(w0[0], e);
-                            w1[0] = ldexp(w1[0], e);
-                            if (pair) {
-                                w0[1] = ldexp(w0[1], e);
-                                w1[1] = ldexp(w1[1], e);
+                        double v[KR][4];
+#pragma clang loop unroll(full)
+                        for (int t = 0; t < KR; ++t) {
+                            const int k = k0 + j + QL * t;
+                            if (k < k1) {
+                                const double *w0 = W + (long)k * ldw + col;
+                                const double *w1 = W + (long)(grp.wr + k) * ldw + col;
+                                v[t][0] = w0[0];
+                                v[t][1] = w1[0];
+                                v[t][2] = pair ? w0[1] : 0.0;
+                                v[t][3] = pair ? w1[1] : 0.0;
+                            }
+                        }
+#pragma clang loop unroll(full)
+                        for (int t = 0; t < KR; ++t) {
+                            const int k = k0 + j + QL * t;
+                            if (k < k1) {
+                                double *w0 = W + (long)k * ldw + col;
+                                double *w1 = W + (long)(grp.wr + k) * ldw + col;
+                                w0[0] = ldexp(v[t][0], e);
+                                w1[0] = ldexp(v[t][1], e);
+                                if (pair) {
+                                    w0[1] = ldexp(v[t][2], e);
+                                    w1[1] = ldexp(v[t][3], e);
+                                }
                             }
                         }
                     }
