@@ -66,6 +66,8 @@ __global__ void k_to_rowmajor(const double *, long, int, int, double *, long);
 __global__ void k_bwd_diag(const double *, long, const double *, long, int, int, int, const int *, const double *,
                            const int *, double *, long, double *, long);
 __global__ void k_subdiag(const double *, long, int, double *);
+__global__ void k_bwd_outer(int, int, int, const double *, long, const double *, long, const double *, const double *,
+                            double *, long, double *, long, const int *, long, long, long);
 __global__ void k_zero(double *, long);
 __global__ void k_zero_int(int *, long);
 
@@ -336,7 +338,7 @@ struct DevBuf {
 
 struct RealWork {
     DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wblk, wgrp, xrel, xfin, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
-    DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp, bkpre, bovf;
+    DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp, bkpre, bovf, sfac;
     DevBuf Ut, YB, part, tcol, tw; // eigenvalue condition numbers
     DevBuf Sq, Pq, Wg, wgexp, wbexp, xbexp, sstart, Xq, Zq, xexp, cfar; // split-int8 digits and exponents
     // split-int8 form: side stream (default priority) for the digit splitting of the pencil
@@ -1393,7 +1395,6 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
     ENSURE(ws.Vt, sizeof(double) * (size_t)n * ldt);
     ENSURE(ws.Y, sizeof(double) * (size_t)n * ldt);
     ENSURE(ws.LamS, sizeof(double) * (size_t)M * n);
-    ENSURE(ws.LamP, sizeof(double) * (size_t)M * n);
     XTG_CHECK(hipMemcpyAsync(ws.kcol.p, kcol.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
     XTG_CHECK(hipMemcpyAsync(ws.coef.p, coef.data(), sizeof(double) * 4 * m, hipMemcpyHostToDevice, st));
     dim3 g32((n + 31) / 32, (n + 31) / 32);
@@ -1443,11 +1444,11 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
     hipLaunchKernelGGL(k_zero_int, dim3((M + 255) / 256), dim3(256), 0, st, ws.bovf.as<int>(), (long)M); XTG_LAUNCH_CHECK("k_zero_int");
     hipLaunchKernelGGL(k_bwd_lambda_t, dim3((M + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, st, (const double *)X, ldx,
                        n, m, nbatch, ws.kcol.as<int>(), ws.coef.as<double>(), ws.colexp.as<int>(),
-                       ws.LamS.as<double>(), ws.LamP.as<double>(), ws.bovf.as<int>()); XTG_LAUNCH_CHECK("k_bwd_lambda_t");
-    // dS_b = V * LamS_b, dP_b = V * LamP_b  (as D[j][i] = sum_c V[j][c] Lam[c][i]), all batches
-    // in one launch each.  v_c is zero below row kcol[c] and kcol ascends, so with the columns
-    // taken in reverse order (negative strides) row j of the product needs only the first
-    // kpre[j] = #{c : kcol[c] >= j} of them: a per-row K bound.
+                       ws.LamS.as<double>(), (double *)nullptr, ws.bovf.as<int>()); XTG_LAUNCH_CHECK("k_bwd_lambda_t");
+    // dS_b = V diag(-a) Lam_b, dP_b = V diag(b) Lam_b  (as D[j][i] = sum_c s_c V[j][c] Lam[c][i]),
+    // all batches and both products in one launch.  v_c is zero below row kcol[c] and kcol
+    // ascends, so with the columns taken in reverse order (negative strides) row j of the
+    // product needs only the first kpre[j] = #{c : kcol[c] >= j} of them: a per-row K bound.
     std::vector<int> kpre(n);
     double kwork = 0.0;
     for (int j = 0, c = 0; j < n; ++j) {
@@ -1460,17 +1461,34 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
     ENSURE(ws.bkpre, sizeof(int) * (size_t)n);
     XTG_CHECK(hipMemcpyAsync(ws.bkpre.p, kpre.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     {
+        // dS_b = V diag(-a) Lam_b and dP_b = V diag(b) Lam_b in one pass over V and Lam_b (the
+        // columns in reverse order, so that row j needs only the first kpre[j] of them)
+        std::vector<double> sf(2 * (size_t)m);
+        for (int k = 0; k < m; ++k) {
+            const int c = m - 1 - k;
+            sf[k] = -coef[4 * c + 0];
+            sf[m + k] = coef[4 * c + 1];
+        }
+        ENSURE(ws.sfac, sizeof(double) * 2 * (size_t)m);
+        XTG_CHECK(hipMemcpyAsync(ws.sfac.p, sf.data(), sizeof(double) * 2 * m, hipMemcpyHostToDevice, st));
         const double *Vrev = V + (long)(m - 1) * ldv;
         const long lrev = (long)(m - 1) * n;
         pr = prof_begin(st);
-        launch_gemm_f64(2, st, n, n, m, 0, Vrev, -ldv, (const double *)nullptr, 0L, ws.LamS.as<double>() + lrev,
-                        -(long)n, dS, ldds, (const int *)nullptr, 0, (const double *)nullptr, ws.bkpre.as<int>(),
-                        (const double *)nullptr, 0, nullptr, nullptr, nbatch, (long)m * n, (long)n * ldds); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
-        launch_gemm_f64(2, st, n, n, m, 0, Vrev, -ldv, (const double *)nullptr, 0L, ws.LamP.as<double>() + lrev,
-                        -(long)n, dP, lddp, (const int *)nullptr, 0, (const double *)nullptr, ws.bkpre.as<int>(),
-                        (const double *)nullptr, 0, nullptr, nullptr, nbatch, (long)m * n, (long)n * lddp); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+        if (n < 2) {
+            // lambda has no rows (k = 0): the products vanish, only the diagonal terms remain
+            XTG_CHECK(hipMemset2DAsync(dS, sizeof(double) * ldds, 0, sizeof(double) * n, (size_t)nbatch * n, st));
+            XTG_CHECK(hipMemset2DAsync(dP, sizeof(double) * lddp, 0, sizeof(double) * n, (size_t)nbatch * n, st));
+        }
+        else {
+            const dim3 grid(((n + 63) / 64) * ((n + 127) / 128), nbatch);
+            hipLaunchKernelGGL(k_bwd_outer, grid, dim3(256), 0, st, n, n, m, Vrev, -ldv,
+                               (const double *)(ws.LamS.as<double>() + lrev), -(long)n,
+                               (const double *)ws.sfac.as<double>(), (const double *)(ws.sfac.as<double>() + m), dS,
+                               ldds, dP, lddp, (const int *)ws.bkpre.as<int>(), (long)m * n, (long)n * ldds,
+                               (long)n * lddp); XTG_LAUNCH_CHECK("k_bwd_outer");
+        }
         prof_end(pr, st, 3, 4.0 * n * kwork * nbatch,
-                 8.0 * 2.0 * nbatch * ((double)n * m + (double)m * n + (double)n * n));
+                 8.0 * nbatch * ((double)n * m + (double)m * n + 2.0 * (double)n * n));
     }
     hipLaunchKernelGGL(k_bwd_diag, dim3(M), dim3(64), 0, st, (const double *)X, ldx, (const double *)ws.Y.as<double>(),
                        ldt, n, m, nbatch, ws.kcol.as<int>(), ws.coef.as<double>(), ws.colexp.as<int>(), dS, ldds, dP,

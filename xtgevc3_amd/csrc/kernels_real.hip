@@ -1083,6 +1083,187 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// k_bwd_outer: both gradient products of the backward in one pass over their shared operands,
+//     D1[a][b] = sum_k s1[k] L[k][a] R[k][b],   D2[a][b] = sum_k s2[k] L[k][a] R[k][b],
+// written to C1 + a ldc1 + b and C2 + a ldc2 + b (L[k][a] = L[a + k ldl], R[k][b] = R[k ldr + b];
+// strides may be negative).  With L = the forward vectors V, R = the adjoint solutions lambda
+// and s1 = -a_c, s2 = b_c (columns in reverse order) these are dL/dS and dL/dP.  kmax[a]
+// bounds k per row a (exact per row: the tile runs to its rows' maximum, and a row's terms past
+// its own bound are exact zeros of L).  Non-finite R entries are staged as zero and an output
+// whose row range includes one becomes NaN, as in k_gemm_f64 (TAG 2).  Batches: gridDim.y,
+// R + y bsr, C1 + y bsc1, C2 + y bsc2.  Tile 128 x 64, K chunks of 16 double-buffered in LDS,
+// four waves of 64 x 32; the factors s1, s2 multiply the A fragment in registers.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bwd_outer(int M, int N, int K, const double *__restrict__ L, long ldl,
+                                                   const double *__restrict__ R, long ldr,
+                                                   const double *__restrict__ s1, const double *__restrict__ s2,
+                                                   double *__restrict__ C1, long ldc1, double *__restrict__ C2,
+                                                   long ldc2, const int *__restrict__ kmax, long bsr, long bsc1,
+                                                   long bsc2)
+{
+    constexpr int GTA = 128, GTB = 64, GK = 16;
+    constexpr int FTA = GTA / 32, FTB = GTB / 32;
+    R += (long)blockIdx.y * bsr;
+    C1 += (long)blockIdx.y * bsc1;
+    C2 += (long)blockIdx.y * bsc2;
+    __shared__ double Ls[2][GK][GTA];
+    __shared__ double Rs[2][GK][GTB];
+    __shared__ double Ss[2][2][GK];
+    __shared__ int nfk[GTB];
+    __shared__ int kred;
+    int tm, tn;
+    xcd_tile(blockIdx.x, (M + GTA - 1) / GTA, (N + GTB - 1) / GTB, tm, tn);
+    const int a0 = tm * GTA, b0 = tn * GTB;
+    const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+    const int wa = (wave >> 1) * (GTA / 2), wb = (wave & 1) * (GTB / 2);
+    if (tid == 0) {
+        kred = 0;
+    }
+    if (tid < GTB) {
+        nfk[tid] = INT_MAX;
+    }
+    __syncthreads();
+    if (tid < GTA && a0 + tid < M) {
+        atomicMax(&kred, kmax[a0 + tid]);
+    }
+    __syncthreads();
+    const int KK = min(K, kred);
+    const int nchunks = (KK + GK - 1) / GK;
+
+    double4_t acc1[FTA][FTB], acc2[FTA][FTB];
+#pragma clang loop unroll(full)
+    for (int i = 0; i < FTA; ++i) {
+#pragma clang loop unroll(full)
+        for (int j = 0; j < FTB; ++j) {
+#pragma clang loop unroll(full)
+            for (int v = 0; v < 4; ++v) {
+                acc1[i][j][v] = 0.0;
+                acc2[i][j][v] = 0.0;
+            }
+        }
+    }
+    // staging: L column pair cl, rows lkl + 4 s (s < 4); R column pair cr, rows lkr + 8 s (s < 2)
+    constexpr int RSL = 256 / (GTA / 2), NSL = GK / RSL, RSR = 256 / (GTB / 2), NSR = GK / RSR;
+    const int cl = 2 * (tid % (GTA / 2)), lkl = tid / (GTA / 2);
+    const int cr = 2 * (tid % (GTB / 2)), lkr = tid / (GTB / 2);
+    const int aa = min(a0 + cl, M - 2), bb = min(b0 + cr, N - 2);
+    const bool odd_a = a0 + cl == M - 1, odd_b = b0 + cr == N - 1;
+    double2 lreg[NSL], rreg[NSR];
+    double sreg = 0.0;
+    int kbase = 0, valid = 0;
+    auto load_chunk = [&](int ch) {
+        const int k0 = ch * GK;
+        kbase = k0;
+        valid = KK - k0;
+#pragma clang loop unroll(full)
+        for (int q = 0; q < NSL; ++q) {
+            const int k = min(k0 + lkl + RSL * q, KK - 1);
+            const double *lp = L + aa + (long)k * ldl;
+            lreg[q] = make_double2(lp[0], lp[1]);
+        }
+#pragma clang loop unroll(full)
+        for (int q = 0; q < NSR; ++q) {
+            const int k = min(k0 + lkr + RSR * q, KK - 1);
+            rreg[q] = *(const double2 *)(R + (long)k * ldr + bb);
+        }
+        if (tid < 2 * GK) {
+            const int k = min(k0 + (tid % GK), KK - 1);
+            sreg = tid < GK ? s1[k] : s2[k];
+        }
+    };
+    auto store_chunk = [&](int buf) {
+#pragma clang loop unroll(full)
+        for (int q = 0; q < NSL; ++q) {
+            const int r = lkl + RSL * q;
+            double2 lv = lreg[q];
+            lv.x = odd_a ? lv.y : lv.x;
+            if (!(r < valid)) {
+                lv = make_double2(0.0, 0.0);
+            }
+            *(double2 *)&Ls[buf][r][cl] = lv;
+        }
+#pragma clang loop unroll(full)
+        for (int q = 0; q < NSR; ++q) {
+            const int r = lkr + RSR * q;
+            double2 rv = rreg[q];
+            rv.x = odd_b ? rv.y : rv.x;
+            if (!(r < valid)) {
+                rv = make_double2(0.0, 0.0);
+            }
+            if (!(isfinite(rv.x) && isfinite(rv.y))) {
+                if (!isfinite(rv.x)) {
+                    atomicMin(&nfk[cr], kbase + r);
+                    rv.x = 0.0;
+                }
+                if (!isfinite(rv.y)) {
+                    atomicMin(&nfk[cr + 1], kbase + r);
+                    rv.y = 0.0;
+                }
+            }
+            *(double2 *)&Rs[buf][r][cr] = rv;
+        }
+        if (tid < 2 * GK) {
+            Ss[buf][tid / GK][tid % GK] = (tid % GK) < valid ? sreg : 0.0;
+        }
+    };
+    if (nchunks > 0) {
+        load_chunk(0);
+        store_chunk(0);
+    }
+    __syncthreads();
+    const int fr = lane % 16, fk = lane / 16;
+    for (int ch = 0; ch < nchunks; ++ch) {
+        const int buf = ch & 1;
+        if (ch + 1 < nchunks) {
+            load_chunk(ch + 1);
+        }
+#pragma clang loop unroll(full)
+        for (int ks = 0; ks < GK; ks += 4) {
+            const double f1 = Ss[buf][0][ks + fk], f2 = Ss[buf][1][ks + fk];
+            double bf[FTB];
+#pragma clang loop unroll(full)
+            for (int j = 0; j < FTB; ++j) {
+                bf[j] = Rs[buf][ks + fk][wb + j * 16 + fr];
+            }
+#pragma clang loop unroll(full)
+            for (int i = 0; i < FTA; ++i) {
+                const double af = Ls[buf][ks + fk][wa + i * 16 + fr];
+                const double a1 = af * f1, a2 = af * f2;
+#pragma clang loop unroll(full)
+                for (int j = 0; j < FTB; ++j) {
+                    acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bf[j], acc1[i][j], 0, 0, 0);
+                    acc2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bf[j], acc2[i][j], 0, 0, 0);
+                }
+            }
+        }
+        if (ch + 1 < nchunks) {
+            store_chunk(buf ^ 1);
+        }
+        __syncthreads();
+    }
+#pragma clang loop unroll(full)
+    for (int i = 0; i < FTA; ++i) {
+#pragma clang loop unroll(full)
+        for (int vv = 0; vv < 4; ++vv) {
+            const int ar = a0 + wa + i * 16 + fk + 4 * vv;
+            if (ar >= M) {
+                continue;
+            }
+            const int kr = kmax[ar];
+#pragma clang loop unroll(full)
+            for (int j = 0; j < FTB; ++j) {
+                const int bc = b0 + wb + j * 16 + fr;
+                if (bc < N) {
+                    const bool bad = nfk[bc - b0] < kr;
+                    C1[(long)ar * ldc1 + bc] = bad ? __builtin_nan("") : acc1[i][j][vv];
+                    C2[(long)ar * ldc2 + bc] = bad ? __builtin_nan("") : acc2[i][j][vv];
+                }
+            }
+        }
+    }
+}
+
 // out[r] = max(out[r], e) with e the exponent bounding max_k max(|A[r + k lda]|, |B[r + k ldb]|)
 // (B may be null) over the column chunk [64 blockIdx.y, +64) of k < ncols, and k >= cfirst[r]
 // when cfirst is given, clamped to [-1000, 1000]; out is filled with -1000 beforehand.
@@ -1405,8 +1586,9 @@ __global__ __launch_bounds__(256) void k_bwd_rhs(const double *__restrict__ V, l
     }
 }
 
-// Lam[(b*m + c) * n + i] = coef * 2^-e * lambda_i  for coef = -a_c (S part) and b_c (P part);
-// 32x32 LDS tiles over (working column q, row i).
+// Lam[(b*m + c) * n + i] = coef * 2^-e * lambda_i  for coef = -a_c (S part) and b_c (P part)
+// (LamP == nullptr: LamS receives 2^-e * lambda_i itself); 32x32 LDS tiles over (working column
+// q, row i).
 // ovf[q] = 1 when lambda of working column q is not representable (2^-e overflows it).
 __global__ void k_bwd_lambda_t(const double *__restrict__ X, long ldx, int n, int m, int nbatch,
                                const int *__restrict__ kcol, const double *__restrict__ coef,
@@ -1438,8 +1620,13 @@ __global__ void k_bwd_lambda_t(const double *__restrict__ X, long ldx, int n, in
             int c = m - 1 - q / nbatch, b = q % nbatch;
             double v = tile[tx][kk];
             long o = (long)(b * m + c) * n + i;
-            LamS[o] = -coef[4 * c + 0] * v;
-            LamP[o] = coef[4 * c + 1] * v;
+            if (LamP == nullptr) {
+                LamS[o] = v; // lambda itself (the fused outer products apply -a_c, b_c)
+            }
+            else {
+                LamS[o] = -coef[4 * c + 0] * v;
+                LamP[o] = coef[4 * c + 1] * v;
+            }
         }
     }
 }
