@@ -141,6 +141,21 @@ int xtgevc3_dtgevc3_device_noalloc(char side, char howmny, const int *select, in
                                    const double *beta, double *VL, int ldvl, double *VR, int ldvr, int mm, int *m,
                                    int *info, void *stream);
 
+/* Backward for either side and through the back-transform (no counterpart in the reference: the
+ * adjoint of the left sweep, src/dtgevc3.cpp:1476-1663, and of the back-transform VR := Z X,
+ * VL := Q Y, :1443-1454 and :1635-1657).  side 'R': V holds normalised right eigenvectors of
+ * the selected real eigenvalues; side 'L': normalised left eigenvectors (y^H (beta S - alpha P) =
+ * 0).  Zb NULL: V as xtgevc3_dtgevc3_device(side, howmny = select ? 'S' : 'A') returns it; Zb given:
+ * V = the back-transformed vectors of those eigenvalues (Zb = Z for side R, Q for side L; the
+ * columns howmny = 'B' returns for them), and dZ (optional, n x n per batch at dZ + b*n*lddz)
+ * receives dL/dZb.  G, nbatch, dS, dP, info as in xtgevc3_dtgevc3_backward_device (FP64).
+ * Returns 0, -1 for a bad side, or XTGEVC3_INFO_DEVICE_ERROR. */
+int xtgevc3_dtgevc3_backward_full_device(char side, int n, const double *S, int lds, const double *P, int ldp,
+                                         const double *alphar, const double *alphai, const double *beta,
+                                         const int *select, const double *Zb, int ldz, const double *V, int ldv,
+                                         const double *G, int ldg, int nbatch, double *dS, int ldds, double *dP,
+                                         int lddp, double *dZ, int lddz, int *info, void *stream);
+
 /* Eigenvalue condition numbers (no counterpart in the reference; the semantics of LAPACK
  * dtgsna with JOB = 'E'): for the eigenvalues chosen by howmny ('A' all, 'S' by select, with
  * dtgevc3's selection rule, complex pairs from alphai) and their left and right eigenvectors

@@ -47,7 +47,7 @@ EXPORTED_SYMBOLS = [
     "xtgevc3_dtgevc3_device", "xtgevc3_dtgevc3_device_ex", "xtgevc3_ztgevc3_device", "xtgevc3_ztgevc3_device_ex",
     "xtgevc3_profile_enable", "xtgevc3_profile_read", "xtgevc3_dtgevc3_backward_device",
     "xtgevc3_dtgevc3_backward_device_ex", "xtgevc3_dtgsna_e_device", "xtgevc3_ztgsna_e_device",
-    "xtgevc3_dtgevc3_device_noalloc",
+    "xtgevc3_dtgevc3_device_noalloc", "xtgevc3_dtgevc3_backward_full_device",
 ]
 
 NOALLOC_MAX_N = 512
@@ -353,3 +353,36 @@ def dtgevc3_backward_device(S, P, alphar, alphai, beta, select, V, G, nbatch=1, 
     if return_info:
         return dS, dP, info.value
     return dS, dP
+
+
+def dtgevc3_backward_full_device(side, S, P, alphar, alphai, beta, select, V, G, nbatch=1, Z=None, dZ=None,
+                                 stream=None, return_info=False):
+    """Backward for side 'R' or 'L' (left eigenvectors), optionally through the back-transform:
+    with Z (Z for right, Q for left vectors) V holds the back-transformed vectors and dZ
+    (allocated here when not given) receives dL/dZ.  Tensors hold column-major storage as in
+    dtgevc3_backward_device.  Returns (dS, dP) or (dS, dP, dZ) when Z is given (and info last
+    with return_info=True)."""
+    import torch
+
+    c = ctypes
+    n = S.shape[0]
+    dS = torch.zeros((nbatch * n, n), dtype=torch.float64, device=S.device)
+    dP = torch.zeros((nbatch * n, n), dtype=torch.float64, device=S.device)
+    if Z is not None and dZ is None:
+        dZ = torch.zeros((nbatch * n, n), dtype=torch.float64, device=S.device)
+    sel = None if select is None else np.ascontiguousarray(select, dtype=np.int32)
+    ev = [np.ascontiguousarray(x, dtype=np.float64) for x in (alphar, alphai, beta)]
+    info = c.c_int(-99)
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    rc = _lib.xtgevc3_dtgevc3_backward_full_device(
+        c.c_char(side.encode()), c.c_int(n), _tptr(S), c.c_int(S.shape[1]), _tptr(P), c.c_int(P.shape[1]),
+        *[_ptr(e) for e in ev], _ptr(sel), _tptr(Z), c.c_int(Z.shape[1] if Z is not None else 1), _tptr(V),
+        c.c_int(V.shape[1]), _tptr(G), c.c_int(G.shape[1]), c.c_int(nbatch), _tptr(dS), c.c_int(n), _tptr(dP),
+        c.c_int(n), _tptr(dZ), c.c_int(n), c.byref(info), c.c_void_p(stream))
+    if rc != 0:
+        raise RuntimeError("xtgevc3 backward failed: " + last_error())
+    if info.value < 0:
+        raise ValueError(f"xtgevc3 backward: info={info.value} (complex or infinite selected eigenvalue)")
+    out = (dS, dP) if Z is None else (dS, dP, dZ)
+    return out + (info.value,) if return_info else out

@@ -798,6 +798,28 @@ int xtgevc3_dtgevc3_backward_device_ex(int n, const double *S, int lds, const do
     return 0;
 }
 
+int xtgevc3_dtgevc3_backward_full_device(char side, int n, const double *S, int lds, const double *P, int ldp,
+                                         const double *alphar, const double *alphai, const double *beta,
+                                         const int *select, const double *Zb, int ldz, const double *V, int ldv,
+                                         const double *G, int ldg, int nbatch, double *dS, int ldds, double *dP,
+                                         int lddp, double *dZ, int lddz, int *info, void *stream)
+{
+    if (side != 'R' && side != 'L') {
+        xtg::set_error(std::string("side must be 'R' or 'L', got '") + side + "'");
+        if (info) {
+            *info = -1;
+        }
+        return -1;
+    }
+    int rc = xtg::real_backward_full_device(side, n, S, lds, P, ldp, alphar, alphai, beta, select, Zb, ldz, V, ldv,
+                                            G, ldg, nbatch, dS, ldds, dP, lddp, dZ, lddz, info,
+                                            (hipStream_t)stream);
+    if (rc != 0) {
+        return xtg::fail(info, "backward");
+    }
+    return 0;
+}
+
 int xtgevc3_dtgsna_e_device(char howmny, const int *select, int n, const double *S, int lds, const double *P,
                             int ldp, const double *alphai, const double *VL, int ldvl, const double *VR, int ldvr,
                             int m, double *s, int *info, void *stream)

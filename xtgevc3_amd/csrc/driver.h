@@ -71,6 +71,13 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
                          const double *G, long ldg, int nbatch, double *dS, long ldds, double *dP, long lddp, int *info,
                          hipStream_t st, int prec = 0);
 
+// backward for either side, optionally through the back-transform (Zb, and dL/dZb into dZ)
+int real_backward_full_device(char side, int n, const double *S, long lds, const double *P, long ldp,
+                              const double *ar, const double *ai, const double *be, const int *select,
+                              const double *Zb, long ldz, const double *V, long ldv, const double *G, long ldg,
+                              int nbatch, double *dS, long ldds, double *dP, long lddp, double *dZ, long lddz,
+                              int *info, hipStream_t st);
+
 int real_tgsna_device(char howmny, const int *select, int n, const double *S, long lds, const double *P, long ldp,
                       const double *ai, const double *VL, long ldvl, const double *VR, long ldvr, int m, double *s,
                       hipStream_t st);

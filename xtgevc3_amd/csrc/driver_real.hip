@@ -59,7 +59,10 @@ __global__ void k_normalize(double *, long, int, const int *, const int *);
 __global__ void k_round_f32(const double *, long, long, float *, long);
 __global__ void k_reverse_transpose(const double *, long, int, double *, long);
 __global__ void k_colnorms(const double *, long, const double *, long, int, int, double *);
-__global__ void k_bwd_rhs(const double *, long, const double *, long, int, int, int, const int *, double *, long);
+__global__ void k_bwd_rhs(const double *, long, const double *, long, int, int, int, const int *, double *, long, int);
+__global__ void k_bwd_permute(const double *, long, int, int, int, int, int, double *, long);
+__global__ void k_bwd_gcorr(const double *, long, const double *, long, int, int, double *, long);
+__global__ void k_bwd_trunc(double *, long, int, int, const int *);
 __global__ void k_bwd_lambda_t(const double *, long, int, int, int, const int *, const double *, const int *, double *,
                                double *, int *);
 __global__ void k_to_rowmajor(const double *, long, int, int, double *, long);
@@ -339,6 +342,8 @@ struct DevBuf {
 struct RealWork {
     DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wblk, wgrp, xrel, xfin, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
     DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp, bkpre, bovf, sfac;
+    // backward of left / back-transformed vectors: working pencil, permuted operands, products
+    DevBuf bfS, bfP, bfV, bfG, bfGc, bfGt, bfVt, bfZt, bfW, bfGh, bfdS, bfdP, bfk;
     DevBuf Ut, YB, part, tcol, tw; // eigenvalue condition numbers
     DevBuf Sq, Pq, Wg, wgexp, wbexp, xbexp, sstart, Xq, Zq, xexp, cfar; // split-int8 digits and exponents
     // split-int8 form: side stream (default priority) for the digit splitting of the pencil
@@ -1317,6 +1322,11 @@ static int real_device_ws(RealWork &ws, char side, char howmny, const int *selec
 // returns it).  alpha_k = S_kk and beta_k = P_kk for the selected real eigenvalues.
 // Returns -1001 through *info when a selected eigenvalue is complex or infinite.
 // ---------------------------------------------------------------------------------------
+static int real_backward_core(RealWork &ws, int n, const double *S, long lds, const double *P, long ldp,
+                              const double *ar, const double *ai, const double *be, const int *select, const double *V,
+                              long ldv, const double *G, long ldg, int nbatch, double *dS, long ldds, double *dP,
+                              long lddp, int *info, hipStream_t st, int prec, bool rhs_given);
+
 int real_backward_device(int n, const double *S, long lds, const double *P, long ldp, const double *ar,
                          const double *ai, const double *be, const int *select, const double *V, long ldv,
                          const double *G, long ldg, int nbatch, double *dS, long ldds, double *dP, long lddp, int *info,
@@ -1328,6 +1338,18 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
     RealWork &ws = g_ws[dev & 63];
     WsOrder order{&ws.ev_last, st};
     XTG_CHECK(order.begin());
+    return real_backward_core(ws, n, S, lds, P, ldp, ar, ai, be, select, V, ldv, G, ldg, nbatch, dS, ldds, dP, lddp,
+                              info, st, prec, false);
+}
+
+// The backward core.  rhs_given: G already holds the adjoint right-hand sides (the cotangent of
+// the unnormalised vector expressed in the Schur basis), V the Schur-basis vectors they belong
+// to; otherwise V is the normalised output and the normalisation's correction is applied to G.
+static int real_backward_core(RealWork &ws, int n, const double *S, long lds, const double *P, long ldp,
+                              const double *ar, const double *ai, const double *be, const int *select, const double *V,
+                              long ldv, const double *G, long ldg, int nbatch, double *dS, long ldds, double *dP,
+                              long lddp, int *info, hipStream_t st, int prec, bool rhs_given)
+{
     std::vector<Unit> units;
     int m = real_units(select ? 'S' : 'A', select, n, ai, units);
     *info = 0;
@@ -1403,7 +1425,8 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
     double *X = ws.X.as<double>();
     hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, X, (long)n * ldx); XTG_LAUNCH_CHECK("k_zero");
     int pr = prof_begin(st);
-    hipLaunchKernelGGL(k_bwd_rhs, dim3((M + 3) / 4), dim3(256), 0, st, V, ldv, G, ldg, n, m, nbatch, ws.kcol.as<int>(), X, ldx); XTG_LAUNCH_CHECK("k_bwd_rhs");
+    hipLaunchKernelGGL(k_bwd_rhs, dim3((M + 3) / 4), dim3(256), 0, st, V, ldv, G, ldg, n, m, nbatch, ws.kcol.as<int>(), X, ldx,
+                       rhs_given ? 0 : 1); XTG_LAUNCH_CHECK("k_bwd_rhs");
     prof_end(pr, st, 3, 0.0, 8.0 * 3.0 * n * M);
     const bool split = prec == XTGEVC3_PREC_SPLIT_I8;
     if (split) {
@@ -1504,6 +1527,171 @@ int real_backward_device(int n, const double *S, long lds, const double *P, long
         novf += ovf[q] != 0;
     }
     *info = novf;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// Backward of selected real eigenvectors for either side, with or without the back-transform.
+//   side 'R': V the right vectors; side 'L': the left vectors (y^H (beta S - alpha P) = 0).
+//   Zb (optional): the back-transform matrix the vectors were formed with (Z for right, Q for
+//   left vectors: V = normalise(Zb x)); then dZ (optional) receives dL/dZb for every batch.
+// Left vectors are right vectors of the working pencil J S^T J, J P^T J (rows reversed; the
+// eigenvalues in reverse order): the right backward runs there and dS = J dS_w^T J.  For
+// back-transformed vectors v = Zb x / s (s the normalisation): with g'' = g - (g . v) sign(v_p) e_p,
+// the adjoint right-hand side is Zb^T g'', the Schur-basis vector w = Zb^T v (= x / s, zero below
+// its eigenvalue's row), dS and dP follow from w as for untransformed vectors, and
+// dL/dZb = G'' W^T.
+// ---------------------------------------------------------------------------------------
+int real_backward_full_device(char side, int n, const double *S, long lds, const double *P, long ldp,
+                              const double *ar, const double *ai, const double *be, const int *select,
+                              const double *Zb, long ldz, const double *V, long ldv, const double *G, long ldg,
+                              int nbatch, double *dS, long ldds, double *dP, long lddp, double *dZ, long lddz,
+                              int *info, hipStream_t st)
+{
+    const bool left = side == 'L' || side == 'l';
+    if (!left && Zb == nullptr) {
+        return real_backward_device(n, S, lds, P, ldp, ar, ai, be, select, V, ldv, G, ldg, nbatch, dS, ldds, dP,
+                                    lddp, info, st, XTGEVC3_PREC_FP64);
+    }
+    int dev = 0;
+    XTG_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    RealWork &ws = g_ws[dev & 63];
+    WsOrder order{&ws.ev_last, st};
+    XTG_CHECK(order.begin());
+    std::vector<Unit> units;
+    const int m = real_units(select ? 'S' : 'A', select, n, ai, units);
+    *info = 0;
+    for (const Unit &u : units) {
+        if (u.w != 1 || std::fabs(be[u.k]) <= std::numeric_limits<double>::min()) {
+            *info = -1001;
+            return 0;
+        }
+    }
+    if (m == 0 || nbatch <= 0) {
+        return 0;
+    }
+    const int M = m * nbatch;
+    // working problem: the pencil, eigenvalues, selection and the operands in its coordinates
+    const double *Sw = S, *Pw = P;
+    long ldsw = lds, ldpw = ldp;
+    std::vector<double> arw(ar, ar + n), aiw(ai, ai + n), bew(be, be + n);
+    std::vector<int> selw;
+    const int *selp = select;
+    const double *Vw = V, *Gw = G;
+    long ldvw = ldv, ldgw = ldg;
+    const double *Zw = Zb;
+    long ldzw = ldz;
+    if (left) {
+        ENSURE(ws.bfS, sizeof(double) * (size_t)n * n);
+        ENSURE(ws.bfP, sizeof(double) * (size_t)n * n);
+        dim3 g32((n + 31) / 32, (n + 31) / 32);
+        hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, S, lds, n, ws.bfS.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
+        hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, P, ldp, n, ws.bfP.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
+        Sw = ws.bfS.as<double>();
+        Pw = ws.bfP.as<double>();
+        ldsw = ldpw = n;
+        for (int i = 0; i < n; ++i) {
+            arw[i] = ar[n - 1 - i];
+            aiw[i] = ai[n - 1 - i];
+            bew[i] = be[n - 1 - i];
+        }
+        if (select != nullptr) {
+            selw.assign(select, select + n);
+            std::reverse(selw.begin(), selw.end());
+            selp = selw.data();
+        }
+        // columns in reverse eigenvalue order; the rows too unless back-transformed (then the
+        // rows are components in the original basis and Q J carries the reversal)
+        ENSURE(ws.bfV, sizeof(double) * (size_t)n * m);
+        ENSURE(ws.bfG, sizeof(double) * (size_t)n * M);
+        const int rr = Zb == nullptr ? 1 : 0;
+        hipLaunchKernelGGL(k_bwd_permute, dim3((n + 255) / 256, m), dim3(256), 0, st, V, ldv, n, m, 1, rr, 1,
+                           ws.bfV.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_bwd_permute");
+        hipLaunchKernelGGL(k_bwd_permute, dim3((n + 255) / 256, M), dim3(256), 0, st, G, ldg, n, m, nbatch, rr, 1,
+                           ws.bfG.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_bwd_permute");
+        Vw = ws.bfV.as<double>();
+        Gw = ws.bfG.as<double>();
+        ldvw = ldgw = n;
+        if (Zb != nullptr) {
+            Zw = Zb + (long)(n - 1) * ldz; // Q J: columns reversed
+            ldzw = -ldz;
+        }
+    }
+    double *dSw = dS, *dPw = dP;
+    long ldsg = ldds, ldpg = lddp;
+    if (left) {
+        ENSURE(ws.bfdS, sizeof(double) * (size_t)n * n * nbatch);
+        ENSURE(ws.bfdP, sizeof(double) * (size_t)n * n * nbatch);
+        dSw = ws.bfdS.as<double>();
+        dPw = ws.bfdP.as<double>();
+        ldsg = ldpg = n;
+    }
+    int rc = 0;
+    if (Zw == nullptr) {
+        rc = real_backward_core(ws, n, Sw, ldsw, Pw, ldpw, arw.data(), aiw.data(), bew.data(), selp, Vw, ldvw, Gw,
+                                ldgw, nbatch, dSw, ldsg, dPw, ldpg, info, st, XTGEVC3_PREC_FP64, false);
+    }
+    else {
+        // the eigenvalue row of each working column (ascending), for the truncation of w
+        std::vector<Unit> uw;
+        real_units(selp ? 'S' : 'A', selp, n, aiw.data(), uw);
+        std::vector<int> kc(m);
+        for (int c = 0; c < m; ++c) {
+            kc[c] = uw[c].k;
+        }
+        ENSURE(ws.bfk, sizeof(int) * (size_t)m);
+        XTG_CHECK(hipMemcpyAsync(ws.bfk.p, kc.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+        const long ldt = ((long)m + 7) / 8 * 8, ldtM = ((long)M + 7) / 8 * 8;
+        ENSURE(ws.bfGc, sizeof(double) * (size_t)n * M);
+        ENSURE(ws.bfGt, sizeof(double) * (size_t)n * ldtM);
+        ENSURE(ws.bfVt, sizeof(double) * (size_t)n * ldt);
+        ENSURE(ws.bfZt, sizeof(double) * (size_t)n * n);
+        ENSURE(ws.bfW, sizeof(double) * (size_t)n * m);
+        ENSURE(ws.bfGh, sizeof(double) * (size_t)n * M);
+        double *Gc = ws.bfGc.as<double>(), *Gt = ws.bfGt.as<double>(), *Vt = ws.bfVt.as<double>();
+        double *Zt = ws.bfZt.as<double>(), *W = ws.bfW.as<double>(), *Gh = ws.bfGh.as<double>();
+        // g'' (the cotangent of the unnormalised vector, in the output basis)
+        hipLaunchKernelGGL(k_bwd_gcorr, dim3(M), dim3(64), 0, st, Vw, ldvw, Gw, ldgw, n, m, Gc, (long)n); XTG_LAUNCH_CHECK("k_bwd_gcorr");
+        // row-major copies: V, G'' (n x m, n x M) and Zb (n x n, its rows)
+        hipLaunchKernelGGL(k_to_rowmajor, dim3((m + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, st, Vw, ldvw, n, m, Vt, ldt); XTG_LAUNCH_CHECK("k_to_rowmajor");
+        hipLaunchKernelGGL(k_to_rowmajor, dim3((M + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, st, (const double *)Gc, (long)n, n, M, Gt, ldtM); XTG_LAUNCH_CHECK("k_to_rowmajor");
+        hipLaunchKernelGGL(k_to_rowmajor, dim3((n + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, st, Zw, ldzw, n, n, Zt, (long)n); XTG_LAUNCH_CHECK("k_to_rowmajor");
+        // W = Zb^T V and Gh = Zb^T G'' (column-major, ld n): D[c][j] = sum_k V(k, c) Zb(k, j)
+        launch_gemm_f64(2, st, m, n, n, 0, (const double *)Vt, ldt, (const double *)nullptr, 0L, (const double *)Zt,
+                        (long)n, W, (long)n, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr,
+                        (const double *)nullptr, 0, nullptr, nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+        launch_gemm_f64(2, st, M, n, n, 0, (const double *)Gt, ldtM, (const double *)nullptr, 0L, (const double *)Zt,
+                        (long)n, Gh, (long)n, (const int *)nullptr, 0, (const double *)nullptr, (const int *)nullptr,
+                        (const double *)nullptr, 0, nullptr, nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+        hipLaunchKernelGGL(k_bwd_trunc, dim3((n + 255) / 256, m), dim3(256), 0, st, W, (long)n, n, m,
+                           (const int *)ws.bfk.as<int>()); XTG_LAUNCH_CHECK("k_bwd_trunc");
+        rc = real_backward_core(ws, n, Sw, ldsw, Pw, ldpw, arw.data(), aiw.data(), bew.data(), selp, W, (long)n, Gh,
+                                (long)n, nbatch, dSw, ldsg, dPw, ldpg, info, st, XTGEVC3_PREC_FP64, true);
+        if (rc == 0 && dZ != nullptr) {
+            // dL/dZb_b = G''_b W^T: D[j][i] = sum_c W(j, c) G''_b(i, c), written to dZ(i, j) (left:
+            // to column n-1-j, as Zb J = Q J)
+            double *Cz = left ? dZ + (long)(n - 1) * lddz : dZ;
+            const long ldcz = left ? -lddz : lddz;
+            launch_gemm_f64(2, st, n, n, m, 0, (const double *)W, (long)n, (const double *)nullptr, 0L,
+                            (const double *)Gc, (long)n, Cz, ldcz, (const int *)nullptr, 0, (const double *)nullptr,
+                            (const int *)nullptr, (const double *)nullptr, 0, nullptr, nullptr, nbatch, (long)m * n,
+                            (long)n * lddz); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+        }
+    }
+    if (rc != 0 || *info < 0) {
+        return rc;
+    }
+    if (left) {
+        // dS_b = J dSw_b^T J (the same map takes the working pencil back)
+        dim3 g32((n + 31) / 32, (n + 31) / 32);
+        for (int b = 0; b < nbatch; ++b) {
+            hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, (const double *)dSw + (long)b * n * n,
+                               (long)n, n, dS + (long)b * n * ldds, ldds); XTG_LAUNCH_CHECK("k_reverse_transpose");
+            hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, (const double *)dPw + (long)b * n * n,
+                               (long)n, n, dP + (long)b * n * lddp, lddp); XTG_LAUNCH_CHECK("k_reverse_transpose");
+        }
+    }
     return 0;
 }
 

@@ -1516,14 +1516,15 @@ namespace xtg {
 
 // RHS of the adjoint solve in working rows: X[(n-1-i)*ldx + q] = g_i - [i == i*] (g.v) sign(v_i*)
 // for i < kcol[c] (zero elsewhere), where working column q = (m-1-c)*nbatch + b holds batch b of
-// forward column c and i* is the first index of max |v_i|.  One workgroup per 4 working
+// forward column c and i* is the first index of max |v_i| (correct == 0: G is already the
+// right-hand side, copied without the correction term).  One workgroup per 4 working
 // columns: each wave reduces one column (lane stride 64, fixed order, so a column's result does
 // not depend on nbatch), then 4 x 512 tiles through LDS write rows of X as 4 contiguous doubles
 // while g is read along i (one column per workgroup wrote X at a full row's stride per value;
 // 8 and 32 columns per workgroup left too few workgroups for one cotangent batch).
 __global__ __launch_bounds__(256) void k_bwd_rhs(const double *__restrict__ V, long ldv, const double *__restrict__ G,
                                                  long ldg, int n, int m, int nbatch, const int *__restrict__ kcol,
-                                                 double *__restrict__ X, long ldx)
+                                                 double *__restrict__ X, long ldx, int correct)
 {
     constexpr int TQ = 4, TI = 512;
     __shared__ double tile[TQ][TI + 1];
@@ -1560,7 +1561,7 @@ __global__ __launch_bounds__(256) void k_bwd_rhs(const double *__restrict__ V, l
         }
         if (lane == 0) {
             s_bi[j] = bi;
-            s_corr[j] = v[bi] < 0.0 ? -dot : dot;
+            s_corr[j] = correct ? (v[bi] < 0.0 ? -dot : dot) : 0.0;
             s_k[j] = kcol[c];
         }
     }
@@ -1628,6 +1629,68 @@ __global__ void k_bwd_lambda_t(const double *__restrict__ X, long ldx, int n, in
                 LamP[o] = coef[4 * c + 1] * v;
             }
         }
+    }
+}
+
+// out(i, b m + c) = A(ri, b m + cc) for i < n, c < m, b < nbatch, with ri = n - 1 - i when
+// rev_rows (else i) and cc = m - 1 - c when rev_cols (else c): reversal of the rows and / or of
+// the columns within each batch of m (the working coordinates of left eigenvectors).
+__global__ void k_bwd_permute(const double *__restrict__ A, long lda, int n, int m, int nbatch, int rev_rows,
+                              int rev_cols, double *__restrict__ out, long ldo)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    if (i >= n || q >= m * nbatch) {
+        return;
+    }
+    const int b = q / m, c = q % m;
+    const int src = b * m + (rev_cols ? m - 1 - c : c);
+    out[i + (long)q * ldo] = A[(rev_rows ? n - 1 - i : i) + (long)src * lda];
+}
+
+// Cotangent of the unnormalised vector: for column q = b m + c of G (batch b, column c) and the
+// output vector v_c (V column c), out(:, q) = g - (g . v) sign(v_p) e_p with p the first index of
+// max |v_i| (the LAPACK normalisation divides by |v_p|).  One wave per column, fixed order.
+__global__ __launch_bounds__(64) void k_bwd_gcorr(const double *__restrict__ V, long ldv,
+                                                  const double *__restrict__ G, long ldg, int n, int m,
+                                                  double *__restrict__ out, long ldo)
+{
+    const int q = blockIdx.x, c = q % m, lane = threadIdx.x;
+    const double *v = V + (long)c * ldv;
+    const double *g = G + (long)q * ldg;
+    double best = -1.0, dot = 0.0;
+    int bi = 0;
+    for (int i = lane; i < n; i += WAVE) {
+        const double a = fabs(v[i]);
+        if (a > best) {
+            best = a;
+            bi = i;
+        }
+        dot = fma(g[i], v[i], dot);
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
+        }
+        dot += __shfl_xor(dot, off);
+    }
+    const double corr = v[bi] < 0.0 ? -dot : dot;
+    for (int i = lane; i < n; i += WAVE) {
+        out[i + (long)q * ldo] = g[i] - (i == bi ? corr : 0.0);
+    }
+}
+
+// W(j, c) = 0 for rows j > kcol[c]: the Schur-basis vector Z^T v of a real eigenvalue k has no
+// components below k (the product leaves rounding noise there).
+__global__ void k_bwd_trunc(double *__restrict__ W, long ldw, int n, int m, const int *__restrict__ kcol)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (j < n && c < m && j > kcol[c]) {
+        W[j + (long)c * ldw] = 0.0;
     }
 }
 
