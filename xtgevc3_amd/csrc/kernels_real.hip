@@ -102,7 +102,25 @@ __device__ __forceinline__ int qbi(int v, int src) { XTG_RB_CASES(row_bcast_i) }
 
 __device__ __forceinline__ cpx qbc(cpx v, int src) { return {qbd(v.r, src), qbd(v.i, src)}; }
 
-constexpr int SP_PITCH = 66; // LDS column pitch (rows) of the staged diagonal block
+// Packed LDS layout of a staged diagonal block: column c keeps the 16-row blocks at and above its
+// diagonal, rows [0, 16 (c / 16 + 1)), at sp[sp_col(c) + row]; the subdiagonal entry (c + 1, c)
+// of every column is kept apart.  2640 {S,P} entries for 65 columns (42 KB).
+__host__ __device__ constexpr int sp_col(int c) { return 16 * ((c >> 4) + 1) * (8 * (c >> 4) + (c & 15)); }
+
+// Rotations inside 16-lane rows (DPP row_ror): ror<1> gives lane j the value of lane j - 1
+// (lane 0 that of lane 15), ror<15> the value of lane j + 1 (lane 15 that of lane 0).
+template <int K> __device__ __forceinline__ double row_ror_d(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x120 + K, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x120 + K, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+template <int K> __device__ __forceinline__ double ror(double v) { return row_ror_d<K>(v); }
+template <int K> __device__ __forceinline__ cpx ror(cpx v) { return {row_ror_d<K>(v.r), row_ror_d<K>(v.i)}; }
+template <int K> __device__ __forceinline__ double2 ror(double2 v)
+{
+    return make_double2(row_ror_d<K>(v.x), row_ror_d<K>(v.y));
+}
 
 // Arithmetic of a unit's vector entries: double for waves of real eigenvalues only, cpx when
 // the wave holds a complex pair (real units then carry a zero imaginary part).
@@ -118,6 +136,7 @@ template <> struct Ar<double> {
     static __device__ __forceinline__ double amax(double x) { return fabs(x); }
     static __device__ __forceinline__ double bc(double v, int src) { return qbd(v, src); }
     static __device__ __forceinline__ double scal(double v, double f) { return v * f; }
+    static __device__ __forceinline__ double fmar(double s, double x, double acc) { return fma(s, x, acc); }
     // shifted entry a*S - b*P (b real for these waves)
     static __device__ __forceinline__ double shift(double a, double bR, double, double2 e) { return a * e.x - bR * e.y; }
     static __device__ __forceinline__ double bmul(double bR, double, double x) { return bR * x; }
@@ -138,6 +157,7 @@ template <> struct Ar<cpx> {
     static __device__ __forceinline__ double amax(cpx x) { return fmax(fabs(x.r), fabs(x.i)); }
     static __device__ __forceinline__ cpx bc(cpx v, int src) { return qbc(v, src); }
     static __device__ __forceinline__ cpx scal(cpx v, double f) { return {v.r * f, v.i * f}; }
+    static __device__ __forceinline__ cpx fmar(double s, cpx x, cpx acc) { return {fma(s, x.r, acc.r), fma(s, x.i, acc.i)}; }
     static __device__ __forceinline__ cpx shift(double a, double bR, double bI, double2 e)
     {
         return {a * e.x - bR * e.y, -bI * e.y};
@@ -156,48 +176,12 @@ template <> struct Ar<cpx> {
 // registers).
 __device__ __forceinline__ double vsel(bool c, double x, double y) { return c ? x : y; }
 __device__ __forceinline__ cpx vsel(bool c, cpx x, cpx y) { return {c ? x.r : y.r, c ? x.i : y.i}; }
-
-// Robust 2x2 bump solve from its factorization (rare path: some entry would exceed the
-// rescale trigger).  Pivoted rows: rp = sw ? r2 : r1, rs = sw ? r1 : r2;
-// y = rs - mlt rp, x2 = y iu, x1 = (rp - q x2) ip.  Returns the scaled solution and the
-// exponent e of the scale 2^-e applied to the right-hand side.
-template <class V> struct Bump2 {
-    V x1, x2;
-    int e;
-};
-
-template <class V>
-__device__ __forceinline__ Bump2<V> bump_solve_scaled(V r1, V r2, bool sw, V mlt, V q, V iu, V ip)
-{
-    typedef Ar<V> A;
-    V rp = vsel(sw, r2, r1), rs = vsel(sw, r1, r2);
-    V y = A::sub(rs, A::mul(mlt, rp));
-    Bump2<V> out;
-    out.e = 0;
-    int ea = rescale_exponent(A::abs1(y), A::abs1(iu));
-    if (ea > 0) {
-        const double f = ldexp(1.0, -ea);
-        y = A::scal(y, f);
-        rp = A::scal(rp, f);
-        out.e += ea;
-    }
-    V x2 = A::mul(y, iu);
-    V w = A::sub(rp, A::mul(q, x2));
-    int eb = rescale_exponent(A::abs1(w), A::abs1(ip));
-    if (eb > 0) {
-        const double f = ldexp(1.0, -eb);
-        w = A::scal(w, f);
-        x2 = A::scal(x2, f);
-        out.e += eb;
-    }
-    out.x1 = A::mul(w, ip);
-    out.x2 = x2;
-    return out;
-}
+__device__ __forceinline__ double2 vsel(bool c, double2 x, double2 y) { return make_double2(c ? x.x : y.x, c ? x.y : y.y); }
 
 // Per-lane description of one unit in one block step (see k_solve_real).
 struct UnitStep {
-    const double2 *sp; // staged {S,P} block, LDS
+    const double2 *sp;  // staged {S,P} block (packed columns, sp_col), LDS
+    const double2 *spd; // its subdiagonal {S,P}_{c+1,c}, LDS
     int nb, j, dl, lol, init;
     unsigned long long bumps;
     double a, bR, bI, bn1;
@@ -229,84 +213,69 @@ struct UnitOut {
 
 // One unit's block step: factor the lane's rows, sweep the block bottom-up starting from the
 // right-hand side `raw`, then write the block's solution, operands and scale state.
+//
+// Rows are handled as positions p = 16 i + j (row slot i of lane j).  The factor gives every
+// position a pivot reciprocal f and, at the lower row of a 2x2 bump of S, an elimination
+// multiplier g, so that each row step is the same: t = v_p - g v_{p-1}, x_p = t f.  A pivoted
+// bump exchanges the equations of its two rows: the position keeps its unknown, takes the other
+// row's right-hand side and updates with the other row's {S,P} entries (eqd = +-1).  The sweep
+// runs slot by slot: inside a slot every row step updates the slot's own rows; the slots above
+// take the slot's rows 1..15 in one batched update before its row 0 (whose 2x2 bump partner
+// may be the slot above's row 15), then row 0 like a single row step.
 template <class V, int QR, int MAXB>
 __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> raw, const UnitOut o)
 {
     typedef Ar<V> A;
-    V vr[QR];
-#pragma clang loop unroll(full)
-    for (int i = 0; i < QR; ++i) {
-        vr[i] = A::mk(raw.v[i].x, raw.v[i].y);
-    }
-    int my_exp = 0;
     const double2 *sp = u.sp;
+    const double2 *spd = u.spd;
     const int nb = u.nb, j = u.j, dl = u.dl, lol = u.lol;
     const double a = u.a, bR = u.bR, bI = u.bI, bn1 = u.bn1;
+    const unsigned long long bumps = u.bumps;
 
-    // starting vector on the unit's own block
-    if (dl >= 0 && dl < nb) {
-        V x0 = A::mk(0.0, 0.0), x1v = A::mk(0.0, 0.0);
-        if (u.init == INIT_UNIT) {
-            x0 = A::mk(1.0, 0.0);
-        }
-        else if (u.init == INIT_PAIR_ROW1 && dl + 1 < nb) {
-            double2 e11 = sp[dl + dl * SP_PITCH], e12 = sp[dl + (dl + 1) * SP_PITCH];
-            x0 = A::mk(-(a * e12.x - bR * e12.y), bI * e12.y);
-            x1v = A::mk(a * e11.x - bR * e11.y, -bI * e11.y);
-        }
-        else if (u.init == INIT_PAIR_ROW2 && dl + 1 < nb) {
-            double2 e22 = sp[(dl + 1) + (dl + 1) * SP_PITCH], e21 = sp[(dl + 1) + dl * SP_PITCH];
-            x0 = A::mk(a * e22.x - bR * e22.y, -bI * e22.y);
-            x1v = A::mk(-a * e21.x, 0.0);
-        }
-#pragma clang loop unroll(full)
-        for (int i = 0; i < QR; ++i) {
-            const int r = QL * i + j;
-            vr[i] = vsel(r == dl, x0, vsel(r == dl + 1 && u.init != INIT_UNIT, x1v, vr[i]));
-        }
-    }
-
-    // factor the shifted diagonal of this lane's rows
-    V fa[QR], fb[QR];
-    unsigned swbits = 0;
+    // ---- factor: f, g and the equation offset of every position
+    V f[QR], g[QR];
+    int eqd[QR];
+    bool anysw = false;
 #pragma clang loop unroll(full)
     for (int i = 0; i < QR; ++i) {
         const int r = QL * i + j;
-        fa[i] = A::mk(0.0, 0.0);
-        fb[i] = A::mk(0.0, 0.0);
+        f[i] = A::mk(0.0, 0.0);
+        g[i] = A::mk(0.0, 0.0);
+        eqd[i] = 0;
         if (r < nb) {
-            const bool lower = r >= 1 && ((u.bumps >> (r - 1)) & 1ull);
-            const bool upper = r + 1 < nb && ((u.bumps >> r) & 1ull);
+            const bool lower = r >= 1 && ((bumps >> (r - 1)) & 1ull);
+            const bool upper = r + 1 < nb && ((bumps >> r) & 1ull);
             if (!lower && !upper) {
-                const double2 ee = sp[r + r * SP_PITCH];
+                const double2 ee = sp[sp_col(r) + r];
                 V c = A::shift(a, bR, bI, ee);
                 const double dmin = fmax(kEps * (fabs(a) * fabs(ee.x) + bn1 * fabs(ee.y)), kSafeMin);
                 if (A::abs1(c) < dmin) {
                     c = A::mk(dmin, 0.0);
                 }
                 const bool solve = r < dl && r >= lol;
-                fa[i] = vsel(solve, A::recip(c), A::mk(r == dl ? 1.0 : 0.0, 0.0));
+                f[i] = vsel(solve, A::recip(c), A::mk(r == dl ? 1.0 : 0.0, 0.0));
             }
             else {
                 const int r1 = lower ? r - 1 : r, r2 = r1 + 1;
-                const double2 e11 = sp[r1 + r1 * SP_PITCH], e12 = sp[r1 + r2 * SP_PITCH];
-                const double2 e21 = sp[r2 + r1 * SP_PITCH], e22 = sp[r2 + r2 * SP_PITCH];
+                const double2 e11 = sp[sp_col(r1) + r1], e12 = sp[sp_col(r2) + r1];
+                const double2 e21 = spd[r1], e22 = sp[sp_col(r2) + r2];
                 V c11 = A::shift(a, bR, bI, e11), c12 = A::shift(a, bR, bI, e12);
                 V c21 = A::shift(a, bR, bI, e21), c22 = A::shift(a, bR, bI, e22);
                 const double dm1 = fmax(kEps * (fabs(a) * (fabs(e11.x) + fabs(e21.x)) + bn1 * fabs(e11.y)), kSafeMin);
                 const double dm2 = fmax(kEps * (fabs(a) * (fabs(e12.x) + fabs(e22.x)) + bn1 * fabs(e22.y)), kSafeMin);
                 const bool s1 = r1 < dl && r1 >= lol, s2 = r2 < dl && r2 >= lol;
                 bool sw = false;
-                V mlt = A::mk(0.0, 0.0), q = A::mk(0.0, 0.0), iu = A::mk(0.0, 0.0), ip = A::mk(0.0, 0.0);
+                V mlt = A::mk(0.0, 0.0), iu = A::mk(0.0, 0.0), ip = A::mk(0.0, 0.0);
                 if (r1 == dl) {
-                    // the unit's own 2x2 block: entries preset, x = rhs
+                    // the unit's own 2x2 block: both entries preset
                     iu = A::mk(1.0, 0.0);
                     ip = A::mk(1.0, 0.0);
                 }
                 else if (s1) {
+                    // pivoted LU of [c11 c12; c21 c22]: pivot row rp = sw ? r2 : r1
                     sw = A::abs1(c21) > A::abs1(c11);
                     V pv = vsel(sw, c21, c11), sv = vsel(sw, c11, c21), tv = vsel(sw, c12, c22);
-                    q = vsel(sw, c22, c12);
+                    const V q = vsel(sw, c22, c12);
                     if (A::abs1(pv) < dm1) {
                         pv = A::mk(dm1, 0.0);
                     }
@@ -325,121 +294,162 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
                     }
                     iu = A::recip(c22);
                 }
-                fa[i] = vsel(lower, q, mlt);
-                fb[i] = vsel(lower, iu, ip);
-                swbits |= sw ? (1u << i) : 0u;
+                f[i] = vsel(lower, iu, ip);
+                g[i] = vsel(lower, mlt, A::mk(0.0, 0.0));
+                eqd[i] = sw ? (lower ? -1 : 1) : 0;
+                anysw = anysw || sw;
             }
         }
     }
 
-    bool pre = false; // row r was solved with row r+1 (2x2 bump)
-    double2 ec[QR], en[QR]; // {S,P} of this lane's rows in column r (ec) and r-1 (en)
+    // ---- right-hand sides by equation row: pivoted bumps exchange their two rows' values
+    double2 rv[QR];
 #pragma clang loop unroll(full)
     for (int i = 0; i < QR; ++i) {
-        ec[i] = make_double2(0.0, 0.0);
-        en[i] = make_double2(0.0, 0.0);
+        rv[i] = raw.v[i];
     }
+    if (__any(anysw)) {
+        double2 up[QR], dn[QR]; // values of lane j + 1 (ror 15) and lane j - 1 (ror 1)
 #pragma clang loop unroll(full)
-    for (int r = MAXB - 1; r >= 0; --r) {
-        if (r < nb) {
-            if (r == nb - 1) {
+        for (int i = 0; i < QR; ++i) {
+            up[i] = ror<15>(raw.v[i]);
+            dn[i] = ror<1>(raw.v[i]);
+        }
 #pragma clang loop unroll(full)
-                for (int i = 0; i < QR; ++i) {
-                    if (QL * i < r) {
-                        ec[i] = sp[QL * i + j + r * SP_PITCH];
+        for (int i = 0; i < QR; ++i) {
+            const double2 zero = make_double2(0.0, 0.0);
+            const double2 un = i + 1 < QR ? up[i + 1 < QR ? i + 1 : i] : zero; // slot below, lane 0
+            const double2 dp = i > 0 ? dn[i > 0 ? i - 1 : 0] : zero;            // slot above, lane 15
+            const double2 vu = vsel(j == QL - 1, un, up[i]);
+            const double2 vd = vsel(j == 0, dp, dn[i]);
+            rv[i] = vsel(eqd[i] > 0, vu, vsel(eqd[i] < 0, vd, rv[i]));
+        }
+    }
+    V vr[QR];
+#pragma clang loop unroll(full)
+    for (int i = 0; i < QR; ++i) {
+        vr[i] = A::mk(rv[i].x, rv[i].y);
+    }
+
+    // starting vector on the unit's own block
+    if (dl >= 0 && dl < nb) {
+        V x0 = A::mk(0.0, 0.0), x1v = A::mk(0.0, 0.0);
+        if (u.init == INIT_UNIT) {
+            x0 = A::mk(1.0, 0.0);
+        }
+        else if (u.init == INIT_PAIR_ROW1 && dl + 1 < nb) {
+            const double2 e11 = sp[sp_col(dl) + dl], e12 = sp[sp_col(dl + 1) + dl];
+            x0 = A::mk(-(a * e12.x - bR * e12.y), bI * e12.y);
+            x1v = A::mk(a * e11.x - bR * e11.y, -bI * e11.y);
+        }
+        else if (u.init == INIT_PAIR_ROW2 && dl + 1 < nb) {
+            const double2 e22 = sp[sp_col(dl + 1) + dl + 1], e21 = spd[dl];
+            x0 = A::mk(a * e22.x - bR * e22.y, -bI * e22.y);
+            x1v = A::mk(-a * e21.x, 0.0);
+        }
+#pragma clang loop unroll(full)
+        for (int i = 0; i < QR; ++i) {
+            const int r = QL * i + j;
+            vr[i] = vsel(r == dl, x0, vsel(r == dl + 1 && u.init != INIT_UNIT, x1v, vr[i]));
+        }
+    }
+
+    // ---- sweep
+    int my_exp = 0;
+#pragma clang loop unroll(full)
+    for (int i = QR - 1; i >= 0; --i) {
+        if (QL * i >= nb) {
+            continue;
+        }
+        // the slot's own coefficients: {S,P} of the lane's equation row at column 16 i + m,
+        // zero where the row takes no update from the column (m <= j: the lane's own and lower
+        // positions; the unit's preset row dl from column dl + 1)
+        const int pos = QL * i + j;
+        const int eqr = pos + eqd[i];
+        double2 dc[QL];
+#pragma clang loop unroll(full)
+        for (int m = 0; m < QL; ++m) {
+            dc[m] = make_double2(0.0, 0.0);
+            if (QL * i + m < MAXB) {
+                const int c = QL * i + m;
+                const double2 e = sp[sp_col(c) + eqr];
+                dc[m] = vsel(m > j && !(pos == dl && c == dl + 1), e, dc[m]);
+            }
+        }
+        V xs[QL];  // the slot's solution entries (broadcast), for the batched update
+        int xe[QL]; // rescale exponent of the slot at the time each was stored
+        int es = 0;
+#pragma clang loop unroll(full)
+        for (int m = 0; m < QL; ++m) {
+            xs[m] = A::mk(0.0, 0.0);
+            xe[m] = 0;
+        }
+#pragma clang loop unroll(full)
+        for (int jr = QL - 1; jr >= 0; --jr) {
+            const int r = QL * i + jr;
+            if (r >= MAXB || r >= nb) {
+                continue;
+            }
+            if (jr == 0 && i > 0 && r + 1 < nb) {
+                // slots above: rows 1..15 of this slot at once,
+                // v -= a (S x) - b (P x) over the slot's columns
+                if (__any(es != 0)) {
+#pragma clang loop unroll(full)
+                    for (int m = 1; m < QL; ++m) {
+                        xs[m] = A::scal(xs[m], ldexp(1.0, xe[m] - es));
+                    }
+                }
+#pragma clang loop unroll(full)
+                for (int k = 0; k < QR; ++k) {
+                    if (k < i) {
+                        const int eqk = QL * k + j + eqd[k];
+                        V as = A::mk(0.0, 0.0), ap = A::mk(0.0, 0.0);
+#pragma clang loop unroll(full)
+                        for (int m = 1; m < QL; ++m) {
+                            const double2 e = sp[sp_col(QL * i + m) + eqk];
+                            as = A::fmar(e.x, xs[m], as);
+                            ap = A::fmar(e.y, xs[m], ap);
+                        }
+                        vr[k] = A::upd(vr[k], make_double2(1.0, 1.0), A::scal(as, a), A::bmul(bR, bI, ap));
                     }
                 }
             }
-            if (r >= 1) {
-#pragma clang loop unroll(full)
-                for (int i = 0; i < QR; ++i) {
-                    if (QL * i < r - 1) {
-                        en[i] = sp[QL * i + j + (r - 1) * SP_PITCH];
-                    }
-                }
+            V t = vr[i];
+            if (r >= 1 && ((bumps >> (r - 1)) & 1ull)) {
+                // lower row of a 2x2 bump: eliminate with the upper row's current value
+                const V prev = (jr > 0 || i == 0) ? ror<1>(vr[i]) : ror<1>(vr[i > 0 ? i - 1 : 0]);
+                t = A::sub(t, A::mul(g[i], prev));
             }
-            if (pre) {
-                pre = false;
+            V x = A::bc(A::mul(t, f[i]), jr);
+            const bool bad = !(A::abs1(x) <= kScaleTrigger);
+            if (__any(bad)) {
+                const int eo = bad ? rescale_exponent(A::abs1(t), A::abs1(f[i])) : 0;
+                const int e = qbi(eo, jr);
+                const double sc = ldexp(1.0, -e);
+#pragma clang loop unroll(full)
+                for (int k = 0; k < QR; ++k) {
+                    vr[k] = A::scal(vr[k], sc);
+                }
+                t = A::scal(t, sc);
+                my_exp -= e;
+                es += e;
+                x = A::bc(A::mul(t, f[i]), jr);
             }
-            else if (r >= 1 && ((u.bumps >> (r - 1)) & 1ull)) {
-                // 2x2 bump at rows (r-1, r)
-                const int j1 = (r - 1) & (QL - 1), i1 = (r - 1) / QL;
-                const int j2 = r & (QL - 1), i2 = r / QL;
-                const V rhs1 = A::bc(vr[i1], j1), rhs2 = A::bc(vr[i2], j2);
-                const bool sw1 = (swbits >> i1) & 1u, sw2 = (swbits >> i2) & 1u;
-                // upper-row lane: y = rs - mlt rp
-                V y = A::bc(A::sub(vsel(sw1, rhs1, rhs2), A::mul(fa[i1], vsel(sw1, rhs2, rhs1))), j1);
-                // lower-row lane: x2 = y / U22, w = rp - U12 x2
-                V x2 = A::mul(y, fb[i2]);
-                V w = A::sub(vsel(sw2, rhs2, rhs1), A::mul(fa[i2], x2));
-                x2 = A::bc(x2, j2);
-                w = A::bc(w, j2);
-                // upper-row lane: x1 = w / pivot
-                V x1 = A::bc(A::mul(w, fb[i1]), j1);
-                const bool bad = !(A::abs1(x1) <= kScaleTrigger && A::abs1(x2) <= kScaleTrigger);
-                if (__any(bad)) {
-                    const V mlt = A::bc(fa[i1], j1), ip = A::bc(fb[i1], j1);
-                    const V q = A::bc(fa[i2], j2), iu = A::bc(fb[i2], j2);
-                    const bool sw = qbi((int)sw1, j1) != 0;
-                    Bump2<V> bs = bump_solve_scaled<V>(rhs1, rhs2, sw, mlt, q, iu, ip);
-                    const int e = bad ? bs.e : 0;
-                    x1 = vsel(bad, bs.x1, x1);
-                    x2 = vsel(bad, bs.x2, x2);
-                    const double f = ldexp(1.0, -e);
+            xs[jr] = x;
+            xe[jr] = es;
+            const V ax = A::scal(x, a), bx = A::bmul(bR, bI, x);
+            vr[i] = vsel(j == jr, x, A::upd(vr[i], dc[jr], ax, bx));
+            if (jr == 0) {
+                // row 16 i into the slots above (not into the preset row dl from column dl + 1)
 #pragma clang loop unroll(full)
-                    for (int i = 0; i < QR; ++i) {
-                        vr[i] = A::scal(vr[i], f);
-                    }
-                    my_exp -= e;
-                }
-                vr[i1] = vsel(j == j1, x1, vr[i1]);
-                vr[i2] = vsel(j == j2, x2, vr[i2]);
-                const V ax1 = A::scal(x1, a), bx1 = A::bmul(bR, bI, x1);
-                const V ax2 = A::scal(x2, a), bx2 = A::bmul(bR, bI, x2);
-#pragma clang loop unroll(full)
-                for (int i = 0; i < QR; ++i) {
-                    if (QL * i + QL - 1 < r - 1) {
-                        vr[i] = A::upd(A::upd(vr[i], en[i], ax1, bx1), ec[i], ax2, bx2);
-                    }
-                    else if (QL * i < r - 1) {
-                        const V nv = A::upd(A::upd(vr[i], en[i], ax1, bx1), ec[i], ax2, bx2);
-                        vr[i] = vsel(QL * i + j < r - 1, nv, vr[i]);
+                for (int k = 0; k < QR; ++k) {
+                    if (k < i) {
+                        const int pk = QL * k + j;
+                        const double2 e = sp[sp_col(r) + pk + eqd[k]];
+                        const double2 ek = vsel(pk == dl && r == dl + 1, make_double2(0.0, 0.0), e);
+                        vr[k] = A::upd(vr[k], ek, ax, bx);
                     }
                 }
-                pre = true;
-            }
-            else {
-                // 1x1 row
-                const int jr = r & (QL - 1), ir = r / QL;
-                V x = A::bc(A::mul(vr[ir], fa[ir]), jr);
-                const bool bad = !(A::abs1(x) <= kScaleTrigger);
-                if (__any(bad)) {
-                    const int eo = bad ? rescale_exponent(A::abs1(vr[ir]), A::abs1(fa[ir])) : 0;
-                    const int e = qbi(eo, jr);
-                    const double f = ldexp(1.0, -e);
-#pragma clang loop unroll(full)
-                    for (int i = 0; i < QR; ++i) {
-                        vr[i] = A::scal(vr[i], f);
-                    }
-                    my_exp -= e;
-                    x = A::bc(A::mul(vr[ir], fa[ir]), jr);
-                }
-                vr[ir] = vsel(j == jr, x, vr[ir]);
-                const V ax = A::scal(x, a), bx = A::bmul(bR, bI, x);
-#pragma clang loop unroll(full)
-                for (int i = 0; i < QR; ++i) {
-                    if (QL * i + QL - 1 < r) {
-                        vr[i] = A::upd(vr[i], ec[i], ax, bx);
-                    }
-                    else if (QL * i < r) {
-                        const V nv = A::upd(vr[i], ec[i], ax, bx);
-                        vr[i] = vsel(QL * i + j < r, nv, vr[i]);
-                    }
-                }
-            }
-#pragma clang loop unroll(full)
-            for (int i = 0; i < QR; ++i) {
-                ec[i] = en[i];
             }
         }
     }
@@ -673,9 +683,10 @@ __global__ __launch_bounds__(256) void k_solve_real(
     int u_end, double *__restrict__ sig, int *__restrict__ colexp, int *__restrict__ blkexp, int *__restrict__ flag,
     int w_s, int w_p, double *__restrict__ sig_grp, int first_in_group, GroupInfo grp, MixedExp mx, SplitOut sq)
 {
-    static_assert(MAXB <= 65 && SP_PITCH >= MAXB + 1 && (SP_PITCH % 2) == 0, "block layout");
+    static_assert(MAXB <= 65, "block layout: the staging decode covers 5 column blocks");
     constexpr int QR = (MAXB + QL - 1) / QL; // rows per lane
-    __shared__ double2 sp[SP_PITCH * MAXB];   // sp[l + r * SP_PITCH] = {S_lr, P_lr}
+    __shared__ double2 sp[sp_col(MAXB)];     // sp[sp_col(c) + l] = {S_lc, P_lc}, l < 16 (c / 16 + 1)
+    __shared__ double2 spd[MAXB];            // spd[c] = {S, P}_{c+1, c}
 
     const int wave = threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
@@ -691,6 +702,7 @@ __global__ __launch_bounds__(256) void k_solve_real(
     const bool pair = ud.w == 2;
     UnitStep us;
     us.sp = sp;
+    us.spd = spd;
     us.nb = nb;
     us.j = j;
     us.dl = act ? ud.d - i0 : -1 - MAXB; // inactive units: every row is a zero row
@@ -713,41 +725,45 @@ __global__ __launch_bounds__(256) void k_solve_real(
         raw.v[i].y = (ld && pair) ? xp[1] : 0.0;
     }
 
-    // stage {S,P} of the diagonal block: item (r, pair t) holds rows 2t, 2t+1 of column r
+    // stage {S,P} of the diagonal block in the packed layout (item it = sp index: column c, row
+    // l with sp_col(c) + l = it; a 16-column block qb starts at 128 qb (qb + 1)) and its
+    // subdiagonal
     {
-        constexpr int NPAIR = SP_PITCH / 2;
-        constexpr int NITEM = NPAIR * MAXB;
-        constexpr int BATCH = 12;
-        for (int q0 = threadIdx.x; q0 < NITEM; q0 += BATCH * blockDim.x) {
-            double2 ts[BATCH], tp[BATCH];
+        constexpr int NI = sp_col(MAXB);
+        constexpr int PER = 11;
+        for (int base = 0; base < NI; base += PER * (int)blockDim.x) {
+            double ts[PER], tp[PER];
 #pragma clang loop unroll(full)
-            for (int k = 0; k < BATCH; ++k) {
-                const int q = q0 + k * blockDim.x;
-                const int r = q / NPAIR, l = 2 * (q % NPAIR);
-                ts[k] = make_double2(0.0, 0.0);
-                tp[k] = make_double2(0.0, 0.0);
-                if (q < NITEM && r < nb && l < nb) {
-                    const double *sa = S + (i0 + l) + (long)(i0 + r) * lds;
-                    const double *pa = P + (i0 + l) + (long)(i0 + r) * ldp;
-                    if (l + 1 < nb) {
-                        ts[k] = *(const double2 *)sa;
-                        tp[k] = *(const double2 *)pa;
-                    }
-                    else { // last row of an odd-length column
-                        ts[k] = make_double2(sa[0], 0.0);
-                        tp[k] = make_double2(pa[0], 0.0);
-                    }
+            for (int k = 0; k < PER; ++k) {
+                const int it = base + (int)threadIdx.x + k * (int)blockDim.x;
+                const int qb = (it >= 256) + (it >= 768) + (it >= 1536) + (it >= 2560);
+                const int off = it - 128 * qb * (qb + 1);
+                // off / (16 (qb + 1)) for off < 256 (qb + 1): (off / 16) * ceil(2^16 / (qb + 1)) / 2^16
+                const int mg = qb == 0 ? 65536 : (qb == 1 ? 32768 : (qb == 2 ? 21846 : (qb == 3 ? 16384 : 13108)));
+                const int cc = ((off >> 4) * mg) >> 16;
+                const int l = off - 16 * (qb + 1) * cc;
+                const int c = 16 * qb + cc;
+                ts[k] = 0.0;
+                tp[k] = 0.0;
+                if (it < NI && c < nb && l < nb) {
+                    ts[k] = S[(i0 + l) + (long)(i0 + c) * lds];
+                    tp[k] = P[(i0 + l) + (long)(i0 + c) * ldp];
                 }
             }
 #pragma clang loop unroll(full)
-            for (int k = 0; k < BATCH; ++k) {
-                const int q = q0 + k * blockDim.x;
-                const int r = q / NPAIR, l = 2 * (q % NPAIR);
-                if (q < NITEM) {
-                    sp[l + r * SP_PITCH] = make_double2(ts[k].x, tp[k].x);
-                    sp[l + 1 + r * SP_PITCH] = make_double2(ts[k].y, tp[k].y);
+            for (int k = 0; k < PER; ++k) {
+                const int it = base + (int)threadIdx.x + k * (int)blockDim.x;
+                if (it < NI) {
+                    sp[it] = make_double2(ts[k], tp[k]);
                 }
             }
+        }
+        for (int c = threadIdx.x; c < MAXB; c += blockDim.x) {
+            double2 e = make_double2(0.0, 0.0);
+            if (c + 1 < nb) {
+                e = make_double2(S[(i0 + c + 1) + (long)(i0 + c) * lds], P[(i0 + c + 1) + (long)(i0 + c) * ldp]);
+            }
+            spd[c] = e;
         }
     }
     __syncthreads();
