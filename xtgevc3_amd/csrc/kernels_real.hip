@@ -1065,34 +1065,103 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
     }
 
     // epilogue: lane holds D[lane/16 + 4*v][lane%16] of each 16x16 tile (v_mfma_f64_16x16x4
-    // layout), D[4*(lane/16) + v][lane%16] for v_mfma_f32_16x16x4
+    // layout), D[4*(lane/16) + v][lane%16] for v_mfma_f32_16x16x4.  The lane's row and column
+    // indices are loaded first and the C reads of a row group are issued one group ahead of its
+    // writes, so that loads of one kind are in flight together (a wait per output would
+    // serialise their round trips).
+    // (each uniform choice is made once around a straight run of loads, and the C reads are
+    // unconditional at clamped addresses: per-load branches would each end in a wait)
+    constexpr int NA = 4 * FTA;
+    int rac[NA], rsa[NA], rkm[NA];
+    long roff[NA];
 #pragma clang loop unroll(full)
     for (int i = 0; i < FTA; ++i) {
 #pragma clang loop unroll(full)
         for (int vv = 0; vv < 4; ++vv) {
-            int ar = a0 + wa + i * 16 + (F32 ? 4 * fk + vv : fk + 4 * vv);
-            if (ar >= M) {
-                continue;
+            const int q = 4 * i + vv;
+            rac[q] = min(a0 + wa + i * 16 + (F32 ? 4 * fk + vv : fk + 4 * vv), M - 1);
+            roff[q] = (long)rac[q] * ldc;
+            rsa[q] = 0;
+            rkm[q] = 0;
+        }
+    }
+    if (outcol != nullptr) {
+#pragma clang loop unroll(full)
+        for (int q = 0; q < NA; ++q) {
+            roff[q] = (long)outcol[rac[q]] * ldc;
+        }
+    }
+    if (F32 && sa != nullptr) {
+#pragma clang loop unroll(full)
+        for (int q = 0; q < NA; ++q) {
+            rsa[q] = sa[rac[q]];
+        }
+    }
+    if (NFG && guard) {
+#pragma clang loop unroll(full)
+        for (int q = 0; q < NA; ++q) {
+            rkm[q] = kmax[rac[q]];
+        }
+    }
+    double csig[FTB];
+    int cbl[FTB], csb[FTB], cnf[FTB];
+#pragma clang loop unroll(full)
+    for (int j = 0; j < FTB; ++j) {
+        cbl[j] = min(b0 + wb + j * 16 + fr, N - 1);
+        csig[j] = 1.0;
+        csb[j] = 0;
+        cnf[j] = NFG ? nfk[cbl[j] - b0] : 0;
+    }
+    if (mode == 1) {
+#pragma clang loop unroll(full)
+        for (int j = 0; j < FTB; ++j) {
+            csig[j] = sig[cbl[j]];
+        }
+    }
+    if (F32 && sb != nullptr) {
+#pragma clang loop unroll(full)
+        for (int j = 0; j < FTB; ++j) {
+            csb[j] = sb[cbl[j]];
+        }
+    }
+    double cv[2][4][FTB];
+#pragma clang loop unroll(full)
+    for (int i = 0; i <= FTA; ++i) {
+        if (mode == 1 && i < FTA) {
+            // C of row group i (issued before the writes of group i - 1)
+#pragma clang loop unroll(full)
+            for (int vv = 0; vv < 4; ++vv) {
+#pragma clang loop unroll(full)
+                for (int j = 0; j < FTB; ++j) {
+                    cv[i & 1][vv][j] = C[roff[4 * i + vv] + cbl[j]];
+                }
             }
-            long rowoff = (long)(outcol ? outcol[ar] : ar) * ldc;
+        }
+        if (i == 0) {
+            continue;
+        }
+        const int g = i - 1;
+#pragma clang loop unroll(full)
+        for (int vv = 0; vv < 4; ++vv) {
+            const int q = 4 * g + vv;
+            const int ar = a0 + wa + g * 16 + (F32 ? 4 * fk + vv : fk + 4 * vv);
 #pragma clang loop unroll(full)
             for (int j = 0; j < FTB; ++j) {
-                int bc = b0 + wb + j * 16 + fr;
-                if (bc < N) {
-                    double d = (double)acc[i][j][vv];
+                const int bc = b0 + wb + j * 16 + fr;
+                if (ar < M && bc < N) {
+                    double d = (double)acc[g][j][vv];
                     if constexpr (F32) {
-                        d = ldexp(d, (sa != nullptr ? sa[ar] : 0) + (sb != nullptr ? sb[bc] : 0));
+                        d = ldexp(d, rsa[q] + csb[j]);
                     }
-                    double *cp = C + rowoff + bc;
                     if (mode == 1) {
-                        d = fma(sig[bc], *cp, d);
+                        d = fma(csig[j], cv[g & 1][vv][j], d);
                     }
                     if constexpr (NFG) {
-                        if (guard && nfk[bc - b0] < kmax[ar]) {
+                        if (guard && cnf[j] < rkm[q]) {
                             d = __builtin_nan("");
                         }
                     }
-                    *cp = d;
+                    C[roff[q] + bc] = d;
                 }
             }
         }

@@ -310,31 +310,87 @@ __global__ __launch_bounds__(WMW *WNW * 64) void k_gemm_i8(
             }
         }
     }
+    // epilogue, per group of 8 output rows (of a 32-row block i): the level sums (their
+    // accumulators end there), the row indices, then all of the group's C reads before its
+    // writes.  Loads are unconditional at clamped addresses, so that loads of one kind are in
+    // flight together (a wait per output would serialise their round trips).
+    int ceb[AN];
+    double csg[AN];
+#pragma clang loop unroll(full)
+    for (int j = 0; j < AN; ++j) {
+        ceb[j] = ER[min(b0 + wb + 32 * j + fr, N - 1)];
+        csg[j] = 1.0;
+    }
+    if (mode == 1 && sig != nullptr) {
+#pragma clang loop unroll(full)
+        for (int j = 0; j < AN; ++j) {
+            csg[j] = sig[min(b0 + wb + 32 * j + fr, N - 1)];
+        }
+    }
+    constexpr int EG = 8;
 #pragma clang loop unroll(full)
     for (int i = 0; i < AM; ++i) {
 #pragma clang loop unroll(full)
-        for (int v = 0; v < 16; ++v) {
-            const int a = a0 + wa + 32 * i + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-            if (a >= M) {
-                continue;
-            }
-            const long rowoff = (long)(outcol ? outcol[a] : a) * ldc;
-            const int ea = EL[a];
+        for (int h = 0; h < 16 / EG; ++h) {
+            double sm[EG][AN];
 #pragma clang loop unroll(full)
-            for (int j = 0; j < AN; ++j) {
-                const int b = b0 + wb + 32 * j + fr;
-                if (b < N) {
+            for (int u = 0; u < EG; ++u) {
+                const int v = EG * h + u;
+#pragma clang loop unroll(full)
+                for (int j = 0; j < AN; ++j) {
                     double sum = (double)acc[LV - 1][i][j][v];
 #pragma clang loop unroll(full)
                     for (int t = LV - 2; t >= 0; --t) {
                         sum = fma(sum, 1.0 / SPLIT_BASE, (double)acc[t][i][j][v]);
                     }
-                    double d = ldexp(sum * (1.0 / (SPLIT_BASE * SPLIT_BASE)), ea + ER[b] + 2);
-                    double *cp = C + rowoff + (long)b * cst;
-                    if (mode == 1) {
-                        d = fma(sig != nullptr ? sig[b] : 1.0, *cp, d);
+                    sm[u][j] = sum * (1.0 / (SPLIT_BASE * SPLIT_BASE));
+                }
+            }
+            int rrow[EG], rea[EG];
+#pragma clang loop unroll(full)
+            for (int u = 0; u < EG; ++u) {
+                const int v = EG * h + u;
+                rrow[u] = min(a0 + wa + 32 * i + 8 * (v / 4) + 4 * (lane / 32) + (v % 4), M - 1);
+                rea[u] = EL[rrow[u]];
+            }
+            if (outcol != nullptr) {
+#pragma clang loop unroll(full)
+                for (int u = 0; u < EG; ++u) {
+                    rrow[u] = outcol[rrow[u]];
+                }
+            }
+            double cv[EG][AN];
+#pragma clang loop unroll(full)
+            for (int u = 0; u < EG; ++u) {
+#pragma clang loop unroll(full)
+                for (int j = 0; j < AN; ++j) {
+                    cv[u][j] = 0.0;
+                }
+            }
+            if (mode == 1) {
+#pragma clang loop unroll(full)
+                for (int u = 0; u < EG; ++u) {
+#pragma clang loop unroll(full)
+                    for (int j = 0; j < AN; ++j) {
+                        const int bcl = min(b0 + wb + 32 * j + fr, N - 1);
+                        cv[u][j] = C[(long)rrow[u] * ldc + (long)bcl * cst];
                     }
-                    *cp = d;
+                }
+            }
+#pragma clang loop unroll(full)
+            for (int u = 0; u < EG; ++u) {
+                const int v = EG * h + u;
+                const int a = a0 + wa + 32 * i + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+#pragma clang loop unroll(full)
+                for (int j = 0; j < AN; ++j) {
+                    const int b = b0 + wb + 32 * j + fr;
+                    if (a < M && b < N) {
+                        double d = ldexp(sm[u][j], rea[u] + ceb[j] + 2);
+                        if (mode == 1) {
+                            d = fma(csg[j], cv[u][j], d);
+                        }
+                        C[(long)rrow[u] * ldc + (long)b * cst] = d;
+                    }
                 }
             }
         }
