@@ -39,7 +39,8 @@ struct UnitDesc {
     int d;                 // first row of the unit's own diagonal block (>= n: none)
     int lo;                // rows < lo lie outside the unit's system and stay zero
     int init;              // INIT_* mode
-    int pad1, pad2, pad3;
+    int idx;               // unit index (the solve's tile-ordered copy: -1 for an empty slot)
+    int pad2, pad3;
 };
 
 // The row blocks of one update group of a sweep, top-down: block j covers local rows

@@ -32,7 +32,7 @@ namespace xtg {
 
 template <int MAXB>
 __global__ void k_solve_real(const double *, long, const double *, long, int, int, unsigned long long, double *, long,
-                             double *, long, const UnitDesc *, const int4 *, int, int, int, int, double *, int *, int *,
+                             double *, long, const UnitDesc *, int, int, int, int, double *, int *, int *,
                              int *, int, int, double *, int, GroupInfo, MixedExp, SplitOut);
 __global__ void k_rowexp(const double *, long, const double *, long, int, int, int *, const int *);
 __global__ void k_tgsna_part(const double *, const double *, const double *, const double *, long, int, int,
@@ -340,7 +340,7 @@ struct DevBuf {
 };
 
 struct RealWork {
-    DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wblk, wgrp, xrel, xfin, desc, tiles, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
+    DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wblk, wgrp, xrel, xfin, desc, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
     DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp, bkpre, bovf, sfac;
     // backward of left / back-transformed vectors: working pencil, permuted operands, products
     DevBuf bfS, bfP, bfV, bfG, bfGc, bfGt, bfVt, bfZt, bfW, bfGh, bfdS, bfdP, bfk;
@@ -467,10 +467,19 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     ENSURE(ws.blkexp, sizeof(int) * (size_t)nsteps * ldx);
     ENSURE(ws.flag, sizeof(int) * 4);
 
-    ENSURE(ws.desc, sizeof(UnitDesc) * units.size());
-    ENSURE(ws.tiles, sizeof(int4) * tiles.size());
+    // the units in tile order (four per tile, idx = unit index, -1 and a copy of the tile's
+    // first unit for an empty slot): a solve wave reads its descriptors in one load
+    std::vector<UnitDesc> tdesc(4 * tiles.size());
+    for (size_t t = 0; t < tiles.size(); ++t) {
+        const int ut[4] = {tiles[t].x, tiles[t].y, tiles[t].z, tiles[t].w};
+        for (int q = 0; q < 4; ++q) {
+            tdesc[4 * t + q] = units[ut[q] >= 0 ? ut[q] : ut[0]];
+            tdesc[4 * t + q].idx = ut[q];
+        }
+    }
+    ENSURE(ws.desc, sizeof(UnitDesc) * tdesc.size());
     ENSURE(ws.rowstep, sizeof(int) * (size_t)n);
-    XTG_CHECK(hipMemcpyAsync(ws.desc.p, units.data(), sizeof(UnitDesc) * units.size(), hipMemcpyHostToDevice, st));
+    XTG_CHECK(hipMemcpyAsync(ws.desc.p, tdesc.data(), sizeof(UnitDesc) * tdesc.size(), hipMemcpyHostToDevice, st));
     MixedExp mx{nullptr, nullptr, nullptr, 0};
     const bool split = prec == XTGEVC3_PREC_SPLIT_I8;
     long qld = 0, qsl = 0;
@@ -503,7 +512,6 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
         hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.xrel.as<int>(), (long)m, -1000000); XTG_LAUNCH_CHECK("k_fill_int");
         mx = MixedExp{ws.wblk.as<int>(), ws.wgrp.as<int>(), ws.xrel.as<int>(), 1};
     }
-    XTG_CHECK(hipMemcpyAsync(ws.tiles.p, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice, st));
     XTG_CHECK(hipMemcpyAsync(ws.rowstep.p, rowstep.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     double *X = ws.X.as<double>();
     double *W = ws.W.as<double>();
@@ -606,7 +614,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
             int pr = prof_begin(st);
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_real<MAXB_REAL>),
                                dim3((ntl + wpb - 1) / wpb), dim3(wpb * WAVE), 0, st, Sw,
-                               ldsw, Pw, ldpw, i0, nb, bumps, X, ldx, Wb, ldw, ws.desc.as<UnitDesc>(), ws.tiles.as<int4>(),
+                               ldsw, Pw, ldpw, i0, nb, bumps, X, ldx, Wb, ldw, ws.desc.as<UnitDesc>(),
                                t_begin, t_end, u_begin, u_end, ws.sig.as<double>(), ws.colexp.as<int>(),
                                ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff, wr + woff,
                                sgb, s == gs ? 1 : 0, grp, mxb,
@@ -1088,8 +1096,7 @@ constexpr size_t NA_BYTES[] = {
     sizeof(int) * NA_N,                                        // colexp
     sizeof(int) * NA_STEPS * NA_LDX,                           // blkexp
     sizeof(int) * 4,                                           // flag
-    sizeof(UnitDesc) * NA_N,                                   // desc
-    sizeof(int4) * 2 * NA_N,                                   // tiles
+    sizeof(UnitDesc) * 4 * 2 * NA_N,                           // desc (tile order: four per tile)
     sizeof(int) * NA_N,                                        // rowstep
     sizeof(int) * NA_N,                                        // outcol
     sizeof(int) * NA_N,                                        // kmax
@@ -1132,7 +1139,7 @@ static hipError_t noalloc_setup(int dev)
     }
     RealWork &ws = g_na_ws[dev];
     DevBuf *slots[] = {&ws.norms, &ws.W,      &ws.sig,  &ws.sig_grp, &ws.colexp, &ws.blkexp,
-                       &ws.flag,  &ws.desc,   &ws.tiles, &ws.rowstep, &ws.outcol, &ws.kmax,
+                       &ws.flag,  &ws.desc,   &ws.rowstep, &ws.outcol, &ws.kmax,
                        &ws.vcol,  &ws.vw,     &ws.X,    &ws.Zt,      &ws.Sw,     &ws.Pw};
     static_assert(sizeof(slots) / sizeof(slots[0]) == sizeof(NA_BYTES) / sizeof(NA_BYTES[0]), "one slot per buffer");
     size_t off = 0;

@@ -197,6 +197,8 @@ template <int QR> struct RawRHS {
 struct UnitOut {
     bool act, pair;
     int ucols, col, i0;
+    int ce0;     // colexp of the unit's columns at entry (loaded at kernel start)
+    double sgp;  // sig_grp of the lane's column at entry
     double *X;
     long ldx;
     double *W;
@@ -375,6 +377,9 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
                 const double2 e = sp[sp_col(c) + eqr];
                 dc[m] = vsel(m > j && !(pos == dl && c == dl + 1), e, dc[m]);
             }
+            // materialise the masked entries here, once per slot (the masks would otherwise
+            // be re-evaluated in every row step that uses them)
+            asm volatile("" : "+v"(dc[m].x), "+v"(dc[m].y));
         }
         V xs[QL];  // the slot's solution entries (broadcast), for the batched update
         int xe[QL]; // rescale exponent of the slot at the time each was stored
@@ -491,7 +496,7 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
     }
     const bool starts_here = dl >= 0 && dl < nb;
     if (act) {
-        const int ce = colexp[col] + my_exp; // pair columns share their exponents
+        const int ce = o.ce0 + my_exp; // pair columns share their exponents
         if (starts_here && !first_in_group) {
             for (int k = w_s + nb + j; k < grp.wr; k += QL) {
                 double *w0 = W + (long)k * ldw + col;
@@ -565,7 +570,7 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
         const int ew = mw > 0.0 ? ilogb(mw) + 1 : -1000000;
         if (act && j < ucols) {
             const int p = col + j;
-            const int ce = colexp[p] + my_exp;
+            const int ce = o.ce0 + my_exp;
             const bool fresh = first_in_group || starts_here;
             mx.wblk[p] = min(max(ew, -1000), 1000);
             if (grp.last) {
@@ -610,7 +615,7 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
             if (grp.last) {
                 // the lower blocks the column reaches, each brought to the final scale by
                 // 2^(ce - blkexp) above
-                const int ce = colexp[p] + my_exp;
+                const int ce = o.ce0 + my_exp;
                 const int d = dl + i0;
                 int eg = ej;
                 for (int jb = 1; jb < GROUP_MAX_BLOCKS; ++jb) {
@@ -633,8 +638,8 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
         sig[p] = sg;
         // cumulative scale over the group's steps (the group's far update applies it);
         // a unit starting in this block has no earlier steps in the group
-        sig_grp[p] = (first_in_group || starts_here) ? sg : sig_grp[p] * sg;
-        const int ce = colexp[p] + my_exp;
+        sig_grp[p] = (first_in_group || starts_here) ? sg : o.sgp * sg;
+        const int ce = o.ce0 + my_exp;
         colexp[p] = ce;
         blkexp[p] = ce;
         if (my_exp != 0) {
@@ -655,17 +660,17 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
 //
 // A unit is one eigenvalue: a real one (one column) or a complex pair (two columns, real
 // and imaginary part).  A 16-lane row serves one unit: lane j keeps rows r = 16 i + j of the
-// block.  A wave holds four units (tiles[t]: unit indices, -1 = empty); waves whose active
-// units are all real sweep in real arithmetic, the others in complex arithmetic (real units
-// then carry a zero imaginary part).  Before the sweep each lane factors its own rows
-// independently: 1x1 rows get the reciprocal of the shifted diagonal (a S_rr - b P_rr,
-// perturbed to the rounding level when tiny), 2x2 bumps of S (rows r-1, r: bit r-1 of
-// `bumps`) get a pivoted LU, upper row lane: A = multiplier, B = 1 / pivot; lower row lane:
-// A = U12, B = 1 / U22.  Rows outside the unit's system get zero coefficients; the unit's own
-// block gets unit coefficients, its entries being preset to the starting vector.  The sweep
-// then runs bottom-up: the owner lanes turn the right-hand side into the solution with these
-// factors, broadcast it in the row (DPP), and every lane updates its rows above; the {S,P}
-// entries of the next column are fetched from LDS one row ahead.
+// block.  A wave holds four units (tdesc[4 t + q], their descriptors in tile order, idx = -1
+// for an empty slot); waves whose active units are all real sweep in real arithmetic, the
+// others in complex arithmetic (real units then carry a zero imaginary part).  Before the
+// sweep each lane factors its own rows independently: 1x1 rows get the reciprocal of the
+// shifted diagonal (a S_rr - b P_rr, perturbed to the rounding level when tiny), 2x2 bumps of
+// S (rows r-1, r: bit r-1 of `bumps`) a pivoted LU whose upper row takes 1 / pivot and whose
+// lower row takes 1 / U22 and the multiplier (unit_block).  Rows outside the unit's system get
+// zero coefficients; the unit's own block gets unit coefficients, its entries being preset to
+// the starting vector.  The sweep then runs bottom-up: the owner lane turns the right-hand side
+// into the solution, broadcasts it in the row (DPP), and the lanes update the rows above: the
+// row slot's own rows at every step, the slots above once per slot.
 // A solution entry above 2^958 makes the unit rescale by a power of two (down to 2^446),
 // recorded as an exponent (blkexp: the column's exponent after this step).
 // Group bookkeeping of the operands W (rows [0, wr): S part, [wr, 2 wr): P part; the group's
@@ -679,7 +684,7 @@ template <int MAXB>
 __global__ __launch_bounds__(256) void k_solve_real(
     const double *__restrict__ S, long lds, const double *__restrict__ P, long ldp, int i0, int nb,
     unsigned long long bumps, double *__restrict__ X, long ldx, double *__restrict__ W, long ldw,
-    const UnitDesc *__restrict__ units, const int4 *__restrict__ tiles, int t_begin, int t_end, int u_begin,
+    const UnitDesc *__restrict__ tdesc, int t_begin, int t_end, int u_begin,
     int u_end, double *__restrict__ sig, int *__restrict__ colexp, int *__restrict__ blkexp, int *__restrict__ flag,
     int w_s, int w_p, double *__restrict__ sig_grp, int first_in_group, GroupInfo grp, MixedExp mx, SplitOut sq)
 {
@@ -694,12 +699,14 @@ __global__ __launch_bounds__(256) void k_solve_real(
     const int cq = lane / QL;
     const int t = min(t_begin + (int)(blockIdx.x * (blockDim.x / WAVE)) + wave, t_end - 1);
     const bool wave_live = t_begin + (int)(blockIdx.x * (blockDim.x / WAVE)) + wave < t_end;
-    const int4 tl = tiles[t];
-    const int u = cq == 0 ? tl.x : (cq == 1 ? tl.y : (cq == 2 ? tl.z : tl.w));
+    const UnitDesc ud = tdesc[4 * t + cq];
+    const int u = ud.idx;
     const bool act = wave_live && u >= 0 && u >= u_begin && u < u_end;
-    const UnitDesc ud = units[act ? u : tl.x];
     const int col = ud.col;
     const bool pair = ud.w == 2;
+    // the unit's column exponent and group scale, read now rather than after the sweep
+    const int ce0 = act ? colexp[col] : 0;
+    const double sgp = (act && j < ud.w) ? sig_grp[col + j] : 1.0;
     UnitStep us;
     us.sp = sp;
     us.spd = spd;
@@ -771,6 +778,8 @@ __global__ __launch_bounds__(256) void k_solve_real(
     UnitOut o;
     o.act = act;
     o.pair = pair;
+    o.ce0 = ce0;
+    o.sgp = sgp;
     o.ucols = ud.w;
     o.col = col;
     o.i0 = i0;
@@ -1581,7 +1590,7 @@ __global__ void k_zero_int(int *__restrict__ X, long count)
 
 namespace xtg {
 template __global__ void k_solve_real<65>(const double *, long, const double *, long, int, int, unsigned long long,
-                                          double *, long, double *, long, const UnitDesc *, const int4 *, int, int,
+                                          double *, long, double *, long, const UnitDesc *, int, int,
                                           int, int, double *, int *, int *, int *, int, int, double *, int, GroupInfo,
                                           MixedExp, SplitOut);
 } // namespace xtg
