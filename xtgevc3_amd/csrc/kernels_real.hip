@@ -381,12 +381,17 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
             // be re-evaluated in every row step that uses them)
             asm volatile("" : "+v"(dc[m].x), "+v"(dc[m].y));
         }
-        V xs[QL];  // the slot's solution entries (broadcast), for the batched update
+        // the slot's solution entries (broadcast) times the shift coefficients, a x and b x, for
+        // the batched update: the products with S and P are formed from the scaled entries, as
+        // in a row step (S x or P x alone can exceed the double range on graded pencils where
+        // S (a x) and P (b x) do not)
+        V xa[QL], xb[QL];
         int xe[QL]; // rescale exponent of the slot at the time each was stored
         int es = 0;
 #pragma clang loop unroll(full)
         for (int m = 0; m < QL; ++m) {
-            xs[m] = A::mk(0.0, 0.0);
+            xa[m] = A::mk(0.0, 0.0);
+            xb[m] = A::mk(0.0, 0.0);
             xe[m] = 0;
         }
 #pragma clang loop unroll(full)
@@ -401,7 +406,9 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
                 if (__any(es != 0)) {
 #pragma clang loop unroll(full)
                     for (int m = 1; m < QL; ++m) {
-                        xs[m] = A::scal(xs[m], ldexp(1.0, xe[m] - es));
+                        const double sc = ldexp(1.0, xe[m] - es);
+                        xa[m] = A::scal(xa[m], sc);
+                        xb[m] = A::scal(xb[m], sc);
                     }
                 }
 #pragma clang loop unroll(full)
@@ -412,10 +419,10 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
 #pragma clang loop unroll(full)
                         for (int m = 1; m < QL; ++m) {
                             const double2 e = sp[sp_col(QL * i + m) + eqk];
-                            as = A::fmar(e.x, xs[m], as);
-                            ap = A::fmar(e.y, xs[m], ap);
+                            as = A::fmar(e.x, xa[m], as);
+                            ap = A::fmar(e.y, xb[m], ap);
                         }
-                        vr[k] = A::upd(vr[k], make_double2(1.0, 1.0), A::scal(as, a), A::bmul(bR, bI, ap));
+                        vr[k] = A::upd(vr[k], make_double2(1.0, 1.0), as, ap);
                     }
                 }
             }
@@ -440,9 +447,10 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
                 es += e;
                 x = A::bc(A::mul(t, f[i]), jr);
             }
-            xs[jr] = x;
-            xe[jr] = es;
             const V ax = A::scal(x, a), bx = A::bmul(bR, bI, x);
+            xa[jr] = ax;
+            xb[jr] = bx;
+            xe[jr] = es;
             vr[i] = vsel(j == jr, x, A::upd(vr[i], dc[jr], ax, bx));
             if (jr == 0) {
                 // row 16 i into the slots above (not into the preset row dl from column dl + 1)
