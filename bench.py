@@ -45,7 +45,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_inputs(n, seed, device, cluster=False):
+def make_inputs(n, seed, device, cluster=False, need_z=True):
     import torch
 
     from xtgevc3_amd import pencils as G
@@ -53,13 +53,15 @@ def make_inputs(n, seed, device, cluster=False):
     t0 = time.time()
     # cluster: diagonal ratios s_kk / p_kk drawn from 64 centres with relative spread 1e-6
     S, P, ar, ai, be = G.real_quasi_triangular(n, seed, cluster_groups=64 if cluster else 0)
-    g = torch.Generator(device=device)
-    g.manual_seed(seed + 1)
-    A = torch.randn(n, n, dtype=torch.float64, device=device, generator=g)
-    Q, R = torch.linalg.qr(A)
-    Q = Q * torch.sign(torch.diagonal(R))[None, :]
-    # column-major storage of Z == row-major storage of Z^T
-    dZ = Q.t().contiguous()
+    dZ = None
+    if need_z:
+        g = torch.Generator(device=device)
+        g.manual_seed(seed + 1)
+        A = torch.randn(n, n, dtype=torch.float64, device=device, generator=g)
+        Q, R = torch.linalg.qr(A)
+        Q = Q * torch.sign(torch.diagonal(R))[None, :]
+        # column-major storage of Z == row-major storage of Z^T
+        dZ = Q.t().contiguous()
     dS = torch.from_numpy(np.ascontiguousarray(S.T)).to(device)
     dP = torch.from_numpy(np.ascontiguousarray(P.T)).to(device)
     torch.cuda.synchronize()
@@ -491,11 +493,10 @@ def main():
 
     n = args.n
     dev = "cuda"
-    S, P, ar, ai, be, dS, dP, dZ = make_inputs(n, args.seed, dev, cluster=args.cluster)
-    dVR = torch.empty_like(dZ)
-    stream = torch.cuda.current_stream()
-
     back = args.howmny == "B"
+    S, P, ar, ai, be, dS, dP, dZ = make_inputs(n, args.seed, dev, cluster=args.cluster, need_z=back)
+    dVR = torch.empty((n, n), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
 
     def step():
         if back:
