@@ -100,7 +100,9 @@ int xtgevc3_ztgevc3_device_ex(char side, char howmny, const int *select, int n, 
  * counterpart in the reference): for the normalised output V (n x m) of
  * xtgevc3_dtgevc3_device(side='R', howmny = select ? 'S' : 'A') and nbatch cotangent batches
  * G (column b*m + c = dL/dV[:, c] of batch b), writes dL/dS and dL/dP of every batch
- * (n x n column-major matrices at dS + b*n*ldds, dP + b*n*lddp).  alpha_k = S_kk and
+ * (n x n column-major matrices at dS + b*n*ldds, dP + b*n*lddp).  Only the entries the forward
+ * reads carry a gradient: dL/dS on the upper Hessenberg band (row i <= column j + 1), dL/dP on
+ * the upper triangle (i <= j); every entry below is written as zero.  alpha_k = S_kk and
  * beta_k = P_kk are differentiated as functions of the diagonals.  Supports real, finite
  * selected eigenvalues only: info = XTGEVC3_INFO_UNSUPPORTED otherwise.  Device arrays,
  * host eigenvalues/select.  info > 0: that many (batch, column) adjoint vectors exceed the

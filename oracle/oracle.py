@@ -294,8 +294,9 @@ def implicit_grad_right(S, P, idx, V, G):
     vector (x_k = 1), v = x / max|x|: with g'' = g - (g.v) sign(v_p) e_p restricted to rows
     < k and lambda = T^{-T} g'' (T = (beta S - alpha P)[0:k, 0:k], dense solve):
         dL/dS[:k, :k+1] = -beta lambda v^T,  dL/dP[:k, :k+1] = alpha lambda v^T,
-        dL/dS[k,k] += lambda^T (P v)[:k],  dL/dP[k,k] += -lambda^T (S v)[:k].
-    Dense O(k^3) per column: small n only."""
+        dL/dS[k,k] += lambda^T (P v)[:k],  dL/dP[k,k] += -lambda^T (S v)[:k],
+    kept on the entries the forward reads (S: upper Hessenberg band, P: upper triangle; zero
+    below).  Dense O(k^3) per column: small n only."""
     n = S.shape[0]
     dS = np.zeros((n, n))
     dP = np.zeros((n, n))
@@ -314,4 +315,8 @@ def implicit_grad_right(S, P, idx, V, G):
         dP[:k, :k + 1] += al * np.outer(lam, v[:k + 1])
         dS[k, k] += lam @ (P[:k, :] @ v)
         dP[k, k] += -(lam @ (S[:k, :] @ v))
+    # the forward reads only the upper Hessenberg band of S and the upper triangle of P: entries
+    # below carry no gradient
+    dS *= np.triu(np.ones((n, n)), -1)
+    dP *= np.triu(np.ones((n, n)))
     return dS, dP

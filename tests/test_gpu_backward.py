@@ -71,6 +71,11 @@ def test_backward_matches_implicit_and_fd(n, nsel, seed, offdiag):
     mask = np.triu(np.ones((n, n))) != 0  # entries that keep the quasi-triangular structure
     cand = np.argwhere(mask)
     pts = cand[rng.choice(len(cand), size=min(60, len(cand)), replace=False)]
+    # and entries below the band, which the forward never reads: gradient and quotient are 0
+    low = np.argwhere(np.tril(np.ones((n, n)), -2) != 0)
+    pts = np.concatenate([pts, low[rng.choice(len(low), size=8, replace=False)]])
+    assert np.all(gS[np.tril(np.ones((n, n)), -2) != 0] == 0.0)
+    assert np.all(gP[np.tril(np.ones((n, n)), -1) != 0] == 0.0)
     worst = 0.0
     for which, base in (("S", S), ("P", P)):
         for (i, j) in pts:

@@ -232,6 +232,9 @@ def test_implicit_gradient_reference_matches_finite_differences(seed):
     rng = np.random.default_rng(seed + 7)
     entries = [(int(i), int(j)) for i, j in np.argwhere(np.triu(np.ones((n, n))) != 0)]
     entries = [entries[q] for q in rng.choice(len(entries), 40, replace=False)] + [(k, k) for k in idx]
+    # below the band the forward reads nothing: the difference quotient and the reference are 0
+    lower = [(int(i), int(j)) for i, j in np.argwhere(np.tril(np.ones((n, n)), -2) != 0)]
+    entries += [lower[q] for q in rng.choice(len(lower), 10, replace=False)]
     for i, j in entries:
         for which in ("S", "P"):
             Mp = (S if which == "S" else P).copy()

@@ -6,7 +6,9 @@ howmny='S' or 'A', no back-transform), for testing the hand-written backward ker
 S, P: (n, n) float64 CUDA tensors in ordinary (row-major) torch layout; alphar/alphai/beta:
 host numpy arrays (the reference's interface takes the eigenvalues as inputs).  For the
 selected real eigenvalues the backward differentiates alpha_k = S[k, k] and
-beta_k = P[k, k] as functions of the diagonals.  Returns V (n, m): column c is the
+beta_k = P[k, k] as functions of the diagonals; dL/dS is nonzero only on the upper Hessenberg
+band and dL/dP only on the upper triangle (the forward reads nothing below them).  Returns
+V (n, m): column c is the
 normalised right eigenvector of the c-th selected eigenvalue (LAPACK convention).
 precision (xtgevc3_amd.PREC_*) selects the form of the forward's and the backward's far
 updates: FP64, split-int8 or FP32 products.

@@ -91,6 +91,13 @@ __device__ __forceinline__ void xcd_tile(int l, int tm_n, int tn_n, int &tm, int
     tn = grp * 8 + rem % gw;
 }
 
+// s_waitcnt with vmcnt = N (0..63), other counters at their maximum (gfx9 encoding)
+template <int N> __device__ __forceinline__ void wait_vm()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
 // Split-int8 form: digits per value and digit-product levels kept (i + j < SPLIT_LEVELS).
 constexpr int SPLIT_DIGITS = 5;
 constexpr int SPLIT_LEVELS = 5;

@@ -63,14 +63,14 @@ __global__ void k_bwd_rhs(const double *, long, const double *, long, int, int, 
 __global__ void k_bwd_permute(const double *, long, int, int, int, int, int, double *, long);
 __global__ void k_bwd_gcorr(const double *, long, const double *, long, int, int, double *, long);
 __global__ void k_bwd_trunc(double *, long, int, int, const int *);
-__global__ void k_bwd_lambda_t(const double *, long, int, int, int, const int *, const double *, const int *, double *,
-                               double *, int *);
+__global__ void k_bwd_lambda_t(const double *, long, int, int, int, const int *, const int *, double *, long, int *,
+                               int *);
 __global__ void k_to_rowmajor(const double *, long, int, int, double *, long);
 __global__ void k_bwd_diag(const double *, long, const double *, long, int, int, int, const int *, const double *,
                            const int *, double *, long, double *, long);
 __global__ void k_subdiag(const double *, long, int, double *);
-__global__ void k_bwd_outer(int, int, int, const double *, long, const double *, long, const double *, const double *,
-                            double *, long, double *, long, const int *, long, long, long);
+__global__ void k_bwd_outer(int, int, const double *, long, const double *, long, const double *, const double *, double *,
+                            long, double *, long, const int *, const int *, const int *, long, long, long, long);
 __global__ void k_zero(double *, long);
 __global__ void k_zero_int(int *, long);
 
@@ -341,7 +341,7 @@ struct DevBuf {
 
 struct RealWork {
     DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wblk, wgrp, xrel, xfin, desc, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
-    DevBuf kcol, coef, Vt, Y, LamS, LamP, sig_grp, bkpre, bovf, sfac;
+    DevBuf kcol, coef, Vt, Y, LamS, bVp, bnfk, btiles, sig_grp, bkpre, bovf, sfac;
     // backward of left / back-transformed vectors: working pencil, permuted operands, products
     DevBuf bfS, bfP, bfV, bfG, bfGc, bfGt, bfVt, bfZt, bfW, bfGh, bfdS, bfdP, bfk;
     DevBuf Ut, YB, part, tcol, tw; // eigenvalue condition numbers
@@ -1151,7 +1151,7 @@ static hipError_t noalloc_setup(int dev)
     }
     // every other buffer of the workspace is a zero-capacity fixed slot: using one is an error
     DevBuf *unused[] = {&ws.rowexp, &ws.zexp, &ws.wblk, &ws.wgrp, &ws.xrel, &ws.xfin, &ws.kcol, &ws.coef,
-                        &ws.Vt,     &ws.Y,    &ws.LamS, &ws.LamP, &ws.bkpre, &ws.Ut,  &ws.YB,   &ws.part,
+                        &ws.Vt,     &ws.Y,    &ws.LamS, &ws.bVp, &ws.bnfk, &ws.btiles, &ws.bkpre, &ws.Ut,  &ws.YB,   &ws.part,
                         &ws.tcol,   &ws.tw,   &ws.Sq,   &ws.Pq,   &ws.Wg,    &ws.wgexp, &ws.wbexp, &ws.xbexp,
                         &ws.sstart, &ws.Xq,   &ws.Zq,   &ws.xexp, &ws.cfar};
     for (DevBuf *b : unused) {
@@ -1423,7 +1423,8 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
     ENSURE(ws.X, sizeof(double) * (size_t)n * ldx);
     ENSURE(ws.Vt, sizeof(double) * (size_t)n * ldt);
     ENSURE(ws.Y, sizeof(double) * (size_t)n * ldt);
-    ENSURE(ws.LamS, sizeof(double) * (size_t)M * n);
+    const long npad = (n + 1) & ~1; // rows of the outer products' operands (16-byte pairs)
+    ENSURE(ws.LamS, sizeof(double) * (size_t)M * npad);
     XTG_CHECK(hipMemcpyAsync(ws.kcol.p, kcol.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
     XTG_CHECK(hipMemcpyAsync(ws.coef.p, coef.data(), sizeof(double) * 4 * m, hipMemcpyHostToDevice, st));
     dim3 g32((n + 31) / 32, (n + 31) / 32);
@@ -1471,38 +1472,51 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
         prof_end(pr, st, 3, 2.0 * n * n * (double)m, 8.0 * ((double)n * n + 2.0 * n * m));
     }
     ENSURE(ws.bovf, sizeof(int) * (size_t)M);
+    ENSURE(ws.bnfk, sizeof(int) * (size_t)nbatch * npad);
     hipLaunchKernelGGL(k_zero_int, dim3((M + 255) / 256), dim3(256), 0, st, ws.bovf.as<int>(), (long)M); XTG_LAUNCH_CHECK("k_zero_int");
-    hipLaunchKernelGGL(k_bwd_lambda_t, dim3((M + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, st, (const double *)X, ldx,
-                       n, m, nbatch, ws.kcol.as<int>(), ws.coef.as<double>(), ws.colexp.as<int>(),
-                       ws.LamS.as<double>(), (double *)nullptr, ws.bovf.as<int>()); XTG_LAUNCH_CHECK("k_bwd_lambda_t");
+    hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.bnfk.as<int>(), (long)nbatch * npad, INT_MAX); XTG_LAUNCH_CHECK("k_fill_int");
+    hipLaunchKernelGGL(k_bwd_lambda_t, dim3((M + 31) / 32, (int)((npad + 31) / 32)), dim3(32, 8), 0, st,
+                       (const double *)X, ldx, n, m, nbatch, ws.kcol.as<int>(), ws.colexp.as<int>(),
+                       ws.LamS.as<double>(), npad, ws.bovf.as<int>(), ws.bnfk.as<int>()); XTG_LAUNCH_CHECK("k_bwd_lambda_t");
+    // the forward vectors with rows padded to npad (zero past n), 16-byte aligned columns
+    ENSURE(ws.bVp, sizeof(double) * (size_t)npad * m);
+    hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, st, ws.bVp.as<double>(), npad * m); XTG_LAUNCH_CHECK("k_zero");
+    hipLaunchKernelGGL(k_bwd_permute, dim3((n + 255) / 256, m), dim3(256), 0, st, V, ldv, n, m, 1, 0, 0,
+                       ws.bVp.as<double>(), npad); XTG_LAUNCH_CHECK("k_bwd_permute");
     // dS_b = V diag(-a) Lam_b, dP_b = V diag(b) Lam_b  (as D[j][i] = sum_c s_c V[j][c] Lam[c][i]),
     // all batches and both products in one launch.  v_c is zero below row kcol[c] and kcol
     // ascends, so with the columns taken in reverse order (negative strides) row j of the
     // product needs only the first kpre[j] = #{c : kcol[c] >= j} of them: a per-row K bound.
+    // Only the support is formed (dS: rows i <= j + 1, dP: i <= j; zeros elsewhere), where entry
+    // (i, j) needs the columns with kcol[c] >= max(j, i + 1) (lambda_c[i] = 0 for kcol[c] <= i).
     std::vector<int> kpre(n);
-    double kwork = 0.0;
     for (int j = 0, c = 0; j < n; ++j) {
         while (c < m && kcol[c] < j) {
             ++c;
         }
         kpre[j] = m - c;
-        kwork += kpre[j];
+    }
+    double kwork = 0.0; // sum over the dS support of the exact K range of each entry
+    for (int j = 0; j < n; ++j) {
+        kwork += (double)j * kpre[j] + (j + 1 < n ? kpre[j + 1] : 0) + (j + 2 < n ? kpre[j + 2] : 0);
     }
     ENSURE(ws.bkpre, sizeof(int) * (size_t)n);
     XTG_CHECK(hipMemcpyAsync(ws.bkpre.p, kpre.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     {
         // dS_b = V diag(-a) Lam_b and dP_b = V diag(b) Lam_b in one pass over V and Lam_b (the
         // columns in reverse order, so that row j needs only the first kpre[j] of them)
-        std::vector<double> sf(2 * (size_t)m);
+        // s1 = sf[0, m), s2 = sf[ms, ms + m), each followed by zeros (the kernel reads whole K steps)
+        const int ms = m + 16;
+        std::vector<double> sf(2 * (size_t)ms, 0.0);
         for (int k = 0; k < m; ++k) {
             const int c = m - 1 - k;
             sf[k] = -coef[4 * c + 0];
-            sf[m + k] = coef[4 * c + 1];
+            sf[ms + k] = coef[4 * c + 1];
         }
-        ENSURE(ws.sfac, sizeof(double) * 2 * (size_t)m);
-        XTG_CHECK(hipMemcpyAsync(ws.sfac.p, sf.data(), sizeof(double) * 2 * m, hipMemcpyHostToDevice, st));
-        const double *Vrev = V + (long)(m - 1) * ldv;
-        const long lrev = (long)(m - 1) * n;
+        ENSURE(ws.sfac, sizeof(double) * 2 * (size_t)ms);
+        XTG_CHECK(hipMemcpyAsync(ws.sfac.p, sf.data(), sizeof(double) * 2 * ms, hipMemcpyHostToDevice, st));
+        const double *Vrev = ws.bVp.as<double>() + (long)(m - 1) * npad;
+        const long lrev = (long)(m - 1) * npad;
         pr = prof_begin(st);
         if (n < 2) {
             // lambda has no rows (k = 0): the products vanish, only the diagonal terms remain
@@ -1510,14 +1524,43 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
             XTG_CHECK(hipMemset2DAsync(dP, sizeof(double) * lddp, 0, sizeof(double) * n, (size_t)nbatch * n, st));
         }
         else {
-            const dim3 grid(((n + 63) / 64) * ((n + 127) / 128), nbatch);
-            hipLaunchKernelGGL(k_bwd_outer, grid, dim3(256), 0, st, n, n, m, Vrev, -ldv,
-                               (const double *)(ws.LamS.as<double>() + lrev), -(long)n,
-                               (const double *)ws.sfac.as<double>(), (const double *)(ws.sfac.as<double>() + m), dS,
-                               ldds, dP, lddp, (const int *)ws.bkpre.as<int>(), (long)m * n, (long)n * ldds,
+            // tile order: tiles with work by descending K range, then those below the band
+            const int ta = (n + 127) / 128, tb = (n + 63) / 64;
+            std::vector<std::pair<int, int>> work; // (-K, packed tile)
+            std::vector<int> tiles;
+            for (int a = 0; a < ta; ++a) {
+                for (int b = 0; b < tb; ++b) {
+                    const int a0 = 128 * a, b0 = 64 * b;
+                    if (b0 > a0 + 128) {
+                        continue;
+                    }
+                    const int kk = std::min(kpre[a0], b0 + 1 < n ? kpre[b0 + 1] : 0);
+                    work.push_back({-kk, (a << 16) | b});
+                }
+            }
+            std::stable_sort(work.begin(), work.end(),
+                             [](const std::pair<int, int> &x, const std::pair<int, int> &y) { return x.first < y.first; });
+            for (const auto &w : work) {
+                tiles.push_back(w.second);
+            }
+            for (int a = 0; a < ta; ++a) {
+                for (int b = 0; b < tb; ++b) {
+                    if (64 * b > 128 * a + 128) {
+                        tiles.push_back((a << 16) | b);
+                    }
+                }
+            }
+            ENSURE(ws.btiles, sizeof(int) * tiles.size());
+            XTG_CHECK(hipMemcpyAsync(ws.btiles.p, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, st));
+            const dim3 grid((unsigned)tiles.size(), nbatch);
+            hipLaunchKernelGGL(k_bwd_outer, grid, dim3(256), 0, st, n, m, Vrev, -npad,
+                               (const double *)(ws.LamS.as<double>() + lrev), -npad,
+                               (const double *)ws.sfac.as<double>(), (const double *)(ws.sfac.as<double>() + ms), dS,
+                               ldds, dP, lddp, (const int *)ws.bkpre.as<int>(), (const int *)ws.bnfk.as<int>(),
+                               (const int *)ws.btiles.as<int>(), (long)m * npad, npad, (long)n * ldds,
                                (long)n * lddp); XTG_LAUNCH_CHECK("k_bwd_outer");
         }
-        prof_end(pr, st, 3, 4.0 * n * kwork * nbatch,
+        prof_end(pr, st, 3, 4.0 * kwork * nbatch,
                  8.0 * nbatch * ((double)n * m + (double)m * n + 2.0 * (double)n * n));
     }
     hipLaunchKernelGGL(k_bwd_diag, dim3(M), dim3(64), 0, st, (const double *)X, ldx, (const double *)ws.Y.as<double>(),

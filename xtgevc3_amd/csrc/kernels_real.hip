@@ -1188,50 +1188,91 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
 // ---------------------------------------------------------------------------------------
 // k_bwd_outer: both gradient products of the backward in one pass over their shared operands,
 //     D1[a][b] = sum_k s1[k] L[k][a] R[k][b],   D2[a][b] = sum_k s2[k] L[k][a] R[k][b],
-// written to C1 + a ldc1 + b and C2 + a ldc2 + b (L[k][a] = L[a + k ldl], R[k][b] = R[k ldr + b];
-// strides may be negative).  With L = the forward vectors V, R = the adjoint solutions lambda
-// and s1 = -a_c, s2 = b_c (columns in reverse order) these are dL/dS and dL/dP.  kmax[a]
-// bounds k per row a (exact per row: the tile runs to its rows' maximum, and a row's terms past
-// its own bound are exact zeros of L).  Non-finite R entries are staged as zero and an output
-// whose row range includes one becomes NaN, as in k_gemm_f64 (TAG 2).  Batches: gridDim.y,
-// R + y bsr, C1 + y bsc1, C2 + y bsc2.  Tile 128 x 64, K chunks of 16 double-buffered in LDS,
-// four waves of 64 x 32; the factors s1, s2 multiply the A fragment in registers.
+// written to C1 + a ldc1 + b and C2 + a ldc2 + b for a, b < n (L[k][a] = L[a + k ldl],
+// R[k][b] = R[k ldr + b]; strides may be negative, k < K).  With L = the forward vectors V,
+// R = the adjoint solutions lambda and s1 = -a_c, s2 = b_c (columns in reverse order) these are
+// dL/dS and dL/dP.  L and R rows hold npad = n rounded up to even entries (zero past n) at
+// 16-byte aligned addresses.  kmax[a] (non-increasing in a) bounds k per row a: a row's terms
+// past its own bound are exact zeros of L.  R holds finite values only: k_bwd_lambda_t stages
+// a non-finite lambda entry as zero and keeps in nfk[b] the first k of such an entry in column
+// b; an output whose exact range [0, kmax[a]) includes it becomes NaN.  Batches: gridDim.y,
+// R + y bsr, nfk + y bsn, C1 + y bsc1, C2 + y bsc2.  tiles[blockIdx.x] = (a tile << 16) | b tile:
+// the tiles with work in descending order of their K range, then the tiles below the band, so
+// that consecutive workgroups (which the hardware deals to the 8 XCDs in turn) spread the work
+// evenly over the XCDs, heaviest first.
+// Support: only the entries the forward reads carry a gradient, C1 (dL/dS) on the upper
+// Hessenberg band b <= a + 1, C2 (dL/dP) on the upper triangle b <= a; every other entry is
+// written as zero.  Tiles wholly below the band only write zeros, and a tile's K range also
+// stops at kmax[b0 + 1] = #{c : k_c > b0}: R[k][b] = lambda_c[b] is an exact zero for k_c <= b.
+// Tile 128 (a) x 64 (b), four waves of 64 x 32 per product, K steps of 16 rows staged by
+// direct global -> LDS loads (16 bytes per lane) into a 3-stage ring two steps ahead (no fence
+// at the stage barrier, so the loads stay in flight across it); the factors s1, s2 (scalar
+// loads per step, zero past the tile's K range; both arrays hold OB_GK entries past K) multiply
+// the A fragment in registers.
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bwd_outer(int M, int N, int K, const double *__restrict__ L, long ldl,
+constexpr int OB_GA = 128, OB_GB = 64, OB_GK = 16, OB_NST = 3;
+__global__ __launch_bounds__(256) void k_bwd_outer(int n, int K, const double *__restrict__ L, long ldl,
                                                    const double *__restrict__ R, long ldr,
                                                    const double *__restrict__ s1, const double *__restrict__ s2,
                                                    double *__restrict__ C1, long ldc1, double *__restrict__ C2,
-                                                   long ldc2, const int *__restrict__ kmax, long bsr, long bsc1,
-                                                   long bsc2)
+                                                   long ldc2, const int *__restrict__ kmax,
+                                                   const int *__restrict__ nfk, const int *__restrict__ tiles,
+                                                   long bsr, long bsn, long bsc1, long bsc2)
 {
-    constexpr int GTA = 128, GTB = 64, GK = 16;
+    constexpr int GTA = OB_GA, GTB = OB_GB, GK = OB_GK;
     constexpr int FTA = GTA / 32, FTB = GTB / 32;
+    constexpr int STAGE = GK * (GTA + GTB); // doubles per ring stage
+    constexpr int IPW = (GK * GTA / 128 + GK * GTB / 128) / 4; // 16-byte load instructions per wave and stage
+    static_assert(IPW == 6, "staging layout: 4 L rows and 2 R row pairs per wave");
     R += (long)blockIdx.y * bsr;
+    nfk += (long)blockIdx.y * bsn;
     C1 += (long)blockIdx.y * bsc1;
     C2 += (long)blockIdx.y * bsc2;
-    __shared__ double Ls[2][GK][GTA];
-    __shared__ double Rs[2][GK][GTB];
-    __shared__ double Ss[2][2][GK];
-    __shared__ int nfk[GTB];
-    __shared__ int kred;
-    int tm, tn;
-    xcd_tile(blockIdx.x, (M + GTA - 1) / GTA, (N + GTB - 1) / GTB, tm, tn);
-    const int a0 = tm * GTA, b0 = tn * GTB;
+    __shared__ __attribute__((aligned(16))) double lds[OB_NST * STAGE];
+    const int npad = (n + 1) & ~1;
+    const int tp = tiles[blockIdx.x];
+    const int a0 = (tp >> 16) * GTA, b0 = (tp & 0xffff) * GTB;
     const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
     const int wa = (wave >> 1) * (GTA / 2), wb = (wave & 1) * (GTB / 2);
-    if (tid == 0) {
-        kred = 0;
+    if (b0 > a0 + GTA) {
+        // below the band: b > a + 1 for every entry of the tile
+        for (int e = tid; e < GTA * GTB; e += 256) {
+            const int ar = a0 + e / GTB, bc = b0 + e % GTB;
+            if (ar < n && bc < n) {
+                C1[(long)ar * ldc1 + bc] = 0.0;
+                C2[(long)ar * ldc2 + bc] = 0.0;
+            }
+        }
+        return;
     }
-    if (tid < GTB) {
-        nfk[tid] = INT_MAX;
-    }
-    __syncthreads();
-    if (tid < GTA && a0 + tid < M) {
-        atomicMax(&kred, kmax[a0 + tid]);
-    }
-    __syncthreads();
-    const int KK = min(K, kred);
-    const int nchunks = (KK + GK - 1) / GK;
+    const int KK = min(min(K, kmax[a0]), b0 + 1 < n ? kmax[b0 + 1] : 0);
+    const int nst = (KK + GK - 1) / GK;
+
+    // per-lane sources: L rows 4 u + wave (u < 4), lane pair 2 lane of the row; R row pairs
+    // 2 t, 2 t + 1 with t = 4 u + wave (u < 2), lanes 0-31 the first row, 32-63 the second.
+    // Pairs past npad are clamped (they only feed outputs that are never stored).
+    const double *lsrc = L + min(a0 + 2 * lane, npad - 2);
+    const double *rsrc = R + min(b0 + 2 * (lane & 31), npad - 2);
+    auto issue = [&](int st) {
+        const int k0 = st * GK;
+        double *stage = lds + (st % OB_NST) * STAGE;
+#pragma clang loop unroll(full)
+        for (int u = 0; u < 4; ++u) {
+            const int r = 4 * u + wave;
+            const double *g = lsrc + (long)min(k0 + r, K - 1) * ldl;
+            __builtin_amdgcn_global_load_lds((const void *)g,
+                                             (__attribute__((address_space(3))) void *)(stage + r * GTA), 16, 0, 0);
+        }
+#pragma clang loop unroll(full)
+        for (int u = 0; u < 2; ++u) {
+            const int t = 4 * u + wave;
+            const int r = 2 * t + (lane >> 5);
+            const double *g = rsrc + (long)min(k0 + r, K - 1) * ldr;
+            __builtin_amdgcn_global_load_lds((const void *)g,
+                                             (__attribute__((address_space(3))) void *)(stage + GK * GTA + 2 * t * GTB),
+                                             16, 0, 0);
+        }
+    };
 
     double4_t acc1[FTA][FTB], acc2[FTA][FTB];
 #pragma clang loop unroll(full)
@@ -1245,92 +1286,57 @@ __global__ __launch_bounds__(256) void k_bwd_outer(int M, int N, int K, const do
             }
         }
     }
-    // staging: L column pair cl, rows lkl + 4 s (s < 4); R column pair cr, rows lkr + 8 s (s < 2)
-    constexpr int RSL = 256 / (GTA / 2), NSL = GK / RSL, RSR = 256 / (GTB / 2), NSR = GK / RSR;
-    const int cl = 2 * (tid % (GTA / 2)), lkl = tid / (GTA / 2);
-    const int cr = 2 * (tid % (GTB / 2)), lkr = tid / (GTB / 2);
-    const int aa = min(a0 + cl, M - 2), bb = min(b0 + cr, N - 2);
-    const bool odd_a = a0 + cl == M - 1, odd_b = b0 + cr == N - 1;
-    double2 lreg[NSL], rreg[NSR];
-    double sreg = 0.0;
-    int kbase = 0, valid = 0;
-    auto load_chunk = [&](int ch) {
-        const int k0 = ch * GK;
-        kbase = k0;
-        valid = KK - k0;
-#pragma clang loop unroll(full)
-        for (int q = 0; q < NSL; ++q) {
-            const int k = min(k0 + lkl + RSL * q, KK - 1);
-            const double *lp = L + aa + (long)k * ldl;
-            lreg[q] = make_double2(lp[0], lp[1]);
-        }
-#pragma clang loop unroll(full)
-        for (int q = 0; q < NSR; ++q) {
-            const int k = min(k0 + lkr + RSR * q, KK - 1);
-            rreg[q] = *(const double2 *)(R + (long)k * ldr + bb);
-        }
-        if (tid < 2 * GK) {
-            const int k = min(k0 + (tid % GK), KK - 1);
-            sreg = tid < GK ? s1[k] : s2[k];
-        }
-    };
-    auto store_chunk = [&](int buf) {
-#pragma clang loop unroll(full)
-        for (int q = 0; q < NSL; ++q) {
-            const int r = lkl + RSL * q;
-            double2 lv = lreg[q];
-            lv.x = odd_a ? lv.y : lv.x;
-            if (!(r < valid)) {
-                lv = make_double2(0.0, 0.0);
-            }
-            *(double2 *)&Ls[buf][r][cl] = lv;
-        }
-#pragma clang loop unroll(full)
-        for (int q = 0; q < NSR; ++q) {
-            const int r = lkr + RSR * q;
-            double2 rv = rreg[q];
-            rv.x = odd_b ? rv.y : rv.x;
-            if (!(r < valid)) {
-                rv = make_double2(0.0, 0.0);
-            }
-            if (!(isfinite(rv.x) && isfinite(rv.y))) {
-                if (!isfinite(rv.x)) {
-                    atomicMin(&nfk[cr], kbase + r);
-                    rv.x = 0.0;
-                }
-                if (!isfinite(rv.y)) {
-                    atomicMin(&nfk[cr + 1], kbase + r);
-                    rv.y = 0.0;
-                }
-            }
-            *(double2 *)&Rs[buf][r][cr] = rv;
-        }
-        if (tid < 2 * GK) {
-            Ss[buf][tid / GK][tid % GK] = (tid % GK) < valid ? sreg : 0.0;
-        }
-    };
-    if (nchunks > 0) {
-        load_chunk(0);
-        store_chunk(0);
+    if (nst > 0) {
+        issue(0);
     }
-    __syncthreads();
+    if (nst > 1) {
+        issue(1);
+    }
     const int fr = lane % 16, fk = lane / 16;
-    for (int ch = 0; ch < nchunks; ++ch) {
-        const int buf = ch & 1;
-        if (ch + 1 < nchunks) {
-            load_chunk(ch + 1);
+    for (int st = 0; st < nst; ++st) {
+        if (st + 1 < nst) {
+            wait_vm<IPW>();
+        }
+        else {
+            wait_vm<0>();
+        }
+        // every wave's loads of this stage have landed and every wave is done with the stage
+        // refilled next
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_sched_barrier(0);
+        if (st + 2 < nst) {
+            issue(st + 2);
+        }
+        const double *Ls = lds + (st % OB_NST) * STAGE;
+        const double *Rs = Ls + GK * GTA;
+        const int k0 = st * GK;
+        // the step's factors (unconditional uniform loads: s1 and s2 hold GK entries past K),
+        // zero past the tile's K range
+        double f1v[GK], f2v[GK];
+#pragma clang loop unroll(full)
+        for (int q = 0; q < GK; ++q) {
+            f1v[q] = s1[k0 + q];
+            f2v[q] = s2[k0 + q];
+        }
+#pragma clang loop unroll(full)
+        for (int q = 0; q < GK; ++q) {
+            if (k0 + q >= KK) {
+                f1v[q] = 0.0;
+                f2v[q] = 0.0;
+            }
         }
 #pragma clang loop unroll(full)
         for (int ks = 0; ks < GK; ks += 4) {
-            const double f1 = Ss[buf][0][ks + fk], f2 = Ss[buf][1][ks + fk];
+            const double f1 = fk == 0 ? f1v[ks] : fk == 1 ? f1v[ks + 1] : fk == 2 ? f1v[ks + 2] : f1v[ks + 3];
+            const double f2 = fk == 0 ? f2v[ks] : fk == 1 ? f2v[ks + 1] : fk == 2 ? f2v[ks + 2] : f2v[ks + 3];
             double bf[FTB];
 #pragma clang loop unroll(full)
             for (int j = 0; j < FTB; ++j) {
-                bf[j] = Rs[buf][ks + fk][wb + j * 16 + fr];
+                bf[j] = Rs[(ks + fk) * GTB + wb + j * 16 + fr];
             }
 #pragma clang loop unroll(full)
             for (int i = 0; i < FTA; ++i) {
-                const double af = Ls[buf][ks + fk][wa + i * 16 + fr];
+                const double af = Ls[(ks + fk) * GTA + wa + i * 16 + fr];
                 const double a1 = af * f1, a2 = af * f2;
 #pragma clang loop unroll(full)
                 for (int j = 0; j < FTB; ++j) {
@@ -1339,27 +1345,31 @@ __global__ __launch_bounds__(256) void k_bwd_outer(int M, int N, int K, const do
                 }
             }
         }
-        if (ch + 1 < nchunks) {
-            store_chunk(buf ^ 1);
-        }
-        __syncthreads();
+    }
+    // epilogue: the first non-finite k of each of the lane's columns, then the outputs
+    int cnf[FTB];
+#pragma clang loop unroll(full)
+    for (int j = 0; j < FTB; ++j) {
+        cnf[j] = nfk[min(b0 + wb + j * 16 + fr, n - 1)];
     }
 #pragma clang loop unroll(full)
     for (int i = 0; i < FTA; ++i) {
 #pragma clang loop unroll(full)
         for (int vv = 0; vv < 4; ++vv) {
             const int ar = a0 + wa + i * 16 + fk + 4 * vv;
-            if (ar >= M) {
+            if (ar >= n) {
                 continue;
             }
             const int kr = kmax[ar];
 #pragma clang loop unroll(full)
             for (int j = 0; j < FTB; ++j) {
                 const int bc = b0 + wb + j * 16 + fr;
-                if (bc < N) {
-                    const bool bad = nfk[bc - b0] < kr;
-                    C1[(long)ar * ldc1 + bc] = bad ? __builtin_nan("") : acc1[i][j][vv];
-                    C2[(long)ar * ldc2 + bc] = bad ? __builtin_nan("") : acc2[i][j][vv];
+                if (bc < n) {
+                    const bool bad = cnf[j] < kr;
+                    const double v1 = bad ? __builtin_nan("") : acc1[i][j][vv];
+                    const double v2 = bad ? __builtin_nan("") : acc2[i][j][vv];
+                    C1[(long)ar * ldc1 + bc] = bc <= ar + 1 ? v1 : 0.0;
+                    C2[(long)ar * ldc2 + bc] = bc <= ar ? v2 : 0.0;
                 }
             }
         }
@@ -1689,14 +1699,14 @@ __global__ __launch_bounds__(256) void k_bwd_rhs(const double *__restrict__ V, l
     }
 }
 
-// Lam[(b*m + c) * n + i] = coef * 2^-e * lambda_i  for coef = -a_c (S part) and b_c (P part)
-// (LamP == nullptr: LamS receives 2^-e * lambda_i itself); 32x32 LDS tiles over (working column
-// q, row i).
-// ovf[q] = 1 when lambda of working column q is not representable (2^-e overflows it).
+// Lam[(b*m + c) * ldl + i] = 2^-e * lambda_i of working column q (batch b, forward column c),
+// for i < ldl (zero for i >= n: ldl pads the rows), in 32x32 LDS tiles over (q, i).  A
+// non-finite entry is written as zero, ovf[q] = 1 marks the column, and nfk[b * ldl + i] keeps
+// the smallest reversed column index m - 1 - c of such an entry in row i (the K index of the
+// outer products, which take the columns in reverse order; nfk starts at INT_MAX).
 __global__ void k_bwd_lambda_t(const double *__restrict__ X, long ldx, int n, int m, int nbatch,
-                               const int *__restrict__ kcol, const double *__restrict__ coef,
-                               const int *__restrict__ colexp, double *__restrict__ LamS, double *__restrict__ LamP,
-                               int *__restrict__ ovf)
+                               const int *__restrict__ kcol, const int *__restrict__ colexp,
+                               double *__restrict__ Lam, long ldl, int *__restrict__ ovf, int *__restrict__ nfk)
 {
     __shared__ double tile[32][33];
     const int q0 = blockIdx.x * 32, i0 = blockIdx.y * 32;
@@ -1709,9 +1719,6 @@ __global__ void k_bwd_lambda_t(const double *__restrict__ X, long ldx, int n, in
             int c = m - 1 - q / nbatch;
             if (i < kcol[c]) {
                 v = ldexp(X[(long)(n - 1 - i) * ldx + q], -colexp[q]);
-                if (!isfinite(v)) {
-                    ovf[q] = 1;
-                }
             }
         }
         tile[kk][tx] = v;
@@ -1719,17 +1726,15 @@ __global__ void k_bwd_lambda_t(const double *__restrict__ X, long ldx, int n, in
     __syncthreads();
     for (int kk = ty; kk < 32; kk += 8) {
         int q = q0 + kk, i = i0 + tx;
-        if (i < n && q < M) {
+        if (i < ldl && q < M) {
             int c = m - 1 - q / nbatch, b = q % nbatch;
             double v = tile[tx][kk];
-            long o = (long)(b * m + c) * n + i;
-            if (LamP == nullptr) {
-                LamS[o] = v; // lambda itself (the fused outer products apply -a_c, b_c)
+            if (!isfinite(v)) {
+                ovf[q] = 1;
+                atomicMin(&nfk[(long)b * ldl + i], m - 1 - c);
+                v = 0.0;
             }
-            else {
-                LamS[o] = -coef[4 * c + 0] * v;
-                LamP[o] = coef[4 * c + 1] * v;
-            }
+            Lam[(long)(b * m + c) * ldl + i] = v; // lambda itself (the outer products apply -a_c, b_c)
         }
     }
 }

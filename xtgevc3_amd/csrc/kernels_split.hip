@@ -167,13 +167,6 @@ constexpr int QK = 32;
 constexpr int NSTAGE = 3;
 constexpr int SPLIT_KMAX = 26624; // 5 * 127^2 * K < 2^31
 
-// s_waitcnt with vmcnt = N (0..63), other counters at their maximum (gfx9 encoding)
-template <int N> __device__ __forceinline__ void wait_vm()
-{
-    static_assert(N >= 0 && N < 64, "vmcnt");
-    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
-}
-
 template <int BM, int BN, int WMW, int WNW>
 __global__ __launch_bounds__(WMW *WNW * 64) void k_gemm_i8(
     int M, int N, int K1, int K2, const signed char *__restrict__ L1, const signed char *__restrict__ L2, long lslice,
