@@ -349,7 +349,7 @@ struct RealWork {
     // split-int8 form: side stream (default priority) for the digit splitting of the pencil
     // and of Z, overlapped with the sweep; events order it against the caller's stream
     hipStream_t side = nullptr;
-    hipEvent_t ev_start = nullptr, ev_pencil = nullptr, ev_z = nullptr;
+    hipEvent_t ev_start = nullptr, ev_pencil = nullptr, ev_z = nullptr, ev_y = nullptr;
     // far-update lookahead: the rows above the next group run on `fs`
     hipStream_t fs = nullptr;
     hipEvent_t ev_wready[2] = {nullptr, nullptr}, ev_fdone[2] = {nullptr, nullptr};
@@ -752,7 +752,7 @@ static hipError_t split_side_init(RealWork &ws)
             ws.side = nullptr;
             return e;
         }
-        for (hipEvent_t *ev : {&ws.ev_start, &ws.ev_pencil, &ws.ev_z}) {
+        for (hipEvent_t *ev : {&ws.ev_start, &ws.ev_pencil, &ws.ev_z, &ws.ev_y}) {
             e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
             if (e != hipSuccess) {
                 return e;
@@ -1425,6 +1425,7 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
     ENSURE(ws.Y, sizeof(double) * (size_t)n * ldt);
     const long npad = (n + 1) & ~1; // rows of the outer products' operands (16-byte pairs)
     ENSURE(ws.LamS, sizeof(double) * (size_t)M * npad);
+    ENSURE(ws.bVp, sizeof(double) * (size_t)npad * m);
     XTG_CHECK(hipMemcpyAsync(ws.kcol.p, kcol.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
     XTG_CHECK(hipMemcpyAsync(ws.coef.p, coef.data(), sizeof(double) * 4 * m, hipMemcpyHostToDevice, st));
     dim3 g32((n + 31) / 32, (n + 31) / 32);
@@ -1456,21 +1457,33 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
         prof_end(pr, ws.side, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * 4.0 * n * (double)n + 2.0 * SPLIT_DIGITS * qsl);
         XTG_CHECK(hipEventRecord(ws.ev_pencil, ws.side));
     }
+    else {
+        XTG_CHECK(split_side_init(ws));
+        XTG_CHECK(hipEventRecord(ws.ev_start, st));
+        XTG_CHECK(hipStreamWaitEvent(ws.side, ws.ev_start, 0));
+    }
+    // beside the sweep, on the side stream (inputs only): Y = P V (the alpha / beta terms need
+    // lambda^T P v) and the forward vectors with rows padded to npad (zero past n, 16-byte aligned
+    // columns) for the outer products
+    hipLaunchKernelGGL(k_to_rowmajor, dim3((m + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, ws.side, V, ldv, n, m,
+                       ws.Vt.as<double>(), ldt); XTG_LAUNCH_CHECK("k_to_rowmajor");
+    {
+        pr = prof_begin(ws.side);
+        launch_gemm_f64(2, ws.side, n, m, n, 0, P, ldp, (const double *)nullptr, 0L,
+                           (const double *)ws.Vt.as<double>(), ldt, ws.Y.as<double>(), ldt, (const int *)nullptr, 0,
+                           (const double *)nullptr, (const int *)nullptr, (const double *)nullptr, 0, nullptr, nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+        prof_end(pr, ws.side, 3, 2.0 * n * n * (double)m, 8.0 * ((double)n * n + 2.0 * n * m));
+    }
+    hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, ws.side, ws.bVp.as<double>(), npad * m); XTG_LAUNCH_CHECK("k_zero");
+    hipLaunchKernelGGL(k_bwd_permute, dim3((n + 255) / 256, m), dim3(256), 0, ws.side, V, ldv, n, m, 1, 0, 0,
+                       ws.bVp.as<double>(), npad); XTG_LAUNCH_CHECK("k_bwd_permute");
+    XTG_CHECK(hipEventRecord(ws.ev_y, ws.side));
     int rc = sweep_real(ws, n, ws.Sw.as<double>(), n, ws.Pw.as<double>(), n, subw, desc, true, ldx, st, prec,
                         split ? ws.ev_pencil : nullptr);
     if (rc) {
         return rc;
     }
-    // Y = P V  (the alpha / beta terms need lambda^T P v)
-    hipLaunchKernelGGL(k_to_rowmajor, dim3((m + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, st, V, ldv, n, m,
-                       ws.Vt.as<double>(), ldt); XTG_LAUNCH_CHECK("k_to_rowmajor");
-    {
-        pr = prof_begin(st);
-        launch_gemm_f64(2, st, n, m, n, 0, P, ldp, (const double *)nullptr, 0L,
-                           (const double *)ws.Vt.as<double>(), ldt, ws.Y.as<double>(), ldt, (const int *)nullptr, 0,
-                           (const double *)nullptr, (const int *)nullptr, (const double *)nullptr, 0, nullptr, nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
-        prof_end(pr, st, 3, 2.0 * n * n * (double)m, 8.0 * ((double)n * n + 2.0 * n * m));
-    }
+    XTG_CHECK(hipStreamWaitEvent(st, ws.ev_y, 0));
     ENSURE(ws.bovf, sizeof(int) * (size_t)M);
     ENSURE(ws.bnfk, sizeof(int) * (size_t)nbatch * npad);
     hipLaunchKernelGGL(k_zero_int, dim3((M + 255) / 256), dim3(256), 0, st, ws.bovf.as<int>(), (long)M); XTG_LAUNCH_CHECK("k_zero_int");
@@ -1478,11 +1491,6 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
     hipLaunchKernelGGL(k_bwd_lambda_t, dim3((M + 31) / 32, (int)((npad + 31) / 32)), dim3(32, 8), 0, st,
                        (const double *)X, ldx, n, m, nbatch, ws.kcol.as<int>(), ws.colexp.as<int>(),
                        ws.LamS.as<double>(), npad, ws.bovf.as<int>(), ws.bnfk.as<int>()); XTG_LAUNCH_CHECK("k_bwd_lambda_t");
-    // the forward vectors with rows padded to npad (zero past n), 16-byte aligned columns
-    ENSURE(ws.bVp, sizeof(double) * (size_t)npad * m);
-    hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, st, ws.bVp.as<double>(), npad * m); XTG_LAUNCH_CHECK("k_zero");
-    hipLaunchKernelGGL(k_bwd_permute, dim3((n + 255) / 256, m), dim3(256), 0, st, V, ldv, n, m, 1, 0, 0,
-                       ws.bVp.as<double>(), npad); XTG_LAUNCH_CHECK("k_bwd_permute");
     // dS_b = V diag(-a) Lam_b, dP_b = V diag(b) Lam_b  (as D[j][i] = sum_c s_c V[j][c] Lam[c][i]),
     // all batches and both products in one launch.  v_c is zero below row kcol[c] and kcol
     // ascends, so with the columns taken in reverse order (negative strides) row j of the
@@ -1563,7 +1571,7 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
         prof_end(pr, st, 3, 4.0 * kwork * nbatch,
                  8.0 * nbatch * ((double)n * m + (double)m * n + 2.0 * (double)n * n));
     }
-    hipLaunchKernelGGL(k_bwd_diag, dim3(M), dim3(64), 0, st, (const double *)X, ldx, (const double *)ws.Y.as<double>(),
+    hipLaunchKernelGGL(k_bwd_diag, dim3((M + 15) / 16), dim3(256), 0, st, (const double *)X, ldx, (const double *)ws.Y.as<double>(),
                        ldt, n, m, nbatch, ws.kcol.as<int>(), ws.coef.as<double>(), ws.colexp.as<int>(), dS, ldds, dP,
                        lddp); XTG_LAUNCH_CHECK("k_bwd_diag");
     XTG_CHECK(hipGetLastError());

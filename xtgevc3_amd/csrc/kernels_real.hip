@@ -1820,26 +1820,53 @@ __global__ void k_to_rowmajor(const double *__restrict__ V, long ldv, int n, int
     }
 }
 
-// One wave per (working column q): dL/dalpha and dL/dbeta folded into dS_kk, dP_kk.
-//   Y = P V (row-major, ldy), lambda_i = 2^-e X[n-1-i][q].
-__global__ __launch_bounds__(64) void k_bwd_diag(const double *__restrict__ X, long ldx, const double *__restrict__ Y,
-                                                 long ldy, int n, int m, int nbatch, const int *__restrict__ kcol,
-                                                 const double *__restrict__ coef, const int *__restrict__ colexp,
-                                                 double *__restrict__ dS, long ldds, double *__restrict__ dP, long lddp)
+// dL/dalpha and dL/dbeta of working column q folded into dS_kk, dP_kk, with
+//   Y = P V (row-major, ldy), lambda_i = 2^-e X[n-1-i][q]:
+// 16 working columns per workgroup, thread t = column t % 16 and row phase t / 16 (a row of X is
+// read as 16 contiguous values), the rows i < k split over the 16 phases (i = phase + 16 r, four
+// running sums per thread), the partial sums added in fixed order.
+constexpr int BD_QC = 16;
+__global__ __launch_bounds__(256) void k_bwd_diag(const double *__restrict__ X, long ldx, const double *__restrict__ Y,
+                                                  long ldy, int n, int m, int nbatch, const int *__restrict__ kcol,
+                                                  const double *__restrict__ coef, const int *__restrict__ colexp,
+                                                  double *__restrict__ dS, long ldds, double *__restrict__ dP, long lddp)
 {
-    const int q = blockIdx.x;
-    const int c = m - 1 - q / nbatch, b = q % nbatch;
-    const int k = kcol[c];
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < k; i += WAVE) {
-        acc = fma(X[(long)(n - 1 - i) * ldx + q], Y[(long)i * ldy + c], acc);
-    }
+    constexpr int QC = BD_QC, PH = 256 / BD_QC;
+    __shared__ double part[PH][QC];
+    const int col = threadIdx.x % QC, ph = threadIdx.x / QC;
+    const int M = m * nbatch;
+    const int q = blockIdx.x * QC + col;
+    const bool live = q < M;
+    const int c = live ? m - 1 - q / nbatch : 0;
+    const int k = live ? kcol[c] : 0;
+    int kw = k; // the largest row count of the workgroup's columns (each wave holds all of them)
+#pragma clang loop unroll(full)
     for (int off = WAVE / 2; off > 0; off >>= 1) {
-        acc += __shfl_xor(acc, off);
+        kw = max(kw, __shfl_xor(kw, off));
     }
-    if (threadIdx.x == 0) {
-        double qv = ldexp(acc, -colexp[q]);
-        double s = coef[4 * c + 2], ratio = coef[4 * c + 3];
+    const double *xq = X + (long)(n - 1) * ldx + (live ? q : 0);
+    const double *yc = Y + c;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i0 = ph; i0 < kw; i0 += 4 * PH) {
+#pragma clang loop unroll(full)
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + PH * u;
+            if (i < k) {
+                acc[u] = fma(xq[-(long)i * ldx], yc[(long)i * ldy], acc[u]);
+            }
+        }
+    }
+    part[ph][col] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    __syncthreads();
+    if (ph == 0 && live) {
+        double tot = 0.0;
+#pragma clang loop unroll(full)
+        for (int r = 0; r < PH; ++r) {
+            tot += part[r][col];
+        }
+        const double qv = ldexp(tot, -colexp[q]);
+        const int b = q % nbatch;
+        const double s = coef[4 * c + 2], ratio = coef[4 * c + 3];
         dS[(long)b * n * ldds + k + (long)k * ldds] += s * qv;
         dP[(long)b * n * lddp + k + (long)k * lddp] += -s * ratio * qv;
     }
