@@ -339,7 +339,34 @@ struct DevBuf {
     template <class T> T *as() { return static_cast<T *>(p); }
 };
 
+// Pinned host memory of a workspace (grows; hipHostMalloc, never hipHostRegister).
+struct HostBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap) {
+            return hipSuccess;
+        }
+        if (p) {
+            hipError_t e = hipHostFree(p);
+            if (e != hipSuccess) {
+                return e;
+            }
+            p = nullptr;
+            cap = 0;
+        }
+        hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e == hipSuccess) {
+            cap = bytes;
+        }
+        return e;
+    }
+};
+
 struct RealWork {
+    HostBuf hnorms;                // column norms and subdiagonal read back to the host
+    hipEvent_t ev_norms = nullptr; // their copy
     DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wblk, wgrp, xrel, xfin, desc, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
     DevBuf kcol, coef, Vt, Y, LamS, bVp, bnfk, btiles, sig_grp, bkpre, bovf, sfac;
     // backward of left / back-transformed vectors: working pencil, permuted operands, products
@@ -395,6 +422,14 @@ void set_error(const std::string &s) { g_err = s; }
 const char *last_error() { return g_err.c_str(); }
 
 #define ENSURE(buf, bytes) XTG_CHECK((buf).ensure(bytes))
+// a nonzero return code of a call is returned at once
+#define XTG_CHECK_RC(x)                                                                            \
+    do {                                                                                           \
+        const int rc_ = (x);                                                                       \
+        if (rc_ != 0) {                                                                            \
+            return rc_;                                                                            \
+        }                                                                                          \
+    } while (0)
 
 // ---------------------------------------------------------------------------------------
 // One side of the real problem on the device.
@@ -421,6 +456,7 @@ struct RealSideArgs {
     int prec;                        // 0: FP64, 1: FP32-product updates and back-transform
     const ZFeed *zf;                 // Z arriving on another stream (null: Z is ready)
     double safemin;                  // smallest normal number of the interface precision
+    bool x_ready;                    // X is allocated for this side and zeroed on the stream
 };
 
 // ---------------------------------------------------------------------------------------
@@ -810,6 +846,37 @@ static hipError_t zcopy_async(RealWork &ws, const double *Z, long ldz, int n, hi
     return e;
 }
 
+// The column 1-norms of S and P and the subdiagonal of S (3n values in ws.norms, written on st)
+// read back to out.  The copy goes to pinned memory and the host waits for it alone, after
+// `pre` has queued on the streams work that does not need the values (the device keeps busy
+// while the host builds the launch descriptors from them); the allocation-free workspace
+// waits for the stream.
+static int read_norms(RealWork &ws, int n, hipStream_t st, std::vector<double> &out, const std::function<int()> &pre)
+{
+    const size_t bytes = sizeof(double) * 3 * (size_t)n;
+    out.resize(3 * (size_t)n);
+    if (ws.fixed) {
+        XTG_CHECK(hipMemcpyAsync(out.data(), ws.norms.p, bytes, hipMemcpyDeviceToHost, st));
+        XTG_CHECK(hipStreamSynchronize(st));
+        return pre ? pre() : 0;
+    }
+    XTG_CHECK(ws.hnorms.ensure(bytes));
+    if (ws.ev_norms == nullptr) {
+        XTG_CHECK(hipEventCreateWithFlags(&ws.ev_norms, hipEventDisableTiming));
+    }
+    XTG_CHECK(hipMemcpyAsync(ws.hnorms.p, ws.norms.p, bytes, hipMemcpyDeviceToHost, st));
+    XTG_CHECK(hipEventRecord(ws.ev_norms, st));
+    if (pre) {
+        if (const int rc = pre()) {
+            return rc;
+        }
+    }
+    XTG_CHECK(hipEventSynchronize(ws.ev_norms));
+    const double *h = static_cast<const double *>(ws.hnorms.p);
+    std::copy(h, h + 3 * (size_t)n, out.begin());
+    return 0;
+}
+
 // Digits of the rows of A (and B, with the exponent of each row taken over both): E[r] from the
 // exact row maxima, then QA (QB) = k-blocked digits over [0, kp) (k-block pitch 32 rows).
 // With cfirst, E[r] bounds only the columns k >= cfirst[r] (the far-update region of a pencil
@@ -893,7 +960,9 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
     XTG_CHECK(hipMemcpyAsync(ws.vcol.p, vcol.data(), sizeof(int) * vcol.size(), hipMemcpyHostToDevice, st));
     XTG_CHECK(hipMemcpyAsync(ws.vw.p, vw.data(), sizeof(int) * vw.size(), hipMemcpyHostToDevice, st));
     ENSURE(ws.X, sizeof(double) * (size_t)n * ldx);
-    hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, ws.X.as<double>(), (long)n * ldx); XTG_LAUNCH_CHECK("k_zero");
+    if (!A.x_ready) {
+        hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, ws.X.as<double>(), (long)n * ldx); XTG_LAUNCH_CHECK("k_zero");
+    }
     const bool split = A.prec == XTGEVC3_PREC_SPLIT_I8;
     if (split) {
         // digits of the working pencil (and of Z for the back-transform) on the side stream,
@@ -1198,9 +1267,21 @@ static int real_device_ws(RealWork &ws, char side, char howmny, const int *selec
     ENSURE(ws.norms, sizeof(double) * 3 * (size_t)n);
     hipLaunchKernelGGL(k_colnorms, dim3(n), dim3(256), 0, st, S, lds, P, ldp, n, 1, ws.norms.as<double>()); XTG_LAUNCH_CHECK("k_colnorms");
     hipLaunchKernelGGL(k_subdiag, dim3((n + 255) / 256), dim3(256), 0, st, S, lds, n, ws.norms.as<double>() + 2 * (long)n); XTG_LAUNCH_CHECK("k_subdiag");
-    std::vector<double> nrm(3 * (size_t)n);
-    XTG_CHECK(hipMemcpyAsync(nrm.data(), ws.norms.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
-    XTG_CHECK(hipStreamSynchronize(st));
+    // queued before the host waits for the norms: the zeroed X of the first side and the
+    // back-transform's copy of Z for the right side (the output overwrites the input VR)
+    const bool right_side = right && VR != nullptr;
+    const bool zcopy_early = right_side && back && zf == nullptr && prec != XTGEVC3_PREC_SPLIT_I8 && !ws.fixed;
+    std::vector<double> nrm;
+    XTG_CHECK_RC(read_norms(ws, n, st, nrm, [&]() -> int {
+        const long ldx = ((long)m + 7) / 8 * 8;
+        ENSURE(ws.X, sizeof(double) * (size_t)n * ldx);
+        hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, ws.X.as<double>(), (long)n * ldx); XTG_LAUNCH_CHECK("k_zero");
+        if (zcopy_early) {
+            ENSURE(ws.Zt, sizeof(double) * (size_t)n * n);
+            XTG_CHECK(zcopy_async(ws, VR, ldvr, n, st));
+        }
+        return 0;
+    }));
     // sub[i] = S[i, i-1]
     std::vector<double> sub(nrm.begin() + 2 * (long)n, nrm.end());
     double an = 0.0, bn = 0.0;
@@ -1234,6 +1315,7 @@ static int real_device_ws(RealWork &ws, char side, char howmny, const int *selec
         A.ldv = ldvr;
         A.zf = zf;
         A.safemin = safemin;
+        A.x_ready = true;
         if (back && zf != nullptr) {
             A.Z = static_cast<const double *>(zf->zr);
             A.ldz = n;
@@ -1251,7 +1333,9 @@ static int real_device_ws(RealWork &ws, char side, char howmny, const int *selec
                                            sizeof(double) * n, n, hipMemcpyDeviceToDevice, st));
             }
             else {
-                XTG_CHECK(zcopy_async(ws, VR, ldvr, n, st));
+                if (!zcopy_early) {
+                    XTG_CHECK(zcopy_async(ws, VR, ldvr, n, st));
+                }
                 A.zf = &zown;
             }
             A.Z = ws.Zt.as<double>();
@@ -1292,6 +1376,7 @@ static int real_device_ws(RealWork &ws, char side, char howmny, const int *selec
         A.ldv = ldvl;
         A.zf = zf;
         A.safemin = safemin;
+        A.x_ready = !right_side; // after a right side, X is zeroed again for this side
         if (back && zf != nullptr) {
             A.Z = static_cast<const double *>(zf->zl);
             A.ldz = n;
@@ -1369,12 +1454,67 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
     if (m == 0 || nbatch <= 0) {
         return 0;
     }
+    // the working columns (k descending, batches adjacent) and their buffers need no scales
+    std::vector<int> kcol(m);
+    for (int c = 0; c < m; ++c) {
+        kcol[c] = units[c].k;
+    }
+    const int M = m * nbatch;
+    const long ldx = ((long)M + 7) / 8 * 8;
+    const long ldt = ((long)m + 7) / 8 * 8;
+    const long npad = (n + 1) & ~1; // rows of the outer products' operands (16-byte pairs)
     ENSURE(ws.norms, sizeof(double) * 3 * (size_t)n);
+    ENSURE(ws.kcol, sizeof(int) * (size_t)m);
+    ENSURE(ws.coef, sizeof(double) * 4 * (size_t)m);
+    ENSURE(ws.Sw, sizeof(double) * (size_t)n * n);
+    ENSURE(ws.Pw, sizeof(double) * (size_t)n * n);
+    ENSURE(ws.X, sizeof(double) * (size_t)n * ldx);
+    ENSURE(ws.Vt, sizeof(double) * (size_t)n * ldt);
+    ENSURE(ws.Y, sizeof(double) * (size_t)n * ldt);
+    ENSURE(ws.LamS, sizeof(double) * (size_t)M * npad);
+    ENSURE(ws.bVp, sizeof(double) * (size_t)npad * m);
+    double *X = ws.X.as<double>();
+    const bool split = prec == XTGEVC3_PREC_SPLIT_I8;
+    // beside the sweep, on the side stream (inputs only): Y = P V (the alpha / beta terms need
+    // lambda^T P v) and the forward vectors with rows padded to npad (zero past n, 16-byte aligned
+    // columns) for the outer products
+    auto side_work = [&]() -> int {
+        hipLaunchKernelGGL(k_to_rowmajor, dim3((m + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, ws.side, V, ldv, n, m,
+                           ws.Vt.as<double>(), ldt); XTG_LAUNCH_CHECK("k_to_rowmajor");
+        const int pr = prof_begin(ws.side);
+        launch_gemm_f64(2, ws.side, n, m, n, 0, P, ldp, (const double *)nullptr, 0L,
+                           (const double *)ws.Vt.as<double>(), ldt, ws.Y.as<double>(), ldt, (const int *)nullptr, 0,
+                           (const double *)nullptr, (const int *)nullptr, (const double *)nullptr, 0, nullptr, nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
+        prof_end(pr, ws.side, 3, 2.0 * n * n * (double)m, 8.0 * ((double)n * n + 2.0 * n * m));
+        hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, ws.side, ws.bVp.as<double>(), npad * m); XTG_LAUNCH_CHECK("k_zero");
+        hipLaunchKernelGGL(k_bwd_permute, dim3((n + 255) / 256, m), dim3(256), 0, ws.side, V, ldv, n, m, 1, 0, 0,
+                           ws.bVp.as<double>(), npad); XTG_LAUNCH_CHECK("k_bwd_permute");
+        XTG_CHECK(hipEventRecord(ws.ev_y, ws.side));
+        return 0;
+    };
     hipLaunchKernelGGL(k_colnorms, dim3(n), dim3(256), 0, st, S, lds, P, ldp, n, 1, ws.norms.as<double>()); XTG_LAUNCH_CHECK("k_colnorms");
     hipLaunchKernelGGL(k_subdiag, dim3((n + 255) / 256), dim3(256), 0, st, S, lds, n, ws.norms.as<double>() + 2 * (long)n); XTG_LAUNCH_CHECK("k_subdiag");
-    std::vector<double> nrm(3 * (size_t)n);
-    XTG_CHECK(hipMemcpyAsync(nrm.data(), ws.norms.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
-    XTG_CHECK(hipStreamSynchronize(st));
+    // queued before the host waits for the norms: the working pencil, the adjoint right-hand
+    // sides and (FP64) the side-stream work
+    std::vector<double> nrm;
+    XTG_CHECK_RC(read_norms(ws, n, st, nrm, [&]() -> int {
+        XTG_CHECK(hipMemcpyAsync(ws.kcol.p, kcol.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+        dim3 g32((n + 31) / 32, (n + 31) / 32);
+        hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, S, lds, n, ws.Sw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
+        hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, P, ldp, n, ws.Pw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
+        hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, X, (long)n * ldx); XTG_LAUNCH_CHECK("k_zero");
+        const int pr = prof_begin(st);
+        hipLaunchKernelGGL(k_bwd_rhs, dim3((M + 3) / 4), dim3(256), 0, st, V, ldv, G, ldg, n, m, nbatch, ws.kcol.as<int>(), X, ldx,
+                           rhs_given ? 0 : 1); XTG_LAUNCH_CHECK("k_bwd_rhs");
+        prof_end(pr, st, 3, 0.0, 8.0 * 3.0 * n * M);
+        if (!split) {
+            XTG_CHECK(split_side_init(ws));
+            XTG_CHECK(hipEventRecord(ws.ev_start, st));
+            XTG_CHECK(hipStreamWaitEvent(ws.side, ws.ev_start, 0));
+            XTG_CHECK_RC(side_work());
+        }
+        return 0;
+    }));
     std::vector<double> sub(nrm.begin() + 2 * (long)n, nrm.end());
     double an = 0.0, bn = 0.0;
     for (int c = 0; c < n; ++c) {
@@ -1387,16 +1527,13 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
     for (int i = 1; i < n; ++i) {
         subw[i] = sub[n - i];
     }
-    // per forward column: a, b, s = a / beta, b / a; working columns k-descending, batches adjacent
-    std::vector<int> kcol(m);
+    // per forward column: a, b, s = a / beta, b / a
     std::vector<double> coef(4 * (size_t)m);
-    const int M = m * nbatch;
     std::vector<ColDesc> desc(M);
     for (int c = 0; c < m; ++c) {
-        const int k = units[c].k;
+        const int k = kcol[c];
         double a, bR, bI;
         shift_real(ar[k], 0.0, be[k], ascale, bscale, a, bR, bI, false);
-        kcol[c] = k;
         coef[4 * c + 0] = a;
         coef[4 * c + 1] = bR;
         coef[4 * c + 2] = a / be[k];
@@ -1414,32 +1551,11 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
             desc[q] = d;
         }
     }
-    const long ldx = ((long)M + 7) / 8 * 8;
-    const long ldt = ((long)m + 7) / 8 * 8;
-    ENSURE(ws.kcol, sizeof(int) * (size_t)m);
-    ENSURE(ws.coef, sizeof(double) * 4 * (size_t)m);
-    ENSURE(ws.Sw, sizeof(double) * (size_t)n * n);
-    ENSURE(ws.Pw, sizeof(double) * (size_t)n * n);
-    ENSURE(ws.X, sizeof(double) * (size_t)n * ldx);
-    ENSURE(ws.Vt, sizeof(double) * (size_t)n * ldt);
-    ENSURE(ws.Y, sizeof(double) * (size_t)n * ldt);
-    const long npad = (n + 1) & ~1; // rows of the outer products' operands (16-byte pairs)
-    ENSURE(ws.LamS, sizeof(double) * (size_t)M * npad);
-    ENSURE(ws.bVp, sizeof(double) * (size_t)npad * m);
-    XTG_CHECK(hipMemcpyAsync(ws.kcol.p, kcol.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
     XTG_CHECK(hipMemcpyAsync(ws.coef.p, coef.data(), sizeof(double) * 4 * m, hipMemcpyHostToDevice, st));
-    dim3 g32((n + 31) / 32, (n + 31) / 32);
-    hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, S, lds, n, ws.Sw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
-    hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, P, ldp, n, ws.Pw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
-    double *X = ws.X.as<double>();
-    hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, X, (long)n * ldx); XTG_LAUNCH_CHECK("k_zero");
-    int pr = prof_begin(st);
-    hipLaunchKernelGGL(k_bwd_rhs, dim3((M + 3) / 4), dim3(256), 0, st, V, ldv, G, ldg, n, m, nbatch, ws.kcol.as<int>(), X, ldx,
-                       rhs_given ? 0 : 1); XTG_LAUNCH_CHECK("k_bwd_rhs");
-    prof_end(pr, st, 3, 0.0, 8.0 * 3.0 * n * M);
-    const bool split = prec == XTGEVC3_PREC_SPLIT_I8;
+    int pr = 0;
     if (split) {
-        // digits of the working pencil on the side stream, after its reversed transposes
+        // digits of the working pencil on the side stream, after its reversed transposes, then
+        // the side-stream work
         const long qld = split_qld(n), qsl = (long)n * qld;
         ENSURE(ws.rowexp, sizeof(int) * (size_t)n);
         ENSURE(ws.Sq, (size_t)SPLIT_DIGITS * qsl);
@@ -1456,28 +1572,8 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
                              ws.rowexp.as<int>(), ws.Sq.as<signed char>(), ws.Pq.as<signed char>(), ws.cfar.as<int>()));
         prof_end(pr, ws.side, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * 4.0 * n * (double)n + 2.0 * SPLIT_DIGITS * qsl);
         XTG_CHECK(hipEventRecord(ws.ev_pencil, ws.side));
+        XTG_CHECK_RC(side_work());
     }
-    else {
-        XTG_CHECK(split_side_init(ws));
-        XTG_CHECK(hipEventRecord(ws.ev_start, st));
-        XTG_CHECK(hipStreamWaitEvent(ws.side, ws.ev_start, 0));
-    }
-    // beside the sweep, on the side stream (inputs only): Y = P V (the alpha / beta terms need
-    // lambda^T P v) and the forward vectors with rows padded to npad (zero past n, 16-byte aligned
-    // columns) for the outer products
-    hipLaunchKernelGGL(k_to_rowmajor, dim3((m + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, ws.side, V, ldv, n, m,
-                       ws.Vt.as<double>(), ldt); XTG_LAUNCH_CHECK("k_to_rowmajor");
-    {
-        pr = prof_begin(ws.side);
-        launch_gemm_f64(2, ws.side, n, m, n, 0, P, ldp, (const double *)nullptr, 0L,
-                           (const double *)ws.Vt.as<double>(), ldt, ws.Y.as<double>(), ldt, (const int *)nullptr, 0,
-                           (const double *)nullptr, (const int *)nullptr, (const double *)nullptr, 0, nullptr, nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<2>");
-        prof_end(pr, ws.side, 3, 2.0 * n * n * (double)m, 8.0 * ((double)n * n + 2.0 * n * m));
-    }
-    hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, ws.side, ws.bVp.as<double>(), npad * m); XTG_LAUNCH_CHECK("k_zero");
-    hipLaunchKernelGGL(k_bwd_permute, dim3((n + 255) / 256, m), dim3(256), 0, ws.side, V, ldv, n, m, 1, 0, 0,
-                       ws.bVp.as<double>(), npad); XTG_LAUNCH_CHECK("k_bwd_permute");
-    XTG_CHECK(hipEventRecord(ws.ev_y, ws.side));
     int rc = sweep_real(ws, n, ws.Sw.as<double>(), n, ws.Pw.as<double>(), n, subw, desc, true, ldx, st, prec,
                         split ? ws.ev_pencil : nullptr);
     if (rc) {
