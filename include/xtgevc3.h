@@ -36,8 +36,8 @@ typedef float _Complex xtgevc3_ccomplex;
 /* precision selector of the *_ex entry points */
 #define XTGEVC3_PREC_FP64 0      /* all arithmetic in FP64 (complex128) */
 #define XTGEVC3_PREC_MIXED_F32 1 /* updates/back-transform on FP32 MFMA, FP64 solves */
-#define XTGEVC3_PREC_MIXED_BF16X3 2 /* updates/back-transform on split-BF16 MFMA (not built) */
-#define XTGEVC3_PREC_SPLIT_I8 3     /* updates/back-transform on exact int8 MFMA digit products */
+#define XTGEVC3_PREC_SPLIT_I8 3  /* updates/back-transform on exact int8 MFMA digit products */
+/* (any other value, 2 included, is rejected with XTGEVC3_INFO_UNSUPPORTED) */
 
 int xtgevc3_abi_version(void);
 const char *xtgevc3_last_error(void);

@@ -175,3 +175,18 @@ def test_python_mirror_query_idiom():
     assert (info, int(w0)) == (0, _req(n))
     m, info, _ = X.dtgevc3("R", "Z", None, S, S, np.ones(n), ai, np.ones(n), VR=np.zeros((n, n), order="F"))
     assert info == -2
+
+
+@pytest.mark.parametrize("precision", [2, 4, -1])
+def test_unknown_precision_is_rejected_before_device_work(precision):
+    """Precision selectors other than FP64, FP32 products and split-int8 return
+    XTGEVC3_INFO_UNSUPPORTED (also through info) without touching any pointer or the GPU."""
+    c = ctypes
+    lib = X.lib()
+    info = c.c_int(-99)
+    m = c.c_int(-5)
+    rc = lib.xtgevc3_dtgevc3_device_ex(c.c_char(b"R"), c.c_char(b"A"), None, c.c_int(4), None, c.c_int(4), None,
+                                       c.c_int(4), None, None, None, None, c.c_int(4), None, c.c_int(4), c.c_int(4),
+                                       c.byref(m), c.byref(info), None, c.c_int(precision))
+    assert rc == X.INFO_UNSUPPORTED and info.value == X.INFO_UNSUPPORTED
+    assert "precision" in X.last_error()

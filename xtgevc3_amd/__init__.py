@@ -39,7 +39,7 @@ _lib.xtgevc3_abi_version.restype = ctypes.c_int
 
 INFO_DEVICE_ERROR = -1000
 INFO_UNSUPPORTED = -1001
-PREC_FP64, PREC_MIXED_F32, PREC_MIXED_BF16X3, PREC_SPLIT_I8 = 0, 1, 2, 3
+PREC_FP64, PREC_MIXED_F32, PREC_SPLIT_I8 = 0, 1, 3
 
 EXPORTED_SYMBOLS = [
     "xtgevc3_abi_version", "xtgevc3_last_error",
