@@ -1498,21 +1498,22 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
     // sides and (FP64) the side-stream work
     std::vector<double> nrm;
     XTG_CHECK_RC(read_norms(ws, n, st, nrm, [&]() -> int {
-        XTG_CHECK(hipMemcpyAsync(ws.kcol.p, kcol.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
-        dim3 g32((n + 31) / 32, (n + 31) / 32);
-        hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, S, lds, n, ws.Sw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
-        hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, P, ldp, n, ws.Pw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
-        hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, X, (long)n * ldx); XTG_LAUNCH_CHECK("k_zero");
-        const int pr = prof_begin(st);
-        hipLaunchKernelGGL(k_bwd_rhs, dim3((M + 3) / 4), dim3(256), 0, st, V, ldv, G, ldg, n, m, nbatch, ws.kcol.as<int>(), X, ldx,
-                           rhs_given ? 0 : 1); XTG_LAUNCH_CHECK("k_bwd_rhs");
-        prof_end(pr, st, 3, 0.0, 8.0 * 3.0 * n * M);
         if (!split) {
             XTG_CHECK(split_side_init(ws));
             XTG_CHECK(hipEventRecord(ws.ev_start, st));
             XTG_CHECK(hipStreamWaitEvent(ws.side, ws.ev_start, 0));
             XTG_CHECK_RC(side_work());
         }
+        XTG_CHECK(hipMemcpyAsync(ws.kcol.p, kcol.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+        dim3 g32((n + 31) / 32, (n + 31) / 32);
+        hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, S, lds, n, ws.Sw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
+        hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, P, ldp, n, ws.Pw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
+        // the right-hand sides: every entry of the rows < n and columns < M is written (zero
+        // below each column's system), and nothing reads the padding columns of X
+        const int pr = prof_begin(st);
+        hipLaunchKernelGGL(k_bwd_rhs, dim3((M + 3) / 4), dim3(256), 0, st, V, ldv, G, ldg, n, m, nbatch, ws.kcol.as<int>(), X, ldx,
+                           rhs_given ? 0 : 1); XTG_LAUNCH_CHECK("k_bwd_rhs");
+        prof_end(pr, st, 3, 0.0, 8.0 * 3.0 * n * M);
         return 0;
     }));
     std::vector<double> sub(nrm.begin() + 2 * (long)n, nrm.end());
