@@ -41,6 +41,8 @@ __global__ void k_tgsna_fin(const double *, int, int, const int *, const int *, 
 __global__ void k_slice_digits(const double *, long, int, int, const double *, long, int, int, int, int, const int *,
                                signed char *, long, long, int, signed char *);
 __global__ void k_xexp_final(const int *, const int *, long, const int *, const int *, int, int, int *);
+__global__ void k_slice_xfix(const double *, long, int, int, int, const int *, const int *, long, const int *,
+                             const int *, signed char *, long, long);
 int launch_gemm_i8(hipStream_t st, int M, int N, int K1, int K2, const signed char *L1, const signed char *L2,
                    long lslice, long kbl, const signed char *R, long rslice, long kbr, const int *EL, const int *ER,
                    double *C, long ldc, int cst, const int *outcol, int mode, const double *sig, const int *kmax);
@@ -472,7 +474,7 @@ static hipError_t far_stream_init(RealWork &ws);
 
 int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *Pw, long ldpw,
                const std::vector<double> &subw, const std::vector<ColDesc> &desc, bool adjoint, long ldx,
-               hipStream_t st, int prec, hipEvent_t pencil_ready)
+               hipStream_t st, int prec, hipEvent_t pencil_ready, bool fixup = true)
 {
     const int m = (int)desc.size();
     std::vector<UnitDesc> units;
@@ -765,10 +767,13 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     if (pencil_ready != nullptr) {
         XTG_CHECK(hipStreamWaitEvent(st, pencil_ready, 0)); // the side stream's digits are in use until here
     }
-    // apply the per-column scale history to the solved blocks
-    dim3 grid((m + 255) / 256, n);
-    hipLaunchKernelGGL(k_fixup_scale, grid, dim3(256), 0, st, X, ldx, n, m, ws.rowstep.as<int>(), ws.blkexp.as<int>(),
-                       ldx, ws.colexp.as<int>(), ws.flag.as<int>()); XTG_LAUNCH_CHECK("k_fixup_scale");
+    // apply the per-column scale history to the solved blocks (fixup == false: the caller's
+    // pass over X applies it, k_slice_xfix)
+    if (fixup) {
+        dim3 grid((m + 255) / 256, n);
+        hipLaunchKernelGGL(k_fixup_scale, grid, dim3(256), 0, st, X, ldx, n, m, ws.rowstep.as<int>(),
+                           ws.blkexp.as<int>(), ldx, ws.colexp.as<int>(), ws.flag.as<int>()); XTG_LAUNCH_CHECK("k_fixup_scale");
+    }
     if (split) {
         // exponents of the final columns of X from the per-step block exponents
         hipLaunchKernelGGL(k_xexp_final, dim3((m + 255) / 256), dim3(256), 0, st, (const int *)ws.xbexp.as<int>(),
@@ -998,8 +1003,10 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
             XTG_CHECK(hipEventRecord(ws.ev_z, ws.side));
         }
     }
+    // the split back-transform makes the digits of X with the scale fix-up in the same pass
+    const bool fused_fix = split && A.back;
     int rc = sweep_real(ws, n, A.Sw, A.ldsw, A.Pw, A.ldpw, *A.subw, desc, false, ldx, st, A.prec,
-                        split ? ws.ev_pencil : nullptr);
+                        split ? ws.ev_pencil : nullptr, !fused_fix);
     if (rc) {
         return rc;
     }
@@ -1038,9 +1045,10 @@ static int run_real_side(RealWork &ws, const RealSideArgs &A, const std::vector<
             const int kb = round_up(n, SPLIT_KSTEP);
             ENSURE(ws.Xq, (size_t)SPLIT_DIGITS * m * kb);
             int pr = prof_begin(st);
-            hipLaunchKernelGGL(k_slice_digits, dim3((m + 63) / 64, (kb + 15) / 16), dim3(256), 0, st, (const double *)X,
-                               ldx, 0, n, (const double *)nullptr, 0L, 0, 0, m, kb, (const int *)ws.xexp.as<int>(),
-                               ws.Xq.as<signed char>(), (long)m * kb, (long)m * SPLIT_KSTEP, 0, (signed char *)nullptr); XTG_LAUNCH_CHECK("k_slice_digits");
+            hipLaunchKernelGGL(k_slice_xfix, dim3((m + 63) / 64, (kb + 15) / 16), dim3(256), 0, st, (const double *)X,
+                               ldx, n, m, kb, (const int *)ws.rowstep.as<int>(), (const int *)ws.blkexp.as<int>(), ldx,
+                               (const int *)ws.colexp.as<int>(), (const int *)ws.xexp.as<int>(),
+                               ws.Xq.as<signed char>(), (long)m * kb, (long)m * SPLIT_KSTEP); XTG_LAUNCH_CHECK("k_slice_xfix");
             prof_end(pr, st, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * (double)n * m + SPLIT_DIGITS * (double)m * kb);
             XTG_CHECK(hipStreamWaitEvent(st, ws.ev_z, 0));
             pr = prof_begin(st);

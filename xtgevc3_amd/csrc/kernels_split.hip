@@ -137,13 +137,69 @@ __global__ void k_xexp_final(const int *__restrict__ xbexp, const int *__restric
     }
     int e = -1000;
     const int ce = colexp[p];
-    for (int s = sstart[p]; s < nsteps; ++s) {
+    int s = sstart[p];
+    // eight steps' loads in flight at a time
+    for (; s + 8 <= nsteps; s += 8) {
+        int eb[8], bb[8];
+#pragma clang loop unroll(full)
+        for (int u = 0; u < 8; ++u) {
+            eb[u] = xbexp[(long)(s + u) * ldb + p];
+            bb[u] = blkexp[(long)(s + u) * ldb + p];
+        }
+#pragma clang loop unroll(full)
+        for (int u = 0; u < 8; ++u) {
+            if (eb[u] > -1000) {
+                e = max(e, eb[u] + ce - bb[u]);
+            }
+        }
+    }
+    for (; s < nsteps; ++s) {
         const int eb = xbexp[(long)s * ldb + p];
         if (eb > -1000) {
             e = max(e, eb + ce - blkexp[(long)s * ldb + p]);
         }
     }
     out[p] = min(max(e, -1000), 1000);
+}
+
+// Digits of the columns of X at their final scale, as k_slice_digits (cmode 0) over
+// A(a, k) = X[k * ldx + a] for k < n, each entry first brought from the exponent of its block's
+// step to the column's final one: 2^(colexp[a] - blkexp[rowstep[k] * ldb + a]) (the scale
+// fix-up of the sweep, fused into this pass over X).
+__global__ __launch_bounds__(256) void k_slice_xfix(const double *__restrict__ X, long ldx, int n, int M, int Kp,
+                                                    const int *__restrict__ rowstep, const int *__restrict__ blkexp,
+                                                    long ldb, const int *__restrict__ colexp,
+                                                    const int *__restrict__ E, signed char *__restrict__ Q, long qs,
+                                                    long kbp)
+{
+    const int a = blockIdx.x * 64 + threadIdx.x / 4;
+    const int k0 = blockIdx.y * 16 + (threadIdx.x % 4) * 4;
+    if (a >= M || k0 >= Kp) {
+        return;
+    }
+    const int e = E[a] + 1;
+    const int ce = colexp[a];
+    int dg[SPLIT_DIGITS][4];
+#pragma clang loop unroll(full)
+    for (int q = 0; q < 4; ++q) {
+        const int kk = k0 + q;
+        double v = 0.0;
+        if (kk < n) {
+            v = ldexp(X[a + (long)kk * ldx], ce - blkexp[(long)rowstep[kk] * ldb + a]);
+        }
+        int d[SPLIT_DIGITS];
+        split_digits(ldexp(v, -e), d);
+#pragma clang loop unroll(full)
+        for (int s = 0; s < SPLIT_DIGITS; ++s) {
+            dg[s][q] = d[s];
+        }
+    }
+    const long qo = (long)(k0 / 32) * kbp + (long)a * 32 + (k0 % 32);
+#pragma clang loop unroll(full)
+    for (int s = 0; s < SPLIT_DIGITS; ++s) {
+        *reinterpret_cast<int *>(Q + qo + s * qs) =
+            (dg[s][0] & 0xff) | ((dg[s][1] & 0xff) << 8) | ((dg[s][2] & 0xff) << 16) | ((dg[s][3] & 0xff) << 24);
+    }
 }
 
 // ---------------------------------------------------------------------------------------
