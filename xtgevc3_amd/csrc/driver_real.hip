@@ -56,6 +56,7 @@ int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const
                      const int *sb, int batch = 1, long bsr = 0, long bsc = 0);
 constexpr int GROUP_STEPS = 4; // row blocks per far update (K = 2 * ~256)
 __global__ void k_fixup_scale(double *, long, int, int, const int *, const int *, long, const int *, const int *);
+__global__ void k_fixup_steps(double *, long, int, const int *, const int *, long, const int *, const int *);
 __global__ void k_copy_out(const double *, long, int, int, double *, long, const int *, int);
 __global__ void k_normalize(double *, long, int, const int *, const int *);
 __global__ void k_round_f32(const double *, long, long, float *, long);
@@ -370,7 +371,7 @@ struct RealWork {
     HostBuf hnorms;                // column norms and subdiagonal read back to the host
     hipEvent_t ev_norms = nullptr; // their copy
     DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wblk, wgrp, xrel, xfin, desc, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
-    DevBuf kcol, coef, Vt, Y, LamS, bVp, bnfk, btiles, sig_grp, bkpre, bovf, sfac;
+    DevBuf kcol, coef, Vt, Y, LamS, bVp, bnfk, btiles, sig_grp, bkpre, bovf, sfac, srow;
     // backward of left / back-transformed vectors: working pencil, permuted operands, products
     DevBuf bfS, bfP, bfV, bfG, bfGc, bfGt, bfVt, bfZt, bfW, bfGh, bfdS, bfdP, bfk;
     DevBuf Ut, YB, part, tcol, tw; // eigenvalue condition numbers
@@ -770,9 +771,18 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     // apply the per-column scale history to the solved blocks (fixup == false: the caller's
     // pass over X applies it, k_slice_xfix)
     if (fixup) {
-        dim3 grid((m + 255) / 256, n);
-        hipLaunchKernelGGL(k_fixup_scale, grid, dim3(256), 0, st, X, ldx, n, m, ws.rowstep.as<int>(),
-                           ws.blkexp.as<int>(), ldx, ws.colexp.as<int>(), ws.flag.as<int>()); XTG_LAUNCH_CHECK("k_fixup_scale");
+        // the row range of each step (steps run bottom-up)
+        std::vector<int> srow(2 * (size_t)nsteps);
+        for (int s = 0; s < nsteps; ++s) {
+            srow[2 * s] = i0s[s];
+            srow[2 * s + 1] = i0s[s] + nbs[s];
+        }
+        ENSURE(ws.srow, sizeof(int) * 2 * (size_t)nsteps);
+        XTG_CHECK(hipMemcpyAsync(ws.srow.p, srow.data(), sizeof(int) * 2 * nsteps, hipMemcpyHostToDevice, st));
+        dim3 grid((m + 255) / 256, nsteps);
+        hipLaunchKernelGGL(k_fixup_steps, grid, dim3(256), 0, st, X, ldx, m, (const int *)ws.srow.as<int>(),
+                           (const int *)ws.blkexp.as<int>(), ldx, (const int *)ws.colexp.as<int>(),
+                           (const int *)ws.flag.as<int>()); XTG_LAUNCH_CHECK("k_fixup_steps");
     }
     if (split) {
         // exponents of the final columns of X from the per-step block exponents
@@ -1175,6 +1185,7 @@ constexpr size_t NA_BYTES[] = {
     sizeof(int) * 4,                                           // flag
     sizeof(UnitDesc) * 4 * 2 * NA_N,                           // desc (tile order: four per tile)
     sizeof(int) * NA_N,                                        // rowstep
+    sizeof(int) * 2 * NA_STEPS,                                // srow
     sizeof(int) * NA_N,                                        // outcol
     sizeof(int) * NA_N,                                        // kmax
     sizeof(int) * NA_N,                                        // vcol
@@ -1216,7 +1227,7 @@ static hipError_t noalloc_setup(int dev)
     }
     RealWork &ws = g_na_ws[dev];
     DevBuf *slots[] = {&ws.norms, &ws.W,      &ws.sig,  &ws.sig_grp, &ws.colexp, &ws.blkexp,
-                       &ws.flag,  &ws.desc,   &ws.rowstep, &ws.outcol, &ws.kmax,
+                       &ws.flag,  &ws.desc,   &ws.rowstep, &ws.srow, &ws.outcol, &ws.kmax,
                        &ws.vcol,  &ws.vw,     &ws.X,    &ws.Zt,      &ws.Sw,     &ws.Pw};
     static_assert(sizeof(slots) / sizeof(slots[0]) == sizeof(NA_BYTES) / sizeof(NA_BYTES[0]), "one slot per buffer");
     size_t off = 0;

@@ -1418,6 +1418,46 @@ __global__ void k_fill_int(int *__restrict__ X, long count, int v)
     }
 }
 
+// X[r][p] *= 2^(colexp[p] - blkexp[s][p]) for the rows r of step s (rows [srow[2s], srow[2s+1])):
+// one thread per column p of a step (consecutive columns, so each row is read and written as
+// contiguous runs), skipping the (step, column) pairs whose exponent did not change after the
+// step.  flag[0] == 0: no column ever rescaled.
+__global__ __launch_bounds__(256) void k_fixup_steps(double *__restrict__ X, long ldx, int m,
+                                                     const int *__restrict__ srow, const int *__restrict__ blkexp,
+                                                     long ldb, const int *__restrict__ colexp,
+                                                     const int *__restrict__ flag)
+{
+    if (*flag == 0) {
+        return;
+    }
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (p >= m) {
+        return;
+    }
+    const int e = colexp[p] - blkexp[(long)s * ldb + p];
+    if (e == 0) {
+        return;
+    }
+    const int r0 = srow[2 * s], r1 = srow[2 * s + 1];
+    double *x = X + (long)r0 * ldx + p;
+    int r = r0;
+    for (; r + 4 <= r1; r += 4, x += 4 * ldx) {
+        double v[4];
+#pragma clang loop unroll(full)
+        for (int u = 0; u < 4; ++u) {
+            v[u] = x[u * ldx];
+        }
+#pragma clang loop unroll(full)
+        for (int u = 0; u < 4; ++u) {
+            x[u * ldx] = ldexp(v[u], e);
+        }
+    }
+    for (; r < r1; ++r, x += ldx) {
+        *x = ldexp(*x, e);
+    }
+}
+
 // X[r][p] *= 2^(colexp[p] - blkexp[step(r)][p]) for the solved rows r <= top(p).
 __global__ void k_fixup_scale(double *__restrict__ X, long ldx, int n, int m, const int *__restrict__ rowstep,
                               const int *__restrict__ blkexp, long ldb, const int *__restrict__ colexp,
