@@ -31,13 +31,13 @@ SPLIT_PRODUCTS = 15  # digit products per multiply-add in the split-int8 form (l
 # each scaled by the counter's reading on a plain 1 GiB device copy (FETCH x2.0, WRITE x1.0);
 # summary and commands in profiles/dtgevc3_R_B_n8192_hbm_traffic.md
 MEASURED_TRAFFIC = {
-    ("xtg::k_gemm_f64<1, true, 128, 128, 16, false>", 8192): 13349.8e6,
-    ("xtg::k_gemm_f64<0, true, 128, 64, 16, false>", 8192): 405.8e6,
-    ("xtg::k_gemm_f64<0, true, 128, 128, 16, false>", 8192): 214.4e6,
+    ("xtg::k_gemm_f64<1, true, 128, 128, 16, false>", 8192): 13453.4e6,
+    ("xtg::k_gemm_f64<0, true, 128, 64, 16, false>", 8192): 411.6e6,
+    ("xtg::k_gemm_f64<0, true, 128, 128, 16, false>", 8192): 215.4e6,
     ("xtg::k_gemm_f64<0, true, 64, 64, 32, false>", 8192): 59.2e6,
     ("xtg::k_gemm_f64<0, true, 32, 64, 32, false>", 8192): 16.3e6,
-    ("xtg::k_solve_real<65>", 8192): 27.3e6,
-    ("xtg::k_gemm_i8<128, 128, 2, 4> (back-transform)", 8192): 5112.3e6,
+    ("xtg::k_solve_real<65>", 8192): 25.5e6,
+    ("xtg::k_gemm_i8<128, 128, 2, 4> (back-transform)", 8192): 5140.1e6,
 }
 
 
@@ -45,7 +45,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_inputs(n, seed, device, cluster=False, need_z=True):
+def make_inputs(n, seed, device, cluster=False, need_z=True, z_host=False):
     import torch
 
     from xtgevc3_amd import pencils as G
@@ -54,7 +54,10 @@ def make_inputs(n, seed, device, cluster=False, need_z=True):
     # cluster: diagonal ratios s_kk / p_kk drawn from 64 centres with relative spread 1e-6
     S, P, ar, ai, be = G.real_quasi_triangular(n, seed, cluster_groups=64 if cluster else 0)
     dZ = None
-    if need_z:
+    if need_z and z_host:
+        # Z from a host QR (no library kernels on the device, e.g. under counter collection)
+        dZ = torch.from_numpy(np.ascontiguousarray(G.random_orthogonal(n, seed + 1).T)).to(device)
+    elif need_z:
         g = torch.Generator(device=device)
         g.manual_seed(seed + 1)
         A = torch.randn(n, n, dtype=torch.float64, device=device, generator=g)
@@ -479,6 +482,8 @@ def main():
                     help="skip the other configurations (n=512 QZ on CPU and GPU, ztgevc3 n=8192, n=16384 clustered)")
     ap.add_argument("--cluster", action="store_true",
                     help="eigenvalues clustered in 64 groups (relative spread 1e-6), as the n=16384 configuration")
+    ap.add_argument("--z-host", action="store_true",
+                    help="make Z by a QR on the host (for runs under rocprofv3 counter collection)")
     ap.add_argument("--howmny", default="B", choices=["A", "B"],
                     help="B: back-transformed (the metric); A: eigenvectors of (S, P), no back-transform")
     args = ap.parse_args()
@@ -494,7 +499,8 @@ def main():
     n = args.n
     dev = "cuda"
     back = args.howmny == "B"
-    S, P, ar, ai, be, dS, dP, dZ = make_inputs(n, args.seed, dev, cluster=args.cluster, need_z=back)
+    S, P, ar, ai, be, dS, dP, dZ = make_inputs(n, args.seed, dev, cluster=args.cluster, need_z=back,
+                                               z_host=args.z_host)
     dVR = torch.empty((n, n), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream()
 
