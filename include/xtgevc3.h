@@ -137,7 +137,7 @@ int xtgevc3_dtgevc3_backward_device_ex(int n, const double *S, int lds, const do
  * GPU (other streams, other host threads without ordering) race.  Like the other device
  * entries it synchronises `stream` once per call to read the pencil's column norms. */
 #define XTGEVC3_NOALLOC_MAX_N 512
-#define XTGEVC3_NOALLOC_SCRATCH_BYTES (13u << 20)
+#define XTGEVC3_NOALLOC_SCRATCH_BYTES (16u << 20)
 int xtgevc3_dtgevc3_device_noalloc(char side, char howmny, const int *select, int n, const double *S, int lds,
                                    const double *P, int ldp, const double *alphar, const double *alphai,
                                    const double *beta, double *VL, int ldvl, double *VR, int ldvr, int mm, int *m,

@@ -54,12 +54,29 @@ struct GroupInfo {
     int last; // the launch is the group's last step
 };
 
+// Operand images of one step of the real sweep, packed once per sweep by k_pack_steps so that
+// the solve kernel reads them as contiguous 16-byte runs (the pencil is column-major with a
+// column pitch of many KB: a block's columns fall into few L2 channels, and every workgroup
+// of a launch reads the same block):
+//   block image at boff (PK_BLK {S,P} pairs): the diagonal block [i0, i0+nb)^2 in the solve
+//   kernel's LDS layout, then its subdiagonal;
+//   panel image at poff: [S | P][g0 + 16 t + fr, i0 + k] of the near update (the group's rows
+//   above the block), row tiles t < nrt of 16 rows, in the lane order of the 16x16x4 FP64 MFMA
+//   A operand: lane fr + 16 fk of k-step kk holds column k = 4 kk' + fk of part kk < kq ? S : P
+//   (kk' = kk mod kq, kq = ks / 4, ks = nb rounded up to a multiple of 4); k-steps 2p and
+//   2p + 1 of a lane are one {x, y} pair at pair index (t kq + p) 64 + lane.
+struct StepPack {
+    int i0, nb, g0, nrt;
+    long boff, poff;
+};
+constexpr int PK_BLK = 2720; // {S,P} pairs of a block image (2640 staged + 65 subdiagonal, padded)
+constexpr int PK_KSMAX = 68; // largest ks (blocks of at most 65 rows)
+
 // Exponent bookkeeping of the mixed (FP32-product) form, written by the real solve kernel
-// when on != 0: wblk[col] bounds |W| of the block just solved (2^wblk), wgrp[col] bounds the
-// group's W (relative to the column's exponent until the group's last step, absolute after
-// it), xrel[col] bounds the solution entries relative to the column's exponent.
+// when on != 0: wgrp[col] bounds the group's W (2^wgrp; relative to the column's exponent
+// until the group's last step, absolute after it), xrel[col] bounds the solution entries
+// relative to the column's exponent.
 struct MixedExp {
-    int *wblk;
     int *wgrp;
     int *xrel;
     int on;

@@ -31,9 +31,10 @@
 namespace xtg {
 
 template <int MAXB>
-__global__ void k_solve_real(const double *, long, const double *, long, int, int, unsigned long long, double *, long,
+__global__ void k_solve_real(const double2 *, const double2 *, int, int, int, int, unsigned long long, double *, long,
                              double *, long, const UnitDesc *, int, int, int, int, double *, int *, int *,
                              int *, int, int, double *, int, GroupInfo, MixedExp, SplitOut);
+__global__ void k_pack_steps(const double *, long, const double *, long, const StepPack *, double2 *);
 __global__ void k_rowexp(const double *, long, const double *, long, int, int, int *, const int *);
 __global__ void k_tgsna_part(const double *, const double *, const double *, const double *, long, int, int,
                              const int *, const int *, int, double *);
@@ -370,11 +371,12 @@ struct HostBuf {
 struct RealWork {
     HostBuf hnorms;                // column norms and subdiagonal read back to the host
     hipEvent_t ev_norms = nullptr; // their copy
-    DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wblk, wgrp, xrel, xfin, desc, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
+    DevBuf X, W, sig, colexp, blkexp, flag, rowexp, zexp, wgrp, xrel, xfin, desc, outcol, kmax, rowstep, vcol, vw, Sw, Pw, Zt, norms;
     DevBuf kcol, coef, Vt, Y, LamS, bVp, bnfk, btiles, sig_grp, bkpre, bovf, sfac, srow;
     // backward of left / back-transformed vectors: working pencil, permuted operands, products
     DevBuf bfS, bfP, bfV, bfG, bfGc, bfGt, bfVt, bfZt, bfW, bfGh, bfdS, bfdP, bfk;
     DevBuf Ut, YB, part, tcol, tw; // eigenvalue condition numbers
+    DevBuf pk, pksteps;            // packed block and panel images of the sweep's steps
     DevBuf Sq, Pq, Wg, wgexp, wbexp, xbexp, sstart, Xq, Zq, xexp, cfar; // split-int8 digits and exponents
     // split-int8 form: side stream (default priority) for the digit splitting of the pencil
     // and of Z, overlapped with the sweep; events order it against the caller's stream
@@ -519,7 +521,7 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     ENSURE(ws.desc, sizeof(UnitDesc) * tdesc.size());
     ENSURE(ws.rowstep, sizeof(int) * (size_t)n);
     XTG_CHECK(hipMemcpyAsync(ws.desc.p, tdesc.data(), sizeof(UnitDesc) * tdesc.size(), hipMemcpyHostToDevice, st));
-    MixedExp mx{nullptr, nullptr, nullptr, 0};
+    MixedExp mx{nullptr, nullptr, 0};
     const bool split = prec == XTGEVC3_PREC_SPLIT_I8;
     long qld = 0, qsl = 0;
     if (split) {
@@ -542,16 +544,39 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     if (prec == 1) {
         // mixed form: row exponents of the working pencil, per-column operand/solution bounds
         ENSURE(ws.rowexp, sizeof(int) * (size_t)n);
-        ENSURE(ws.wblk, sizeof(int) * (size_t)m);
         ENSURE(ws.wgrp, sizeof(int) * 2 * (size_t)m);
         ENSURE(ws.xrel, sizeof(int) * (size_t)m);
         hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.rowexp.as<int>(), (long)n, -1000); XTG_LAUNCH_CHECK("k_fill_int");
         hipLaunchKernelGGL(k_rowexp, dim3((n + 255) / 256, (n + 63) / 64), dim3(256), 0, st, Sw, ldsw, Pw, ldpw, n, n,
                            ws.rowexp.as<int>(), (const int *)nullptr); XTG_LAUNCH_CHECK("k_rowexp");
         hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.xrel.as<int>(), (long)m, -1000000); XTG_LAUNCH_CHECK("k_fill_int");
-        mx = MixedExp{ws.wblk.as<int>(), ws.wgrp.as<int>(), ws.xrel.as<int>(), 1};
+        mx = MixedExp{ws.wgrp.as<int>(), ws.xrel.as<int>(), 1};
     }
     XTG_CHECK(hipMemcpyAsync(ws.rowstep.p, rowstep.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    // operand images of every step: the diagonal block and the near update's panel (the
+    // group's rows above the block), packed in one launch
+    std::vector<StepPack> spk(nsteps);
+    long pko = 0;
+    int pk_items = PK_BLK;
+    for (int s = 0; s < nsteps; ++s) {
+        const int ge = std::min(nsteps, s / GROUP_STEPS * GROUP_STEPS + GROUP_STEPS);
+        StepPack &q = spk[s];
+        q.i0 = i0s[s];
+        q.nb = nbs[s];
+        q.g0 = i0s[ge - 1];
+        q.nrt = (q.i0 - q.g0 + 15) / 16;
+        q.boff = pko;
+        pko += PK_BLK;
+        q.poff = pko;
+        pko += (long)q.nrt * ((q.nb + 3) / 4) * 64;
+        pk_items = std::max(pk_items, PK_BLK + q.nrt * ((q.nb + 3) / 4) * 128);
+    }
+    ENSURE(ws.pk, sizeof(double2) * (size_t)pko);
+    ENSURE(ws.pksteps, sizeof(StepPack) * (size_t)nsteps);
+    XTG_CHECK(hipMemcpyAsync(ws.pksteps.p, spk.data(), sizeof(StepPack) * nsteps, hipMemcpyHostToDevice, st));
+    const double2 *pk = ws.pk.as<double2>();
+    hipLaunchKernelGGL(k_pack_steps, dim3((pk_items + 255) / 256, nsteps), dim3(256), 0, st, Sw, ldsw, Pw, ldpw,
+                       (const StepPack *)ws.pksteps.as<StepPack>(), ws.pk.as<double2>()); XTG_LAUNCH_CHECK("k_pack_steps");
     double *X = ws.X.as<double>();
     double *W = ws.W.as<double>();
     hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, ws.blkexp.as<int>(), (long)nsteps * ldx); XTG_LAUNCH_CHECK("k_zero_int");
@@ -650,32 +675,21 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
                 }
             }
             const int wpb = std::min(SOLVE_WAVES_H, ntl);
+            // near update (fused into the solve): the group's rows [r0, i0) above this block
+            const int r0 = adjoint ? std::max(g0, std::min(desc[c_begin].lo, i0)) : g0;
+            const int M = i0 - r0;
             int pr = prof_begin(st);
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_solve_real<MAXB_REAL>),
-                               dim3((ntl + wpb - 1) / wpb), dim3(wpb * WAVE), 0, st, Sw,
-                               ldsw, Pw, ldpw, i0, nb, bumps, X, ldx, Wb, ldw, ws.desc.as<UnitDesc>(),
-                               t_begin, t_end, u_begin, u_end, ws.sig.as<double>(), ws.colexp.as<int>(),
-                               ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff, wr + woff,
-                               sgb, s == gs ? 1 : 0, grp, mxb,
+                               dim3((ntl + wpb - 1) / wpb), dim3(wpb * WAVE), 0, st, pk + spk[s].boff,
+                               M > 0 ? pk + spk[s].poff : (const double2 *)nullptr, g0, r0, i0, nb, bumps, X, ldx, Wb,
+                               ldw, ws.desc.as<UnitDesc>(), t_begin, t_end, u_begin, u_end, ws.sig.as<double>(),
+                               ws.colexp.as<int>(), ws.blkexp.as<int>() + (long)s * ldx, ws.flag.as<int>(), woff,
+                               wr + woff, sgb, s == gs ? 1 : 0, grp, mxb,
                                split ? SplitOut{ws.wbexp.as<int>() + (long)s * ldx, wgexpb,
                                                 ws.xbexp.as<int>() + (long)s * ldx}
                                      : SplitOut{nullptr, nullptr, nullptr}); XTG_LAUNCH_CHECK("k_solve_real");
-            prof_end(pr, st, 0, 2.0 * nb * nb * N, 8.0 * (4.0 * nb * N + 2.0 * nb * nb));
-            // near update: the group's rows above this block
-            const int r0 = adjoint ? std::max(g0, std::min(desc[c_begin].lo, i0)) : g0;
-            const int M = i0 - r0;
-            // (FP64 also in the split-int8 form: its short products run faster there)
-            if (M > 0) {
-                pr = prof_begin(st);
-                const int gt = launch_gemm_f64(0, st, M, N, nb, nb, Sw + r0 + (long)i0 * ldsw,
-                                   ldsw, Pw + r0 + (long)i0 * ldpw, ldpw, (const double *)(Wb + (long)woff * ldw + c_begin),
-                                   ldw, X + (long)r0 * ldx + c_begin, ldx, (const int *)nullptr, 1,
-                                   (const double *)(ws.sig.as<double>() + c_begin), (const int *)nullptr,
-                                   (const double *)(Wb + (long)(wr + woff) * ldw + c_begin), split ? 0 : prec,
-                                   mx.on ? ws.rowexp.as<int>() + r0 : nullptr, mx.on ? mx.wblk + c_begin : nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
-                prof_end(pr, st, update_kind(gt), 2.0 * M * N * (2.0 * nb),
-                         8.0 * (2.0 * M * N + 2.0 * nb * M + 2.0 * nb * N));
-            }
+            prof_end(pr, st, 0, 2.0 * nb * nb * N + 2.0 * std::max(M, 0) * N * (2.0 * nb),
+                     8.0 * (4.0 * nb * N + 2.0 * nb * nb + 2.0 * std::max(M, 0) * N + 2.0 * nb * std::max(M, 0)));
         }
         if (!any_active) {
             continue;
@@ -1186,6 +1200,9 @@ constexpr size_t NA_BYTES[] = {
     sizeof(UnitDesc) * 4 * 2 * NA_N,                           // desc (tile order: four per tile)
     sizeof(int) * NA_N,                                        // rowstep
     sizeof(int) * 2 * NA_STEPS,                                // srow
+    // pk: a block image per step, and panels of at most 13 + 9 + 5 row tiles per group of 4 steps
+    sizeof(double2) * (NA_STEPS * PK_BLK + (NA_STEPS + 3) / 4 * 27 * (PK_KSMAX / 4) * 64),
+    sizeof(StepPack) * NA_STEPS,                               // pksteps
     sizeof(int) * NA_N,                                        // outcol
     sizeof(int) * NA_N,                                        // kmax
     sizeof(int) * NA_N,                                        // vcol
@@ -1227,7 +1244,7 @@ static hipError_t noalloc_setup(int dev)
     }
     RealWork &ws = g_na_ws[dev];
     DevBuf *slots[] = {&ws.norms, &ws.W,      &ws.sig,  &ws.sig_grp, &ws.colexp, &ws.blkexp,
-                       &ws.flag,  &ws.desc,   &ws.rowstep, &ws.srow, &ws.outcol, &ws.kmax,
+                       &ws.flag,  &ws.desc,   &ws.rowstep, &ws.srow, &ws.pk,     &ws.pksteps, &ws.outcol, &ws.kmax,
                        &ws.vcol,  &ws.vw,     &ws.X,    &ws.Zt,      &ws.Sw,     &ws.Pw};
     static_assert(sizeof(slots) / sizeof(slots[0]) == sizeof(NA_BYTES) / sizeof(NA_BYTES[0]), "one slot per buffer");
     size_t off = 0;
@@ -1238,7 +1255,7 @@ static hipError_t noalloc_setup(int dev)
         off += na_slot(NA_BYTES[i]);
     }
     // every other buffer of the workspace is a zero-capacity fixed slot: using one is an error
-    DevBuf *unused[] = {&ws.rowexp, &ws.zexp, &ws.wblk, &ws.wgrp, &ws.xrel, &ws.xfin, &ws.kcol, &ws.coef,
+    DevBuf *unused[] = {&ws.rowexp, &ws.zexp, &ws.wgrp, &ws.xrel, &ws.xfin, &ws.kcol, &ws.coef,
                         &ws.Vt,     &ws.Y,    &ws.LamS, &ws.bVp, &ws.bnfk, &ws.btiles, &ws.bkpre, &ws.Ut,  &ws.YB,   &ws.part,
                         &ws.tcol,   &ws.tw,   &ws.Sq,   &ws.Pq,   &ws.Wg,    &ws.wgexp, &ws.wbexp, &ws.xbexp,
                         &ws.sstart, &ws.Xq,   &ws.Zq,   &ws.xexp, &ws.cfar};

@@ -28,6 +28,9 @@ struct cpx {
     double r, i;
 };
 
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ cpx cmul(cpx a, cpx b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
 __device__ __forceinline__ cpx csub(cpx a, cpx b) { return {a.r - b.r, a.i - b.i}; }
 __device__ __forceinline__ double abs1(cpx a) { return fabs(a.r) + fabs(a.i); }
@@ -106,6 +109,20 @@ __device__ __forceinline__ cpx qbc(cpx v, int src) { return {qbd(v.r, src), qbd(
 // diagonal, rows [0, 16 (c / 16 + 1)), at sp[sp_col(c) + row]; the subdiagonal entry (c + 1, c)
 // of every column is kept apart.  2640 {S,P} entries for 65 columns (42 KB).
 __host__ __device__ constexpr int sp_col(int c) { return 16 * ((c >> 4) + 1) * (8 * (c >> 4) + (c & 15)); }
+static_assert(sp_col(65) + 65 <= PK_BLK, "block image");
+
+// Column c and row l of packed LDS item it = sp_col(c) + l (c < 80): a 16-column block qb
+// starts at 128 qb (qb + 1) and holds 16 (qb + 1) rows per column.
+__device__ __forceinline__ void sp_decode(int it, int &c, int &l)
+{
+    const int qb = (it >= 256) + (it >= 768) + (it >= 1536) + (it >= 2560);
+    const int off = it - 128 * qb * (qb + 1);
+    // off / (16 (qb + 1)) for off < 256 (qb + 1): (off / 16) * ceil(2^16 / (qb + 1)) / 2^16
+    const int mg = qb == 0 ? 65536 : (qb == 1 ? 32768 : (qb == 2 ? 21846 : (qb == 3 ? 16384 : 13108)));
+    const int cc = ((off >> 4) * mg) >> 16;
+    l = off - 16 * (qb + 1) * cc;
+    c = 16 * qb + cc;
+}
 
 // Rotations inside 16-lane rows (DPP row_ror): ror<1> gives lane j the value of lane j - 1
 // (lane 0 that of lane 15), ror<15> the value of lane j + 1 (lane 15 that of lane 0).
@@ -211,7 +228,14 @@ struct UnitOut {
     GroupInfo grp;
     MixedExp mx;
     SplitOut sq;
+    // fused near update (wl == nullptr: none): the block's operand rows of the unit's columns
+    // into LDS, wl[k * WL_LD + slot] (S part k < ks, P part ks + k), and the column scale of
+    // each slot into wsig (pair units use slots wslot and wslot + 1)
+    double *wl;
+    double *wsig;
+    int wslot, ks;
 };
+constexpr int WL_LD = 32; // LDS pitch of the near update's operand (columns of a workgroup)
 
 // One unit's block step: factor the lane's rows, sweep the block bottom-up starting from the
 // right-hand side `raw`, then write the block's solution, operands and scale state.
@@ -502,6 +526,35 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
             }
         }
     }
+    if (o.wl != nullptr) {
+        // the same operand values into LDS for the fused near update; rows [nb, ks) and
+        // inactive units are zero
+        constexpr bool CW = std::is_same<V, cpx>::value;
+#pragma clang loop unroll(full)
+        for (int i = 0; i < QR; ++i) {
+            const int r = QL * i + j;
+            if (r < o.ks) {
+                const bool live = act && r < nb;
+                const V v = vr[i];
+                const V bx = A::bmul(bR, bI, v);
+                double *w0 = o.wl + r * WL_LD + o.wslot;
+                double *w1 = o.wl + (o.ks + r) * WL_LD + o.wslot;
+                w0[0] = live ? -a * A::re(v) : 0.0;
+                w1[0] = live ? A::re(bx) : 0.0;
+                if (CW) {
+                    w0[1] = live ? -a * A::im(v) : 0.0;
+                    w1[1] = live ? A::im(bx) : 0.0;
+                }
+            }
+        }
+        if (j == 0) {
+            const double sg = ldexp(1.0, my_exp);
+            o.wsig[o.wslot] = sg;
+            if (CW) {
+                o.wsig[o.wslot + 1] = sg;
+            }
+        }
+    }
     const bool starts_here = dl >= 0 && dl < nb;
     if (act) {
         const int ce = o.ce0 + my_exp; // pair columns share their exponents
@@ -580,7 +633,6 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
             const int p = col + j;
             const int ce = o.ce0 + my_exp;
             const bool fresh = first_in_group || starts_here;
-            mx.wblk[p] = min(max(ew, -1000), 1000);
             if (grp.last) {
                 mx.wgrp[p] = min(max(fresh ? ew : max(ew, ce + mx.wgrp[p]), -1000), 1000);
             }
@@ -687,10 +739,64 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
 //     the group's lower blocks, which it never wrote;
 //   * in the group's last (top) step each unit brings its operand rows of the lower blocks
 //     to its final scale: W *= 2^(colexp - blkexp[step of the block]).
+// The block {S,P} comes from its packed image pkb (k_pack_steps).  With pkp != nullptr the
+// launch also applies the block to the group's rows above it (the near update), for rows
+// [r0n, i0) of the workgroup's columns:
+//   X[row][c] := sig[c] X[row][c] + sum_k S[row][i0+k] W_s[k][c] + P[row][i0+k] W_p[k][c]
+// with W_s = -a x, W_p = b x of this block (the values written to W) and sig the column's
+// rescale of this step.  The workgroup's columns (4 slots per real wave, 8 per complex wave,
+// at most 32) take the products on 16x16x4 FP64 MFMA over the panel image pkp (rows from g0n
+// in tiles of 16, the panel's k-steps: the S part over ks = nb rounded up to 4, then the P
+// part), the operand from LDS; each wave takes every nw-th row tile.
 // ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pack_steps(const double *__restrict__ S, long lds, const double *__restrict__ P,
+                                                    long ldp, const StepPack *__restrict__ steps,
+                                                    double2 *__restrict__ pk)
+{
+    const StepPack st = steps[blockIdx.y];
+    const int it = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i0 = st.i0, nb = st.nb;
+    if (it < PK_BLK) {
+        constexpr int NI = sp_col(65);
+        double2 v = make_double2(0.0, 0.0);
+        if (it < NI) {
+            int c, l;
+            sp_decode(it, c, l);
+            if (c < nb && l < nb) {
+                v = make_double2(S[(i0 + l) + (long)(i0 + c) * lds], P[(i0 + l) + (long)(i0 + c) * ldp]);
+            }
+        }
+        else if (it < NI + 65) {
+            const int c = it - NI;
+            if (c + 1 < nb) {
+                v = make_double2(S[(i0 + c + 1) + (long)(i0 + c) * lds], P[(i0 + c + 1) + (long)(i0 + c) * ldp]);
+            }
+        }
+        pk[st.boff + it] = v;
+        return;
+    }
+    const int q = it - PK_BLK; // a double of the panel image
+    const int kq = (nb + 3) / 4, per_rt = kq * 128;
+    if (q >= st.nrt * per_rt) {
+        return;
+    }
+    const int rt = q / per_rt, rem = q - rt * per_rt;
+    const int p = rem >> 7, lane = (rem >> 1) & 63, h = rem & 1;
+    const int kk = 2 * p + h;
+    const int fr = lane & 15, fk = lane >> 4;
+    const bool spart = kk < kq;
+    const int kc = 4 * (spart ? kk : kk - kq) + fk;
+    const int row = st.g0 + 16 * rt + fr;
+    double v = 0.0;
+    if (row < i0 && kc < nb) {
+        v = spart ? S[row + (long)(i0 + kc) * lds] : P[row + (long)(i0 + kc) * ldp];
+    }
+    reinterpret_cast<double *>(pk + st.poff)[q] = v;
+}
+
 template <int MAXB>
 __global__ __launch_bounds__(256) void k_solve_real(
-    const double *__restrict__ S, long lds, const double *__restrict__ P, long ldp, int i0, int nb,
+    const double2 *__restrict__ pkb, const double2 *__restrict__ pkp, int g0n, int r0n, int i0, int nb,
     unsigned long long bumps, double *__restrict__ X, long ldx, double *__restrict__ W, long ldw,
     const UnitDesc *__restrict__ tdesc, int t_begin, int t_end, int u_begin,
     int u_end, double *__restrict__ sig, int *__restrict__ colexp, int *__restrict__ blkexp, int *__restrict__ flag,
@@ -698,20 +804,38 @@ __global__ __launch_bounds__(256) void k_solve_real(
 {
     static_assert(MAXB <= 65, "block layout: the staging decode covers 5 column blocks");
     constexpr int QR = (MAXB + QL - 1) / QL; // rows per lane
+    constexpr int NI = sp_col(MAXB);
     __shared__ double2 sp[sp_col(MAXB)];     // sp[sp_col(c) + l] = {S_lc, P_lc}, l < 16 (c / 16 + 1)
     __shared__ double2 spd[MAXB];            // spd[c] = {S, P}_{c+1, c}
+    __shared__ double wl[2 * PK_KSMAX * WL_LD];
+    __shared__ double wsig[WL_LD];
+    __shared__ int wtyp[4];
+    __shared__ int ucol[4][4]; // working column of each unit (-1: inactive), bit 30: a pair
 
     const int wave = threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
+    const int nw = blockDim.x / WAVE;
     const int j = lane & (QL - 1);
     const int cq = lane / QL;
     const int t = min(t_begin + (int)(blockIdx.x * (blockDim.x / WAVE)) + wave, t_end - 1);
     const bool wave_live = t_begin + (int)(blockIdx.x * (blockDim.x / WAVE)) + wave < t_end;
+
+    // the block image: one 16-byte load per item, issued first (it needs no descriptor)
+    constexpr int PER = 11;
+    static_assert(PER * 256 >= NI + MAXB, "one pass of 256 threads");
+    double2 stg[PER];
+#pragma clang loop unroll(full)
+    for (int k = 0; k < PER; ++k) {
+        const int it = (int)threadIdx.x + k * (int)blockDim.x;
+        stg[k] = it < NI + MAXB ? pkb[it] : make_double2(0.0, 0.0);
+    }
+
     const UnitDesc ud = tdesc[4 * t + cq];
     const int u = ud.idx;
     const bool act = wave_live && u >= 0 && u >= u_begin && u < u_end;
     const int col = ud.col;
     const bool pair = ud.w == 2;
+    const bool cpxw = __any(act && pair);
     // the unit's column exponent and group scale, read now rather than after the sweep
     const int ce0 = act ? colexp[col] : 0;
     const double sgp = (act && j < ud.w) ? sig_grp[col + j] : 1.0;
@@ -729,7 +853,7 @@ __global__ __launch_bounds__(256) void k_solve_real(
     us.bI = ud.bI;
     us.bn1 = fabs(ud.bR) + fabs(ud.bI);
 
-    // right-hand side of the unit's rows in the block (loads issued before the staging)
+    // right-hand side of the unit's rows in the block
     RawRHS<QR> raw;
 #pragma clang loop unroll(full)
     for (int i = 0; i < QR; ++i) {
@@ -740,45 +864,36 @@ __global__ __launch_bounds__(256) void k_solve_real(
         raw.v[i].y = (ld && pair) ? xp[1] : 0.0;
     }
 
-    // stage {S,P} of the diagonal block in the packed layout (item it = sp index: column c, row
-    // l with sp_col(c) + l = it; a 16-column block qb starts at 128 qb (qb + 1)) and its
-    // subdiagonal
-    {
-        constexpr int NI = sp_col(MAXB);
-        constexpr int PER = 11;
-        for (int base = 0; base < NI; base += PER * (int)blockDim.x) {
-            double ts[PER], tp[PER];
+    // block image into LDS (items past the staged columns: the subdiagonal)
 #pragma clang loop unroll(full)
-            for (int k = 0; k < PER; ++k) {
-                const int it = base + (int)threadIdx.x + k * (int)blockDim.x;
-                const int qb = (it >= 256) + (it >= 768) + (it >= 1536) + (it >= 2560);
-                const int off = it - 128 * qb * (qb + 1);
-                // off / (16 (qb + 1)) for off < 256 (qb + 1): (off / 16) * ceil(2^16 / (qb + 1)) / 2^16
-                const int mg = qb == 0 ? 65536 : (qb == 1 ? 32768 : (qb == 2 ? 21846 : (qb == 3 ? 16384 : 13108)));
-                const int cc = ((off >> 4) * mg) >> 16;
-                const int l = off - 16 * (qb + 1) * cc;
-                const int c = 16 * qb + cc;
-                ts[k] = 0.0;
-                tp[k] = 0.0;
-                if (it < NI && c < nb && l < nb) {
-                    ts[k] = S[(i0 + l) + (long)(i0 + c) * lds];
-                    tp[k] = P[(i0 + l) + (long)(i0 + c) * ldp];
-                }
+    for (int k = 0; k < PER; ++k) {
+        const int it = (int)threadIdx.x + k * (int)blockDim.x;
+        if (it < NI) {
+            sp[it] = stg[k];
+        }
+        else if (it < NI + MAXB) {
+            spd[it - NI] = stg[k];
+        }
+    }
+    if (blockDim.x < 256) {
+        // fewer than 4 waves: the rest of the image
+        for (int it = (int)threadIdx.x + PER * (int)blockDim.x; it < NI + MAXB; it += (int)blockDim.x) {
+            const double2 e = pkb[it];
+            if (it < NI) {
+                sp[it] = e;
             }
-#pragma clang loop unroll(full)
-            for (int k = 0; k < PER; ++k) {
-                const int it = base + (int)threadIdx.x + k * (int)blockDim.x;
-                if (it < NI) {
-                    sp[it] = make_double2(ts[k], tp[k]);
-                }
+            else {
+                spd[it - NI] = e;
             }
         }
-        for (int c = threadIdx.x; c < MAXB; c += blockDim.x) {
-            double2 e = make_double2(0.0, 0.0);
-            if (c + 1 < nb) {
-                e = make_double2(S[(i0 + c + 1) + (long)(i0 + c) * lds], P[(i0 + c + 1) + (long)(i0 + c) * ldp]);
-            }
-            spd[c] = e;
+    }
+    const bool near = pkp != nullptr;
+    if (near) {
+        if (lane == 0) {
+            wtyp[wave] = cpxw ? 8 : 4;
+        }
+        if (j == 0) {
+            ucol[wave][cq] = act ? col | (pair ? 1 << 30 : 0) : -1;
         }
     }
     __syncthreads();
@@ -806,11 +921,129 @@ __global__ __launch_bounds__(256) void k_solve_real(
     o.grp = grp;
     o.mx = mx;
     o.sq = sq;
-    if (!__any(act && pair)) {
+    o.wl = nullptr;
+    o.wsig = wsig;
+    o.wslot = 0;
+    o.ks = (nb + 3) & ~3;
+    // near update: slots of the workgroup's columns (a real wave 4, a complex wave 8: two per
+    // unit), this wave's from base; cidx[ct] = the working column of slot 16 ct + (lane % 16)
+    int nslot = 0;
+    int cidx[2] = {-1, -1};
+    if (near) {
+        int base = 0;
+        for (int w = 0; w < nw; ++w) {
+            base += w < wave ? wtyp[w] : 0;
+            nslot += wtyp[w];
+        }
+        o.wl = wl;
+        o.wslot = base + (cpxw ? 2 : 1) * cq;
+#pragma clang loop unroll(full)
+        for (int ct = 0; ct < 2; ++ct) {
+            int sl = 16 * ct + (lane & 15);
+            for (int w = 0; w < nw; ++w) {
+                const int tw = wtyp[w];
+                if (sl >= 0 && sl < tw) {
+                    const int per = tw / 4, uc = ucol[w][sl / per];
+                    const int c = uc & ~(1 << 30);
+                    cidx[ct] = uc < 0 ? -1 : (sl % per == 0 ? c : ((uc >> 30) & 1 ? c + 1 : -1));
+                }
+                sl -= tw;
+            }
+        }
+    }
+    if (!cpxw) {
         unit_block<double, QR, MAXB>(us, raw, o);
     }
     else {
         unit_block<cpx, QR, MAXB>(us, raw, o);
+    }
+    if (!near) {
+        return;
+    }
+
+    // ---- near update on MFMA: the wave's row tiles rt of the panel (every nw-th), the panel's
+    // k-step pairs and the outputs' current X loaded one tile ahead of the products
+    const int ks = o.ks, kq = ks / 4, nks = 2 * kq;
+    const int nrt = (i0 - g0n + 15) / 16;
+    const int nct = (nslot + 15) / 16;
+    const int fr = lane & 15, fk = lane >> 4;
+    constexpr int KQM = PK_KSMAX / 4;
+    auto load_tile = [&](int rt, double2 (&af)[KQM], double (&cv)[2][4]) {
+        const double2 *pa = pkp + (long)rt * kq * 64 + lane;
+#pragma clang loop unroll(full)
+        for (int p = 0; p < KQM; ++p) {
+            af[p] = p < kq ? pa[p * 64] : make_double2(0.0, 0.0);
+        }
+#pragma clang loop unroll(full)
+        for (int v = 0; v < 4; ++v) {
+            const int row = g0n + 16 * rt + fk + 4 * v;
+            const bool ok = row >= r0n && row < i0;
+#pragma clang loop unroll(full)
+            for (int ct = 0; ct < 2; ++ct) {
+                cv[ct][v] = (ok && cidx[ct] >= 0) ? X[(long)row * ldx + cidx[ct]] : 0.0;
+            }
+        }
+    };
+    double csg[2];
+    auto tile = [&](int rt, const double2 (&af)[KQM], const double (&cv)[2][4]) {
+        double4_t acc[2];
+#pragma clang loop unroll(full)
+        for (int ct = 0; ct < 2; ++ct) {
+#pragma clang loop unroll(full)
+            for (int v = 0; v < 4; ++v) {
+                acc[ct][v] = 0.0;
+            }
+        }
+#pragma clang loop unroll(full)
+        for (int kk = 0; kk < 2 * KQM; ++kk) {
+            if (kk < nks) {
+                const double av = (kk & 1) ? af[kk >> 1].y : af[kk >> 1].x;
+                const double *wr = wl + (4 * kk + fk) * WL_LD + fr;
+                acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, wr[0], acc[0], 0, 0, 0);
+                if (nct > 1) {
+                    acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, wr[16], acc[1], 0, 0, 0);
+                }
+            }
+        }
+#pragma clang loop unroll(full)
+        for (int v = 0; v < 4; ++v) {
+            const int row = g0n + 16 * rt + fk + 4 * v;
+#pragma clang loop unroll(full)
+            for (int ct = 0; ct < 2; ++ct) {
+                if (row >= r0n && row < i0 && cidx[ct] >= 0 && ct < nct) {
+                    X[(long)row * ldx + cidx[ct]] = fma(csg[ct], cv[ct][v], acc[ct][v]);
+                }
+            }
+        }
+    };
+    // the wave's tiles rt = rtw + nw k, k < cnt, taken from k = blockIdx.x mod cnt on (every
+    // workgroup reads the whole panel: started at one place they would all meet in the same L2
+    // channels)
+    const int rtw = (r0n - g0n) / 16 + wave;
+    const int cnt = rtw < nrt ? (nrt - rtw + nw - 1) / nw : 0;
+    const int k0 = cnt > 0 ? (int)(blockIdx.x % (unsigned)cnt) : 0;
+    auto rt_of = [&](int k) { return rtw + nw * (k0 + k < cnt ? k0 + k : k0 + k - cnt); };
+    double2 afA[KQM], afB[KQM];
+    double cvA[2][4], cvB[2][4];
+    if (cnt > 0) {
+        load_tile(rt_of(0), afA, cvA); // the panel and X do not depend on the other waves' solves
+    }
+    __syncthreads();
+#pragma clang loop unroll(full)
+    for (int ct = 0; ct < 2; ++ct) {
+        csg[ct] = wsig[16 * ct + fr];
+    }
+    for (int k = 0; k < cnt; k += 2) {
+        if (k + 1 < cnt) {
+            load_tile(rt_of(k + 1), afB, cvB);
+        }
+        tile(rt_of(k), afA, cvA);
+        if (k + 2 < cnt) {
+            load_tile(rt_of(k + 2), afA, cvA);
+        }
+        if (k + 1 < cnt) {
+            tile(rt_of(k + 1), afB, cvB);
+        }
     }
 }
 
@@ -832,8 +1065,6 @@ __global__ __launch_bounds__(256) void k_solve_real(
 // clamped to the last one: they only feed outputs that are never stored.
 // WIDE: 16-byte loads of column pairs (M, N >= 2); otherwise one double per load.
 // ---------------------------------------------------------------------------------------
-typedef double double4_t __attribute__((ext_vector_type(4)));
-typedef float float4_t __attribute__((ext_vector_type(4)));
 
 // F32 (mixed form): operands rounded to FP32 at the LDS store, products and accumulation on
 // v_mfma_f32_16x16x4 (FP32), the running C stays FP64 (C = sig * C + D in FP64).  To keep
@@ -1647,7 +1878,7 @@ __global__ void k_zero_int(int *__restrict__ X, long count)
 } // namespace xtg
 
 namespace xtg {
-template __global__ void k_solve_real<65>(const double *, long, const double *, long, int, int, unsigned long long,
+template __global__ void k_solve_real<65>(const double2 *, const double2 *, int, int, int, int, unsigned long long,
                                           double *, long, double *, long, const UnitDesc *, int, int,
                                           int, int, double *, int *, int *, int *, int, int, double *, int, GroupInfo,
                                           MixedExp, SplitOut);
