@@ -54,7 +54,8 @@ constexpr int SOLVE_WAVES_H = 4; // waves per solve workgroup (they share the st
 int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const double *L, long ldl, const double *L2,
                      long ldl2, const double *R, long ldr, double *C, long ldc, const int *outcol, int mode,
                      const double *sig, const int *kmax, const double *R2, int prec, const int *sa,
-                     const int *sb, int batch = 1, long bsr = 0, long bsc = 0);
+                     const int *sb, int batch = 1, long bsr = 0, long bsc = 0, const int *afl = nullptr,
+                     int aoff = 0);
 constexpr int GROUP_STEPS = 4; // row blocks per far update (K = 2 * ~256)
 __global__ void k_fixup_scale(double *, long, int, int, const int *, const int *, long, const int *, const int *);
 __global__ void k_fixup_steps(double *, long, int, const int *, const int *, long, const int *, const int *);
@@ -63,6 +64,7 @@ __global__ void k_normalize(double *, long, int, const int *, const int *);
 __global__ void k_round_f32(const double *, long, long, float *, long);
 __global__ void k_reverse_transpose(const double *, long, int, double *, long);
 __global__ void k_colnorms(const double *, long, const double *, long, int, int, double *);
+template <int TQ>
 __global__ void k_bwd_rhs(const double *, long, const double *, long, int, int, int, const int *, double *, long, int);
 __global__ void k_bwd_permute(const double *, long, int, int, int, int, int, double *, long);
 __global__ void k_bwd_gcorr(const double *, long, const double *, long, int, int, double *, long);
@@ -377,11 +379,12 @@ struct RealWork {
     DevBuf bfS, bfP, bfV, bfG, bfGc, bfGt, bfVt, bfZt, bfW, bfGh, bfdS, bfdP, bfk;
     DevBuf Ut, YB, part, tcol, tw; // eigenvalue condition numbers
     DevBuf pk, pksteps;            // packed block and panel images of the sweep's steps
+    DevBuf colo;                   // adjoint sweep: first row of each column's system
     DevBuf Sq, Pq, Wg, wgexp, wbexp, xbexp, sstart, Xq, Zq, xexp, cfar; // split-int8 digits and exponents
     // split-int8 form: side stream (default priority) for the digit splitting of the pencil
     // and of Z, overlapped with the sweep; events order it against the caller's stream
     hipStream_t side = nullptr;
-    hipEvent_t ev_start = nullptr, ev_pencil = nullptr, ev_z = nullptr, ev_y = nullptr;
+    hipEvent_t ev_start = nullptr, ev_pencil = nullptr, ev_z = nullptr, ev_y = nullptr, ev_rhs = nullptr;
     // far-update lookahead: the rows above the next group run on `fs`
     hipStream_t fs = nullptr;
     hipEvent_t ev_wready[2] = {nullptr, nullptr}, ev_fdone[2] = {nullptr, nullptr};
@@ -521,6 +524,15 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
     ENSURE(ws.desc, sizeof(UnitDesc) * tdesc.size());
     ENSURE(ws.rowstep, sizeof(int) * (size_t)n);
     XTG_CHECK(hipMemcpyAsync(ws.desc.p, tdesc.data(), sizeof(UnitDesc) * tdesc.size(), hipMemcpyHostToDevice, st));
+    if (adjoint) {
+        // first row of each column's system (ascending): the far updates skip the tiles above it
+        std::vector<int> lo(m);
+        for (int c = 0; c < m; ++c) {
+            lo[c] = desc[c].lo;
+        }
+        ENSURE(ws.colo, sizeof(int) * (size_t)m);
+        XTG_CHECK(hipMemcpyAsync(ws.colo.p, lo.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+    }
     MixedExp mx{nullptr, nullptr, 0};
     const bool split = prec == XTGEVC3_PREC_SPLIT_I8;
     long qld = 0, qsl = 0;
@@ -755,7 +767,8 @@ int sweep_real(RealWork &ws, int n, const double *Sw, long ldsw, const double *P
                                    X + (long)ra * ldx + c_begin, ldx, (const int *)nullptr, 1,
                                    (const double *)(sgb + c_begin), (const int *)nullptr,
                                    (const double *)nullptr, prec, mx.on ? ws.rowexp.as<int>() + ra : nullptr,
-                                   mx.on ? mxb.wgrp + c_begin : nullptr); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
+                                   mx.on ? mxb.wgrp + c_begin : nullptr, 1, 0, 0,
+                                   adjoint ? ws.colo.as<int>() + c_begin : nullptr, ra); XTG_LAUNCH_CHECK("k_gemm_f64<0>");
                 prof_end(pr, fst, update_kind(gt), 2.0 * Mp * N * (2.0 * wr),
                          8.0 * (2.0 * Mp * N + 2.0 * wr * Mp + 2.0 * wr * N));
             }
@@ -817,7 +830,7 @@ static hipError_t split_side_init(RealWork &ws)
             ws.side = nullptr;
             return e;
         }
-        for (hipEvent_t *ev : {&ws.ev_start, &ws.ev_pencil, &ws.ev_z, &ws.ev_y}) {
+        for (hipEvent_t *ev : {&ws.ev_start, &ws.ev_pencil, &ws.ev_z, &ws.ev_y, &ws.ev_rhs}) {
             e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
             if (e != hipSuccess) {
                 return e;
@@ -1258,7 +1271,7 @@ static hipError_t noalloc_setup(int dev)
     DevBuf *unused[] = {&ws.rowexp, &ws.zexp, &ws.wgrp, &ws.xrel, &ws.xfin, &ws.kcol, &ws.coef,
                         &ws.Vt,     &ws.Y,    &ws.LamS, &ws.bVp, &ws.bnfk, &ws.btiles, &ws.bkpre, &ws.Ut,  &ws.YB,   &ws.part,
                         &ws.tcol,   &ws.tw,   &ws.Sq,   &ws.Pq,   &ws.Wg,    &ws.wgexp, &ws.wbexp, &ws.xbexp,
-                        &ws.sstart, &ws.Xq,   &ws.Zq,   &ws.xexp, &ws.cfar};
+                        &ws.sstart, &ws.Xq,   &ws.Zq,   &ws.xexp, &ws.cfar, &ws.colo};
     for (DevBuf *b : unused) {
         b->fixed = true;
     }
@@ -1511,9 +1524,59 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
     ENSURE(ws.bVp, sizeof(double) * (size_t)npad * m);
     double *X = ws.X.as<double>();
     const bool split = prec == XTGEVC3_PREC_SPLIT_I8;
+    // dS_b = V diag(-a) Lam_b, dP_b = V diag(b) Lam_b  (as D[j][i] = sum_c s_c V[j][c] Lam[c][i]),
+    // all batches and both products in one launch.  v_c is zero below row kcol[c] and kcol
+    // ascends, so with the columns taken in reverse order (negative strides) row j of the
+    // product needs only the first kpre[j] = #{c : kcol[c] >= j} of them: a per-row K bound.
+    // Only the support is formed (dS: rows i <= j + 1, dP: i <= j; zeros elsewhere), where entry
+    // (i, j) needs the columns with kcol[c] >= max(j, i + 1) (lambda_c[i] = 0 for kcol[c] <= i).
+    std::vector<int> kpre(n);
+    for (int j = 0, c = 0; j < n; ++j) {
+        while (c < m && kcol[c] < j) {
+            ++c;
+        }
+        kpre[j] = m - c;
+    }
+    // tiles of the products: those with work by descending K range (the hardware deals
+    // consecutive workgroups to the XCDs in turn), then, as a second list, the tiles wholly below
+    // the band, which only write zeros and run on the side stream beside the sweep
+    std::vector<int> tiles;
+    int nwork = 0, nzero = 0;
+    if (n >= 2) {
+        const int ta = (n + 127) / 128, tb = (n + 63) / 64;
+        std::vector<std::pair<int, int>> work; // (-K, packed tile)
+        for (int a = 0; a < ta; ++a) {
+            for (int b = 0; b < tb; ++b) {
+                const int a0 = 128 * a, b0 = 64 * b;
+                if (b0 > a0 + 128) {
+                    continue;
+                }
+                const int kk = std::min(kpre[a0], b0 + 1 < n ? kpre[b0 + 1] : 0);
+                work.push_back({-kk, (a << 16) | b});
+            }
+        }
+        std::stable_sort(work.begin(), work.end(),
+                         [](const std::pair<int, int> &x, const std::pair<int, int> &y) { return x.first < y.first; });
+        for (const auto &w : work) {
+            tiles.push_back(w.second);
+        }
+        nwork = (int)tiles.size();
+        for (int a = 0; a < ta; ++a) {
+            for (int b = 0; b < tb; ++b) {
+                if (64 * b > 128 * a + 128) {
+                    tiles.push_back((a << 16) | b);
+                }
+            }
+        }
+        nzero = (int)tiles.size() - nwork;
+        ENSURE(ws.btiles, sizeof(int) * tiles.size());
+        XTG_CHECK(hipMemcpyAsync(ws.btiles.p, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, st));
+    }
     // beside the sweep, on the side stream (inputs only): Y = P V (the alpha / beta terms need
-    // lambda^T P v) and the forward vectors with rows padded to npad (zero past n, 16-byte aligned
-    // columns) for the outer products
+    // lambda^T P v) and the forward vectors with rows padded to npad (zero past n, 16-byte
+    // aligned columns) for the outer products; then (zero_work) the zero tiles of dS and dP.
+    // Both wait for the adjoint right-hand sides (ev_rhs on st), so that they meet the
+    // latency-bound sweep rather than the bandwidth-bound passes before it; ev_y marks their end
     auto side_work = [&]() -> int {
         hipLaunchKernelGGL(k_to_rowmajor, dim3((m + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, ws.side, V, ldv, n, m,
                            ws.Vt.as<double>(), ldt); XTG_LAUNCH_CHECK("k_to_rowmajor");
@@ -1525,6 +1588,18 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
         hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, ws.side, ws.bVp.as<double>(), npad * m); XTG_LAUNCH_CHECK("k_zero");
         hipLaunchKernelGGL(k_bwd_permute, dim3((n + 255) / 256, m), dim3(256), 0, ws.side, V, ldv, n, m, 1, 0, 0,
                            ws.bVp.as<double>(), npad); XTG_LAUNCH_CHECK("k_bwd_permute");
+        return 0;
+    };
+    auto zero_work = [&]() -> int {
+        XTG_CHECK(hipStreamWaitEvent(ws.side, ws.ev_rhs, 0));
+        if (nzero > 0) {
+            // below-band tiles: no operand is read
+            hipLaunchKernelGGL(k_bwd_outer, dim3((unsigned)nzero, nbatch), dim3(256), 0, ws.side, n, m,
+                               (const double *)nullptr, 0L, (const double *)nullptr, 0L, (const double *)nullptr,
+                               (const double *)nullptr, dS, ldds, dP, lddp, (const int *)nullptr,
+                               (const int *)nullptr, (const int *)ws.btiles.as<int>() + nwork, 0L, 0L, (long)n * ldds,
+                               (long)n * lddp); XTG_LAUNCH_CHECK("k_bwd_outer");
+        }
         XTG_CHECK(hipEventRecord(ws.ev_y, ws.side));
         return 0;
     };
@@ -1534,12 +1609,6 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
     // sides and (FP64) the side-stream work
     std::vector<double> nrm;
     XTG_CHECK_RC(read_norms(ws, n, st, nrm, [&]() -> int {
-        if (!split) {
-            XTG_CHECK(split_side_init(ws));
-            XTG_CHECK(hipEventRecord(ws.ev_start, st));
-            XTG_CHECK(hipStreamWaitEvent(ws.side, ws.ev_start, 0));
-            XTG_CHECK_RC(side_work());
-        }
         XTG_CHECK(hipMemcpyAsync(ws.kcol.p, kcol.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
         dim3 g32((n + 31) / 32, (n + 31) / 32);
         hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, S, lds, n, ws.Sw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
@@ -1547,9 +1616,24 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
         // the right-hand sides: every entry of the rows < n and columns < M is written (zero
         // below each column's system), and nothing reads the padding columns of X
         const int pr = prof_begin(st);
-        hipLaunchKernelGGL(k_bwd_rhs, dim3((M + 3) / 4), dim3(256), 0, st, V, ldv, G, ldg, n, m, nbatch, ws.kcol.as<int>(), X, ldx,
-                           rhs_given ? 0 : 1); XTG_LAUNCH_CHECK("k_bwd_rhs");
+        if (M >= 4096) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bwd_rhs<16>), dim3((M + 15) / 16), dim3(256), 0, st, V, ldv, G, ldg, n,
+                               m, nbatch, ws.kcol.as<int>(), X, ldx, rhs_given ? 0 : 1); XTG_LAUNCH_CHECK("k_bwd_rhs");
+        }
+        else {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bwd_rhs<4>), dim3((M + 3) / 4), dim3(256), 0, st, V, ldv, G, ldg, n, m,
+                               nbatch, ws.kcol.as<int>(), X, ldx, rhs_given ? 0 : 1); XTG_LAUNCH_CHECK("k_bwd_rhs");
+        }
         prof_end(pr, st, 3, 0.0, 8.0 * 3.0 * n * M);
+        XTG_CHECK(split_side_init(ws));
+        XTG_CHECK(hipEventRecord(ws.ev_rhs, st));
+        if (!split) {
+            // the side work starts after the right-hand sides (bandwidth-bound, on the chain)
+            // and runs beside the latency-bound sweep
+            XTG_CHECK(hipStreamWaitEvent(ws.side, ws.ev_rhs, 0));
+            XTG_CHECK_RC(side_work());
+            XTG_CHECK_RC(zero_work());
+        }
         return 0;
     }));
     std::vector<double> sub(nrm.begin() + 2 * (long)n, nrm.end());
@@ -1610,6 +1694,7 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
         prof_end(pr, ws.side, XTGEVC3_PROF_SPLIT_DIGITS, 0.0, 8.0 * 4.0 * n * (double)n + 2.0 * SPLIT_DIGITS * qsl);
         XTG_CHECK(hipEventRecord(ws.ev_pencil, ws.side));
         XTG_CHECK_RC(side_work());
+        XTG_CHECK_RC(zero_work());
     }
     int rc = sweep_real(ws, n, ws.Sw.as<double>(), n, ws.Pw.as<double>(), n, subw, desc, true, ldx, st, prec,
                         split ? ws.ev_pencil : nullptr);
@@ -1624,19 +1709,6 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
     hipLaunchKernelGGL(k_bwd_lambda_t, dim3((M + 31) / 32, (int)((npad + 31) / 32)), dim3(32, 8), 0, st,
                        (const double *)X, ldx, n, m, nbatch, ws.kcol.as<int>(), ws.colexp.as<int>(),
                        ws.LamS.as<double>(), npad, ws.bovf.as<int>(), ws.bnfk.as<int>()); XTG_LAUNCH_CHECK("k_bwd_lambda_t");
-    // dS_b = V diag(-a) Lam_b, dP_b = V diag(b) Lam_b  (as D[j][i] = sum_c s_c V[j][c] Lam[c][i]),
-    // all batches and both products in one launch.  v_c is zero below row kcol[c] and kcol
-    // ascends, so with the columns taken in reverse order (negative strides) row j of the
-    // product needs only the first kpre[j] = #{c : kcol[c] >= j} of them: a per-row K bound.
-    // Only the support is formed (dS: rows i <= j + 1, dP: i <= j; zeros elsewhere), where entry
-    // (i, j) needs the columns with kcol[c] >= max(j, i + 1) (lambda_c[i] = 0 for kcol[c] <= i).
-    std::vector<int> kpre(n);
-    for (int j = 0, c = 0; j < n; ++j) {
-        while (c < m && kcol[c] < j) {
-            ++c;
-        }
-        kpre[j] = m - c;
-    }
     double kwork = 0.0; // sum over the dS support of the exact K range of each entry
     for (int j = 0; j < n; ++j) {
         kwork += (double)j * kpre[j] + (j + 1 < n ? kpre[j + 1] : 0) + (j + 2 < n ? kpre[j + 2] : 0);
@@ -1665,35 +1737,8 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
             XTG_CHECK(hipMemset2DAsync(dP, sizeof(double) * lddp, 0, sizeof(double) * n, (size_t)nbatch * n, st));
         }
         else {
-            // tile order: tiles with work by descending K range, then those below the band
-            const int ta = (n + 127) / 128, tb = (n + 63) / 64;
-            std::vector<std::pair<int, int>> work; // (-K, packed tile)
-            std::vector<int> tiles;
-            for (int a = 0; a < ta; ++a) {
-                for (int b = 0; b < tb; ++b) {
-                    const int a0 = 128 * a, b0 = 64 * b;
-                    if (b0 > a0 + 128) {
-                        continue;
-                    }
-                    const int kk = std::min(kpre[a0], b0 + 1 < n ? kpre[b0 + 1] : 0);
-                    work.push_back({-kk, (a << 16) | b});
-                }
-            }
-            std::stable_sort(work.begin(), work.end(),
-                             [](const std::pair<int, int> &x, const std::pair<int, int> &y) { return x.first < y.first; });
-            for (const auto &w : work) {
-                tiles.push_back(w.second);
-            }
-            for (int a = 0; a < ta; ++a) {
-                for (int b = 0; b < tb; ++b) {
-                    if (64 * b > 128 * a + 128) {
-                        tiles.push_back((a << 16) | b);
-                    }
-                }
-            }
-            ENSURE(ws.btiles, sizeof(int) * tiles.size());
-            XTG_CHECK(hipMemcpyAsync(ws.btiles.p, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, st));
-            const dim3 grid((unsigned)tiles.size(), nbatch);
+            // the tiles with work (the zero tiles ran on the side stream)
+            const dim3 grid((unsigned)nwork, nbatch);
             hipLaunchKernelGGL(k_bwd_outer, grid, dim3(256), 0, st, n, m, Vrev, -npad,
                                (const double *)(ws.LamS.as<double>() + lrev), -npad,
                                (const double *)ws.sfac.as<double>(), (const double *)(ws.sfac.as<double>() + ms), dS,

@@ -451,14 +451,14 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
                 }
             }
             V t = vr[i];
-            if (r >= 1 && ((bumps >> (r - 1)) & 1ull)) {
+            if (__builtin_expect(r >= 1 && ((bumps >> (r - 1)) & 1ull), 0)) {
                 // lower row of a 2x2 bump: eliminate with the upper row's current value
                 const V prev = (jr > 0 || i == 0) ? ror<1>(vr[i]) : ror<1>(vr[i > 0 ? i - 1 : 0]);
                 t = A::sub(t, A::mul(g[i], prev));
             }
             V x = A::bc(A::mul(t, f[i]), jr);
             const bool bad = !(A::abs1(x) <= kScaleTrigger);
-            if (__any(bad)) {
+            if (__builtin_expect(__any(bad), 0)) {
                 const int eo = bad ? rescale_exponent(A::abs1(t), A::abs1(f[i])) : 0;
                 const int e = qbi(eo, jr);
                 const double sc = ldexp(1.0, -e);
@@ -1055,6 +1055,9 @@ __global__ __launch_bounds__(256) void k_solve_real(
 //   mode 0: C = D;  mode 1: C = sig[b] * C + D.
 //   kmax (optional): per-a upper bound of k (exclusive); a tile uses the max over its rows,
 //   so triangular operands skip their zero part.
+//   afl (optional, non-decreasing in b): rows a < afl[b] - aoff of column b are not needed (rows
+//   outside the column's system); a tile whose rows all lie above afl[b0] - aoff of its first
+//   column is skipped, and tiles are taken column tiles fastest (not XCD-grouped).
 //   Batches (gridDim.y > 1): batch y reads R + y*bsr (and R2 + y*bsr) and writes C + y*bsc;
 //   L, kmax and the other arguments are shared.
 // Workgroup: 256 threads (2x2 waves), tile GT (a) x GT (b) (each wave GT/2 x GT/2), K chunks
@@ -1078,7 +1081,8 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
                                                   long ldc, const int *__restrict__ outcol, int mode,
                                                   const double *__restrict__ sig, const int *__restrict__ kmax,
                                                   const double *__restrict__ R2, const int *__restrict__ sa,
-                                                  const int *__restrict__ sb, long bsr, long bsc)
+                                                  const int *__restrict__ sb, long bsr, long bsc,
+                                                  const int *__restrict__ afl, int aoff)
 {
     R += (long)blockIdx.y * bsr;
     C += (long)blockIdx.y * bsc;
@@ -1094,9 +1098,21 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
 
     // XCD-aware tile order; with a per-row K bound the heaviest row tiles come first
     int tm, tn;
-    xcd_tile(blockIdx.x, (M + GTA - 1) / GTA, (N + GTB - 1) / GTB, tm, tn);
+    if (afl == nullptr) {
+        xcd_tile(blockIdx.x, (M + GTA - 1) / GTA, (N + GTB - 1) / GTB, tm, tn);
+    }
+    else {
+        // column tiles fastest: consecutive workgroups (dealt to the 8 XCDs in turn) take
+        // consecutive column tiles, so the skipped tiles spread evenly over the XCDs
+        const int tn_n = (N + GTB - 1) / GTB;
+        tm = blockIdx.x / tn_n;
+        tn = blockIdx.x - tm * tn_n;
+    }
     const int a0 = tm * GTA;
     const int b0 = tn * GTB;
+    if (afl != nullptr && aoff + a0 + GTA <= afl[b0]) {
+        return;
+    }
     const int tid = threadIdx.x;
     const int wave = tid / WAVE;
     const int lane = tid % WAVE;
@@ -1900,62 +1916,46 @@ namespace xtg {
 // RHS of the adjoint solve in working rows: X[(n-1-i)*ldx + q] = g_i - [i == i*] (g.v) sign(v_i*)
 // for i < kcol[c] (zero elsewhere), where working column q = (m-1-c)*nbatch + b holds batch b of
 // forward column c and i* is the first index of max |v_i| (correct == 0: G is already the
-// right-hand side, copied without the correction term).  One workgroup per 4 working
-// columns: each wave reduces one column (lane stride 64, fixed order, so a column's result does
-// not depend on nbatch), then 4 x 512 tiles through LDS write rows of X as 4 contiguous doubles
-// while g is read along i (one column per workgroup wrote X at a full row's stride per value;
-// 8 and 32 columns per workgroup left too few workgroups for one cotangent batch).
+// right-hand side, copied without the correction term).  One workgroup per TQ working
+// columns and one pass over G: TQ x 512 tiles of g go through LDS to rows of X (TQ contiguous
+// doubles per row) while each thread accumulates g.v over its rows (rows i with the same
+// i mod 256, in order), then the partial sums are added in a fixed tree (so a column's
+// result depends on neither nbatch nor TQ) and the row i* of each column is rewritten with
+// its correction.  TQ = 4 keeps one cotangent batch on enough workgroups; with many batches
+// TQ = 16 writes whole 128-byte lines of X.
+template <int TQ>
 __global__ __launch_bounds__(256) void k_bwd_rhs(const double *__restrict__ V, long ldv, const double *__restrict__ G,
                                                  long ldg, int n, int m, int nbatch, const int *__restrict__ kcol,
                                                  double *__restrict__ X, long ldx, int correct)
 {
-    constexpr int TQ = 4, TI = 512;
+    constexpr int TI = 512;
+    static_assert((TQ * TI) % 256 == 0 && TI == 2 * 256, "a thread takes rows ii and ii + 256 of each column");
     __shared__ double tile[TQ][TI + 1];
-    __shared__ double s_corr[TQ];
-    __shared__ int s_bi[TQ], s_k[TQ];
+    __shared__ double red[256 / WAVE][TQ];
+    __shared__ int s_k[TQ];
     const int M = m * nbatch, q0 = blockIdx.x * TQ;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    for (int j = wave; j < TQ; j += 256 / WAVE) {
-        const int q = q0 + j;
-        if (q >= M) {
-            continue;
-        }
-        const int c = m - 1 - q / nbatch, b = q % nbatch;
-        const double *v = V + (long)c * ldv;
-        const double *g = G + (long)(b * m + c) * ldg;
-        double best = -1.0, dot = 0.0;
-        int bi = 0;
-        for (int i = lane; i < n; i += WAVE) {
-            const double a = fabs(v[i]);
-            if (a > best) {
-                best = a;
-                bi = i;
-            }
-            dot = fma(g[i], v[i], dot);
-        }
-        for (int off = WAVE / 2; off > 0; off >>= 1) {
-            const double ob = __shfl_xor(best, off);
-            const int oi = __shfl_xor(bi, off);
-            if (ob > best || (ob == best && oi < bi)) {
-                best = ob;
-                bi = oi;
-            }
-            dot += __shfl_xor(dot, off);
-        }
-        if (lane == 0) {
-            s_bi[j] = bi;
-            s_corr[j] = correct ? (v[bi] < 0.0 ? -dot : dot) : 0.0;
-            s_k[j] = kcol[c];
-        }
+    if (threadIdx.x < TQ) {
+        const int q = q0 + threadIdx.x;
+        s_k[threadIdx.x] = q < M ? kcol[m - 1 - q / nbatch] : 0;
     }
     __syncthreads();
+    double dot[TQ];
+#pragma clang loop unroll(full)
+    for (int j = 0; j < TQ; ++j) {
+        dot[j] = 0.0;
+    }
     for (int i0 = 0; i0 < n; i0 += TI) {
-        for (int t = threadIdx.x; t < TQ * TI; t += 256) {
+#pragma clang loop unroll(full)
+        for (int u = 0; u < TQ * TI / 256; ++u) {
+            const int t = (int)threadIdx.x + 256 * u;
             const int j = t / TI, ii = t % TI, i = i0 + ii, q = q0 + j;
             double val = 0.0;
-            if (q < M && i < s_k[j]) {
+            if (q < M && i < n) {
                 const int c = m - 1 - q / nbatch, b = q % nbatch;
-                val = G[(long)(b * m + c) * ldg + i] - (i == s_bi[j] ? s_corr[j] : 0.0);
+                const double g = G[(long)(b * m + c) * ldg + i];
+                dot[j] = fma(g, V[(long)c * ldv + i], dot[j]);
+                val = i < s_k[j] ? g : 0.0;
             }
             tile[j][ii] = val;
         }
@@ -1968,7 +1968,62 @@ __global__ __launch_bounds__(256) void k_bwd_rhs(const double *__restrict__ V, l
         }
         __syncthreads();
     }
+    if (!correct) {
+        return;
+    }
+#pragma clang loop unroll(full)
+    for (int j = 0; j < TQ; ++j) {
+        double d = dot[j];
+#pragma clang loop unroll(full)
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+            d += __shfl_xor(d, off);
+        }
+        if (lane == 0) {
+            red[wave][j] = d;
+        }
+    }
+    __syncthreads();
+    // row i* of each column: the first index of max |v_i| (one column per wave at a time)
+    for (int j = wave; j < TQ; j += 256 / WAVE) {
+        const int q = q0 + j;
+        if (q >= M) {
+            continue;
+        }
+        const int c = m - 1 - q / nbatch, b = q % nbatch;
+        const double *v = V + (long)c * ldv;
+        double best = -1.0;
+        int bi = 0;
+        for (int i = lane; i < n; i += WAVE) {
+            const double a = fabs(v[i]);
+            if (a > best) {
+                best = a;
+                bi = i;
+            }
+        }
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(best, off);
+            const int oi = __shfl_xor(bi, off);
+            if (ob > best || (ob == best && oi < bi)) {
+                best = ob;
+                bi = oi;
+            }
+        }
+        if (lane == 0 && bi < s_k[j]) {
+            double tot = 0.0;
+#pragma clang loop unroll(full)
+            for (int w = 0; w < 256 / WAVE; ++w) {
+                tot += red[w][j];
+            }
+            const double corr = v[bi] < 0.0 ? -tot : tot;
+            X[(long)(n - 1 - bi) * ldx + q] = G[(long)(b * m + c) * ldg + bi] - corr;
+        }
+    }
 }
+
+template __global__ void k_bwd_rhs<4>(const double *, long, const double *, long, int, int, int, const int *, double *,
+                                      long, int);
+template __global__ void k_bwd_rhs<16>(const double *, long, const double *, long, int, int, int, const int *, double *,
+                                       long, int);
 
 // Lam[(b*m + c) * ldl + i] = 2^-e * lambda_i of working column q (batch b, forward column c),
 // for i < ldl (zero for i >= n: ldl pads the rows), in 32x32 LDS tiles over (q, i).  A
@@ -2188,7 +2243,7 @@ int gemm_config(int M, int N, int K, int batch)
 int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const double *L, long ldl, const double *L2,
                      long ldl2, const double *R, long ldr, double *C, long ldc, const int *outcol, int mode,
                      const double *sig, const int *kmax, const double *R2, int prec, const int *sa,
-                     const int *sb, int batch, long bsr, long bsc)
+                     const int *sb, int batch, long bsr, long bsc, const int *afl, int aoff)
 {
     const bool wide = M >= 2 && N >= 2;
     const int cfg = gemm_config(M, N, K1 + K2, batch);
@@ -2198,12 +2253,13 @@ int launch_gemm_f64(int tag, hipStream_t st, int M, int N, int K1, int K2, const
 #define XTG_GEMM_LAUNCH(T, WD, GA, GB, K)                                                                            \
     if (prec == 1 && T != 2) {                                                                                       \
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, WD, GA, GB, K, true>), grid, dim3(256), 0, st, M, N, K1, K2, \
-                           L, ldl, L2, ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2, sa, sb, bsr, bsc);         \
+                           L, ldl, L2, ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2, sa, sb, bsr, bsc, afl,    \
+                           aoff);                                                                                    \
     }                                                                                                                \
     else {                                                                                                           \
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gemm_f64<T, WD, GA, GB, K>), grid, dim3(256), 0, st, M, N, K1, K2, L,  \
                            ldl, L2, ldl2, R, ldr, C, ldc, outcol, mode, sig, kmax, R2, (const int *)nullptr,         \
-                           (const int *)nullptr, bsr, bsc);                                                          \
+                           (const int *)nullptr, bsr, bsc, afl, aoff);                                               \
     }
 #define XTG_GEMM_WIDE(T, GA, GB, K)                                                                                  \
     if (wide) {                                                                                                      \
