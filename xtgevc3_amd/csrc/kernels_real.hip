@@ -74,9 +74,7 @@ template <int SRC> __device__ __forceinline__ int row_bcast_i(int v)
 
 template <int SRC> __device__ __forceinline__ double row_bcast_d(double v)
 {
-    int lo = row_bcast_i<SRC>(__double2loint(v));
-    int hi = row_bcast_i<SRC>(__double2hiint(v));
-    return __hiloint2double(hi, lo);
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + SRC, 0xF, 0xF, false); // one v_mov_b64_dpp
 }
 
 #define XTG_RB_CASES(F)                                                                                          \
@@ -421,7 +419,9 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
 #pragma clang loop unroll(full)
         for (int jr = QL - 1; jr >= 0; --jr) {
             const int r = QL * i + jr;
-            if (r >= MAXB || r >= nb) {
+            // rows nb <= r < MAXB run too: their factors, right-hand sides and {S,P} entries are
+            // zero, so they solve to zero and change nothing (no per-row test on nb)
+            if (r >= MAXB) {
                 continue;
             }
             if (jr == 0 && i > 0 && r + 1 < nb) {
