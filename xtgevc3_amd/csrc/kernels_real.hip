@@ -416,6 +416,9 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
             xb[m] = A::mk(0.0, 0.0);
             xe[m] = 0;
         }
+        // the next row's owner lane keeps its updated value in tn (the select that stores a
+        // row's solution in its owner lane then stays off the dependent chain)
+        V tn = A::mk(0.0, 0.0);
 #pragma clang loop unroll(full)
         for (int jr = QL - 1; jr >= 0; --jr) {
             const int r = QL * i + jr;
@@ -450,14 +453,16 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
                     }
                 }
             }
-            V t = vr[i];
+            V t = (jr < QL - 1 && r + 1 < MAXB) ? tn : vr[i];
             if (__builtin_expect(r >= 1 && ((bumps >> (r - 1)) & 1ull), 0)) {
                 // lower row of a 2x2 bump: eliminate with the upper row's current value
                 const V prev = (jr > 0 || i == 0) ? ror<1>(vr[i]) : ror<1>(vr[i > 0 ? i - 1 : 0]);
                 t = A::sub(t, A::mul(g[i], prev));
             }
-            V x = A::bc(A::mul(t, f[i]), jr);
-            const bool bad = !(A::abs1(x) <= kScaleTrigger);
+            const V tf = A::mul(t, f[i]);
+            V x = A::bc(tf, jr);
+            // tested on the owner lane before the broadcast (every lane of the row receives its value)
+            const bool bad = j == jr && !(A::abs1(tf) <= kScaleTrigger);
             if (__builtin_expect(__any(bad), 0)) {
                 const int eo = bad ? rescale_exponent(A::abs1(t), A::abs1(f[i])) : 0;
                 const int e = qbi(eo, jr);
@@ -475,7 +480,8 @@ __device__ __forceinline__ void unit_block(const UnitStep u, const RawRHS<QR> ra
             xa[jr] = ax;
             xb[jr] = bx;
             xe[jr] = es;
-            vr[i] = vsel(j == jr, x, A::upd(vr[i], dc[jr], ax, bx));
+            tn = A::upd(vr[i], dc[jr], ax, bx);
+            vr[i] = vsel(j == jr, x, tn);
             if (jr == 0) {
                 // row 16 i into the slots above (not into the preset row dl from column dl + 1)
 #pragma clang loop unroll(full)
