@@ -31,13 +31,12 @@ SPLIT_PRODUCTS = 15  # digit products per multiply-add in the split-int8 form (l
 # each scaled by the counter's reading on a plain 1 GiB device copy (FETCH x2.0, WRITE x1.0);
 # summary and commands in profiles/dtgevc3_R_B_n8192_hbm_traffic.md
 MEASURED_TRAFFIC = {
-    ("xtg::k_gemm_f64<1, true, 128, 128, 16, false>", 8192): 13453.4e6,
-    ("xtg::k_gemm_f64<0, true, 128, 64, 16, false>", 8192): 411.6e6,
-    ("xtg::k_gemm_f64<0, true, 128, 128, 16, false>", 8192): 215.4e6,
+    ("xtg::k_gemm_f64<1, true, 128, 128, 16, false>", 8192): 11731.3e6,
+    ("xtg::k_gemm_f64<0, true, 128, 64, 16, false>", 8192): 409.5e6,
+    ("xtg::k_gemm_f64<0, true, 128, 128, 16, false>", 8192): 215.7e6,
     ("xtg::k_gemm_f64<0, true, 64, 64, 32, false>", 8192): 59.2e6,
-    ("xtg::k_gemm_f64<0, true, 32, 64, 32, false>", 8192): 16.3e6,
-    ("xtg::k_solve_real<65>", 8192): 25.5e6,
-    ("xtg::k_gemm_i8<128, 128, 2, 4> (back-transform)", 8192): 5140.1e6,
+    ("xtg::k_solve_real<65>", 8192): 40.3e6,
+    ("xtg::k_gemm_i8<128, 128, 2, 4> (back-transform)", 8192): 4899.9e6,
 }
 
 
