@@ -69,11 +69,12 @@ __global__ void k_bwd_rhs(const double *, long, const double *, long, int, int, 
 __global__ void k_bwd_permute(const double *, long, int, int, int, int, int, double *, long);
 __global__ void k_bwd_gcorr(const double *, long, const double *, long, int, int, double *, long);
 __global__ void k_bwd_trunc(double *, long, int, int, const int *);
-__global__ void k_bwd_lambda_t(const double *, long, int, int, int, const int *, const int *, double *, long, int *,
-                               int *);
+__global__ void k_bwd_lambda_t(const double *, long, int, int, int, const int *, const int *, const int *, double *,
+                               long, int *, int *);
 __global__ void k_to_rowmajor(const double *, long, int, int, double *, long);
 __global__ void k_bwd_diag(const double *, long, const double *, long, int, int, int, const int *, const double *,
-                           const int *, double *, long, double *, long);
+                           const int *, const int *, double *);
+__global__ void k_bwd_diag_add(const double *, int, int, int, const int *, double *, long, double *, long);
 __global__ void k_subdiag(const double *, long, int, double *);
 __global__ void k_bwd_outer(int, int, const double *, long, const double *, long, const double *, const double *, double *,
                             long, double *, long, const int *, const int *, const int *, long, long, long, long);
@@ -380,6 +381,7 @@ struct RealWork {
     DevBuf Ut, YB, part, tcol, tw; // eigenvalue condition numbers
     DevBuf pk, pksteps;            // packed block and panel images of the sweep's steps
     DevBuf colo;                   // adjoint sweep: first row of each column's system
+    DevBuf bdg;                    // backward: the diagonal (alpha, beta) terms of each working column
     DevBuf Sq, Pq, Wg, wgexp, wbexp, xbexp, sstart, Xq, Zq, xexp, cfar; // split-int8 digits and exponents
     // split-int8 form: side stream (default priority) for the digit splitting of the pencil
     // and of Z, overlapped with the sweep; events order it against the caller's stream
@@ -1271,7 +1273,7 @@ static hipError_t noalloc_setup(int dev)
     DevBuf *unused[] = {&ws.rowexp, &ws.zexp, &ws.wgrp, &ws.xrel, &ws.xfin, &ws.kcol, &ws.coef,
                         &ws.Vt,     &ws.Y,    &ws.LamS, &ws.bVp, &ws.bnfk, &ws.btiles, &ws.bkpre, &ws.Ut,  &ws.YB,   &ws.part,
                         &ws.tcol,   &ws.tw,   &ws.Sq,   &ws.Pq,   &ws.Wg,    &ws.wgexp, &ws.wbexp, &ws.xbexp,
-                        &ws.sstart, &ws.Xq,   &ws.Zq,   &ws.xexp, &ws.cfar, &ws.colo};
+                        &ws.sstart, &ws.Xq,   &ws.Zq,   &ws.xexp, &ws.cfar, &ws.colo, &ws.bdg};
     for (DevBuf *b : unused) {
         b->fixed = true;
     }
@@ -1609,10 +1611,16 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
     // sides and (FP64) the side-stream work
     std::vector<double> nrm;
     XTG_CHECK_RC(read_norms(ws, n, st, nrm, [&]() -> int {
-        XTG_CHECK(hipMemcpyAsync(ws.kcol.p, kcol.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+        // the working pencil (reversed transposes) on the side stream beside the right-hand
+        // sides; ev_z marks it, and the sweep waits for it
+        XTG_CHECK(split_side_init(ws));
+        XTG_CHECK(hipEventRecord(ws.ev_start, st));
+        XTG_CHECK(hipStreamWaitEvent(ws.side, ws.ev_start, 0));
         dim3 g32((n + 31) / 32, (n + 31) / 32);
-        hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, S, lds, n, ws.Sw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
-        hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, st, P, ldp, n, ws.Pw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
+        hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, ws.side, S, lds, n, ws.Sw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
+        hipLaunchKernelGGL(k_reverse_transpose, g32, dim3(32, 8), 0, ws.side, P, ldp, n, ws.Pw.as<double>(), (long)n); XTG_LAUNCH_CHECK("k_reverse_transpose");
+        XTG_CHECK(hipEventRecord(ws.ev_z, ws.side));
+        XTG_CHECK(hipMemcpyAsync(ws.kcol.p, kcol.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
         // the right-hand sides: every entry of the rows < n and columns < M is written (zero
         // below each column's system), and nothing reads the padding columns of X
         const int pr = prof_begin(st);
@@ -1625,7 +1633,6 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
                                nbatch, ws.kcol.as<int>(), X, ldx, rhs_given ? 0 : 1); XTG_LAUNCH_CHECK("k_bwd_rhs");
         }
         prof_end(pr, st, 3, 0.0, 8.0 * 3.0 * n * M);
-        XTG_CHECK(split_side_init(ws));
         XTG_CHECK(hipEventRecord(ws.ev_rhs, st));
         if (!split) {
             // the side work starts after the right-hand sides (bandwidth-bound, on the chain)
@@ -1696,8 +1703,10 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
         XTG_CHECK_RC(side_work());
         XTG_CHECK_RC(zero_work());
     }
+    XTG_CHECK(hipStreamWaitEvent(st, ws.ev_z, 0)); // the working pencil
+    // (no scale fix-up pass: k_bwd_lambda_t and k_bwd_diag read each row at its step's exponent)
     int rc = sweep_real(ws, n, ws.Sw.as<double>(), n, ws.Pw.as<double>(), n, subw, desc, true, ldx, st, prec,
-                        split ? ws.ev_pencil : nullptr);
+                        split ? ws.ev_pencil : nullptr, false);
     if (rc) {
         return rc;
     }
@@ -1707,8 +1716,19 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
     hipLaunchKernelGGL(k_zero_int, dim3((M + 255) / 256), dim3(256), 0, st, ws.bovf.as<int>(), (long)M); XTG_LAUNCH_CHECK("k_zero_int");
     hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, st, ws.bnfk.as<int>(), (long)nbatch * npad, INT_MAX); XTG_LAUNCH_CHECK("k_fill_int");
     hipLaunchKernelGGL(k_bwd_lambda_t, dim3((M + 31) / 32, (int)((npad + 31) / 32)), dim3(32, 8), 0, st,
-                       (const double *)X, ldx, n, m, nbatch, ws.kcol.as<int>(), ws.colexp.as<int>(),
-                       ws.LamS.as<double>(), npad, ws.bovf.as<int>(), ws.bnfk.as<int>()); XTG_LAUNCH_CHECK("k_bwd_lambda_t");
+                       (const double *)X, ldx, n, m, nbatch, ws.kcol.as<int>(), (const int *)ws.rowstep.as<int>(),
+                       (const int *)ws.blkexp.as<int>(), ws.LamS.as<double>(), npad, ws.bovf.as<int>(),
+                       ws.bnfk.as<int>()); XTG_LAUNCH_CHECK("k_bwd_lambda_t");
+    // the diagonal (alpha, beta) terms on the side stream, beside the outer products (they read
+    // X and Y only); added into dS_kk, dP_kk after them
+    ENSURE(ws.bdg, sizeof(double) * 2 * (size_t)M);
+    XTG_CHECK(hipEventRecord(ws.ev_start, st));
+    XTG_CHECK(hipStreamWaitEvent(ws.side, ws.ev_start, 0));
+    hipLaunchKernelGGL(k_bwd_diag, dim3((M + 15) / 16), dim3(256), 0, ws.side, (const double *)X, ldx,
+                       (const double *)ws.Y.as<double>(), ldt, n, m, nbatch, ws.kcol.as<int>(), ws.coef.as<double>(),
+                       (const int *)ws.rowstep.as<int>(), (const int *)ws.blkexp.as<int>(),
+                       ws.bdg.as<double>()); XTG_LAUNCH_CHECK("k_bwd_diag");
+    XTG_CHECK(hipEventRecord(ws.ev_y, ws.side));
     double kwork = 0.0; // sum over the dS support of the exact K range of each entry
     for (int j = 0; j < n; ++j) {
         kwork += (double)j * kpre[j] + (j + 1 < n ? kpre[j + 1] : 0) + (j + 2 < n ? kpre[j + 2] : 0);
@@ -1749,9 +1769,9 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
         prof_end(pr, st, 3, 4.0 * kwork * nbatch,
                  8.0 * nbatch * ((double)n * m + (double)m * n + 2.0 * (double)n * n));
     }
-    hipLaunchKernelGGL(k_bwd_diag, dim3((M + 15) / 16), dim3(256), 0, st, (const double *)X, ldx, (const double *)ws.Y.as<double>(),
-                       ldt, n, m, nbatch, ws.kcol.as<int>(), ws.coef.as<double>(), ws.colexp.as<int>(), dS, ldds, dP,
-                       lddp); XTG_LAUNCH_CHECK("k_bwd_diag");
+    XTG_CHECK(hipStreamWaitEvent(st, ws.ev_y, 0));
+    hipLaunchKernelGGL(k_bwd_diag_add, dim3((M + 255) / 256), dim3(256), 0, st, (const double *)ws.bdg.as<double>(), n,
+                       m, nbatch, ws.kcol.as<int>(), dS, ldds, dP, lddp); XTG_LAUNCH_CHECK("k_bwd_diag_add");
     XTG_CHECK(hipGetLastError());
     // info > 0: the number of (batch, column) adjoints whose lambda exceeds the double range;
     // the gradient entries that depend on them are NaN, every other entry is finite and exact

@@ -2031,14 +2031,16 @@ template __global__ void k_bwd_rhs<4>(const double *, long, const double *, long
 template __global__ void k_bwd_rhs<16>(const double *, long, const double *, long, int, int, int, const int *, double *,
                                        long, int);
 
-// Lam[(b*m + c) * ldl + i] = 2^-e * lambda_i of working column q (batch b, forward column c),
-// for i < ldl (zero for i >= n: ldl pads the rows), in 32x32 LDS tiles over (q, i).  A
+// Lam[(b*m + c) * ldl + i] = lambda_i = 2^-e X[n-1-i][q] of working column q (batch b, forward
+// column c; e = blkexp of the column at the step that solved working row n-1-i, whose scale the
+// row keeps), for i < ldl (zero for i >= n: ldl pads the rows), in 32x32 LDS tiles over (q, i).  A
 // non-finite entry is written as zero, ovf[q] = 1 marks the column, and nfk[b * ldl + i] keeps
 // the smallest reversed column index m - 1 - c of such an entry in row i (the K index of the
 // outer products, which take the columns in reverse order; nfk starts at INT_MAX).
 __global__ void k_bwd_lambda_t(const double *__restrict__ X, long ldx, int n, int m, int nbatch,
-                               const int *__restrict__ kcol, const int *__restrict__ colexp,
-                               double *__restrict__ Lam, long ldl, int *__restrict__ ovf, int *__restrict__ nfk)
+                               const int *__restrict__ kcol, const int *__restrict__ rowstep,
+                               const int *__restrict__ blkexp, double *__restrict__ Lam, long ldl,
+                               int *__restrict__ ovf, int *__restrict__ nfk)
 {
     __shared__ double tile[32][33];
     const int q0 = blockIdx.x * 32, i0 = blockIdx.y * 32;
@@ -2050,7 +2052,8 @@ __global__ void k_bwd_lambda_t(const double *__restrict__ X, long ldx, int n, in
         if (i < n && q < M) {
             int c = m - 1 - q / nbatch;
             if (i < kcol[c]) {
-                v = ldexp(X[(long)(n - 1 - i) * ldx + q], -colexp[q]);
+                const long r = n - 1 - i;
+                v = ldexp(X[r * ldx + q], -blkexp[(long)rowstep[r] * ldx + q]);
             }
         }
         tile[kk][tx] = v;
@@ -2152,16 +2155,18 @@ __global__ void k_to_rowmajor(const double *__restrict__ V, long ldv, int n, int
     }
 }
 
-// dL/dalpha and dL/dbeta of working column q folded into dS_kk, dP_kk, with
-//   Y = P V (row-major, ldy), lambda_i = 2^-e X[n-1-i][q]:
+// dL/dalpha and dL/dbeta of working column q, the terms t_S = s qv and t_P = -s (b/a) qv with
+// qv = lambda^T (P v), Y = P V (row-major, ldy), lambda_i = 2^-e X[n-1-i][q] (e = blkexp of the
+// column at the step that solved working row n-1-i: the sweep's rows stay at their steps'
+// scales), written to out[2q], out[2q + 1] (k_bwd_diag_add folds them into dS_kk, dP_kk):
 // 16 working columns per workgroup, thread t = column t % 16 and row phase t / 16 (a row of X is
 // read as 16 contiguous values), the rows i < k split over the 16 phases (i = phase + 16 r, four
 // running sums per thread), the partial sums added in fixed order.
 constexpr int BD_QC = 16;
 __global__ __launch_bounds__(256) void k_bwd_diag(const double *__restrict__ X, long ldx, const double *__restrict__ Y,
                                                   long ldy, int n, int m, int nbatch, const int *__restrict__ kcol,
-                                                  const double *__restrict__ coef, const int *__restrict__ colexp,
-                                                  double *__restrict__ dS, long ldds, double *__restrict__ dP, long lddp)
+                                                  const double *__restrict__ coef, const int *__restrict__ rowstep,
+                                                  const int *__restrict__ blkexp, double *__restrict__ out)
 {
     constexpr int QC = BD_QC, PH = 256 / BD_QC;
     __shared__ double part[PH][QC];
@@ -2184,7 +2189,8 @@ __global__ __launch_bounds__(256) void k_bwd_diag(const double *__restrict__ X, 
         for (int u = 0; u < 4; ++u) {
             const int i = i0 + PH * u;
             if (i < k) {
-                acc[u] = fma(xq[-(long)i * ldx], yc[(long)i * ldy], acc[u]);
+                const double lam = ldexp(xq[-(long)i * ldx], -blkexp[(long)rowstep[n - 1 - i] * ldx + q]);
+                acc[u] = fma(lam, yc[(long)i * ldy], acc[u]);
             }
         }
     }
@@ -2196,12 +2202,25 @@ __global__ __launch_bounds__(256) void k_bwd_diag(const double *__restrict__ X, 
         for (int r = 0; r < PH; ++r) {
             tot += part[r][col];
         }
-        const double qv = ldexp(tot, -colexp[q]);
-        const int b = q % nbatch;
+        const double qv = tot;
         const double s = coef[4 * c + 2], ratio = coef[4 * c + 3];
-        dS[(long)b * n * ldds + k + (long)k * ldds] += s * qv;
-        dP[(long)b * n * lddp + k + (long)k * lddp] += -s * ratio * qv;
+        out[2 * (long)q] = s * qv;
+        out[2 * (long)q + 1] = -s * ratio * qv;
     }
+}
+
+// dS_kk += t_S, dP_kk += t_P of each working column q (batch b = q % nbatch, forward column
+// c = m - 1 - q / nbatch, k = kcol[c]; one column per thread, every (b, k) once).
+__global__ void k_bwd_diag_add(const double *__restrict__ t, int n, int m, int nbatch, const int *__restrict__ kcol,
+                               double *__restrict__ dS, long ldds, double *__restrict__ dP, long lddp)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m * nbatch) {
+        return;
+    }
+    const int c = m - 1 - q / nbatch, b = q % nbatch, k = kcol[c];
+    dS[(long)b * n * ldds + k + (long)k * ldds] += t[2 * (long)q];
+    dP[(long)b * n * lddp + k + (long)k * lddp] += t[2 * (long)q + 1];
 }
 
 // Tile configuration of a product with M x N outputs and depth K: the one with the smallest
