@@ -78,6 +78,7 @@ __global__ void k_bwd_diag_add(const double *, int, int, int, const int *, doubl
 __global__ void k_subdiag(const double *, long, int, double *);
 __global__ void k_bwd_outer(int, int, const double *, long, const double *, long, const double *, const double *, double *,
                             long, double *, long, const int *, const int *, const int *, long, long, long, long);
+__global__ void k_bwd_zero_tiles(int, double *, long, double *, long, const int *, long, long);
 __global__ void k_zero(double *, long);
 __global__ void k_zero_int(int *, long);
 
@@ -1595,12 +1596,10 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
     auto zero_work = [&]() -> int {
         XTG_CHECK(hipStreamWaitEvent(ws.side, ws.ev_rhs, 0));
         if (nzero > 0) {
-            // below-band tiles: no operand is read
-            hipLaunchKernelGGL(k_bwd_outer, dim3((unsigned)nzero, nbatch), dim3(256), 0, ws.side, n, m,
-                               (const double *)nullptr, 0L, (const double *)nullptr, 0L, (const double *)nullptr,
-                               (const double *)nullptr, dS, ldds, dP, lddp, (const int *)nullptr,
-                               (const int *)nullptr, (const int *)ws.btiles.as<int>() + nwork, 0L, 0L, (long)n * ldds,
-                               (long)n * lddp); XTG_LAUNCH_CHECK("k_bwd_outer");
+            // below-band tiles: zeros only, no operand is read
+            hipLaunchKernelGGL(k_bwd_zero_tiles, dim3((unsigned)nzero, nbatch), dim3(256), 0, ws.side, n, dS, ldds, dP,
+                               lddp, (const int *)ws.btiles.as<int>() + nwork, (long)n * ldds, (long)n * lddp);
+            XTG_LAUNCH_CHECK("k_bwd_zero_tiles");
         }
         XTG_CHECK(hipEventRecord(ws.ev_y, ws.side));
         return 0;
@@ -1738,8 +1737,9 @@ static int real_backward_core(RealWork &ws, int n, const double *S, long lds, co
     {
         // dS_b = V diag(-a) Lam_b and dP_b = V diag(b) Lam_b in one pass over V and Lam_b (the
         // columns in reverse order, so that row j needs only the first kpre[j] of them)
-        // s1 = sf[0, m), s2 = sf[ms, ms + m), each followed by zeros (the kernel reads whole K steps)
-        const int ms = m + 16;
+        // s1 = sf[0, m), s2 = sf[ms, ms + m), each followed by zeros (the kernel reads whole K steps,
+        // 16 bytes per lane: ms is even, so both start 16-byte aligned)
+        const int ms = (m + 17) & ~1;
         std::vector<double> sf(2 * (size_t)ms, 0.0);
         for (int k = 0; k < m; ++k) {
             const int c = m - 1 - k;

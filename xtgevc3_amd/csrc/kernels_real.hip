@@ -1450,21 +1450,23 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K1, int K2, 
 // a non-finite lambda entry as zero and keeps in nfk[b] the first k of such an entry in column
 // b; an output whose exact range [0, kmax[a]) includes it becomes NaN.  Batches: gridDim.y,
 // R + y bsr, nfk + y bsn, C1 + y bsc1, C2 + y bsc2.  tiles[blockIdx.x] = (a tile << 16) | b tile:
-// the tiles with work in descending order of their K range, then the tiles below the band, so
-// that consecutive workgroups (which the hardware deals to the 8 XCDs in turn) spread the work
+// the tiles with work (b0 <= a0 + 128) in descending order of their K range, so that
+// consecutive workgroups (which the hardware deals to the 8 XCDs in turn) spread the work
 // evenly over the XCDs, heaviest first.
 // Support: only the entries the forward reads carry a gradient, C1 (dL/dS) on the upper
 // Hessenberg band b <= a + 1, C2 (dL/dP) on the upper triangle b <= a; every other entry is
-// written as zero.  Tiles wholly below the band only write zeros, and a tile's K range also
+// written as zero (the tiles wholly below the band by k_bwd_zero_tiles).  A tile's K range
 // stops at kmax[b0 + 1] = #{c : k_c > b0}: R[k][b] = lambda_c[b] is an exact zero for k_c <= b.
 // Tile 128 (a) x 64 (b), four waves of 64 x 32 per product, K steps of 16 rows staged by
-// direct global -> LDS loads (16 bytes per lane) into a 3-stage ring two steps ahead (no fence
-// at the stage barrier, so the loads stay in flight across it); the factors s1, s2 (scalar
-// loads per step, zero past the tile's K range; both arrays hold OB_GK entries past K) multiply
-// the A fragment in registers.
+// direct global -> LDS loads (16 bytes per lane) into a 2-stage ring one step ahead, together
+// with the step's factors s1, s2 (both arrays hold OB_GK entries past K), which multiply the A
+// fragment in registers (zero past the tile's K range).  50 KB of LDS and at most 168 VGPRs, so
+// that three workgroups share a compute unit: while one waits for its loads or writes its
+// outputs, the other two keep the FP64 MFMA pipes busy.  Each wave skips its MFMA steps past
+// its own K range.
 // ---------------------------------------------------------------------------------------
-constexpr int OB_GA = 128, OB_GB = 64, OB_GK = 16, OB_NST = 3;
-__global__ __launch_bounds__(256) void k_bwd_outer(int n, int K, const double *__restrict__ L, long ldl,
+constexpr int OB_GA = 128, OB_GB = 64, OB_GK = 16, OB_NST = 2;
+__global__ __launch_bounds__(256, 3) void k_bwd_outer(int n, int K, const double *__restrict__ L, long ldl,
                                                    const double *__restrict__ R, long ldr,
                                                    const double *__restrict__ s1, const double *__restrict__ s2,
                                                    double *__restrict__ C1, long ldc1, double *__restrict__ C2,
@@ -1474,32 +1476,41 @@ __global__ __launch_bounds__(256) void k_bwd_outer(int n, int K, const double *_
 {
     constexpr int GTA = OB_GA, GTB = OB_GB, GK = OB_GK;
     constexpr int FTA = GTA / 32, FTB = GTB / 32;
-    constexpr int STAGE = GK * (GTA + GTB); // doubles per ring stage
+    constexpr int FOFF = GK * (GTA + GTB); // the step's factors s1, s2 after the operand rows
+    constexpr int STAGE = FOFF + 128;      // doubles per ring stage
     constexpr int IPW = (GK * GTA / 128 + GK * GTB / 128) / 4; // 16-byte load instructions per wave and stage
     static_assert(IPW == 6, "staging layout: 4 L rows and 2 R row pairs per wave");
-    R += (long)blockIdx.y * bsr;
-    nfk += (long)blockIdx.y * bsn;
-    C1 += (long)blockIdx.y * bsc1;
-    C2 += (long)blockIdx.y * bsc2;
+    // (tile, batch) of this workgroup: with a multiple of 8 batches each XCD (the hardware deals
+    // consecutive workgroups to the 8 XCDs in turn) takes whole batches, tile-major, so that its
+    // L2 holds the lambda of its own batches only
+    int bx = blockIdx.x, by = blockIdx.y;
+    if (gridDim.y % 8 == 0) {
+        const long id = (long)blockIdx.y * gridDim.x + blockIdx.x;
+        const long pos = id / 8;
+        const int bpx = gridDim.y / 8;
+        bx = (int)(pos / bpx);
+        by = (int)(id % 8) + 8 * (int)(pos % bpx);
+    }
+    R += (long)by * bsr;
+    nfk += (long)by * bsn;
+    C1 += (long)by * bsc1;
+    C2 += (long)by * bsc2;
     __shared__ __attribute__((aligned(16))) double lds[OB_NST * STAGE];
     const int npad = (n + 1) & ~1;
-    const int tp = tiles[blockIdx.x];
+    const int tp = tiles[bx];
     const int a0 = (tp >> 16) * GTA, b0 = (tp & 0xffff) * GTB;
     const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
     const int wa = (wave >> 1) * (GTA / 2), wb = (wave & 1) * (GTB / 2);
-    if (b0 > a0 + GTA) {
-        // below the band: b > a + 1 for every entry of the tile
-        for (int e = tid; e < GTA * GTB; e += 256) {
-            const int ar = a0 + e / GTB, bc = b0 + e % GTB;
-            if (ar < n && bc < n) {
-                C1[(long)ar * ldc1 + bc] = 0.0;
-                C2[(long)ar * ldc2 + bc] = 0.0;
-            }
-        }
-        return;
-    }
     const int KK = min(min(K, kmax[a0]), b0 + 1 < n ? kmax[b0 + 1] : 0);
     const int nst = (KK + GK - 1) / GK;
+    // this wave's own K range (its first row and first column bound every term of its block;
+    // the terms of k in [KW, KK) are exact zeros for its outputs), zero when every entry of its
+    // block lies below the band; its MFMA steps past KW are skipped
+    int KW = a0 + wa < n ? min(KK, kmax[a0 + wa]) : 0;
+    KW = min(KW, b0 + wb + 1 < n ? kmax[b0 + wb + 1] : 0);
+    if (b0 + wb > a0 + wa + 64) {
+        KW = 0;
+    }
 
     // per-lane sources: L rows 4 u + wave (u < 4), lane pair 2 lane of the row; R row pairs
     // 2 t, 2 t + 1 with t = 4 u + wave (u < 2), lanes 0-31 the first row, 32-63 the second.
@@ -1525,6 +1536,14 @@ __global__ __launch_bounds__(256) void k_bwd_outer(int n, int K, const double *_
                                              (__attribute__((address_space(3))) void *)(stage + GK * GTA + 2 * t * GTB),
                                              16, 0, 0);
         }
+        if (wave == 0) {
+            // s1[k0, k0 + 16) by lanes 0-7, s2[k0, k0 + 16) by lanes 8-15 (both arrays hold GK
+            // entries past K); lanes 16-63 repeat lanes 0-15 into the padding
+            const int l = lane % 16;
+            const double *g = (l < 8 ? s1 : s2) + k0 + 2 * (l % 8);
+            __builtin_amdgcn_global_load_lds((const void *)g, (__attribute__((address_space(3))) void *)(stage + FOFF),
+                                             16, 0, 0);
+        }
     };
 
     double4_t acc1[FTA][FTB], acc2[FTA][FTB];
@@ -1542,59 +1561,42 @@ __global__ __launch_bounds__(256) void k_bwd_outer(int n, int K, const double *_
     if (nst > 0) {
         issue(0);
     }
-    if (nst > 1) {
-        issue(1);
-    }
     const int fr = lane % 16, fk = lane / 16;
     for (int st = 0; st < nst; ++st) {
-        if (st + 1 < nst) {
-            wait_vm<IPW>();
-        }
-        else {
-            wait_vm<0>();
-        }
+        wait_vm<0>();
         // every wave's loads of this stage have landed and every wave is done with the stage
         // refilled next
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_sched_barrier(0);
-        if (st + 2 < nst) {
-            issue(st + 2);
+        if (st + 1 < nst) {
+            issue(st + 1);
         }
         const double *Ls = lds + (st % OB_NST) * STAGE;
         const double *Rs = Ls + GK * GTA;
+        const double *Fs = Ls + FOFF;
         const int k0 = st * GK;
-        // the step's factors (unconditional uniform loads: s1 and s2 hold GK entries past K),
-        // zero past the tile's K range
-        double f1v[GK], f2v[GK];
-#pragma clang loop unroll(full)
-        for (int q = 0; q < GK; ++q) {
-            f1v[q] = s1[k0 + q];
-            f2v[q] = s2[k0 + q];
-        }
-#pragma clang loop unroll(full)
-        for (int q = 0; q < GK; ++q) {
-            if (k0 + q >= KK) {
-                f1v[q] = 0.0;
-                f2v[q] = 0.0;
-            }
-        }
 #pragma clang loop unroll(full)
         for (int ks = 0; ks < GK; ks += 4) {
-            const double f1 = fk == 0 ? f1v[ks] : fk == 1 ? f1v[ks + 1] : fk == 2 ? f1v[ks + 2] : f1v[ks + 3];
-            const double f2 = fk == 0 ? f2v[ks] : fk == 1 ? f2v[ks + 1] : fk == 2 ? f2v[ks + 2] : f2v[ks + 3];
-            double bf[FTB];
-#pragma clang loop unroll(full)
-            for (int j = 0; j < FTB; ++j) {
-                bf[j] = Rs[(ks + fk) * GTB + wb + j * 16 + fr];
-            }
-#pragma clang loop unroll(full)
-            for (int i = 0; i < FTA; ++i) {
-                const double af = Ls[(ks + fk) * GTA + wa + i * 16 + fr];
-                const double a1 = af * f1, a2 = af * f2;
+            if (k0 + ks < KW) {
+                // the lane's factors for k = k0 + ks + fk, zero past the tile's K range (the
+                // staged rows there repeat row K - 1)
+                const bool in = k0 + ks + fk < KK;
+                const double f1 = in ? Fs[ks + fk] : 0.0;
+                const double f2 = in ? Fs[GK + ks + fk] : 0.0;
+                double bf[FTB];
 #pragma clang loop unroll(full)
                 for (int j = 0; j < FTB; ++j) {
-                    acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bf[j], acc1[i][j], 0, 0, 0);
-                    acc2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bf[j], acc2[i][j], 0, 0, 0);
+                    bf[j] = Rs[(ks + fk) * GTB + wb + j * 16 + fr];
+                }
+#pragma clang loop unroll(full)
+                for (int i = 0; i < FTA; ++i) {
+                    const double af = Ls[(ks + fk) * GTA + wa + i * 16 + fr];
+                    const double a1 = af * f1, a2 = af * f2;
+#pragma clang loop unroll(full)
+                    for (int j = 0; j < FTB; ++j) {
+                        acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bf[j], acc1[i][j], 0, 0, 0);
+                        acc2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bf[j], acc2[i][j], 0, 0, 0);
+                    }
                 }
             }
         }
@@ -1626,6 +1628,31 @@ __global__ __launch_bounds__(256) void k_bwd_outer(int n, int K, const double *_
                 }
             }
         }
+    }
+}
+
+// k_bwd_zero_tiles: the tiles of k_bwd_outer's grid that lie wholly below the band (b0 > a0 + 128)
+// hold zeros only, in both gradients; this kernel writes them.  It holds no LDS and few
+// registers, so that its workgroups fit on the compute units beside the block solves of the
+// adjoint sweep that it runs next to (on a second stream), instead of keeping them out as
+// k_bwd_outer's 50 KB operand ring would.  tiles[blockIdx.x] = (a tile << 16) | b tile as in
+// k_bwd_outer; batches gridDim.y, C1 + y bsc1, C2 + y bsc2.
+__global__ __launch_bounds__(256) void k_bwd_zero_tiles(int n, double *__restrict__ C1, long ldc1,
+                                                        double *__restrict__ C2, long ldc2,
+                                                        const int *__restrict__ tiles, long bsc1, long bsc2)
+{
+    C1 += (long)blockIdx.y * bsc1;
+    C2 += (long)blockIdx.y * bsc2;
+    const int tp = tiles[blockIdx.x];
+    const int a0 = (tp >> 16) * OB_GA, b0 = (tp & 0xffff) * OB_GB;
+    const int bc = b0 + (int)threadIdx.x % OB_GB;
+    if (bc >= n) {
+        return;
+    }
+    const int a1 = min(a0 + OB_GA, n);
+    for (int ar = a0 + (int)threadIdx.x / OB_GB; ar < a1; ar += 256 / OB_GB) {
+        C1[(long)ar * ldc1 + bc] = 0.0;
+        C2[(long)ar * ldc2 + bc] = 0.0;
     }
 }
 
