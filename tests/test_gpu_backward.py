@@ -93,12 +93,14 @@ def test_backward_matches_implicit_and_fd(n, nsel, seed, offdiag):
     assert worst < 1e-6, worst
 
 
-def test_backward_batched_matches_single():
+@pytest.mark.parametrize("nb", [4, 8])
+def test_backward_batched_matches_single(nb):
+    # nb = 8: the outer products deal whole batches to the XCDs (batch counts divisible by 8)
     import torch
 
     import xtgevc3_amd as X
 
-    n, nsel, nb = 90, 16, 4
+    n, nsel = 90, 16
     S, P, ar, ai, be, sel, pick = _problem(n, nsel, 9)
     m = len(pick)
     dS = X.fortran_device(S)
