@@ -93,14 +93,15 @@ def test_backward_matches_implicit_and_fd(n, nsel, seed, offdiag):
     assert worst < 1e-6, worst
 
 
-@pytest.mark.parametrize("nb", [4, 8])
-def test_backward_batched_matches_single(nb):
-    # nb = 8: the outer products deal whole batches to the XCDs (batch counts divisible by 8)
+@pytest.mark.parametrize("n,nb", [(90, 4), (90, 8), (197, 8)])
+def test_backward_batched_matches_single(n, nb):
+    # nb = 8: the outer products deal whole batches to the XCDs (batch counts divisible by 8);
+    # n = 197: odd, partial edge tiles and tiles wholly below the band
     import torch
 
     import xtgevc3_amd as X
 
-    n, nsel = 90, 16
+    nsel = 16
     S, P, ar, ai, be, sel, pick = _problem(n, nsel, 9)
     m = len(pick)
     dS = X.fortran_device(S)
@@ -111,6 +112,11 @@ def test_backward_batched_matches_single(nb):
     Gs = rng.standard_normal((n, m * nb))
     Gt = torch.tensor(np.ascontiguousarray(Gs.T), device="cuda")
     bS, bP = X.dtgevc3_backward_device(dS, dP, ar, ai, be, sel, Vt, Gt, nbatch=nb)
+    assert bool(torch.isfinite(bS).all()) and bool(torch.isfinite(bP).all())
+    for b in range(nb):
+        # tensor row j holds column j: dS is zero below row j + 1 of column j, dP below row j
+        assert not bool(torch.triu(bS[b * n:(b + 1) * n], diagonal=2).any())
+        assert not bool(torch.triu(bP[b * n:(b + 1) * n], diagonal=1).any())
     for b in range(nb):
         Gb = torch.tensor(np.ascontiguousarray(Gs[:, b * m:(b + 1) * m].T), device="cuda")
         sS, sP = X.dtgevc3_backward_device(dS, dP, ar, ai, be, sel, Vt, Gb, nbatch=1)
